@@ -1,0 +1,627 @@
+// Host side of the C ABI (include/gral_pf.h): argument checking, the
+// prologue and epilogue of PrognosticFlowfield.Calculate(), device memory, and
+// the SIMPLE-iteration driver with the reference's convergence bookkeeping.
+//
+// The prologue (PrognosticFlowfield.cs:145-579) runs once per situation on the
+// host: it needs the full-height columns (topwind at NKK-1, the initial UKS
+// over all levels) and the init-time transcendentals (MathF.Log, Math.Pow),
+// which are evaluated with the C library like .NET on Linux does.  Only levels
+// 0..KADVMAX+1 -- everything the iterations touch -- are moved to the GPU.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gral_pf.h"
+#include "pf_device.h"
+
+extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_t s, long long *launches);
+extern "C" int pf_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
+
+namespace {
+
+inline float p2f(float x) { return x * x; }
+inline float fmax_net(float a, float b) {  // Math.Max(float,float)
+    if (a != b) { if (!std::isnan(a)) return b < a ? a : b; return a; }
+    return std::signbit(b) ? a : b;
+}
+inline float fmin_net(float a, float b) {  // Math.Min(float,float)
+    if (a != b) { if (!std::isnan(a)) return a < b ? a : b; return a; }
+    return std::signbit(a) ? a : b;
+}
+inline double dmax_net(double a, double b) {
+    if (a != b) { if (!std::isnan(a)) return b < a ? a : b; return a; }
+    return std::signbit(b) ? a : b;
+}
+
+}  // namespace
+
+struct gral_pf_handle {
+    // grid
+    int nii = 0, njj = 0, nkk = 0, kadvmax = 0, vcf = 0, nkp = 0, nkh = 0;
+    float dxk = 0, dyk = 0;
+    std::vector<float> dzk, hok;
+    // host state (full depth nkh = nkk+2)
+    std::vector<uint8_t> adv, vegp;
+    std::vector<int16_t> kk;
+    std::vector<float> cutk, cov, ahk, veg;
+    std::vector<float> z0w, z0s, obi, usti, z0i;
+    bool masks = false, rough = false, fields = false, has_veg = false;
+    std::vector<float> uk, vk, wk, uks, vks, wks, dpm, dpmnew, turb, tdiss;
+    std::vector<int> vi;
+    // run state
+    gral_pf_params prm{};
+    bool prepared = false, done = false, host_stale = false;
+    int iter_loops = 1, iters_run = 0, processed = 0;
+    double delta_mean = 0, last_delta = 0;
+    float delta_first = 0, delta_finish = 0;
+    float topwind = 0;
+    double geschw_max = 0;
+    int unreal[3] = {0, 0, 0};
+    gral_pf_timing timing{};
+    // device
+    Dom d{};
+    std::vector<void *> dev;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+
+    size_t n2() const { return (size_t)(nii + 2) * (size_t)(njj + 2); }
+    size_t i2(int i, int j) const { return (size_t)i * (size_t)(njj + 2) + (size_t)j; }
+    size_t i3(int i, int j, int k) const { return i2(i, j) * (size_t)nkh + (size_t)k; }
+};
+
+static int fail(gral_pf_handle *h, int code, const std::string &msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((h), GRAL_PF_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+static int dalloc(gral_pf_handle *h, T **p, size_t n) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T) + 256);
+    if (e != hipSuccess) return fail(h, GRAL_PF_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    h->dev.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+extern "C" int gral_pf_create(const gral_pf_grid *g, gral_pf_handle **out) {
+    if (!out) return GRAL_PF_EINVAL;
+    *out = nullptr;
+    if (!g || !g->dzk || !g->hokart) return GRAL_PF_EINVAL;
+    if (g->nii < 5 || g->njj < 5 || g->nkk < 3 || g->kadvmax < 2 || g->kadvmax > g->nkk || g->vertcellsff < 1 ||
+        !(g->dxk > 0) || !(g->dyk > 0))
+        return GRAL_PF_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GRAL_PF_EDEVICE;
+    gral_pf_handle *h = new (std::nothrow) gral_pf_handle();
+    if (!h) return GRAL_PF_ENOMEM;
+    h->nii = g->nii; h->njj = g->njj; h->nkk = g->nkk; h->kadvmax = g->kadvmax; h->vcf = g->vertcellsff;
+    h->dxk = g->dxk; h->dyk = g->dyk;
+    h->nkh = g->nkk + 2;
+    h->nkp = ((g->kadvmax + 2) + 3) & ~3;
+    h->dzk.assign(g->dzk, g->dzk + h->nkh);
+    h->hok.assign(g->hokart, g->hokart + h->nkh);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        gral_pf_destroy(h);
+        return GRAL_PF_EDEVICE;
+    }
+    *out = h;
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_set_masks(gral_pf_handle *h, const uint8_t *advdom, const int16_t *kkart, const float *cutk,
+                                 const float *cov, const float *ahk, const float *veg, const uint8_t *veg_present) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!advdom || !kkart || !cutk || !cov || !ahk) return fail(h, GRAL_PF_EINVAL, "set_masks: null mask array");
+    if ((veg == nullptr) != (veg_present == nullptr))
+        return fail(h, GRAL_PF_EINVAL, "set_masks: veg and veg_present must both be given or both be NULL");
+    size_t n2 = h->n2();
+    h->adv.assign(advdom, advdom + n2);
+    h->kk.assign(kkart, kkart + n2);
+    h->cutk.assign(cutk, cutk + n2);
+    h->cov.assign(cov, cov + n2);
+    h->ahk.assign(ahk, ahk + n2);
+    for (size_t c = 0; c < n2; ++c) {
+        if (h->adv[c] > 1) return fail(h, GRAL_PF_EINVAL, "set_masks: ADVDOM must be 0 or 1");
+        if (h->kk[c] < 0 || h->kk[c] + 1 > h->nkk - 1) return fail(h, GRAL_PF_EINVAL, "set_masks: KKART out of range");
+    }
+    h->has_veg = veg != nullptr;
+    if (h->has_veg) {
+        h->veg.assign(veg, veg + n2 * (size_t)h->nkh);
+        h->vegp.assign(veg_present, veg_present + n2);
+    } else {
+        h->veg.clear();
+        h->vegp.clear();
+    }
+    h->masks = true;
+    h->prepared = false;
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_set_roughness(gral_pf_handle *h, const float *z0_wall, const float *z0_solver, const float *ob_init,
+                                     const float *ust_init, const float *z0_init) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!z0_wall || !z0_solver || !ob_init || !ust_init || !z0_init) return fail(h, GRAL_PF_EINVAL, "set_roughness: null array");
+    size_t n2 = h->n2();
+    h->z0w.assign(z0_wall, z0_wall + n2);
+    h->z0s.assign(z0_solver, z0_solver + n2);
+    h->obi.assign(ob_init, ob_init + n2);
+    h->usti.assign(ust_init, ust_init + n2);
+    h->z0i.assign(z0_init, z0_init + n2);
+    h->rough = true;
+    h->prepared = false;
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_upload_fields(gral_pf_handle *h, const float *uk, const float *vk, const float *wk) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!uk || !vk || !wk) return fail(h, GRAL_PF_EINVAL, "upload_fields: null field");
+    size_t n3 = h->n2() * (size_t)h->nkh;
+    h->uk.assign(uk, uk + n3);
+    h->vk.assign(vk, vk + n3);
+    h->wk.assign(wk, wk + n3);
+    h->fields = true;
+    h->prepared = false;
+    return GRAL_PF_OK;
+}
+
+static void free_device(gral_pf_handle *h) {
+    for (void *p : h->dev) (void)hipFree(p);
+    h->dev.clear();
+}
+
+// ---------------------------------------------------------------------------
+// prologue, PrognosticFlowfield.cs:145-579
+// ---------------------------------------------------------------------------
+static int prologue(gral_pf_handle *h, const gral_pf_params *p, std::vector<float> &ap0, std::vector<float> &ht,
+                    std::vector<float> &ho, std::vector<double> &ta, std::vector<double> &tden, std::vector<int> &tbr,
+                    float &dtime, double &frund, float &ug, float &vg, float &ufx) {
+    const int NII = h->nii, NJJ = h->njj, NKK = h->nkk;
+    const int model = p->turbulence_model;
+    const size_t n3 = h->n2() * (size_t)h->nkh;
+    const float *DZK = h->dzk.data();
+    auto &UK = h->uk, &VK = h->vk, &WK = h->wk;
+    h->uks.assign(n3, 0.0f); h->vks.assign(n3, 0.0f); h->wks.assign(n3, 0.0f);
+    h->dpm.assign(n3, 0.0f); h->dpmnew.assign(n3, 0.0f);
+    if (model != 1) { h->turb.assign(n3, 0.0f); h->tdiss.assign(n3, 0.0f); }
+    else { h->turb.clear(); h->tdiss.clear(); }
+    int ri = p->refp_i, rj = p->refp_j;
+    float topwind = std::sqrt(p2f(UK[h->i3(ri, rj, NKK - 1)]) + p2f(VK[h->i3(ri, rj, NKK - 1)]));
+    topwind = fmax_net(topwind, 0.01f);
+    float spd = std::sqrt(p2f(UK[h->i3(ri, rj, h->kadvmax)]) + p2f(VK[h->i3(ri, rj, h->kadvmax)]));
+    dtime = 0.5f * fmin_net(h->dxk, DZK[1]) / fmax_net(0.01f, spd);
+    dtime = fmin_net(dtime, 5.0f);
+    const float area = h->dxk * h->dyk;
+    ap0.assign((size_t)h->nkp, 0.0f);
+    for (int k = 1; k <= h->kadvmax; ++k) ap0[k] = area * DZK[k] / dtime;
+    if (model != 1)
+        for (int i = 1; i <= NII; ++i)
+            for (int j = 1; j <= NJJ; ++j)
+                for (int k = 1; k <= NKK + 1; ++k) { h->turb[h->i3(i, j, k)] = 0.001f; h->tdiss[h->i3(i, j, k)] = 0.0000001f; }
+    frund = (double)((model == 2 ? 10000.0f : 8000.0f) / topwind);
+    // initial cell averages on the 3x3-dilated sub-domain footprint, and GESCHW_MAX
+    auto A = [&](int i, int j) { return (int)h->adv[h->i2(i, j)]; };
+    double gmax = 0;
+    for (int i = 1; i <= NII; ++i)
+        for (int j = 1; j <= NJJ; ++j) {
+            int ip = std::min(i + 1, NII + 1), jp = std::min(j + 1, NJJ + 1);
+            bool dil = A(i, j) == 1 || A(i - 1, j) == 1 || A(ip, j) == 1 || A(i, j - 1) == 1 || A(i - 1, j - 1) == 1 ||
+                       A(ip, j - 1) == 1 || A(i, jp) == 1 || A(i - 1, jp) == 1 || A(ip, jp) == 1;
+            if (!dil) continue;
+            const size_t o = h->i3(i, j, 0), oip = h->i3(i + 1, j, 0), ojp = h->i3(i, j + 1, 0);
+            for (int k = 1; k <= NKK - 1; ++k) {
+                float u = 0.5f * (UK[o + k] + UK[oip + k]);
+                float v = 0.5f * (VK[o + k] + VK[ojp + k]);
+                h->uks[o + k] = u;
+                h->vks[o + k] = v;
+                h->wks[o + k] = 0.5f * (WK[o + k] + WK[o + k + 1]);
+                gmax = dmax_net(gmax, std::sqrt((double)(p2f(u) + p2f(v))));
+            }
+        }
+    h->geschw_max = gmax;
+    h->vi.assign(h->n2(), 0);
+    for (int i = 1; i <= NII; ++i)
+        for (int j = 1; j <= NJJ; ++j) h->vi[h->i2(i, j)] = std::min(NKK - 1, h->kk[h->i2(i, j)] + h->vcf);
+    // wall-function helpterms at k = KKART+1
+    ht.assign(h->n2(), 0.0f); ho.assign(h->n2(), 0.0f);
+    ta.assign(h->n2(), 0.0); tden.assign(h->n2(), 1.0); tbr.assign(h->n2(), 0);
+    for (int i = 1; i <= NII; ++i)
+        for (int j = 1; j <= NJJ; ++j) {
+            size_t c = h->i2(i, j);
+            int KK = h->kk[c];
+            int KSTART = (h->cutk[c] == 0) ? KK + 1 : 1;
+            int k = KK + 1;
+            if (k < KSTART || k > h->vi[c]) continue;
+            float rough = h->z0w[c];
+            if (h->cutk[c] < 1) {
+                if ((double)rough >= (double)DZK[k] * 0.1)
+                    ht[c] = 0.4f / logf((DZK[k] + DZK[k + 1] * 0.5f) / DZK[k] * 0.1f);
+                else
+                    ht[c] = 0.4f / logf(DZK[k] * 0.5f / rough);
+            } else {
+                ho[c] = 0.4f / logf(DZK[k] * 0.5f / p->building_z0);
+            }
+            // TKE terrain wall function constants (TKE-prognostic-microscale.cs:291-306)
+            double z0 = (double)h->z0s[c];
+            if (z0 >= (double)DZK[k] * 0.5) {
+                double hz = (double)DZK[k] + (double)DZK[k + 1] * 0.5;
+                tbr[c] = 1; ta[c] = 0.4 / std::log(hz / z0); tden[c] = hz * 0.4;
+            } else {
+                tbr[c] = 0; ta[c] = 0.4 / std::log((double)DZK[k] * 0.5 / z0); tden[c] = (double)DZK[k] * 0.5 * 0.4;
+            }
+        }
+    ug = h->uks[h->i3(ri, rj, NKK - 1)];
+    vg = h->vks[h->i3(ri, rj, NKK - 1)];
+    // TKE initialisation and the ghost-row copies done in the same loop
+    for (int i = 1; i <= NII; ++i)
+        for (int j = 1; j <= NJJ; ++j) {
+            size_t c = h->i2(i, j);
+            float ob = h->obi[c], ust = h->usti[c], rough = h->z0i[c];
+            for (int k = 1; k <= NKK - 1; ++k) {
+                if (i == 1) UK[h->i3(NII + 1, j, k)] = UK[h->i3(NII, j, k)];
+                if (j == 1) VK[h->i3(i, NJJ + 1, k)] = VK[h->i3(i, NJJ, k)];
+                if (model == 1) continue;
+                float V0int = p->v0_1;
+                double wh = dmax_net(std::sqrt((double)(p2f(UK[h->i3(i, j, k)]) + p2f(VK[h->i3(i, j, k)]))), 0.01);
+                float U0int = (float)(wh * (0.2 * std::pow(wh, -0.9) + 0.32 * (double)rough + 0.18));
+                U0int = (float)dmax_net((double)U0int, 0.3) * p->std_dev_v;
+                float varw;
+                if (ob < 0) varw = p2f(ust) * p2f(1.15f + 0.1f * powf(p->bdlayheight / (-ob), 0.67f)) * p2f(p->std_dev_w);
+                else varw = p2f(ust * p->std_dev_w * 1.25f);
+                h->turb[h->i3(i, j, k)] = (float)(0.5 * (double)(p2f(U0int) + p2f(V0int) + varw));
+            }
+        }
+    ufx = (float)(0.4 / std::log((double)h->dxk * 0.5 / (double)p->building_z0));
+    // zero velocities (and k, eps) inside buildings
+    for (int i = 1; i <= NII; ++i)
+        for (int j = 1; j <= NJJ; ++j) {
+            size_t c = h->i2(i, j);
+            if (h->adv[c] != 1) continue;
+            int VI = h->vi[c], KK = h->kk[c];
+            for (int k = 1; k <= VI && k <= KK; ++k) {
+                UK[h->i3(i, j, k)] = 0;
+                if (i == NII) UK[h->i3(i + 1, j, k)] = 0;
+                VK[h->i3(i, j, k)] = 0;
+                if (j == NJJ) VK[h->i3(i, j + 1, k)] = 0;
+                WK[h->i3(i, j, k)] = 0;
+                if (k == VI) WK[h->i3(i, j, k + 1)] = 0;
+                if (model != 1) { h->turb[h->i3(i, j, k)] = 0; h->tdiss[h->i3(i, j, k)] = 0; }
+                h->uks[h->i3(i, j, k)] = 0; h->vks[h->i3(i, j, k)] = 0; h->wks[h->i3(i, j, k)] = 0;
+            }
+        }
+    h->topwind = topwind;
+    return 0;
+}
+
+// copy levels [0, nkp) of a host full-depth field into a device [.,.,nkp] field
+static int put_field(gral_pf_handle *h, float *dptr, const std::vector<float> &host) {
+    size_t n2 = h->n2();
+    std::vector<float> tmp(n2 * (size_t)h->nkp, 0.0f);
+    int kc = std::min(h->nkp, h->nkh);
+    for (size_t c = 0; c < n2; ++c) std::memcpy(&tmp[c * h->nkp], &host[c * h->nkh], sizeof(float) * kc);
+    HIPCHK(h, hipMemcpyAsync(dptr, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+static int get_field(gral_pf_handle *h, const float *dptr, std::vector<float> &host) {
+    size_t n2 = h->n2();
+    std::vector<float> tmp(n2 * (size_t)h->nkp);
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), dptr, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int kc = std::min(h->nkp, h->nkh);
+    for (size_t c = 0; c < n2; ++c) std::memcpy(&host[c * h->nkh], &tmp[c * h->nkp], sizeof(float) * kc);
+    return 0;
+}
+template <class T>
+static int put_vec(gral_pf_handle *h, T *dptr, const std::vector<T> &v) {
+    HIPCHK(h, hipMemcpyAsync(dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!p) return fail(h, GRAL_PF_EINVAL, "prepare: null params");
+    if (!h->masks || !h->rough || !h->fields) return fail(h, GRAL_PF_ESTATE, "prepare: masks, roughness and fields must be set first");
+    if (p->turbulence_model < 0 || p->turbulence_model > 2) return fail(h, GRAL_PF_EINVAL, "prepare: turbulence_model must be 0, 1 or 2");
+    if (p->simd_width != 8 && p->simd_width != 16) return fail(h, GRAL_PF_EINVAL, "prepare: simd_width must be 8 or 16");
+    if (p->simd_width == 16) return fail(h, GRAL_PF_EINVAL, "prepare: Vector512 semantics (simd_width 16) are not implemented");
+    if (h->vcf < p->simd_width) return fail(h, GRAL_PF_EINVAL, "prepare: vertcellsff must be >= simd_width");
+    if (p->iter_max < 1 || p->iter_min < 0 || p->iter_max > 10000000) return fail(h, GRAL_PF_EINVAL, "prepare: bad iteration limits");
+    if (p->refp_i < 1 || p->refp_i > h->nii || p->refp_j < 1 || p->refp_j > h->njj) return fail(h, GRAL_PF_EINVAL, "prepare: reference point outside the grid");
+    // the solver reads KKART+1 <= VerticalIndex <= KADVMAX; check the grid agrees
+    for (int i = 1; i <= h->nii; ++i)
+        for (int j = 1; j <= h->njj; ++j) {
+            size_t c = h->i2(i, j);
+            int v = std::min(h->nkk - 1, h->kk[c] + h->vcf);
+            if (h->adv[c] == 1 && v > h->kadvmax) return fail(h, GRAL_PF_EINVAL, "prepare: KKART+VertCellsFF exceeds KADVMAX in a sub-domain column");
+        }
+    h->prm = *p;
+    std::vector<float> ap0, ht, ho;
+    std::vector<double> ta, tden;
+    std::vector<int> tbr;
+    float dtime, ug, vg, ufx;
+    double frund;
+    prologue(h, p, ap0, ht, ho, ta, tden, tbr, dtime, frund, ug, vg, ufx);
+
+    free_device(h);
+    Dom &d = h->d;
+    d = Dom{};
+    d.nii = h->nii; d.njj = h->njj; d.nkk = h->nkk; d.nkp = h->nkp; d.nj2 = h->njj + 2; d.kadvmax = h->kadvmax;
+    d.model = p->turbulence_model; d.simd = p->simd_width;
+    d.dxk = h->dxk; d.dyk = h->dyk; d.area = h->dxk * h->dyk; d.dtime = dtime; d.relax = p->relax; d.relaxp = p->relaxp;
+    d.cor = p->coriolis; d.ug = ug; d.vg = vg;
+    d.vishmin = (p->turbulence_model == 2) ? 0.01f : 0.0f;
+    d.visvmin = (p->turbulence_model == 2) ? 0.01f : 0.0f;
+    d.bz0 = p->building_z0; d.ufx = ufx; d.potdtdz = p->potdtdz; d.frund = frund;
+    const size_t n2 = h->n2(), n3 = n2 * (size_t)h->nkp;
+    int rc = 0;
+    float *fdzk, *fhok, *fap0, *fapp, *fabp, *fatp, *fcutk, *fcov, *fht, *fho, *fz0s;
+    uint8_t *fadv, *fvegp = nullptr;
+    int *fkk, *fvi, *ftbr;
+    double *fta, *ftden;
+    if ((rc = dalloc(h, &fdzk, h->nkh)) || (rc = dalloc(h, &fhok, h->nkh)) || (rc = dalloc(h, &fap0, h->nkp)) ||
+        (rc = dalloc(h, &fapp, h->nkp)) || (rc = dalloc(h, &fabp, h->nkp)) || (rc = dalloc(h, &fatp, h->nkp)) ||
+        (rc = dalloc(h, &fadv, n2)) || (rc = dalloc(h, &fkk, n2)) || (rc = dalloc(h, &fvi, n2)) ||
+        (rc = dalloc(h, &fcutk, n2)) || (rc = dalloc(h, &fcov, n2)) || (rc = dalloc(h, &fht, n2)) ||
+        (rc = dalloc(h, &fho, n2)) || (rc = dalloc(h, &fz0s, n2)) || (rc = dalloc(h, &fta, n2)) ||
+        (rc = dalloc(h, &ftden, n2)) || (rc = dalloc(h, &ftbr, n2)))
+        return rc;
+    float *F[12];
+    for (int f = 0; f < 12; ++f)
+        if ((rc = dalloc(h, &F[f], n3))) return rc;
+    d.uk = F[0]; d.vk = F[1]; d.wk = F[2]; d.uks = F[3]; d.vks = F[4]; d.wks = F[5]; d.dpm = F[6]; d.dpmnew = F[7];
+    d.div = F[8]; d.turb = F[9]; d.tdiss = F[10];
+    float *fveg = F[11];
+    if ((rc = dalloc(h, &d.prow_p, (size_t)d.nj2 * d.nkp)) || (rc = dalloc(h, &d.prow_q, (size_t)d.nj2 * d.nkp))) return rc;
+    unsigned *fres;
+    if ((rc = dalloc(h, &fres, (size_t)p->iter_max + 1))) return rc;
+    d.resid = fres;
+    // per-level pressure coefficients (PrognosticFlowfield.cs:855-866), column independent
+    std::vector<float> app(h->nkp, 0.0f), abp(h->nkp, 0.0f), atp(h->nkp, 0.0f);
+    const float *DZK = h->dzk.data();
+    float area = d.area;
+    for (int k = 1; k <= h->kadvmax && k + 1 < h->nkh; ++k) {
+        app[k] = 2.0f * (h->dyk * DZK[k] / h->dxk + h->dxk * DZK[k] / h->dyk + area / DZK[k]);
+        abp[k] = (k > 1) ? area / (0.5f * (DZK[k - 1] + DZK[k])) : area / DZK[k];
+        atp[k] = area / (0.5f * (DZK[k + 1] + DZK[k]));
+    }
+    std::vector<int> kk32(n2);
+    for (size_t c = 0; c < n2; ++c) kk32[c] = h->kk[c];
+    if ((rc = put_vec(h, fdzk, h->dzk)) || (rc = put_vec(h, fhok, h->hok)) || (rc = put_vec(h, fap0, ap0)) ||
+        (rc = put_vec(h, fapp, app)) || (rc = put_vec(h, fabp, abp)) || (rc = put_vec(h, fatp, atp)) ||
+        (rc = put_vec(h, fadv, h->adv)) || (rc = put_vec(h, fkk, kk32)) || (rc = put_vec(h, fvi, h->vi)) ||
+        (rc = put_vec(h, fcutk, h->cutk)) || (rc = put_vec(h, fcov, h->cov)) || (rc = put_vec(h, fht, ht)) ||
+        (rc = put_vec(h, fho, ho)) || (rc = put_vec(h, fz0s, h->z0s)) || (rc = put_vec(h, fta, ta)) ||
+        (rc = put_vec(h, ftden, tden)) || (rc = put_vec(h, ftbr, tbr)))
+        return rc;
+    d.dzk = fdzk; d.hok = fhok; d.ap0 = fap0; d.app = fapp; d.abp = fabp; d.atp = fatp;
+    d.adv = fadv; d.kk = fkk; d.vi = fvi; d.cutk = fcutk; d.cov = fcov; d.ht = fht; d.ho = fho; d.z0s = fz0s;
+    d.tke_a = fta; d.tke_den = ftden; d.tke_br = ftbr;
+    if (h->has_veg) {
+        if ((rc = dalloc(h, &fvegp, n2)) || (rc = put_vec(h, fvegp, h->vegp)) || (rc = put_field(h, fveg, h->veg))) return rc;
+        d.vegp = fvegp; d.veg = fveg;
+    } else {
+        d.vegp = nullptr; d.veg = nullptr;
+    }
+    std::vector<float> zeros;
+    if ((rc = put_field(h, d.uk, h->uk)) || (rc = put_field(h, d.vk, h->vk)) || (rc = put_field(h, d.wk, h->wk)) ||
+        (rc = put_field(h, d.uks, h->uks)) || (rc = put_field(h, d.vks, h->vks)) || (rc = put_field(h, d.wks, h->wks)) ||
+        (rc = put_field(h, d.dpm, h->dpm)) || (rc = put_field(h, d.dpmnew, h->dpmnew)))
+        return rc;
+    HIPCHK(h, hipMemsetAsync(d.div, 0, n3 * sizeof(float), h->stream));
+    if (p->turbulence_model != 1) {
+        if ((rc = put_field(h, d.turb, h->turb)) || (rc = put_field(h, d.tdiss, h->tdiss))) return rc;
+    } else {
+        HIPCHK(h, hipMemsetAsync(d.turb, 0, n3 * sizeof(float), h->stream));
+        HIPCHK(h, hipMemsetAsync(d.tdiss, 0, n3 * sizeof(float), h->stream));
+    }
+    if (pf_fill_u32(d.resid, (size_t)p->iter_max + 1, 0u, h->stream)) return fail(h, GRAL_PF_EDEVICE, "fill resid");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->prepared = true;
+    h->done = false;
+    h->host_stale = false;
+    h->iter_loops = 1;
+    h->iters_run = 0;
+    h->processed = 0;
+    h->delta_mean = 0;
+    h->last_delta = 0;
+    h->delta_first = 0;
+    h->delta_finish = 0;
+    h->unreal[0] = h->unreal[1] = h->unreal[2] = 0;
+    h->timing = gral_pf_timing{};
+    return GRAL_PF_OK;
+}
+
+static void fill_stats(gral_pf_handle *h, gral_pf_stats *st) {
+    if (!st) return;
+    st->iteration_loops = h->iter_loops;
+    st->iterations_run = h->iters_run;
+    st->delta_first = h->delta_first;
+    st->delta_finish = h->delta_finish;
+    st->convergence_problem = h->delta_finish > h->delta_first;
+    st->unreal_u = h->unreal[0]; st->unreal_v = h->unreal[1]; st->unreal_w = h->unreal[2];
+    st->topwind = h->topwind; st->dtime = h->d.dtime; st->ug = h->d.ug; st->vg = h->d.vg;
+    st->frund = h->d.frund; st->geschw_max = h->geschw_max;
+    st->last_delta = h->last_delta;
+}
+
+// Convergence bookkeeping of one finished iteration (PrognosticFlowfield.cs:1144-1178).
+// Returns true when the reference would leave the loop here.
+static bool book_iteration(gral_pf_handle *h, int it, float dumax, bool check) {
+    const gral_pf_params &p = h->prm;
+    double tw2 = (double)(h->topwind * h->topwind);
+    if (it >= p.iter_max) h->delta_finish = (float)(h->delta_mean / tw2);
+    if ((it % 100 == 0) && (it > 1)) {
+        double delta = h->delta_mean / tw2;
+        if (it == 100) h->delta_first = (float)delta;
+        h->last_delta = delta;
+        if (check && delta < 0.012 && it >= p.iter_min) return true;
+        h->delta_mean = 0;
+    } else {
+        h->delta_mean += (double)dumax;
+    }
+    return false;
+}
+
+extern "C" int gral_pf_iterate(gral_pf_handle *h, int32_t n, int32_t check, gral_pf_stats *st) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "iterate: call prepare first");
+    if (n < 0) return fail(h, GRAL_PF_EINVAL, "iterate: n < 0");
+    h->timing = gral_pf_timing{};
+    int todo = std::min(n, h->prm.iter_max - h->iter_loops + 1);
+    if (h->done) todo = 0;
+    long long launches = 0;
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    int started = 0;
+    std::vector<unsigned> bits;
+    while (started < todo) {
+        // launch up to the next multiple of 100 (where the reference may stop)
+        int it0 = h->iter_loops + started;
+        int chunk = todo - started;
+        if (check) chunk = std::min(chunk, 100 - ((it0 - 1) % 100));
+        for (int c = 0; c < chunk; ++c) {
+            int it = it0 + c;
+            int e = pf_launch_iteration(&h->d, it, it - 1, h->stream, &launches);
+            if (e) return fail(h, GRAL_PF_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString((hipError_t)e));
+        }
+        started += chunk;
+        // bookkeeping of the iterations just launched
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        bits.resize((size_t)chunk);
+        HIPCHK(h, hipMemcpy(bits.data(), h->d.resid + (it0 - 1), sizeof(unsigned) * chunk, hipMemcpyDeviceToHost));
+        bool stop = false;
+        for (int c = 0; c < chunk && !stop; ++c) {
+            float du;
+            std::memcpy(&du, &bits[c], 4);
+            int it = it0 + c;
+            h->iters_run++;
+            stop = book_iteration(h, it, du, check != 0);
+            if (!stop) h->iter_loops = it + 1;
+            else h->iter_loops = it;
+        }
+        if (stop) { h->done = true; break; }
+    }
+    if (h->iter_loops > h->prm.iter_max) h->done = true;
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->timing.ms_total = ms;
+    h->timing.launches = launches;
+    long long act = 0;
+    for (int i = 2; i <= h->nii - 1; ++i)
+        for (int j = 2; j <= h->njj - 1; ++j) {
+            size_t c = h->i2(i, j);
+            if (h->adv[c] == 1) act += h->vi[c] - h->kk[c];
+        }
+    h->timing.active_cells = act;
+    h->host_stale = true;
+    fill_stats(h, st);
+    return GRAL_PF_OK;
+}
+
+static int sync_host(gral_pf_handle *h) {
+    if (!h->host_stale) return 0;
+    int rc;
+    if ((rc = get_field(h, h->d.uk, h->uk)) || (rc = get_field(h, h->d.vk, h->vk)) || (rc = get_field(h, h->d.wk, h->wk)) ||
+        (rc = get_field(h, h->d.uks, h->uks)) || (rc = get_field(h, h->d.vks, h->vks)) ||
+        (rc = get_field(h, h->d.wks, h->wks)) || (rc = get_field(h, h->d.dpm, h->dpm)) ||
+        (rc = get_field(h, h->d.dpmnew, h->dpmnew)))
+        return rc;
+    if (h->prm.turbulence_model != 1)
+        if ((rc = get_field(h, h->d.turb, h->turb)) || (rc = get_field(h, h->d.tdiss, h->tdiss))) return rc;
+    // VKS edge-row copies above the device levels (PrognosticFlowfield.cs:1019-1039)
+    if (h->iters_run > 0) {
+        for (int j = 2; j < h->njj; ++j) {
+            if (h->adv[h->i2(1, j)] == 1)
+                for (int k = h->nkp; k <= h->nkk; ++k) h->vks[h->i3(1, j, k)] = h->vks[h->i3(2, j, k)];
+            if (h->adv[h->i2(h->nii - 1, j)] == 1)
+                for (int k = h->nkp; k <= h->nkk; ++k) h->vks[h->i3(h->nii, j, k)] = h->vks[h->i3(h->nii - 1, j, k)];
+        }
+    }
+    h->host_stale = false;
+    return 0;
+}
+
+extern "C" int gral_pf_finish(gral_pf_handle *h, gral_pf_stats *st) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "finish: call prepare first");
+    int rc = sync_host(h);
+    if (rc) return rc;
+    // unrealistic wind speeds (PrognosticFlowfield.cs:1192-1243)
+    int nu = 0, nv = 0, nw = 0;
+    float limit = (float)(10 * h->geschw_max);
+    for (int i = 1; i <= h->nii; ++i)
+        for (int j = 1; j <= h->njj; ++j) {
+            size_t c = h->i2(i, j);
+            if (h->adv[c] != 1) continue;
+            int kmax = std::min(h->nkk - 1, h->kk[c] + h->vcf);
+            for (int k = 1; k <= kmax; ++k) {
+                float u = std::fabs(h->uk[h->i3(i, j, k)]), v = std::fabs(h->vk[h->i3(i, j, k)]), w = std::fabs(h->wk[h->i3(i, j, k)]);
+                if (u > 55 || u > limit) ++nu;
+                if (v > 55 || v > limit) ++nv;
+                if (w > 20 || w > limit) ++nw;
+            }
+        }
+    h->unreal[0] = nu; h->unreal[1] = nv; h->unreal[2] = nw;
+    fill_stats(h, st);
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_solve(gral_pf_handle *h, const gral_pf_params *p, gral_pf_stats *st) {
+    int rc = gral_pf_prepare(h, p);
+    if (rc) return rc;
+    rc = gral_pf_iterate(h, p->iter_max, 1, st);
+    if (rc) return rc;
+    return gral_pf_finish(h, st);
+}
+
+extern "C" int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, float *wk, float *uks, float *vks,
+                                       float *wks, float *dpm, float *dpmnew, float *turb, float *tdiss) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "download: call prepare first");
+    int rc = sync_host(h);
+    if (rc) return rc;
+    size_t n3 = h->n2() * (size_t)h->nkh;
+    auto cp = [&](float *dst, const std::vector<float> &src) {
+        if (!dst) return;
+        if (src.size() == n3) std::memcpy(dst, src.data(), n3 * sizeof(float));
+        else std::memset(dst, 0, n3 * sizeof(float));
+    };
+    cp(uk, h->uk); cp(vk, h->vk); cp(wk, h->wk); cp(uks, h->uks); cp(vks, h->vks); cp(wks, h->wks);
+    cp(dpm, h->dpm); cp(dpmnew, h->dpmnew); cp(turb, h->turb); cp(tdiss, h->tdiss);
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_last_timing(gral_pf_handle *h, gral_pf_timing *t) {
+    if (!h || !t) return GRAL_PF_EINVAL;
+    *t = h->timing;
+    return GRAL_PF_OK;
+}
+
+extern "C" const char *gral_pf_last_error(const gral_pf_handle *h) {
+    if (!h) return "null handle";
+    return h->err.c_str();
+}
+
+extern "C" void gral_pf_destroy(gral_pf_handle *h) {
+    if (!h) return;
+    free_device(h);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}

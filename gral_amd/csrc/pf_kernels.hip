@@ -1,0 +1,1026 @@
+// HIP kernels of the prognostic microscale flow solver (gfx950).
+//
+// One SIMPLE iteration (PrognosticFlowfield.cs:584-1178) is a sequence of
+//   stencil passes   - sub-domain BCs, divergence, velocity correction, averages:
+//                      one thread per (column, level), k contiguous -> coalesced;
+//   Gauss-Seidel passes - pressure TDMA, U/V/W momentum, TKE/eps:
+//                      the reference updates columns in place in lexicographic
+//                      sweep order (a outer, b inner).  A column depends on the
+//                      new values of (a-1,b) and (a,b-1) only, so all columns of
+//                      one anti-diagonal a+b=t are independent.  Each pass is a
+//                      sequence of per-diagonal launches; one 64-lane wavefront
+//                      per column computes the coefficients of all levels in
+//                      parallel (lanes over k), stages them in LDS, and one lane
+//                      runs the Thomas recurrences, which are serial in k.
+// This replays the reference's deterministic (single-thread) order exactly.
+#include "pf_device.h"
+
+#define WAVE 64
+
+// ------------------------------------------------------------------------
+// diagonal -> column mapping
+// ------------------------------------------------------------------------
+// Sweep indices a in [oa, ob] (outer), b in [ia, ib] (inner); diagonal t
+// holds the columns with (a-oa)+(b-ia) == t, enumerated by a.
+__device__ __forceinline__ bool diag_col(int t, int oa, int ob, int ia, int ib, int idx, int &a, int &b) {
+    int nb = ib - ia;
+    int amin = max(oa, oa + t - nb);
+    int amax = min(ob, oa + t);
+    a = amin + idx;
+    if (a > amax) return false;
+    b = ia + (t - (a - oa));
+    return true;
+}
+
+// ------------------------------------------------------------------------
+// utility
+// ------------------------------------------------------------------------
+__global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) p[g] = v;
+}
+__global__ void k_fill_f32(float *p, size_t n, float v) {
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) p[g] = v;
+}
+
+// ------------------------------------------------------------------------
+// Sub-domain boundary conditions, PrognosticFlowfield.cs:595-714
+// ------------------------------------------------------------------------
+// E/W: each row j is independent; within a row the reference scans i upwards
+// and later i read what earlier i wrote, so one thread per (j,k) scans i.
+__global__ void k_bc_ew(Dom d) {
+    int j = blockIdx.y + 2;                   // 2..NJJ
+    int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (k > d.kadvmax) return;
+    for (int i = 3; i < d.nii; ++i) {
+        size_t c = c2(d, i, j);
+        if (d.adv[c] != 1) continue;
+        if (k > d.vi[c]) continue;
+        bool iM = (d.adv[c2(d, i - 1, j)] < 1) || (i == 3);
+        bool iP = (d.adv[c2(d, i + 1, j)] < 1) || (i == d.nii - 1);
+        if (iM) {
+            float u = d.uk[c3(d, i - 2, j) + k];
+            if (u >= 0) d.uk[c3(d, i - 1, j) + k] = u;
+            d.vk[c3(d, i - 2, j) + k] = d.vk[c3(d, i - 1, j) + k];
+            d.wk[c3(d, i - 2, j) + k] = d.wk[c3(d, i - 1, j) + k];
+        }
+        if (iP) {
+            float u = d.uk[c3(d, i + 2, j) + k];
+            if (u <= 0) d.uk[c3(d, i + 1, j) + k] = u;
+            d.vk[c3(d, i + 1, j) + k] = d.vk[c3(d, i, j) + k];
+            d.wk[c3(d, i + 1, j) + k] = d.wk[c3(d, i, j) + k];
+        }
+    }
+}
+
+__global__ void k_bc_ns(Dom d) {
+    int i = blockIdx.y + 2;                   // 2..NII
+    int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (k > d.kadvmax) return;
+    for (int j = 3; j <= d.njj - 1; ++j) {
+        size_t c = c2(d, i, j);
+        if (d.adv[c] != 1) continue;
+        if (k > d.vi[c]) continue;
+        bool jM = (d.adv[c2(d, i, j - 1)] < 1) || (j == 3);
+        bool jP = (d.adv[c2(d, i, j + 1)] < 1) || (j == d.njj - 1);
+        if (jM) {
+            float v = d.vk[c3(d, i, j - 2) + k];
+            if (v >= 0) d.vk[c3(d, i, j - 1) + k] = v;
+            d.uk[c3(d, i, j - 2) + k] = d.uk[c3(d, i, j - 1) + k];
+            d.wk[c3(d, i, j - 2) + k] = d.wk[c3(d, i, j - 1) + k];
+        }
+        if (jP) {
+            float v = d.vk[c3(d, i, j + 2) + k];
+            if (v <= 0) d.vk[c3(d, i, j + 1) + k] = v;
+            d.uk[c3(d, i, j + 1) + k] = d.uk[c3(d, i, j) + k];
+            d.wk[c3(d, i, j + 1) + k] = d.wk[c3(d, i, j) + k];
+        }
+    }
+}
+
+// top copy VI -> VI+1 (thread per column i in [2,NII-1], j in [2,NJJ-1])
+__global__ void k_bc_top(Dom d) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    int ni = d.nii - 2, nj = d.njj - 2;
+    if (g >= ni * nj) return;
+    int i = 2 + g / nj, j = 2 + g % nj;
+    size_t c = c2(d, i, j);
+    if (d.adv[c] != 1) return;
+    int k = d.vi[c];
+    size_t o = c3(d, i, j);
+    d.wk[o + k + 1] = d.wk[o + k];
+    d.uk[o + k + 1] = d.uk[o + k];
+    d.vk[o + k + 1] = d.vk[o + k];
+}
+
+// ------------------------------------------------------------------------
+// Mass divergence, PrognosticFlowfield.cs:723-758.  Block = one column.
+// ------------------------------------------------------------------------
+__global__ void k_divergence(Dom d) {
+    int nj = d.njj - 2;
+    int i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
+    size_t c = c2(d, i, j);
+    if (d.adv[c] != 1) return;
+    int KK = d.kk[c], VI = d.vi[c];
+    size_t o = c3(d, i, j), oip = c3(d, i + 1, j), ojp = c3(d, i, j + 1);
+    for (int k = KK + 1 + threadIdx.x; k <= VI; k += blockDim.x) {
+        float dz = d.dzk[k];
+        float t1 = (d.uk[o + k] - d.uk[oip + k]) * d.dyk * dz;
+        float t2 = (d.vk[o + k] - d.vk[ojp + k]) * d.dxk * dz;
+        float v;
+        if (k > KK + 1) v = t1 + t2 + (d.wk[o + k] - d.wk[o + k + 1]) * d.area;
+        else v = t1 + t2 - d.wk[o + k + 1] * d.area;
+        d.div[o + k] = v;
+        d.dpm[o + k] = 0;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Pressure-correction GS-TDMA, PrognosticFlowfield.cs:770-901.
+// Sweep: a = j1 in [2,NJJ-1] (outer), b = i1 in [2,NII-1] (inner).
+// PIMP/QIMP live per sweep row in global scratch, zeroed at the start of the
+// pass, so the top coupling PIMP[VI+1] sees what the last earlier column of
+// the same row left there, as in the reference.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(WAVE) void k_pressure(Dom d, int t, int IS, int JS, int slot) {
+    extern __shared__ float sm[];
+    int a, b;
+    if (!diag_col(t, 2, d.njj - 1, 2, d.nii - 1, blockIdx.x, a, b)) return;
+    int j = (JS == -1) ? d.njj - a + 1 : a;
+    int i = (IS == -1) ? d.nii - b + 1 : b;
+    size_t c = c2(d, i, j);
+    if (d.adv[c] != 1) return;
+    const int nkp = d.nkp, tid = threadIdx.x;
+    float *sDIM = sm, *sP = sDIM + nkp, *sQ = sP + nkp, *sN = sQ + nkp, *sO = sN + nkp;
+    int KK = d.kk[c], VI = d.vi[c];
+    size_t o = c3(d, i, j), oim = c3(d, i - 1, j), oip = c3(d, i + 1, j), ojm = c3(d, i, j - 1), ojp = c3(d, i, j + 1);
+    float DXK = d.dxk, DYK = d.dyk;
+    for (int k = KK + 1 + tid; k <= VI; k += WAVE) {
+        float dz = d.dzk[k];
+        float DIM = d.div[o + k] / d.dtime + (d.dpm[oim + k] + d.dpm[oip + k]) / DXK * DYK * dz +
+                    (d.dpm[ojm + k] - d.dpm[ojp + k]) / DYK * DXK * dz;
+        sDIM[k] = DIM;
+    }
+    for (int k = KK + tid; k <= VI; k += WAVE) sO[k] = d.dpm[o + k];
+    size_t row = (size_t)j * nkp;
+    __syncthreads();
+    if (tid == 0) {
+        float Pn = d.prow_p[row + VI + 1], Qn = d.prow_q[row + VI + 1];
+        for (int k = VI; k > KK; --k) {
+            float ATP = d.atp[k];
+            float TERMP = 1 / (d.app[k] - ATP * Pn);
+            float P = d.abp[k] * TERMP;
+            float Q = (ATP * Qn + sDIM[k]) * TERMP;
+            sP[k] = P; sQ[k] = Q;
+            Pn = P; Qn = Q;
+        }
+        float dprev = sO[KK];
+        for (int k = KK + 1; k <= VI; ++k) {
+            float temp = sP[k] * dprev + sQ[k];
+            sN[k] = temp;
+            dprev = temp;
+        }
+    }
+    __syncthreads();
+    float rmax = 0.0f;
+    for (int k = KK + 1 + tid; k <= VI; k += WAVE) {
+        float temp = sN[k];
+        double q = (double)conv_to_int((double)temp * d.frund) / d.frund;
+        float r = (float)fabs((double)sO[k] - q);
+        rmax = r > rmax ? r : rmax;
+        d.dpm[o + k] = temp;
+        d.dpmnew[o + k] = d.dpmnew[o + k] + d.relaxp * temp;
+        d.prow_p[row + k] = sP[k];
+        d.prow_q[row + k] = sQ[k];
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        float other = __shfl_xor(rmax, off);
+        rmax = other > rmax ? other : rmax;
+    }
+    if (tid == 0 && rmax > 0.0f) atomicMax(&d.resid[slot], __float_as_uint(rmax));
+}
+
+// ------------------------------------------------------------------------
+// Velocity correction with FP64 quantisation, PrognosticFlowfield.cs:904-949
+// ------------------------------------------------------------------------
+__global__ void k_correction(Dom d) {
+    int nj = d.njj - 2;
+    int i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
+    size_t c = c2(d, i, j);
+    if (!(d.adv[c] == 1 || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1)) return;
+    int KK = d.kk[c], VI = d.vi[c], KKiM = d.kk[c2(d, i - 1, j)], KKjM = d.kk[c2(d, i, j - 1)];
+    size_t o = c3(d, i, j), oim = c3(d, i - 1, j), ojm = c3(d, i, j - 1);
+    for (int k = 1 + threadIdx.x; k <= VI; k += blockDim.x) {
+        float dpmk = d.dpm[o + k];
+        if (i > 2 && KK < k && KKiM < k) {
+            float e = d.uk[o + k] + (d.dpm[oim + k] - dpmk) / d.dxk * d.dtime;
+            d.uk[o + k] = quantise(e, d.frund);
+        }
+        if (j > 2 && KK < k && KKjM < k) {
+            float e = d.vk[o + k] + (d.dpm[ojm + k] - dpmk) / d.dyk * d.dtime;
+            d.vk[o + k] = quantise(e, d.frund);
+        }
+        if (KK + 1 < k) {
+            float e = d.wk[o + k] + (d.dpm[o + k - 1] - dpmk) / (d.dzk[k] + d.dzk[k - 1]) * 2.0f * d.dtime;
+            d.wk[o + k] = quantise(e, d.frund);
+        } else {
+            d.wk[o + k] = 0.0f;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Cell-centred averages, PrognosticFlowfield.cs:958-1014 (+ UKS row copies)
+// ------------------------------------------------------------------------
+__global__ void k_averages(Dom d) {
+    int nj = d.njj - 2;
+    int i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
+    size_t c = c2(d, i, j);
+    if (!(d.adv[c] == 1 || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1)) return;
+    int KK = d.kk[c], VI = d.vi[c];
+    int KSTART = (d.cutk[c] == 0) ? KK + 1 : 1;
+    size_t o = c3(d, i, j), oip = c3(d, i + 1, j), ojp = c3(d, i, j + 1);
+    bool copy_s = (j == 2);
+    bool copy_n = (j == d.njj - 1) && (d.adv[c2(d, i, d.njj)] == 1);
+    size_t os = c3(d, i, 1), on = c3(d, i, d.njj);
+    for (int k = KSTART + threadIdx.x; k <= VI; k += blockDim.x) {
+        if (KK < k) {
+            d.uks[o + k] = 0.5f * (d.uk[o + k] + d.uk[oip + k]);
+            d.vks[o + k] = 0.5f * (d.vk[o + k] + d.vk[ojp + k]);
+        }
+        if (KK + 1 < k) d.wks[o + k] = 0.5f * (d.wk[o + k] + d.wk[o + k + 1]);
+        if (copy_s) d.uks[os + k] = d.uks[o + k];
+        if (copy_n) d.uks[on + k] = d.uks[o + k];
+    }
+}
+
+// VKS[1][j] = VKS[2][j], VKS[NII][j] = VKS[NII-1][j] (PrognosticFlowfield.cs:1019-1039)
+__global__ void k_avg_edges(Dom d) {
+    int j = blockIdx.x + 2;                  // 2..NJJ-1
+    bool w = d.adv[c2(d, 1, j)] == 1, e = d.adv[c2(d, d.nii - 1, j)] == 1;
+    size_t o1 = c3(d, 1, j), o2 = c3(d, 2, j), on = c3(d, d.nii, j), on1 = c3(d, d.nii - 1, j);
+    for (int k = threadIdx.x; k < d.nkp; k += blockDim.x) {
+        if (w) d.vks[o1 + k] = d.vks[o2 + k];
+        if (e) d.vks[on + k] = d.vks[on1 + k];
+    }
+}
+
+// ------------------------------------------------------------------------
+// Mixing-length momentum (turbulence model 1): U/V/W-prognostic-microscale_1.cs
+// ------------------------------------------------------------------------
+// The reference evaluates a column in Vector<float> blocks of W lanes starting
+// at KSTART, KSTART+W, ..., the last block moved down to KEND-W.  Every lane
+// quantity is lane-local except the gate of the eddy-viscosity cross terms,
+// which tests the block start (k_block > KKART+1); block_base() returns the
+// start of the block whose lane k enters the recurrence.
+__device__ __forceinline__ int block_base(int k, int KSTART, int KEND, int W) {
+    int base = KSTART + ((k - KSTART) / W) * W;
+    if (base > KEND - W) base = KEND - W;
+    return base;
+}
+
+struct ColInfo {
+    int i, j, KK, KSTART, VI, W;
+    bool as, an, aw, ae, vegp;
+    float cutk, cov, ht, ho, z0s;
+};
+
+__device__ __forceinline__ ColInfo col_info(const Dom &d, int i, int j) {
+    ColInfo q;
+    size_t c = c2(d, i, j);
+    q.i = i; q.j = j; q.KK = d.kk[c]; q.VI = d.vi[c];
+    q.cutk = d.cutk[c]; q.cov = d.cov[c]; q.ht = d.ht[c]; q.ho = d.ho[c]; q.z0s = d.z0s[c];
+    q.KSTART = (q.cutk == 0) ? q.KK + 1 : 1;
+    q.vegp = d.vegp != nullptr && d.vegp[c] != 0;
+    q.as = (d.adv[c2(d, i, j - 1)] < 1) || (j == 2);
+    q.an = (d.adv[c2(d, i, j + 1)] < 1) || (j == d.njj - 1);
+    q.aw = (d.adv[c2(d, i - 1, j)] < 1) || (i == 2);
+    q.ae = (d.adv[c2(d, i + 1, j)] < 1) || (i == d.nii - 1);
+    q.W = d.simd;
+    return q;
+}
+
+// mixing-length eddy viscosity (U-prognostic-microscale_1.cs:216-268), lane-local
+__device__ __forceinline__ float ml_visc(const Dom &d, const ColInfo &q, int k, const float *UK, const float *VK) {
+    if (!(k > q.KK + 1)) return 0.0f;
+    float dz = d.dzk[k];
+    float zs = d.hok[k] - d.hok[q.KK] - dz * 0.5f;
+    float mixL = vminf(0.071f * zs, 100.0f);
+    if (q.cov > 0) mixL *= (1.0f - 0.99f * q.cov);
+    float in = (UK[k + 1] - UK[k - 1]) / (2.0f * dz);
+    float dudz = in * in;
+    in = (VK[k + 1] - VK[k - 1]) / (2.0f * dz);
+    float dvdz = in * in;
+    return vminf(mixL * mixL * sqrtf(0.5f * (dudz + dvdz)), 15.0f);
+}
+
+// power-law weight D*max(0,(1-0.1P)^5) in float (Vector path)
+__device__ __forceinline__ float plaw_f(float D, float P) {
+    float in = 1.0f - 0.1f * P;
+    return D * vmaxf(0.0f, in * in * in * in * in);
+}
+
+struct Coef { float A, B, C, D; };
+
+// Diffusion, advection and power-law coefficients shared by the three
+// mixing-length solvers.  F* are the face fluxes of the component.
+__device__ __forceinline__ void ml_coeffs(const Dom &d, const ColInfo &q, int k, float VIS, float FE, float FW,
+                                          float FS, float FN, float FT, float FB, float &AW1, float &AS1,
+                                          float &AE1, float &AN1, float &BIM, float &CIM, float &AIM) {
+    float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
+    float mv = vmaxf(VIS, d.vishmin);
+    float DE = mv * dykdzk / d.dxk, DW = DE;
+    float DS = mv * dxkdzk / d.dyk, DN = DS;
+    float DT = mv * d.area / dz;
+    float DB = (k > q.KK + 1) ? DT : 0.0f;
+    if (q.as) DS = 0; if (q.an) DN = 0; if (q.aw) DW = 0; if (q.ae) DE = 0;
+    DE = vmaxf(DE, 0.0001f); DB = vmaxf(DB, 0.0001f); DW = vmaxf(DW, 0.0001f);
+    DS = vmaxf(DS, 0.0001f); DN = vmaxf(DN, 0.0001f); DT = vmaxf(DT, 0.0001f);
+    float PE = fabsf(FE / DE), PB = fabsf(FB / DB), PW = fabsf(FW / DW);
+    float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
+    BIM = plaw_f(DT, PT) + vmaxf(-FT, 0.0f);
+    CIM = plaw_f(DB, PB) + vmaxf(FB, 0.0f);
+    AE1 = plaw_f(DE, PE) + vmaxf(-FE, 0.0f);
+    AW1 = plaw_f(DW, PW) + vmaxf(FW, 0.0f);
+    AS1 = plaw_f(DS, PS) + vmaxf(FS, 0.0f);
+    AN1 = plaw_f(DN, PN) + vmaxf(-FN, 0.0f);
+    AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + d.ap0[k];
+}
+
+// wall function at k = KKART+1 for U and V (U-prognostic-microscale_1.cs:419-450)
+__device__ __forceinline__ float ml_wall_uv(const Dom &d, const ColInfo &q, int k, float vel, const float *UKS,
+                                            const float *UKS_W, const float *VKS, const float *VKS_W) {
+    float in = 0.5f * (UKS_W[k] + UKS[k]);
+    float in2 = 0.5f * (VKS_W[k] + VKS[k]);
+    float wh = vmaxf(sqrtf(in * in + in2 * in2), 0.1f);
+    float us;
+    if (q.cutk < 1) {
+        us = q.ht * wh;
+        if (q.z0s >= d.dzk[k] * 0.1f) {
+            int ind = k + 1;
+            float a = 0.5f * (UKS_W[ind] + UKS[ind]);
+            float b = 0.5f * (VKS_W[ind] + VKS[ind]);
+            us = q.ht * sqrtf(a * a + b * b);
+        }
+    } else {
+        us = q.ho * wh;
+    }
+    return vel / wh * (us * us) * d.area;
+}
+
+__device__ Coef u1_cell(const Dom &d, const ColInfo &q, int k) {
+    int i = q.i, j = q.j;
+    const float *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j);
+    const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    const float *UK_W = d.uk + c3(d, i - 1, j), *UK_E = d.uk + c3(d, i + 1, j);
+    const float *UK_S = d.uk + c3(d, i, j - 1), *UK_N = d.uk + c3(d, i, j + 1);
+    const float *UKS_W = d.uks + c3(d, i - 1, j);
+    const float *VKS_W = d.vks + c3(d, i - 1, j), *VKS_WS = d.vks + c3(d, i - 1, j - 1);
+    const float *VKS_S = d.vks + c3(d, i, j - 1), *VKS_WN = d.vks + c3(d, i - 1, j + 1);
+    const float *VKS_N = d.vks + c3(d, i, j + 1), *VKS_EN = d.vks + c3(d, i + 1, j + 1);
+    const float *VKS_ES = d.vks + c3(d, i + 1, j - 1);
+    const float *WKS_W = d.wks + c3(d, i - 1, j), *WK_E = d.wk + c3(d, i + 1, j);
+    const float *DPMNEW = d.dpmnew + c3(d, i, j), *DPMNEW_W = d.dpmnew + c3(d, i - 1, j);
+    float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
+    float VIS = ml_visc(d, q, k, UK, VK);
+    float FE = UKS[k] * dykdzk;
+    float FW = UKS_W[k] * dykdzk;
+    float FS = 0.25f * (VKS_W[k] + VKS[k] + VKS_WS[k] + VKS_S[k]) * dxkdzk;
+    float FN = 0.25f * (VKS_W[k] + VKS[k] + VKS_WN[k] + VKS_N[k]) * dxkdzk;
+    float FT = 0.25f * (WKS_W[k] + WKS[k] + WKS_W[k + 1] + WKS[k + 1]) * d.area;
+    float FB = (k > q.KK + 1) ? 0.25f * (WKS_W[k] + WKS[k] + WKS_W[k - 1] + WKS[k - 1]) * d.area : 0.0f;
+    float AW1, AS1, AE1, AN1, BIM, CIM, AIM;
+    ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, AW1, AS1, AE1, AN1, BIM, CIM, AIM);
+    float ap0 = d.ap0[k];
+    float ddpx = DPMNEW_W[k] - DPMNEW[k];
+    float u = UK[k];
+    float D;
+    if (q.vegp) {
+        float in2 = 0.5f * (VKS_W[k] + VKS[k]);
+        float in = 0.5f * (UKS_W[k] + UKS[k]);
+        float wh = vmaxf(sqrtf(in * in + in2 * in2), 0.01f);
+        float veg = d.veg[c3(d, i, j) + k - 1];
+        D = AW1 * UK_W[k] + AS1 * UK_S[k] + AE1 * UK_E[k] + AN1 * UK_N[k] + ap0 * 0.5f * (UKS_W[k] + UKS[k]) +
+            ddpx * dykdzk + (d.cor * (d.ug - u) - veg * u * wh) * d.area * dz;
+    } else {
+        D = AW1 * UK_W[k] + AS1 * UK_S[k] + AE1 * UK_E[k] + AN1 * UK_N[k] + ap0 * 0.5f * (UKS_W[k] + UKS[k]) +
+            ddpx * dykdzk + d.cor * (d.ug - u) * d.area * dz;
+    }
+    if (k == q.KK + 1) D -= ml_wall_uv(d, q, k, u, UKS, UKS_W, VKS, VKS_W);
+    if (block_base(k, q.KSTART, q.VI + 1, q.W) > q.KK + 1) {
+        float DUDXE = (UK_E[k] - u) * dykdzk;
+        float DUDXW = (u - UK_W[k]) * dykdzk;
+        float DVDXN = (0.5f * (VKS_EN[k] + VKS_N[k]) - 0.5f * (VKS_N[k] + VKS_WN[k])) * dxkdzk;
+        float DVDXS = (0.5f * (VKS_ES[k] + VKS_S[k]) - 0.5f * (VKS_S[k] + VKS_WS[k])) * dxkdzk;
+        float DWDXT = (0.5f * (WK_E[k] + WKS[k]) - 0.5f * (WKS[k] + WKS_W[k])) * d.area;
+        float DWDXB = (0.5f * (WK_E[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKS_W[k - 1])) * d.area;
+        D += VIS / d.dxk * (DUDXE - DUDXW + DVDXN - DVDXS + DWDXT - DWDXB);
+    }
+    return Coef{AIM, BIM, CIM, D};
+}
+
+__device__ Coef v1_cell(const Dom &d, const ColInfo &q, int k) {
+    int i = q.i, j = q.j;
+    const float *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j);
+    const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    const float *VK_W = d.vk + c3(d, i - 1, j), *VK_E = d.vk + c3(d, i + 1, j);
+    const float *VK_S = d.vk + c3(d, i, j - 1), *VK_N = d.vk + c3(d, i, j + 1);
+    const float *UKS_W = d.uks + c3(d, i - 1, j), *UKS_E = d.uks + c3(d, i + 1, j);
+    const float *UKS_S = d.uks + c3(d, i, j - 1), *UKS_ES = d.uks + c3(d, i + 1, j - 1);
+    const float *UKS_WS = d.uks + c3(d, i - 1, j - 1), *UKS_EN = d.uks + c3(d, i + 1, j + 1);
+    const float *UKS_WN = d.uks + c3(d, i - 1, j + 1);
+    const float *VKS_S = d.vks + c3(d, i, j - 1), *VKS_W = d.vks + c3(d, i - 1, j);
+    const float *WKS_S = d.wks + c3(d, i, j - 1), *WK_N = d.wk + c3(d, i, j + 1);
+    const float *DPMNEW = d.dpmnew + c3(d, i, j), *DPMNEW_S = d.dpmnew + c3(d, i, j - 1);
+    float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
+    float VIS = ml_visc(d, q, k, UK, VK);
+    float FE = 0.25f * (UKS[k] + UKS_E[k] + UKS_S[k] + UKS_ES[k]) * dykdzk;
+    float FW = 0.25f * (UKS[k] + UKS_W[k] + UKS_S[k] + UKS_WS[k]) * dykdzk;
+    float FS = VKS_S[k] * dxkdzk;
+    float FN = VKS[k] * dxkdzk;
+    float FT = 0.25f * (WKS_S[k] + WKS[k] + WKS_S[k + 1] + WKS[k + 1]) * d.area;
+    float FB = (k > q.KK + 1) ? 0.25f * (WKS_S[k] + WKS[k] + WKS_S[k - 1] + WKS[k - 1]) * d.area : 0.0f;
+    float AW1, AS1, AE1, AN1, BIM, CIM, AIM;
+    ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, AW1, AS1, AE1, AN1, BIM, CIM, AIM);
+    float ap0 = d.ap0[k];
+    float ddpy = DPMNEW_S[k] - DPMNEW[k];
+    float v = VK[k];
+    float D;
+    if (q.vegp) {
+        float in2 = 0.5f * (VKS_W[k] + VKS[k]);
+        float in = 0.5f * (UKS_W[k] + UKS[k]);
+        float wh = vmaxf(sqrtf(in * in + in2 * in2), 0.01f);
+        float veg = d.veg[c3(d, i, j) + k - 1];
+        D = AW1 * VK_W[k] + AS1 * VK_S[k] + AE1 * VK_E[k] + AN1 * VK_N[k] + ap0 * 0.5f * (VKS_S[k] + VKS[k]) +
+            ddpy * dxkdzk + (d.cor * (d.vg - v) - veg * v * wh) * d.area * dz;
+    } else {
+        D = AW1 * VK_W[k] + AS1 * VK_S[k] + AE1 * VK_E[k] + AN1 * VK_N[k] + ap0 * 0.5f * (VKS_S[k] + VKS[k]) +
+            ddpy * dxkdzk + d.cor * (d.vg - v) * d.area * dz;
+    }
+    if (k == q.KK + 1) D -= ml_wall_uv(d, q, k, v, UKS, UKS_W, VKS, VKS_W);
+    if (block_base(k, q.KSTART, q.VI + 1, q.W) > q.KK + 1) {
+        float DVDYN = (VK_N[k] - v) * dxkdzk;
+        float DVDYS = (v - VK_S[k]) * dxkdzk;
+        float DUDYE = (0.5f * (UKS_EN[k] + UKS_E[k]) - 0.5f * (UKS_E[k] + UKS_ES[k])) * dxkdzk;
+        float DUDYW = (0.5f * (UKS_WN[k] + UKS_W[k]) - 0.5f * (UKS_W[k] + UKS_WS[k])) * dxkdzk;
+        float DWDYT = (0.5f * (WK_N[k] + WKS[k]) - 0.5f * (WKS[k] + WKS_S[k])) * d.area;
+        // the reference takes WKS of the southern column at k, not k-1, here
+        float DWDYB = (0.5f * (WK_N[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKS_S[k])) * d.area;
+        D += VIS / d.dxk * (DUDYE - DUDYW + DVDYN - DVDYS + DWDYT - DWDYB);
+    }
+    return Coef{AIM, BIM, CIM, D};
+}
+
+__device__ Coef w1_cell(const Dom &d, const ColInfo &q, int k) {
+    int i = q.i, j = q.j;
+    const float *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j), *WK = d.wk + c3(d, i, j);
+    const float *WK_W = d.wk + c3(d, i - 1, j), *WK_E = d.wk + c3(d, i + 1, j);
+    const float *WK_S = d.wk + c3(d, i, j - 1), *WK_N = d.wk + c3(d, i, j + 1);
+    const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    const float *UKS_W = d.uks + c3(d, i - 1, j), *UKS_E = d.uks + c3(d, i + 1, j);
+    const float *VKS_S = d.vks + c3(d, i, j - 1), *VKS_N = d.vks + c3(d, i, j + 1);
+    const float *DPMNEW = d.dpmnew + c3(d, i, j);
+    float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
+    float VIS = ml_visc(d, q, k, UK, VK);
+    float FE = 0.25f * (UKS[k] + UKS_E[k] + UKS[k - 1] + UKS_E[k - 1]) * dykdzk;
+    float FW = 0.25f * (UKS[k] + UKS_W[k] + UKS[k - 1] + UKS_W[k - 1]) * dykdzk;
+    float FS = 0.25f * (VKS[k] + VKS_S[k] + VKS[k - 1] + VKS_S[k - 1]) * dxkdzk;
+    float FN = 0.25f * (VKS[k] + VKS_N[k] + VKS[k - 1] + VKS_N[k - 1]) * dxkdzk;
+    float FT = WKS[k] * d.area;
+    float FB = (k > q.KK + 1) ? WKS[k - 1] * d.area : 0.0f;
+    float AW1, AS1, AE1, AN1, BIM, CIM, AIM;
+    ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, AW1, AS1, AE1, AN1, BIM, CIM, AIM);
+    float ap0 = d.ap0[k];
+    float ddpz = DPMNEW[k - 1] - DPMNEW[k];
+    float w = WK[k];
+    float D;
+    if (q.vegp) {
+        float in2 = 0.5f * (VKS_S[k] + VKS[k]);
+        float in = 0.5f * (UKS_W[k] + UKS[k]);
+        float wh = vmaxf(sqrtf(in * in + in2 * in2), 0.01f);
+        float veg = d.veg[c3(d, i, j) + k - 1];
+        D = AW1 * WK_W[k] + AS1 * WK_S[k] + AE1 * WK_E[k] + AN1 * WK_N[k] + ap0 * 0.5f * (WKS[k - 1] + WKS[k]) +
+            ddpz * d.area - veg * w * wh * d.area * dz;
+    } else {
+        D = AW1 * WK_W[k] + AS1 * WK_S[k] + AE1 * WK_E[k] + AN1 * WK_N[k] + ap0 * 0.5f * (WKS[k - 1] + WKS[k]) +
+            ddpz * d.area;
+    }
+    if (block_base(k, q.KSTART, q.VI + 1, q.W) > q.KK + 1) {
+        float DUDZE = (0.5f * (UKS[k + 1] + UKS[k]) - 0.5f * (UKS[k] + UKS[k - 1])) * dykdzk;
+        float DUDZW = (0.5f * (UKS_W[k + 1] + UKS_W[k]) - 0.5f * (UKS_W[k] + UKS_W[k - 1])) * dykdzk;
+        float DVDZN = (0.5f * (VKS[k + 1] + VKS[k]) - 0.5f * (VKS[k] + VKS[k - 1])) * dxkdzk;
+        float DVDZS = (0.5f * (VKS_S[k + 1] + VKS_S[k]) - 0.5f * (VKS_S[k] + VKS_S[k - 1])) * dxkdzk;
+        float DWDZT = (WK[k + 1] - w) * d.area;
+        float DWDZB = (w - WK[k - 1]) * d.area;
+        D += VIS / dz * (DUDZE - DUDZW + DVDZN - DVDZS + DWDZT - DWDZB);
+    }
+    return Coef{AIM, BIM, CIM, D};
+}
+
+// ------------------------------------------------------------------------
+// k-eps (model 2) and no-diffusion (model 0) momentum, _2.cs / _0.cs
+// ------------------------------------------------------------------------
+__device__ __forceinline__ float plaw_d(float D, float P) {   // D*FloatMax(0,(float)Pow5(1-0.1*P))
+    double x = 1.0 - 0.1 * (double)P;
+    return D * vmaxf(0.0f, (float)(x * x * x * x * x));
+}
+
+// wall function of the scalar U/V solvers (U-prognostic-microscale_2.cs:188-212)
+__device__ __forceinline__ float ke_wall_uv(const Dom &d, const ColInfo &q, int k, float vel, const float *UKSim,
+                                            const float *UKS, const float *VKSim, const float *VKS) {
+    double a = 0.5 * (double)(UKSim[k] + UKS[k]);
+    double b = 0.5 * (double)(VKSim[k] + VKS[k]);
+    float wh = vmaxf((float)sqrt(a * a + b * b), 0.01f);
+    float us;
+    if (q.cutk < 1) {
+        us = q.ht * wh;
+        if ((double)q.z0s >= (double)d.dzk[k] * 0.1) {
+            double a1 = 0.5 * (double)(UKSim[k + 1] + UKS[k + 1]);
+            double b1 = 0.5 * (double)(VKSim[k + 1] + VKS[k + 1]);
+            us = q.ht * (float)sqrt(a1 * a1 + b1 * b1);
+        }
+    } else {
+        us = q.ho * wh;
+    }
+    return vel / wh * (us * us) * d.area;
+}
+
+template <bool IS_V, bool DIFF>
+__device__ Coef ke_uv_cell(const Dom &d, const ColInfo &q, int k) {
+    int i = q.i, j = q.j;
+    const float *X = (IS_V ? d.vk : d.uk) + c3(d, i, j);
+    const float *Xim = (IS_V ? d.vk : d.uk) + c3(d, i - 1, j), *Xip = (IS_V ? d.vk : d.uk) + c3(d, i + 1, j);
+    const float *Xjm = (IS_V ? d.vk : d.uk) + c3(d, i, j - 1), *Xjp = (IS_V ? d.vk : d.uk) + c3(d, i, j + 1);
+    const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    const float *UKSim = d.uks + c3(d, i - 1, j), *VKSim = d.vks + c3(d, i - 1, j);
+    float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
+    float VIS = 0, DE = 0, DW = 0, DS = 0, DN = 0, DT = 0, DB = 0;
+    if (DIFF) {
+        float zs = d.hok[k] - d.hok[q.KK] - dz * 0.5f;
+        VIS = sqrtf(d.turb[c3(d, i, j) + k]) * zs * 0.09f;
+        DE = vmaxf(VIS, d.vishmin) * dykdzk / d.dxk; DW = DE;
+        DS = vmaxf(VIS, d.vishmin) * dxkdzk / d.dyk; DN = DS;
+        DT = vmaxf(VIS, d.vishmin) * d.area / dz;
+        if (k > q.KK + 1) DB = DT;
+        if (q.as) DS = 0; if (q.an) DN = 0; if (q.aw) DW = 0; if (q.ae) DE = 0;
+    }
+    float FE, FW, FS, FN, FT, FB = 0;
+    if (!IS_V) {
+        const float *VKSimjm = d.vks + c3(d, i - 1, j - 1), *VKSjm = d.vks + c3(d, i, j - 1);
+        const float *VKSimjp = d.vks + c3(d, i - 1, j + 1), *VKSjp = d.vks + c3(d, i, j + 1);
+        const float *WKSim = d.wks + c3(d, i - 1, j);
+        FE = UKS[k] * dykdzk;
+        FW = UKSim[k] * dykdzk;
+        FS = 0.25f * (VKSim[k] + VKS[k] + VKSimjm[k] + VKSjm[k]) * dxkdzk;
+        FN = 0.25f * (VKSim[k] + VKS[k] + VKSimjp[k] + VKSjp[k]) * dxkdzk;
+        FT = 0.25f * (WKSim[k] + WKS[k] + WKSim[k + 1] + WKS[k + 1]) * d.area;
+        if (k > q.KK + 1) FB = 0.25f * (WKSim[k] + WKS[k] + WKSim[k - 1] + WKS[k - 1]) * d.area;
+    } else {
+        const float *UKSip = d.uks + c3(d, i + 1, j), *UKSjm = d.uks + c3(d, i, j - 1);
+        const float *UKSipjm = d.uks + c3(d, i + 1, j - 1), *UKSimjm = d.uks + c3(d, i - 1, j - 1);
+        const float *VKSjm = d.vks + c3(d, i, j - 1), *WKSjm = d.wks + c3(d, i, j - 1);
+        FE = 0.25f * (UKS[k] + UKSip[k] + UKSjm[k] + UKSipjm[k]) * dykdzk;
+        FW = 0.25f * (UKS[k] + UKSim[k] + UKSjm[k] + UKSimjm[k]) * dykdzk;
+        FS = VKSjm[k] * dxkdzk;
+        FN = VKS[k] * dxkdzk;
+        FT = 0.25f * (WKSjm[k] + WKS[k] + WKSjm[k + 1] + WKS[k + 1]) * d.area;
+        if (k > q.KK + 1) FB = 0.25f * (WKSjm[k] + WKS[k] + WKSjm[k - 1] + WKS[k - 1]) * d.area;
+    }
+    float BIM, CIM, AE1, AW1, AS1, AN1;
+    if (DIFF) {
+        DE = vmaxf(DE, 0.0001f); DB = vmaxf(DB, 0.0001f); DW = vmaxf(DW, 0.0001f);
+        DS = vmaxf(DS, 0.0001f); DN = vmaxf(DN, 0.0001f); DT = vmaxf(DT, 0.0001f);
+        float PE = fabsf(FE / DE), PB = fabsf(FB / DB), PW = fabsf(FW / DW);
+        float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
+        BIM = plaw_d(DT, PT) + vmaxf(-FT, 0.0f);
+        CIM = plaw_d(DB, PB) + vmaxf(FB, 0.0f);
+        AE1 = plaw_d(DE, PE) + vmaxf(-FE, 0.0f);
+        AW1 = plaw_d(DW, PW) + vmaxf(FW, 0.0f);
+        AS1 = plaw_d(DS, PS) + vmaxf(FS, 0.0f);
+        AN1 = plaw_d(DN, PN) + vmaxf(-FN, 0.0f);
+    } else {
+        BIM = vmaxf(-FT, 0.0f); CIM = vmaxf(FB, 0.0f); AE1 = vmaxf(-FE, 0.0f);
+        AW1 = vmaxf(FW, 0.0f); AS1 = vmaxf(FS, 0.0f); AN1 = vmaxf(-FN, 0.0f);
+    }
+    float ap0 = d.ap0[k];
+    float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + ap0;
+    const float *DPMNEW = d.dpmnew + c3(d, i, j);
+    const float *DPMNEWm = d.dpmnew + (IS_V ? c3(d, i, j - 1) : c3(d, i - 1, j));
+    float ddp = DPMNEWm[k] - DPMNEW[k];
+    float x = X[k];
+    float partial = AW1 * Xim[k] + AS1 * Xjm[k] + AE1 * Xip[k] + AN1 * Xjp[k];
+    float ssum = IS_V ? (d.vks[c3(d, i, j - 1) + k] + VKS[k]) : (UKSim[k] + UKS[k]);
+    double t5 = (double)ap0 * 0.5 * (double)ssum;
+    float t6 = ddp * (IS_V ? dxkdzk : dykdzk);
+    float t7 = d.cor * ((IS_V ? d.vg : d.ug) - x) * d.area * dz;
+    float D = (float)((double)partial + t5 + (double)t6 + (double)t7);
+    if (k == q.KK + 1) D -= ke_wall_uv(d, q, k, x, UKSim, UKS, VKSim, VKS);
+    if (DIFF && k > q.KK + 1) {
+        float ADD;
+        if (!IS_V) {
+            const float *VKSipjp = d.vks + c3(d, i + 1, j + 1), *VKSjp = d.vks + c3(d, i, j + 1);
+            const float *VKSimjp = d.vks + c3(d, i - 1, j + 1), *VKSipjm = d.vks + c3(d, i + 1, j - 1);
+            const float *VKSjm = d.vks + c3(d, i, j - 1), *VKSimjm = d.vks + c3(d, i - 1, j - 1);
+            const float *WKip = d.wk + c3(d, i + 1, j), *WKSim = d.wks + c3(d, i - 1, j);
+            float DUDXE = (Xip[k] - x) * dykdzk;
+            float DUDXW = (x - Xim[k]) * dykdzk;
+            float DVDXN = (0.5f * (VKSipjp[k] + VKSjp[k]) - 0.5f * (VKSjp[k] + VKSimjp[k])) * dxkdzk;
+            float DVDXS = (0.5f * (VKSipjm[k] + VKSjm[k]) - 0.5f * (VKSjm[k] + VKSimjm[k])) * dxkdzk;
+            float DWDXT = (0.5f * (WKip[k] + WKS[k]) - 0.5f * (WKS[k] + WKSim[k])) * d.area;
+            float DWDXB = (0.5f * (WKip[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKSim[k - 1])) * d.area;
+            ADD = VIS / d.dxk * (DUDXE - DUDXW + DVDXN - DVDXS + DWDXT - DWDXB);
+        } else {
+            const float *UKSipjp = d.uks + c3(d, i + 1, j + 1), *UKSip = d.uks + c3(d, i + 1, j);
+            const float *UKSipjm = d.uks + c3(d, i + 1, j - 1), *UKSimjp = d.uks + c3(d, i - 1, j + 1);
+            const float *UKSimjm = d.uks + c3(d, i - 1, j - 1);
+            const float *WKjp = d.wk + c3(d, i, j + 1), *WKSjm = d.wks + c3(d, i, j - 1);
+            float DVDYN = (Xjp[k] - x) * dxkdzk;
+            float DVDYS = (x - Xjm[k]) * dxkdzk;
+            float DUDYE = (0.5f * (UKSipjp[k] + UKSip[k]) - 0.5f * (UKSip[k] + UKSipjm[k])) * dxkdzk;
+            float DUDYW = (0.5f * (UKSimjp[k] + UKSim[k]) - 0.5f * (UKSim[k] + UKSimjm[k])) * dxkdzk;
+            float DWDYT = (0.5f * (WKjp[k] + WKS[k]) - 0.5f * (WKS[k] + WKSjm[k])) * d.area;
+            float DWDYB = (0.5f * (WKjp[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKSjm[k - 1])) * d.area;
+            ADD = VIS / d.dxk * (DUDYE - DUDYW + DVDYN - DVDYS + DWDYT - DWDYB);
+        }
+        D += ADD;
+    }
+    return Coef{AIM, BIM, CIM, D};
+}
+
+template <bool DIFF>
+__device__ Coef ke_w_cell(const Dom &d, const ColInfo &q, int k) {
+    int i = q.i, j = q.j;
+    const float *WK = d.wk + c3(d, i, j);
+    const float *WKim = d.wk + c3(d, i - 1, j), *WKip = d.wk + c3(d, i + 1, j);
+    const float *WKjm = d.wk + c3(d, i, j - 1), *WKjp = d.wk + c3(d, i, j + 1);
+    const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    const float *UKSim = d.uks + c3(d, i - 1, j), *UKSip = d.uks + c3(d, i + 1, j);
+    const float *VKSjm = d.vks + c3(d, i, j - 1), *VKSjp = d.vks + c3(d, i, j + 1);
+    const float *DPMNEW = d.dpmnew + c3(d, i, j);
+    float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
+    float uks = UKS[k], uks_m = UKS[k - 1], vks = VKS[k], vks_m = VKS[k - 1];
+    float VIS = 0, DE = 0, DW = 0, DS = 0, DN = 0, DT = 0, DB = 0;
+    if (DIFF) {
+        float zs = d.hok[k] - d.hok[q.KK] - dz * 0.5f;
+        VIS = sqrtf(d.turb[c3(d, i, j) + k]) * zs * 0.09f;
+        DE = vmaxf(VIS, d.vishmin) * dykdzk / d.dxk; DW = DE;
+        DS = vmaxf(VIS, d.vishmin) * dxkdzk / d.dyk; DN = DS;
+        DT = vmaxf(VIS, d.vishmin) * d.area / dz;
+        if (k > q.KK + 1) DB = DT;
+        if (q.as) DS = 0; if (q.an) DN = 0; if (q.aw) DW = 0; if (q.ae) DE = 0;
+    }
+    float FE = 0.25f * (uks + UKSip[k] + uks_m + UKSip[k - 1]) * dykdzk;
+    float FW = 0.25f * (uks + UKSim[k] + uks_m + UKSim[k - 1]) * dykdzk;
+    float FS = 0.25f * (vks + VKSjm[k] + vks_m + VKSjm[k - 1]) * dxkdzk;
+    float FN = 0.25f * (vks + VKSjp[k] + vks_m + VKSjp[k - 1]) * dxkdzk;
+    float FT = WKS[k] * d.area;
+    float FB = (k > q.KK + 1) ? WKS[k - 1] * d.area : 0.0f;
+    float BIM, CIM, AE1, AW1, AS1, AN1;
+    if (DIFF) {
+        DE = vmaxf(DE, 0.0001f); DB = vmaxf(DB, 0.0001f); DW = vmaxf(DW, 0.0001f);
+        DS = vmaxf(DS, 0.0001f); DN = vmaxf(DN, 0.0001f); DT = vmaxf(DT, 0.0001f);
+        float PE = fabsf(FE / DE), PB = fabsf(FB / DB), PW = fabsf(FW / DW);
+        float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
+        BIM = plaw_d(DT, PT) + vmaxf(-FT, 0.0f);
+        CIM = plaw_d(DB, PB) + vmaxf(FB, 0.0f);
+        AE1 = plaw_d(DE, PE) + vmaxf(-FE, 0.0f);
+        AW1 = plaw_d(DW, PW) + vmaxf(FW, 0.0f);
+        AS1 = plaw_d(DS, PS) + vmaxf(FS, 0.0f);
+        AN1 = plaw_d(DN, PN) + vmaxf(-FN, 0.0f);
+    } else {
+        BIM = vmaxf(-FT, 0.0f); CIM = vmaxf(FB, 0.0f); AE1 = vmaxf(-FE, 0.0f);
+        AW1 = vmaxf(FW, 0.0f); AS1 = vmaxf(FS, 0.0f); AN1 = vmaxf(-FN, 0.0f);
+    }
+    float ap0 = d.ap0[k];
+    float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + ap0;
+    float ddpz = DPMNEW[k - 1] - DPMNEW[k];
+    float partial = AW1 * WKim[k] + AS1 * WKjm[k] + AE1 * WKip[k] + AN1 * WKjp[k];
+    float D = (float)((double)partial + (double)ap0 * 0.5 * (double)(WKS[k - 1] + WKS[k]) + (double)(ddpz * d.area));
+    if (DIFF && k > q.KK + 1) {
+        float DUDZE = (0.5f * (UKS[k + 1] + uks) - 0.5f * (uks + uks_m)) * dykdzk;
+        float DUDZW = (0.5f * (UKSim[k + 1] + UKSim[k]) - 0.5f * (UKSim[k] + UKSim[k - 1])) * dykdzk;
+        float DVDZN = (0.5f * (VKS[k + 1] + vks) - 0.5f * (vks + vks_m)) * dxkdzk;
+        float DVDZS = (0.5f * (VKSjm[k + 1] + VKSjm[k]) - 0.5f * (VKSjm[k] + VKSjm[k - 1])) * dxkdzk;
+        float DWDZT = (WK[k + 1] - WK[k]) * d.area;
+        float DWDZB = (WK[k] - WK[k - 1]) * d.area;
+        D += VIS / dz * (DUDZE - DUDZW + DVDZN - DVDZS + DWDZT - DWDZB);
+    }
+    return Coef{AIM, BIM, CIM, D};
+}
+
+// ------------------------------------------------------------------------
+// Momentum column kernel: one wavefront per column of diagonal t.
+//   COMP 0:U 1:V 2:W;  KIND 0: mixing length (Vector<float> path, TERMP form)
+//   1: k-eps, 2: no diffusion (scalar path, division form)
+// ------------------------------------------------------------------------
+template <int COMP, int KIND>
+__global__ __launch_bounds__(WAVE) void k_momentum(Dom d, int t, int IS, int JS) {
+    extern __shared__ float sm[];
+    const int nkp = d.nkp, tid = threadIdx.x;
+    // sweep ranges (outer i1, inner j1): U i1 from 3; V j1 from 3
+    const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2;
+    int a, b;
+    if (!diag_col(t, oa, d.nii - 1, ia, d.njj - 1, blockIdx.x, a, b)) return;
+    int i = (IS == -1) ? d.nii - a + 1 : a;
+    int j = (JS == -1) ? d.njj - b + 1 : b;
+    if (d.adv[c2(d, i, j)] != 1) return;
+    ColInfo q = col_info(d, i, j);
+    float *sA = sm, *sB = sA + nkp, *sC = sB + nkp, *sD = sC + nkp, *sX = sD + nkp;
+    float *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk) + c3(d, i, j);
+    for (int k = q.KSTART + tid; k <= q.VI; k += WAVE) {
+        Coef cf;
+        if (KIND == 0) cf = (COMP == 0) ? u1_cell(d, q, k) : (COMP == 1) ? v1_cell(d, q, k) : w1_cell(d, q, k);
+        else if (KIND == 1) cf = (COMP == 0) ? ke_uv_cell<false, true>(d, q, k)
+                               : (COMP == 1) ? ke_uv_cell<true, true>(d, q, k) : ke_w_cell<true>(d, q, k);
+        else cf = (COMP == 0) ? ke_uv_cell<false, false>(d, q, k)
+                  : (COMP == 1) ? ke_uv_cell<true, false>(d, q, k) : ke_w_cell<false>(d, q, k);
+        sA[k] = cf.A; sB[k] = cf.B; sC[k] = cf.C; sD[k] = cf.D;
+    }
+    for (int k = q.KSTART + tid; k <= q.VI + 1; k += WAVE) sX[k] = X[k];
+    __syncthreads();
+    if (tid == 0) {
+        const int KKP1 = q.KK + 1;
+        float Pp = 0, Qp = 0;
+        for (int k = q.KSTART; k <= q.VI; ++k) {
+            float A = sA[k], B = sB[k], Cc = sC[k], Dd = sD[k], P, Q;
+            if (KIND == 0) {
+                if (k > KKP1) { float T = 1 / (A - Cc * Pp); P = B * T; Q = (Dd + Cc * Qp) * T; }
+                else { float T = 1 / A; P = B * T; Q = Dd * T; }
+            } else {
+                if (k > KKP1) { P = B / (A - Cc * Pp); Q = (Dd + Cc * Qp) / (A - Cc * Pp); }
+                else { P = B / A; Q = Dd / A; }
+            }
+            sA[k] = P; sB[k] = Q;
+            Pp = P; Qp = Q;
+        }
+        // relaxed back-substitution; U/V skip levels inside the building or its
+        // upstream neighbour, W updates every level (W-prognostic-microscale_1.cs:445-448)
+        int KKm = q.KK;
+        if (COMP == 0) KKm = max(q.KK, d.kk[c2(d, i - 1, j)]);
+        if (COMP == 1) KKm = max(q.KK, d.kk[c2(d, i, j - 1)]);
+        float up = sX[q.VI + 1];
+        for (int k = q.VI; k >= q.KSTART; --k) {
+            float x = sX[k];
+            if (COMP == 2 || KKm < k) {
+                x = x + d.relax * (sA[k] * up + sB[k] - x);
+                sX[k] = x;
+            }
+            up = x;
+        }
+    }
+    __syncthreads();
+    for (int k = q.KSTART + tid; k <= q.VI; k += WAVE) X[k] = sX[k];
+}
+
+// ------------------------------------------------------------------------
+// TKE / dissipation (k-eps), TKE-prognostic-microscale.cs:25-352
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(WAVE) void k_tke(Dom d, int t, int IS, int JS) {
+    extern __shared__ float sm[];
+    const int nkp = d.nkp, tid = threadIdx.x;
+    int a, b;
+    if (!diag_col(t, 2, d.nii - 1, 2, d.njj - 1, blockIdx.x, a, b)) return;
+    int i = (IS == -1) ? d.nii - a + 1 : a;
+    int j = (JS == -1) ? d.njj - b + 1 : b;
+    size_t c = c2(d, i, j);
+    if (d.adv[c] != 1) return;
+    ColInfo q = col_info(d, i, j);
+    float *sA = sm, *sB1 = sA + nkp, *sC1 = sB1 + nkp, *sD1 = sC1 + nkp, *sB2 = sD1 + nkp, *sC2 = sB2 + nkp,
+          *sD2 = sC2 + nkp, *sT = sD2 + nkp, *sE = sT + nkp, *sWT = sE + nkp, *sWE = sWT + nkp;
+    int *sWF = (int *)(sWE + nkp);
+    const float DXK = d.dxk, DYK = d.dyk, AREA = d.area, DXK_Rez = 1 / DXK, DYK_Rez = 1 / DYK;
+    const float Ceps = 1, Ceps1 = 1.44f, Ceps2 = 1.92f;
+    size_t o = c3(d, i, j);
+    const float *TURB = d.turb + o, *TDISS = d.tdiss + o;
+    const float *TURBim = d.turb + c3(d, i - 1, j), *TURBip = d.turb + c3(d, i + 1, j);
+    const float *TURBjm = d.turb + c3(d, i, j - 1), *TURBjp = d.turb + c3(d, i, j + 1);
+    const float *TDISSim = d.tdiss + c3(d, i - 1, j), *TDISSip = d.tdiss + c3(d, i + 1, j);
+    const float *TDISSjm = d.tdiss + c3(d, i, j - 1), *TDISSjp = d.tdiss + c3(d, i, j + 1);
+    const float *UK = d.uk + o, *VK = d.vk + o, *WK = d.wk + o;
+    const float *UKip = d.uk + c3(d, i + 1, j), *UKjp = d.uk + c3(d, i, j + 1), *UKipjp = d.uk + c3(d, i + 1, j + 1);
+    const float *UKjm = d.uk + c3(d, i, j - 1), *UKipjm = d.uk + c3(d, i + 1, j - 1);
+    const float *VKjp = d.vk + c3(d, i, j + 1), *VKip = d.vk + c3(d, i + 1, j), *VKipjp = d.vk + c3(d, i + 1, j + 1);
+    const float *VKim = d.vk + c3(d, i - 1, j), *VKimjp = d.vk + c3(d, i - 1, j + 1);
+    const float *WKip = d.wk + c3(d, i + 1, j), *WKim = d.wk + c3(d, i - 1, j);
+    const float *WKjp = d.wk + c3(d, i, j + 1), *WKjm = d.wk + c3(d, i, j - 1);
+    int KK = q.KK;
+    int kkip = d.kk[c2(d, i + 1, j)], kkim = d.kk[c2(d, i - 1, j)], kkjp = d.kk[c2(d, i, j + 1)], kkjm = d.kk[c2(d, i, j - 1)];
+    bool cip = d.cutk[c2(d, i + 1, j)] > 0, cim = d.cutk[c2(d, i - 1, j)] > 0;
+    bool cjp = d.cutk[c2(d, i, j + 1)] > 0, cjm = d.cutk[c2(d, i, j - 1)] > 0;
+    for (int k = q.KSTART + tid; k <= q.VI; k += WAVE) {
+        float dz = d.dzk[k];
+        float dxkdzk = DXK * dz, dykdzk = DYK * dz, DZK_Rez = 1 / dz;
+        float zs = d.hok[k] - d.hok[KK] - dz * 0.5f;
+        float VIS = sqrtf(TURB[k]) * zs * 0.09f;
+        VIS = mminf(VIS, 15.0f);
+        float DE = mmaxf(VIS, d.vishmin) * dykdzk * DXK_Rez, DW = DE;
+        float DS = mmaxf(VIS, d.vishmin) * dxkdzk * DYK_Rez, DN = DS;
+        float DT = mmaxf(VIS, d.vishmin) * AREA * DZK_Rez, DB = 0;
+        float DTEPS = mmaxf(VIS / Ceps, d.visvmin) * AREA * DZK_Rez, DBEPS = 0;
+        if (k > KK + 1) { DB = DT; DBEPS = DTEPS; }
+        if (q.as) DS = 0; if (q.an) DN = 0; if (q.aw) DW = 0; if (q.ae) DE = 0;
+        float FE = UKip[k] * dykdzk, FW = UK[k] * dykdzk, FS = VK[k] * dxkdzk, FN = VKjp[k] * dxkdzk;
+        float FT = WK[k + 1] * AREA, FB = 0;
+        if (k > KK + 1) FB = WK[k] * AREA;
+        DE = mmaxf(DE, 0.0001f); DB = mmaxf(DB, 0.0001f); DW = mmaxf(DW, 0.0001f);
+        DS = mmaxf(DS, 0.0001f); DN = mmaxf(DN, 0.0001f); DT = mmaxf(DT, 0.0001f);
+        DBEPS = mmaxf(DBEPS, 0.0001f); DTEPS = mmaxf(DTEPS, 0.0001f);
+        float PE = fabsf(FE / DE), PB = fabsf(FB / DB), PW = fabsf(FW / DW);
+        float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
+        float PBEPS = fabsf(FN / DBEPS);   // the reference uses FN here
+        float PTEPS = fabsf(FT / DTEPS);
+        float x;
+        x = 1 - 0.1f * PT; float BIM = DT * mmaxf(0, x * x * x * x * x) + mmaxf(-FT, 0);
+        x = 1 - 0.1f * PB; float CIM = DB * mmaxf(0, x * x * x * x * x) + mmaxf(FB, 0);
+        x = 1 - 0.1f * PE; float AE1 = DE * mmaxf(0, x * x * x * x * x) + mmaxf(-FE, 0);
+        x = 1 - 0.1f * PW; float AW1 = DW * mmaxf(0, x * x * x * x * x) + mmaxf(FW, 0);
+        x = 1 - 0.1f * PS; float AS1 = DS * mmaxf(0, x * x * x * x * x) + mmaxf(FS, 0);
+        x = 1 - 0.1f * PN; float AN1 = DN * mmaxf(0, x * x * x * x * x) + mmaxf(-FN, 0);
+        // the reference weights the eps system with DT/DB, not DTEPS/DBEPS
+        x = 1 - 0.1f * PTEPS; float BIMEPS = DT * mmaxf(0, x * x * x * x * x) + mmaxf(-FT, 0);
+        x = 1 - 0.1f * PBEPS; float CIMEPS = DB * mmaxf(0, x * x * x * x * x) + mmaxf(FB, 0);
+        float ap0 = d.ap0[k];
+        float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + ap0;
+        float UX1 = UKip[k], UX2 = UK[k];
+        float VX1 = 0.5f * (VKip[k] + VKipjp[k]), VX2 = 0.5f * (VKim[k] + VKimjp[k]);
+        float WX1 = 0.5f * (WKip[k] + WKip[k + 1]), WX2 = 0.5f * (WKim[k] + WKim[k + 1]);
+        float UY1 = 0.5f * (UKjp[k] + UKipjp[k]), UY2 = 0.5f * (UKjm[k] + UKipjm[k]);
+        float VY1 = VKjp[k], VY2 = VK[k];
+        float WY1 = 0.5f * (WKjp[k] + WKjp[k + 1]), WY2 = 0.5f * (WKjm[k] + WKjm[k + 1]);
+        float DUDX = (UX1 - UX2) * DXK_Rez;
+        float DVDX = (VX1 - VX2) * DXK_Rez * 0.5f;
+        float DWDX = (WX1 - WX2) * DXK_Rez * 0.5f;
+        float DUDY = (UY1 - UY2) * DYK_Rez * 0.5f;
+        float DVDY = (VY1 - VY2) * DYK_Rez;
+        float DWDY = (WY1 - WY2) * DYK_Rez * 0.5f;
+        float DUDZ = UK[k + 1] * DZK_Rez * 0.5f;
+        float DVDZ = VK[k + 1] * DZK_Rez * 0.5f;
+        float DWDZ = WK[k + 1] * DZK_Rez * 0.5f;
+        if (k > KK + 1) {
+            DUDZ = (UK[k + 1] - UK[k - 1]) * DZK_Rez * 0.5f;
+            DVDZ = (VK[k + 1] - VK[k - 1]) * DZK_Rez * 0.5f;
+            DWDZ = (WK[k + 1] - WK[k - 1]) * DZK_Rez * 0.5f;
+        }
+        float s1 = DUDY + DVDX, s2 = DUDZ + DWDX, s3 = DVDZ + DWDY;
+        float PROTURB = VIS * AREA * dz *
+                        (2.0f * (DUDX * DUDX) + 2.0f * (DVDY * DVDY) + 2.0f * (DWDZ * DWDZ) + s1 * s1 + s2 * s2 + s3 * s3 - TDISS[k]);
+        float PROBOU = (float)((double)(-d.potdtdz) * 9.81 / 283.0 * (double)VIS * 1.35 * (double)AREA * (double)dz);
+        float DIMTURB = AW1 * TURBim[k] + AS1 * TURBjm[k] + AE1 * TURBip[k] + AN1 * TURBjp[k] + ap0 * TURB[k] + PROTURB + PROBOU;
+        float DIMTDISS = AW1 * TDISSim[k] + AS1 * TDISSjm[k] + AE1 * TDISSip[k] + AN1 * TDISSjp[k] + ap0 * TDISS[k] +
+                         (TDISS[k] / TURB[k] * (Ceps1 * PROTURB - Ceps2 * TDISS[k]));
+        sA[k] = AIM; sB1[k] = BIM; sC1[k] = CIM; sD1[k] = DIMTURB; sB2[k] = BIMEPS; sC2[k] = CIMEPS; sD2[k] = DIMTDISS;
+        // wall functions of the back-substitution (they read only UK/VK/WK)
+        int wf = 0; float wt = 0, we = 0;
+        if (KK < k) {
+            float us;
+            if ((k == KK + 1) && (q.cutk == 0)) {
+                wf = 1;
+                size_t cc = c2(d, i, j);
+                if (d.tke_br[cc]) {
+                    double s = sqrt((0.5 * (double)(UK[k + 1] + UKip[k + 1])) * (0.5 * (double)(UK[k + 1] + UKip[k + 1])) +
+                                    (0.5 * (double)(VK[k + 1] + VKjp[k + 1])) * (0.5 * (double)(VK[k + 1] + VKjp[k + 1])));
+                    us = (float)(d.tke_a[cc] * s);
+                } else {
+                    double s = sqrt((0.5 * (double)(UK[k + 1] + UKip[k])) * (0.5 * (double)(UK[k + 1] + UKip[k])) +
+                                    (0.5 * (double)(VK[k] + VKjp[k])) * (0.5 * (double)(VK[k] + VKjp[k])));
+                    us = (float)(d.tke_a[cc] * s);
+                }
+                wt = (float)(3.33 * (double)(us * us));
+                we = (float)((double)(us * us * us) / d.tke_den[cc]);
+            } else if ((k == KK + 1) && (q.cutk > 0)) {
+                wf = 1;
+                double a1 = 0.5 * (double)(UK[k] + UKip[k]), b1 = 0.5 * (double)(VK[k] + VKjp[k]);
+                us = q.ho * (float)sqrt(a1 * a1 + b1 * b1);
+                wt = (float)(3.33 * (double)(us * us));
+                we = (float)((double)(us * us * us) / ((double)dz * 0.5 * 0.4));
+            } else if (((k == kkip + 1) && cip) || ((k == kkim + 1) && cim)) {
+                wf = 1;
+                double a1 = 0.5 * (double)(WK[k] + WK[k + 1]), b1 = 0.5 * (double)(VK[k] + VKjp[k]);
+                us = (float)((double)d.ufx * sqrt(a1 * a1 + b1 * b1));
+                wt = (float)(3.33 * (double)(us * us));
+                we = (float)((double)(us * us * us) / ((double)d.bz0 * 0.4));
+            } else if (((k == kkjp + 1) && cjp) || ((k == kkjm + 1) && cjm)) {
+                wf = 1;
+                double a1 = 0.5 * (double)(WK[k] + WK[k + 1]), b1 = 0.5 * (double)(UK[k] + UKip[k]);
+                us = (float)((double)d.ufx * sqrt(a1 * a1 + b1 * b1));
+                wt = (float)(3.33 * (double)(us * us));
+                we = (float)((double)(us * us * us) / ((double)d.bz0 * 0.4));
+            }
+        }
+        sWF[k] = wf; sWT[k] = wt; sWE[k] = we;
+    }
+    for (int k = q.KSTART + tid; k <= q.VI + 1; k += WAVE) { sT[k] = TURB[k]; sE[k] = TDISS[k]; }
+    __syncthreads();
+    if (tid == 0) {
+        float P1 = 0, Q1 = 0, P2 = 0, Q2 = 0;
+        for (int k = q.KSTART; k <= q.VI; ++k) {
+            float A = sA[k];
+            float p1, q1, p2, q2;
+            if (k > KK + 1) {
+                p1 = sB1[k] / (A - sC1[k] * P1);
+                q1 = (sD1[k] + sC1[k] * Q1) / (A - sC1[k] * P1);
+                p2 = sB2[k] / (A - sC2[k] * P2);
+                q2 = (sD2[k] + sC2[k] * Q2) / (A - sC2[k] * P2);
+            } else {
+                p1 = sB1[k] / A; q1 = sD1[k] / A; p2 = sB2[k] / A; q2 = sD2[k] / A;
+            }
+            sB1[k] = p1; sD1[k] = q1; sB2[k] = p2; sD2[k] = q2;
+            P1 = p1; Q1 = q1; P2 = p2; Q2 = q2;
+        }
+        float tup = sT[q.VI + 1], eup = sE[q.VI + 1];
+        for (int k = q.VI; k >= q.KSTART; --k) {
+            float tv = sT[k], ev = sE[k];
+            if (KK < k) {
+                if (sWF[k]) { tv = sWT[k]; ev = sWE[k]; }
+                else { tv = sB1[k] * tup + sD1[k]; ev = sB2[k] * eup + sD2[k]; }
+                tv = (float)mmaxd((double)tv, 0.001);
+                ev = (float)mmaxd((double)ev, 0.0000001);
+                sT[k] = tv; sE[k] = ev;
+            }
+            tup = tv; eup = ev;
+        }
+    }
+    __syncthreads();
+    float *TW = d.turb + o, *EW = d.tdiss + o;
+    for (int k = q.KSTART + tid; k <= q.VI; k += WAVE) { TW[k] = sT[k]; EW[k] = sE[k]; }
+}
+
+// ------------------------------------------------------------------------
+// host-side launchers
+// ------------------------------------------------------------------------
+static inline int ndiag(int oa, int ob, int ia, int ib) { return (ob - oa) + (ib - ia) + 1; }
+static inline int diag_len(int t, int oa, int ob, int ia, int ib) {
+    int nb = ib - ia;
+    int amin = oa + t - nb > oa ? oa + t - nb : oa;
+    int amax = ob < oa + t ? ob : oa + t;
+    return amax - amin + 1;
+}
+
+extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_t s, long long *launches) {
+    const Dom &d = *dp;
+    long long nl = 0;
+    int nk = d.kadvmax;
+    int kb = (nk + 63) / 64;
+    int IS = 1, JS = 1;
+    switch (iter % 4) {
+        case 0: IS = -1; JS = 1; break;
+        case 1: IS = -1; JS = -1; break;
+        case 2: IS = 1; JS = -1; break;
+        default: break;
+    }
+    int ncols = (d.nii - 2) * (d.njj - 2);
+    hipLaunchKernelGGL(k_bc_ew, dim3(kb, d.njj - 1), dim3(64), 0, s, d); ++nl;
+    hipLaunchKernelGGL(k_bc_ns, dim3(kb, d.nii - 1), dim3(64), 0, s, d); ++nl;
+    hipLaunchKernelGGL(k_bc_top, dim3((ncols + 255) / 256), dim3(256), 0, s, d); ++nl;
+    hipLaunchKernelGGL(k_divergence, dim3(ncols), dim3(64), 0, s, d); ++nl;
+    size_t nrow = (size_t)d.nj2 * d.nkp;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_p, nrow, 0.0f); ++nl;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_q, nrow, 0.0f); ++nl;
+    {
+        int oa = 2, ob = d.njj - 1, ia = 2, ib = d.nii - 1;
+        size_t shm = sizeof(float) * 5 * d.nkp;
+        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t) {
+            hipLaunchKernelGGL(k_pressure, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, d, t, IS, JS, slot);
+            ++nl;
+        }
+    }
+    hipLaunchKernelGGL(k_correction, dim3(ncols), dim3(64), 0, s, d); ++nl;
+    hipLaunchKernelGGL(k_averages, dim3(ncols), dim3(64), 0, s, d); ++nl;
+    hipLaunchKernelGGL(k_avg_edges, dim3(d.njj - 2), dim3(64), 0, s, d); ++nl;
+    size_t shm = sizeof(float) * 5 * d.nkp;
+    auto sweep = [&](auto kern, int oa, int ia, int is, int js) {
+        int ob = d.nii - 1, ib = d.njj - 1;
+        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t) {
+            hipLaunchKernelGGL(kern, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, d, t, is, js);
+            ++nl;
+        }
+    };
+    if (d.model == 1) {
+        sweep(k_momentum<0, 0>, 3, 2, IS, JS);
+        sweep(k_momentum<1, 0>, 2, 3, -IS, -JS);
+        sweep(k_momentum<2, 0>, 2, 2, IS, JS);
+    } else if (d.model == 2) {
+        sweep(k_momentum<0, 1>, 3, 2, IS, JS);
+        sweep(k_momentum<1, 1>, 2, 3, IS, JS);
+        sweep(k_momentum<2, 1>, 2, 2, IS, JS);
+        size_t shm_t = sizeof(float) * 12 * d.nkp;
+        int oa = 2, ob = d.nii - 1, ia = 2, ib = d.njj - 1;
+        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t) {
+            hipLaunchKernelGGL(k_tke, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm_t, s, d, t, IS, JS);
+            ++nl;
+        }
+    } else {
+        sweep(k_momentum<0, 2>, 3, 2, IS, JS);
+        sweep(k_momentum<1, 2>, 2, 3, IS, JS);
+        sweep(k_momentum<2, 2>, 2, 2, IS, JS);
+    }
+    if (launches) *launches += nl;
+    return (int)hipGetLastError();
+}
+
+extern "C" int pf_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return (int)hipGetLastError();
+}

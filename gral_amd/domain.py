@@ -1,0 +1,327 @@
+"""Synthetic GRAL microscale domains: grid, building masks and initial wind field.
+
+This is the harness that stands in for the callers of the flow solver in the
+reference (``MicroscaleFlat.Calculate``, ``ReadBuildings``,
+``ReadGeometryAndDomains.CalculateSubDomainsForPrognosticWindSimulation``).  It
+produces exactly the in-memory state that ``PrognosticFlowfield.Calculate``
+reads, in the dense ghost-padded layout of the C-ABI:
+
+* 2-D arrays ``[nii+2, njj+2]`` (index 0 and nii+1 / njj+1 are the ghost rows
+  of the reference's jagged arrays);
+* 3-D arrays ``[nii+2, njj+2, nkk+2]``, k contiguous.
+
+Reference anchors (paths relative to the reference's ``src/``):
+
+* vertical grid: ``ProgramFunctions.cs:27-77`` (``GenerateVerticalGridFlat``)
+* sub-domain mask ADVDOM: ``ReadGeometryAndDomains.cs:321-349``
+* KKART / AHK / initial profile / KADVMAX: ``MicroscaleFlat.cs:37-197``
+* reference point: ``MicroscaleTerrainSearchRefPoint.cs:27-103``
+* wind direction convention: ``Input_pgt.cs:90-93``
+* Coriolis parameter: ``Program.cs:348``
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+
+F32 = np.float32
+
+
+def vertical_grid_flat(dz1: float = 1.0, stretch: float = 1.01, top: float = 800.0):
+    """Flat-terrain vertical grid in float32 (ProgramFunctions.cs:27-77).
+
+    Returns (nkk, dzk[nkk+2], hokart[nkk+2]) with index 0 unused for dzk and
+    hokart[0] = 0, stopping at the first level whose top reaches ``top`` m.
+    """
+    dzk = [F32(0.0), F32(dz1)]
+    hok = [F32(0.0), F32(dz1)]
+    s = F32(stretch)
+    nkk = 0
+    for i in range(2, 2048):
+        dzk.append(F32(dzk[i - 1] * s))
+        hok.append(F32(hok[i - 1] + dzk[i]))
+        if hok[i] >= top:
+            nkk = i
+            break
+    nkk = min(nkk, 2048 - 2)
+    dz = np.zeros(nkk + 2, dtype=F32)
+    ho = np.zeros(nkk + 2, dtype=F32)
+    dz[: nkk + 1] = np.array(dzk[: nkk + 1], dtype=F32)
+    ho[: nkk + 1] = np.array(hok[: nkk + 1], dtype=F32)
+    return nkk, dz, ho
+
+
+def search_reference_point(advdom: np.ndarray):
+    """Spiral search for the first sub-domain cell from the domain border
+    (MicroscaleTerrainSearchRefPoint.cs:27-103).  ``advdom`` is [nii+2, njj+2]."""
+    xmax, ymax = advdom.shape
+    searchmax = min(xmax, ymax) // 2
+    delta = 1
+    i = j = 1
+    found = False
+    while not found and delta < searchmax:
+        i = delta
+        j = delta
+        while j < (ymax - delta) and not found:
+            if advdom[i, j] == 1:
+                found = True
+            else:
+                j += 1
+        while i < (xmax - delta) and not found:
+            if advdom[i, j] == 1:
+                found = True
+            else:
+                i += 1
+        while j > delta and not found:
+            if advdom[i, j] == 1:
+                found = True
+            else:
+                j -= 1
+        while i > delta and not found:
+            if advdom[i, j] == 1:
+                found = True
+            else:
+                i -= 1
+        delta += 1
+    if not found:
+        return 0, 0
+    return i, j
+
+
+@dataclasses.dataclass
+class Building:
+    i0: int
+    i1: int
+    j0: int
+    j1: int
+    height: float
+
+
+@dataclasses.dataclass
+class Case:
+    """All inputs of one flow situation (one call of PrognosticFlowfield.Calculate)."""
+
+    name: str
+    nii: int
+    njj: int
+    nkk: int
+    kadvmax: int
+    vertcellsff: int
+    dxk: float
+    dyk: float
+    dzk: np.ndarray
+    hokart: np.ndarray
+    advdom: np.ndarray
+    kkart: np.ndarray
+    cutk: np.ndarray
+    cov: np.ndarray
+    ahk: np.ndarray
+    z0_wall: np.ndarray
+    z0_solver: np.ndarray
+    ob_init: np.ndarray
+    ust_init: np.ndarray
+    z0_init: np.ndarray
+    uk: np.ndarray
+    vk: np.ndarray
+    wk: np.ndarray
+    refp: tuple
+    veg: Optional[np.ndarray] = None
+    veg_present: Optional[np.ndarray] = None
+    turbulence_model: int = 1
+    simd_width: int = 8
+    iter_min: int = 100
+    iter_max: int = 500
+    relax: float = 0.1
+    relaxp: float = 1.0
+    building_z0: float = 0.001
+    coriolis: float = 0.0
+    potdtdz: float = 0.0
+    std_dev_v: float = 1.0
+    std_dev_w: float = 1.0
+    bdlayheight: float = 800.0
+    v0_1: float = 0.0
+
+    def active_cells(self) -> int:
+        """Cells advanced by one SIMPLE iteration: ADVDOM(i,j)=1, 2<=i<=NII-1,
+        2<=j<=NJJ-1, KKART < k <= VerticalIndex."""
+        adv = self.advdom[2:self.nii, 2:self.njj] == 1
+        kk = self.kkart[2:self.nii, 2:self.njj].astype(np.int64)
+        vi = np.minimum(self.nkk - 1, kk + self.vertcellsff)
+        return int(((vi - kk) * adv).sum())
+
+    def copy(self) -> "Case":
+        c = dataclasses.replace(self)
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            if isinstance(v, np.ndarray):
+                setattr(c, f.name, v.copy())
+        return c
+
+
+def coriolis_parameter(latitude_deg: float) -> float:
+    """Program.cs:348: (float)(2*7.29e-5*sin(|lat|*3.1415/180))."""
+    return float(F32(2 * 7.29 * 0.00001 * math.sin(abs(latitude_deg) * 3.1415 / 180)))
+
+
+def wind_components(speed: float, direction_deg: float):
+    """Input_pgt.cs:90-93: meteorological direction -> (u, v)."""
+    a = (270.0 - float(F32(direction_deg))) * math.pi / 180.0
+    a = float(F32(a))
+    return float(F32(speed * math.cos(a))), float(F32(speed * math.sin(a)))
+
+
+def _log_profile(z: np.ndarray, ustar: float, z0: float) -> np.ndarray:
+    return np.where(z > z0, ustar / 0.4 * np.log(np.maximum(z, z0) / z0), 0.0)
+
+
+def build_case(name: str, nii: int, njj: int, buildings, *, dx: float = 2.0, dz1: float = 1.0,
+               stretch: float = 1.01, vertcellsff: int = 30, speed10: float = 3.0,
+               direction: float = 270.0, z0: float = 0.2, obukhov: float = 1000.0,
+               latitude: float = 47.0, subdomain_factor: int = 15, turbulence_model: int = 1,
+               iterations: int = 200, veg_lad=None, veg_cov=None) -> Case:
+    """Assemble a flat-terrain case from a list of cuboid ``Building`` objects."""
+    nkk, dzk, hok = vertical_grid_flat(dz1, stretch)
+    n2 = (nii + 2, njj + 2)
+    cutk = np.zeros(n2, dtype=F32)
+    for b in buildings:
+        sl = (slice(max(1, b.i0), min(nii, b.i1) + 1), slice(max(1, b.j0), min(njj, b.j1) + 1))
+        cutk[sl] = np.maximum(cutk[sl], F32(b.height))
+    # sub-domain mask (ReadGeometryAndDomains.cs:321-349): building cells dilated
+    # by (int)(factor*CUTK/DX) in i and (int)(factor*CUTK/DY) in j, clipped to [1,NII]x[1,NJJ]
+    advdom = np.zeros(n2, dtype=np.uint8)
+    for b in buildings:
+        h = F32(b.height)
+        ip = int(np.float32(subdomain_factor) * h / F32(dx))
+        jp = int(np.float32(subdomain_factor) * h / F32(dx))
+        advdom[max(1, b.i0 - ip):min(nii, b.i1 + ip) + 1, max(1, b.j0 - jp):min(njj, b.j1 + jp) + 1] = 1
+    if veg_lad is not None:
+        # vegetation areas are prognostic sub-domains as well (ReadVegetation.cs:213-225)
+        advdom[1:nii + 1, 1:njj + 1] |= (veg_cov[1:nii + 1, 1:njj + 1] > 0).astype(np.uint8)
+
+    # KKART / AHK / initial profile (MicroscaleFlat.cs:56-197)
+    vertk = (hok[0:nkk] + dzk[1:nkk + 1] * F32(0.5)).astype(F32)       # vertk[k-1] for k = 1..nkk
+    blocked = vertk[None, None, :] <= cutk[:, :, None]                    # [i, j, k-1]
+    kkart = np.zeros(n2, dtype=np.int16)
+    anyb = blocked.any(axis=2)
+    last = nkk - 1 - np.argmax(blocked[:, :, ::-1], axis=2)               # highest blocked k-1
+    kkart[anyb] = (last[anyb] + 1).astype(np.int16)
+    ahk = np.zeros(n2, dtype=F32)
+    ahk[anyb] = hok[kkart[anyb]]
+    kadv = 1
+    bmask = (cutk > 0) & anyb
+    if bmask.any():
+        kadv = max(kadv, int(kkart[bmask].max()))
+    kadvmax = min(nkk, kadv + vertcellsff)
+
+    ustar = speed10 * 0.4 / math.log(10.0 / z0)
+    prof = _log_profile(vertk.astype(np.float64), ustar, z0)             # [nkk] for k = 1..nkk
+    u0, v0 = wind_components(1.0, direction)
+    uk = np.zeros((nii + 2, njj + 2, nkk + 2), dtype=F32)
+    vk = np.zeros_like(uk)
+    wk = np.zeros_like(uk)
+    up = (prof * u0).astype(F32)
+    vp = (prof * v0).astype(F32)
+    uk[1:nii + 1, 1:njj + 1, 1:nkk + 1] = up[None, None, :]
+    vk[1:nii + 1, 1:njj + 1, 1:nkk + 1] = vp[None, None, :]
+    inside = blocked[1:nii + 1, 1:njj + 1, :]
+    uk[1:nii + 1, 1:njj + 1, 1:nkk + 1][inside] = 0
+    vk[1:nii + 1, 1:njj + 1, 1:nkk + 1][inside] = 0
+    # faces of blocked cells: UK at i+1 and VK at j+1 (MicroscaleFlat.cs:153-163)
+    west_face = blocked[1:nii, 1:njj + 1, :]                              # cell i-1 blocked -> UK[i] = 0
+    uk[2:nii + 1, 1:njj + 1, 1:nkk + 1][west_face] = 0
+    south_face = blocked[1:nii + 1, 1:njj, :]
+    vk[1:nii + 1, 2:njj + 1, 1:nkk + 1][south_face] = 0
+    uk[nii + 1, 1:njj + 1, :] = uk[nii, 1:njj + 1, :]
+    vk[1:nii + 1, njj + 1, :] = vk[1:nii + 1, njj, :]
+
+    refp = search_reference_point(advdom)
+    if refp == (0, 0):
+        raise ValueError("no prognostic sub-domain in this case")
+    zfield = np.full(n2, F32(z0), dtype=F32)
+    cov = np.zeros(n2, dtype=F32) if veg_cov is None else veg_cov.astype(F32)
+    veg = None
+    veg_present = None
+    if veg_lad is not None:
+        veg = veg_lad.astype(F32)
+        veg_present = (veg_cov > 0).astype(np.uint8)
+    case = Case(name=name, nii=nii, njj=njj, nkk=nkk, kadvmax=kadvmax, vertcellsff=vertcellsff,
+                dxk=float(F32(dx)), dyk=float(F32(dx)), dzk=dzk, hokart=hok, advdom=advdom, kkart=kkart,
+                cutk=cutk, cov=cov, ahk=ahk, z0_wall=zfield.copy(), z0_solver=zfield.copy(),
+                ob_init=np.full(n2, F32(obukhov), dtype=F32), ust_init=np.full(n2, F32(ustar), dtype=F32),
+                z0_init=zfield.copy(), uk=uk, vk=vk, wk=wk, refp=refp, veg=veg, veg_present=veg_present,
+                turbulence_model=turbulence_model, iter_min=iterations, iter_max=iterations,
+                coriolis=coriolis_parameter(latitude))
+    return case
+
+
+def config1(turbulence_model: int = 1, iterations: int = 200) -> Case:
+    """Plain solver, flat terrain: 100x100, dx=2 m, dz1=1 m stretched by 1.01,
+    VertCellsFF=30, one 20x20x15 m building at the centre, neutral log
+    profile 3 m/s at 10 m, z0=0.2 m, wind from 270 deg, 200 iterations."""
+    return build_case("C1", 100, 100, [Building(46, 55, 46, 55, 15.0)], dx=2.0, vertcellsff=30,
+                      speed10=3.0, direction=270.0, z0=0.2, turbulence_model=turbulence_model,
+                      iterations=iterations)
+
+
+def city_buildings(nii: int, njj: int, dx: float, seed: int, density: float = 0.30,
+                   fmin: float = 8.0, fmax: float = 40.0, hmin: float = 6.0, hmax: float = 40.0,
+                   margin: int = 4):
+    """Cuboids dropped by a Poisson process until ``density`` of the plan area is
+    covered; footprints uniform [fmin, fmax] m per side, heights uniform [hmin, hmax] m."""
+    rng = np.random.default_rng(seed)
+    covered = np.zeros((nii + 2, njj + 2), dtype=bool)
+    target = density * nii * njj
+    out = []
+    while covered.sum() < target:
+        wi = max(1, int(round(rng.uniform(fmin, fmax) / dx)))
+        wj = max(1, int(round(rng.uniform(fmin, fmax) / dx)))
+        h = float(rng.uniform(hmin, hmax))
+        i0 = int(rng.integers(margin + 1, max(margin + 2, nii - margin - wi + 1)))
+        j0 = int(rng.integers(margin + 1, max(margin + 2, njj - margin - wj + 1)))
+        b = Building(i0, i0 + wi - 1, j0, j0 + wj - 1, h)
+        out.append(b)
+        covered[b.i0:b.i1 + 1, b.j0:b.j1 + 1] = True
+    return out
+
+
+def config2(direction: float = 270.0, turbulence_model: int = 1, iterations: int = 200,
+            n: int = 400, vertcellsff: int = 40) -> Case:
+    """Plain solver on a seeded random city block: 400x400, dx=2 m, VertCellsFF=40,
+    cuboids at 30 % plan-area density (footprints 8-40 m, heights 6-40 m, geometry
+    seed 20260001), log profile with a seeded speed in [1, 6] m/s at 10 m
+    (seed 20260002), z0=0.5 m; ``direction`` is one of 0, 45, ..., 315 deg."""
+    bl = city_buildings(n, n, 2.0, 20260001)
+    rng = np.random.default_rng(20260002)
+    speeds = rng.uniform(1.0, 6.0, size=8)
+    sp = float(speeds[int(round(direction / 45.0)) % 8])
+    return build_case(f"C2-{int(direction)}", n, n, bl, dx=2.0, vertcellsff=vertcellsff, speed10=sp,
+                      direction=direction, z0=0.5, turbulence_model=turbulence_model,
+                      iterations=iterations)
+
+
+def tiny_case(n: int = 8, vertcellsff: int = 8, building=(3, 5, 4, 6, 4.5), direction: float = 250.0,
+              turbulence_model: int = 1, iterations: int = 3, veg: bool = False, seed: int = 7) -> Case:
+    """A few-cell domain for brute-force checks.  A single building; optional
+    vegetation patch (COV, VEG) next to it; direction off-axis so that both U
+    and V are non-zero."""
+    bl = [Building(*building)] if building is not None else []
+    veg_lad = veg_cov = None
+    if veg:
+        nkk, _, _ = vertical_grid_flat()
+        rng = np.random.default_rng(seed)
+        veg_cov = np.zeros((n + 2, n + 2), dtype=F32)
+        veg_cov[2:4, 2:n] = F32(0.6)
+        veg_lad = np.zeros((n + 2, n + 2, nkk + 2), dtype=F32)
+        lad = rng.uniform(0.5, 2.0, size=(n + 2, n + 2, nkk + 2)).astype(F32)
+        # VEG = 0.3*COV^3*LAD below 6 m (ReadVegetation.cs:107-111)
+        veg_lad[2:4, 2:n, 0:7] = (F32(0.3) * veg_cov[2:4, 2:n, None] ** 3 * lad[2:4, 2:n, 0:7]).astype(F32)
+    c = build_case("tiny", n, n, bl, dx=2.0, vertcellsff=vertcellsff, speed10=3.0, direction=direction,
+                   z0=0.2, turbulence_model=turbulence_model, iterations=iterations,
+                   veg_lad=veg_lad, veg_cov=veg_cov)
+    if building is None:
+        c.advdom[1:n + 1, 1:n + 1] = 1
+        c.refp = search_reference_point(c.advdom)
+    return c

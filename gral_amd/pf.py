@@ -1,0 +1,200 @@
+"""ctypes binding of libgral_pf.so (the C ABI declared in include/gral_pf.h).
+
+``PrognosticFlowfield`` mirrors the reference entry point
+``PrognosticFlowfield.Calculate()`` (src/PrognosticFlowfield.cs:32): it takes
+one situation (a ``gral_amd.domain.Case``: the state GRAL's Program.* statics
+hold before the call), runs the SIMPLE loop on the GPU and returns the
+updated fields.  The library must be built (``gral_amd.build``); there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgral_pf.so")
+
+FP = C.POINTER(C.c_float)
+U8P = C.POINTER(C.c_uint8)
+I16P = C.POINTER(C.c_int16)
+
+EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral_pf_upload_fields",
+           "gral_pf_prepare", "gral_pf_iterate", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
+           "gral_pf_last_timing", "gral_pf_last_error", "gral_pf_destroy"]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nii", C.c_int32), ("njj", C.c_int32), ("nkk", C.c_int32), ("kadvmax", C.c_int32),
+                ("vertcellsff", C.c_int32), ("dxk", C.c_float), ("dyk", C.c_float), ("dzk", FP), ("hokart", FP)]
+
+
+class Params(C.Structure):
+    _fields_ = [("turbulence_model", C.c_int32), ("simd_width", C.c_int32), ("iter_min", C.c_int32),
+                ("iter_max", C.c_int32), ("relax", C.c_float), ("relaxp", C.c_float), ("building_z0", C.c_float),
+                ("coriolis", C.c_float), ("potdtdz", C.c_float), ("std_dev_v", C.c_float), ("std_dev_w", C.c_float),
+                ("bdlayheight", C.c_float), ("v0_1", C.c_float), ("refp_i", C.c_int32), ("refp_j", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iteration_loops", C.c_int32), ("iterations_run", C.c_int32), ("delta_first", C.c_float),
+                ("delta_finish", C.c_float), ("convergence_problem", C.c_int32), ("unreal_u", C.c_int32),
+                ("unreal_v", C.c_int32), ("unreal_w", C.c_int32), ("topwind", C.c_float), ("dtime", C.c_float),
+                ("ug", C.c_float), ("vg", C.c_float), ("frund", C.c_double), ("geschw_max", C.c_double),
+                ("last_delta", C.c_double)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("ms_total", C.c_double), ("ms_pressure", C.c_double), ("ms_momentum", C.c_double),
+                ("ms_stencil", C.c_double), ("launches", C.c_int64), ("active_cells", C.c_int64)]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} not found: build it with `python -m gral_amd.build` (no CPU fallback exists)")
+    lib = C.CDLL(path)
+    H = C.c_void_p
+    lib.gral_pf_create.argtypes = [C.POINTER(Grid), C.POINTER(H)]
+    lib.gral_pf_set_masks.argtypes = [H, U8P, I16P, FP, FP, FP, FP, U8P]
+    lib.gral_pf_set_roughness.argtypes = [H, FP, FP, FP, FP, FP]
+    lib.gral_pf_upload_fields.argtypes = [H, FP, FP, FP]
+    lib.gral_pf_prepare.argtypes = [H, C.POINTER(Params)]
+    lib.gral_pf_iterate.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(Stats)]
+    lib.gral_pf_finish.argtypes = [H, C.POINTER(Stats)]
+    lib.gral_pf_solve.argtypes = [H, C.POINTER(Params), C.POINTER(Stats)]
+    lib.gral_pf_download_fields.argtypes = [H] + [FP] * 10
+    lib.gral_pf_last_timing.argtypes = [H, C.POINTER(Timing)]
+    lib.gral_pf_last_error.argtypes = [H]
+    lib.gral_pf_last_error.restype = C.c_char_p
+    lib.gral_pf_destroy.argtypes = [H]
+    lib.gral_pf_destroy.restype = None
+    for n in EXPORTS[:-2]:
+        getattr(lib, n).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, t=FP):
+    if a is None:
+        return C.cast(None, t)
+    return a.ctypes.data_as(t)
+
+
+class PrognosticFlowfield:
+    """One situation on the GPU: create -> set_masks/roughness/fields -> prepare
+    -> iterate -> finish -> download."""
+
+    def __init__(self, case):
+        self.lib = load()
+        self.case = case
+        c = case
+        self._keep = {"dzk": _f32(c.dzk), "hokart": _f32(c.hokart)}
+        g = Grid(c.nii, c.njj, c.nkk, c.kadvmax, c.vertcellsff, c.dxk, c.dyk, _ptr(self._keep["dzk"]),
+                 _ptr(self._keep["hokart"]))
+        h = C.c_void_p()
+        rc = self.lib.gral_pf_create(C.byref(g), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"gral_pf_create failed ({rc})")
+        self.h = h
+        k = self._keep
+        k.update(advdom=np.ascontiguousarray(c.advdom, dtype=np.uint8), kkart=np.ascontiguousarray(c.kkart, dtype=np.int16),
+                 cutk=_f32(c.cutk), cov=_f32(c.cov), ahk=_f32(c.ahk))
+        veg = None if c.veg is None else _f32(c.veg)
+        vegp = None if c.veg_present is None else np.ascontiguousarray(c.veg_present, dtype=np.uint8)
+        k.update(veg=veg, vegp=vegp)
+        self._check(self.lib.gral_pf_set_masks(h, _ptr(k["advdom"], U8P), _ptr(k["kkart"], I16P), _ptr(k["cutk"]),
+                                               _ptr(k["cov"]), _ptr(k["ahk"]), _ptr(veg), _ptr(vegp, U8P)))
+        k.update(z0w=_f32(c.z0_wall), z0s=_f32(c.z0_solver), obi=_f32(c.ob_init), usti=_f32(c.ust_init), z0i=_f32(c.z0_init))
+        self._check(self.lib.gral_pf_set_roughness(h, _ptr(k["z0w"]), _ptr(k["z0s"]), _ptr(k["obi"]), _ptr(k["usti"]),
+                                                   _ptr(k["z0i"])))
+        uk, vk, wk = _f32(c.uk), _f32(c.vk), _f32(c.wk)
+        self._check(self.lib.gral_pf_upload_fields(h, _ptr(uk), _ptr(vk), _ptr(wk)))
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.gral_pf_last_error(self.h)
+            raise RuntimeError(f"gral_pf call failed ({rc}): {msg.decode() if msg else ''}")
+
+    def params(self, iterations=None):
+        c = self.case
+        it_min = c.iter_min if iterations is None else iterations
+        it_max = c.iter_max if iterations is None else iterations
+        return Params(c.turbulence_model, c.simd_width, it_min, it_max, c.relax, c.relaxp, c.building_z0, c.coriolis,
+                      c.potdtdz, c.std_dev_v, c.std_dev_w, c.bdlayheight, c.v0_1, int(c.refp[0]), int(c.refp[1]))
+
+    def prepare(self, iterations=None):
+        self._params = self.params(iterations)
+        self._check(self.lib.gral_pf_prepare(self.h, C.byref(self._params)))
+
+    def iterate(self, n, check_convergence=False):
+        st = Stats()
+        self._check(self.lib.gral_pf_iterate(self.h, int(n), int(bool(check_convergence)), C.byref(st)))
+        return _stats(st)
+
+    def finish(self):
+        st = Stats()
+        self._check(self.lib.gral_pf_finish(self.h, C.byref(st)))
+        return _stats(st)
+
+    def solve(self, iterations=None):
+        """The whole Calculate(): prologue, SIMPLE loop with the reference's
+        convergence test, epilogue checks."""
+        self._params = self.params(iterations)
+        st = Stats()
+        self._check(self.lib.gral_pf_solve(self.h, C.byref(self._params), C.byref(st)))
+        return _stats(st)
+
+    def timing(self):
+        t = Timing()
+        self._check(self.lib.gral_pf_last_timing(self.h, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in Timing._fields_}
+
+    def download(self):
+        shp = self.case.uk.shape
+        names = ("uk", "vk", "wk", "uks", "vks", "wks", "dpm", "dpmnew", "turb", "tdiss")
+        out = {n: np.zeros(shp, dtype=np.float32) for n in names}
+        self._check(self.lib.gral_pf_download_fields(self.h, *[_ptr(out[n]) for n in names]))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gral_pf_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stats(st):
+    return {n: getattr(st, n) for n, _ in Stats._fields_}
+
+
+def run(case, iterations=None):
+    """Convenience: solve ``case`` on the GPU; returns dict of fields + 'stats'."""
+    with PrognosticFlowfield(case) as pf:
+        stats = pf.solve(iterations)
+        out = pf.download()
+    out["stats"] = stats
+    return out

@@ -1,0 +1,139 @@
+/*
+ * gral_pf.h -- C ABI of the MI355X prognostic microscale flow solver.
+ *
+ * Drop-in for GRAL's PrognosticFlowfield.Calculate() (src/PrognosticFlowfield.cs:32),
+ * the SIMPLE loop called once per weather situation from MicroscaleFlat.cs:381
+ * and MicroscaleTerrain.cs:847.  The reference method takes no arguments and
+ * works on Program.* statics; this ABI passes the same state explicitly as
+ * blittable pointers (GRAL.csproj:23 sets DisableRuntimeMarshalling=true).
+ *
+ * Layout of every array argument: dense, ghost-padded, C order, index 0 kept
+ * (unused) as in the reference's jagged arrays:
+ *   2-D  [nii+2][njj+2]           element (i,j)   at  i*(njj+2) + j
+ *   3-D  [nii+2][njj+2][nkk+2]    element (i,j,k) at (i*(njj+2) + j)*(nkk+2) + k
+ * A C# caller packs Program.UK[i][j][k] etc. into pinned staging buffers of
+ * this shape (missing jagged rows -> zeros).
+ *
+ * Conventions: every function returns 0 on success or a negative GRAL_PF_E*
+ * code and never aborts; gral_pf_last_error() gives the message.  The handle
+ * owns all device memory; host pointers are only read/written during the
+ * call.  One handle per host thread; calls on a handle are serialised.
+ */
+#ifndef GRAL_PF_H
+#define GRAL_PF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRAL_PF_OK 0
+#define GRAL_PF_EINVAL (-1)   /* bad argument / unsupported option */
+#define GRAL_PF_EDEVICE (-2)  /* HIP runtime error (no device, launch failure) */
+#define GRAL_PF_ENOMEM (-3)   /* host or device allocation failed */
+#define GRAL_PF_ESTATE (-4)   /* call out of order (e.g. iterate before prepare) */
+
+typedef struct gral_pf_handle gral_pf_handle;
+
+/* Grid of one situation.  Replaces Program.NII/NJJ/NKK/KADVMAX/VertCellsFF/
+ * DXK/DYK/DZK/HOKART (ProgramDeclarations.cs:72-92, :1011). */
+typedef struct {
+    int32_t nii, njj, nkk;      /* cells in x, y, z */
+    int32_t kadvmax;            /* highest level the solver touches (MicroscaleFlat.cs:197) */
+    int32_t vertcellsff;        /* prognostic cells above ground (micro_vert_layers.txt) */
+    float dxk, dyk;             /* horizontal cell size [m] */
+    const float *dzk;           /* [nkk+2] layer thickness, index 0 unused */
+    const float *hokart;        /* [nkk+2] level top height, hokart[0] = 0 */
+} gral_pf_grid;
+
+/* Run options and scalars read by Calculate(): turbulence_model.txt,
+ * Integrationtime.txt, relaxation_factors.txt, building_roughness.txt
+ * (PrognosticFlowfield.cs:176-291), Program.CorolisParam, PotdTdz,
+ * StdDeviationV/W, BdLayHeight, V0[1], SubDomainRefPos. */
+typedef struct {
+    int32_t turbulence_model;   /* 0 no diffusion, 1 mixing length (default), 2 k-eps */
+    int32_t simd_width;         /* Vector<float>.Count semantics of the mixing-length solvers: 8 */
+    int32_t iter_min, iter_max; /* IterationStepsMin/Max */
+    float relax, relaxp;        /* velocity / pressure relaxation */
+    float building_z0;          /* building wall roughness [m] */
+    float coriolis;             /* Coriolis parameter [1/s] */
+    float potdtdz;              /* potential temperature gradient [K/m] (k-eps buoyancy) */
+    float std_dev_v, std_dev_w; /* StdDeviationV/W (TKE initialisation) */
+    float bdlayheight;          /* boundary-layer height [m] (TKE initialisation) */
+    float v0_1;                 /* Program.V0[1] (TKE initialisation) */
+    int32_t refp_i, refp_j;     /* reference point (MicroscaleTerrain.GetSubDomainPoint) */
+} gral_pf_params;
+
+/* What Calculate() reports (console lines and Problemreport_GRAL.txt,
+ * PrognosticFlowfield.cs:1144-1245). */
+typedef struct {
+    int32_t iteration_loops;    /* IterationLoops at exit ("Total number of iterations") */
+    int32_t iterations_run;     /* SIMPLE iterations executed */
+    float delta_first;          /* convergence measure at iteration 100 */
+    float delta_finish;         /* convergence measure at the last iteration */
+    int32_t convergence_problem;/* delta_finish > delta_first ("Convergence not fullfilled") */
+    int32_t unreal_u, unreal_v, unreal_w; /* unrealistic wind-speed counters */
+    float topwind, dtime, ug, vg;
+    double frund, geschw_max;
+    double last_delta;          /* last value of DeltaMean/topwind^2 tested (every 100 iterations) */
+} gral_pf_stats;
+
+/* Replaces the array allocations of PrognosticFlowfield.cs:45-106 (device side). */
+int gral_pf_create(const gral_pf_grid *grid, gral_pf_handle **out);
+
+/* Geometry: Program.ADVDOM, KKART, CUTK, COV, AHK, VEG (nullable columns ->
+ * veg_present[i][j]=0; veg may be NULL when no vegetation exists). */
+int gral_pf_set_masks(gral_pf_handle *h, const uint8_t *advdom, const int16_t *kkart,
+                      const float *cutk, const float *cov, const float *ahk,
+                      const float *veg, const uint8_t *veg_present);
+
+/* Per-column roughness / stability, resolved by the caller from Z0Gral,
+ * Z0Gramm, OLGral, Ob, USternGral, Ustern exactly as the reference indexes them:
+ *   z0_wall   -- 'rough' of the wall helpterms     (PrognosticFlowfield.cs:365-381)
+ *   z0_solver -- 'Z0' inside the U/V/W/TKE solvers  (U-prognostic-microscale_1.cs:158-177)
+ *   ob_init, ust_init, z0_init -- TURB initialisation (PrognosticFlowfield.cs:415-436) */
+int gral_pf_set_roughness(gral_pf_handle *h, const float *z0_wall, const float *z0_solver,
+                          const float *ob_init, const float *ust_init, const float *z0_init);
+
+/* Initial wind field Program.UK/VK/WK (the state MicroscaleFlat/MicroscaleTerrain
+ * leave before the prognostic call). */
+int gral_pf_upload_fields(gral_pf_handle *h, const float *uk, const float *vk, const float *wk);
+
+/* Prologue of Calculate() (PrognosticFlowfield.cs:145-579) and transfer of the
+ * solver state to the GPU. */
+int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p);
+
+/* Runs up to n SIMPLE iterations (PrognosticFlowfield.cs:584-1178) on the
+ * device-resident state.  check_convergence=1 applies the reference's stop
+ * test every 100 iterations; 0 runs exactly n iterations (or until iter_max). */
+int gral_pf_iterate(gral_pf_handle *h, int32_t n, int32_t check_convergence, gral_pf_stats *st);
+
+/* Epilogue checks (PrognosticFlowfield.cs:1187-1245) and transfer back to host. */
+int gral_pf_finish(gral_pf_handle *h, gral_pf_stats *st);
+
+/* The whole of Calculate(): prepare + iterate(iter_max, check) + finish. */
+int gral_pf_solve(gral_pf_handle *h, const gral_pf_params *p, gral_pf_stats *st);
+
+/* Copy the solver state to host buffers (each may be NULL), full depth
+ * [nii+2][njj+2][nkk+2]: Program.UK/VK/WK/UKS/VKS/WKS/DPM/DPMNEW/TURB/TDISS. */
+int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, float *wk, float *uks,
+                            float *vks, float *wks, float *dpm, float *dpmnew, float *turb,
+                            float *tdiss);
+
+/* Time (ms, HIP events on the solver's stream) and launch count of the most
+ * recent gral_pf_iterate call, per kernel family; for measurement. */
+typedef struct {
+    double ms_total;
+    double ms_pressure, ms_momentum, ms_stencil;
+    int64_t launches;
+    int64_t active_cells;       /* cells advanced per SIMPLE iteration */
+} gral_pf_timing;
+int gral_pf_last_timing(gral_pf_handle *h, gral_pf_timing *t);
+
+const char *gral_pf_last_error(const gral_pf_handle *h);
+void gral_pf_destroy(gral_pf_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

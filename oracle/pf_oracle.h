@@ -1,0 +1,78 @@
+/*
+ * pf_oracle.h -- CPU restatement of GRAL's prognostic microscale flow solver
+ * (PrognosticFlowfield.Calculate and the U/V/W/TKE column solvers it calls),
+ * in the reference's deterministic single-thread mode (UseFixedRndSeedVal).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the parity checker for the HIP
+ * solver in gral_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product library never links or calls it.
+ *
+ * Array layout: dense, ghost-padded, index 0 present but unused as in the
+ * reference's jagged arrays.
+ *   2-D arrays  [nii+2][njj+2]            idx = i*(njj+2) + j
+ *   3-D arrays  [nii+2][njj+2][nkk+2]     idx = (i*(njj+2) + j)*(nkk+2) + k
+ */
+#ifndef GRAL_PF_ORACLE_H
+#define GRAL_PF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nii, njj, nkk;          /* Program.NII/NJJ/NKK */
+    int kadvmax;                /* Program.KADVMAX (after MicroscaleFlat) */
+    int vertcellsff;            /* Program.VertCellsFF */
+    float dxk, dyk;             /* Program.DXK/DYK */
+    int turbulence_model;       /* turbulence_model.txt: 0, 1 (default), 2 */
+    int simd_width;             /* Vector<float>.Count semantics of the model-1 solvers: 8 */
+    int iter_min, iter_max;     /* Integrationtime.txt */
+    float relax, relaxp;        /* relaxation_factors.txt */
+    float building_z0;          /* building_roughness.txt */
+    float coriolis;             /* Program.CorolisParam */
+    float potdtdz;              /* Program.PotdTdz */
+    float std_dev_v, std_dev_w; /* Program.StdDeviationV/W */
+    float bdlayheight;          /* Program.BdLayHeight */
+    float v0_1;                 /* Program.V0[1] */
+    int refp_i, refp_j;         /* Program.SubDomainRefPos */
+} orc_params;
+
+typedef struct {
+    const float *dzk, *hokart;  /* [nkk+2] */
+    const uint8_t *advdom;      /* [nii+2][njj+2] */
+    const int16_t *kkart;       /* [nii+2][njj+2] */
+    const float *cutk, *cov, *ahk;
+    const float *z0_wall;       /* roughness used by the wall helpterms (PrognosticFlowfield.cs:365-381) */
+    const float *z0_solver;     /* Z0 used inside the U/V/W/TKE solvers (U-prognostic-microscale_1.cs:158-177) */
+    const float *ob_init, *ust_init, *z0_init;  /* TURB init (PrognosticFlowfield.cs:415-436) */
+    const float *veg;           /* [nii+2][njj+2][nkk+2] or NULL */
+    const uint8_t *veg_present; /* [nii+2][njj+2]: VEG[i][j] != null, or NULL */
+} orc_inputs;
+
+typedef struct {
+    float *uk, *vk, *wk;                    /* in/out, [nii+2][njj+2][nkk+2] */
+    float *uks, *vks, *wks;                 /* out (required) */
+    float *dpm, *dpmnew, *div;              /* out (required) */
+    float *turb, *tdiss;                    /* out (required when turbulence_model != 1) */
+} orc_fields;
+
+typedef struct {
+    int iteration_loops;        /* IterationLoops at exit (PrognosticFlowfield.cs:1245) */
+    int iterations_run;         /* number of SIMPLE iterations executed */
+    float delta_first, delta_finish;
+    float topwind, dtime, ug, vg, ustern_factor_x;
+    double frund, geschw_max;
+    int unreal_u, unreal_v, unreal_w;
+    int convergence_problem;    /* DeltaFinish > DeltaFirst */
+    float *delta_umax_hist;     /* optional [iter_max]: DeltaUMax of each iteration */
+} orc_stats;
+
+/* Full PrognosticFlowfield.Calculate(): prologue, SIMPLE loop, epilogue checks.
+ * Returns 0 on success, negative on invalid input. */
+int orc_calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

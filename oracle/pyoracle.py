@@ -1,0 +1,115 @@
+"""ctypes binding of the CPU oracle (pf_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product (gral_amd) never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_pf.so")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("nii", C.c_int), ("njj", C.c_int), ("nkk", C.c_int), ("kadvmax", C.c_int),
+                ("vertcellsff", C.c_int), ("dxk", C.c_float), ("dyk", C.c_float),
+                ("turbulence_model", C.c_int), ("simd_width", C.c_int), ("iter_min", C.c_int),
+                ("iter_max", C.c_int), ("relax", C.c_float), ("relaxp", C.c_float),
+                ("building_z0", C.c_float), ("coriolis", C.c_float), ("potdtdz", C.c_float),
+                ("std_dev_v", C.c_float), ("std_dev_w", C.c_float), ("bdlayheight", C.c_float),
+                ("v0_1", C.c_float), ("refp_i", C.c_int), ("refp_j", C.c_int)]
+
+
+FP = C.POINTER(C.c_float)
+U8P = C.POINTER(C.c_uint8)
+I16P = C.POINTER(C.c_int16)
+
+
+class OrcInputs(C.Structure):
+    _fields_ = [("dzk", FP), ("hokart", FP), ("advdom", U8P), ("kkart", I16P), ("cutk", FP),
+                ("cov", FP), ("ahk", FP), ("z0_wall", FP), ("z0_solver", FP), ("ob_init", FP),
+                ("ust_init", FP), ("z0_init", FP), ("veg", FP), ("veg_present", U8P)]
+
+
+class OrcFields(C.Structure):
+    _fields_ = [("uk", FP), ("vk", FP), ("wk", FP), ("uks", FP), ("vks", FP), ("wks", FP),
+                ("dpm", FP), ("dpmnew", FP), ("div", FP), ("turb", FP), ("tdiss", FP)]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("iteration_loops", C.c_int), ("iterations_run", C.c_int),
+                ("delta_first", C.c_float), ("delta_finish", C.c_float), ("topwind", C.c_float),
+                ("dtime", C.c_float), ("ug", C.c_float), ("vg", C.c_float),
+                ("ustern_factor_x", C.c_float), ("frund", C.c_double), ("geschw_max", C.c_double),
+                ("unreal_u", C.c_int), ("unreal_v", C.c_int), ("unreal_w", C.c_int),
+                ("convergence_problem", C.c_int), ("delta_umax_hist", FP)]
+
+
+_lib = None
+
+
+def load(build_if_missing: bool = True):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH) and build_if_missing:
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    _lib = C.CDLL(LIB_PATH)
+    _lib.orc_calculate.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcInputs), C.POINTER(OrcFields),
+                                   C.POINTER(OrcStats)]
+    _lib.orc_calculate.restype = C.c_int
+    return _lib
+
+
+def _p(a, t=FP):
+    if a is None:
+        return C.cast(None, t)
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(t)
+
+
+def run(case, iterations=None, simd_width=None, keep_history=True):
+    """Run PrognosticFlowfield.Calculate on ``case`` (a gral_amd.domain.Case)
+    with the deterministic CPU oracle.  Returns a dict of output fields and stats.
+    The input case is not modified."""
+    lib = load()
+    c = case
+    nit = c.iter_max if iterations is None else iterations
+    prm = OrcParams(c.nii, c.njj, c.nkk, c.kadvmax, c.vertcellsff, c.dxk, c.dyk, c.turbulence_model,
+                    c.simd_width if simd_width is None else simd_width,
+                    c.iter_min if iterations is None else iterations, nit, c.relax, c.relaxp,
+                    c.building_z0, c.coriolis, c.potdtdz, c.std_dev_v, c.std_dev_w, c.bdlayheight,
+                    c.v0_1, int(c.refp[0]), int(c.refp[1]))
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    keep = dict(dzk=f32(c.dzk), hokart=f32(c.hokart), advdom=np.ascontiguousarray(c.advdom, dtype=np.uint8),
+                kkart=np.ascontiguousarray(c.kkart, dtype=np.int16), cutk=f32(c.cutk), cov=f32(c.cov),
+                ahk=f32(c.ahk), z0_wall=f32(c.z0_wall), z0_solver=f32(c.z0_solver), ob_init=f32(c.ob_init),
+                ust_init=f32(c.ust_init), z0_init=f32(c.z0_init),
+                veg=None if c.veg is None else f32(c.veg),
+                veg_present=None if c.veg_present is None else np.ascontiguousarray(c.veg_present, dtype=np.uint8))
+    inp = OrcInputs(_p(keep["dzk"]), _p(keep["hokart"]), _p(keep["advdom"], U8P), _p(keep["kkart"], I16P),
+                    _p(keep["cutk"]), _p(keep["cov"]), _p(keep["ahk"]), _p(keep["z0_wall"]),
+                    _p(keep["z0_solver"]), _p(keep["ob_init"]), _p(keep["ust_init"]), _p(keep["z0_init"]),
+                    _p(keep["veg"]), _p(keep["veg_present"], U8P))
+    shp = c.uk.shape
+    out = {n: np.zeros(shp, dtype=np.float32) for n in ("uks", "vks", "wks", "dpm", "dpmnew", "div", "turb", "tdiss")}
+    out["uk"] = f32(c.uk).copy()
+    out["vk"] = f32(c.vk).copy()
+    out["wk"] = f32(c.wk).copy()
+    fld = OrcFields(*[_p(out[n]) for n in ("uk", "vk", "wk", "uks", "vks", "wks", "dpm", "dpmnew", "div", "turb", "tdiss")])
+    hist = np.zeros(max(nit, 1), dtype=np.float32)
+    st = OrcStats()
+    st.delta_umax_hist = _p(hist) if keep_history else C.cast(None, FP)
+    rc = lib.orc_calculate(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_calculate failed: {rc}")
+    stats = {n: getattr(st, n) for n, _ in OrcStats._fields_ if n != "delta_umax_hist"}
+    stats["delta_umax_hist"] = hist[: st.iterations_run].copy()
+    out["stats"] = stats
+    return out
