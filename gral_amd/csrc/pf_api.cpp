@@ -17,7 +17,12 @@
 #include "../../include/gral_pf.h"
 #include "pf_device.h"
 
-extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_t s, long long *launches);
+struct Probe {
+    hipEvent_t *ev;
+    int *fam;
+    int cap, n;
+};
+extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_t s, long long *launches, Probe *probe);
 extern "C" int pf_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
 
 namespace {
@@ -389,6 +394,7 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     unsigned *fres;
     if ((rc = dalloc(h, &fres, (size_t)p->iter_max + 1))) return rc;
     d.resid = fres;
+    d.resid_cap = p->iter_max + 1;
     // per-level pressure coefficients (PrognosticFlowfield.cs:855-866), column independent
     std::vector<float> app(h->nkp, 0.0f), abp(h->nkp, 0.0f), atp(h->nkp, 0.0f);
     const float *DZK = h->dzk.data();
@@ -460,6 +466,9 @@ static void fill_stats(gral_pf_handle *h, gral_pf_stats *st) {
 
 // Convergence bookkeeping of one finished iteration (PrognosticFlowfield.cs:1144-1178).
 // Returns true when the reference would leave the loop here.
+// iterations launched between two host synchronisations
+static const int kMaxChunk = 20;
+
 static bool book_iteration(gral_pf_handle *h, int it, float dumax, bool check) {
     const gral_pf_params &p = h->prm;
     double tw2 = (double)(h->topwind * h->topwind);
@@ -485,34 +494,47 @@ extern "C" int gral_pf_iterate(gral_pf_handle *h, int32_t n, int32_t check, gral
     if (h->done) todo = 0;
     long long launches = 0;
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-    int started = 0;
     std::vector<unsigned> bits;
-    while (started < todo) {
-        // launch up to the next multiple of 100 (where the reference may stop)
-        int it0 = h->iter_loops + started;
-        int chunk = todo - started;
+    int remaining = todo;
+    while (remaining > 0 && !h->done) {
+        // launch up to kMaxChunk iterations, never past a multiple of 100 when the
+        // reference's stop test applies, and never past iter_max (the size of resid)
+        const int it0 = h->iter_loops;
+        int chunk = std::min(remaining, kMaxChunk);
         if (check) chunk = std::min(chunk, 100 - ((it0 - 1) % 100));
+        if (it0 < 1 || it0 - 1 + chunk > h->prm.iter_max)
+            return fail(h, GRAL_PF_ESTATE, "iterate: iteration counter out of range");
         for (int c = 0; c < chunk; ++c) {
             int it = it0 + c;
-            int e = pf_launch_iteration(&h->d, it, it - 1, h->stream, &launches);
-            if (e) return fail(h, GRAL_PF_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString((hipError_t)e));
+            int e = pf_launch_iteration(&h->d, it, it - 1, h->stream, &launches, nullptr);
+            if (e) return fail(h, GRAL_PF_EDEVICE, std::string("kernel launch failed at iteration ") + std::to_string(it) +
+                                                       ": " + hipGetErrorString((hipError_t)e));
         }
-        started += chunk;
+        remaining -= chunk;
         // bookkeeping of the iterations just launched
         HIPCHK(h, hipStreamSynchronize(h->stream));
+        {
+            hipError_t le = hipGetLastError();
+            if (le != hipSuccess)
+                return fail(h, GRAL_PF_EDEVICE, std::string("pending HIP error after iterations ") + std::to_string(it0) +
+                                                    ".." + std::to_string(it0 + chunk - 1) + ": " + hipGetErrorString(le));
+        }
         bits.resize((size_t)chunk);
-        HIPCHK(h, hipMemcpy(bits.data(), h->d.resid + (it0 - 1), sizeof(unsigned) * chunk, hipMemcpyDeviceToHost));
-        bool stop = false;
-        for (int c = 0; c < chunk && !stop; ++c) {
+        HIPCHK(h, hipMemcpyAsync(bits.data(), h->d.resid + (it0 - 1), sizeof(unsigned) * chunk, hipMemcpyDeviceToHost,
+                                 h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (int c = 0; c < chunk; ++c) {
             float du;
             std::memcpy(&du, &bits[c], 4);
             int it = it0 + c;
             h->iters_run++;
-            stop = book_iteration(h, it, du, check != 0);
-            if (!stop) h->iter_loops = it + 1;
-            else h->iter_loops = it;
+            if (book_iteration(h, it, du, check != 0)) {
+                h->iter_loops = it;      // the reference breaks before IterationLoops++
+                h->done = true;
+                break;
+            }
+            h->iter_loops = it + 1;
         }
-        if (stop) { h->done = true; break; }
     }
     if (h->iter_loops > h->prm.iter_max) h->done = true;
     HIPCHK(h, hipEventRecord(h->ev1, h->stream));
@@ -604,6 +626,53 @@ extern "C" int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, 
     cp(uk, h->uk); cp(vk, h->vk); cp(wk, h->wk); cp(uks, h->uks); cp(vks, h->vks); cp(wks, h->wks);
     cp(dpm, h->dpm); cp(dpmnew, h->dpmnew); cp(turb, h->turb); cp(tdiss, h->tdiss);
     return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_profile_iterations(gral_pf_handle *h, int32_t n, gral_pf_family_time *out, int32_t nfam) {
+    if (!h || !out || nfam < 1) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "profile: call prepare first");
+    for (int f = 0; f < nfam; ++f) out[f] = gral_pf_family_time{};
+    int todo = std::min(n, h->prm.iter_max - h->iter_loops + 1);
+    if (h->done) todo = 0;
+    const int cap = 8 * (h->nii + h->njj) + 64;
+    std::vector<hipEvent_t> ev(2 * (size_t)cap);
+    std::vector<int> fam(cap);
+    for (auto &e : ev) HIPCHK(h, hipEventCreate(&e));
+    int rc = 0;
+    long long act = 0;
+    for (int i = 2; i <= h->nii - 1; ++i)
+        for (int j = 2; j <= h->njj - 1; ++j) {
+            size_t c = h->i2(i, j);
+            if (h->adv[c] == 1) act += h->vi[c] - h->kk[c];
+        }
+    std::vector<unsigned> bits(1);
+    for (int r = 0; r < todo && !rc; ++r) {
+        Probe pr{ev.data(), fam.data(), cap, 0};
+        long long launches = 0;
+        int it = h->iter_loops;
+        if (it < 1 || it > h->prm.iter_max) { rc = fail(h, GRAL_PF_ESTATE, "profile: iteration counter out of range"); break; }
+        int e = pf_launch_iteration(&h->d, it, it - 1, h->stream, &launches, &pr);
+        if (e) { rc = fail(h, GRAL_PF_EDEVICE, "profile: launch failed"); break; }
+        if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = fail(h, GRAL_PF_EDEVICE, "profile: sync failed"); break; }
+        for (int q = 0; q < pr.n; ++q) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]);
+            if (fam[q] < nfam) { out[fam[q]].ms += ms; out[fam[q]].launches += 1; }
+        }
+        for (int f = 0; f < nfam; ++f) out[f].cells += act;
+        (void)hipMemcpyAsync(bits.data(), h->d.resid + (it - 1), sizeof(unsigned), hipMemcpyDeviceToHost, h->stream);
+        (void)hipStreamSynchronize(h->stream);
+        float du;
+        std::memcpy(&du, &bits[0], 4);
+        h->iters_run++;
+        bool stop = book_iteration(h, it, du, false);
+        (void)stop;
+        h->iter_loops = it + 1;
+    }
+    if (h->iter_loops > h->prm.iter_max) h->done = true;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    h->host_stale = true;
+    return rc;
 }
 
 extern "C" int gral_pf_last_timing(gral_pf_handle *h, gral_pf_timing *t) {

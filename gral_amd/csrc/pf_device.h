@@ -32,6 +32,7 @@ struct Dom {
     // scratch
     float *prow_p, *prow_q;          // pressure PIMP/QIMP per sweep row [nj2][nkp]
     unsigned *resid;                 // DeltaUMax per iteration (float bits, values >= 0)
+    int resid_cap;                   // entries in resid
 };
 
 __device__ __forceinline__ size_t c2(const Dom &d, int i, int j) { return (size_t)i * (size_t)d.nj2 + (size_t)j; }

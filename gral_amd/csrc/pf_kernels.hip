@@ -198,7 +198,7 @@ __global__ __launch_bounds__(WAVE) void k_pressure(Dom d, int t, int IS, int JS,
         float other = __shfl_xor(rmax, off);
         rmax = other > rmax ? other : rmax;
     }
-    if (tid == 0 && rmax > 0.0f) atomicMax(&d.resid[slot], __float_as_uint(rmax));
+    if (tid == 0 && rmax > 0.0f && slot >= 0 && slot < d.resid_cap) atomicMax(&d.resid[slot], __float_as_uint(rmax));
 }
 
 // ------------------------------------------------------------------------
@@ -957,9 +957,32 @@ static inline int diag_len(int t, int oa, int ob, int ia, int ib) {
     return amax - amin + 1;
 }
 
-extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_t s, long long *launches) {
+// Optional per-launch timing: events recorded on the solver's stream right
+// before and after every launch, tagged with the kernel family.
+struct Probe {
+    hipEvent_t *ev;      // 2*cap events
+    int *fam;            // cap family tags
+    int cap, n;
+};
+enum { FAM_STENCIL = 0, FAM_PRESSURE = 1, FAM_U = 2, FAM_V = 3, FAM_W = 4, FAM_TKE = 5 };
+
+struct Launcher {
+    hipStream_t s;
+    Probe *pr;
+    long long n = 0;
+    template <class F>
+    void go(int fam, F &&f) {
+        bool rec = pr && pr->n < pr->cap;
+        if (rec) { (void)hipEventRecord(pr->ev[2 * pr->n], s); }
+        f();
+        if (rec) { (void)hipEventRecord(pr->ev[2 * pr->n + 1], s); pr->fam[pr->n] = fam; pr->n++; }
+        ++n;
+    }
+};
+
+extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_t s, long long *launches, Probe *probe) {
     const Dom &d = *dp;
-    long long nl = 0;
+    Launcher L{s, probe};
     int nk = d.kadvmax;
     int kb = (nk + 63) / 64;
     int IS = 1, JS = 1;
@@ -970,52 +993,50 @@ extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_
         default: break;
     }
     int ncols = (d.nii - 2) * (d.njj - 2);
-    hipLaunchKernelGGL(k_bc_ew, dim3(kb, d.njj - 1), dim3(64), 0, s, d); ++nl;
-    hipLaunchKernelGGL(k_bc_ns, dim3(kb, d.nii - 1), dim3(64), 0, s, d); ++nl;
-    hipLaunchKernelGGL(k_bc_top, dim3((ncols + 255) / 256), dim3(256), 0, s, d); ++nl;
-    hipLaunchKernelGGL(k_divergence, dim3(ncols), dim3(64), 0, s, d); ++nl;
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(kb, d.njj - 1), dim3(64), 0, s, d); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ns, dim3(kb, d.nii - 1), dim3(64), 0, s, d); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_top, dim3((ncols + 255) / 256), dim3(256), 0, s, d); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncols), dim3(64), 0, s, d); });
     size_t nrow = (size_t)d.nj2 * d.nkp;
-    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_p, nrow, 0.0f); ++nl;
-    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_q, nrow, 0.0f); ++nl;
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_p, nrow, 0.0f); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_q, nrow, 0.0f); });
     {
         int oa = 2, ob = d.njj - 1, ia = 2, ib = d.nii - 1;
         size_t shm = sizeof(float) * 5 * d.nkp;
-        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t) {
-            hipLaunchKernelGGL(k_pressure, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, d, t, IS, JS, slot);
-            ++nl;
-        }
+        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t)
+            L.go(FAM_PRESSURE, [&] {
+                hipLaunchKernelGGL(k_pressure, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, d, t, IS, JS, slot);
+            });
     }
-    hipLaunchKernelGGL(k_correction, dim3(ncols), dim3(64), 0, s, d); ++nl;
-    hipLaunchKernelGGL(k_averages, dim3(ncols), dim3(64), 0, s, d); ++nl;
-    hipLaunchKernelGGL(k_avg_edges, dim3(d.njj - 2), dim3(64), 0, s, d); ++nl;
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_correction, dim3(ncols), dim3(64), 0, s, d); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_averages, dim3(ncols), dim3(64), 0, s, d); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_avg_edges, dim3(d.njj - 2), dim3(64), 0, s, d); });
     size_t shm = sizeof(float) * 5 * d.nkp;
-    auto sweep = [&](auto kern, int oa, int ia, int is, int js) {
+    auto sweep = [&](int fam, auto kern, int oa, int ia, int is, int js) {
         int ob = d.nii - 1, ib = d.njj - 1;
-        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t) {
-            hipLaunchKernelGGL(kern, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, d, t, is, js);
-            ++nl;
-        }
+        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t)
+            L.go(fam, [&] { hipLaunchKernelGGL(kern, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, d, t, is, js); });
     };
     if (d.model == 1) {
-        sweep(k_momentum<0, 0>, 3, 2, IS, JS);
-        sweep(k_momentum<1, 0>, 2, 3, -IS, -JS);
-        sweep(k_momentum<2, 0>, 2, 2, IS, JS);
+        sweep(FAM_U, k_momentum<0, 0>, 3, 2, IS, JS);
+        sweep(FAM_V, k_momentum<1, 0>, 2, 3, -IS, -JS);
+        sweep(FAM_W, k_momentum<2, 0>, 2, 2, IS, JS);
     } else if (d.model == 2) {
-        sweep(k_momentum<0, 1>, 3, 2, IS, JS);
-        sweep(k_momentum<1, 1>, 2, 3, IS, JS);
-        sweep(k_momentum<2, 1>, 2, 2, IS, JS);
+        sweep(FAM_U, k_momentum<0, 1>, 3, 2, IS, JS);
+        sweep(FAM_V, k_momentum<1, 1>, 2, 3, IS, JS);
+        sweep(FAM_W, k_momentum<2, 1>, 2, 2, IS, JS);
         size_t shm_t = sizeof(float) * 12 * d.nkp;
         int oa = 2, ob = d.nii - 1, ia = 2, ib = d.njj - 1;
-        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t) {
-            hipLaunchKernelGGL(k_tke, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm_t, s, d, t, IS, JS);
-            ++nl;
-        }
+        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t)
+            L.go(FAM_TKE, [&] {
+                hipLaunchKernelGGL(k_tke, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm_t, s, d, t, IS, JS);
+            });
     } else {
-        sweep(k_momentum<0, 2>, 3, 2, IS, JS);
-        sweep(k_momentum<1, 2>, 2, 3, IS, JS);
-        sweep(k_momentum<2, 2>, 2, 2, IS, JS);
+        sweep(FAM_U, k_momentum<0, 2>, 3, 2, IS, JS);
+        sweep(FAM_V, k_momentum<1, 2>, 2, 3, IS, JS);
+        sweep(FAM_W, k_momentum<2, 2>, 2, 2, IS, JS);
     }
-    if (launches) *launches += nl;
+    if (launches) *launches += L.n;
     return (int)hipGetLastError();
 }
 

@@ -23,7 +23,8 @@ I16P = C.POINTER(C.c_int16)
 
 EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral_pf_upload_fields",
            "gral_pf_prepare", "gral_pf_iterate", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
-           "gral_pf_last_timing", "gral_pf_last_error", "gral_pf_destroy"]
+           "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_last_error", "gral_pf_destroy"]
+FAMILIES = ("stencil", "pressure", "u", "v", "w", "tke")
 
 
 class Grid(C.Structure):
@@ -51,6 +52,10 @@ class Timing(C.Structure):
                 ("ms_stencil", C.c_double), ("launches", C.c_int64), ("active_cells", C.c_int64)]
 
 
+class FamilyTime(C.Structure):
+    _fields_ = [("ms", C.c_double), ("launches", C.c_int64), ("cells", C.c_int64)]
+
+
 _lib = None
 
 
@@ -73,6 +78,7 @@ def load(path: str = LIB_PATH):
     lib.gral_pf_solve.argtypes = [H, C.POINTER(Params), C.POINTER(Stats)]
     lib.gral_pf_download_fields.argtypes = [H] + [FP] * 10
     lib.gral_pf_last_timing.argtypes = [H, C.POINTER(Timing)]
+    lib.gral_pf_profile_iterations.argtypes = [H, C.c_int32, C.POINTER(FamilyTime), C.c_int32]
     lib.gral_pf_last_error.argtypes = [H]
     lib.gral_pf_last_error.restype = C.c_char_p
     lib.gral_pf_destroy.argtypes = [H]
@@ -161,6 +167,14 @@ class PrognosticFlowfield:
         t = Timing()
         self._check(self.lib.gral_pf_last_timing(self.h, C.byref(t)))
         return {n: getattr(t, n) for n, _ in Timing._fields_}
+
+    def profile_families(self, n):
+        """Run n more iterations with every launch bracketed by HIP events;
+        returns {family: {ms, launches, cells}} for the families that ran."""
+        arr = (FamilyTime * len(FAMILIES))()
+        self._check(self.lib.gral_pf_profile_iterations(self.h, int(n), arr, len(FAMILIES)))
+        return {f: {"ms": arr[q].ms, "launches": arr[q].launches, "cells": arr[q].cells}
+                for q, f in enumerate(FAMILIES) if arr[q].launches > 0}
 
     def download(self):
         shp = self.case.uk.shape

@@ -130,6 +130,17 @@ typedef struct {
 } gral_pf_timing;
 int gral_pf_last_timing(gral_pf_handle *h, gral_pf_timing *t);
 
+/* Measurement: runs n SIMPLE iterations (continuing the solve) with HIP events
+ * around every single kernel launch, and sums the device time per kernel
+ * family: 0 stencil passes, 1 pressure sweep, 2 U, 3 V, 4 W momentum sweeps,
+ * 5 TKE sweep.  cells = active cells advanced per iteration times iterations. */
+typedef struct {
+    double ms;
+    int64_t launches;
+    int64_t cells;
+} gral_pf_family_time;
+int gral_pf_profile_iterations(gral_pf_handle *h, int32_t n, gral_pf_family_time *out, int32_t nfam);
+
 const char *gral_pf_last_error(const gral_pf_handle *h);
 void gral_pf_destroy(gral_pf_handle *h);
 

@@ -55,3 +55,33 @@ def test_tiny_vegetation(model):
 def test_config1_short(iterations):
     c = domain.config1(turbulence_model=1, iterations=iterations)
     _compare(c, iterations)
+
+
+def small_city(model, iterations, n=48, seed=11):
+    bl = domain.city_buildings(n, n, 2.0, seed, density=0.25, fmin=6.0, fmax=16.0, hmin=4.0, hmax=14.0, margin=3)
+    return domain.build_case(f"city{n}", n, n, bl, dx=2.0, vertcellsff=12, speed10=4.0, direction=235.0, z0=0.5,
+                             turbulence_model=model, iterations=iterations)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", [1, 2, 0])
+def test_small_city(model):
+    # many buildings of different heights: stale PIMP tops, building columns
+    # starting at k=1, wall functions on roofs and terrain, all four sweep phases
+    _compare(small_city(model, 9), 9)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", [2, 0])
+def test_config1_other_models(model):
+    c = domain.config1(turbulence_model=model, iterations=6)
+    _compare(c, 6)
+
+
+@pytest.mark.gpu
+def test_config1_200_iterations():
+    # the full fixed-iteration parity run of config 1 (200 SIMPLE iterations,
+    # convergence bookkeeping at 100 and 200)
+    c = domain.config1(turbulence_model=1, iterations=200)
+    rep = _compare(c, 200)
+    print(rep)
