@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -403,6 +405,48 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
         app[k] = 2.0f * (h->dyk * DZK[k] / h->dxk + h->dxk * DZK[k] / h->dyk + area / DZK[k]);
         abp[k] = (k > 1) ? area / (0.5f * (DZK[k - 1] + DZK[k])) : area / DZK[k];
         atp[k] = area / (0.5f * (DZK[k + 1] + DZK[k]));
+    }
+    // sweep kernels: tiled wavefront (default) or one launch per anti-diagonal
+    const char *sw = std::getenv("GRAL_PF_SWEEP");
+    d.tiled = !(sw && std::strcmp(sw, "diag") == 0);
+    d.planes = nullptr;
+    d.ptab[0] = d.ptab[1] = d.ttab[0] = d.ttab[1] = nullptr;
+    if (d.tiled) {
+        if (p->turbulence_model == 1) {
+            float *pl;
+            if ((rc = dalloc(h, &pl, 16 * n3))) return rc;
+            d.planes = pl;
+        }
+        // PIMP/TERMP of the pressure TDMA for every column, in the reference's
+        // row order (fresh row arrays per j row, columns in i1 order for IS=+1 and
+        // IS=-1), so the stale top value PIMP[VI+1] is part of the table
+        std::vector<float> tp(n3, 0.0f), tt(n3, 0.0f), prow((size_t)h->nkh + 1, 0.0f);
+        for (int v = 0; v < 2; ++v) {
+            std::fill(tp.begin(), tp.end(), 0.0f);
+            std::fill(tt.begin(), tt.end(), 0.0f);
+            for (int j = 2; j <= h->njj - 1; ++j) {
+                std::fill(prow.begin(), prow.end(), 0.0f);
+                for (int i1 = 2; i1 <= h->nii - 1; ++i1) {
+                    int i = (v == 1) ? h->nii - i1 + 1 : i1;
+                    size_t c = h->i2(i, j);
+                    if (h->adv[c] != 1) continue;
+                    int KK = h->kk[c], VI = h->vi[c];
+                    size_t o = c * (size_t)h->nkp;
+                    for (int k = VI; k > KK; --k) {
+                        float TERMP = 1 / (app[k] - atp[k] * prow[k + 1]);
+                        float P = abp[k] * TERMP;
+                        prow[k] = P;
+                        tp[o + k] = P;
+                        tt[o + k] = TERMP;
+                    }
+                }
+            }
+            float *dp, *dt;
+            if ((rc = dalloc(h, &dp, n3)) || (rc = dalloc(h, &dt, n3)) || (rc = put_vec(h, dp, tp)) || (rc = put_vec(h, dt, tt)))
+                return rc;
+            d.ptab[v] = dp;
+            d.ttab[v] = dt;
+        }
     }
     std::vector<int> kk32(n2);
     for (size_t c = 0; c < n2; ++c) kk32[c] = h->kk[c];

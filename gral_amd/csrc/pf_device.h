@@ -33,6 +33,10 @@ struct Dom {
     float *prow_p, *prow_q;          // pressure PIMP/QIMP per sweep row [nj2][nkp]
     unsigned *resid;                 // DeltaUMax per iteration (float bits, values >= 0)
     int resid_cap;                   // entries in resid
+    // tiled sweeps
+    int tiled;                       // 1: tiled wavefront kernels, 0: one launch per anti-diagonal
+    float *planes;                   // [16][nii+2][njj+2][nkp] mixing-length coefficient planes
+    const float *ptab[2], *ttab[2];  // pressure PIMP/TERMP per column for IS=+1 / IS=-1
 };
 
 __device__ __forceinline__ size_t c2(const Dom &d, int i, int j) { return (size_t)i * (size_t)d.nj2 + (size_t)j; }

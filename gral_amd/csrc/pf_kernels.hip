@@ -369,12 +369,26 @@ __device__ __forceinline__ float ml_wall_uv(const Dom &d, const ColInfo &q, int 
     return vel / wh * (us * us) * d.area;
 }
 
-__device__ Coef u1_cell(const Dom &d, const ColInfo &q, int k) {
+// Pieces of the mixing-length source term D of one cell.  Everything here is
+// fixed during a sweep (it reads only arrays the sweep does not write, plus the
+// cell's own column, which is updated only when its own turn comes), except
+// the four neighbour values of the swept component, which enter as
+//   D = AW1*X_W + AS1*X_S + AE1*X_E + AN1*X_N + e + f + g  [- w]  [+ cross]
+// summed left to right exactly as the reference does.
+struct MlPieces {
+    float AW1, AS1, AE1, AN1, A, B, C;
+    float e, f, g, w;          // g: Coriolis(+vegetation) term of U/V; vegetation drag (subtracted) of W
+    float c0, X1, X2, X3, X4;  // cross terms: c0 = VIS/DXK, X* the neighbour-free gradients (W: X1 = whole term)
+};
+
+__device__ __forceinline__ bool ml_cross_on(const ColInfo &q, int k) {
+    return block_base(k, q.KSTART, q.VI + 1, q.W) > q.KK + 1;
+}
+
+__device__ MlPieces u1_pieces(const Dom &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
     const float *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j);
     const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
-    const float *UK_W = d.uk + c3(d, i - 1, j), *UK_E = d.uk + c3(d, i + 1, j);
-    const float *UK_S = d.uk + c3(d, i, j - 1), *UK_N = d.uk + c3(d, i, j + 1);
     const float *UKS_W = d.uks + c3(d, i - 1, j);
     const float *VKS_W = d.vks + c3(d, i - 1, j), *VKS_WS = d.vks + c3(d, i - 1, j - 1);
     const float *VKS_S = d.vks + c3(d, i, j - 1), *VKS_WN = d.vks + c3(d, i - 1, j + 1);
@@ -390,42 +404,34 @@ __device__ Coef u1_cell(const Dom &d, const ColInfo &q, int k) {
     float FN = 0.25f * (VKS_W[k] + VKS[k] + VKS_WN[k] + VKS_N[k]) * dxkdzk;
     float FT = 0.25f * (WKS_W[k] + WKS[k] + WKS_W[k + 1] + WKS[k + 1]) * d.area;
     float FB = (k > q.KK + 1) ? 0.25f * (WKS_W[k] + WKS[k] + WKS_W[k - 1] + WKS[k - 1]) * d.area : 0.0f;
-    float AW1, AS1, AE1, AN1, BIM, CIM, AIM;
-    ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, AW1, AS1, AE1, AN1, BIM, CIM, AIM);
+    MlPieces o;
+    ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, o.AW1, o.AS1, o.AE1, o.AN1, o.B, o.C, o.A);
     float ap0 = d.ap0[k];
-    float ddpx = DPMNEW_W[k] - DPMNEW[k];
     float u = UK[k];
-    float D;
+    o.e = ap0 * 0.5f * (UKS_W[k] + UKS[k]);
+    o.f = (DPMNEW_W[k] - DPMNEW[k]) * dykdzk;
     if (q.vegp) {
         float in2 = 0.5f * (VKS_W[k] + VKS[k]);
         float in = 0.5f * (UKS_W[k] + UKS[k]);
         float wh = vmaxf(sqrtf(in * in + in2 * in2), 0.01f);
         float veg = d.veg[c3(d, i, j) + k - 1];
-        D = AW1 * UK_W[k] + AS1 * UK_S[k] + AE1 * UK_E[k] + AN1 * UK_N[k] + ap0 * 0.5f * (UKS_W[k] + UKS[k]) +
-            ddpx * dykdzk + (d.cor * (d.ug - u) - veg * u * wh) * d.area * dz;
+        o.g = (d.cor * (d.ug - u) - veg * u * wh) * d.area * dz;
     } else {
-        D = AW1 * UK_W[k] + AS1 * UK_S[k] + AE1 * UK_E[k] + AN1 * UK_N[k] + ap0 * 0.5f * (UKS_W[k] + UKS[k]) +
-            ddpx * dykdzk + d.cor * (d.ug - u) * d.area * dz;
+        o.g = d.cor * (d.ug - u) * d.area * dz;
     }
-    if (k == q.KK + 1) D -= ml_wall_uv(d, q, k, u, UKS, UKS_W, VKS, VKS_W);
-    if (block_base(k, q.KSTART, q.VI + 1, q.W) > q.KK + 1) {
-        float DUDXE = (UK_E[k] - u) * dykdzk;
-        float DUDXW = (u - UK_W[k]) * dykdzk;
-        float DVDXN = (0.5f * (VKS_EN[k] + VKS_N[k]) - 0.5f * (VKS_N[k] + VKS_WN[k])) * dxkdzk;
-        float DVDXS = (0.5f * (VKS_ES[k] + VKS_S[k]) - 0.5f * (VKS_S[k] + VKS_WS[k])) * dxkdzk;
-        float DWDXT = (0.5f * (WK_E[k] + WKS[k]) - 0.5f * (WKS[k] + WKS_W[k])) * d.area;
-        float DWDXB = (0.5f * (WK_E[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKS_W[k - 1])) * d.area;
-        D += VIS / d.dxk * (DUDXE - DUDXW + DVDXN - DVDXS + DWDXT - DWDXB);
-    }
-    return Coef{AIM, BIM, CIM, D};
+    o.w = (k == q.KK + 1) ? ml_wall_uv(d, q, k, u, UKS, UKS_W, VKS, VKS_W) : 0.0f;
+    o.c0 = VIS / d.dxk;
+    o.X1 = (0.5f * (VKS_EN[k] + VKS_N[k]) - 0.5f * (VKS_N[k] + VKS_WN[k])) * dxkdzk;        // DVDXN
+    o.X2 = (0.5f * (VKS_ES[k] + VKS_S[k]) - 0.5f * (VKS_S[k] + VKS_WS[k])) * dxkdzk;        // DVDXS
+    o.X3 = (0.5f * (WK_E[k] + WKS[k]) - 0.5f * (WKS[k] + WKS_W[k])) * d.area;                // DWDXT
+    o.X4 = (0.5f * (WK_E[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKS_W[k - 1])) * d.area; // DWDXB
+    return o;
 }
 
-__device__ Coef v1_cell(const Dom &d, const ColInfo &q, int k) {
+__device__ MlPieces v1_pieces(const Dom &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
     const float *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j);
     const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
-    const float *VK_W = d.vk + c3(d, i - 1, j), *VK_E = d.vk + c3(d, i + 1, j);
-    const float *VK_S = d.vk + c3(d, i, j - 1), *VK_N = d.vk + c3(d, i, j + 1);
     const float *UKS_W = d.uks + c3(d, i - 1, j), *UKS_E = d.uks + c3(d, i + 1, j);
     const float *UKS_S = d.uks + c3(d, i, j - 1), *UKS_ES = d.uks + c3(d, i + 1, j - 1);
     const float *UKS_WS = d.uks + c3(d, i - 1, j - 1), *UKS_EN = d.uks + c3(d, i + 1, j + 1);
@@ -441,42 +447,34 @@ __device__ Coef v1_cell(const Dom &d, const ColInfo &q, int k) {
     float FN = VKS[k] * dxkdzk;
     float FT = 0.25f * (WKS_S[k] + WKS[k] + WKS_S[k + 1] + WKS[k + 1]) * d.area;
     float FB = (k > q.KK + 1) ? 0.25f * (WKS_S[k] + WKS[k] + WKS_S[k - 1] + WKS[k - 1]) * d.area : 0.0f;
-    float AW1, AS1, AE1, AN1, BIM, CIM, AIM;
-    ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, AW1, AS1, AE1, AN1, BIM, CIM, AIM);
+    MlPieces o;
+    ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, o.AW1, o.AS1, o.AE1, o.AN1, o.B, o.C, o.A);
     float ap0 = d.ap0[k];
-    float ddpy = DPMNEW_S[k] - DPMNEW[k];
     float v = VK[k];
-    float D;
+    o.e = ap0 * 0.5f * (VKS_S[k] + VKS[k]);
+    o.f = (DPMNEW_S[k] - DPMNEW[k]) * dxkdzk;
     if (q.vegp) {
         float in2 = 0.5f * (VKS_W[k] + VKS[k]);
         float in = 0.5f * (UKS_W[k] + UKS[k]);
         float wh = vmaxf(sqrtf(in * in + in2 * in2), 0.01f);
         float veg = d.veg[c3(d, i, j) + k - 1];
-        D = AW1 * VK_W[k] + AS1 * VK_S[k] + AE1 * VK_E[k] + AN1 * VK_N[k] + ap0 * 0.5f * (VKS_S[k] + VKS[k]) +
-            ddpy * dxkdzk + (d.cor * (d.vg - v) - veg * v * wh) * d.area * dz;
+        o.g = (d.cor * (d.vg - v) - veg * v * wh) * d.area * dz;
     } else {
-        D = AW1 * VK_W[k] + AS1 * VK_S[k] + AE1 * VK_E[k] + AN1 * VK_N[k] + ap0 * 0.5f * (VKS_S[k] + VKS[k]) +
-            ddpy * dxkdzk + d.cor * (d.vg - v) * d.area * dz;
+        o.g = d.cor * (d.vg - v) * d.area * dz;
     }
-    if (k == q.KK + 1) D -= ml_wall_uv(d, q, k, v, UKS, UKS_W, VKS, VKS_W);
-    if (block_base(k, q.KSTART, q.VI + 1, q.W) > q.KK + 1) {
-        float DVDYN = (VK_N[k] - v) * dxkdzk;
-        float DVDYS = (v - VK_S[k]) * dxkdzk;
-        float DUDYE = (0.5f * (UKS_EN[k] + UKS_E[k]) - 0.5f * (UKS_E[k] + UKS_ES[k])) * dxkdzk;
-        float DUDYW = (0.5f * (UKS_WN[k] + UKS_W[k]) - 0.5f * (UKS_W[k] + UKS_WS[k])) * dxkdzk;
-        float DWDYT = (0.5f * (WK_N[k] + WKS[k]) - 0.5f * (WKS[k] + WKS_S[k])) * d.area;
-        // the reference takes WKS of the southern column at k, not k-1, here
-        float DWDYB = (0.5f * (WK_N[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKS_S[k])) * d.area;
-        D += VIS / d.dxk * (DUDYE - DUDYW + DVDYN - DVDYS + DWDYT - DWDYB);
-    }
-    return Coef{AIM, BIM, CIM, D};
+    o.w = (k == q.KK + 1) ? ml_wall_uv(d, q, k, v, UKS, UKS_W, VKS, VKS_W) : 0.0f;
+    o.c0 = VIS / d.dxk;
+    o.X1 = (0.5f * (UKS_EN[k] + UKS_E[k]) - 0.5f * (UKS_E[k] + UKS_ES[k])) * dxkdzk;        // DUDYE
+    o.X2 = (0.5f * (UKS_WN[k] + UKS_W[k]) - 0.5f * (UKS_W[k] + UKS_WS[k])) * dxkdzk;        // DUDYW
+    o.X3 = (0.5f * (WK_N[k] + WKS[k]) - 0.5f * (WKS[k] + WKS_S[k])) * d.area;                // DWDYT
+    // the reference takes WKS of the southern column at k, not k-1, here
+    o.X4 = (0.5f * (WK_N[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKS_S[k])) * d.area;    // DWDYB
+    return o;
 }
 
-__device__ Coef w1_cell(const Dom &d, const ColInfo &q, int k) {
+__device__ MlPieces w1_pieces(const Dom &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
     const float *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j), *WK = d.wk + c3(d, i, j);
-    const float *WK_W = d.wk + c3(d, i - 1, j), *WK_E = d.wk + c3(d, i + 1, j);
-    const float *WK_S = d.wk + c3(d, i, j - 1), *WK_N = d.wk + c3(d, i, j + 1);
     const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
     const float *UKS_W = d.uks + c3(d, i - 1, j), *UKS_E = d.uks + c3(d, i + 1, j);
     const float *VKS_S = d.vks + c3(d, i, j - 1), *VKS_N = d.vks + c3(d, i, j + 1);
@@ -489,34 +487,93 @@ __device__ Coef w1_cell(const Dom &d, const ColInfo &q, int k) {
     float FN = 0.25f * (VKS[k] + VKS_N[k] + VKS[k - 1] + VKS_N[k - 1]) * dxkdzk;
     float FT = WKS[k] * d.area;
     float FB = (k > q.KK + 1) ? WKS[k - 1] * d.area : 0.0f;
-    float AW1, AS1, AE1, AN1, BIM, CIM, AIM;
-    ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, AW1, AS1, AE1, AN1, BIM, CIM, AIM);
+    MlPieces o;
+    ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, o.AW1, o.AS1, o.AE1, o.AN1, o.B, o.C, o.A);
     float ap0 = d.ap0[k];
-    float ddpz = DPMNEW[k - 1] - DPMNEW[k];
     float w = WK[k];
-    float D;
+    o.e = ap0 * 0.5f * (WKS[k - 1] + WKS[k]);
+    o.f = (DPMNEW[k - 1] - DPMNEW[k]) * d.area;
     if (q.vegp) {
         float in2 = 0.5f * (VKS_S[k] + VKS[k]);
         float in = 0.5f * (UKS_W[k] + UKS[k]);
         float wh = vmaxf(sqrtf(in * in + in2 * in2), 0.01f);
         float veg = d.veg[c3(d, i, j) + k - 1];
-        D = AW1 * WK_W[k] + AS1 * WK_S[k] + AE1 * WK_E[k] + AN1 * WK_N[k] + ap0 * 0.5f * (WKS[k - 1] + WKS[k]) +
-            ddpz * d.area - veg * w * wh * d.area * dz;
+        o.g = veg * w * wh * d.area * dz;
     } else {
-        D = AW1 * WK_W[k] + AS1 * WK_S[k] + AE1 * WK_E[k] + AN1 * WK_N[k] + ap0 * 0.5f * (WKS[k - 1] + WKS[k]) +
-            ddpz * d.area;
+        o.g = 0.0f;
     }
-    if (block_base(k, q.KSTART, q.VI + 1, q.W) > q.KK + 1) {
-        float DUDZE = (0.5f * (UKS[k + 1] + UKS[k]) - 0.5f * (UKS[k] + UKS[k - 1])) * dykdzk;
-        float DUDZW = (0.5f * (UKS_W[k + 1] + UKS_W[k]) - 0.5f * (UKS_W[k] + UKS_W[k - 1])) * dykdzk;
-        float DVDZN = (0.5f * (VKS[k + 1] + VKS[k]) - 0.5f * (VKS[k] + VKS[k - 1])) * dxkdzk;
-        float DVDZS = (0.5f * (VKS_S[k + 1] + VKS_S[k]) - 0.5f * (VKS_S[k] + VKS_S[k - 1])) * dxkdzk;
-        float DWDZT = (WK[k + 1] - w) * d.area;
-        float DWDZB = (w - WK[k - 1]) * d.area;
-        D += VIS / dz * (DUDZE - DUDZW + DVDZN - DVDZS + DWDZT - DWDZB);
-    }
-    return Coef{AIM, BIM, CIM, D};
+    o.w = 0.0f;
+    float DUDZE = (0.5f * (UKS[k + 1] + UKS[k]) - 0.5f * (UKS[k] + UKS[k - 1])) * dykdzk;
+    float DUDZW = (0.5f * (UKS_W[k + 1] + UKS_W[k]) - 0.5f * (UKS_W[k] + UKS_W[k - 1])) * dykdzk;
+    float DVDZN = (0.5f * (VKS[k + 1] + VKS[k]) - 0.5f * (VKS[k] + VKS[k - 1])) * dxkdzk;
+    float DVDZS = (0.5f * (VKS_S[k + 1] + VKS_S[k]) - 0.5f * (VKS_S[k] + VKS_S[k - 1])) * dxkdzk;
+    float DWDZT = (WK[k + 1] - w) * d.area;
+    float DWDZB = (w - WK[k - 1]) * d.area;
+    o.c0 = 0.0f;
+    o.X1 = VIS / dz * (DUDZE - DUDZW + DVDZN - DVDZS + DWDZT - DWDZB);
+    o.X2 = o.X3 = o.X4 = 0.0f;
+    return o;
 }
+
+// D of one cell from its pieces and the four neighbour values (and the cell's
+// own old value x), in the reference's order of operations.
+template <int COMP>
+__device__ __forceinline__ float ml_source(const Dom &d, const ColInfo &q, int k, const MlPieces &o, float xw, float xs,
+                                           float xe, float xn, float x) {
+    float D = o.AW1 * xw + o.AS1 * xs + o.AE1 * xe + o.AN1 * xn + o.e + o.f;
+    if (COMP != 2) {
+        D = D + o.g;
+        if (k == q.KK + 1) D -= o.w;
+        if (ml_cross_on(q, k)) {
+            float dz = d.dzk[k];
+            if (COMP == 0) {
+                float dykdzk = d.dyk * dz;
+                float DUDXE = (xe - x) * dykdzk;
+                float DUDXW = (x - xw) * dykdzk;
+                D += o.c0 * (DUDXE - DUDXW + o.X1 - o.X2 + o.X3 - o.X4);
+            } else {
+                float dxkdzk = d.dxk * dz;
+                float DVDYN = (xn - x) * dxkdzk;
+                float DVDYS = (x - xs) * dxkdzk;
+                D += o.c0 * (o.X1 - o.X2 + DVDYN - DVDYS + o.X3 - o.X4);
+            }
+        }
+    } else {
+        if (q.vegp) D = D - o.g;
+        if (ml_cross_on(q, k)) D += o.X1;
+    }
+    return D;
+}
+
+template <int COMP>
+__device__ __forceinline__ MlPieces ml_pieces(const Dom &d, const ColInfo &q, int k) {
+    if (COMP == 0) return u1_pieces(d, q, k);
+    if (COMP == 1) return v1_pieces(d, q, k);
+    return w1_pieces(d, q, k);
+}
+
+// neighbour values X_W, X_S, X_E, X_N of the swept component at level k
+template <int COMP>
+__device__ __forceinline__ void ml_neighbours(const Dom &d, int i, int j, int k, float &xw, float &xs, float &xe,
+                                              float &xn, float &x) {
+    const float *X = COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk;
+    xw = X[c3(d, i - 1, j) + k];
+    xe = X[c3(d, i + 1, j) + k];
+    xs = X[c3(d, i, j - 1) + k];
+    xn = X[c3(d, i, j + 1) + k];
+    x = X[c3(d, i, j) + k];
+}
+
+template <int COMP>
+__device__ Coef ml_cell(const Dom &d, const ColInfo &q, int k) {
+    MlPieces o = ml_pieces<COMP>(d, q, k);
+    float xw, xs, xe, xn, x;
+    ml_neighbours<COMP>(d, q.i, q.j, k, xw, xs, xe, xn, x);
+    return Coef{o.A, o.B, o.C, ml_source<COMP>(d, q, k, o, xw, xs, xe, xn, x)};
+}
+__device__ Coef u1_cell(const Dom &d, const ColInfo &q, int k) { return ml_cell<0>(d, q, k); }
+__device__ Coef v1_cell(const Dom &d, const ColInfo &q, int k) { return ml_cell<1>(d, q, k); }
+__device__ Coef w1_cell(const Dom &d, const ColInfo &q, int k) { return ml_cell<2>(d, q, k); }
 
 // ------------------------------------------------------------------------
 // k-eps (model 2) and no-diffusion (model 0) momentum, _2.cs / _0.cs
@@ -947,6 +1004,231 @@ __global__ __launch_bounds__(WAVE) void k_tke(Dom d, int t, int IS, int JS) {
 }
 
 // ------------------------------------------------------------------------
+// Tiled wavefront path (mixing-length model and pressure)
+// ------------------------------------------------------------------------
+// The sweep plane (a, b) is cut into TILE x TILE tiles.  Tiles on one tile
+// anti-diagonal are independent and run as the workgroups of one launch; the
+// tile anti-diagonals run in order, one launch each.  Inside a tile, wave r
+// owns row b = b0 + r and at local step s handles column a = a0 + s - r, so
+// every local step is one anti-diagonal of the tile.  Each step is three
+// phases separated by workgroup barriers: (1) all lanes of every active wave
+// assemble the column's source terms (lanes over k) into LDS, (2) lane 0 of
+// every active wave runs the serial Thomas recurrences, (3) all lanes write
+// the new column back.  Neighbour values written by other waves in earlier
+// steps are read from global memory after the barrier; values from other
+// tiles were written by earlier launches.
+#define TILE 16
+
+// coefficient planes written by k_ml_coef, read by k_ml_tile
+enum { PL_AW = 0, PL_AS, PL_AE, PL_AN, PL_C, PL_T, PL_P, PL_E, PL_F, PL_G, PL_W, PL_C0, PL_X1, PL_X2, PL_X3, PL_X4, PL_N };
+
+__device__ __forceinline__ void sweep_to_ij(const Dom &d, int comp_is_pressure, int a, int b, int IS, int JS, int &i, int &j) {
+    if (comp_is_pressure) {   // a = j1, b = i1
+        j = (JS == -1) ? d.njj - a + 1 : a;
+        i = (IS == -1) ? d.nii - b + 1 : b;
+    } else {                  // a = i1, b = j1
+        i = (IS == -1) ? d.nii - a + 1 : a;
+        j = (JS == -1) ? d.njj - b + 1 : b;
+    }
+}
+
+// All neighbour-independent work of one mixing-length sweep, for every column
+// of the sweep at once: coefficients, source pieces, and the P/TERMP half of
+// the Thomas recurrence (it depends only on A, B, C).
+template <int COMP>
+__global__ __launch_bounds__(WAVE) void k_ml_coef(Dom d, float *planes, int IS, int JS) {
+    extern __shared__ float sm[];
+    const int nkp = d.nkp, tid = threadIdx.x;
+    const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2;
+    const int na = d.nii - 1 - oa + 1, nb = d.njj - 1 - ia + 1;
+    int col = blockIdx.x;
+    if (col >= na * nb) return;
+    int a = oa + col / nb, b = ia + col % nb;
+    int i, j;
+    sweep_to_ij(d, 0, a, b, IS, JS, i, j);
+    if (d.adv[c2(d, i, j)] != 1) return;
+    ColInfo q = col_info(d, i, j);
+    float *sA = sm, *sB = sA + nkp, *sC = sB + nkp;
+    const size_t n3 = (size_t)(d.nii + 2) * d.nj2 * nkp;
+    const size_t o = c3(d, i, j);
+    for (int k = q.KSTART + tid; k <= q.VI; k += WAVE) {
+        MlPieces p = ml_pieces<COMP>(d, q, k);
+        sA[k] = p.A; sB[k] = p.B; sC[k] = p.C;
+        float *P = planes + o + k;
+        P[PL_AW * n3] = p.AW1; P[PL_AS * n3] = p.AS1; P[PL_AE * n3] = p.AE1; P[PL_AN * n3] = p.AN1;
+        P[PL_C * n3] = p.C; P[PL_E * n3] = p.e; P[PL_F * n3] = p.f; P[PL_G * n3] = p.g; P[PL_W * n3] = p.w;
+        P[PL_C0 * n3] = p.c0; P[PL_X1 * n3] = p.X1; P[PL_X2 * n3] = p.X2; P[PL_X3 * n3] = p.X3; P[PL_X4 * n3] = p.X4;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int KKP1 = q.KK + 1;
+        float Pp = 0;
+        for (int k = q.KSTART; k <= q.VI; ++k) {
+            float T = (k > KKP1) ? 1 / (sA[k] - sC[k] * Pp) : 1 / sA[k];
+            float P = sB[k] * T;
+            planes[PL_T * n3 + o + k] = T;
+            planes[PL_P * n3 + o + k] = P;
+            Pp = P;
+        }
+    }
+}
+
+__device__ __forceinline__ bool tile_coords(int tau, int nta, int ntb, int &A, int &B) {
+    int amin = max(0, tau - (ntb - 1));
+    A = amin + blockIdx.x;
+    B = tau - A;
+    return A < nta && B >= 0;
+}
+
+template <int COMP>
+__global__ __launch_bounds__(TILE * WAVE) void k_ml_tile(Dom d, const float *planes, int tau, int IS, int JS) {
+    extern __shared__ float sm[];
+    const int nkp = d.nkp;
+    const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2, ob = d.nii - 1, ib = d.njj - 1;
+    const int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
+    int A, B;
+    if (!tile_coords(tau, nta, ntb, A, B)) return;     // uniform over the workgroup
+    const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int a0 = oa + A * TILE, b = ia + B * TILE + r;
+    float *sD = sm + (size_t)r * 5 * nkp, *sQ = sD + nkp, *sX = sQ + nkp, *sT = sX + nkp, *sPr = sT + nkp;
+    const size_t n3 = (size_t)(d.nii + 2) * d.nj2 * nkp;
+    float *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk);
+    for (int s = 0; s < 2 * TILE - 1; ++s) {
+        int a = a0 + s - r;
+        bool act = (s - r >= 0) && (s - r < TILE) && a <= ob && b <= ib;
+        int i = 0, j = 0;
+        ColInfo q;
+        if (act) {
+            sweep_to_ij(d, 0, a, b, IS, JS, i, j);
+            act = d.adv[c2(d, i, j)] == 1;
+        }
+        if (act) {
+            q = col_info(d, i, j);
+            const size_t o = c3(d, i, j);
+            for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) {
+                const float *P = planes + o + k;
+                MlPieces p;
+                p.AW1 = P[PL_AW * n3]; p.AS1 = P[PL_AS * n3]; p.AE1 = P[PL_AE * n3]; p.AN1 = P[PL_AN * n3];
+                p.e = P[PL_E * n3]; p.f = P[PL_F * n3]; p.g = P[PL_G * n3]; p.w = P[PL_W * n3];
+                p.c0 = P[PL_C0 * n3]; p.X1 = P[PL_X1 * n3]; p.X2 = P[PL_X2 * n3]; p.X3 = P[PL_X3 * n3]; p.X4 = P[PL_X4 * n3];
+                float xw, xs, xe, xn, x;
+                ml_neighbours<COMP>(d, i, j, k, xw, xs, xe, xn, x);
+                sD[k] = ml_source<COMP>(d, q, k, p, xw, xs, xe, xn, x);
+                sQ[k] = P[PL_C * n3];
+                sT[k] = P[PL_T * n3];
+                sPr[k] = P[PL_P * n3];
+            }
+            for (int k = q.KSTART + lane; k <= q.VI + 1; k += WAVE) sX[k] = X[o + k];
+        }
+        __syncthreads();
+        if (act && lane == 0) {
+            const int KKP1 = q.KK + 1;
+            float Qp = 0;
+            for (int k = q.KSTART; k <= q.VI; ++k) {
+                float Q = (k > KKP1) ? (sD[k] + sQ[k] * Qp) * sT[k] : sD[k] * sT[k];
+                sQ[k] = Q;
+                Qp = Q;
+            }
+            int KKm = q.KK;
+            if (COMP == 0) KKm = max(q.KK, d.kk[c2(d, i - 1, j)]);
+            if (COMP == 1) KKm = max(q.KK, d.kk[c2(d, i, j - 1)]);
+            float up = sX[q.VI + 1];
+            for (int k = q.VI; k >= q.KSTART; --k) {
+                float x = sX[k];
+                if (COMP == 2 || KKm < k) {
+                    x = x + d.relax * (sPr[k] * up + sQ[k] - x);
+                    sX[k] = x;
+                }
+                up = x;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            const size_t o = c3(d, i, j);
+            for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) X[o + k] = sX[k];
+        }
+        __syncthreads();
+    }
+}
+
+// Pressure sweep on tiles.  PIMP/TERMP of every column depend only on the
+// geometry (including the stale top value of the row), so they are tabulated
+// once per solve for both i directions (ptab/ttab); only the QIMP recurrence,
+// with its own stale top value from the row scratch, depends on the flow.
+__global__ __launch_bounds__(TILE * WAVE) void k_pressure_tile(Dom d, const float *ptab, const float *ttab, int tau,
+                                                              int IS, int JS, int slot) {
+    extern __shared__ float sm[];
+    const int nkp = d.nkp;
+    const int oa = 2, ia = 2, ob = d.njj - 1, ib = d.nii - 1;
+    const int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
+    int A, B;
+    if (!tile_coords(tau, nta, ntb, A, B)) return;
+    const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int a0 = oa + A * TILE, b = ia + B * TILE + r;
+    float *sD = sm + (size_t)r * 5 * nkp, *sQ = sD + nkp, *sO = sQ + nkp, *sT = sO + nkp, *sPr = sT + nkp;
+    const float DXK = d.dxk, DYK = d.dyk;
+    float rmax = 0.0f;
+    for (int s = 0; s < 2 * TILE - 1; ++s) {
+        int a = a0 + s - r;
+        bool act = (s - r >= 0) && (s - r < TILE) && a <= ob && b <= ib;
+        int i = 0, j = 0, KK = 0, VI = 0;
+        if (act) {
+            sweep_to_ij(d, 1, a, b, IS, JS, i, j);
+            act = d.adv[c2(d, i, j)] == 1;
+        }
+        if (act) {
+            size_t c = c2(d, i, j);
+            KK = d.kk[c]; VI = d.vi[c];
+            size_t o = c3(d, i, j), oim = c3(d, i - 1, j), oip = c3(d, i + 1, j), ojm = c3(d, i, j - 1), ojp = c3(d, i, j + 1);
+            for (int k = KK + 1 + lane; k <= VI; k += WAVE) {
+                float dz = d.dzk[k];
+                sD[k] = d.div[o + k] / d.dtime + (d.dpm[oim + k] + d.dpm[oip + k]) / DXK * DYK * dz +
+                        (d.dpm[ojm + k] - d.dpm[ojp + k]) / DYK * DXK * dz;
+                sT[k] = ttab[o + k];
+                sPr[k] = ptab[o + k];
+            }
+            for (int k = KK + lane; k <= VI; k += WAVE) sO[k] = d.dpm[o + k];
+        }
+        __syncthreads();
+        if (act && lane == 0) {
+            const float *qrow = d.prow_q + (size_t)j * nkp;
+            float Qn = qrow[VI + 1];
+            for (int k = VI; k > KK; --k) {
+                float Q = (d.atp[k] * Qn + sD[k]) * sT[k];
+                sQ[k] = Q;
+                Qn = Q;
+            }
+            float dprev = sO[KK];
+            for (int k = KK + 1; k <= VI; ++k) {
+                float temp = sPr[k] * dprev + sQ[k];
+                sD[k] = temp;
+                dprev = temp;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            size_t o = c3(d, i, j);
+            float *qrow = d.prow_q + (size_t)j * nkp;
+            for (int k = KK + 1 + lane; k <= VI; k += WAVE) {
+                float temp = sD[k];
+                double qv = (double)conv_to_int((double)temp * d.frund) / d.frund;
+                float rr = (float)fabs((double)sO[k] - qv);
+                rmax = rr > rmax ? rr : rmax;
+                d.dpm[o + k] = temp;
+                d.dpmnew[o + k] = d.dpmnew[o + k] + d.relaxp * temp;
+                qrow[k] = sQ[k];
+            }
+        }
+        __syncthreads();
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        float other = __shfl_xor(rmax, off);
+        rmax = other > rmax ? other : rmax;
+    }
+    if (lane == 0 && rmax > 0.0f && slot >= 0 && slot < d.resid_cap) atomicMax(&d.resid[slot], __float_as_uint(rmax));
+}
+
+// ------------------------------------------------------------------------
 // host-side launchers
 // ------------------------------------------------------------------------
 static inline int ndiag(int oa, int ob, int ia, int ib) { return (ob - oa) + (ib - ia) + 1; }
@@ -998,9 +1280,20 @@ extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_top, dim3((ncols + 255) / 256), dim3(256), 0, s, d); });
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncols), dim3(64), 0, s, d); });
     size_t nrow = (size_t)d.nj2 * d.nkp;
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_p, nrow, 0.0f); });
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_q, nrow, 0.0f); });
-    {
+    if (d.tiled) {
+        int oa = 2, ob = d.njj - 1, ia = 2, ib = d.nii - 1;
+        int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
+        size_t shm = sizeof(float) * 5 * d.nkp * TILE;
+        const float *pt = d.ptab[IS == 1 ? 0 : 1], *tt = d.ttab[IS == 1 ? 0 : 1];
+        for (int tau = 0; tau < nta + ntb - 1; ++tau) {
+            int cnt = min(nta - 1, tau) - max(0, tau - (ntb - 1)) + 1;
+            L.go(FAM_PRESSURE, [&] {
+                hipLaunchKernelGGL(k_pressure_tile, dim3(cnt), dim3(TILE * WAVE), shm, s, d, pt, tt, tau, IS, JS, slot);
+            });
+        }
+    } else {
+        L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_p, nrow, 0.0f); });
         int oa = 2, ob = d.njj - 1, ia = 2, ib = d.nii - 1;
         size_t shm = sizeof(float) * 5 * d.nkp;
         for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t)
@@ -1017,7 +1310,22 @@ extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_
         for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t)
             L.go(fam, [&] { hipLaunchKernelGGL(kern, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, d, t, is, js); });
     };
-    if (d.model == 1) {
+    auto tiled_sweep = [&](int fam, auto coef, auto tile, int oa, int ia, int is, int js) {
+        int ob = d.nii - 1, ib = d.njj - 1;
+        int na = ob - oa + 1, nb = ib - ia + 1;
+        L.go(fam, [&] { hipLaunchKernelGGL(coef, dim3(na * nb), dim3(WAVE), sizeof(float) * 3 * d.nkp, s, d, d.planes, is, js); });
+        int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
+        size_t shm_t = sizeof(float) * 5 * d.nkp * TILE;
+        for (int tau = 0; tau < nta + ntb - 1; ++tau) {
+            int cnt = min(nta - 1, tau) - max(0, tau - (ntb - 1)) + 1;
+            L.go(fam, [&] { hipLaunchKernelGGL(tile, dim3(cnt), dim3(TILE * WAVE), shm_t, s, d, (const float *)d.planes, tau, is, js); });
+        }
+    };
+    if (d.model == 1 && d.tiled) {
+        tiled_sweep(FAM_U, k_ml_coef<0>, k_ml_tile<0>, 3, 2, IS, JS);
+        tiled_sweep(FAM_V, k_ml_coef<1>, k_ml_tile<1>, 2, 3, -IS, -JS);
+        tiled_sweep(FAM_W, k_ml_coef<2>, k_ml_tile<2>, 2, 2, IS, JS);
+    } else if (d.model == 1) {
         sweep(FAM_U, k_momentum<0, 0>, 3, 2, IS, JS);
         sweep(FAM_V, k_momentum<1, 0>, 2, 3, -IS, -JS);
         sweep(FAM_W, k_momentum<2, 0>, 2, 2, IS, JS);

@@ -6,6 +6,8 @@ anti-diagonal wavefronts with the same arithmetic, so the bar here is bit
 equality of every field; the north-star tolerance (relative L2 <= 1e-4,
 max-abs <= 1e-3 m/s) is asserted as well so a failure says how far off it is.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -16,12 +18,24 @@ REL_L2_TOL = 1e-4
 MAX_ABS_TOL = 1e-3
 
 
-def _compare(case, iterations):
+def _compare(case, iterations, sweeps=("tile", "diag")):
     from gral_amd import pf
     from oracle import pyoracle
-    print(f"[parity] case={case.name} model={case.turbulence_model} n={case.nii}x{case.njj} it={iterations}", flush=True)
     ref = pyoracle.run(case, iterations=iterations)
-    got = pf.run(case, iterations=iterations)
+    rep = None
+    for sw in sweeps:
+        print(f"[parity] case={case.name} model={case.turbulence_model} n={case.nii}x{case.njj} it={iterations} "
+              f"sweep={sw}", flush=True)
+        os.environ["GRAL_PF_SWEEP"] = sw
+        try:
+            got = pf.run(case, iterations=iterations)
+        finally:
+            os.environ.pop("GRAL_PF_SWEEP", None)
+        rep = _check(case, got, ref)
+    return rep
+
+
+def _check(case, got, ref):
     fields = FIELDS + (KE_FIELDS if case.turbulence_model != 1 else ())
     rep = diff_report(got, ref, fields)
     for f, r in rep.items():
@@ -83,5 +97,5 @@ def test_config1_200_iterations():
     # the full fixed-iteration parity run of config 1 (200 SIMPLE iterations,
     # convergence bookkeeping at 100 and 200)
     c = domain.config1(turbulence_model=1, iterations=200)
-    rep = _compare(c, 200)
+    rep = _compare(c, 200, sweeps=("tile",))
     print(rep)
