@@ -36,7 +36,21 @@ def main():
     ap.add_argument("--direction", type=float, default=270.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iterations", type=int, default=2)
+    ap.add_argument("--no-family-profile", action="store_true",
+                    help="skip the per-launch event timing (for counter-collection runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": n * 4, "calibrate_copy_bytes_written": n * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this solver runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
@@ -59,6 +73,10 @@ def main():
 
     # dominant kernel: per-launch device time of each sweep family over a few
     # instrumented iterations (events bracket single launches on the solver's stream)
+    if args.no_family_profile:
+        solver.close()
+        print(json.dumps({"metric": "SIMPLE cell-iters/s", "value": value, "ms_per_step": ms, "steps": args.steps}))
+        return
     prof = solver.profile_families(4)
     dom = max(prof, key=lambda k: prof[k]["ms"])
     pdom = prof[dom]

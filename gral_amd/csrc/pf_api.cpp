@@ -24,7 +24,8 @@ struct Probe {
     int *fam;
     int cap, n;
 };
-extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_t s, long long *launches, Probe *probe);
+extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, int slot, hipStream_t s,
+                                   long long *launches, Probe *probe);
 extern "C" int pf_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
 
 namespace {
@@ -68,8 +69,10 @@ struct gral_pf_handle {
     double geschw_max = 0;
     int unreal[3] = {0, 0, 0};
     gral_pf_timing timing{};
+    long long active = 0;           // active cells (advanced per SIMPLE iteration)
     // device
     Dom d{};
+    Dom *dd = nullptr;              // device copy of d (a one-situation launch array)
     std::vector<void *> dev;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -409,6 +412,7 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     // sweep kernels: tiled wavefront (default) or one launch per anti-diagonal
     const char *sw = std::getenv("GRAL_PF_SWEEP");
     d.tiled = !(sw && std::strcmp(sw, "diag") == 0);
+    if (h->kadvmax + 2 > 128) d.tiled = 0;   // the tiled kernels hold at most 128 levels per column
     d.planes = nullptr;
     d.ptab[0] = d.ptab[1] = d.ttab[0] = d.ttab[1] = nullptr;
     if (d.tiled) {
@@ -448,6 +452,21 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
             d.ttab[v] = dt;
         }
     }
+    // per-column metadata of the tiled kernels: KK, VI, KSTART | adv<<16 | vegp<<17,
+    // max(KK, KK[i-1]) | max(KK, KK[j-1])<<16
+    std::vector<int4> meta(n2, make_int4(0, -1, 0, 0));
+    for (int i = 1; i <= h->nii; ++i)
+        for (int j = 1; j <= h->njj; ++j) {
+            size_t c = h->i2(i, j);
+            int KK = h->kk[c];
+            int KSTART = (h->cutk[c] == 0) ? KK + 1 : 1;
+            int vegp = (h->has_veg && h->vegp[c]) ? 1 : 0;
+            int kmi = std::max(KK, (int)h->kk[h->i2(i - 1, j)]), kmj = std::max(KK, (int)h->kk[h->i2(i, j - 1)]);
+            meta[c] = make_int4(KK, h->vi[c], KSTART | ((int)h->adv[c] << 16) | (vegp << 17), kmi | (kmj << 16));
+        }
+    int4 *fmeta;
+    if ((rc = dalloc(h, &fmeta, n2)) || (rc = put_vec(h, fmeta, meta))) return rc;
+    d.meta = fmeta;
     std::vector<int> kk32(n2);
     for (size_t c = 0; c < n2; ++c) kk32[c] = h->kk[c];
     if ((rc = put_vec(h, fdzk, h->dzk)) || (rc = put_vec(h, fhok, h->hok)) || (rc = put_vec(h, fap0, ap0)) ||
@@ -479,7 +498,15 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
         HIPCHK(h, hipMemsetAsync(d.tdiss, 0, n3 * sizeof(float), h->stream));
     }
     if (pf_fill_u32(d.resid, (size_t)p->iter_max + 1, 0u, h->stream)) return fail(h, GRAL_PF_EDEVICE, "fill resid");
+    if ((rc = dalloc(h, &h->dd, 1))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->dd, &h->d, sizeof(Dom), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->active = 0;
+    for (int i = 2; i <= h->nii - 1; ++i)
+        for (int j = 2; j <= h->njj - 1; ++j) {
+            size_t c = h->i2(i, j);
+            if (h->adv[c] == 1) h->active += h->vi[c] - h->kk[c];
+        }
     h->prepared = true;
     h->done = false;
     h->host_stale = false;
@@ -529,72 +556,107 @@ static bool book_iteration(gral_pf_handle *h, int it, float dumax, bool check) {
     return false;
 }
 
+// Lock-step driver shared by single and batched solves.  Runs up to n SIMPLE
+// iterations of the situations in hs that are not finished; all of them must
+// stand at the same iteration.  Between host synchronisations it launches up
+// to kMaxChunk iterations, never past a multiple of 100 when the stop test
+// applies and never past any situation's iter_max (the size of its residual
+// history); after each chunk it books the residuals of every situation and
+// drops the finished ones (early retirement).  dd holds room for hs.size()
+// Dom entries in device memory.
+struct RunResult {
+    long long launches = 0, cell_iters = 0;
+    float ms = 0;
+    std::string err;
+    int code = 0;
+};
+
+static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                         int n, bool check, Probe *probe, RunResult &R) {
+    auto bad = [&](int code, const std::string &m) { R.code = code; R.err = m; };
+    std::vector<gral_pf_handle *> act;
+    for (auto *h : hs)
+        if (!h->done) act.push_back(h);
+    if (hipEventRecord(e0, s) != hipSuccess) return bad(GRAL_PF_EDEVICE, "hipEventRecord failed");
+    std::vector<Dom> hd;
+    std::vector<unsigned> bits;
+    bool upload = true;
+    int remaining = n;
+    while (remaining > 0 && !act.empty()) {
+        const int it0 = act[0]->iter_loops;
+        int chunk = std::min(remaining, kMaxChunk);
+        if (check) chunk = std::min(chunk, 100 - ((it0 - 1) % 100));
+        for (auto *h : act) {
+            if (h->iter_loops != it0) return bad(GRAL_PF_ESTATE, "iterate: situations of a batch must stand at the same iteration");
+            chunk = std::min(chunk, h->prm.iter_max - it0 + 1);
+        }
+        if (it0 < 1 || chunk < 1) return bad(GRAL_PF_ESTATE, "iterate: iteration counter out of range");
+        if (upload) {
+            hd.resize(act.size());
+            for (size_t q = 0; q < act.size(); ++q) hd[q] = act[q]->d;
+            if (hipMemcpyAsync(dd, hd.data(), sizeof(Dom) * hd.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+                return bad(GRAL_PF_EDEVICE, "upload of the situation table failed");
+            upload = false;
+        }
+        for (int c = 0; c < chunk; ++c) {
+            int it = it0 + c;
+            int e = pf_launch_iteration(hd.data(), (int)hd.size(), dd, it, it - 1, s, &R.launches, probe);
+            if (e)
+                return bad(GRAL_PF_EDEVICE, std::string("kernel launch failed at iteration ") + std::to_string(it) + ": " +
+                                                hipGetErrorString((hipError_t)e));
+        }
+        remaining -= chunk;
+        if (hipStreamSynchronize(s) != hipSuccess) return bad(GRAL_PF_EDEVICE, "stream synchronisation failed");
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess)
+            return bad(GRAL_PF_EDEVICE, std::string("pending HIP error after iterations ") + std::to_string(it0) + ".." +
+                                            std::to_string(it0 + chunk - 1) + ": " + hipGetErrorString(le));
+        // bookkeeping of the iterations just run, situation by situation
+        std::vector<gral_pf_handle *> keep;
+        bits.resize((size_t)chunk);
+        for (auto *h : act) {
+            if (hipMemcpy(bits.data(), h->d.resid + (it0 - 1), sizeof(unsigned) * chunk, hipMemcpyDeviceToHost) != hipSuccess)
+                return bad(GRAL_PF_EDEVICE, "residual download failed");
+            for (int c = 0; c < chunk; ++c) {
+                float du;
+                std::memcpy(&du, &bits[c], 4);
+                int it = it0 + c;
+                h->iters_run++;
+                R.cell_iters += h->active;
+                if (book_iteration(h, it, du, check)) {
+                    h->iter_loops = it;      // the reference breaks before IterationLoops++
+                    h->done = true;
+                    break;
+                }
+                h->iter_loops = it + 1;
+            }
+            if (h->iter_loops > h->prm.iter_max) h->done = true;
+            h->host_stale = true;
+            if (!h->done) keep.push_back(h);
+        }
+        if (keep.size() != act.size()) {
+            act.swap(keep);
+            upload = true;
+        }
+    }
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&R.ms, e0, e1) != hipSuccess)
+        return bad(GRAL_PF_EDEVICE, "event timing failed");
+}
+
 extern "C" int gral_pf_iterate(gral_pf_handle *h, int32_t n, int32_t check, gral_pf_stats *st) {
     if (!h) return GRAL_PF_EINVAL;
     if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "iterate: call prepare first");
     if (n < 0) return fail(h, GRAL_PF_EINVAL, "iterate: n < 0");
     h->timing = gral_pf_timing{};
-    int todo = std::min(n, h->prm.iter_max - h->iter_loops + 1);
-    if (h->done) todo = 0;
-    long long launches = 0;
-    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-    std::vector<unsigned> bits;
-    int remaining = todo;
-    while (remaining > 0 && !h->done) {
-        // launch up to kMaxChunk iterations, never past a multiple of 100 when the
-        // reference's stop test applies, and never past iter_max (the size of resid)
-        const int it0 = h->iter_loops;
-        int chunk = std::min(remaining, kMaxChunk);
-        if (check) chunk = std::min(chunk, 100 - ((it0 - 1) % 100));
-        if (it0 < 1 || it0 - 1 + chunk > h->prm.iter_max)
-            return fail(h, GRAL_PF_ESTATE, "iterate: iteration counter out of range");
-        for (int c = 0; c < chunk; ++c) {
-            int it = it0 + c;
-            int e = pf_launch_iteration(&h->d, it, it - 1, h->stream, &launches, nullptr);
-            if (e) return fail(h, GRAL_PF_EDEVICE, std::string("kernel launch failed at iteration ") + std::to_string(it) +
-                                                       ": " + hipGetErrorString((hipError_t)e));
-        }
-        remaining -= chunk;
-        // bookkeeping of the iterations just launched
-        HIPCHK(h, hipStreamSynchronize(h->stream));
-        {
-            hipError_t le = hipGetLastError();
-            if (le != hipSuccess)
-                return fail(h, GRAL_PF_EDEVICE, std::string("pending HIP error after iterations ") + std::to_string(it0) +
-                                                    ".." + std::to_string(it0 + chunk - 1) + ": " + hipGetErrorString(le));
-        }
-        bits.resize((size_t)chunk);
-        HIPCHK(h, hipMemcpyAsync(bits.data(), h->d.resid + (it0 - 1), sizeof(unsigned) * chunk, hipMemcpyDeviceToHost,
-                                 h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
-        for (int c = 0; c < chunk; ++c) {
-            float du;
-            std::memcpy(&du, &bits[c], 4);
-            int it = it0 + c;
-            h->iters_run++;
-            if (book_iteration(h, it, du, check != 0)) {
-                h->iter_loops = it;      // the reference breaks before IterationLoops++
-                h->done = true;
-                break;
-            }
-            h->iter_loops = it + 1;
-        }
-    }
-    if (h->iter_loops > h->prm.iter_max) h->done = true;
-    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
-    HIPCHK(h, hipEventSynchronize(h->ev1));
-    float ms = 0;
-    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    h->timing.ms_total = ms;
-    h->timing.launches = launches;
-    long long act = 0;
-    for (int i = 2; i <= h->nii - 1; ++i)
-        for (int j = 2; j <= h->njj - 1; ++j) {
-            size_t c = h->i2(i, j);
-            if (h->adv[c] == 1) act += h->vi[c] - h->kk[c];
-        }
-    h->timing.active_cells = act;
-    h->host_stale = true;
+    RunResult R;
+    run_lockstep({h}, h->dd, h->stream, h->ev0, h->ev1, n, check != 0, nullptr, R);
+    if (R.code) return fail(h, R.code, R.err);
+    h->timing.ms_total = R.ms;
+    h->timing.launches = R.launches;
+    h->timing.active_cells = h->active;
+    h->timing.cell_iterations = R.cell_iters;
+    h->timing.situations = 1;
     fill_stats(h, st);
     return GRAL_PF_OK;
 }
@@ -672,51 +734,156 @@ extern "C" int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, 
     return GRAL_PF_OK;
 }
 
-extern "C" int gral_pf_profile_iterations(gral_pf_handle *h, int32_t n, gral_pf_family_time *out, int32_t nfam) {
-    if (!h || !out || nfam < 1) return GRAL_PF_EINVAL;
-    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "profile: call prepare first");
+// Runs n iterations one at a time with HIP events around every launch and
+// sums the device time per kernel family.
+static void profile_group(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                          int n, gral_pf_family_time *out, int nfam, RunResult &R) {
     for (int f = 0; f < nfam; ++f) out[f] = gral_pf_family_time{};
-    int todo = std::min(n, h->prm.iter_max - h->iter_loops + 1);
-    if (h->done) todo = 0;
-    const int cap = 8 * (h->nii + h->njj) + 64;
-    std::vector<hipEvent_t> ev(2 * (size_t)cap);
+    int nmax = 0;
+    for (auto *h : hs) nmax = std::max(nmax, h->nii + h->njj);
+    const int cap = 8 * nmax + 64;
+    std::vector<hipEvent_t> ev(2 * (size_t)cap, nullptr);
     std::vector<int> fam(cap);
-    for (auto &e : ev) HIPCHK(h, hipEventCreate(&e));
-    int rc = 0;
-    long long act = 0;
-    for (int i = 2; i <= h->nii - 1; ++i)
-        for (int j = 2; j <= h->njj - 1; ++j) {
-            size_t c = h->i2(i, j);
-            if (h->adv[c] == 1) act += h->vi[c] - h->kk[c];
-        }
-    std::vector<unsigned> bits(1);
-    for (int r = 0; r < todo && !rc; ++r) {
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { R.code = GRAL_PF_EDEVICE; R.err = "hipEventCreate failed"; break; }
+    for (int r = 0; r < n && !R.code; ++r) {
         Probe pr{ev.data(), fam.data(), cap, 0};
-        long long launches = 0;
-        int it = h->iter_loops;
-        if (it < 1 || it > h->prm.iter_max) { rc = fail(h, GRAL_PF_ESTATE, "profile: iteration counter out of range"); break; }
-        int e = pf_launch_iteration(&h->d, it, it - 1, h->stream, &launches, &pr);
-        if (e) { rc = fail(h, GRAL_PF_EDEVICE, "profile: launch failed"); break; }
-        if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = fail(h, GRAL_PF_EDEVICE, "profile: sync failed"); break; }
+        RunResult one;
+        run_lockstep(hs, dd, s, e0, e1, 1, false, &pr, one);
+        if (one.code) { R.code = one.code; R.err = one.err; break; }
         for (int q = 0; q < pr.n; ++q) {
             float ms = 0;
             (void)hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]);
             if (fam[q] < nfam) { out[fam[q]].ms += ms; out[fam[q]].launches += 1; }
         }
-        for (int f = 0; f < nfam; ++f) out[f].cells += act;
-        (void)hipMemcpyAsync(bits.data(), h->d.resid + (it - 1), sizeof(unsigned), hipMemcpyDeviceToHost, h->stream);
-        (void)hipStreamSynchronize(h->stream);
-        float du;
-        std::memcpy(&du, &bits[0], 4);
-        h->iters_run++;
-        bool stop = book_iteration(h, it, du, false);
-        (void)stop;
-        h->iter_loops = it + 1;
+        for (int f = 0; f < nfam; ++f) out[f].cells += one.cell_iters;
+        R.launches += one.launches;
+        R.cell_iters += one.cell_iters;
+        R.ms += one.ms;
     }
-    if (h->iter_loops > h->prm.iter_max) h->done = true;
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    h->host_stale = true;
-    return rc;
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+}
+
+extern "C" int gral_pf_profile_iterations(gral_pf_handle *h, int32_t n, gral_pf_family_time *out, int32_t nfam) {
+    if (!h || !out || nfam < 1 || n < 0) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "profile: call prepare first");
+    RunResult R;
+    profile_group({h}, h->dd, h->stream, h->ev0, h->ev1, n, out, nfam, R);
+    if (R.code) return fail(h, R.code, R.err);
+    return GRAL_PF_OK;
+}
+
+// ------------------------------------------------------------------------
+// Batches: several prepared situations advanced in lock step, every launch
+// covering all of them (grid slice z = situation).
+// ------------------------------------------------------------------------
+struct gral_pf_batch {
+    std::vector<gral_pf_handle *> hs;
+    Dom *dd = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    gral_pf_timing timing{};
+    std::string err;
+};
+
+static int bfail(gral_pf_batch *b, int code, const std::string &msg) {
+    if (b) b->err = msg;
+    return code;
+}
+
+extern "C" int gral_pf_batch_create(gral_pf_handle *const *handles, int32_t n, gral_pf_batch **out) {
+    if (!out) return GRAL_PF_EINVAL;
+    *out = nullptr;
+    if (!handles || n < 1) return GRAL_PF_EINVAL;
+    for (int q = 0; q < n; ++q) {
+        if (!handles[q] || !handles[q]->prepared) return GRAL_PF_ESTATE;
+        for (int r = 0; r < q; ++r)
+            if (handles[r] == handles[q]) return GRAL_PF_EINVAL;
+        const Dom &a = handles[0]->d, &c = handles[q]->d;
+        if (a.model != c.model || a.tiled != c.tiled) return GRAL_PF_EINVAL;
+    }
+    gral_pf_batch *b = new (std::nothrow) gral_pf_batch();
+    if (!b) return GRAL_PF_ENOMEM;
+    b->hs.assign(handles, handles + n);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+        hipEventCreate(&b->ev1) != hipSuccess || hipMalloc((void **)&b->dd, sizeof(Dom) * (size_t)n) != hipSuccess) {
+        gral_pf_batch_destroy(b);
+        return GRAL_PF_EDEVICE;
+    }
+    *out = b;
+    return GRAL_PF_OK;
+}
+
+static int batch_ready(gral_pf_batch *b) {
+    for (auto *h : b->hs)
+        if (!h->prepared) return bfail(b, GRAL_PF_ESTATE, "batch: a situation was not prepared");
+    return 0;
+}
+
+static void batch_stats(gral_pf_batch *b, gral_pf_stats *st) {
+    if (!st) return;
+    for (size_t q = 0; q < b->hs.size(); ++q) fill_stats(b->hs[q], st + q);
+}
+
+extern "C" int gral_pf_batch_iterate(gral_pf_batch *b, int32_t n, int32_t check, gral_pf_stats *st) {
+    if (!b) return GRAL_PF_EINVAL;
+    if (n < 0) return bfail(b, GRAL_PF_EINVAL, "batch iterate: n < 0");
+    int rc = batch_ready(b);
+    if (rc) return rc;
+    b->timing = gral_pf_timing{};
+    RunResult R;
+    run_lockstep(b->hs, b->dd, b->stream, b->ev0, b->ev1, n, check != 0, nullptr, R);
+    if (R.code) return bfail(b, R.code, R.err);
+    b->timing.ms_total = R.ms;
+    b->timing.launches = R.launches;
+    for (auto *h : b->hs) b->timing.active_cells += h->active;
+    b->timing.cell_iterations = R.cell_iters;
+    b->timing.situations = (int32_t)b->hs.size();
+    batch_stats(b, st);
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_batch_solve(gral_pf_batch *b, gral_pf_stats *st) {
+    if (!b) return GRAL_PF_EINVAL;
+    int rc = batch_ready(b);
+    if (rc) return rc;
+    int imax = 0;
+    for (auto *h : b->hs) imax = std::max(imax, h->prm.iter_max);
+    if ((rc = gral_pf_batch_iterate(b, imax, 1, nullptr))) return rc;
+    for (size_t q = 0; q < b->hs.size(); ++q)
+        if ((rc = gral_pf_finish(b->hs[q], st ? st + q : nullptr))) return bfail(b, rc, b->hs[q]->err);
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_batch_profile_iterations(gral_pf_batch *b, int32_t n, gral_pf_family_time *out, int32_t nfam) {
+    if (!b || !out || nfam < 1 || n < 0) return GRAL_PF_EINVAL;
+    int rc = batch_ready(b);
+    if (rc) return rc;
+    RunResult R;
+    profile_group(b->hs, b->dd, b->stream, b->ev0, b->ev1, n, out, nfam, R);
+    if (R.code) return bfail(b, R.code, R.err);
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_batch_last_timing(gral_pf_batch *b, gral_pf_timing *t) {
+    if (!b || !t) return GRAL_PF_EINVAL;
+    *t = b->timing;
+    return GRAL_PF_OK;
+}
+
+extern "C" const char *gral_pf_batch_last_error(const gral_pf_batch *b) {
+    if (!b) return "null batch";
+    return b->err.c_str();
+}
+
+extern "C" void gral_pf_batch_destroy(gral_pf_batch *b) {
+    if (!b) return;
+    if (b->dd) (void)hipFree(b->dd);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
 }
 
 extern "C" int gral_pf_last_timing(gral_pf_handle *h, gral_pf_timing *t) {

@@ -37,6 +37,7 @@ struct Dom {
     int tiled;                       // 1: tiled wavefront kernels, 0: one launch per anti-diagonal
     float *planes;                   // [16][nii+2][njj+2][nkp] mixing-length coefficient planes
     const float *ptab[2], *ttab[2];  // pressure PIMP/TERMP per column for IS=+1 / IS=-1
+    const int4 *meta;                // per column: KK, VI, KSTART | adv<<16 | vegp<<17, KKm_i | KKm_j<<16
 };
 
 __device__ __forceinline__ size_t c2(const Dom &d, int i, int j) { return (size_t)i * (size_t)d.nj2 + (size_t)j; }

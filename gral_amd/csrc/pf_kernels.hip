@@ -39,9 +39,16 @@ __global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) p[g] = v;
 }
-__global__ void k_fill_f32(float *p, size_t n, float v) {
+// zero the pressure row scratch of every situation (PIMP/QIMP rows are fresh
+// arrays at the start of each pressure pass)
+__global__ void k_zero_rows(const Dom *__restrict__ ds, int both) {
+    const Dom &d = ds[blockIdx.z];
+    size_t n = (size_t)d.nj2 * d.nkp;
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < n) p[g] = v;
+    if (g < n) {
+        d.prow_q[g] = 0.0f;
+        if (both) d.prow_p[g] = 0.0f;
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -49,7 +56,10 @@ __global__ void k_fill_f32(float *p, size_t n, float v) {
 // ------------------------------------------------------------------------
 // E/W: each row j is independent; within a row the reference scans i upwards
 // and later i read what earlier i wrote, so one thread per (j,k) scans i.
-__global__ void k_bc_ew(Dom d) {
+__global__ void k_bc_ew(const Dom *__restrict__ ds) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    if ((int)blockIdx.y >= d.njj - 1) return;
+
     int j = blockIdx.y + 2;                   // 2..NJJ
     int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (k > d.kadvmax) return;
@@ -74,7 +84,10 @@ __global__ void k_bc_ew(Dom d) {
     }
 }
 
-__global__ void k_bc_ns(Dom d) {
+__global__ void k_bc_ns(const Dom *__restrict__ ds) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    if ((int)blockIdx.y >= d.nii - 1) return;
+
     int i = blockIdx.y + 2;                   // 2..NII
     int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (k > d.kadvmax) return;
@@ -100,7 +113,9 @@ __global__ void k_bc_ns(Dom d) {
 }
 
 // top copy VI -> VI+1 (thread per column i in [2,NII-1], j in [2,NJJ-1])
-__global__ void k_bc_top(Dom d) {
+__global__ void k_bc_top(const Dom *__restrict__ ds) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     int ni = d.nii - 2, nj = d.njj - 2;
     if (g >= ni * nj) return;
@@ -117,7 +132,10 @@ __global__ void k_bc_top(Dom d) {
 // ------------------------------------------------------------------------
 // Mass divergence, PrognosticFlowfield.cs:723-758.  Block = one column.
 // ------------------------------------------------------------------------
-__global__ void k_divergence(Dom d) {
+__global__ void k_divergence(const Dom *__restrict__ ds) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    if ((int)blockIdx.x >= (d.nii - 2) * (d.njj - 2)) return;
+
     int nj = d.njj - 2;
     int i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
     size_t c = c2(d, i, j);
@@ -143,7 +161,9 @@ __global__ void k_divergence(Dom d) {
 // pass, so the top coupling PIMP[VI+1] sees what the last earlier column of
 // the same row left there, as in the reference.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(WAVE) void k_pressure(Dom d, int t, int IS, int JS, int slot) {
+__global__ __launch_bounds__(WAVE) void k_pressure(const Dom *__restrict__ ds, int t, int IS, int JS, int slot) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+
     extern __shared__ float sm[];
     int a, b;
     if (!diag_col(t, 2, d.njj - 1, 2, d.nii - 1, blockIdx.x, a, b)) return;
@@ -204,7 +224,10 @@ __global__ __launch_bounds__(WAVE) void k_pressure(Dom d, int t, int IS, int JS,
 // ------------------------------------------------------------------------
 // Velocity correction with FP64 quantisation, PrognosticFlowfield.cs:904-949
 // ------------------------------------------------------------------------
-__global__ void k_correction(Dom d) {
+__global__ void k_correction(const Dom *__restrict__ ds) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    if ((int)blockIdx.x >= (d.nii - 2) * (d.njj - 2)) return;
+
     int nj = d.njj - 2;
     int i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
     size_t c = c2(d, i, j);
@@ -233,7 +256,10 @@ __global__ void k_correction(Dom d) {
 // ------------------------------------------------------------------------
 // Cell-centred averages, PrognosticFlowfield.cs:958-1014 (+ UKS row copies)
 // ------------------------------------------------------------------------
-__global__ void k_averages(Dom d) {
+__global__ void k_averages(const Dom *__restrict__ ds) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    if ((int)blockIdx.x >= (d.nii - 2) * (d.njj - 2)) return;
+
     int nj = d.njj - 2;
     int i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
     size_t c = c2(d, i, j);
@@ -256,7 +282,10 @@ __global__ void k_averages(Dom d) {
 }
 
 // VKS[1][j] = VKS[2][j], VKS[NII][j] = VKS[NII-1][j] (PrognosticFlowfield.cs:1019-1039)
-__global__ void k_avg_edges(Dom d) {
+__global__ void k_avg_edges(const Dom *__restrict__ ds) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    if ((int)blockIdx.x >= d.njj - 2) return;
+
     int j = blockIdx.x + 2;                  // 2..NJJ-1
     bool w = d.adv[c2(d, 1, j)] == 1, e = d.adv[c2(d, d.nii - 1, j)] == 1;
     size_t o1 = c3(d, 1, j), o2 = c3(d, 2, j), on = c3(d, d.nii, j), on1 = c3(d, d.nii - 1, j);
@@ -274,7 +303,12 @@ __global__ void k_avg_edges(Dom d) {
 // quantity is lane-local except the gate of the eddy-viscosity cross terms,
 // which tests the block start (k_block > KKART+1); block_base() returns the
 // start of the block whose lane k enters the recurrence.
+// Vector<float>.Count of the reference's default (AVX2) build; the only width
+// the host accepts
+#define SIMDW 8
 __device__ __forceinline__ int block_base(int k, int KSTART, int KEND, int W) {
+    (void)W;
+    W = SIMDW;
     int base = KSTART + ((k - KSTART) / W) * W;
     if (base > KEND - W) base = KEND - W;
     return base;
@@ -518,14 +552,13 @@ __device__ MlPieces w1_pieces(const Dom &d, const ColInfo &q, int k) {
 // D of one cell from its pieces and the four neighbour values (and the cell's
 // own old value x), in the reference's order of operations.
 template <int COMP>
-__device__ __forceinline__ float ml_source(const Dom &d, const ColInfo &q, int k, const MlPieces &o, float xw, float xs,
-                                           float xe, float xn, float x) {
+__device__ __forceinline__ float ml_source(const Dom &d, const ColInfo &q, int k, float dz, const MlPieces &o, float xw,
+                                           float xs, float xe, float xn, float x) {
     float D = o.AW1 * xw + o.AS1 * xs + o.AE1 * xe + o.AN1 * xn + o.e + o.f;
     if (COMP != 2) {
         D = D + o.g;
         if (k == q.KK + 1) D -= o.w;
         if (ml_cross_on(q, k)) {
-            float dz = d.dzk[k];
             if (COMP == 0) {
                 float dykdzk = d.dyk * dz;
                 float DUDXE = (xe - x) * dykdzk;
@@ -569,7 +602,7 @@ __device__ Coef ml_cell(const Dom &d, const ColInfo &q, int k) {
     MlPieces o = ml_pieces<COMP>(d, q, k);
     float xw, xs, xe, xn, x;
     ml_neighbours<COMP>(d, q.i, q.j, k, xw, xs, xe, xn, x);
-    return Coef{o.A, o.B, o.C, ml_source<COMP>(d, q, k, o, xw, xs, xe, xn, x)};
+    return Coef{o.A, o.B, o.C, ml_source<COMP>(d, q, k, d.dzk[k], o, xw, xs, xe, xn, x)};
 }
 __device__ Coef u1_cell(const Dom &d, const ColInfo &q, int k) { return ml_cell<0>(d, q, k); }
 __device__ Coef v1_cell(const Dom &d, const ColInfo &q, int k) { return ml_cell<1>(d, q, k); }
@@ -772,7 +805,9 @@ __device__ Coef ke_w_cell(const Dom &d, const ColInfo &q, int k) {
 //   1: k-eps, 2: no diffusion (scalar path, division form)
 // ------------------------------------------------------------------------
 template <int COMP, int KIND>
-__global__ __launch_bounds__(WAVE) void k_momentum(Dom d, int t, int IS, int JS) {
+__global__ __launch_bounds__(WAVE) void k_momentum(const Dom *__restrict__ ds, int t, int IS, int JS) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+
     extern __shared__ float sm[];
     const int nkp = d.nkp, tid = threadIdx.x;
     // sweep ranges (outer i1, inner j1): U i1 from 3; V j1 from 3
@@ -833,7 +868,9 @@ __global__ __launch_bounds__(WAVE) void k_momentum(Dom d, int t, int IS, int JS)
 // ------------------------------------------------------------------------
 // TKE / dissipation (k-eps), TKE-prognostic-microscale.cs:25-352
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(WAVE) void k_tke(Dom d, int t, int IS, int JS) {
+__global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t, int IS, int JS) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+
     extern __shared__ float sm[];
     const int nkp = d.nkp, tid = threadIdx.x;
     int a, b;
@@ -1020,6 +1057,7 @@ __global__ __launch_bounds__(WAVE) void k_tke(Dom d, int t, int IS, int JS) {
 #define TILE 16
 
 // coefficient planes written by k_ml_coef, read by k_ml_tile
+// array of structures: 16 floats per cell, in the order of the four float4 loads
 enum { PL_AW = 0, PL_AS, PL_AE, PL_AN, PL_C, PL_T, PL_P, PL_E, PL_F, PL_G, PL_W, PL_C0, PL_X1, PL_X2, PL_X3, PL_X4, PL_N };
 
 __device__ __forceinline__ void sweep_to_ij(const Dom &d, int comp_is_pressure, int a, int b, int IS, int JS, int &i, int &j) {
@@ -1032,43 +1070,85 @@ __device__ __forceinline__ void sweep_to_ij(const Dom &d, int comp_is_pressure, 
     }
 }
 
-// All neighbour-independent work of one mixing-length sweep, for every column
-// of the sweep at once: coefficients, source pieces, and the P/TERMP half of
-// the Thomas recurrence (it depends only on A, B, C).
+// workgroup barrier that orders LDS only; global loads stay in flight across it
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// odd LDS row stride: lanes of one wave that walk different columns at the
+// same relative level hit different banks
+__device__ __forceinline__ int lds_stride(int nkp) { return nkp | 1; }
+
+#define COEF_COLS 32
+// All neighbour-independent work of one mixing-length sweep for every column
+// at once.  A block of 4 waves takes COEF_COLS consecutive columns: the waves
+// compute the coefficients of their columns (lanes over k), then one wave runs
+// the P/TERMP recurrence of all COEF_COLS columns at once (lane = column), and
+// the waves write T and P back along k.
 template <int COMP>
-__global__ __launch_bounds__(WAVE) void k_ml_coef(Dom d, float *planes, int IS, int JS) {
+__global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds, int IS, int JS) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    float *planes = d.planes;
+
     extern __shared__ float sm[];
-    const int nkp = d.nkp, tid = threadIdx.x;
+    const int nkp = d.nkp, nkl = lds_stride(nkp);
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2;
     const int na = d.nii - 1 - oa + 1, nb = d.njj - 1 - ia + 1;
-    int col = blockIdx.x;
-    if (col >= na * nb) return;
-    int a = oa + col / nb, b = ia + col % nb;
-    int i, j;
-    sweep_to_ij(d, 0, a, b, IS, JS, i, j);
-    if (d.adv[c2(d, i, j)] != 1) return;
-    ColInfo q = col_info(d, i, j);
-    float *sA = sm, *sB = sA + nkp, *sC = sB + nkp;
     const size_t n3 = (size_t)(d.nii + 2) * d.nj2 * nkp;
-    const size_t o = c3(d, i, j);
-    for (int k = q.KSTART + tid; k <= q.VI; k += WAVE) {
-        MlPieces p = ml_pieces<COMP>(d, q, k);
-        sA[k] = p.A; sB[k] = p.B; sC[k] = p.C;
-        float *P = planes + o + k;
-        P[PL_AW * n3] = p.AW1; P[PL_AS * n3] = p.AS1; P[PL_AE * n3] = p.AE1; P[PL_AN * n3] = p.AN1;
-        P[PL_C * n3] = p.C; P[PL_E * n3] = p.e; P[PL_F * n3] = p.f; P[PL_G * n3] = p.g; P[PL_W * n3] = p.w;
-        P[PL_C0 * n3] = p.c0; P[PL_X1 * n3] = p.X1; P[PL_X2 * n3] = p.X2; P[PL_X3 * n3] = p.X3; P[PL_X4 * n3] = p.X4;
+    int *sinfo = (int *)(sm + 3 * COEF_COLS * nkl);       // per column: active, KSTART, KK, VI
+    const int col0 = blockIdx.x * COEF_COLS;
+    for (int c = wave; c < COEF_COLS; c += 4) {
+        float *sA = sm + (size_t)c * 3 * nkl, *sB = sA + nkl, *sC = sB + nkl;
+        int col = col0 + c;
+        bool act = col < na * nb;
+        int i = 0, j = 0;
+        if (act) {
+            sweep_to_ij(d, 0, oa + col / nb, ia + col % nb, IS, JS, i, j);
+            act = d.adv[c2(d, i, j)] == 1;
+        }
+        if (lane == 0) sinfo[c * 4] = act;
+        if (!act) continue;
+        ColInfo q = col_info(d, i, j);
+        if (lane == 0) { sinfo[c * 4 + 1] = q.KSTART; sinfo[c * 4 + 2] = q.KK; sinfo[c * 4 + 3] = q.VI; }
+        const size_t o = c3(d, i, j);
+        for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) {
+            MlPieces p = ml_pieces<COMP>(d, q, k);
+            sA[k] = p.A; sB[k] = p.B; sC[k] = p.C;
+            float4 *P = (float4 *)(planes + (o + k) * PL_N);
+            P[0] = make_float4(p.AW1, p.AS1, p.AE1, p.AN1);
+            P[1] = make_float4(p.C, 0.0f, 0.0f, p.e);          // T and P are filled after the recurrence
+            P[2] = make_float4(p.f, p.g, p.w, p.c0);
+            P[3] = make_float4(p.X1, p.X2, p.X3, p.X4);
+        }
     }
-    __syncthreads();
-    if (tid == 0) {
-        const int KKP1 = q.KK + 1;
+    lds_barrier();
+    if (wave == 0 && lane < COEF_COLS && sinfo[lane * 4]) {
+        float *__restrict__ A = sm + (size_t)lane * 3 * nkl;
+        float *__restrict__ Bq = A + nkl;
+        float *__restrict__ Cq = Bq + nkl;
+        const int KSTART = sinfo[lane * 4 + 1], KKP1 = sinfo[lane * 4 + 2] + 1, VI = sinfo[lane * 4 + 3];
         float Pp = 0;
-        for (int k = q.KSTART; k <= q.VI; ++k) {
-            float T = (k > KKP1) ? 1 / (sA[k] - sC[k] * Pp) : 1 / sA[k];
-            float P = sB[k] * T;
-            planes[PL_T * n3 + o + k] = T;
-            planes[PL_P * n3 + o + k] = P;
+        for (int k = KSTART; k <= VI; ++k) {
+            float T = (k > KKP1) ? 1 / (A[k] - Cq[k] * Pp) : 1 / A[k];
+            float P = Bq[k] * T;
+            A[k] = T;          // T replaces A, P replaces B
+            Bq[k] = P;
             Pp = P;
+        }
+    }
+    lds_barrier();
+    for (int c = wave; c < COEF_COLS; c += 4) {
+        if (!sinfo[c * 4]) continue;
+        int col = col0 + c, i, j;
+        sweep_to_ij(d, 0, oa + col / nb, ia + col % nb, IS, JS, i, j);
+        const float *sA = sm + (size_t)c * 3 * nkl, *sB = sA + nkl;
+        const size_t o = c3(d, i, j);
+        for (int k = sinfo[c * 4 + 1] + lane; k <= sinfo[c * 4 + 3]; k += WAVE) {
+            planes[(o + k) * PL_N + PL_T] = sA[k];
+            planes[(o + k) * PL_N + PL_P] = sB[k];
         }
     }
 }
@@ -1080,74 +1160,298 @@ __device__ __forceinline__ bool tile_coords(int tau, int nta, int ntb, int &A, i
     return A < nta && B >= 0;
 }
 
+// Serial Thomas recurrences of one column (one lane), software-pipelined: the
+// operands of the next CH levels are read from LDS while the current CH levels
+// are computed.  Pointers are relative to the first level of the recurrence and
+// the loops run over n levels; the LDS slots carry 2*CH floats of padding on
+// both sides, so the look-ahead reads need no clamping (their values are unused).
+#define CH 4
+#define SLOT_PAD (2 * CH)
+__device__ __forceinline__ int lds_slot(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
+
+// forward: Q[t] = (D[t] + C[t]*Q[t-1]) * T[t], t = 0..n-1, Q[-1] = q0
+__device__ __forceinline__ void chain_fwd(const float *__restrict__ D, const float *__restrict__ C,
+                                          const float *__restrict__ T, float *__restrict__ Q, int n, float q0) {
+    float Qp = q0;
+    float dv[CH], cv[CH], tv[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) { dv[u] = D[u]; cv[u] = C[u]; tv[u] = T[u]; }
+    for (int t = 0; t < n; t += CH) {
+        float dn[CH], cn[CH], tn[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { dn[u] = D[t + CH + u]; cn[u] = C[t + CH + u]; tn[u] = T[t + CH + u]; }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            if (t + u < n) {
+                float q = (dv[u] + cv[u] * Qp) * tv[u];
+                Q[t + u] = q;
+                Qp = q;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { dv[u] = dn[u]; cv[u] = cn[u]; tv[u] = tn[u]; }
+    }
+}
+
+// relaxed back-substitution downwards: X[-t] += relax*(P[-t]*X[-t+1] + Q[-t] - X[-t]), t = 0..n-1
+__device__ __forceinline__ void chain_back(const float *__restrict__ P, const float *__restrict__ Q,
+                                           float *__restrict__ X, float relax, int n) {
+    float up = X[1];
+    float pv[CH], qv[CH], xv[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) { pv[u] = P[-u]; qv[u] = Q[-u]; xv[u] = X[-u]; }
+    for (int t = 0; t < n; t += CH) {
+        float pn[CH], qn[CH], xn[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { pn[u] = P[-(t + CH + u)]; qn[u] = Q[-(t + CH + u)]; xn[u] = X[-(t + CH + u)]; }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            if (t + u < n) {
+                float x = xv[u];
+                x = x + relax * (pv[u] * up + qv[u] - x);
+                X[-(t + u)] = x;
+                up = x;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { pv[u] = pn[u]; qv[u] = qn[u]; xv[u] = xn[u]; }
+    }
+}
+
+// pressure, downwards from the top: Q[-t] = (A[-t]*Q[-t+1] + D[-t]) * T[-t] with
+// A (= ATP) held in Q on entry and Q[1] = qtop, t = 0..n-1
+__device__ __forceinline__ void chain_pdown(const float *__restrict__ D, const float *__restrict__ T, float *__restrict__ Q,
+                                            float qtop, int n) {
+    float Qn = qtop;
+    float av[CH], dv[CH], tv[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) { av[u] = Q[-u]; dv[u] = D[-u]; tv[u] = T[-u]; }
+    for (int t = 0; t < n; t += CH) {
+        float an[CH], dn[CH], tn[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { an[u] = Q[-(t + CH + u)]; dn[u] = D[-(t + CH + u)]; tn[u] = T[-(t + CH + u)]; }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            if (t + u < n) {
+                float q = (av[u] * Qn + dv[u]) * tv[u];
+                Q[-(t + u)] = q;
+                Qn = q;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { av[u] = an[u]; dv[u] = dn[u]; tv[u] = tn[u]; }
+    }
+}
+
+// pressure, upwards: N[t] = P[t]*N[t-1] + Q[t], t = 0..n-1, N[-1] = bottom
+__device__ __forceinline__ void chain_pup(const float *__restrict__ P, const float *__restrict__ Q, float *__restrict__ N,
+                                          float bottom, int n) {
+    float dp = bottom;
+    float pv[CH], qv[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) { pv[u] = P[u]; qv[u] = Q[u]; }
+    for (int t = 0; t < n; t += CH) {
+        float pn[CH], qn[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { pn[u] = P[t + CH + u]; qn[u] = Q[t + CH + u]; }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            if (t + u < n) {
+                float x = pv[u] * dp + qv[u];
+                N[t + u] = x;
+                dp = x;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { pv[u] = pn[u]; qv[u] = qn[u]; }
+    }
+}
+
+// per-level values of one column that do not change during the sweep, loaded
+// one local step ahead
+struct MlPre {
+    float aw, as, ae, an, e, f, g, w, c0, x1, x2, x3, x4, c, t, p;
+    float x, xa, xb;      // own old value; old neighbours in the +IS (xa) and +JS (xb) directions
+    float ni, nj;         // -IS / -JS neighbours, prefetched only when they lie in an earlier tile
+};
+
 template <int COMP>
-__global__ __launch_bounds__(TILE * WAVE) void k_ml_tile(Dom d, const float *planes, int tau, int IS, int JS) {
+__device__ __forceinline__ void ml_prefetch(const Dom &d, const float *planes, size_t n3, int i, int j, int k, int IS, int JS,
+                                            MlPre &r) {
+    (void)n3;
+    const float4 *P = (const float4 *)(planes + (c3(d, i, j) + k) * PL_N);
+    float4 v0 = P[0], v1 = P[1], v2 = P[2], v3 = P[3];
+    r.aw = v0.x; r.as = v0.y; r.ae = v0.z; r.an = v0.w;
+    r.c = v1.x; r.t = v1.y; r.p = v1.z; r.e = v1.w;
+    r.f = v2.x; r.g = v2.y; r.w = v2.z; r.c0 = v2.w;
+    r.x1 = v3.x; r.x2 = v3.y; r.x3 = v3.z; r.x4 = v3.w;
+    const float *X = COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk;
+    r.x = X[c3(d, i, j) + k];
+    r.xa = X[c3(d, i + IS, j) + k];
+    r.xb = X[c3(d, i, j + JS) + k];
+}
+
+// column metadata from the per-column table (see gral_pf_prepare)
+struct Meta {
+    int KK, VI, KSTART, KKmi, KKmj;
+    bool adv, vegp;
+};
+__device__ __forceinline__ Meta unpack_meta(int4 m) {
+    Meta r;
+    r.KK = m.x; r.VI = m.y; r.KSTART = m.z & 0xffff; r.adv = (m.z >> 16) & 1; r.vegp = (m.z >> 17) & 1;
+    r.KKmi = m.w & 0xffff; r.KKmj = (m.w >> 16) & 0xffff;
+    return r;
+}
+__device__ __forceinline__ ColInfo meta_col(const Meta &m, int i, int j, int W) {
+    ColInfo q;
+    q.i = i; q.j = j; q.KK = m.KK; q.KSTART = m.KSTART; q.VI = m.VI; q.W = W; q.vegp = m.vegp;
+    q.as = q.an = q.aw = q.ae = false; q.cutk = q.cov = q.ht = q.ho = q.z0s = 0.0f;
+    return q;
+}
+
+// Mixing-length sweep on tiles.  Per local step:
+//  (1) wave r assembles the source terms D of its column (lanes over k) from
+//      values prefetched one step earlier and from the two neighbours updated
+//      earlier in this sweep, which inside the tile come from the previous
+//      step's column buffers in LDS; it stores D, C, T, P, the reset rows of Q
+//      and the whole old column in LDS, then prefetches its next column;
+//  (2) wave 0 runs the Q recurrence and the relaxed back-substitution of all
+//      the tile's active columns together, lane r = column of wave r;
+//  (3) wave r writes its column back to global memory (fire and forget: no
+//      later read in this launch depends on those stores).
+// Column buffers are double buffered by step parity; all barriers order LDS
+// only, and no global load inside the step loop waits on another one.
+template <int COMP>
+__global__ __launch_bounds__(TILE * WAVE) void k_ml_tile(const Dom *__restrict__ ds, int tau, int IS, int JS) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const float *planes = d.planes;
+
     extern __shared__ float sm[];
-    const int nkp = d.nkp;
+    const int nkp = d.nkp, nkl = lds_slot(nkp);
     const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2, ob = d.nii - 1, ib = d.njj - 1;
     const int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
     int A, B;
     if (!tile_coords(tau, nta, ntb, A, B)) return;     // uniform over the workgroup
     const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int a0 = oa + A * TILE, b = ia + B * TILE + r;
-    float *sD = sm + (size_t)r * 5 * nkp, *sQ = sD + nkp, *sX = sQ + nkp, *sT = sX + nkp, *sPr = sT + nkp;
+    // every per-row array is a padded slot: element k at slot + SLOT_PAD + k
+    float *sD = sm + (size_t)r * 5 * nkl + SLOT_PAD, *sC = sD + nkl, *sT = sC + nkl, *sP = sT + nkl, *sQ = sP + nkl;
+    float *xbuf = sm + (size_t)TILE * 5 * nkl + SLOT_PAD;      // [2][TILE] slots of whole columns
+    float *sdzk = sm + (size_t)TILE * 7 * nkl;                // [nkp]
+    int4 *smeta = (int4 *)(((uintptr_t)(sdzk + nkp) + 15) & ~(uintptr_t)15);   // [TILE rows][TILE columns]
+    int *sinfo = (int *)(smeta + TILE * TILE);                // per row: active, KSTART, KK, VI, KKm
     const size_t n3 = (size_t)(d.nii + 2) * d.nj2 * nkp;
     float *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk);
-    for (int s = 0; s < 2 * TILE - 1; ++s) {
-        int a = a0 + s - r;
-        bool act = (s - r >= 0) && (s - r < TILE) && a <= ob && b <= ib;
-        int i = 0, j = 0;
-        ColInfo q;
-        if (act) {
+    // per-launch staging: the metadata of this wave's row and the layer thicknesses
+    if (lane < TILE) {
+        int a = a0 + lane;
+        int4 m = make_int4(0, -1, 0, 0);
+        if (a <= ob && b <= ib) {
+            int i, j;
             sweep_to_ij(d, 0, a, b, IS, JS, i, j);
-            act = d.adv[c2(d, i, j)] == 1;
+            m = d.meta[c2(d, i, j)];
+            m.z |= 1 << 20;                                    // in range
         }
-        if (act) {
-            q = col_info(d, i, j);
-            const size_t o = c3(d, i, j);
-            for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) {
-                const float *P = planes + o + k;
-                MlPieces p;
-                p.AW1 = P[PL_AW * n3]; p.AS1 = P[PL_AS * n3]; p.AE1 = P[PL_AE * n3]; p.AN1 = P[PL_AN * n3];
-                p.e = P[PL_E * n3]; p.f = P[PL_F * n3]; p.g = P[PL_G * n3]; p.w = P[PL_W * n3];
-                p.c0 = P[PL_C0 * n3]; p.X1 = P[PL_X1 * n3]; p.X2 = P[PL_X2 * n3]; p.X3 = P[PL_X3 * n3]; p.X4 = P[PL_X4 * n3];
-                float xw, xs, xe, xn, x;
-                ml_neighbours<COMP>(d, i, j, k, xw, xs, xe, xn, x);
-                sD[k] = ml_source<COMP>(d, q, k, p, xw, xs, xe, xn, x);
-                sQ[k] = P[PL_C * n3];
-                sT[k] = P[PL_T * n3];
-                sPr[k] = P[PL_P * n3];
-            }
-            for (int k = q.KSTART + lane; k <= q.VI + 1; k += WAVE) sX[k] = X[o + k];
+        smeta[r * TILE + lane] = m;
+    }
+    for (int k = threadIdx.x; k < nkp; k += TILE * WAVE) sdzk[k] = d.dzk[k];
+    __syncthreads();
+    MlPre pre[2];
+    float xfull[2];
+    Meta mn;
+    int in_ = 0, jn_ = 0;
+    bool inr_n = false;
+    auto locate = [&](int s) {
+        int l = s - r;
+        inr_n = false;
+        mn.adv = false;
+        if (l >= 0 && l < TILE) {
+            int4 m4 = smeta[r * TILE + l];
+            inr_n = (m4.z >> 20) & 1;
+            mn = unpack_meta(m4);
+            mn.adv = mn.adv && inr_n;
         }
-        __syncthreads();
-        if (act && lane == 0) {
-            const int KKP1 = q.KK + 1;
-            float Qp = 0;
-            for (int k = q.KSTART; k <= q.VI; ++k) {
-                float Q = (k > KKP1) ? (sD[k] + sQ[k] * Qp) * sT[k] : sD[k] * sT[k];
-                sQ[k] = Q;
-                Qp = Q;
+        if (inr_n) {
+            sweep_to_ij(d, 0, a0 + l, b, IS, JS, in_, jn_);
+            const size_t o = c3(d, in_, jn_);
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = lane + WAVE * m;
+                if (k < nkp) xfull[m] = X[o + k];
             }
-            int KKm = q.KK;
-            if (COMP == 0) KKm = max(q.KK, d.kk[c2(d, i - 1, j)]);
-            if (COMP == 1) KKm = max(q.KK, d.kk[c2(d, i, j - 1)]);
-            float up = sX[q.VI + 1];
-            for (int k = q.VI; k >= q.KSTART; --k) {
-                float x = sX[k];
-                if (COMP == 2 || KKm < k) {
-                    x = x + d.relax * (sPr[k] * up + sQ[k] - x);
-                    sX[k] = x;
+        }
+        if (mn.adv) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = mn.KSTART + lane + WAVE * m;
+                if (k <= mn.VI) {
+                    ml_prefetch<COMP>(d, planes, n3, in_, jn_, k, IS, JS, pre[m]);
+                    if (l == 0) pre[m].ni = X[c3(d, in_ - IS, jn_) + k];
+                    if (r == 0) pre[m].nj = X[c3(d, in_, jn_ - JS) + k];
                 }
-                up = x;
             }
         }
-        __syncthreads();
+    };
+    locate(0);
+    for (int s = 0; s < 2 * TILE - 1; ++s) {
+        const bool act = mn.adv, inr = inr_n;
+        const Meta mt = mn;
+        const int i = in_, j = jn_;
+        const ColInfo q = meta_col(mt, i, j, d.simd);
+        float *xcur = xbuf + (size_t)((s & 1) * TILE + r) * nkl;
+        const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r) * nkl;
+        const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r - 1) * nkl;
+        const bool w_in = (s - r >= 1), s_in = (r >= 1);
+        if (inr) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = lane + WAVE * m;
+                if (k < nkp) xcur[k] = xfull[m];
+            }
+        }
+        if (act) {
+            const int KKP1 = q.KK + 1;
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = q.KSTART + lane + WAVE * m;
+                if (k <= q.VI) {
+                    const MlPre &r_ = pre[m];
+                    // neighbours updated earlier in this sweep: -IS and -JS directions
+                    float xnew_i = w_in ? xw_lds[k] : r_.ni;
+                    float xnew_j = s_in ? xs_lds[k] : r_.nj;
+                    float xw = (IS == 1) ? xnew_i : r_.xa, xe = (IS == 1) ? r_.xa : xnew_i;
+                    float xs = (JS == 1) ? xnew_j : r_.xb, xn = (JS == 1) ? r_.xb : xnew_j;
+                    MlPieces pc;
+                    pc.AW1 = r_.aw; pc.AS1 = r_.as; pc.AE1 = r_.ae; pc.AN1 = r_.an; pc.e = r_.e; pc.f = r_.f;
+                    pc.g = r_.g; pc.w = r_.w; pc.c0 = r_.c0; pc.X1 = r_.x1; pc.X2 = r_.x2; pc.X3 = r_.x3; pc.X4 = r_.x4;
+                    float D = ml_source<COMP>(d, q, k, sdzk[k], pc, xw, xs, xe, xn, r_.x);
+                    sD[k] = D; sC[k] = r_.c; sT[k] = r_.t; sP[k] = r_.p;
+                    if (k <= KKP1) sQ[k] = D * r_.t;
+                }
+            }
+            if (lane == 0) {
+                int KKm = (COMP == 0) ? mt.KKmi : (COMP == 1) ? mt.KKmj : q.KK;
+                sinfo[r * 5 + 1] = q.KSTART; sinfo[r * 5 + 2] = q.KK; sinfo[r * 5 + 3] = q.VI; sinfo[r * 5 + 4] = KKm;
+            }
+        }
+        if (lane == 0) sinfo[r * 5] = act;
+        locate(s + 1);                                   // prefetch of the next column, in flight across phases 2-3
+        lds_barrier();
+        if (r == 0 && lane < TILE && sinfo[lane * 5]) {
+            float *D = sm + (size_t)lane * 5 * nkl + SLOT_PAD;
+            float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl;
+            float *Xs = xbuf + (size_t)((s & 1) * TILE + lane) * nkl;
+            const int KSTART = sinfo[lane * 5 + 1], KK = sinfo[lane * 5 + 2], VI = sinfo[lane * 5 + 3], KKm = sinfo[lane * 5 + 4];
+            chain_fwd(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, VI - KK - 1, Q[KK + 1]);
+            const int kend = (COMP == 2) ? KSTART : max(KSTART, KKm + 1);
+            chain_back(P + VI, Q + VI, Xs + VI, d.relax, VI - kend + 1);
+        }
+        lds_barrier();
         if (act) {
             const size_t o = c3(d, i, j);
-            for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) X[o + k] = sX[k];
+            for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) X[o + k] = xcur[k];
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -1155,65 +1459,144 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_tile(Dom d, const float *pla
 // geometry (including the stale top value of the row), so they are tabulated
 // once per solve for both i directions (ptab/ttab); only the QIMP recurrence,
 // with its own stale top value from the row scratch, depends on the flow.
-__global__ __launch_bounds__(TILE * WAVE) void k_pressure_tile(Dom d, const float *ptab, const float *ttab, int tau,
-                                                              int IS, int JS, int slot) {
+// Same step structure and LDS column buffers (of DPM) as k_ml_tile.  The row
+// scratch of QIMP is global: it is read in phase 2 of the step after the one
+// that wrote it, so phase 3 ends with a full barrier.
+struct PrPre {
+    float div, t, p, pa, pb;       // DIV, TERMP, PIMP, old DPM in the +IS (i) and +JS (j) directions
+    float ni, nj;                  // -IS / -JS neighbours when they lie in an earlier tile
+};
+
+__global__ __launch_bounds__(TILE * WAVE) void k_pressure_tile(const Dom *__restrict__ ds, int tau, int IS, int JS, int slot) {
+    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const float *ptab = d.ptab[IS == 1 ? 0 : 1], *ttab = d.ttab[IS == 1 ? 0 : 1];
+
     extern __shared__ float sm[];
-    const int nkp = d.nkp;
+    const int nkp = d.nkp, nkl = lds_slot(nkp);
     const int oa = 2, ia = 2, ob = d.njj - 1, ib = d.nii - 1;
     const int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
     int A, B;
     if (!tile_coords(tau, nta, ntb, A, B)) return;
     const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int a0 = oa + A * TILE, b = ia + B * TILE + r;
-    float *sD = sm + (size_t)r * 5 * nkp, *sQ = sD + nkp, *sO = sQ + nkp, *sT = sO + nkp, *sPr = sT + nkp;
+    float *sD = sm + (size_t)r * 4 * nkl + SLOT_PAD, *sQ = sD + nkl, *sT = sQ + nkl, *sP = sT + nkl;
+    float *xbuf = sm + (size_t)TILE * 4 * nkl + SLOT_PAD;      // [2][TILE] slots of whole DPM columns
+    float *sdzk = sm + (size_t)TILE * 6 * nkl, *satp = sdzk + nkp;
+    int4 *smeta = (int4 *)(((uintptr_t)(satp + nkp) + 15) & ~(uintptr_t)15);
+    int *sinfo = (int *)(smeta + TILE * TILE);                // per row: active, KK, VI, j
+    if (lane < TILE) {
+        int a = a0 + lane;
+        int4 m = make_int4(0, -1, 0, 0);
+        if (a <= ob && b <= ib) {
+            int i, j;
+            sweep_to_ij(d, 1, a, b, IS, JS, i, j);
+            m = d.meta[c2(d, i, j)];
+            m.z |= 1 << 20;
+        }
+        smeta[r * TILE + lane] = m;
+    }
+    for (int k = threadIdx.x; k < nkp; k += TILE * WAVE) { sdzk[k] = d.dzk[k]; satp[k] = d.atp[k]; }
+    __syncthreads();
     const float DXK = d.dxk, DYK = d.dyk;
     float rmax = 0.0f;
+    PrPre pre[2];
+    float xfull[2];
+    int in_ = 0, jn_ = 0, KKn = 0, VIn = 0;
+    bool inr_n = false, actn = false;
+    auto locate = [&](int s) {
+        int l = s - r;
+        inr_n = false;
+        actn = false;
+        if (l >= 0 && l < TILE) {
+            Meta m = unpack_meta(smeta[r * TILE + l]);
+            inr_n = (smeta[r * TILE + l].z >> 20) & 1;
+            actn = m.adv && inr_n;
+            KKn = m.KK; VIn = m.VI;
+        }
+        if (inr_n) {
+            sweep_to_ij(d, 1, a0 + l, b, IS, JS, in_, jn_);
+            const size_t o = c3(d, in_, jn_);
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = lane + WAVE * m;
+                if (k < nkp) xfull[m] = d.dpm[o + k];
+            }
+        }
+        if (actn) {
+            size_t o = c3(d, in_, jn_);
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = KKn + 1 + lane + WAVE * m;
+                if (k <= VIn) {
+                    pre[m].div = d.div[o + k];
+                    pre[m].t = ttab[o + k];
+                    pre[m].p = ptab[o + k];
+                    pre[m].pa = d.dpm[c3(d, in_ + IS, jn_) + k];
+                    pre[m].pb = d.dpm[c3(d, in_, jn_ + JS) + k];
+                    if (l == 0) pre[m].nj = d.dpm[c3(d, in_, jn_ - JS) + k];
+                    if (r == 0) pre[m].ni = d.dpm[c3(d, in_ - IS, jn_) + k];
+                }
+            }
+        }
+    };
+    locate(0);
     for (int s = 0; s < 2 * TILE - 1; ++s) {
-        int a = a0 + s - r;
-        bool act = (s - r >= 0) && (s - r < TILE) && a <= ob && b <= ib;
-        int i = 0, j = 0, KK = 0, VI = 0;
+        const bool act = actn, inr = inr_n;
+        const int i = in_, j = jn_, KK = KKn, VI = VIn;
+        float *xcur = xbuf + (size_t)((s & 1) * TILE + r) * nkl;
+        const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r) * nkl;
+        const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r - 1) * nkl;
+        const bool w_in = (s - r >= 1), s_in = (r >= 1);
+        if (inr) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = lane + WAVE * m;
+                if (k < nkp) xcur[k] = xfull[m];
+            }
+        }
         if (act) {
-            sweep_to_ij(d, 1, a, b, IS, JS, i, j);
-            act = d.adv[c2(d, i, j)] == 1;
-        }
-        if (act) {
-            size_t c = c2(d, i, j);
-            KK = d.kk[c]; VI = d.vi[c];
-            size_t o = c3(d, i, j), oim = c3(d, i - 1, j), oip = c3(d, i + 1, j), ojm = c3(d, i, j - 1), ojp = c3(d, i, j + 1);
-            for (int k = KK + 1 + lane; k <= VI; k += WAVE) {
-                float dz = d.dzk[k];
-                sD[k] = d.div[o + k] / d.dtime + (d.dpm[oim + k] + d.dpm[oip + k]) / DXK * DYK * dz +
-                        (d.dpm[ojm + k] - d.dpm[ojp + k]) / DYK * DXK * dz;
-                sT[k] = ttab[o + k];
-                sPr[k] = ptab[o + k];
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = KK + 1 + lane + WAVE * m;
+                if (k <= VI) {
+                    const PrPre &r_ = pre[m];
+                    // DPM of the neighbours updated earlier in this sweep: (a-1,b) is the
+                    // -JS row neighbour, (a,b-1) the -IS column neighbour
+                    float newj = w_in ? xw_lds[k] : r_.nj;
+                    float newi = s_in ? xs_lds[k] : r_.ni;
+                    float dim_ = (IS == 1) ? newi : r_.pa, dip = (IS == 1) ? r_.pa : newi;
+                    float djm = (JS == 1) ? newj : r_.pb, djp = (JS == 1) ? r_.pb : newj;
+                    float dz = sdzk[k];
+                    sD[k] = r_.div / d.dtime + (dim_ + dip) / DXK * DYK * dz + (djm - djp) / DYK * DXK * dz;
+                    sT[k] = r_.t;
+                    sP[k] = r_.p;
+                    sQ[k] = satp[k];                  // consumed and replaced by QIMP in phase 2
+                }
             }
-            for (int k = KK + lane; k <= VI; k += WAVE) sO[k] = d.dpm[o + k];
+            if (lane == 0) { sinfo[r * 4 + 1] = KK; sinfo[r * 4 + 2] = VI; sinfo[r * 4 + 3] = j; }
         }
-        __syncthreads();
-        if (act && lane == 0) {
-            const float *qrow = d.prow_q + (size_t)j * nkp;
-            float Qn = qrow[VI + 1];
-            for (int k = VI; k > KK; --k) {
-                float Q = (d.atp[k] * Qn + sD[k]) * sT[k];
-                sQ[k] = Q;
-                Qn = Q;
-            }
-            float dprev = sO[KK];
-            for (int k = KK + 1; k <= VI; ++k) {
-                float temp = sPr[k] * dprev + sQ[k];
-                sD[k] = temp;
-                dprev = temp;
-            }
+        if (lane == 0) sinfo[r * 4] = act;
+        locate(s + 1);
+        lds_barrier();
+        if (r == 0 && lane < TILE && sinfo[lane * 4]) {
+            float *Dd = sm + (size_t)lane * 4 * nkl + SLOT_PAD;
+            float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl;
+            const float *O = xbuf + (size_t)((s & 1) * TILE + lane) * nkl;
+            const int KK2 = sinfo[lane * 4 + 1], VI2 = sinfo[lane * 4 + 2], jr = sinfo[lane * 4 + 3];
+            // Q[k] holds ATP[k] on entry; the new DPM replaces D
+            chain_pdown(Dd + VI2, T + VI2, Q + VI2, d.prow_q[(size_t)jr * nkp + VI2 + 1], VI2 - KK2);
+            chain_pup(P + KK2 + 1, Q + KK2 + 1, Dd + KK2 + 1, O[KK2], VI2 - KK2);
         }
-        __syncthreads();
+        lds_barrier();
         if (act) {
             size_t o = c3(d, i, j);
             float *qrow = d.prow_q + (size_t)j * nkp;
             for (int k = KK + 1 + lane; k <= VI; k += WAVE) {
                 float temp = sD[k];
                 double qv = (double)conv_to_int((double)temp * d.frund) / d.frund;
-                float rr = (float)fabs((double)sO[k] - qv);
+                float rr = (float)fabs((double)xcur[k] - qv);
                 rmax = rr > rmax ? rr : rmax;
+                xcur[k] = temp;                             // column buffer now holds the new DPM
                 d.dpm[o + k] = temp;
                 d.dpmnew[o + k] = d.dpmnew[o + k] + d.relaxp * temp;
                 qrow[k] = sQ[k];
@@ -1231,12 +1614,23 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_tile(Dom d, const floa
 // ------------------------------------------------------------------------
 // host-side launchers
 // ------------------------------------------------------------------------
+static inline int lds_slot_h(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
 static inline int ndiag(int oa, int ob, int ia, int ib) { return (ob - oa) + (ib - ia) + 1; }
 static inline int diag_len(int t, int oa, int ob, int ia, int ib) {
     int nb = ib - ia;
     int amin = oa + t - nb > oa ? oa + t - nb : oa;
     int amax = ob < oa + t ? ob : oa + t;
     return amax - amin + 1;
+}
+// longest diagonal t over the situations; the pressure sweep runs a over j
+// (outer) and b over i, the momentum sweeps a over i and b over j
+static inline int diag_len_max(const Dom *hd, int nsit, int t, int oa, int ia, bool pressure) {
+    int m = 0;
+    for (int q = 0; q < nsit; ++q) {
+        int ob = pressure ? hd[q].njj - 1 : hd[q].nii - 1, ib = pressure ? hd[q].nii - 1 : hd[q].njj - 1;
+        if (t < ndiag(oa, ob, ia, ib)) m = max(m, diag_len(t, oa, ob, ia, ib));
+    }
+    return m;
 }
 
 // Optional per-launch timing: events recorded on the solver's stream right
@@ -1262,10 +1656,23 @@ struct Launcher {
     }
 };
 
-extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_t s, long long *launches, Probe *probe) {
-    const Dom &d = *dp;
+// Launches one SIMPLE iteration of nsit situations that advance in lock step
+// (same iteration number, hence the same sweep directions).  hd: host copies
+// of the situations' Dom, dd: the same array in device memory.  Grid slice z
+// of every launch is situation z; grid x/y are sized for the largest
+// situation and each kernel skips what lies outside its own grid.  All
+// situations share the turbulence model and the sweep kind.
+extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, int slot, hipStream_t s,
+                                   long long *launches, Probe *probe) {
+    if (nsit < 1) return 0;
     Launcher L{s, probe};
-    int nk = d.kadvmax;
+    const Dom &d0 = hd[0];
+    int nii = 0, njj = 0, nk = 0, nkp = 0, nj2 = 0;
+    for (int q = 0; q < nsit; ++q) {
+        nii = max(nii, hd[q].nii); njj = max(njj, hd[q].njj); nk = max(nk, hd[q].kadvmax);
+        nkp = max(nkp, hd[q].nkp); nj2 = max(nj2, hd[q].nj2);
+    }
+    const unsigned Z = (unsigned)nsit;
     int kb = (nk + 63) / 64;
     int IS = 1, JS = 1;
     switch (iter % 4) {
@@ -1274,71 +1681,96 @@ extern "C" int pf_launch_iteration(const Dom *dp, int iter, int slot, hipStream_
         case 2: IS = 1; JS = -1; break;
         default: break;
     }
-    int ncols = (d.nii - 2) * (d.njj - 2);
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(kb, d.njj - 1), dim3(64), 0, s, d); });
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ns, dim3(kb, d.nii - 1), dim3(64), 0, s, d); });
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_top, dim3((ncols + 255) / 256), dim3(256), 0, s, d); });
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncols), dim3(64), 0, s, d); });
-    size_t nrow = (size_t)d.nj2 * d.nkp;
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_q, nrow, 0.0f); });
-    if (d.tiled) {
-        int oa = 2, ob = d.njj - 1, ia = 2, ib = d.nii - 1;
-        int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
-        size_t shm = sizeof(float) * 5 * d.nkp * TILE;
-        const float *pt = d.ptab[IS == 1 ? 0 : 1], *tt = d.ttab[IS == 1 ? 0 : 1];
-        for (int tau = 0; tau < nta + ntb - 1; ++tau) {
-            int cnt = min(nta - 1, tau) - max(0, tau - (ntb - 1)) + 1;
+    int ncols = (nii - 2) * (njj - 2);
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(kb, njj - 1, Z), dim3(64), 0, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ns, dim3(kb, nii - 1, Z), dim3(64), 0, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_top, dim3((ncols + 255) / 256, 1, Z), dim3(256), 0, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncols, 1, Z), dim3(64), 0, s, dd); });
+    size_t nrow = (size_t)nj2 * nkp;
+    L.go(FAM_STENCIL, [&] {
+        hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((nrow + 255) / 256), 1, Z), dim3(256), 0, s, dd, d0.tiled ? 0 : 1);
+    });
+    if (d0.tiled) {
+        int nt = 0;
+        for (int q = 0; q < nsit; ++q) {
+            int nta = (hd[q].njj - 1 - 2 + TILE) / TILE, ntb = (hd[q].nii - 1 - 2 + TILE) / TILE;
+            nt = max(nt, nta + ntb - 1);
+        }
+        size_t shm = sizeof(float) * ((size_t)6 * lds_slot_h(nkp) * TILE + 2 * nkp + 8) + sizeof(int4) * TILE * TILE + sizeof(int) * 4 * TILE;
+        for (int tau = 0; tau < nt; ++tau) {
+            int cnt = 0;
+            for (int q = 0; q < nsit; ++q) {
+                int nta = (hd[q].njj - 1 - 2 + TILE) / TILE, ntb = (hd[q].nii - 1 - 2 + TILE) / TILE;
+                if (tau < nta + ntb - 1) cnt = max(cnt, min(nta - 1, tau) - max(0, tau - (ntb - 1)) + 1);
+            }
             L.go(FAM_PRESSURE, [&] {
-                hipLaunchKernelGGL(k_pressure_tile, dim3(cnt), dim3(TILE * WAVE), shm, s, d, pt, tt, tau, IS, JS, slot);
+                hipLaunchKernelGGL(k_pressure_tile, dim3(cnt, 1, Z), dim3(TILE * WAVE), shm, s, dd, tau, IS, JS, slot);
             });
         }
     } else {
-        L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, s, d.prow_p, nrow, 0.0f); });
-        int oa = 2, ob = d.njj - 1, ia = 2, ib = d.nii - 1;
-        size_t shm = sizeof(float) * 5 * d.nkp;
-        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t)
-            L.go(FAM_PRESSURE, [&] {
-                hipLaunchKernelGGL(k_pressure, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, d, t, IS, JS, slot);
-            });
+        int oa = 2, ob = njj - 1, ia = 2, ib = nii - 1;
+        size_t shm = sizeof(float) * 5 * nkp;
+        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t) {
+            int len = diag_len_max(hd, nsit, t, 2, 2, true);
+            if (len > 0)
+                L.go(FAM_PRESSURE, [&] {
+                    hipLaunchKernelGGL(k_pressure, dim3(len, 1, Z), dim3(WAVE), shm, s, dd, t, IS, JS, slot);
+                });
+        }
     }
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_correction, dim3(ncols), dim3(64), 0, s, d); });
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_averages, dim3(ncols), dim3(64), 0, s, d); });
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_avg_edges, dim3(d.njj - 2), dim3(64), 0, s, d); });
-    size_t shm = sizeof(float) * 5 * d.nkp;
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_correction, dim3(ncols, 1, Z), dim3(64), 0, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_averages, dim3(ncols, 1, Z), dim3(64), 0, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_avg_edges, dim3(njj - 2, 1, Z), dim3(64), 0, s, dd); });
+    size_t shm = sizeof(float) * 5 * nkp;
+    // per-diagonal sweeps: sweep ranges a in [oa, nii-1] (outer), b in [ia, njj-1]
     auto sweep = [&](int fam, auto kern, int oa, int ia, int is, int js) {
-        int ob = d.nii - 1, ib = d.njj - 1;
-        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t)
-            L.go(fam, [&] { hipLaunchKernelGGL(kern, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, d, t, is, js); });
-    };
-    auto tiled_sweep = [&](int fam, auto coef, auto tile, int oa, int ia, int is, int js) {
-        int ob = d.nii - 1, ib = d.njj - 1;
-        int na = ob - oa + 1, nb = ib - ia + 1;
-        L.go(fam, [&] { hipLaunchKernelGGL(coef, dim3(na * nb), dim3(WAVE), sizeof(float) * 3 * d.nkp, s, d, d.planes, is, js); });
-        int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
-        size_t shm_t = sizeof(float) * 5 * d.nkp * TILE;
-        for (int tau = 0; tau < nta + ntb - 1; ++tau) {
-            int cnt = min(nta - 1, tau) - max(0, tau - (ntb - 1)) + 1;
-            L.go(fam, [&] { hipLaunchKernelGGL(tile, dim3(cnt), dim3(TILE * WAVE), shm_t, s, d, (const float *)d.planes, tau, is, js); });
+        int nd = ndiag(oa, nii - 1, ia, njj - 1);
+        for (int t = 0; t < nd; ++t) {
+            int len = diag_len_max(hd, nsit, t, oa, ia, false);
+            if (len > 0)
+                L.go(fam, [&] { hipLaunchKernelGGL(kern, dim3(len, 1, Z), dim3(WAVE), shm, s, dd, t, is, js); });
         }
     };
-    if (d.model == 1 && d.tiled) {
+    auto tiled_sweep = [&](int fam, auto coef, auto tile, int oa, int ia, int is, int js) {
+        int nblk = 0, nt = 0;
+        for (int q = 0; q < nsit; ++q) {
+            int ob = hd[q].nii - 1, ib = hd[q].njj - 1;
+            int na = ob - oa + 1, nb = ib - ia + 1;
+            nblk = max(nblk, (na * nb + COEF_COLS - 1) / COEF_COLS);
+            nt = max(nt, (ob - oa + TILE) / TILE + (ib - ia + TILE) / TILE - 1);
+        }
+        size_t shm_c = sizeof(float) * 3 * (nkp | 1) * COEF_COLS + sizeof(int) * 4 * COEF_COLS;
+        L.go(fam, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
+        size_t shm_t = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + nkp + 8) + sizeof(int4) * TILE * TILE + sizeof(int) * 5 * TILE;
+        for (int tau = 0; tau < nt; ++tau) {
+            int cnt = 0;
+            for (int q = 0; q < nsit; ++q) {
+                int ob = hd[q].nii - 1, ib = hd[q].njj - 1;
+                int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
+                if (tau < nta + ntb - 1) cnt = max(cnt, min(nta - 1, tau) - max(0, tau - (ntb - 1)) + 1);
+            }
+            L.go(fam, [&] { hipLaunchKernelGGL(tile, dim3(cnt, 1, Z), dim3(TILE * WAVE), shm_t, s, dd, tau, is, js); });
+        }
+    };
+    if (d0.model == 1 && d0.tiled) {
         tiled_sweep(FAM_U, k_ml_coef<0>, k_ml_tile<0>, 3, 2, IS, JS);
         tiled_sweep(FAM_V, k_ml_coef<1>, k_ml_tile<1>, 2, 3, -IS, -JS);
         tiled_sweep(FAM_W, k_ml_coef<2>, k_ml_tile<2>, 2, 2, IS, JS);
-    } else if (d.model == 1) {
+    } else if (d0.model == 1) {
         sweep(FAM_U, k_momentum<0, 0>, 3, 2, IS, JS);
         sweep(FAM_V, k_momentum<1, 0>, 2, 3, -IS, -JS);
         sweep(FAM_W, k_momentum<2, 0>, 2, 2, IS, JS);
-    } else if (d.model == 2) {
+    } else if (d0.model == 2) {
         sweep(FAM_U, k_momentum<0, 1>, 3, 2, IS, JS);
         sweep(FAM_V, k_momentum<1, 1>, 2, 3, IS, JS);
         sweep(FAM_W, k_momentum<2, 1>, 2, 2, IS, JS);
-        size_t shm_t = sizeof(float) * 12 * d.nkp;
-        int oa = 2, ob = d.nii - 1, ia = 2, ib = d.njj - 1;
-        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t)
-            L.go(FAM_TKE, [&] {
-                hipLaunchKernelGGL(k_tke, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm_t, s, d, t, IS, JS);
-            });
+        size_t shm_t = sizeof(float) * 12 * nkp;
+        int nd = ndiag(2, nii - 1, 2, njj - 1);
+        for (int t = 0; t < nd; ++t) {
+            int len = diag_len_max(hd, nsit, t, 2, 2, false);
+            if (len > 0)
+                L.go(FAM_TKE, [&] { hipLaunchKernelGGL(k_tke, dim3(len, 1, Z), dim3(WAVE), shm_t, s, dd, t, IS, JS); });
+        }
     } else {
         sweep(FAM_U, k_momentum<0, 2>, 3, 2, IS, JS);
         sweep(FAM_V, k_momentum<1, 2>, 2, 3, IS, JS);

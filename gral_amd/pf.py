@@ -23,7 +23,9 @@ I16P = C.POINTER(C.c_int16)
 
 EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral_pf_upload_fields",
            "gral_pf_prepare", "gral_pf_iterate", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
-           "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_last_error", "gral_pf_destroy"]
+           "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
+           "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
+           "gral_pf_last_error", "gral_pf_destroy", "gral_pf_batch_last_error", "gral_pf_batch_destroy"]
 FAMILIES = ("stencil", "pressure", "u", "v", "w", "tke")
 
 
@@ -48,8 +50,8 @@ class Stats(C.Structure):
 
 
 class Timing(C.Structure):
-    _fields_ = [("ms_total", C.c_double), ("ms_pressure", C.c_double), ("ms_momentum", C.c_double),
-                ("ms_stencil", C.c_double), ("launches", C.c_int64), ("active_cells", C.c_int64)]
+    _fields_ = [("ms_total", C.c_double), ("launches", C.c_int64), ("active_cells", C.c_int64),
+                ("cell_iterations", C.c_int64), ("situations", C.c_int32)]
 
 
 class FamilyTime(C.Structure):
@@ -79,11 +81,18 @@ def load(path: str = LIB_PATH):
     lib.gral_pf_download_fields.argtypes = [H] + [FP] * 10
     lib.gral_pf_last_timing.argtypes = [H, C.POINTER(Timing)]
     lib.gral_pf_profile_iterations.argtypes = [H, C.c_int32, C.POINTER(FamilyTime), C.c_int32]
-    lib.gral_pf_last_error.argtypes = [H]
-    lib.gral_pf_last_error.restype = C.c_char_p
-    lib.gral_pf_destroy.argtypes = [H]
-    lib.gral_pf_destroy.restype = None
-    for n in EXPORTS[:-2]:
+    lib.gral_pf_batch_create.argtypes = [C.POINTER(H), C.c_int32, C.POINTER(H)]
+    lib.gral_pf_batch_iterate.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(Stats)]
+    lib.gral_pf_batch_solve.argtypes = [H, C.POINTER(Stats)]
+    lib.gral_pf_batch_profile_iterations.argtypes = [H, C.c_int32, C.POINTER(FamilyTime), C.c_int32]
+    lib.gral_pf_batch_last_timing.argtypes = [H, C.POINTER(Timing)]
+    for n in ("gral_pf_last_error", "gral_pf_batch_last_error"):
+        getattr(lib, n).argtypes = [H]
+        getattr(lib, n).restype = C.c_char_p
+    for n in ("gral_pf_destroy", "gral_pf_batch_destroy"):
+        getattr(lib, n).argtypes = [H]
+        getattr(lib, n).restype = None
+    for n in EXPORTS[:-4]:
         getattr(lib, n).restype = C.c_int
     _lib = lib
     return lib
@@ -187,6 +196,65 @@ class PrognosticFlowfield:
         if getattr(self, "h", None):
             self.lib.gral_pf_destroy(self.h)
             self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Several prepared situations advanced in lock step by the same launches
+    (gral_pf_batch_*).  Each situation keeps its own convergence test and
+    leaves the batch when it stops; results equal plain solves bit for bit."""
+
+    def __init__(self, solvers):
+        self.lib = load()
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        b = C.c_void_p()
+        rc = self.lib.gral_pf_batch_create(arr, len(self.solvers), C.byref(b))
+        if rc != 0:
+            raise RuntimeError(f"gral_pf_batch_create failed ({rc})")
+        self.b = b
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.gral_pf_batch_last_error(self.b)
+            raise RuntimeError(f"gral_pf_batch call failed ({rc}): {msg.decode() if msg else ''}")
+
+    def iterate(self, n, check_convergence=False):
+        st = (Stats * len(self.solvers))()
+        self._check(self.lib.gral_pf_batch_iterate(self.b, int(n), int(bool(check_convergence)), st))
+        return [_stats(x) for x in st]
+
+    def solve(self):
+        st = (Stats * len(self.solvers))()
+        self._check(self.lib.gral_pf_batch_solve(self.b, st))
+        return [_stats(x) for x in st]
+
+    def timing(self):
+        t = Timing()
+        self._check(self.lib.gral_pf_batch_last_timing(self.b, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in Timing._fields_}
+
+    def profile_families(self, n):
+        arr = (FamilyTime * len(FAMILIES))()
+        self._check(self.lib.gral_pf_batch_profile_iterations(self.b, int(n), arr, len(FAMILIES)))
+        return {f: {"ms": arr[q].ms, "launches": arr[q].launches, "cells": arr[q].cells}
+                for q, f in enumerate(FAMILIES) if arr[q].launches > 0}
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.gral_pf_batch_destroy(self.b)
+            self.b = None
 
     def __enter__(self):
         return self

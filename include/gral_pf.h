@@ -120,13 +120,16 @@ int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, float *wk, 
                             float *vks, float *wks, float *dpm, float *dpmnew, float *turb,
                             float *tdiss);
 
-/* Time (ms, HIP events on the solver's stream) and launch count of the most
- * recent gral_pf_iterate call, per kernel family; for measurement. */
+/* Device time (ms, HIP events on the solver's stream around the whole call)
+ * and work of the most recent gral_pf_iterate / gral_pf_batch_iterate call;
+ * for measurement.  No reference counterpart ("Flow field [s]" of
+ * Program.cs:592 is the closest). */
 typedef struct {
     double ms_total;
-    double ms_pressure, ms_momentum, ms_stencil;
     int64_t launches;
-    int64_t active_cells;       /* cells advanced per SIMPLE iteration */
+    int64_t active_cells;       /* active cells of all situations (advanced per SIMPLE iteration) */
+    int64_t cell_iterations;    /* active cells x SIMPLE iterations actually run, summed over situations */
+    int32_t situations;         /* situations in the call (1 for gral_pf_iterate) */
 } gral_pf_timing;
 int gral_pf_last_timing(gral_pf_handle *h, gral_pf_timing *t);
 
@@ -143,6 +146,39 @@ int gral_pf_profile_iterations(gral_pf_handle *h, int32_t n, gral_pf_family_time
 
 const char *gral_pf_last_error(const gral_pf_handle *h);
 void gral_pf_destroy(gral_pf_handle *h);
+
+/* ---- Batches of weather situations ------------------------------------
+ * GRAL calls Calculate() once per weather situation, one after the other
+ * (Program.cs main loop -> MicroscaleFlat.cs:381 / MicroscaleTerrain.cs:847);
+ * a project holds hundreds of them.  A batch advances several prepared
+ * situations in lock step, each kernel launch covering all of them, so the
+ * GPU is filled even though one situation's Gauss-Seidel sweeps expose only
+ * one anti-diagonal of columns at a time.  Each situation keeps its own
+ * grid size, fields, convergence test and iteration limits, and leaves the
+ * batch when the reference's stop test (PrognosticFlowfield.cs:1149-1166)
+ * ends it (early retirement); its results are bit-identical to a plain
+ * gral_pf_solve of the same situation.  Situations of one batch must use the
+ * same turbulence model.  The handles stay owned by the caller and must
+ * outlive the batch; after the batch, gral_pf_finish/download_fields work on
+ * each handle as after gral_pf_iterate. */
+typedef struct gral_pf_batch gral_pf_batch;
+
+/* handles: n prepared situations (gral_pf_prepare), all at the same iteration. */
+int gral_pf_batch_create(gral_pf_handle *const *handles, int32_t n, gral_pf_batch **out);
+
+/* Up to n lock-step SIMPLE iterations; stats (may be NULL) receives one entry
+ * per situation, in handle order. */
+int gral_pf_batch_iterate(gral_pf_batch *b, int32_t n, int32_t check_convergence, gral_pf_stats *stats);
+
+/* Iterates every situation to its own stop (convergence test or iter_max),
+ * then runs gral_pf_finish on each; stats: one entry per situation. */
+int gral_pf_batch_solve(gral_pf_batch *b, gral_pf_stats *stats);
+
+/* As gral_pf_profile_iterations, over the whole batch. */
+int gral_pf_batch_profile_iterations(gral_pf_batch *b, int32_t n, gral_pf_family_time *out, int32_t nfam);
+int gral_pf_batch_last_timing(gral_pf_batch *b, gral_pf_timing *t);
+const char *gral_pf_batch_last_error(const gral_pf_batch *b);
+void gral_pf_batch_destroy(gral_pf_batch *b);
 
 #ifdef __cplusplus
 }
