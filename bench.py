@@ -1,9 +1,13 @@
 """Benchmark: SIMPLE cell-iterations/s of the prognostic flow solver on one MI355X.
 
-Workload: the plain solver on the seeded random city block (400 x 400 cells,
-dx = 2 m, 40 prognostic layers, cuboids at 30 % plan-area density, log inflow,
-wind from 270 deg).  A step is one SIMPLE iteration of that situation with the
-state resident in HBM (no host transfers in the timed region).
+Workload: the seeded random city block (400 x 400 cells, dx = 2 m, 40
+prognostic layers, cuboids at 30 % plan-area density, log inflow, z0 = 0.5 m)
+under its 8 wind directions (0, 45, ..., 315 deg), the 8 situations advanced
+together as one lock-step batch.  Each situation is the plain solver (same
+order, same arithmetic, bit-identical to a solve on its own).  A step is one
+SIMPLE iteration of all situations with the state resident in HBM (no host
+transfers in the timed region).  The single-direction (270 deg) plain solve is
+timed beside it.
 
 Prints one JSON line (see the repository's bench contract).  ``--gpus N`` with
 N > 1 exits: this build targets one GPU.
@@ -19,10 +23,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic FP32 bytes per active cell for one pass of each kernel family:
+# algorithmic FP32 bytes per active cell for one pass of each sweep family:
 # every array the pass reads or writes counted once (pressure: DIV, DPM, DPMNEW
 # read; DPM, DPMNEW written; momentum (mixing length): 7 arrays read, 1 written)
 ALG_BYTES = {"pressure": 5 * 4, "u": 8 * 4, "v": 8 * 4, "w": 8 * 4}
+KERNEL_OF = {"pressure": "k_pressure_tile", "u": "k_ml_tile<0>", "v": "k_ml_tile<1>", "w": "k_ml_tile<2>"}
 ALG_BYTES_ITER = 188          # whole SIMPLE iteration, mixing-length model
 HBM_PEAK_GBS = 8000.0
 
@@ -33,7 +38,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=400, help="grid cells per side (city block)")
-    ap.add_argument("--direction", type=float, default=270.0)
+    ap.add_argument("--directions", type=int, default=8, help="wind directions 0, 45, ... solved as one batch")
+    ap.add_argument("--no-plain", action="store_true", help="skip the single-situation plain solve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iterations", type=int, default=2)
     ap.add_argument("--no-family-profile", action="store_true",
@@ -58,59 +64,80 @@ def main():
 
     from gral_amd import domain, pf
 
-    case = domain.config2(direction=args.direction, iterations=args.warmup + args.steps + 16, n=args.n)
-    active = case.active_cells()
-    solver = pf.PrognosticFlowfield(case)
-    solver.prepare(iterations=args.warmup + args.steps + 16)
+    total = args.warmup + args.steps + 16
+    directions = [45.0 * q for q in range(args.directions)]
+    cases = [domain.config2(direction=d, iterations=total, n=args.n) for d in directions]
+    solvers = [pf.PrognosticFlowfield(c) for c in cases]
+    for sv in solvers:
+        sv.prepare(iterations=total)
+    batch = pf.Batch(solvers)
     if args.warmup:
-        solver.iterate(args.warmup)
+        batch.iterate(args.warmup)
     t0 = time.perf_counter()
-    solver.iterate(args.steps)
+    batch.iterate(args.steps)
     wall = time.perf_counter() - t0
-    tm = solver.timing()
+    tm = batch.timing()
     ms = tm["ms_total"] / args.steps
-    value = active * args.steps / (tm["ms_total"] / 1e3)
+    value = tm["cell_iterations"] / (tm["ms_total"] / 1e3)
 
-    # dominant kernel: per-launch device time of each sweep family over a few
-    # instrumented iterations (events bracket single launches on the solver's stream)
     if args.no_family_profile:
-        solver.close()
+        batch.close()
         print(json.dumps({"metric": "SIMPLE cell-iters/s", "value": value, "ms_per_step": ms, "steps": args.steps}))
         return
-    prof = solver.profile_families(4)
-    dom = max(prof, key=lambda k: prof[k]["ms"])
-    pdom = prof[dom]
+    # dominant kernel: per-launch device time of each sweep family over a few
+    # instrumented iterations (events bracket single launches on the batch's stream)
+    prof = batch.profile_families(2)
+    batch.close()
+    for sv in solvers:
+        sv.close()
+    sweeps = {k: v for k, v in prof.items() if k in ALG_BYTES}
+    dom = max(sweeps, key=lambda k: sweeps[k]["ms"])
+    pdom = sweeps[dom]
     avg_ms = pdom["ms"] / max(1, pdom["launches"])
-    bytes_per_launch = ALG_BYTES.get(dom, 0) * pdom["cells"] / max(1, pdom["launches"])
+    bytes_per_launch = ALG_BYTES[dom] * pdom["cells"] / max(1, pdom["launches"])
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    roofline = {"bound": "hbm", "kernel": KERNEL_OF[dom], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "avg_launch_us": avg_ms * 1e3, "launches_per_iter": pdom["launches"] / 4,
-                "alg_bytes_per_cell": ALG_BYTES.get(dom, 0)}
-    solver.close()
+                "avg_launch_us": avg_ms * 1e3, "launches_per_iter": pdom["launches"] / 2,
+                "alg_bytes_per_cell": ALG_BYTES[dom], "alg_bytes_per_launch": bytes_per_launch}
+
+    # the plain single-situation solve (270 deg), timed the same way
+    plain = None
+    if not args.no_plain:
+        c270 = domain.config2(direction=270.0, iterations=total, n=args.n)
+        with pf.PrognosticFlowfield(c270) as sv:
+            sv.prepare(iterations=total)
+            if args.warmup:
+                sv.iterate(args.warmup)
+            sv.iterate(args.steps)
+            tp = sv.timing()
+        plain = {"value": tp["cell_iterations"] / (tp["ms_total"] / 1e3), "ms_per_step": tp["ms_total"] / args.steps,
+                 "direction_deg": 270.0, "active_cells": tp["active_cells"]}
 
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import pyoracle
-        c0 = case.copy()
-        t = time.perf_counter(); pyoracle.run(c0, iterations=0, keep_history=False); tp = time.perf_counter() - t
+        c0 = domain.config2(direction=270.0, iterations=args.cpu_iterations, n=args.n)
+        t = time.perf_counter(); pyoracle.run(c0, iterations=0, keep_history=False); t_pro = time.perf_counter() - t
         t = time.perf_counter(); pyoracle.run(c0, iterations=args.cpu_iterations, keep_history=False)
-        tc = time.perf_counter() - t - tp
-        cpu = {"value": active * args.cpu_iterations / tc, "unit": "cell-iter/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_iterations} SIMPLE iterations of the same situation, deterministic "
+        tc = time.perf_counter() - t - t_pro
+        cpu = {"value": c0.active_cells() * args.cpu_iterations / tc, "unit": "cell-iter/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_iterations} SIMPLE iterations of the 270 deg situation, deterministic "
                          f"single-thread C restatement, prologue excluded ({tc:.1f} s)"}
 
     line = {"metric": "SIMPLE cell-iters/s", "value": value, "unit": "cell-iter/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": f"city block {case.nii}x{case.njj}, dx=2 m, 40 prognostic layers, "
-                                   f"30% buildings (seed 20260001), wind {args.direction:g} deg, plain solver, "
-                                   f"mixing-length model", "active_cells": active, "kadvmax": case.kadvmax,
+            "config": {"workload": f"city block {cases[0].nii}x{cases[0].njj}, dx=2 m, 40 prognostic layers, "
+                                   f"30% buildings (seed 20260001), {len(cases)} wind directions "
+                                   f"({', '.join(f'{d:g}' for d in directions)} deg) as one lock-step batch, "
+                                   f"plain solver per situation, mixing-length model",
+                       "situations": len(cases), "active_cells": tm["active_cells"], "kadvmax": cases[0].kadvmax,
                        "parallelism": "single GPU"},
             "wall_s_timed": wall, "alg_bytes_per_cell_iter": ALG_BYTES_ITER,
             "iteration_frac_of_hbm_roofline": value * ALG_BYTES_ITER / (HBM_PEAK_GBS * 1e9),
-            "roofline": roofline, "cpu_baseline": cpu,
-            "kernel_families_ms_per_iter": {k: v["ms"] / 4 for k, v in prof.items()}}
+            "roofline": roofline, "cpu_baseline": cpu, "plain_single_situation": plain,
+            "kernel_families_ms_per_iter": {k: v["ms"] / 2 for k, v in prof.items()}}
     print(json.dumps(line))
 
 

@@ -1640,7 +1640,7 @@ struct Probe {
     int *fam;            // cap family tags
     int cap, n;
 };
-enum { FAM_STENCIL = 0, FAM_PRESSURE = 1, FAM_U = 2, FAM_V = 3, FAM_W = 4, FAM_TKE = 5 };
+enum { FAM_STENCIL = 0, FAM_PRESSURE = 1, FAM_U = 2, FAM_V = 3, FAM_W = 4, FAM_TKE = 5, FAM_COEF = 6 };
 
 struct Launcher {
     hipStream_t s;
@@ -1740,7 +1740,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
             nt = max(nt, (ob - oa + TILE) / TILE + (ib - ia + TILE) / TILE - 1);
         }
         size_t shm_c = sizeof(float) * 3 * (nkp | 1) * COEF_COLS + sizeof(int) * 4 * COEF_COLS;
-        L.go(fam, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
+        L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
         size_t shm_t = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + nkp + 8) + sizeof(int4) * TILE * TILE + sizeof(int) * 5 * TILE;
         for (int tau = 0; tau < nt; ++tau) {
             int cnt = 0;

@@ -26,7 +26,7 @@ EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
            "gral_pf_last_error", "gral_pf_destroy", "gral_pf_batch_last_error", "gral_pf_batch_destroy"]
-FAMILIES = ("stencil", "pressure", "u", "v", "w", "tke")
+FAMILIES = ("stencil", "pressure", "u", "v", "w", "tke", "coef")
 
 
 class Grid(C.Structure):

@@ -136,7 +136,8 @@ int gral_pf_last_timing(gral_pf_handle *h, gral_pf_timing *t);
 /* Measurement: runs n SIMPLE iterations (continuing the solve) with HIP events
  * around every single kernel launch, and sums the device time per kernel
  * family: 0 stencil passes, 1 pressure sweep, 2 U, 3 V, 4 W momentum sweeps,
- * 5 TKE sweep.  cells = active cells advanced per iteration times iterations. */
+ * 5 TKE sweep, 6 the neighbour-independent coefficient pre-pass of the tiled
+ * mixing-length sweeps.  cells = active cells advanced per iteration times iterations. */
 typedef struct {
     double ms;
     int64_t launches;

@@ -61,7 +61,22 @@ typedef struct {
     int *VI; float *HT, *HO;          /* VerticalIndex, Ustern terrain/obstacle helpterms */
     float *AP0;
     float *P1, *Q1, *P2, *Q2;         /* column scratch (PIMU/QIMU, PIMTDISS...) */
+    orc_capture *cap;                 /* optional checking hook */
+    int iter, comp;                   /* current SIMPLE iteration and sweep (for the hook) */
 } St;
+
+static bool cap_on(const St *s, int comp, int i, int j) {
+    const orc_capture *c = s->cap;
+    return c && c->comp == comp && c->i == i && c->j == j && c->iteration == s->iter;
+}
+
+static void cap_column(St *s, int comp, int i, int j, const float *x, bool after) {
+    if (!cap_on(s, comp, i, j)) return;
+    float *dst = after ? s->cap->x_new : s->cap->x_old;
+    memcpy(dst, x, sizeof(float) * (size_t)s->NKP);
+    if (after) s->cap->captured = 1;
+}
+
 
 #define IX2(s, i, j) ((size_t)(i) * (size_t)(s)->NJ2 + (size_t)(j))
 #define IX3(s, i, j, k) (IX2(s, i, j) * (size_t)(s)->NKP + (size_t)(k))
@@ -186,7 +201,9 @@ static float pressure(St *s, int IS, int JS) {
                 float TERMP = 1 / (APP - ATP * PIMP[k + 1]);
                 PIMP[k] = ABP * TERMP;
                 QIMP[k] = (ATP * QIMP[k + 1] + DIM) * TERMP;
+                if (cap_on(s, 3, i, j)) { s->cap->a[k] = APP; s->cap->b[k] = ABP; s->cap->c[k] = ATP; s->cap->d[k] = DIM; }
             }
+            cap_column(s, 3, i, j, DPM, false);
             for (int k = KK + 1; k <= VIc; ++k) {
                 float temp = PIMP[k] * DPM[k - 1] + QIMP[k];
                 double q = (double)conv_to_int_d((double)temp * s->Frund) / s->Frund;
@@ -194,6 +211,7 @@ static float pressure(St *s, int IS, int JS) {
                 DPM[k] = temp;
                 DPMNEW[k] += s->relaxp * DPM[k];
             }
+            cap_column(s, 3, i, j, DPM, true);
         }
         if (DeltaUMaxIntern > DeltaUMax) DeltaUMax = DeltaUMaxIntern;
     }
@@ -296,6 +314,9 @@ static void ml_column(St *s, int i, int j, lane_fn fn, void *ctx) {
         if (end) { kint_s = k_real - k; k_real -= kint_s; }
         for (int kint = kint_s; kint < W; ++kint) {
             int idx = k_real + kint;
+            if (idx < KEND && cap_on(s, s->comp, i, j)) {
+                s->cap->a[idx] = A[kint]; s->cap->b[idx] = B[kint]; s->cap->c[idx] = C[kint]; s->cap->d[idx] = D[kint];
+            }
             if (idx < KEND) {
                 if (idx > KKP1) {
                     float TERMP = 1 / (A[kint] - C[kint] * P[idx - 1]);
@@ -617,6 +638,7 @@ static void w1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
 
 /* U_PrognosticMicroscaleV1.Calculate(IS, JS): i1 in [3,NII-1], j1 in [2,NJJ-1] */
 static void u1_sweep(St *s, int IS, int JS) {
+    s->comp = 0;
     uctx c = { s->UG };
     for (int i1 = 3; i1 < s->NII; ++i1) {
         int i = (IS == -1) ? s->NII - i1 + 1 : i1;
@@ -628,8 +650,10 @@ static void u1_sweep(St *s, int IS, int JS) {
             int KK = KK(s, i, j), VIc = s->VI[IX2(s, i, j)];
             int KSTART = (s->CUTK[IX2(s, i, j)] == 0) ? KK + 1 : 1;
             int KKm = KK > KK(s, i - 1, j) ? KK : KK(s, i - 1, j);
+            cap_column(s, 0, i, j, UK, false);
             for (int k = VIc; k >= KSTART; --k)
                 if (KKm < k) UK[k] += s->relax * (s->P1[k] * UK[k + 1] + s->Q1[k] - UK[k]);
+            cap_column(s, 0, i, j, UK, true);
         }
     }
 }
@@ -637,6 +661,7 @@ static void u1_sweep(St *s, int IS, int JS) {
 /* V_PrognosticMicroscaleV1.Calculate(IS, JS): i1 in [2,NII-1], j1 in [3,NJJ-1] */
 static void v1_sweep(St *s, int IS, int JS) {
     uctx c = { s->VG };
+    s->comp = 1;
     for (int i1 = 2; i1 < s->NII; ++i1) {
         int i = (IS == -1) ? s->NII - i1 + 1 : i1;
         for (int j1 = 3; j1 <= s->NJJ - 1; ++j1) {
@@ -647,14 +672,17 @@ static void v1_sweep(St *s, int IS, int JS) {
             int KK = KK(s, i, j), VIc = s->VI[IX2(s, i, j)];
             int KSTART = (s->CUTK[IX2(s, i, j)] == 0) ? KK + 1 : 1;
             int KKm = KK > KK(s, i, j - 1) ? KK : KK(s, i, j - 1);
+            cap_column(s, 1, i, j, VK, false);
             for (int k = VIc; k >= KSTART; --k)
                 if (KKm < k) VK[k] += s->relax * (s->P1[k] * VK[k + 1] + s->Q1[k] - VK[k]);
+            cap_column(s, 1, i, j, VK, true);
         }
     }
 }
 
 /* W_PrognosticMicroscaleV1.Calculate(IS, JS): i1 in [2,NII-1], j1 in [2,NJJ-1] */
 static void w1_sweep(St *s, int IS, int JS) {
+    s->comp = 2;
     for (int i1 = 2; i1 < s->NII; ++i1) {
         int i = (IS == -1) ? s->NII - i1 + 1 : i1;
         for (int j1 = 2; j1 < s->NJJ; ++j1) {
@@ -664,8 +692,10 @@ static void w1_sweep(St *s, int IS, int JS) {
             float *WK = COL(s, WK, i, j);
             int KK = KK(s, i, j), VIc = s->VI[IX2(s, i, j)];
             int KSTART = (s->CUTK[IX2(s, i, j)] == 0) ? KK + 1 : 1;
+            cap_column(s, 2, i, j, WK, false);
             for (int k = VIc; k >= KSTART; --k)
                 WK[k] += s->relax * (s->P1[k] * WK[k + 1] + s->Q1[k] - WK[k]);
+            cap_column(s, 2, i, j, WK, true);
         }
     }
 }
@@ -1072,7 +1102,23 @@ static bool dilated(const St *s, int i, int j) {
            ADV(s, i, jp) == 1 || ADV(s, i - 1, jp) == 1 || ADV(s, ip, jp) == 1;
 }
 
+float orc_powlaw_vector(float d, float peclet) { return ml_pow(d, peclet); }
+float orc_powlaw_scalar(float d, float peclet) { return pow_d(d, peclet); }
+int orc_conv_to_int(double x) { return conv_to_int_d(x); }
+
+static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap);
+
 int orc_calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st) {
+    return calculate(p, in, f, st, NULL);
+}
+
+int orc_calculate_capture(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap) {
+    if (cap && (!cap->a || !cap->b || !cap->c || !cap->d || !cap->x_old || !cap->x_new)) return -7;
+    if (cap) cap->captured = 0;
+    return calculate(p, in, f, st, cap);
+}
+
+static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap) {
     if (!p || !in || !f || !st) return -1;
     if (p->simd_width != 8 && p->simd_width != 16) return -2;
     if (p->turbulence_model < 0 || p->turbulence_model > 2) return -3;
@@ -1080,6 +1126,7 @@ int orc_calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_
     if (p->vertcellsff < p->simd_width) return -5;
     St S, *s = &S;
     memset(s, 0, sizeof(S));
+    s->cap = cap;
     s->NII = p->nii; s->NJJ = p->njj; s->NKK = p->nkk; s->KADVMAX = p->kadvmax; s->VCF = p->vertcellsff;
     s->NJ2 = p->njj + 2; s->NKP = p->nkk + 2; s->W = p->simd_width; s->model = p->turbulence_model;
     s->DXK = p->dxk; s->DYK = p->dyk; s->relax = p->relax; s->relaxp = p->relaxp;
@@ -1207,6 +1254,7 @@ int orc_calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_
     float DeltaFinish = 0, DeltaFirst = 0;
     double DeltaMean = 0;
     while (IterationLoops <= p->iter_max) {
+        s->iter = IterationLoops;
         boundary_conditions(s);
         divergence(s);
         int IS = 1, JS = 1;

@@ -72,6 +72,28 @@ typedef struct {
  * Returns 0 on success, negative on invalid input. */
 int orc_calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st);
 
+/* ---- checking hooks (closed-form and textbook checks of the oracle) ---- */
+
+/* Records, for one column (i,j) during one SIMPLE iteration, the coefficients
+ * with which a mixing-length momentum sweep (comp 0 U, 1 V, 2 W) or the
+ * pressure sweep (comp 3) enters its Thomas recurrence, and the column before
+ * and after the update.  Momentum: a[k] x[k] = b[k] x[k+1] + c[k] x[k-1] + d[k]
+ * (c dropped at k <= KKART+1).  Pressure: a = APP, b = ABP (bottom), c = ATP
+ * (top), d = DIM, APP x[k] = ABP x[k-1] + ATP x[k+1] + DIM. */
+typedef struct {
+    int comp, i, j, iteration;
+    float *a, *b, *c, *d;       /* [nkk+2] each */
+    float *x_old, *x_new;       /* [nkk+2] each */
+    int captured;               /* set to 1 when recorded */
+} orc_capture;
+int orc_calculate_capture(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st,
+                          orc_capture *cap);
+
+/* single formulas of the solver */
+float orc_powlaw_vector(float d, float peclet);   /* D*Vector.Max(0,(1-0.1P)^5), U-prognostic-microscale_1.cs:348-370 */
+float orc_powlaw_scalar(float d, float peclet);   /* D*FloatMax(0,(float)Pow5(1-0.1P)), U-prognostic-microscale_2.cs:180-185 */
+int orc_conv_to_int(double x);                     /* ProgramFunctions.cs:792-804 */
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,11 @@ class OrcStats(C.Structure):
                 ("convergence_problem", C.c_int), ("delta_umax_hist", FP)]
 
 
+class OrcCapture(C.Structure):
+    _fields_ = [("comp", C.c_int), ("i", C.c_int), ("j", C.c_int), ("iteration", C.c_int),
+                ("a", FP), ("b", FP), ("c", FP), ("d", FP), ("x_old", FP), ("x_new", FP), ("captured", C.c_int)]
+
+
 _lib = None
 
 
@@ -64,6 +69,14 @@ def load(build_if_missing: bool = True):
     _lib.orc_calculate.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcInputs), C.POINTER(OrcFields),
                                    C.POINTER(OrcStats)]
     _lib.orc_calculate.restype = C.c_int
+    _lib.orc_calculate_capture.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcInputs), C.POINTER(OrcFields),
+                                           C.POINTER(OrcStats), C.POINTER(OrcCapture)]
+    _lib.orc_calculate_capture.restype = C.c_int
+    for n in ("orc_powlaw_vector", "orc_powlaw_scalar"):
+        getattr(_lib, n).argtypes = [C.c_float, C.c_float]
+        getattr(_lib, n).restype = C.c_float
+    _lib.orc_conv_to_int.argtypes = [C.c_double]
+    _lib.orc_conv_to_int.restype = C.c_int
     return _lib
 
 
@@ -74,7 +87,7 @@ def _p(a, t=FP):
     return a.ctypes.data_as(t)
 
 
-def run(case, iterations=None, simd_width=None, keep_history=True):
+def run(case, iterations=None, simd_width=None, keep_history=True, capture=None):
     """Run PrognosticFlowfield.Calculate on ``case`` (a gral_amd.domain.Case)
     with the deterministic CPU oracle.  Returns a dict of output fields and stats.
     The input case is not modified."""
@@ -106,9 +119,20 @@ def run(case, iterations=None, simd_width=None, keep_history=True):
     hist = np.zeros(max(nit, 1), dtype=np.float32)
     st = OrcStats()
     st.delta_umax_hist = _p(hist) if keep_history else C.cast(None, FP)
-    rc = lib.orc_calculate(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st))
+    cap = None
+    if capture is not None:
+        comp, ci, cj, cit = capture
+        nk = c.nkk + 2
+        capbuf = {n: np.zeros(nk, dtype=np.float32) for n in ("a", "b", "c", "d", "x_old", "x_new")}
+        cap = OrcCapture(comp, ci, cj, cit, *[_p(capbuf[n]) for n in ("a", "b", "c", "d", "x_old", "x_new")], 0)
+        rc = lib.orc_calculate_capture(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st), C.byref(cap))
+    else:
+        rc = lib.orc_calculate(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st))
     if rc != 0:
         raise RuntimeError(f"orc_calculate failed: {rc}")
+    if cap is not None:
+        capbuf["captured"] = bool(cap.captured)
+        out["capture"] = capbuf
     stats = {n: getattr(st, n) for n, _ in OrcStats._fields_ if n != "delta_umax_hist"}
     stats["delta_umax_hist"] = hist[: st.iterations_run].copy()
     out["stats"] = stats
