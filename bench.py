@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 # every array the pass reads or writes counted once (pressure: DIV, DPM, DPMNEW
 # read; DPM, DPMNEW written; momentum (mixing length): 7 arrays read, 1 written)
 ALG_BYTES = {"pressure": 5 * 4, "u": 8 * 4, "v": 8 * 4, "w": 8 * 4}
-KERNEL_OF = {"pressure": "k_pressure_tile", "u": "k_ml_tile<0>", "v": "k_ml_tile<1>", "w": "k_ml_tile<2>"}
+KERNEL_OF = {"pressure": "k_pressure_strip", "u": "k_ml_strip<0>", "v": "k_ml_strip<1>", "w": "k_ml_strip<2>"}
 ALG_BYTES_ITER = 188          # whole SIMPLE iteration, mixing-length model
 HBM_PEAK_GBS = 8000.0
 

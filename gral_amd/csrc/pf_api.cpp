@@ -27,6 +27,17 @@ struct Probe {
 extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, int slot, hipStream_t s,
                                    long long *launches, Probe *probe);
 extern "C" int pf_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
+struct DiagArgs {
+    int nii, njj, nkk, nkh, nj2;
+    int level, terrain;
+    float dxk, dyk, dtime;
+    const float *dzk;
+    const float *app, *atp, *abp_in, *abp_bot;
+    const int *kk;
+    const uint8_t *adv;
+    float *uk, *vk, *wk, *dpm, *div;
+};
+extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStream_t s);
 
 namespace {
 
@@ -70,6 +81,7 @@ struct gral_pf_handle {
     int unreal[3] = {0, 0, 0};
     gral_pf_timing timing{};
     long long active = 0;           // active cells (advanced per SIMPLE iteration)
+    float diag_dtime = 0;           // time step of the last diagnostic post-pass
     // device
     Dom d{};
     Dom *dd = nullptr;              // device copy of d (a one-situation launch array)
@@ -409,10 +421,22 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
         abp[k] = (k > 1) ? area / (0.5f * (DZK[k - 1] + DZK[k])) : area / DZK[k];
         atp[k] = area / (0.5f * (DZK[k + 1] + DZK[k]));
     }
-    // sweep kernels: tiled wavefront (default) or one launch per anti-diagonal
+    // sweep kernels: strip-pipelined (default), tiled wavefront, or one launch
+    // per anti-diagonal (GRAL_PF_SWEEP = strip | tile | diag)
     const char *sw = std::getenv("GRAL_PF_SWEEP");
-    d.tiled = !(sw && std::strcmp(sw, "diag") == 0);
-    if (h->kadvmax + 2 > 128) d.tiled = 0;   // the tiled kernels hold at most 128 levels per column
+    d.tiled = 2;
+    if (sw && std::strcmp(sw, "tile") == 0) d.tiled = 1;
+    if (sw && std::strcmp(sw, "diag") == 0) d.tiled = 0;
+    if (h->kadvmax + 2 > 128) d.tiled = 0;   // the tiled and strip kernels hold at most 128 levels per column
+    {
+        // strip synchronisation: header + one block per sweep kind (pressure, U, V, W)
+        int nstrip = (std::max(h->nii, h->njj) + 15) / 16 + 1;
+        d.sync_stride = ((2 + nstrip) + 63) & ~63;
+        int *fs;
+        if ((rc = dalloc(h, &fs, (size_t)SYNC_BLOCK0 + 4 * (size_t)d.sync_stride))) return rc;
+        HIPCHK(h, hipMemsetAsync(fs, 0, sizeof(int) * ((size_t)SYNC_BLOCK0 + 4 * (size_t)d.sync_stride), h->stream));
+        d.sync = fs;
+    }
     d.planes = nullptr;
     d.ptab[0] = d.ptab[1] = d.ttab[0] = d.ttab[1] = nullptr;
     if (d.tiled) {
@@ -615,8 +639,12 @@ static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipSt
         std::vector<gral_pf_handle *> keep;
         bits.resize((size_t)chunk);
         for (auto *h : act) {
-            if (hipMemcpy(bits.data(), h->d.resid + (it0 - 1), sizeof(unsigned) * chunk, hipMemcpyDeviceToHost) != hipSuccess)
+            int serr = 0;
+            if (hipMemcpy(bits.data(), h->d.resid + (it0 - 1), sizeof(unsigned) * chunk, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(&serr, h->d.sync, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
                 return bad(GRAL_PF_EDEVICE, "residual download failed");
+            if (serr)
+                return bad(GRAL_PF_EDEVICE, "strip sweep: a wait for the previous strip timed out (results invalid)");
             for (int c = 0; c < chunk; ++c) {
                 float du;
                 std::memcpy(&du, &bits[c], 4);
@@ -720,7 +748,7 @@ extern "C" int gral_pf_solve(gral_pf_handle *h, const gral_pf_params *p, gral_pf
 extern "C" int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, float *wk, float *uks, float *vks,
                                        float *wks, float *dpm, float *dpmnew, float *turb, float *tdiss) {
     if (!h) return GRAL_PF_EINVAL;
-    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "download: call prepare first");
+    if (!h->prepared && !h->fields) return fail(h, GRAL_PF_ESTATE, "download: upload fields first");
     int rc = sync_host(h);
     if (rc) return rc;
     size_t n3 = h->n2() * (size_t)h->nkh;
@@ -884,6 +912,86 @@ extern "C" void gral_pf_batch_destroy(gral_pf_batch *b) {
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+}
+
+// DiagnosticFlowfield.Calculate (DiagnosticFlowfield.cs:24-436) and the final
+// W mass adjustment of the caller (MicroscaleFlat.cs:388-462 or
+// MicroscaleTerrain.cs:862-933) on the handle's current fields, full height.
+// Temporary device arrays hold the full columns; afterwards the solver's own
+// device fields are refreshed from the result.
+extern "C" int gral_pf_diagnostic(gral_pf_handle *h, int32_t flow_field_level, int32_t terrain) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (flow_field_level != 1 && flow_field_level != 2) return fail(h, GRAL_PF_EINVAL, "diagnostic: flow_field_level must be 1 or 2");
+    if (!h->masks || !h->fields) return fail(h, GRAL_PF_ESTATE, "diagnostic: masks and fields must be set first");
+    int rc = 0;
+    if (h->prepared && (rc = sync_host(h))) return rc;
+    const size_t n2 = h->n2(), nkh = (size_t)h->nkh, n3 = n2 * nkh;
+    if (h->dpm.size() != n3) h->dpm.assign(n3, 0.0f);
+    // DTIME = (float)(0.5*Math.Min(DXK,DYK)/Math.Sqrt(Pow2(UK)+Pow2(VK))) at (1,1,KADVMAX), :30-31
+    float u = h->uk[h->i3(1, 1, h->kadvmax)], v = h->vk[h->i3(1, 1, h->kadvmax)];
+    float s2 = u * u + v * v;
+    float dt = (float)(0.5 * (double)fmin_net(h->dxk, h->dyk) / std::sqrt((double)s2));
+    dt = fmin_net(dt, 5.0f);
+    std::vector<float> app(nkh, 0.0f), atp(nkh, 0.0f), abi(nkh, 0.0f), abb(nkh, 0.0f);
+    const float DXK = h->dxk, DYK = h->dyk;
+    const float *DZK = h->dzk.data();
+    for (int k = 1; k <= h->nkk - 1; ++k) {
+        app[k] = 2 * (DYK * DZK[k] / DXK + DXK * DZK[k] / DYK + DXK * DYK / DZK[k]);
+        abi[k] = DXK * DYK / (0.5f * (DZK[k - 1] + DZK[k]));
+        abb[k] = DXK * DYK / DZK[k];
+        atp[k] = (DXK * DYK) / (0.5f * (DZK[k + 1] + DZK[k]));
+    }
+    std::vector<int> kk32(n2);
+    for (size_t c = 0; c < n2; ++c) kk32[c] = h->kk[c];
+    std::vector<void *> tmp;
+    auto talloc = [&](void **p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes + 256) != hipSuccess) return fail(h, GRAL_PF_ENOMEM, "diagnostic: hipMalloc failed");
+        tmp.push_back(*p);
+        return 0;
+    };
+    auto cleanup = [&]() { for (void *p : tmp) (void)hipFree(p); };
+    DiagArgs g{};
+    g.nii = h->nii; g.njj = h->njj; g.nkk = h->nkk; g.nkh = (int)nkh; g.nj2 = h->njj + 2;
+    g.level = flow_field_level; g.terrain = terrain ? 1 : 0;
+    g.dxk = DXK; g.dyk = DYK; g.dtime = dt;
+    void *p[13];
+    size_t sz[13] = {nkh * 4, nkh * 4, nkh * 4, nkh * 4, nkh * 4, n2 * 4, n2, n3 * 4, n3 * 4, n3 * 4, n3 * 4, n3 * 4, 0};
+    for (int q = 0; q < 12; ++q)
+        if ((rc = talloc(&p[q], sz[q]))) { cleanup(); return rc; }
+    const void *src[12] = {h->dzk.data(), app.data(), atp.data(), abi.data(), abb.data(), kk32.data(), h->adv.data(),
+                           h->uk.data(), h->vk.data(), h->wk.data(), h->dpm.data(), nullptr};
+    for (int q = 0; q < 11; ++q)
+        if (hipMemcpyAsync(p[q], src[q], sz[q], hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+            cleanup();
+            return fail(h, GRAL_PF_EDEVICE, "diagnostic: upload failed");
+        }
+    if (hipMemsetAsync(p[11], 0, sz[11], h->stream) != hipSuccess) { cleanup(); return fail(h, GRAL_PF_EDEVICE, "diagnostic: memset failed"); }
+    g.dzk = (const float *)p[0]; g.app = (const float *)p[1]; g.atp = (const float *)p[2];
+    g.abp_in = (const float *)p[3]; g.abp_bot = (const float *)p[4];
+    g.kk = (const int *)p[5]; g.adv = (const uint8_t *)p[6];
+    g.uk = (float *)p[7]; g.vk = (float *)p[8]; g.wk = (float *)p[9]; g.dpm = (float *)p[10]; g.div = (float *)p[11];
+    int iters = (flow_field_level == 2) ? 10 : 5;
+    int e = pf_launch_diagnostic(&g, iters, h->stream);
+    if (e || hipStreamSynchronize(h->stream) != hipSuccess) {
+        cleanup();
+        return fail(h, GRAL_PF_EDEVICE, "diagnostic: kernel launch failed");
+    }
+    float *dst[4] = {h->uk.data(), h->vk.data(), h->wk.data(), h->dpm.data()};
+    for (int q = 0; q < 4; ++q)
+        if (hipMemcpy(dst[q], p[7 + q], n3 * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            cleanup();
+            return fail(h, GRAL_PF_EDEVICE, "diagnostic: download failed");
+        }
+    cleanup();
+    h->diag_dtime = dt;
+    h->host_stale = false;
+    if (h->prepared) {
+        // keep the solver's device state (levels below nkp) equal to the host's
+        if ((rc = put_field(h, h->d.uk, h->uk)) || (rc = put_field(h, h->d.vk, h->vk)) || (rc = put_field(h, h->d.wk, h->wk)) ||
+            (rc = put_field(h, h->d.dpm, h->dpm)))
+            return rc;
+    }
+    return GRAL_PF_OK;
 }
 
 extern "C" int gral_pf_last_timing(gral_pf_handle *h, gral_pf_timing *t) {

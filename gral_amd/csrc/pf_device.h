@@ -11,37 +11,73 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-struct Dom {
+// Pointer members are declared through a policy so that the same layout
+// serves the host (generic pointers) and the kernels (global-memory pointers,
+// which compile to global_* instead of flat_* memory instructions).
+struct GenericPtr { template <class T> using ptr = T *; };
+struct GlobalPtr { template <class T> using ptr = T __attribute__((address_space(1))) *; };
+
+template <class PP>
+struct DomT {
+    template <class T> using P = typename PP::template ptr<T>;
     int nii, njj, nkk, nkp, nj2, kadvmax;
     int model, simd;
     float dxk, dyk, area, dtime, relax, relaxp, cor, ug, vg, vishmin, visvmin, bz0, ufx, potdtdz;
     double frund;
     // per level
-    const float *dzk, *hok;          // [nkk+2]
-    const float *ap0;                // [nkp]  AP0[k] = DXK*DYK*DZK[k]/DTIME
-    const float *app, *abp, *atp;    // [nkp]  pressure TDMA coefficients (column independent)
+    P<const float> dzk, hok;         // [nkk+2]
+    P<const float> ap0;              // [nkp]  AP0[k] = DXK*DYK*DZK[k]/DTIME
+    P<const float> app, abp, atp;    // [nkp]  pressure TDMA coefficients (column independent)
     // per column
-    const uint8_t *adv, *vegp;
-    const int *kk, *vi;
-    const float *cutk, *cov, *ht, *ho, *z0s;
-    const double *tke_a, *tke_den;   // TKE terrain wall function: 0.4/ln(.) and TDISS divisor
-    const int *tke_br;               // 1: Z0 >= DZK*0.5 branch, 0: otherwise
+    P<const uint8_t> adv, vegp;
+    P<const int> kk, vi;
+    P<const float> cutk, cov, ht, ho, z0s;
+    P<const double> tke_a, tke_den;  // TKE terrain wall function: 0.4/ln(.) and TDISS divisor
+    P<const int> tke_br;             // 1: Z0 >= DZK*0.5 branch, 0: otherwise
     // fields
-    float *uk, *vk, *wk, *uks, *vks, *wks, *dpm, *dpmnew, *div, *turb, *tdiss;
-    const float *veg;
+    P<float> uk, vk, wk, uks, vks, wks, dpm, dpmnew, div, turb, tdiss;
+    P<const float> veg;
     // scratch
-    float *prow_p, *prow_q;          // pressure PIMP/QIMP per sweep row [nj2][nkp]
-    unsigned *resid;                 // DeltaUMax per iteration (float bits, values >= 0)
+    P<float> prow_p, prow_q;         // pressure PIMP/QIMP per sweep row [nj2][nkp]
+    P<unsigned> resid;               // DeltaUMax per iteration (float bits, values >= 0)
     int resid_cap;                   // entries in resid
-    // tiled sweeps
-    int tiled;                       // 1: tiled wavefront kernels, 0: one launch per anti-diagonal
-    float *planes;                   // [16][nii+2][njj+2][nkp] mixing-length coefficient planes
-    const float *ptab[2], *ttab[2];  // pressure PIMP/TERMP per column for IS=+1 / IS=-1
-    const int4 *meta;                // per column: KK, VI, KSTART | adv<<16 | vegp<<17, KKm_i | KKm_j<<16
+    // sweep kernels
+    int tiled;                       // 2: strip-pipelined, 1: tiled wavefront, 0: one launch per anti-diagonal
+    P<int> sync;                     // strip sweeps: [64] header (sticky error flag at 0), then 4 blocks
+    int sync_stride;                 //   of sync_stride ints (ticket, unused, progress per strip)
+    P<float> planes;                 // [cells][16] mixing-length coefficient records
+    P<const float> ptab[2], ttab[2]; // pressure PIMP/TERMP per column for IS=+1 / IS=-1
+    P<const int4> meta;              // per column: KK, VI, KSTART | adv<<16 | vegp<<17, KKm_i | KKm_j<<16
 };
+using Dom = DomT<GenericPtr>;        // host side, and the descriptor array handed to the kernels
+using DomG = DomT<GlobalPtr>;        // the kernels' view of the same bytes
+static_assert(sizeof(Dom) == sizeof(DomG), "descriptor layouts must agree");
+typedef float __attribute__((address_space(1))) gf32;
+typedef int __attribute__((address_space(1))) gi32;
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef v4f __attribute__((address_space(1))) gv4f;
+typedef v4i __attribute__((address_space(1))) gv4i;
 
-__device__ __forceinline__ size_t c2(const Dom &d, int i, int j) { return (size_t)i * (size_t)d.nj2 + (size_t)j; }
-__device__ __forceinline__ size_t c3(const Dom &d, int i, int j) { return c2(d, i, j) * (size_t)d.nkp; }
+// 16-byte loads and stores through global-memory pointers (p 16-byte aligned)
+__device__ __forceinline__ float4 ldg4(const gf32 *p) {
+    v4f v = *(const gv4f *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void stg4(gf32 *p, float a, float b, float c, float e) {
+    v4f v = {a, b, c, e};
+    *(gv4f *)p = v;
+}
+template <class T>
+__device__ __forceinline__ int4 ldg4i(const T *p) {
+    v4i v = *(const gv4i *)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
+#define SYNC_BLOCK0 64               // Dom::sync: sticky error flag at 0, sweep blocks from here
+
+__device__ __forceinline__ size_t c2(const DomG &d, int i, int j) { return (size_t)i * (size_t)d.nj2 + (size_t)j; }
+__device__ __forceinline__ size_t c3(const DomG &d, int i, int j) { return c2(d, i, j) * (size_t)d.nkp; }
 
 // Vector.Max / Vector.Min / FloatMax of the reference: SSE maxps/minps semantics.
 __device__ __forceinline__ float vmaxf(float a, float b) { return a > b ? a : b; }

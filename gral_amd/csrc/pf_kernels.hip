@@ -41,14 +41,16 @@ __global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
 }
 // zero the pressure row scratch of every situation (PIMP/QIMP rows are fresh
 // arrays at the start of each pressure pass)
+// and the strip synchronisation blocks of the iteration's sweeps
 __global__ void k_zero_rows(const Dom *__restrict__ ds, int both) {
-    const Dom &d = ds[blockIdx.z];
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];
     size_t n = (size_t)d.nj2 * d.nkp;
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) {
         d.prow_q[g] = 0.0f;
         if (both) d.prow_p[g] = 0.0f;
     }
+    if (d.sync && g < (size_t)4 * d.sync_stride) d.sync[SYNC_BLOCK0 + g] = 0;
 }
 
 // ------------------------------------------------------------------------
@@ -57,7 +59,7 @@ __global__ void k_zero_rows(const Dom *__restrict__ ds, int both) {
 // E/W: each row j is independent; within a row the reference scans i upwards
 // and later i read what earlier i wrote, so one thread per (j,k) scans i.
 __global__ void k_bc_ew(const Dom *__restrict__ ds) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     if ((int)blockIdx.y >= d.njj - 1) return;
 
     int j = blockIdx.y + 2;                   // 2..NJJ
@@ -85,7 +87,7 @@ __global__ void k_bc_ew(const Dom *__restrict__ ds) {
 }
 
 __global__ void k_bc_ns(const Dom *__restrict__ ds) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     if ((int)blockIdx.y >= d.nii - 1) return;
 
     int i = blockIdx.y + 2;                   // 2..NII
@@ -114,7 +116,7 @@ __global__ void k_bc_ns(const Dom *__restrict__ ds) {
 
 // top copy VI -> VI+1 (thread per column i in [2,NII-1], j in [2,NJJ-1])
 __global__ void k_bc_top(const Dom *__restrict__ ds) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
 
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     int ni = d.nii - 2, nj = d.njj - 2;
@@ -133,7 +135,7 @@ __global__ void k_bc_top(const Dom *__restrict__ ds) {
 // Mass divergence, PrognosticFlowfield.cs:723-758.  Block = one column.
 // ------------------------------------------------------------------------
 __global__ void k_divergence(const Dom *__restrict__ ds) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     if ((int)blockIdx.x >= (d.nii - 2) * (d.njj - 2)) return;
 
     int nj = d.njj - 2;
@@ -162,7 +164,7 @@ __global__ void k_divergence(const Dom *__restrict__ ds) {
 // the same row left there, as in the reference.
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(WAVE) void k_pressure(const Dom *__restrict__ ds, int t, int IS, int JS, int slot) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
 
     extern __shared__ float sm[];
     int a, b;
@@ -218,14 +220,14 @@ __global__ __launch_bounds__(WAVE) void k_pressure(const Dom *__restrict__ ds, i
         float other = __shfl_xor(rmax, off);
         rmax = other > rmax ? other : rmax;
     }
-    if (tid == 0 && rmax > 0.0f && slot >= 0 && slot < d.resid_cap) atomicMax(&d.resid[slot], __float_as_uint(rmax));
+    if (tid == 0 && rmax > 0.0f && slot >= 0 && slot < d.resid_cap) atomicMax((unsigned *)&d.resid[slot], __float_as_uint(rmax));
 }
 
 // ------------------------------------------------------------------------
 // Velocity correction with FP64 quantisation, PrognosticFlowfield.cs:904-949
 // ------------------------------------------------------------------------
 __global__ void k_correction(const Dom *__restrict__ ds) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     if ((int)blockIdx.x >= (d.nii - 2) * (d.njj - 2)) return;
 
     int nj = d.njj - 2;
@@ -257,7 +259,7 @@ __global__ void k_correction(const Dom *__restrict__ ds) {
 // Cell-centred averages, PrognosticFlowfield.cs:958-1014 (+ UKS row copies)
 // ------------------------------------------------------------------------
 __global__ void k_averages(const Dom *__restrict__ ds) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     if ((int)blockIdx.x >= (d.nii - 2) * (d.njj - 2)) return;
 
     int nj = d.njj - 2;
@@ -283,7 +285,7 @@ __global__ void k_averages(const Dom *__restrict__ ds) {
 
 // VKS[1][j] = VKS[2][j], VKS[NII][j] = VKS[NII-1][j] (PrognosticFlowfield.cs:1019-1039)
 __global__ void k_avg_edges(const Dom *__restrict__ ds) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     if ((int)blockIdx.x >= d.njj - 2) return;
 
     int j = blockIdx.x + 2;                  // 2..NJJ-1
@@ -320,7 +322,7 @@ struct ColInfo {
     float cutk, cov, ht, ho, z0s;
 };
 
-__device__ __forceinline__ ColInfo col_info(const Dom &d, int i, int j) {
+__device__ __forceinline__ ColInfo col_info(const DomG &d, int i, int j) {
     ColInfo q;
     size_t c = c2(d, i, j);
     q.i = i; q.j = j; q.KK = d.kk[c]; q.VI = d.vi[c];
@@ -336,7 +338,7 @@ __device__ __forceinline__ ColInfo col_info(const Dom &d, int i, int j) {
 }
 
 // mixing-length eddy viscosity (U-prognostic-microscale_1.cs:216-268), lane-local
-__device__ __forceinline__ float ml_visc(const Dom &d, const ColInfo &q, int k, const float *UK, const float *VK) {
+__device__ __forceinline__ float ml_visc(const DomG &d, const ColInfo &q, int k, const gf32 *UK, const gf32 *VK) {
     if (!(k > q.KK + 1)) return 0.0f;
     float dz = d.dzk[k];
     float zs = d.hok[k] - d.hok[q.KK] - dz * 0.5f;
@@ -359,7 +361,7 @@ struct Coef { float A, B, C, D; };
 
 // Diffusion, advection and power-law coefficients shared by the three
 // mixing-length solvers.  F* are the face fluxes of the component.
-__device__ __forceinline__ void ml_coeffs(const Dom &d, const ColInfo &q, int k, float VIS, float FE, float FW,
+__device__ __forceinline__ void ml_coeffs(const DomG &d, const ColInfo &q, int k, float VIS, float FE, float FW,
                                           float FS, float FN, float FT, float FB, float &AW1, float &AS1,
                                           float &AE1, float &AN1, float &BIM, float &CIM, float &AIM) {
     float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
@@ -383,8 +385,8 @@ __device__ __forceinline__ void ml_coeffs(const Dom &d, const ColInfo &q, int k,
 }
 
 // wall function at k = KKART+1 for U and V (U-prognostic-microscale_1.cs:419-450)
-__device__ __forceinline__ float ml_wall_uv(const Dom &d, const ColInfo &q, int k, float vel, const float *UKS,
-                                            const float *UKS_W, const float *VKS, const float *VKS_W) {
+__device__ __forceinline__ float ml_wall_uv(const DomG &d, const ColInfo &q, int k, float vel, const gf32 *UKS,
+                                            const gf32 *UKS_W, const gf32 *VKS, const gf32 *VKS_W) {
     float in = 0.5f * (UKS_W[k] + UKS[k]);
     float in2 = 0.5f * (VKS_W[k] + VKS[k]);
     float wh = vmaxf(sqrtf(in * in + in2 * in2), 0.1f);
@@ -419,17 +421,17 @@ __device__ __forceinline__ bool ml_cross_on(const ColInfo &q, int k) {
     return block_base(k, q.KSTART, q.VI + 1, q.W) > q.KK + 1;
 }
 
-__device__ MlPieces u1_pieces(const Dom &d, const ColInfo &q, int k) {
+__device__ MlPieces u1_pieces(const DomG &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
-    const float *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j);
-    const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
-    const float *UKS_W = d.uks + c3(d, i - 1, j);
-    const float *VKS_W = d.vks + c3(d, i - 1, j), *VKS_WS = d.vks + c3(d, i - 1, j - 1);
-    const float *VKS_S = d.vks + c3(d, i, j - 1), *VKS_WN = d.vks + c3(d, i - 1, j + 1);
-    const float *VKS_N = d.vks + c3(d, i, j + 1), *VKS_EN = d.vks + c3(d, i + 1, j + 1);
-    const float *VKS_ES = d.vks + c3(d, i + 1, j - 1);
-    const float *WKS_W = d.wks + c3(d, i - 1, j), *WK_E = d.wk + c3(d, i + 1, j);
-    const float *DPMNEW = d.dpmnew + c3(d, i, j), *DPMNEW_W = d.dpmnew + c3(d, i - 1, j);
+    const gf32 *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j);
+    const gf32 *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    const gf32 *UKS_W = d.uks + c3(d, i - 1, j);
+    const gf32 *VKS_W = d.vks + c3(d, i - 1, j), *VKS_WS = d.vks + c3(d, i - 1, j - 1);
+    const gf32 *VKS_S = d.vks + c3(d, i, j - 1), *VKS_WN = d.vks + c3(d, i - 1, j + 1);
+    const gf32 *VKS_N = d.vks + c3(d, i, j + 1), *VKS_EN = d.vks + c3(d, i + 1, j + 1);
+    const gf32 *VKS_ES = d.vks + c3(d, i + 1, j - 1);
+    const gf32 *WKS_W = d.wks + c3(d, i - 1, j), *WK_E = d.wk + c3(d, i + 1, j);
+    const gf32 *DPMNEW = d.dpmnew + c3(d, i, j), *DPMNEW_W = d.dpmnew + c3(d, i - 1, j);
     float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
     float VIS = ml_visc(d, q, k, UK, VK);
     float FE = UKS[k] * dykdzk;
@@ -462,17 +464,17 @@ __device__ MlPieces u1_pieces(const Dom &d, const ColInfo &q, int k) {
     return o;
 }
 
-__device__ MlPieces v1_pieces(const Dom &d, const ColInfo &q, int k) {
+__device__ MlPieces v1_pieces(const DomG &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
-    const float *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j);
-    const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
-    const float *UKS_W = d.uks + c3(d, i - 1, j), *UKS_E = d.uks + c3(d, i + 1, j);
-    const float *UKS_S = d.uks + c3(d, i, j - 1), *UKS_ES = d.uks + c3(d, i + 1, j - 1);
-    const float *UKS_WS = d.uks + c3(d, i - 1, j - 1), *UKS_EN = d.uks + c3(d, i + 1, j + 1);
-    const float *UKS_WN = d.uks + c3(d, i - 1, j + 1);
-    const float *VKS_S = d.vks + c3(d, i, j - 1), *VKS_W = d.vks + c3(d, i - 1, j);
-    const float *WKS_S = d.wks + c3(d, i, j - 1), *WK_N = d.wk + c3(d, i, j + 1);
-    const float *DPMNEW = d.dpmnew + c3(d, i, j), *DPMNEW_S = d.dpmnew + c3(d, i, j - 1);
+    const gf32 *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j);
+    const gf32 *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    const gf32 *UKS_W = d.uks + c3(d, i - 1, j), *UKS_E = d.uks + c3(d, i + 1, j);
+    const gf32 *UKS_S = d.uks + c3(d, i, j - 1), *UKS_ES = d.uks + c3(d, i + 1, j - 1);
+    const gf32 *UKS_WS = d.uks + c3(d, i - 1, j - 1), *UKS_EN = d.uks + c3(d, i + 1, j + 1);
+    const gf32 *UKS_WN = d.uks + c3(d, i - 1, j + 1);
+    const gf32 *VKS_S = d.vks + c3(d, i, j - 1), *VKS_W = d.vks + c3(d, i - 1, j);
+    const gf32 *WKS_S = d.wks + c3(d, i, j - 1), *WK_N = d.wk + c3(d, i, j + 1);
+    const gf32 *DPMNEW = d.dpmnew + c3(d, i, j), *DPMNEW_S = d.dpmnew + c3(d, i, j - 1);
     float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
     float VIS = ml_visc(d, q, k, UK, VK);
     float FE = 0.25f * (UKS[k] + UKS_E[k] + UKS_S[k] + UKS_ES[k]) * dykdzk;
@@ -506,13 +508,13 @@ __device__ MlPieces v1_pieces(const Dom &d, const ColInfo &q, int k) {
     return o;
 }
 
-__device__ MlPieces w1_pieces(const Dom &d, const ColInfo &q, int k) {
+__device__ MlPieces w1_pieces(const DomG &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
-    const float *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j), *WK = d.wk + c3(d, i, j);
-    const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
-    const float *UKS_W = d.uks + c3(d, i - 1, j), *UKS_E = d.uks + c3(d, i + 1, j);
-    const float *VKS_S = d.vks + c3(d, i, j - 1), *VKS_N = d.vks + c3(d, i, j + 1);
-    const float *DPMNEW = d.dpmnew + c3(d, i, j);
+    const gf32 *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j), *WK = d.wk + c3(d, i, j);
+    const gf32 *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    const gf32 *UKS_W = d.uks + c3(d, i - 1, j), *UKS_E = d.uks + c3(d, i + 1, j);
+    const gf32 *VKS_S = d.vks + c3(d, i, j - 1), *VKS_N = d.vks + c3(d, i, j + 1);
+    const gf32 *DPMNEW = d.dpmnew + c3(d, i, j);
     float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
     float VIS = ml_visc(d, q, k, UK, VK);
     float FE = 0.25f * (UKS[k] + UKS_E[k] + UKS[k - 1] + UKS_E[k - 1]) * dykdzk;
@@ -552,7 +554,7 @@ __device__ MlPieces w1_pieces(const Dom &d, const ColInfo &q, int k) {
 // D of one cell from its pieces and the four neighbour values (and the cell's
 // own old value x), in the reference's order of operations.
 template <int COMP>
-__device__ __forceinline__ float ml_source(const Dom &d, const ColInfo &q, int k, float dz, const MlPieces &o, float xw,
+__device__ __forceinline__ float ml_source(const DomG &d, const ColInfo &q, int k, float dz, const MlPieces &o, float xw,
                                            float xs, float xe, float xn, float x) {
     float D = o.AW1 * xw + o.AS1 * xs + o.AE1 * xe + o.AN1 * xn + o.e + o.f;
     if (COMP != 2) {
@@ -579,7 +581,7 @@ __device__ __forceinline__ float ml_source(const Dom &d, const ColInfo &q, int k
 }
 
 template <int COMP>
-__device__ __forceinline__ MlPieces ml_pieces(const Dom &d, const ColInfo &q, int k) {
+__device__ __forceinline__ MlPieces ml_pieces(const DomG &d, const ColInfo &q, int k) {
     if (COMP == 0) return u1_pieces(d, q, k);
     if (COMP == 1) return v1_pieces(d, q, k);
     return w1_pieces(d, q, k);
@@ -587,9 +589,9 @@ __device__ __forceinline__ MlPieces ml_pieces(const Dom &d, const ColInfo &q, in
 
 // neighbour values X_W, X_S, X_E, X_N of the swept component at level k
 template <int COMP>
-__device__ __forceinline__ void ml_neighbours(const Dom &d, int i, int j, int k, float &xw, float &xs, float &xe,
+__device__ __forceinline__ void ml_neighbours(const DomG &d, int i, int j, int k, float &xw, float &xs, float &xe,
                                               float &xn, float &x) {
-    const float *X = COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk;
+    const gf32 *X = COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk;
     xw = X[c3(d, i - 1, j) + k];
     xe = X[c3(d, i + 1, j) + k];
     xs = X[c3(d, i, j - 1) + k];
@@ -598,15 +600,15 @@ __device__ __forceinline__ void ml_neighbours(const Dom &d, int i, int j, int k,
 }
 
 template <int COMP>
-__device__ Coef ml_cell(const Dom &d, const ColInfo &q, int k) {
+__device__ Coef ml_cell(const DomG &d, const ColInfo &q, int k) {
     MlPieces o = ml_pieces<COMP>(d, q, k);
     float xw, xs, xe, xn, x;
     ml_neighbours<COMP>(d, q.i, q.j, k, xw, xs, xe, xn, x);
     return Coef{o.A, o.B, o.C, ml_source<COMP>(d, q, k, d.dzk[k], o, xw, xs, xe, xn, x)};
 }
-__device__ Coef u1_cell(const Dom &d, const ColInfo &q, int k) { return ml_cell<0>(d, q, k); }
-__device__ Coef v1_cell(const Dom &d, const ColInfo &q, int k) { return ml_cell<1>(d, q, k); }
-__device__ Coef w1_cell(const Dom &d, const ColInfo &q, int k) { return ml_cell<2>(d, q, k); }
+__device__ Coef u1_cell(const DomG &d, const ColInfo &q, int k) { return ml_cell<0>(d, q, k); }
+__device__ Coef v1_cell(const DomG &d, const ColInfo &q, int k) { return ml_cell<1>(d, q, k); }
+__device__ Coef w1_cell(const DomG &d, const ColInfo &q, int k) { return ml_cell<2>(d, q, k); }
 
 // ------------------------------------------------------------------------
 // k-eps (model 2) and no-diffusion (model 0) momentum, _2.cs / _0.cs
@@ -617,8 +619,8 @@ __device__ __forceinline__ float plaw_d(float D, float P) {   // D*FloatMax(0,(f
 }
 
 // wall function of the scalar U/V solvers (U-prognostic-microscale_2.cs:188-212)
-__device__ __forceinline__ float ke_wall_uv(const Dom &d, const ColInfo &q, int k, float vel, const float *UKSim,
-                                            const float *UKS, const float *VKSim, const float *VKS) {
+__device__ __forceinline__ float ke_wall_uv(const DomG &d, const ColInfo &q, int k, float vel, const gf32 *UKSim,
+                                            const gf32 *UKS, const gf32 *VKSim, const gf32 *VKS) {
     double a = 0.5 * (double)(UKSim[k] + UKS[k]);
     double b = 0.5 * (double)(VKSim[k] + VKS[k]);
     float wh = vmaxf((float)sqrt(a * a + b * b), 0.01f);
@@ -637,13 +639,13 @@ __device__ __forceinline__ float ke_wall_uv(const Dom &d, const ColInfo &q, int 
 }
 
 template <bool IS_V, bool DIFF>
-__device__ Coef ke_uv_cell(const Dom &d, const ColInfo &q, int k) {
+__device__ Coef ke_uv_cell(const DomG &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
-    const float *X = (IS_V ? d.vk : d.uk) + c3(d, i, j);
-    const float *Xim = (IS_V ? d.vk : d.uk) + c3(d, i - 1, j), *Xip = (IS_V ? d.vk : d.uk) + c3(d, i + 1, j);
-    const float *Xjm = (IS_V ? d.vk : d.uk) + c3(d, i, j - 1), *Xjp = (IS_V ? d.vk : d.uk) + c3(d, i, j + 1);
-    const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
-    const float *UKSim = d.uks + c3(d, i - 1, j), *VKSim = d.vks + c3(d, i - 1, j);
+    const gf32 *X = (IS_V ? d.vk : d.uk) + c3(d, i, j);
+    const gf32 *Xim = (IS_V ? d.vk : d.uk) + c3(d, i - 1, j), *Xip = (IS_V ? d.vk : d.uk) + c3(d, i + 1, j);
+    const gf32 *Xjm = (IS_V ? d.vk : d.uk) + c3(d, i, j - 1), *Xjp = (IS_V ? d.vk : d.uk) + c3(d, i, j + 1);
+    const gf32 *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    const gf32 *UKSim = d.uks + c3(d, i - 1, j), *VKSim = d.vks + c3(d, i - 1, j);
     float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
     float VIS = 0, DE = 0, DW = 0, DS = 0, DN = 0, DT = 0, DB = 0;
     if (DIFF) {
@@ -657,9 +659,9 @@ __device__ Coef ke_uv_cell(const Dom &d, const ColInfo &q, int k) {
     }
     float FE, FW, FS, FN, FT, FB = 0;
     if (!IS_V) {
-        const float *VKSimjm = d.vks + c3(d, i - 1, j - 1), *VKSjm = d.vks + c3(d, i, j - 1);
-        const float *VKSimjp = d.vks + c3(d, i - 1, j + 1), *VKSjp = d.vks + c3(d, i, j + 1);
-        const float *WKSim = d.wks + c3(d, i - 1, j);
+        const gf32 *VKSimjm = d.vks + c3(d, i - 1, j - 1), *VKSjm = d.vks + c3(d, i, j - 1);
+        const gf32 *VKSimjp = d.vks + c3(d, i - 1, j + 1), *VKSjp = d.vks + c3(d, i, j + 1);
+        const gf32 *WKSim = d.wks + c3(d, i - 1, j);
         FE = UKS[k] * dykdzk;
         FW = UKSim[k] * dykdzk;
         FS = 0.25f * (VKSim[k] + VKS[k] + VKSimjm[k] + VKSjm[k]) * dxkdzk;
@@ -667,9 +669,9 @@ __device__ Coef ke_uv_cell(const Dom &d, const ColInfo &q, int k) {
         FT = 0.25f * (WKSim[k] + WKS[k] + WKSim[k + 1] + WKS[k + 1]) * d.area;
         if (k > q.KK + 1) FB = 0.25f * (WKSim[k] + WKS[k] + WKSim[k - 1] + WKS[k - 1]) * d.area;
     } else {
-        const float *UKSip = d.uks + c3(d, i + 1, j), *UKSjm = d.uks + c3(d, i, j - 1);
-        const float *UKSipjm = d.uks + c3(d, i + 1, j - 1), *UKSimjm = d.uks + c3(d, i - 1, j - 1);
-        const float *VKSjm = d.vks + c3(d, i, j - 1), *WKSjm = d.wks + c3(d, i, j - 1);
+        const gf32 *UKSip = d.uks + c3(d, i + 1, j), *UKSjm = d.uks + c3(d, i, j - 1);
+        const gf32 *UKSipjm = d.uks + c3(d, i + 1, j - 1), *UKSimjm = d.uks + c3(d, i - 1, j - 1);
+        const gf32 *VKSjm = d.vks + c3(d, i, j - 1), *WKSjm = d.wks + c3(d, i, j - 1);
         FE = 0.25f * (UKS[k] + UKSip[k] + UKSjm[k] + UKSipjm[k]) * dykdzk;
         FW = 0.25f * (UKS[k] + UKSim[k] + UKSjm[k] + UKSimjm[k]) * dykdzk;
         FS = VKSjm[k] * dxkdzk;
@@ -695,8 +697,8 @@ __device__ Coef ke_uv_cell(const Dom &d, const ColInfo &q, int k) {
     }
     float ap0 = d.ap0[k];
     float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + ap0;
-    const float *DPMNEW = d.dpmnew + c3(d, i, j);
-    const float *DPMNEWm = d.dpmnew + (IS_V ? c3(d, i, j - 1) : c3(d, i - 1, j));
+    const gf32 *DPMNEW = d.dpmnew + c3(d, i, j);
+    const gf32 *DPMNEWm = d.dpmnew + (IS_V ? c3(d, i, j - 1) : c3(d, i - 1, j));
     float ddp = DPMNEWm[k] - DPMNEW[k];
     float x = X[k];
     float partial = AW1 * Xim[k] + AS1 * Xjm[k] + AE1 * Xip[k] + AN1 * Xjp[k];
@@ -709,10 +711,10 @@ __device__ Coef ke_uv_cell(const Dom &d, const ColInfo &q, int k) {
     if (DIFF && k > q.KK + 1) {
         float ADD;
         if (!IS_V) {
-            const float *VKSipjp = d.vks + c3(d, i + 1, j + 1), *VKSjp = d.vks + c3(d, i, j + 1);
-            const float *VKSimjp = d.vks + c3(d, i - 1, j + 1), *VKSipjm = d.vks + c3(d, i + 1, j - 1);
-            const float *VKSjm = d.vks + c3(d, i, j - 1), *VKSimjm = d.vks + c3(d, i - 1, j - 1);
-            const float *WKip = d.wk + c3(d, i + 1, j), *WKSim = d.wks + c3(d, i - 1, j);
+            const gf32 *VKSipjp = d.vks + c3(d, i + 1, j + 1), *VKSjp = d.vks + c3(d, i, j + 1);
+            const gf32 *VKSimjp = d.vks + c3(d, i - 1, j + 1), *VKSipjm = d.vks + c3(d, i + 1, j - 1);
+            const gf32 *VKSjm = d.vks + c3(d, i, j - 1), *VKSimjm = d.vks + c3(d, i - 1, j - 1);
+            const gf32 *WKip = d.wk + c3(d, i + 1, j), *WKSim = d.wks + c3(d, i - 1, j);
             float DUDXE = (Xip[k] - x) * dykdzk;
             float DUDXW = (x - Xim[k]) * dykdzk;
             float DVDXN = (0.5f * (VKSipjp[k] + VKSjp[k]) - 0.5f * (VKSjp[k] + VKSimjp[k])) * dxkdzk;
@@ -721,10 +723,10 @@ __device__ Coef ke_uv_cell(const Dom &d, const ColInfo &q, int k) {
             float DWDXB = (0.5f * (WKip[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKSim[k - 1])) * d.area;
             ADD = VIS / d.dxk * (DUDXE - DUDXW + DVDXN - DVDXS + DWDXT - DWDXB);
         } else {
-            const float *UKSipjp = d.uks + c3(d, i + 1, j + 1), *UKSip = d.uks + c3(d, i + 1, j);
-            const float *UKSipjm = d.uks + c3(d, i + 1, j - 1), *UKSimjp = d.uks + c3(d, i - 1, j + 1);
-            const float *UKSimjm = d.uks + c3(d, i - 1, j - 1);
-            const float *WKjp = d.wk + c3(d, i, j + 1), *WKSjm = d.wks + c3(d, i, j - 1);
+            const gf32 *UKSipjp = d.uks + c3(d, i + 1, j + 1), *UKSip = d.uks + c3(d, i + 1, j);
+            const gf32 *UKSipjm = d.uks + c3(d, i + 1, j - 1), *UKSimjp = d.uks + c3(d, i - 1, j + 1);
+            const gf32 *UKSimjm = d.uks + c3(d, i - 1, j - 1);
+            const gf32 *WKjp = d.wk + c3(d, i, j + 1), *WKSjm = d.wks + c3(d, i, j - 1);
             float DVDYN = (Xjp[k] - x) * dxkdzk;
             float DVDYS = (x - Xjm[k]) * dxkdzk;
             float DUDYE = (0.5f * (UKSipjp[k] + UKSip[k]) - 0.5f * (UKSip[k] + UKSipjm[k])) * dxkdzk;
@@ -739,15 +741,15 @@ __device__ Coef ke_uv_cell(const Dom &d, const ColInfo &q, int k) {
 }
 
 template <bool DIFF>
-__device__ Coef ke_w_cell(const Dom &d, const ColInfo &q, int k) {
+__device__ Coef ke_w_cell(const DomG &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
-    const float *WK = d.wk + c3(d, i, j);
-    const float *WKim = d.wk + c3(d, i - 1, j), *WKip = d.wk + c3(d, i + 1, j);
-    const float *WKjm = d.wk + c3(d, i, j - 1), *WKjp = d.wk + c3(d, i, j + 1);
-    const float *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
-    const float *UKSim = d.uks + c3(d, i - 1, j), *UKSip = d.uks + c3(d, i + 1, j);
-    const float *VKSjm = d.vks + c3(d, i, j - 1), *VKSjp = d.vks + c3(d, i, j + 1);
-    const float *DPMNEW = d.dpmnew + c3(d, i, j);
+    const gf32 *WK = d.wk + c3(d, i, j);
+    const gf32 *WKim = d.wk + c3(d, i - 1, j), *WKip = d.wk + c3(d, i + 1, j);
+    const gf32 *WKjm = d.wk + c3(d, i, j - 1), *WKjp = d.wk + c3(d, i, j + 1);
+    const gf32 *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    const gf32 *UKSim = d.uks + c3(d, i - 1, j), *UKSip = d.uks + c3(d, i + 1, j);
+    const gf32 *VKSjm = d.vks + c3(d, i, j - 1), *VKSjp = d.vks + c3(d, i, j + 1);
+    const gf32 *DPMNEW = d.dpmnew + c3(d, i, j);
     float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
     float uks = UKS[k], uks_m = UKS[k - 1], vks = VKS[k], vks_m = VKS[k - 1];
     float VIS = 0, DE = 0, DW = 0, DS = 0, DN = 0, DT = 0, DB = 0;
@@ -806,7 +808,7 @@ __device__ Coef ke_w_cell(const Dom &d, const ColInfo &q, int k) {
 // ------------------------------------------------------------------------
 template <int COMP, int KIND>
 __global__ __launch_bounds__(WAVE) void k_momentum(const Dom *__restrict__ ds, int t, int IS, int JS) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
 
     extern __shared__ float sm[];
     const int nkp = d.nkp, tid = threadIdx.x;
@@ -819,7 +821,7 @@ __global__ __launch_bounds__(WAVE) void k_momentum(const Dom *__restrict__ ds, i
     if (d.adv[c2(d, i, j)] != 1) return;
     ColInfo q = col_info(d, i, j);
     float *sA = sm, *sB = sA + nkp, *sC = sB + nkp, *sD = sC + nkp, *sX = sD + nkp;
-    float *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk) + c3(d, i, j);
+    gf32 *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk) + c3(d, i, j);
     for (int k = q.KSTART + tid; k <= q.VI; k += WAVE) {
         Coef cf;
         if (KIND == 0) cf = (COMP == 0) ? u1_cell(d, q, k) : (COMP == 1) ? v1_cell(d, q, k) : w1_cell(d, q, k);
@@ -869,7 +871,7 @@ __global__ __launch_bounds__(WAVE) void k_momentum(const Dom *__restrict__ ds, i
 // TKE / dissipation (k-eps), TKE-prognostic-microscale.cs:25-352
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t, int IS, int JS) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
 
     extern __shared__ float sm[];
     const int nkp = d.nkp, tid = threadIdx.x;
@@ -886,18 +888,18 @@ __global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t,
     const float DXK = d.dxk, DYK = d.dyk, AREA = d.area, DXK_Rez = 1 / DXK, DYK_Rez = 1 / DYK;
     const float Ceps = 1, Ceps1 = 1.44f, Ceps2 = 1.92f;
     size_t o = c3(d, i, j);
-    const float *TURB = d.turb + o, *TDISS = d.tdiss + o;
-    const float *TURBim = d.turb + c3(d, i - 1, j), *TURBip = d.turb + c3(d, i + 1, j);
-    const float *TURBjm = d.turb + c3(d, i, j - 1), *TURBjp = d.turb + c3(d, i, j + 1);
-    const float *TDISSim = d.tdiss + c3(d, i - 1, j), *TDISSip = d.tdiss + c3(d, i + 1, j);
-    const float *TDISSjm = d.tdiss + c3(d, i, j - 1), *TDISSjp = d.tdiss + c3(d, i, j + 1);
-    const float *UK = d.uk + o, *VK = d.vk + o, *WK = d.wk + o;
-    const float *UKip = d.uk + c3(d, i + 1, j), *UKjp = d.uk + c3(d, i, j + 1), *UKipjp = d.uk + c3(d, i + 1, j + 1);
-    const float *UKjm = d.uk + c3(d, i, j - 1), *UKipjm = d.uk + c3(d, i + 1, j - 1);
-    const float *VKjp = d.vk + c3(d, i, j + 1), *VKip = d.vk + c3(d, i + 1, j), *VKipjp = d.vk + c3(d, i + 1, j + 1);
-    const float *VKim = d.vk + c3(d, i - 1, j), *VKimjp = d.vk + c3(d, i - 1, j + 1);
-    const float *WKip = d.wk + c3(d, i + 1, j), *WKim = d.wk + c3(d, i - 1, j);
-    const float *WKjp = d.wk + c3(d, i, j + 1), *WKjm = d.wk + c3(d, i, j - 1);
+    const gf32 *TURB = d.turb + o, *TDISS = d.tdiss + o;
+    const gf32 *TURBim = d.turb + c3(d, i - 1, j), *TURBip = d.turb + c3(d, i + 1, j);
+    const gf32 *TURBjm = d.turb + c3(d, i, j - 1), *TURBjp = d.turb + c3(d, i, j + 1);
+    const gf32 *TDISSim = d.tdiss + c3(d, i - 1, j), *TDISSip = d.tdiss + c3(d, i + 1, j);
+    const gf32 *TDISSjm = d.tdiss + c3(d, i, j - 1), *TDISSjp = d.tdiss + c3(d, i, j + 1);
+    const gf32 *UK = d.uk + o, *VK = d.vk + o, *WK = d.wk + o;
+    const gf32 *UKip = d.uk + c3(d, i + 1, j), *UKjp = d.uk + c3(d, i, j + 1), *UKipjp = d.uk + c3(d, i + 1, j + 1);
+    const gf32 *UKjm = d.uk + c3(d, i, j - 1), *UKipjm = d.uk + c3(d, i + 1, j - 1);
+    const gf32 *VKjp = d.vk + c3(d, i, j + 1), *VKip = d.vk + c3(d, i + 1, j), *VKipjp = d.vk + c3(d, i + 1, j + 1);
+    const gf32 *VKim = d.vk + c3(d, i - 1, j), *VKimjp = d.vk + c3(d, i - 1, j + 1);
+    const gf32 *WKip = d.wk + c3(d, i + 1, j), *WKim = d.wk + c3(d, i - 1, j);
+    const gf32 *WKjp = d.wk + c3(d, i, j + 1), *WKjm = d.wk + c3(d, i, j - 1);
     int KK = q.KK;
     int kkip = d.kk[c2(d, i + 1, j)], kkim = d.kk[c2(d, i - 1, j)], kkjp = d.kk[c2(d, i, j + 1)], kkjm = d.kk[c2(d, i, j - 1)];
     bool cip = d.cutk[c2(d, i + 1, j)] > 0, cim = d.cutk[c2(d, i - 1, j)] > 0;
@@ -1036,7 +1038,7 @@ __global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t,
         }
     }
     __syncthreads();
-    float *TW = d.turb + o, *EW = d.tdiss + o;
+    gf32 *TW = d.turb + o, *EW = d.tdiss + o;
     for (int k = q.KSTART + tid; k <= q.VI; k += WAVE) { TW[k] = sT[k]; EW[k] = sE[k]; }
 }
 
@@ -1060,7 +1062,7 @@ __global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t,
 // array of structures: 16 floats per cell, in the order of the four float4 loads
 enum { PL_AW = 0, PL_AS, PL_AE, PL_AN, PL_C, PL_T, PL_P, PL_E, PL_F, PL_G, PL_W, PL_C0, PL_X1, PL_X2, PL_X3, PL_X4, PL_N };
 
-__device__ __forceinline__ void sweep_to_ij(const Dom &d, int comp_is_pressure, int a, int b, int IS, int JS, int &i, int &j) {
+__device__ __forceinline__ void sweep_to_ij(const DomG &d, int comp_is_pressure, int a, int b, int IS, int JS, int &i, int &j) {
     if (comp_is_pressure) {   // a = j1, b = i1
         j = (JS == -1) ? d.njj - a + 1 : a;
         i = (IS == -1) ? d.nii - b + 1 : b;
@@ -1081,55 +1083,92 @@ __device__ __forceinline__ void lds_barrier() {
 // same relative level hit different banks
 __device__ __forceinline__ int lds_stride(int nkp) { return nkp | 1; }
 
+// column metadata from the per-column table (see gral_pf_prepare)
+struct Meta {
+    int KK, VI, KSTART, KKmi, KKmj;
+    bool adv, vegp;
+};
+__device__ __forceinline__ Meta unpack_meta(int4 m) {
+    Meta r;
+    r.KK = m.x; r.VI = m.y; r.KSTART = m.z & 0xffff; r.adv = (m.z >> 16) & 1; r.vegp = (m.z >> 17) & 1;
+    r.KKmi = m.w & 0xffff; r.KKmj = (m.w >> 16) & 0xffff;
+    return r;
+}
 #define COEF_COLS 32
 // All neighbour-independent work of one mixing-length sweep for every column
-// at once.  A block of 4 waves takes COEF_COLS consecutive columns: the waves
-// compute the coefficients of their columns (lanes over k), then one wave runs
-// the P/TERMP recurrence of all COEF_COLS columns at once (lane = column), and
-// the waves write T and P back along k.
+// at once.  A block of 4 waves takes COEF_COLS consecutive columns of the
+// sweep; their active levels are laid out one after another and dealt to the
+// block's 256 lanes (a lane per level, whatever the column heights), which
+// compute the coefficients.  One wave then runs the P/TERMP recurrence of all
+// COEF_COLS columns at once (lane = column), and the lanes write the last part
+// of each cell's record.  Each 64-byte record is written once, as 4 float4.
 template <int COMP>
 __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds, int IS, int JS) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
-    float *planes = d.planes;
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
+    gf32 *planes = d.planes;
 
     extern __shared__ float sm[];
     const int nkp = d.nkp, nkl = lds_stride(nkp);
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int lane = threadIdx.x % WAVE;
     const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2;
     const int na = d.nii - 1 - oa + 1, nb = d.njj - 1 - ia + 1;
-    const size_t n3 = (size_t)(d.nii + 2) * d.nj2 * nkp;
-    int *sinfo = (int *)(sm + 3 * COEF_COLS * nkl);       // per column: active, KSTART, KK, VI
+    // per column c: A, B, C, e over k at sm + c*4*nkl
+    int *scol = (int *)(sm + 4 * COEF_COLS * nkl);         // [COEF_COLS][8]: KSTART, KK, VI, i, j, first item, active
+    int *stot = scol + 8 * COEF_COLS;
     const int col0 = blockIdx.x * COEF_COLS;
-    for (int c = wave; c < COEF_COLS; c += 4) {
-        float *sA = sm + (size_t)c * 3 * nkl, *sB = sA + nkl, *sC = sB + nkl;
-        int col = col0 + c;
-        bool act = col < na * nb;
-        int i = 0, j = 0;
-        if (act) {
-            sweep_to_ij(d, 0, oa + col / nb, ia + col % nb, IS, JS, i, j);
-            act = d.adv[c2(d, i, j)] == 1;
+    if (threadIdx.x < WAVE) {
+        int c = lane, n = 0;
+        if (c < COEF_COLS) {
+            int col = col0 + c;
+            int i = 0, j = 0;
+            Meta m;
+            m.adv = false;
+            if (col < na * nb) {
+                sweep_to_ij(d, 0, oa + col / nb, ia + col % nb, IS, JS, i, j);
+                m = unpack_meta(ldg4i(d.meta + c2(d, i, j)));
+            }
+            if (m.adv) n = m.VI - m.KSTART + 1;
+            scol[c * 8 + 0] = m.KSTART; scol[c * 8 + 1] = m.KK; scol[c * 8 + 2] = m.VI;
+            scol[c * 8 + 3] = i; scol[c * 8 + 4] = j; scol[c * 8 + 6] = m.adv ? 1 : 0;
         }
-        if (lane == 0) sinfo[c * 4] = act;
-        if (!act) continue;
-        ColInfo q = col_info(d, i, j);
-        if (lane == 0) { sinfo[c * 4 + 1] = q.KSTART; sinfo[c * 4 + 2] = q.KK; sinfo[c * 4 + 3] = q.VI; }
-        const size_t o = c3(d, i, j);
-        for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) {
-            MlPieces p = ml_pieces<COMP>(d, q, k);
-            sA[k] = p.A; sB[k] = p.B; sC[k] = p.C;
-            float4 *P = (float4 *)(planes + (o + k) * PL_N);
-            P[0] = make_float4(p.AW1, p.AS1, p.AE1, p.AN1);
-            P[1] = make_float4(p.C, 0.0f, 0.0f, p.e);          // T and P are filled after the recurrence
-            P[2] = make_float4(p.f, p.g, p.w, p.c0);
-            P[3] = make_float4(p.X1, p.X2, p.X3, p.X4);
+        // inclusive prefix sum of the level counts over the lanes
+        int incl = n;
+        for (int off = 1; off < WAVE; off <<= 1) {
+            int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
         }
+        if (c < COEF_COLS) scol[c * 8 + 5] = incl - n;
+        if (lane == WAVE - 1) stot[0] = incl;
     }
     lds_barrier();
-    if (wave == 0 && lane < COEF_COLS && sinfo[lane * 4]) {
-        float *__restrict__ A = sm + (size_t)lane * 3 * nkl;
+    const int total = stot[0];
+    // the column holding item t is the last c with first[c] <= t: first[] does
+    // not decrease, and every later column starts beyond t
+    auto column_of = [&](int t) {
+        int lo = 0;
+        for (int step = COEF_COLS / 2; step > 0; step >>= 1)
+            if (scol[(lo + step) * 8 + 5] <= t) lo += step;
+        return lo;
+    };
+    for (int t = threadIdx.x; t < total; t += 4 * WAVE) {
+        const int c = column_of(t);
+        const int k = scol[c * 8 + 0] + (t - scol[c * 8 + 5]);
+        const ColInfo q = col_info(d, scol[c * 8 + 3], scol[c * 8 + 4]);
+        MlPieces p = ml_pieces<COMP>(d, q, k);
+        float *sA = sm + (size_t)c * 4 * nkl;
+        sA[k] = p.A; sA[nkl + k] = p.B; sA[2 * nkl + k] = p.C; sA[3 * nkl + k] = p.e;
+        gf32 *P = planes + (c3(d, q.i, q.j) + k) * PL_N;
+        stg4(P, p.AW1, p.AS1, p.AE1, p.AN1);
+        stg4(P + 8, p.f, p.g, p.w, p.c0);
+        stg4(P + 12, p.X1, p.X2, p.X3, p.X4);
+    }
+    lds_barrier();
+    if (threadIdx.x < COEF_COLS && scol[threadIdx.x * 8 + 6]) {
+        const int c = threadIdx.x;
+        float *__restrict__ A = sm + (size_t)c * 4 * nkl;
         float *__restrict__ Bq = A + nkl;
         float *__restrict__ Cq = Bq + nkl;
-        const int KSTART = sinfo[lane * 4 + 1], KKP1 = sinfo[lane * 4 + 2] + 1, VI = sinfo[lane * 4 + 3];
+        const int KSTART = scol[c * 8 + 0], KKP1 = scol[c * 8 + 1] + 1, VI = scol[c * 8 + 2];
         float Pp = 0;
         for (int k = KSTART; k <= VI; ++k) {
             float T = (k > KKP1) ? 1 / (A[k] - Cq[k] * Pp) : 1 / A[k];
@@ -1140,16 +1179,12 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
         }
     }
     lds_barrier();
-    for (int c = wave; c < COEF_COLS; c += 4) {
-        if (!sinfo[c * 4]) continue;
-        int col = col0 + c, i, j;
-        sweep_to_ij(d, 0, oa + col / nb, ia + col % nb, IS, JS, i, j);
-        const float *sA = sm + (size_t)c * 3 * nkl, *sB = sA + nkl;
-        const size_t o = c3(d, i, j);
-        for (int k = sinfo[c * 4 + 1] + lane; k <= sinfo[c * 4 + 3]; k += WAVE) {
-            planes[(o + k) * PL_N + PL_T] = sA[k];
-            planes[(o + k) * PL_N + PL_P] = sB[k];
-        }
+    for (int t = threadIdx.x; t < total; t += 4 * WAVE) {
+        const int c = column_of(t);
+        const int k = scol[c * 8 + 0] + (t - scol[c * 8 + 5]);
+        const float *sA = sm + (size_t)c * 4 * nkl;
+        gf32 *P = planes + (c3(d, scol[c * 8 + 3], scol[c * 8 + 4]) + k) * PL_N;
+        stg4(P + 4, sA[2 * nkl + k], sA[k], sA[nkl + k], sA[3 * nkl + k]);   // C, T, P, e
     }
 }
 
@@ -1169,102 +1204,131 @@ __device__ __forceinline__ bool tile_coords(int tau, int nta, int ntb, int &A, i
 #define SLOT_PAD (2 * CH)
 __device__ __forceinline__ int lds_slot(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
 
+// The chunk loops below are unrolled over the largest column the tiled and
+// strip kernels hold (MAXCH chunks of CH levels) and leave early at the
+// wave-wide longest chain (nmax): straight-line code lets the compiler wait
+// for exactly the LDS reads a chunk needs, while the reads of the next chunk
+// are in flight.  Reads stay within CH levels past a lane's own range (the
+// slot padding); a lane that is done re-reads its first chunk, and levels
+// outside a lane's range are stored to a per-lane dummy word, so the chunks
+// are branch-free.
+#define MAXCH (128 / CH)
+
 // forward: Q[t] = (D[t] + C[t]*Q[t-1]) * T[t], t = 0..n-1, Q[-1] = q0
 __device__ __forceinline__ void chain_fwd(const float *__restrict__ D, const float *__restrict__ C,
-                                          const float *__restrict__ T, float *__restrict__ Q, int n, float q0) {
+                                          const float *__restrict__ T, float *__restrict__ Q, int n, int nmax, float q0,
+                                          float *dump) {
     float Qp = q0;
-    float dv[CH], cv[CH], tv[CH];
+    float cd[CH], cc[CH], ct[CH];
 #pragma unroll
-    for (int u = 0; u < CH; ++u) { dv[u] = D[u]; cv[u] = C[u]; tv[u] = T[u]; }
-    for (int t = 0; t < n; t += CH) {
-        float dn[CH], cn[CH], tn[CH];
+    for (int u = 0; u < CH; ++u) { cd[u] = D[u]; cc[u] = C[u]; ct[u] = T[u]; }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) { dn[u] = D[t + CH + u]; cn[u] = C[t + CH + u]; tn[u] = T[t + CH + u]; }
+    for (int h = 0; h < MAXCH; ++h) {
+        const int t = h * CH;
+        if (t >= nmax) break;
+        float nd[CH], nc[CH], nt[CH];
+        const int o = (t + CH < n) ? t + CH : 0;          // lanes already past their end re-read level 0
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { nd[u] = D[o + u]; nc[u] = C[o + u]; nt[u] = T[o + u]; }
+        __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            if (t + u < n) {
-                float q = (dv[u] + cv[u] * Qp) * tv[u];
-                Q[t + u] = q;
-                Qp = q;
-            }
+            float q = (cd[u] + cc[u] * Qp) * ct[u];
+            *((t + u < n) ? &Q[t + u] : dump) = q;
+            Qp = q;
         }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) { dv[u] = dn[u]; cv[u] = cn[u]; tv[u] = tn[u]; }
+        for (int u = 0; u < CH; ++u) { cd[u] = nd[u]; cc[u] = nc[u]; ct[u] = nt[u]; }
     }
 }
 
 // relaxed back-substitution downwards: X[-t] += relax*(P[-t]*X[-t+1] + Q[-t] - X[-t]), t = 0..n-1
 __device__ __forceinline__ void chain_back(const float *__restrict__ P, const float *__restrict__ Q,
-                                           float *__restrict__ X, float relax, int n) {
+                                           float *__restrict__ X, float relax, int n, int nmax, float *dump) {
     float up = X[1];
-    float pv[CH], qv[CH], xv[CH];
+    float cp[CH], cq[CH], cx[CH];
 #pragma unroll
-    for (int u = 0; u < CH; ++u) { pv[u] = P[-u]; qv[u] = Q[-u]; xv[u] = X[-u]; }
-    for (int t = 0; t < n; t += CH) {
-        float pn[CH], qn[CH], xn[CH];
+    for (int u = 0; u < CH; ++u) { cp[u] = P[-u]; cq[u] = Q[-u]; cx[u] = X[-u]; }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) { pn[u] = P[-(t + CH + u)]; qn[u] = Q[-(t + CH + u)]; xn[u] = X[-(t + CH + u)]; }
+    for (int h = 0; h < MAXCH; ++h) {
+        const int t = h * CH;
+        if (t >= nmax) break;
+        float np[CH], nq[CH], nx[CH];
+        const int o = (t + CH < n) ? t + CH : 0;
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { np[u] = P[-(o + u)]; nq[u] = Q[-(o + u)]; nx[u] = X[-(o + u)]; }
+        __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            if (t + u < n) {
-                float x = xv[u];
-                x = x + relax * (pv[u] * up + qv[u] - x);
-                X[-(t + u)] = x;
-                up = x;
-            }
+            float x = cx[u] + relax * (cp[u] * up + cq[u] - cx[u]);
+            *((t + u < n) ? &X[-(t + u)] : dump) = x;
+            up = x;
         }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) { pv[u] = pn[u]; qv[u] = qn[u]; xv[u] = xn[u]; }
+        for (int u = 0; u < CH; ++u) { cp[u] = np[u]; cq[u] = nq[u]; cx[u] = nx[u]; }
     }
 }
 
 // pressure, downwards from the top: Q[-t] = (A[-t]*Q[-t+1] + D[-t]) * T[-t] with
 // A (= ATP) held in Q on entry and Q[1] = qtop, t = 0..n-1
 __device__ __forceinline__ void chain_pdown(const float *__restrict__ D, const float *__restrict__ T, float *__restrict__ Q,
-                                            float qtop, int n) {
+                                            float qtop, int n, int nmax, float *dump) {
     float Qn = qtop;
-    float av[CH], dv[CH], tv[CH];
+    float ca[CH], cd[CH], ct[CH];
 #pragma unroll
-    for (int u = 0; u < CH; ++u) { av[u] = Q[-u]; dv[u] = D[-u]; tv[u] = T[-u]; }
-    for (int t = 0; t < n; t += CH) {
-        float an[CH], dn[CH], tn[CH];
+    for (int u = 0; u < CH; ++u) { ca[u] = Q[-u]; cd[u] = D[-u]; ct[u] = T[-u]; }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) { an[u] = Q[-(t + CH + u)]; dn[u] = D[-(t + CH + u)]; tn[u] = T[-(t + CH + u)]; }
+    for (int h = 0; h < MAXCH; ++h) {
+        const int t = h * CH;
+        if (t >= nmax) break;
+        float na[CH], nd[CH], nt[CH];
+        const int o = (t + CH < n) ? t + CH : 0;
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { na[u] = Q[-(o + u)]; nd[u] = D[-(o + u)]; nt[u] = T[-(o + u)]; }
+        __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            if (t + u < n) {
-                float q = (av[u] * Qn + dv[u]) * tv[u];
-                Q[-(t + u)] = q;
-                Qn = q;
-            }
+            float q = (ca[u] * Qn + cd[u]) * ct[u];
+            *((t + u < n) ? &Q[-(t + u)] : dump) = q;
+            Qn = q;
         }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) { av[u] = an[u]; dv[u] = dn[u]; tv[u] = tn[u]; }
+        for (int u = 0; u < CH; ++u) { ca[u] = na[u]; cd[u] = nd[u]; ct[u] = nt[u]; }
     }
 }
 
 // pressure, upwards: N[t] = P[t]*N[t-1] + Q[t], t = 0..n-1, N[-1] = bottom
 __device__ __forceinline__ void chain_pup(const float *__restrict__ P, const float *__restrict__ Q, float *__restrict__ N,
-                                          float bottom, int n) {
+                                          float bottom, int n, int nmax, float *dump) {
     float dp = bottom;
-    float pv[CH], qv[CH];
+    float cp[CH], cq[CH];
 #pragma unroll
-    for (int u = 0; u < CH; ++u) { pv[u] = P[u]; qv[u] = Q[u]; }
-    for (int t = 0; t < n; t += CH) {
-        float pn[CH], qn[CH];
+    for (int u = 0; u < CH; ++u) { cp[u] = P[u]; cq[u] = Q[u]; }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) { pn[u] = P[t + CH + u]; qn[u] = Q[t + CH + u]; }
+    for (int h = 0; h < MAXCH; ++h) {
+        const int t = h * CH;
+        if (t >= nmax) break;
+        float np[CH], nq[CH];
+        const int o = (t + CH < n) ? t + CH : 0;
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { np[u] = P[o + u]; nq[u] = Q[o + u]; }
+        __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            if (t + u < n) {
-                float x = pv[u] * dp + qv[u];
-                N[t + u] = x;
-                dp = x;
-            }
+            float x = cp[u] * dp + cq[u];
+            *((t + u < n) ? &N[t + u] : dump) = x;
+            dp = x;
         }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) { pv[u] = pn[u]; qv[u] = qn[u]; }
+        for (int u = 0; u < CH; ++u) { cp[u] = np[u]; cq[u] = nq[u]; }
     }
+}
+
+// largest value of v over the lanes of the wave that pass `on` (0 if none)
+__device__ __forceinline__ int wave_max(int v, bool on) {
+    int m = on ? v : 0;
+    for (int off = WAVE / 2; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+    return __builtin_amdgcn_readfirstlane(m);
 }
 
 // per-level values of one column that do not change during the sweep, loaded
@@ -1276,32 +1340,21 @@ struct MlPre {
 };
 
 template <int COMP>
-__device__ __forceinline__ void ml_prefetch(const Dom &d, const float *planes, size_t n3, int i, int j, int k, int IS, int JS,
+__device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *planes, size_t n3, int i, int j, int k, int IS, int JS,
                                             MlPre &r) {
     (void)n3;
-    const float4 *P = (const float4 *)(planes + (c3(d, i, j) + k) * PL_N);
-    float4 v0 = P[0], v1 = P[1], v2 = P[2], v3 = P[3];
+    const gf32 *P = planes + (c3(d, i, j) + k) * PL_N;
+    float4 v0 = ldg4(P), v1 = ldg4(P + 4), v2 = ldg4(P + 8), v3 = ldg4(P + 12);
     r.aw = v0.x; r.as = v0.y; r.ae = v0.z; r.an = v0.w;
     r.c = v1.x; r.t = v1.y; r.p = v1.z; r.e = v1.w;
     r.f = v2.x; r.g = v2.y; r.w = v2.z; r.c0 = v2.w;
     r.x1 = v3.x; r.x2 = v3.y; r.x3 = v3.z; r.x4 = v3.w;
-    const float *X = COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk;
+    const gf32 *X = COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk;
     r.x = X[c3(d, i, j) + k];
     r.xa = X[c3(d, i + IS, j) + k];
     r.xb = X[c3(d, i, j + JS) + k];
 }
 
-// column metadata from the per-column table (see gral_pf_prepare)
-struct Meta {
-    int KK, VI, KSTART, KKmi, KKmj;
-    bool adv, vegp;
-};
-__device__ __forceinline__ Meta unpack_meta(int4 m) {
-    Meta r;
-    r.KK = m.x; r.VI = m.y; r.KSTART = m.z & 0xffff; r.adv = (m.z >> 16) & 1; r.vegp = (m.z >> 17) & 1;
-    r.KKmi = m.w & 0xffff; r.KKmj = (m.w >> 16) & 0xffff;
-    return r;
-}
 __device__ __forceinline__ ColInfo meta_col(const Meta &m, int i, int j, int W) {
     ColInfo q;
     q.i = i; q.j = j; q.KK = m.KK; q.KSTART = m.KSTART; q.VI = m.VI; q.W = W; q.vegp = m.vegp;
@@ -1323,8 +1376,8 @@ __device__ __forceinline__ ColInfo meta_col(const Meta &m, int i, int j, int W) 
 // only, and no global load inside the step loop waits on another one.
 template <int COMP>
 __global__ __launch_bounds__(TILE * WAVE) void k_ml_tile(const Dom *__restrict__ ds, int tau, int IS, int JS) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
-    const float *planes = d.planes;
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
+    const gf32 *planes = d.planes;
 
     extern __shared__ float sm[];
     const int nkp = d.nkp, nkl = lds_slot(nkp);
@@ -1338,10 +1391,11 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_tile(const Dom *__restrict__
     float *sD = sm + (size_t)r * 5 * nkl + SLOT_PAD, *sC = sD + nkl, *sT = sC + nkl, *sP = sT + nkl, *sQ = sP + nkl;
     float *xbuf = sm + (size_t)TILE * 5 * nkl + SLOT_PAD;      // [2][TILE] slots of whole columns
     float *sdzk = sm + (size_t)TILE * 7 * nkl;                // [nkp]
-    int4 *smeta = (int4 *)(((uintptr_t)(sdzk + nkp) + 15) & ~(uintptr_t)15);   // [TILE rows][TILE columns]
+    int4 *smeta = (int4 *)(sm + (((size_t)TILE * 7 * nkl + nkp + 3) & ~(size_t)3));   // [TILE rows][TILE columns], 16-byte aligned
     int *sinfo = (int *)(smeta + TILE * TILE);                // per row: active, KSTART, KK, VI, KKm
+    float *sdump = (float *)(sinfo + 5 * TILE);               // [WAVE] stores of the chains outside a column
     const size_t n3 = (size_t)(d.nii + 2) * d.nj2 * nkp;
-    float *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk);
+    gf32 *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk);
     // per-launch staging: the metadata of this wave's row and the layer thicknesses
     if (lane < TILE) {
         int a = a0 + lane;
@@ -1349,7 +1403,7 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_tile(const Dom *__restrict__
         if (a <= ob && b <= ib) {
             int i, j;
             sweep_to_ij(d, 0, a, b, IS, JS, i, j);
-            m = d.meta[c2(d, i, j)];
+            m = ldg4i(d.meta + c2(d, i, j));
             m.z |= 1 << 20;                                    // in range
         }
         smeta[r * TILE + lane] = m;
@@ -1437,14 +1491,20 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_tile(const Dom *__restrict__
         if (lane == 0) sinfo[r * 5] = act;
         locate(s + 1);                                   // prefetch of the next column, in flight across phases 2-3
         lds_barrier();
-        if (r == 0 && lane < TILE && sinfo[lane * 5]) {
-            float *D = sm + (size_t)lane * 5 * nkl + SLOT_PAD;
-            float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl;
-            float *Xs = xbuf + (size_t)((s & 1) * TILE + lane) * nkl;
-            const int KSTART = sinfo[lane * 5 + 1], KK = sinfo[lane * 5 + 2], VI = sinfo[lane * 5 + 3], KKm = sinfo[lane * 5 + 4];
-            chain_fwd(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, VI - KK - 1, Q[KK + 1]);
+        if (r == 0) {
+            const bool on = lane < TILE && sinfo[(lane & (TILE - 1)) * 5];
+            const int cl = lane & (TILE - 1);
+            const int KSTART = sinfo[cl * 5 + 1], KK = sinfo[cl * 5 + 2], VI = sinfo[cl * 5 + 3], KKm = sinfo[cl * 5 + 4];
             const int kend = (COMP == 2) ? KSTART : max(KSTART, KKm + 1);
-            chain_back(P + VI, Q + VI, Xs + VI, d.relax, VI - kend + 1);
+            const int nf = VI - KK - 1, nb = VI - kend + 1;
+            const int nfmax = wave_max(nf, on), nbmax = wave_max(nb, on);
+            if (on) {
+                float *D = sm + (size_t)cl * 5 * nkl + SLOT_PAD;
+                float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl;
+                float *Xs = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
+                chain_fwd(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane);
+                chain_back(P + VI, Q + VI, Xs + VI, d.relax, nb, nbmax, sdump + lane);
+            }
         }
         lds_barrier();
         if (act) {
@@ -1468,8 +1528,8 @@ struct PrPre {
 };
 
 __global__ __launch_bounds__(TILE * WAVE) void k_pressure_tile(const Dom *__restrict__ ds, int tau, int IS, int JS, int slot) {
-    const Dom &d = ds[blockIdx.z];                   // situation of this launch slice
-    const float *ptab = d.ptab[IS == 1 ? 0 : 1], *ttab = d.ttab[IS == 1 ? 0 : 1];
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
+    const gf32 *ptab = d.ptab[IS == 1 ? 0 : 1], *ttab = d.ttab[IS == 1 ? 0 : 1];
 
     extern __shared__ float sm[];
     const int nkp = d.nkp, nkl = lds_slot(nkp);
@@ -1482,15 +1542,16 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_tile(const Dom *__rest
     float *sD = sm + (size_t)r * 4 * nkl + SLOT_PAD, *sQ = sD + nkl, *sT = sQ + nkl, *sP = sT + nkl;
     float *xbuf = sm + (size_t)TILE * 4 * nkl + SLOT_PAD;      // [2][TILE] slots of whole DPM columns
     float *sdzk = sm + (size_t)TILE * 6 * nkl, *satp = sdzk + nkp;
-    int4 *smeta = (int4 *)(((uintptr_t)(satp + nkp) + 15) & ~(uintptr_t)15);
+    int4 *smeta = (int4 *)(sm + (((size_t)TILE * 6 * nkl + 2 * nkp + 3) & ~(size_t)3));
     int *sinfo = (int *)(smeta + TILE * TILE);                // per row: active, KK, VI, j
+    float *sdump = (float *)(sinfo + 4 * TILE);               // [WAVE] stores of the chains outside a column
     if (lane < TILE) {
         int a = a0 + lane;
         int4 m = make_int4(0, -1, 0, 0);
         if (a <= ob && b <= ib) {
             int i, j;
             sweep_to_ij(d, 1, a, b, IS, JS, i, j);
-            m = d.meta[c2(d, i, j)];
+            m = ldg4i(d.meta + c2(d, i, j));
             m.z |= 1 << 20;
         }
         smeta[r * TILE + lane] = m;
@@ -1578,19 +1639,24 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_tile(const Dom *__rest
         if (lane == 0) sinfo[r * 4] = act;
         locate(s + 1);
         lds_barrier();
-        if (r == 0 && lane < TILE && sinfo[lane * 4]) {
-            float *Dd = sm + (size_t)lane * 4 * nkl + SLOT_PAD;
-            float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl;
-            const float *O = xbuf + (size_t)((s & 1) * TILE + lane) * nkl;
-            const int KK2 = sinfo[lane * 4 + 1], VI2 = sinfo[lane * 4 + 2], jr = sinfo[lane * 4 + 3];
-            // Q[k] holds ATP[k] on entry; the new DPM replaces D
-            chain_pdown(Dd + VI2, T + VI2, Q + VI2, d.prow_q[(size_t)jr * nkp + VI2 + 1], VI2 - KK2);
-            chain_pup(P + KK2 + 1, Q + KK2 + 1, Dd + KK2 + 1, O[KK2], VI2 - KK2);
+        if (r == 0) {
+            const int cl = lane & (TILE - 1);
+            const bool on = lane < TILE && sinfo[cl * 4];
+            const int KK2 = sinfo[cl * 4 + 1], VI2 = sinfo[cl * 4 + 2], jr = sinfo[cl * 4 + 3];
+            const int nmax = wave_max(VI2 - KK2, on);
+            if (on) {
+                float *Dd = sm + (size_t)cl * 4 * nkl + SLOT_PAD;
+                float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl;
+                const float *O = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
+                // Q[k] holds ATP[k] on entry; the new DPM replaces D
+                chain_pdown(Dd + VI2, T + VI2, Q + VI2, d.prow_q[(size_t)jr * nkp + VI2 + 1], VI2 - KK2, nmax, sdump + lane);
+                chain_pup(P + KK2 + 1, Q + KK2 + 1, Dd + KK2 + 1, O[KK2], VI2 - KK2, nmax, sdump + lane);
+            }
         }
         lds_barrier();
         if (act) {
             size_t o = c3(d, i, j);
-            float *qrow = d.prow_q + (size_t)j * nkp;
+            gf32 *qrow = d.prow_q + (size_t)j * nkp;
             for (int k = KK + 1 + lane; k <= VI; k += WAVE) {
                 float temp = sD[k];
                 double qv = (double)conv_to_int((double)temp * d.frund) / d.frund;
@@ -1608,13 +1674,369 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_tile(const Dom *__rest
         float other = __shfl_xor(rmax, off);
         rmax = other > rmax ? other : rmax;
     }
-    if (lane == 0 && rmax > 0.0f && slot >= 0 && slot < d.resid_cap) atomicMax(&d.resid[slot], __float_as_uint(rmax));
+    if (lane == 0 && rmax > 0.0f && slot >= 0 && slot < d.resid_cap) atomicMax((unsigned *)&d.resid[slot], __float_as_uint(rmax));
 }
 
 // ------------------------------------------------------------------------
-// host-side launchers
+// Strip-pipelined sweeps.  One workgroup owns a strip of TILE sweep rows
+// (b) and walks the whole a range with the same per-step structure as the
+// tiled kernels (wave r handles row b0+r, column a = oa+s-r at local step
+// s).  Strip B reads the last row of strip B-1 (its new values, and for the
+// pressure sweep the QIMP row scratch) and nothing else from other strips,
+// so it only has to stay TILE steps behind strip B-1: the serial path of a
+// sweep is about (a range) + TILE * (number of strips) steps instead of the
+// (2*TILE-1) * (number of tile diagonals) of the tiled launches.
+//
+// Synchronisation, per situation and sweep kind, in int sync[...]:
+//   [0] ticket counter: strips are numbered in the order their workgroups
+//       start, so the strip a workgroup waits for is held by a workgroup
+//       that is already running (no dependence on dispatch order);
+//   [2 + B] steps of strip B whose stores are complete and released;
+// the blocks are cleared at the start of every iteration.  d.sync[0] is the
+// situation's sticky error flag: set when a wait exceeded the time limit.
+// Wave TILE-1 publishes its strip's progress with an agent-scope release;
+// wave 0 of the next strip polls it and acquires before reading.  Every
+// wait is bounded (STRIP_TIMEOUT_TICKS of the constant 100 MHz clock): on
+// timeout the strip sets the error flag and goes on without waiting, so the
+// kernel always ends; the host reports the error.
 // ------------------------------------------------------------------------
-static inline int lds_slot_h(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
+#define SYNC_HDR 2
+#define STRIP_TIMEOUT_TICKS 200000000ull
+
+__device__ __forceinline__ int load_agent(const gi32 *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(gi32 *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct StripWait {
+    const gi32 *prev;              // progress of the previous strip (null for strip 0)
+    gi32 *error;
+    int seen;                      // progress of the previous strip known so far
+    int cap;                       // steps of a strip
+    unsigned long long deadline;
+    // called by every lane of one wave; returns once the previous strip has
+    // completed min(target, cap) steps (or the time limit passed)
+    __device__ __forceinline__ void wait(int target) {
+        target = min(target, cap);
+        if (seen >= target) return;
+        for (int spins = 0;; ++spins) {
+            int p = __builtin_amdgcn_readfirstlane(load_agent(prev));
+            if (p >= target) { seen = p; break; }
+            if (wall_clock64() > deadline || spins > (1 << 22)) {
+                store_agent(error, 1);
+                seen = 0x7fffffff;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+};
+
+// called by every lane of the strip's last wave: progress v (steps whose
+// stores this wave issued) becomes visible with the stores before it
+__device__ __forceinline__ void strip_publish(gi32 *mine, int v, int lane) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) store_agent(mine, v);
+}
+
+template <int COMP>
+__global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict__ ds, int IS, int JS) {
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
+    const gf32 *planes = d.planes;
+    extern __shared__ float sm[];
+    const int nkp = d.nkp, nkl = lds_slot(nkp);
+    const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2, ob = d.nii - 1, ib = d.njj - 1;
+    const int nstrip = (ib - ia + TILE) / TILE, nsteps = (ob - oa + 1) + TILE - 1;
+    const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    gi32 *sync = d.sync + SYNC_BLOCK0 + (size_t)(1 + COMP) * d.sync_stride;
+    float *sD = sm + (size_t)r * 5 * nkl + SLOT_PAD, *sC = sD + nkl, *sT = sC + nkl, *sP = sT + nkl, *sQ = sP + nkl;
+    float *xbuf = sm + (size_t)TILE * 5 * nkl + SLOT_PAD;      // [2][TILE] slots of whole columns
+    float *sdzk = sm + (size_t)TILE * 7 * nkl;                // [nkp]
+    int *sinfo = (int *)(sdzk + nkp);                         // per row: active, KSTART, KK, VI, KKm
+    int *sctl = sinfo + 5 * TILE;                             // strip index of this workgroup
+    float *sdump = (float *)(sctl + 4);                       // [WAVE] stores of the chains outside a column
+    if (threadIdx.x == 0) sctl[0] = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = threadIdx.x; k < nkp; k += TILE * WAVE) sdzk[k] = d.dzk[k];
+    __syncthreads();
+    const int B = sctl[0];
+    if (B >= nstrip) return;                                  // uniform over the workgroup
+    const int b = ia + B * TILE + r;
+    const bool row_ok = b <= ib;
+    StripWait sw{B > 0 ? sync + SYNC_HDR + B - 1 : nullptr, d.sync, B > 0 ? 0 : 0x7fffffff, nsteps,
+                 wall_clock64() + STRIP_TIMEOUT_TICKS};
+    gi32 *mine = sync + SYNC_HDR + B;
+    const size_t n3 = (size_t)(d.nii + 2) * d.nj2 * nkp;
+    gf32 *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk);
+    MlPre pre[2];
+    float xfull[2];
+    Meta mn;
+    int in_ = 0, jn_ = 0;
+    bool inr_n = false;
+    auto locate = [&](int s) {
+        int l = s - r;                                        // column a = oa + l
+        inr_n = row_ok && l >= 0 && oa + l <= ob;
+        mn.adv = false;
+        if (inr_n) {
+            sweep_to_ij(d, 0, oa + l, b, IS, JS, in_, jn_);
+            mn = unpack_meta(ldg4i(d.meta + c2(d, in_, jn_)));
+            const size_t o = c3(d, in_, jn_);
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = lane + WAVE * m;
+                if (k < nkp) xfull[m] = X[o + k];
+            }
+        }
+        if (mn.adv) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = mn.KSTART + lane + WAVE * m;
+                if (k <= mn.VI) {
+                    ml_prefetch<COMP>(d, planes, n3, in_, jn_, k, IS, JS, pre[m]);
+                    if (l == 0) pre[m].ni = X[c3(d, in_ - IS, jn_) + k];
+                    if (r == 0) pre[m].nj = X[c3(d, in_, jn_ - JS) + k];
+                }
+            }
+        }
+    };
+    if (r == 0) sw.wait(TILE);
+    locate(0);
+    for (int s = 0; s < nsteps; ++s) {
+        const bool act = mn.adv, inr = inr_n;
+        const Meta mt = mn;
+        const int i = in_, j = jn_;
+        const ColInfo q = meta_col(mt, i, j, SIMDW);
+        float *xcur = xbuf + (size_t)((s & 1) * TILE + r) * nkl;
+        const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r) * nkl;
+        const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r - 1) * nkl;
+        const bool w_in = (s - r >= 1), s_in = (r >= 1);
+        if (inr) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = lane + WAVE * m;
+                if (k < nkp) xcur[k] = xfull[m];
+            }
+        }
+        if (act) {
+            const int KKP1 = q.KK + 1;
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = q.KSTART + lane + WAVE * m;
+                if (k <= q.VI) {
+                    const MlPre &r_ = pre[m];
+                    float xnew_i = w_in ? xw_lds[k] : r_.ni;
+                    float xnew_j = s_in ? xs_lds[k] : r_.nj;
+                    float xw = (IS == 1) ? xnew_i : r_.xa, xe = (IS == 1) ? r_.xa : xnew_i;
+                    float xs = (JS == 1) ? xnew_j : r_.xb, xn = (JS == 1) ? r_.xb : xnew_j;
+                    MlPieces pc;
+                    pc.AW1 = r_.aw; pc.AS1 = r_.as; pc.AE1 = r_.ae; pc.AN1 = r_.an; pc.e = r_.e; pc.f = r_.f;
+                    pc.g = r_.g; pc.w = r_.w; pc.c0 = r_.c0; pc.X1 = r_.x1; pc.X2 = r_.x2; pc.X3 = r_.x3; pc.X4 = r_.x4;
+                    float D = ml_source<COMP>(d, q, k, sdzk[k], pc, xw, xs, xe, xn, r_.x);
+                    sD[k] = D; sC[k] = r_.c; sT[k] = r_.t; sP[k] = r_.p;
+                    if (k <= KKP1) sQ[k] = D * r_.t;
+                }
+            }
+            if (lane == 0) {
+                int KKm = (COMP == 0) ? mt.KKmi : (COMP == 1) ? mt.KKmj : q.KK;
+                sinfo[r * 5 + 1] = q.KSTART; sinfo[r * 5 + 2] = q.KK; sinfo[r * 5 + 3] = q.VI; sinfo[r * 5 + 4] = KKm;
+            }
+        }
+        if (lane == 0) sinfo[r * 5] = act;
+        if (r == 0) sw.wait(s + 1 + TILE);             // the previous strip's row for column oa+s+1
+        locate(s + 1);
+        lds_barrier();
+        if (r == 0) {
+            const bool on = lane < TILE && sinfo[(lane & (TILE - 1)) * 5];
+            const int cl = lane & (TILE - 1);
+            const int KSTART = sinfo[cl * 5 + 1], KK = sinfo[cl * 5 + 2], VI = sinfo[cl * 5 + 3], KKm = sinfo[cl * 5 + 4];
+            const int kend = (COMP == 2) ? KSTART : max(KSTART, KKm + 1);
+            const int nf = VI - KK - 1, nb = VI - kend + 1;
+            const int nfmax = wave_max(nf, on), nbmax = wave_max(nb, on);
+            if (on) {
+                float *D = sm + (size_t)cl * 5 * nkl + SLOT_PAD;
+                float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl;
+                float *Xs = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
+                chain_fwd(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane);
+                chain_back(P + VI, Q + VI, Xs + VI, d.relax, nb, nbmax, sdump + lane);
+            }
+        }
+        lds_barrier();
+        if (r == TILE - 1 && s > 0) strip_publish(mine, s, lane);   // steps 0..s-1 written
+        if (act) {
+            const size_t o = c3(d, i, j);
+            for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) X[o + k] = xcur[k];
+        }
+        lds_barrier();
+    }
+    if (r == TILE - 1) strip_publish(mine, nsteps, lane);
+}
+
+// Pressure sweep on strips.  The QIMP row scratch of sweep row a travels
+// with the wavefront through LDS (ring slot (a - oa) mod TILE): wave 0 loads
+// it from global memory (as strip B-1 left it), each wave updates it with its
+// column, and the last wave stores it back for strip B+1.
+__global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__restrict__ ds, int IS, int JS, int slot) {
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];
+    const gf32 *ptab = d.ptab[IS == 1 ? 0 : 1], *ttab = d.ttab[IS == 1 ? 0 : 1];
+    extern __shared__ float sm[];
+    const int nkp = d.nkp, nkl = lds_slot(nkp);
+    const int oa = 2, ia = 2, ob = d.njj - 1, ib = d.nii - 1;
+    const int nstrip = (ib - ia + TILE) / TILE, nsteps = (ob - oa + 1) + TILE - 1;
+    const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    gi32 *sync = d.sync + SYNC_BLOCK0;
+    float *sD = sm + (size_t)r * 4 * nkl + SLOT_PAD, *sQ = sD + nkl, *sT = sQ + nkl, *sP = sT + nkl;
+    float *xbuf = sm + (size_t)TILE * 4 * nkl + SLOT_PAD;      // [2][TILE] slots of whole DPM columns
+    float *ring = sm + (size_t)TILE * 6 * nkl + SLOT_PAD;      // [TILE] QIMP row scratch slots
+    float *sdzk = sm + (size_t)TILE * 7 * nkl, *satp = sdzk + nkp;
+    int *sinfo = (int *)(satp + nkp);                         // per row: active, KK, VI, ring slot
+    int *sctl = sinfo + 4 * TILE;
+    float *sdump = (float *)(sctl + 4);                       // [WAVE] stores of the chains outside a column
+    if (threadIdx.x == 0) sctl[0] = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = threadIdx.x; k < nkp; k += TILE * WAVE) { sdzk[k] = d.dzk[k]; satp[k] = d.atp[k]; }
+    __syncthreads();
+    const int B = sctl[0];
+    if (B >= nstrip) return;
+    const int b = ia + B * TILE + r;
+    const bool row_ok = b <= ib;
+    StripWait sw{B > 0 ? sync + SYNC_HDR + B - 1 : nullptr, d.sync, B > 0 ? 0 : 0x7fffffff, nsteps,
+                 wall_clock64() + STRIP_TIMEOUT_TICKS};
+    gi32 *mine = sync + SYNC_HDR + B;
+    const float DXK = d.dxk, DYK = d.dyk;
+    float rmax = 0.0f;
+    PrPre pre[2];
+    float xfull[2], qfull[2];
+    int in_ = 0, jn_ = 0, KKn = 0, VIn = 0;
+    bool inr_n = false, actn = false;
+    auto locate = [&](int s) {
+        int l = s - r;                                        // sweep row a = oa + l
+        inr_n = row_ok && l >= 0 && oa + l <= ob;
+        actn = false;
+        if (inr_n) {
+            sweep_to_ij(d, 1, oa + l, b, IS, JS, in_, jn_);
+            Meta m = unpack_meta(ldg4i(d.meta + c2(d, in_, jn_)));
+            actn = m.adv;
+            KKn = m.KK; VIn = m.VI;
+            const size_t o = c3(d, in_, jn_);
+#pragma unroll
+            for (int m2 = 0; m2 < 2; ++m2) {
+                int k = lane + WAVE * m2;
+                if (k < nkp) xfull[m2] = d.dpm[o + k];
+            }
+            if (r == 0) {                                     // the row scratch as strip B-1 left it
+#pragma unroll
+                for (int m2 = 0; m2 < 2; ++m2) {
+                    int k = lane + WAVE * m2;
+                    if (k < nkp) qfull[m2] = d.prow_q[(size_t)jn_ * nkp + k];
+                }
+            }
+        }
+        if (actn) {
+            size_t o = c3(d, in_, jn_);
+#pragma unroll
+            for (int m2 = 0; m2 < 2; ++m2) {
+                int k = KKn + 1 + lane + WAVE * m2;
+                if (k <= VIn) {
+                    pre[m2].div = d.div[o + k];
+                    pre[m2].t = ttab[o + k];
+                    pre[m2].p = ptab[o + k];
+                    pre[m2].pa = d.dpm[c3(d, in_ + IS, jn_) + k];
+                    pre[m2].pb = d.dpm[c3(d, in_, jn_ + JS) + k];
+                    if (l == 0) pre[m2].nj = d.dpm[c3(d, in_, jn_ - JS) + k];
+                    if (r == 0) pre[m2].ni = d.dpm[c3(d, in_ - IS, jn_) + k];
+                }
+            }
+        }
+    };
+    if (r == 0) sw.wait(TILE);
+    locate(0);
+    for (int s = 0; s < nsteps; ++s) {
+        const bool act = actn, inr = inr_n;
+        const int i = in_, j = jn_, KK = KKn, VI = VIn;
+        const int rs = (s - r) & (TILE - 1);                  // ring slot of this wave's sweep row
+        float *xcur = xbuf + (size_t)((s & 1) * TILE + r) * nkl;
+        const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r) * nkl;
+        const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r - 1) * nkl;
+        const bool w_in = (s - r >= 1), s_in = (r >= 1);
+        if (inr) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = lane + WAVE * m;
+                if (k < nkp) {
+                    xcur[k] = xfull[m];
+                    if (r == 0) ring[(size_t)rs * nkl + k] = qfull[m];
+                }
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = KK + 1 + lane + WAVE * m;
+                if (k <= VI) {
+                    const PrPre &r_ = pre[m];
+                    float newj = w_in ? xw_lds[k] : r_.nj;
+                    float newi = s_in ? xs_lds[k] : r_.ni;
+                    float dim_ = (IS == 1) ? newi : r_.pa, dip = (IS == 1) ? r_.pa : newi;
+                    float djm = (JS == 1) ? newj : r_.pb, djp = (JS == 1) ? r_.pb : newj;
+                    float dz = sdzk[k];
+                    sD[k] = r_.div / d.dtime + (dim_ + dip) / DXK * DYK * dz + (djm - djp) / DYK * DXK * dz;
+                    sT[k] = r_.t;
+                    sP[k] = r_.p;
+                    sQ[k] = satp[k];                  // consumed and replaced by QIMP in phase 2
+                }
+            }
+            if (lane == 0) { sinfo[r * 4 + 1] = KK; sinfo[r * 4 + 2] = VI; sinfo[r * 4 + 3] = rs; }
+        }
+        if (lane == 0) sinfo[r * 4] = act;
+        if (r == 0) sw.wait(s + 1 + TILE);
+        locate(s + 1);
+        lds_barrier();
+        if (r == 0) {
+            const int cl = lane & (TILE - 1);
+            const bool on = lane < TILE && sinfo[cl * 4];
+            const int KK2 = sinfo[cl * 4 + 1], VI2 = sinfo[cl * 4 + 2], rs2 = sinfo[cl * 4 + 3];
+            const int nmax = wave_max(VI2 - KK2, on);
+            if (on) {
+                float *Dd = sm + (size_t)cl * 4 * nkl + SLOT_PAD;
+                float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl;
+                const float *O = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
+                // Q[k] holds ATP[k] on entry; the new DPM replaces D
+                chain_pdown(Dd + VI2, T + VI2, Q + VI2, ring[(size_t)rs2 * nkl + VI2 + 1], VI2 - KK2, nmax, sdump + lane);
+                chain_pup(P + KK2 + 1, Q + KK2 + 1, Dd + KK2 + 1, O[KK2], VI2 - KK2, nmax, sdump + lane);
+            }
+        }
+        lds_barrier();
+        if (r == TILE - 1 && s > 0) strip_publish(mine, s, lane);
+        if (act) {
+            size_t o = c3(d, i, j);
+            float *qring = ring + (size_t)rs * nkl;
+            for (int k = KK + 1 + lane; k <= VI; k += WAVE) {
+                float temp = sD[k];
+                double qv = (double)conv_to_int((double)temp * d.frund) / d.frund;
+                float rr = (float)fabs((double)xcur[k] - qv);
+                rmax = rr > rmax ? rr : rmax;
+                xcur[k] = temp;                             // column buffer now holds the new DPM
+                d.dpm[o + k] = temp;
+                d.dpmnew[o + k] = d.dpmnew[o + k] + d.relaxp * temp;
+                qring[k] = sQ[k];
+            }
+        }
+        if (r == TILE - 1 && inr) {                           // hand the row scratch to strip B+1
+            const float *qring = ring + (size_t)rs * nkl;
+            gf32 *qrow = d.prow_q + (size_t)j * nkp;
+            for (int k = lane; k < nkp; k += WAVE) qrow[k] = qring[k];
+        }
+        lds_barrier();
+    }
+    if (r == TILE - 1) strip_publish(mine, nsteps, lane);
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        float other = __shfl_xor(rmax, off);
+        rmax = other > rmax ? other : rmax;
+    }
+    if (lane == 0 && rmax > 0.0f && slot >= 0 && slot < d.resid_cap) atomicMax((unsigned *)&d.resid[slot], __float_as_uint(rmax));
+}
+
 static inline int ndiag(int oa, int ob, int ia, int ib) { return (ob - oa) + (ib - ia) + 1; }
 static inline int diag_len(int t, int oa, int ob, int ia, int ib) {
     int nb = ib - ia;
@@ -1622,6 +2044,158 @@ static inline int diag_len(int t, int oa, int ob, int ia, int ib) {
     int amax = ob < oa + t ? ob : oa + t;
     return amax - amin + 1;
 }
+
+// ------------------------------------------------------------------------
+// Diagnostic post-pass: DiagnosticFlowfield.Calculate (DiagnosticFlowfield.cs:
+// 24-436) and the final vertical-velocity mass adjustment of MicroscaleFlat
+// (MicroscaleFlat.cs:388-462) or MicroscaleTerrain (MicroscaleTerrain.cs:
+// 862-933).  It works on full-height columns (levels 0..NKK+1), in its own
+// arrays; one situation per launch, passed by value.
+// ------------------------------------------------------------------------
+struct DiagArgs {
+    int nii, njj, nkk, nkh, nj2;   // nkh = NKK + 2 levels per stored column
+    int level, terrain;            // FlowFieldLevel (1 diagnostic, 2 prognostic); caller
+    float dxk, dyk, dtime;
+    const float *dzk;              // [nkh]
+    const float *app, *atp, *abp_in, *abp_bot;   // [nkh] pressure coefficients by level
+    const int *kk;                 // [nii+2][njj+2]
+    const uint8_t *adv;
+    float *uk, *vk, *wk, *dpm, *div;             // [nii+2][njj+2][nkh]
+};
+__device__ __forceinline__ size_t dg3(const DiagArgs &g, int i, int j) { return ((size_t)i * g.nj2 + j) * (size_t)g.nkh; }
+
+// DPM = 0 for i in [1,NII], j in [1,NJJ], k in [1,NKK+1] (DiagnosticFlowfield.cs:40-49)
+__global__ void k_dg_zero(DiagArgs g) {
+    int c = blockIdx.x, i = 1 + c / g.njj, j = 1 + c % g.njj;
+    if (i > g.nii) return;
+    size_t o = dg3(g, i, j);
+    for (int k = 1 + threadIdx.x; k <= g.nkk + 1; k += blockDim.x) g.dpm[o + k] = 0.0f;
+}
+
+// mass divergence of every interior column (:53-80); block = one column
+__global__ void k_dg_divergence(DiagArgs g) {
+    int nj = g.njj - 2, i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
+    int KK = g.kk[(size_t)i * g.nj2 + j];
+    size_t o = dg3(g, i, j), oip = dg3(g, i + 1, j), ojp = dg3(g, i, j + 1);
+    const float DXK = g.dxk, DYK = g.dyk;
+    for (int k = KK + 1 + threadIdx.x; k <= g.nkk - 1; k += blockDim.x) {
+        float dz = g.dzk[k];
+        float t = (g.uk[o + k] - g.uk[oip + k]) * DYK * dz + (g.vk[o + k] - g.vk[ojp + k]) * DXK * dz;
+        g.div[o + k] = (k > KK + 1) ? t + (g.wk[o + k] - g.wk[o + k + 1]) * DXK * DYK : t - g.wk[o + k + 1] * DXK * DYK;
+    }
+}
+
+// pressure correction (:84-341), one anti-diagonal of the sweep per launch,
+// one wave per column: lanes build DIM, lane 0 runs the two recurrences over
+// the whole column.  Columns are solved over their full height, so the row
+// scratch above the top (PIMP/QIMP[NKK]) is always zero.
+__global__ __launch_bounds__(WAVE) void k_dg_pressure(DiagArgs g, int t, int IS, int JS) {
+    extern __shared__ float sm[];
+    int a, b;
+    if (!diag_col(t, 2, g.njj - 1, 2, g.nii - 1, blockIdx.x, a, b)) return;
+    int j = (JS == -1) ? g.njj - a + 1 : a;
+    int i = (IS == -1) ? g.nii - b + 1 : b;
+    int KK = g.kk[(size_t)i * g.nj2 + j];
+    const int nkh = g.nkh;
+    float *sDIM = sm, *sP = sDIM + nkh, *sQ = sP + nkh, *sN = sQ + nkh;
+    size_t o = dg3(g, i, j), oim = dg3(g, i - 1, j), oip = dg3(g, i + 1, j), ojm = dg3(g, i, j - 1), ojp = dg3(g, i, j + 1);
+    const float DXK = g.dxk, DYK = g.dyk;
+    for (int k = KK + 1 + threadIdx.x; k <= g.nkk - 1; k += WAVE) {
+        float dz = g.dzk[k];
+        sDIM[k] = g.div[o + k] / g.dtime + (g.dpm[oim + k] + g.dpm[oip + k]) / DXK * DYK * dz +
+                  (g.dpm[ojm + k] - g.dpm[ojp + k]) / DYK * DXK * dz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float Pn = 0.0f, Qn = 0.0f;
+        for (int k = g.nkk - 1; k > KK; --k) {
+            float ATP = g.atp[k];
+            float ABP = (k > KK + 1) ? g.abp_in[k] : g.abp_bot[k];
+            float TERMP = 1 / (g.app[k] - ATP * Pn);
+            float P = ABP * TERMP;
+            float Q = (ATP * Qn + sDIM[k]) * TERMP;
+            sP[k] = P; sQ[k] = Q;
+            Pn = P; Qn = Q;
+        }
+        float dprev = 0.0f;                               // DPM[0] = 0, and DPM[k] = 0 up to KKART
+        for (int k = 1; k <= g.nkk - 1; ++k) {
+            float v = (KK < k) ? sP[k] * dprev + sQ[k] : 0.0f;
+            sN[k] = v;
+            dprev = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) g.dpm[o] = 0.0f;
+    for (int k = 1 + threadIdx.x; k <= g.nkk - 1; k += WAVE) g.dpm[o + k] = sN[k];
+}
+
+// velocity correction without quantisation (:345-410); thread per (column, level)
+__global__ void k_dg_correction(DiagArgs g) {
+    int nj = g.njj - 2, i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
+    size_t c = (size_t)i * g.nj2 + j;
+    int KK = g.kk[c], KKim = g.kk[c - g.nj2], KKjm = g.kk[c - 1];
+    size_t o = dg3(g, i, j), oim = dg3(g, i - 1, j), ojm = dg3(g, i, j - 1);
+    const float DXK = g.dxk, DYK = g.dyk, DT = g.dtime;
+    for (int k = 1 + threadIdx.x; k < g.nkk; k += blockDim.x) {
+        float d0 = g.dpm[o + k];
+        float DDPX = g.dpm[oim + k] - d0;
+        float DDPY = g.dpm[ojm + k] - d0;
+        float DDPZ = (k > KK + 1) ? g.dpm[o + k - 1] - d0 : 0.0f;
+        g.uk[o + k] = (KK < k && KKim < k) ? g.uk[o + k] + DDPX / DXK * DT : 0.0f;
+        g.vk[o + k] = (KK < k && KKjm < k) ? g.vk[o + k] + DDPY / DYK * DT : 0.0f;
+        g.wk[o + k] = (KK + 1 < k) ? g.wk[o + k] + DDPZ * 2 / (g.dzk[k] + g.dzk[k - 1]) * DT : 0.0f;
+    }
+}
+
+// vertical velocity from the mass balance of each cell, serial in k; thread per column
+__global__ void k_dg_wmass(DiagArgs g) {
+    int nj = g.njj - 2, ncol = (g.nii - 2) * nj;
+    int cidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cidx >= ncol) return;
+    int i = 2 + cidx / nj, j = 2 + cidx % nj;
+    size_t c = (size_t)i * g.nj2 + j;
+    if (!g.terrain && !(g.level <= 1 || g.adv[c] == 0)) return;
+    const float DXK = g.dxk, DYK = g.dyk, DSN = g.terrain ? g.dyk : g.dxk;
+    int KK = g.kk[c], KKim = g.kk[c - g.nj2], KKip = g.kk[c + g.nj2], KKjm = g.kk[c - 1], KKjp = g.kk[c + 1];
+    size_t o = dg3(g, i, j), oip = dg3(g, i + 1, j), ojp = dg3(g, i, j + 1);
+    for (int k = KK + 1; k < g.nkk; ++k) {
+        float dz = g.dzk[k];
+        float fwo1 = (k <= KKim) ? 0.0f : dz * DYK * g.uk[o + k];
+        float fwo2 = (k <= KKip) ? 0.0f : dz * DYK * g.uk[oip + k];
+        float fsn1 = (k <= KKjm) ? 0.0f : dz * DSN * g.vk[o + k];
+        float fsn2 = (k <= KKjp) ? 0.0f : dz * DSN * g.vk[ojp + k];
+        float fbt1 = ((k <= KK + 1) || (k == 1)) ? 0.0f : DXK * DYK * g.wk[o + k];
+        g.wk[o + k + 1] = (fwo1 - fwo2 + fsn1 - fsn2 + fbt1) / (DXK * DYK);
+    }
+}
+
+extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStream_t s) {
+    const DiagArgs &g = *ga;
+    int ncol = (g.nii - 2) * (g.njj - 2);
+    hipLaunchKernelGGL(k_dg_zero, dim3(g.nii * g.njj), dim3(64), 0, s, g);
+    for (int it = 1; it <= iterations; ++it) {
+        hipLaunchKernelGGL(k_dg_divergence, dim3(ncol), dim3(64), 0, s, g);
+        int IS = 1, JS = 1;
+        switch (it % 4) {
+            case 0: IS = 1; JS = 1; break;
+            case 1: IS = 1; JS = -1; break;
+            case 2: IS = -1; JS = -1; break;
+            default: IS = -1; JS = 1; break;
+        }
+        int oa = 2, ob = g.njj - 1, ia = 2, ib = g.nii - 1;
+        size_t shm = sizeof(float) * 4 * (size_t)g.nkh;
+        for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t)
+            hipLaunchKernelGGL(k_dg_pressure, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, g, t, IS, JS);
+        hipLaunchKernelGGL(k_dg_correction, dim3(ncol), dim3(64), 0, s, g);
+    }
+    hipLaunchKernelGGL(k_dg_wmass, dim3((ncol + 63) / 64), dim3(64), 0, s, g);
+    return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// host-side launchers
+// ------------------------------------------------------------------------
+static inline int lds_slot_h(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
 // longest diagonal t over the situations; the pressure sweep runs a over j
 // (outer) and b over i, the momentum sweeps a over i and b over j
 static inline int diag_len_max(const Dom *hd, int nsit, int t, int oa, int ia, bool pressure) {
@@ -1687,16 +2261,24 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_top, dim3((ncols + 255) / 256, 1, Z), dim3(256), 0, s, dd); });
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncols, 1, Z), dim3(64), 0, s, dd); });
     size_t nrow = (size_t)nj2 * nkp;
+    for (int q = 0; q < nsit; ++q) nrow = max(nrow, (size_t)4 * hd[q].sync_stride);
     L.go(FAM_STENCIL, [&] {
         hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((nrow + 255) / 256), 1, Z), dim3(256), 0, s, dd, d0.tiled ? 0 : 1);
     });
-    if (d0.tiled) {
+    if (d0.tiled == 2) {
+        int nst = 0;
+        for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].nii - 1 - 2 + TILE) / TILE);
+        size_t shm = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + 2 * nkp + 8 + WAVE) + sizeof(int) * (5 * TILE + 4);
+        L.go(FAM_PRESSURE, [&] {
+            hipLaunchKernelGGL(k_pressure_strip, dim3(nst, 1, Z), dim3(TILE * WAVE), shm, s, dd, IS, JS, slot);
+        });
+    } else if (d0.tiled) {
         int nt = 0;
         for (int q = 0; q < nsit; ++q) {
             int nta = (hd[q].njj - 1 - 2 + TILE) / TILE, ntb = (hd[q].nii - 1 - 2 + TILE) / TILE;
             nt = max(nt, nta + ntb - 1);
         }
-        size_t shm = sizeof(float) * ((size_t)6 * lds_slot_h(nkp) * TILE + 2 * nkp + 8) + sizeof(int4) * TILE * TILE + sizeof(int) * 4 * TILE;
+        size_t shm = sizeof(float) * ((size_t)6 * lds_slot_h(nkp) * TILE + 2 * nkp + 8 + WAVE) + sizeof(int4) * TILE * TILE + sizeof(int) * 4 * TILE;
         for (int tau = 0; tau < nt; ++tau) {
             int cnt = 0;
             for (int q = 0; q < nsit; ++q) {
@@ -1739,9 +2321,9 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
             nblk = max(nblk, (na * nb + COEF_COLS - 1) / COEF_COLS);
             nt = max(nt, (ob - oa + TILE) / TILE + (ib - ia + TILE) / TILE - 1);
         }
-        size_t shm_c = sizeof(float) * 3 * (nkp | 1) * COEF_COLS + sizeof(int) * 4 * COEF_COLS;
+        size_t shm_c = sizeof(float) * 4 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
         L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
-        size_t shm_t = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + nkp + 8) + sizeof(int4) * TILE * TILE + sizeof(int) * 5 * TILE;
+        size_t shm_t = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + nkp + 8 + WAVE) + sizeof(int4) * TILE * TILE + sizeof(int) * 5 * TILE;
         for (int tau = 0; tau < nt; ++tau) {
             int cnt = 0;
             for (int q = 0; q < nsit; ++q) {
@@ -1752,7 +2334,24 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
             L.go(fam, [&] { hipLaunchKernelGGL(tile, dim3(cnt, 1, Z), dim3(TILE * WAVE), shm_t, s, dd, tau, is, js); });
         }
     };
-    if (d0.model == 1 && d0.tiled) {
+    auto strip_sweep = [&](int fam, auto coef, auto strip, int oa, int ia, int is, int js) {
+        int nblk = 0, nst = 0;
+        for (int q = 0; q < nsit; ++q) {
+            int ob = hd[q].nii - 1, ib = hd[q].njj - 1;
+            int na = ob - oa + 1, nb = ib - ia + 1;
+            nblk = max(nblk, (na * nb + COEF_COLS - 1) / COEF_COLS);
+            nst = max(nst, (ib - ia + TILE) / TILE);
+        }
+        size_t shm_c = sizeof(float) * 4 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
+        L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
+        size_t shm_t = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + nkp + 8 + WAVE) + sizeof(int) * (6 * TILE + 4);
+        L.go(fam, [&] { hipLaunchKernelGGL(strip, dim3(nst, 1, Z), dim3(TILE * WAVE), shm_t, s, dd, is, js); });
+    };
+    if (d0.model == 1 && d0.tiled == 2) {
+        strip_sweep(FAM_U, k_ml_coef<0>, k_ml_strip<0>, 3, 2, IS, JS);
+        strip_sweep(FAM_V, k_ml_coef<1>, k_ml_strip<1>, 2, 3, -IS, -JS);
+        strip_sweep(FAM_W, k_ml_coef<2>, k_ml_strip<2>, 2, 2, IS, JS);
+    } else if (d0.model == 1 && d0.tiled) {
         tiled_sweep(FAM_U, k_ml_coef<0>, k_ml_tile<0>, 3, 2, IS, JS);
         tiled_sweep(FAM_V, k_ml_coef<1>, k_ml_tile<1>, 2, 3, -IS, -JS);
         tiled_sweep(FAM_W, k_ml_coef<2>, k_ml_tile<2>, 2, 2, IS, JS);

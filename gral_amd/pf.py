@@ -23,6 +23,7 @@ I16P = C.POINTER(C.c_int16)
 
 EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral_pf_upload_fields",
            "gral_pf_prepare", "gral_pf_iterate", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
+           "gral_pf_diagnostic",
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
            "gral_pf_last_error", "gral_pf_destroy", "gral_pf_batch_last_error", "gral_pf_batch_destroy"]
@@ -79,6 +80,7 @@ def load(path: str = LIB_PATH):
     lib.gral_pf_finish.argtypes = [H, C.POINTER(Stats)]
     lib.gral_pf_solve.argtypes = [H, C.POINTER(Params), C.POINTER(Stats)]
     lib.gral_pf_download_fields.argtypes = [H] + [FP] * 10
+    lib.gral_pf_diagnostic.argtypes = [H, C.c_int32, C.c_int32]
     lib.gral_pf_last_timing.argtypes = [H, C.POINTER(Timing)]
     lib.gral_pf_profile_iterations.argtypes = [H, C.c_int32, C.POINTER(FamilyTime), C.c_int32]
     lib.gral_pf_batch_create.argtypes = [C.POINTER(H), C.c_int32, C.POINTER(H)]
@@ -184,6 +186,10 @@ class PrognosticFlowfield:
         self._check(self.lib.gral_pf_profile_iterations(self.h, int(n), arr, len(FAMILIES)))
         return {f: {"ms": arr[q].ms, "launches": arr[q].launches, "cells": arr[q].cells}
                 for q, f in enumerate(FAMILIES) if arr[q].launches > 0}
+
+    def diagnostic(self, flow_field_level=2, terrain=False):
+        """The mass-conservation post-pass (DiagnosticFlowfield + final W step)."""
+        self._check(self.lib.gral_pf_diagnostic(self.h, int(flow_field_level), int(bool(terrain))))
 
     def download(self):
         shp = self.case.uk.shape

@@ -114,8 +114,21 @@ int gral_pf_finish(gral_pf_handle *h, gral_pf_stats *st);
 /* The whole of Calculate(): prepare + iterate(iter_max, check) + finish. */
 int gral_pf_solve(gral_pf_handle *h, const gral_pf_params *p, gral_pf_stats *st);
 
+/* The mass-conservation post-pass that follows the prognostic solve:
+ * DiagnosticFlowfield.Calculate() (DiagnosticFlowfield.cs:24-436; 10 pressure
+ * iterations for flow_field_level 2, 5 for level 1) and the caller's final
+ * adjustment of the vertical velocity from the mass balance of each cell:
+ * terrain = 0 as MicroscaleFlat.cs:388-462 (outside the sub-domains at level
+ * 2, everywhere at level 1), terrain = 1 as MicroscaleTerrain.cs:862-933
+ * (every column).  Works on the full-height fields of the handle (after
+ * gral_pf_iterate/finish, or right after upload_fields for a purely
+ * diagnostic run); download_fields returns the result. */
+int gral_pf_diagnostic(gral_pf_handle *h, int32_t flow_field_level, int32_t terrain);
+
 /* Copy the solver state to host buffers (each may be NULL), full depth
- * [nii+2][njj+2][nkk+2]: Program.UK/VK/WK/UKS/VKS/WKS/DPM/DPMNEW/TURB/TDISS. */
+ * [nii+2][njj+2][nkk+2]: Program.UK/VK/WK/UKS/VKS/WKS/DPM/DPMNEW/TURB/TDISS.
+ * Valid after prepare, or after upload_fields (+ diagnostic) for a purely
+ * diagnostic run, where the fields the solver never set come back as zeros. */
 int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, float *wk, float *uks,
                             float *vks, float *wks, float *dpm, float *dpmnew, float *turb,
                             float *tdiss);

@@ -1321,3 +1321,120 @@ static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, o
     free(s->P1); free(s->Q1); free(s->P2); free(s->Q2);
     return 0;
 }
+
+/* ===================================================================== */
+/*  DiagnosticFlowfield.Calculate, DiagnosticFlowfield.cs:24-436, and the   */
+/*  vertical-velocity mass adjustment of MicroscaleFlat.cs:388-462 (or      */
+/*  MicroscaleTerrain.cs:862-933)                                           */
+/* ===================================================================== */
+int orc_diagnostic(const orc_diag_params *p, const float *DZK, const uint8_t *ADVDOM, const int16_t *KKART,
+                   float *UK, float *VK, float *WK, float *DPM, float *DIV, float *dtime_out) {
+    if (!p || !DZK || !ADVDOM || !KKART || !UK || !VK || !WK || !DPM || !DIV) return -1;
+    if (p->flow_field_level != 1 && p->flow_field_level != 2) return -2;
+    const int NII = p->nii, NJJ = p->njj, NKK = p->nkk, NJ2 = NJJ + 2, NKH = NKK + 2;
+    const float DXK = p->dxk, DYK = p->dyk;
+#define D2(i, j) ((size_t)(i) * (size_t)NJ2 + (size_t)(j))
+#define D3(i, j, k) (D2(i, j) * (size_t)NKH + (size_t)(k))
+    /* DTIME = (float)(0.5*Math.Min(DXK,DYK)/Math.Sqrt(Pow2(UK)+Pow2(VK))) at (1,1,KADVMAX), :30-31 */
+    float u = UK[D3(1, 1, p->kadvmax)], v = VK[D3(1, 1, p->kadvmax)];
+    float s2 = u * u + v * v;
+    float DTIME = (float)(0.5 * (double)mminf_(DXK, DYK) / sqrt((double)s2));
+    DTIME = mminf_(DTIME, 5.0f);
+    if (dtime_out) *dtime_out = DTIME;
+    int Iterations = (p->flow_field_level == 2) ? 10 : 5;
+    if (p->iterations > 0) Iterations = p->iterations;
+    for (int i = 1; i <= NII; ++i)
+        for (int j = 1; j <= NJJ; ++j)
+            for (int k = 1; k <= NKK + 1; ++k) DPM[D3(i, j, k)] = 0;
+    float *PIMP = (float *)calloc((size_t)NKK + 2, sizeof(float));
+    float *QIMP = (float *)calloc((size_t)NKK + 2, sizeof(float));
+    for (int it = 1; it <= Iterations; ++it) {
+        /* mass divergence, :53-80 */
+        for (int i = 2; i < NII; ++i)
+            for (int j = 2; j < NJJ; ++j) {
+                int KK = KKART[D2(i, j)];
+                const float *uk = &UK[D3(i, j, 0)], *ukp = &UK[D3(i + 1, j, 0)];
+                const float *vk = &VK[D3(i, j, 0)], *vkp = &VK[D3(i, j + 1, 0)], *wk = &WK[D3(i, j, 0)];
+                float *dv = &DIV[D3(i, j, 0)];
+                for (int k = 1; k <= NKK - 1; ++k) {
+                    if (!(KK < k)) continue;
+                    if (k > KK + 1)
+                        dv[k] = (uk[k] - ukp[k]) * DYK * DZK[k] + (vk[k] - vkp[k]) * DXK * DZK[k] + (wk[k] - wk[k + 1]) * DXK * DYK;
+                    else
+                        dv[k] = (uk[k] - ukp[k]) * DYK * DZK[k] + (vk[k] - vkp[k]) * DXK * DZK[k] - wk[k + 1] * DXK * DYK;
+                }
+            }
+        /* pressure correction, :84-341: four sweep orders by IterationLoops % 4.
+         * Every column is solved over its full height, so the scratch above the
+         * top level (PIMP/QIMP[NKK]) is never written: zeros. */
+        int IS = 1, JS = 1;
+        switch (it % 4) {
+            case 0: IS = 1; JS = 1; break;
+            case 1: IS = 1; JS = -1; break;
+            case 2: IS = -1; JS = -1; break;
+            default: IS = -1; JS = 1; break;
+        }
+        for (int j1 = 2; j1 < NJJ; ++j1) {
+            int j = (JS == -1) ? NJJ - j1 + 1 : j1;
+            for (int i1 = 2; i1 < NII; ++i1) {
+                int i = (IS == -1) ? NII - i1 + 1 : i1;
+                int KK = KKART[D2(i, j)];
+                const float *dv = &DIV[D3(i, j, 0)];
+                float *dpm = &DPM[D3(i, j, 0)];
+                const float *dim_ = &DPM[D3(i - 1, j, 0)], *dip = &DPM[D3(i + 1, j, 0)];
+                const float *djm = &DPM[D3(i, j - 1, 0)], *djp = &DPM[D3(i, j + 1, 0)];
+                for (int k = NKK - 1; k >= 1; --k) {
+                    if (!(KK < k)) continue;
+                    float DIM = dv[k] / DTIME + (dim_[k] + dip[k]) / DXK * DYK * DZK[k] + (djm[k] - djp[k]) / DYK * DXK * DZK[k];
+                    float APP = 2 * (DYK * DZK[k] / DXK + DXK * DZK[k] / DYK + DXK * DYK / DZK[k]);
+                    float ABP = (k > KK + 1) ? DXK * DYK / (0.5f * (DZK[k - 1] + DZK[k])) : DXK * DYK / DZK[k];
+                    float ATP = (DXK * DYK) / (0.5f * (DZK[k + 1] + DZK[k]));
+                    float TERMP = 1 / (APP - ATP * PIMP[k + 1]);
+                    PIMP[k] = ABP * TERMP;
+                    QIMP[k] = (ATP * QIMP[k + 1] + DIM) * TERMP;
+                }
+                dpm[0] = 0;
+                for (int k = 1; k <= NKK - 1; ++k)
+                    dpm[k] = (KK < k) ? PIMP[k] * dpm[k - 1] + QIMP[k] : 0.0f;
+            }
+        }
+        /* velocity correction, :345-410 (no quantisation here) */
+        for (int i = 2; i < NII; ++i)
+            for (int j = 2; j < NJJ; ++j) {
+                int KK = KKART[D2(i, j)], KKim = KKART[D2(i - 1, j)], KKjm = KKART[D2(i, j - 1)];
+                const float *dpm = &DPM[D3(i, j, 0)], *dpmi = &DPM[D3(i - 1, j, 0)], *dpmj = &DPM[D3(i, j - 1, 0)];
+                float *uk = &UK[D3(i, j, 0)], *vk = &VK[D3(i, j, 0)], *wk = &WK[D3(i, j, 0)];
+                for (int k = 1; k < NKK; ++k) {
+                    float DDPX = dpmi[k] - dpm[k];
+                    float DDPY = dpmj[k] - dpm[k];
+                    float DDPZ = (k > KK + 1) ? dpm[k - 1] - dpm[k] : 0.0f;
+                    if (KK < k && KKim < k) uk[k] += DDPX / DXK * DTIME; else uk[k] = 0;
+                    if (KK < k && KKjm < k) vk[k] += DDPY / DYK * DTIME; else vk[k] = 0;
+                    if (KK + 1 < k) wk[k] += DDPZ * 2 / (DZK[k] + DZK[k - 1]) * DTIME; else wk[k] = 0;
+                }
+            }
+    }
+    free(PIMP); free(QIMP);
+    /* vertical velocity from the mass balance of each cell, MicroscaleFlat.cs:388-462 */
+    for (int i = 2; i < NII; ++i)
+        for (int j = 2; j < NJJ; ++j) {
+            if (!p->terrain && !(p->flow_field_level <= 1 || ADVDOM[D2(i, j)] == 0)) continue;
+            const float DSN = p->terrain ? DYK : DXK;   /* MicroscaleTerrain uses DYK on the j faces */
+            const float *uk = &UK[D3(i, j, 0)], *uki = &UK[D3(i + 1, j, 0)];
+            const float *vk = &VK[D3(i, j, 0)], *vkj = &VK[D3(i, j + 1, 0)];
+            float *wk = &WK[D3(i, j, 0)];
+            int KK = KKART[D2(i, j)];
+            for (int k = 1; k < NKK; ++k) {
+                if (!(k > KK)) continue;
+                float fwo1 = (k <= KKART[D2(i - 1, j)]) ? 0.0f : DZK[k] * DYK * uk[k];
+                float fwo2 = (k <= KKART[D2(i + 1, j)]) ? 0.0f : DZK[k] * DYK * uki[k];
+                float fsn1 = (k <= KKART[D2(i, j - 1)]) ? 0.0f : DZK[k] * DSN * vk[k];
+                float fsn2 = (k <= KKART[D2(i, j + 1)]) ? 0.0f : DZK[k] * DSN * vkj[k];
+                float fbt1 = ((k <= KK + 1) || (k == 1)) ? 0.0f : DXK * DYK * wk[k];
+                wk[k + 1] = (fwo1 - fwo2 + fsn1 - fsn2 + fbt1) / (DXK * DYK);
+            }
+        }
+#undef D2
+#undef D3
+    return 0;
+}

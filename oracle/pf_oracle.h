@@ -72,6 +72,25 @@ typedef struct {
  * Returns 0 on success, negative on invalid input. */
 int orc_calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st);
 
+/* DiagnosticFlowfield.Calculate() (DiagnosticFlowfield.cs:24-436) followed by
+ * the final mass adjustment of the vertical velocity (MicroscaleFlat.cs:388-462,
+ * or MicroscaleTerrain.cs:862-933 with terrain = 1).
+ * flow_field_level 1 (diagnostic): 5 pressure iterations, W adjusted in every
+ * column; 2 (prognostic): 10 iterations, W adjusted outside the sub-domains
+ * (ADVDOM == 0).  iterations > 0 overrides the count (checks only).
+ * Fields are full height [nii+2][njj+2][nkk+2]: uk/vk/wk in/out, dpm out
+ * (levels 1..NKK+1 of columns 1..NII x 1..NJJ are reset), div scratch. */
+typedef struct {
+    int nii, njj, nkk, kadvmax;
+    float dxk, dyk;
+    int flow_field_level;
+    int iterations;
+    int terrain;                /* 1: the step as MicroscaleTerrain.cs:862-933 does it (every column, DYK on
+                                   the j faces); 0: as MicroscaleFlat.cs:388-462 */
+} orc_diag_params;
+int orc_diagnostic(const orc_diag_params *p, const float *dzk, const uint8_t *advdom, const int16_t *kkart,
+                   float *uk, float *vk, float *wk, float *dpm, float *div, float *dtime_out);
+
 /* ---- checking hooks (closed-form and textbook checks of the oracle) ---- */
 
 /* Records, for one column (i,j) during one SIMPLE iteration, the coefficients

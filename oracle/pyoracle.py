@@ -56,6 +56,11 @@ class OrcCapture(C.Structure):
                 ("a", FP), ("b", FP), ("c", FP), ("d", FP), ("x_old", FP), ("x_new", FP), ("captured", C.c_int)]
 
 
+class OrcDiagParams(C.Structure):
+    _fields_ = [("nii", C.c_int), ("njj", C.c_int), ("nkk", C.c_int), ("kadvmax", C.c_int), ("dxk", C.c_float),
+                ("dyk", C.c_float), ("flow_field_level", C.c_int), ("iterations", C.c_int), ("terrain", C.c_int)]
+
+
 _lib = None
 
 
@@ -77,6 +82,8 @@ def load(build_if_missing: bool = True):
         getattr(_lib, n).restype = C.c_float
     _lib.orc_conv_to_int.argtypes = [C.c_double]
     _lib.orc_conv_to_int.restype = C.c_int
+    _lib.orc_diagnostic.argtypes = [C.POINTER(OrcDiagParams), FP, U8P, I16P, FP, FP, FP, FP, FP, FP]
+    _lib.orc_diagnostic.restype = C.c_int
     return _lib
 
 
@@ -136,4 +143,28 @@ def run(case, iterations=None, simd_width=None, keep_history=True, capture=None)
     stats = {n: getattr(st, n) for n, _ in OrcStats._fields_ if n != "delta_umax_hist"}
     stats["delta_umax_hist"] = hist[: st.iterations_run].copy()
     out["stats"] = stats
+    return out
+
+
+def diagnostic(case, fields=None, flow_field_level=2, iterations=0, terrain=False):
+    """DiagnosticFlowfield.Calculate() and the final vertical-velocity mass
+    adjustment of MicroscaleFlat on ``fields`` (dict with uk, vk, wk, dpm; by
+    default the case's initial wind field and zero DPM).  Returns new arrays
+    uk, vk, wk, dpm, div and the time step used."""
+    lib = load()
+    c = case
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).copy()
+    src = fields if fields is not None else {"uk": c.uk, "vk": c.vk, "wk": c.wk, "dpm": np.zeros_like(c.uk)}
+    out = {n: f32(src[n]) for n in ("uk", "vk", "wk", "dpm")}
+    out["div"] = np.zeros_like(out["uk"])
+    dzk = np.ascontiguousarray(c.dzk, dtype=np.float32)
+    adv = np.ascontiguousarray(c.advdom, dtype=np.uint8)
+    kk = np.ascontiguousarray(c.kkart, dtype=np.int16)
+    prm = OrcDiagParams(c.nii, c.njj, c.nkk, c.kadvmax, c.dxk, c.dyk, flow_field_level, iterations, int(bool(terrain)))
+    dt = C.c_float(0)
+    rc = lib.orc_diagnostic(C.byref(prm), _p(dzk), _p(adv, U8P), _p(kk, I16P), _p(out["uk"]), _p(out["vk"]),
+                            _p(out["wk"]), _p(out["dpm"]), _p(out["div"]), C.cast(C.pointer(dt), FP))
+    if rc != 0:
+        raise RuntimeError(f"orc_diagnostic failed: {rc}")
+    out["dtime"] = dt.value
     return out

@@ -14,7 +14,9 @@ Fixture files:
                               few-cell domain (inputs regenerated from the spec)
   golden.json              -- per case: generator spec, SHA-256 of the inputs
                               and of every output field (float32 bit patterns,
-                              levels 0..KADVMAX+1), and the solver statistics
+                              levels 0..KADVMAX+1), and the solver statistics;
+                              for the post-pass cases (key "diagnostic") the
+                              digests cover the full column height
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -48,6 +50,15 @@ CASES = [
     ("c1_m1_it200", "config1", dict(turbulence_model=1), 200, False),
     ("c1_m2_it4", "config1", dict(turbulence_model=2), 4, False),
 ]
+
+
+# mass-conservation post-pass after the prognostic iterations (full-height digests)
+# (name, generator, kwargs, prognostic iterations, flow_field_level, terrain)
+DIAG_CASES = [
+    ("c1_m1_it4_diag2", "config1", dict(turbulence_model=1), 4, 2, False),
+    ("tiny_m1_diag1", "tiny_case", dict(n=10, vertcellsff=9, turbulence_model=1), 0, 1, False),
+]
+DIAG_FIELDS = ("uk", "vk", "wk", "dpm")
 
 
 def make_case(gen, kwargs, iterations):
@@ -91,6 +102,19 @@ def main():
         if full:
             np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **{f: crop(r[f], c.kadvmax) for f in flds})
         print(name, meta[name]["stats"]["iteration_loops"], flush=True)
+    for name, gen, kwargs, its, level, terrain in DIAG_CASES:
+        c = make_case(gen, kwargs, max(its, 1))
+        f = None
+        if its:
+            r = pyoracle.run(c, iterations=its, keep_history=False)
+            f = {k: r[k] for k in DIAG_FIELDS}
+        d = pyoracle.diagnostic(c, f, flow_field_level=level, terrain=terrain)
+        meta[name] = {
+            "generator": gen, "kwargs": kwargs, "iterations": its, "kadvmax": int(c.kadvmax), "diagnostic": level,
+            "terrain": terrain, "input_sha256": input_digest(c),
+            "field_sha256": {k: digest(d[k]) for k in DIAG_FIELDS}, "dtime": float(d["dtime"]),
+        }
+        print(name, flush=True)
     with open(os.path.join(HERE, "golden.json"), "w") as fh:
         json.dump(meta, fh, indent=1, sort_keys=True)
 

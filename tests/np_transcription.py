@@ -838,3 +838,98 @@ class Solver:
 
 def run(case, iterations):
     return Solver(case, iterations).run()
+
+
+# ----------------------------------------------------------------------------
+# DiagnosticFlowfield.Calculate (DiagnosticFlowfield.cs:24-436) and the final
+# vertical-velocity mass adjustment of MicroscaleFlat.cs:388-462.  Stencil
+# passes are whole-array float32 numpy operations (elementwise IEEE single
+# precision, same operation order as the C#); the Gauss-Seidel pressure sweep
+# loops over columns in Python.
+# ----------------------------------------------------------------------------
+def diagnostic(case, fields, level, iterations=0, terrain=False):
+    c = case
+    NII, NJJ, NKK = c.nii, c.njj, c.nkk
+    DXK, DYK = F(c.dxk), F(c.dyk)
+    DZK = c.dzk.astype(F)
+    KK = c.kkart.astype(np.int64)
+    UK = fields["uk"].astype(F).copy()
+    VK = fields["vk"].astype(F).copy()
+    WK = fields["wk"].astype(F).copy()
+    DPM = fields["dpm"].astype(F).copy()
+    DIV = np.zeros_like(UK)
+    u, v = UK[1, 1, c.kadvmax], VK[1, 1, c.kadvmax]
+    s2 = F(F(u * u) + F(v * v))
+    DTIME = F(0.5 * float(net_min(DXK, DYK)) / math.sqrt(float(s2)) if s2 > 0 else math.inf)
+    DTIME = F(net_min(float(DTIME), 5.0))
+    n_it = iterations if iterations > 0 else (10 if level == 2 else 5)
+    DPM[1:NII + 1, 1:NJJ + 1, 1:NKK + 2] = 0
+    I = slice(2, NII)
+    J = slice(2, NJJ)
+    ks = np.arange(NKK + 2)
+    above = ks[None, None, :] > KK[:, :, None]                       # KKART < k
+    for it in range(1, n_it + 1):
+        # divergence over every interior column, k = 1..NKK-1
+        dz = DZK[None, None, 1:NKK]
+        t1 = F(UK[I, J, 1:NKK] - UK[3:NII + 1, J, 1:NKK]) * DYK * dz
+        t2 = F(VK[I, J, 1:NKK] - VK[I, 3:NJJ + 1, 1:NKK]) * DXK * dz
+        top = WK[I, J, 2:NKK + 1] * DXK * DYK
+        inner = t1 + t2 + F(WK[I, J, 1:NKK] - WK[I, J, 2:NKK + 1]) * DXK * DYK
+        bottom = t1 + t2 - top
+        kk = KK[I, J][:, :, None]
+        k3 = ks[None, None, 1:NKK]
+        new = np.where(k3 > kk + 1, inner, bottom).astype(F)
+        DIV[I, J, 1:NKK] = np.where(k3 > kk, new, DIV[I, J, 1:NKK])
+        # pressure sweep
+        IS, JS = {0: (1, 1), 1: (1, -1), 2: (-1, -1), 3: (-1, 1)}[it % 4]
+        PIMP = np.zeros(NKK + 2, dtype=F)
+        QIMP = np.zeros(NKK + 2, dtype=F)
+        for j1 in range(2, NJJ):
+            j = NJJ - j1 + 1 if JS == -1 else j1
+            for i1 in range(2, NII):
+                i = NII - i1 + 1 if IS == -1 else i1
+                kb = int(KK[i, j])
+                for k in range(NKK - 1, kb, -1):
+                    dzk = DZK[k]
+                    DIM = F(F(F(DIV[i, j, k] / DTIME) + F(F(F(DPM[i - 1, j, k] + DPM[i + 1, j, k]) / DXK) * DYK) * dzk)
+                            + F(F(F(F(DPM[i, j - 1, k] - DPM[i, j + 1, k]) / DYK) * DXK) * dzk))
+                    APP = F(F(2) * F(F(F(F(DYK * dzk) / DXK) + F(F(DXK * dzk) / DYK)) + F(F(DXK * DYK) / dzk)))
+                    if k > kb + 1:
+                        ABP = F(F(DXK * DYK) / F(F(0.5) * F(DZK[k - 1] + dzk)))
+                    else:
+                        ABP = F(F(DXK * DYK) / dzk)
+                    ATP = F(F(DXK * DYK) / F(F(0.5) * F(DZK[k + 1] + dzk)))
+                    TERMP = F(F(1) / F(APP - F(ATP * PIMP[k + 1])))
+                    PIMP[k] = F(ABP * TERMP)
+                    QIMP[k] = F(F(F(ATP * QIMP[k + 1]) + DIM) * TERMP)
+                DPM[i, j, 0] = 0
+                for k in range(1, NKK):
+                    DPM[i, j, k] = F(F(PIMP[k] * DPM[i, j, k - 1]) + QIMP[k]) if kb < k else F(0)
+        # velocity correction, whole interior at once
+        d0 = DPM[I, J, 1:NKK]
+        DDPX = F(DPM[1:NII - 1, J, 1:NKK] - d0)
+        DDPY = F(DPM[I, 1:NJJ - 1, 1:NKK] - d0)
+        DDPZ = np.where(k3 > kk + 1, F(DPM[I, J, 0:NKK - 1] - d0), F(0)).astype(F)
+        okU = (k3 > kk) & (k3 > KK[1:NII - 1, J][:, :, None])
+        okV = (k3 > kk) & (k3 > KK[I, 1:NJJ - 1][:, :, None])
+        UK[I, J, 1:NKK] = np.where(okU, F(UK[I, J, 1:NKK] + F(F(DDPX / DXK) * DTIME)), F(0))
+        VK[I, J, 1:NKK] = np.where(okV, F(VK[I, J, 1:NKK] + F(F(DDPY / DYK) * DTIME)), F(0))
+        dzs = F(DZK[None, None, 1:NKK] + DZK[None, None, 0:NKK - 1])
+        WK[I, J, 1:NKK] = np.where(k3 > kk + 1, F(WK[I, J, 1:NKK] + F(F(F(DDPZ * F(2)) / dzs) * DTIME)), F(0))
+    # vertical velocity from the mass balance of each cell (serial in k per column)
+    for i in range(2, NII):
+        for j in range(2, NJJ):
+            if not terrain and not (level <= 1 or c.advdom[i, j] == 0):
+                continue
+            DSN = DYK if terrain else DXK
+            kb = int(KK[i, j])
+            for k in range(1, NKK):
+                if k <= kb:
+                    continue
+                fwo1 = F(0) if k <= KK[i - 1, j] else F(F(DZK[k] * DYK) * UK[i, j, k])
+                fwo2 = F(0) if k <= KK[i + 1, j] else F(F(DZK[k] * DYK) * UK[i + 1, j, k])
+                fsn1 = F(0) if k <= KK[i, j - 1] else F(F(DZK[k] * DSN) * VK[i, j, k])
+                fsn2 = F(0) if k <= KK[i, j + 1] else F(F(DZK[k] * DSN) * VK[i, j + 1, k])
+                fbt1 = F(0) if (k <= kb + 1 or k == 1) else F(F(DXK * DYK) * WK[i, j, k])
+                WK[i, j, k + 1] = F(F(F(F(F(fwo1 - fwo2) + fsn1) - fsn2) + fbt1) / F(DXK * DYK))
+    return {"uk": UK, "vk": VK, "wk": WK, "dpm": DPM, "div": DIV, "dtime": float(DTIME)}

@@ -14,7 +14,9 @@ from tests.golden import make_golden as mg
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 META = json.load(open(os.path.join(GOLDEN, "golden.json")))
-CPU_CASES = [n for n in META if n != "c1_m1_it200"]
+SOLVE = [n for n, m in META.items() if "diagnostic" not in m]
+DIAG = [n for n, m in META.items() if "diagnostic" in m]
+CPU_CASES = [n for n in SOLVE if n != "c1_m1_it200"]
 
 
 def _check(name, out):
@@ -35,7 +37,7 @@ def _check(name, out):
 @pytest.mark.parametrize("name", list(META))
 def test_generator_reproduces_inputs(name):
     m = META[name]
-    c = mg.make_case(m["generator"], m["kwargs"], m["iterations"])
+    c = mg.make_case(m["generator"], m["kwargs"], max(1, m["iterations"]))
     assert mg.input_digest(c) == m["input_sha256"]
     assert c.kadvmax == m["kadvmax"]
 
@@ -49,7 +51,8 @@ def test_oracle_reproduces_fixture(name):
 
 
 def test_fixture_arrays_match_digests():
-    for name, m in META.items():
+    for name in SOLVE:
+        m = META[name]
         path = os.path.join(GOLDEN, f"{name}.npz")
         if not os.path.exists(path):
             continue
@@ -58,8 +61,45 @@ def test_fixture_arrays_match_digests():
             assert mg.digest(z[f]) == h, (name, f)
 
 
+def _diag_oracle(name):
+    from oracle import pyoracle
+    m = META[name]
+    c = mg.make_case(m["generator"], m["kwargs"], max(1, m["iterations"]))
+    f = None
+    if m["iterations"]:
+        r = pyoracle.run(c, iterations=m["iterations"], keep_history=False)
+        f = {k: r[k] for k in mg.DIAG_FIELDS}
+    return pyoracle.diagnostic(c, f, flow_field_level=m["diagnostic"], terrain=m["terrain"])
+
+
+@pytest.mark.parametrize("name", DIAG)
+def test_oracle_reproduces_diagnostic_fixture(name):
+    d = _diag_oracle(name)
+    for k, h in META[name]["field_sha256"].items():
+        assert mg.digest(d[k]) == h, (name, k)
+    assert d["dtime"] == META[name]["dtime"]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", list(META))
+@pytest.mark.parametrize("name", DIAG)
+def test_gpu_reproduces_diagnostic_fixture(name):
+    from gral_amd import pf
+    m = META[name]
+    c = mg.make_case(m["generator"], m["kwargs"], max(1, m["iterations"]))
+    print(f"[golden] {name}: {m['generator']} {m['kwargs']} it={m['iterations']} level={m['diagnostic']}", flush=True)
+    with pf.PrognosticFlowfield(c) as s:
+        if m["iterations"]:
+            s.prepare(m["iterations"])
+            s.iterate(m["iterations"])
+            s.finish()
+        s.diagnostic(m["diagnostic"], m["terrain"])
+        out = s.download()
+    for k, h in m["field_sha256"].items():
+        assert mg.digest(out[k]) == h, (name, k)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", SOLVE)
 def test_gpu_reproduces_fixture(name):
     from gral_amd import pf
     m = META[name]

@@ -122,3 +122,44 @@ def test_pressure_column_solve_against_banded_solver():
     got = cap["x_new"][1:vi + 1].astype(np.float64)
     assert np.abs(exact).max() > 0
     assert np.abs(got - exact).max() <= 5e-7 * np.abs(exact).max()
+
+
+def _imbalance(case, f):
+    """FP64 net volume flux out of every interior cell above the ground (blocked
+    faces closed), i.e. the mass balance the diagnostic pass restores."""
+    c = case
+    dx, dy = float(c.dxk), float(c.dyk)
+    dz = c.dzk.astype(np.float64)
+    U, V, W = (f[n].astype(np.float64) for n in ("uk", "vk", "wk"))
+    kk = c.kkart.astype(int)
+    out = []
+    for i in range(2, c.nii):
+        for j in range(2, c.njj):
+            for k in range(kk[i, j] + 1, c.nkk):
+                fw = 0.0 if k <= kk[i - 1, j] else dz[k] * dy * U[i, j, k]
+                fe = 0.0 if k <= kk[i + 1, j] else dz[k] * dy * U[i + 1, j, k]
+                fs = 0.0 if k <= kk[i, j - 1] else dz[k] * dx * V[i, j, k]
+                fn = 0.0 if k <= kk[i, j + 1] else dz[k] * dx * V[i, j + 1, k]
+                fb = 0.0 if (k <= kk[i, j] + 1 or k == 1) else dx * dy * W[i, j, k]
+                out.append(fw - fe + fs - fn + fb - dx * dy * W[i, j, k + 1])
+    return np.abs(np.array(out))
+
+
+def test_diagnostic_w_adjustment_closes_every_cell():
+    # in diagnostic mode (level 1) the final step sets W at each cell top so that
+    # the cell's net flux vanishes: a property of the result, not of the code path
+    c = domain.tiny_case(n=10, vertcellsff=9, turbulence_model=1, iterations=1)
+    r = pyoracle.diagnostic(c, None, flow_field_level=1)
+    scale = np.abs(r["uk"]).max() * float(c.dzk.max()) * float(c.dxk)
+    assert _imbalance(c, r).max() <= 1e-5 * scale
+
+
+def test_diagnostic_pressure_iterations_reduce_the_mass_imbalance():
+    # each pressure-correction iteration moves the field towards zero divergence;
+    # measured over the cells of the sub-domain, which the final W step leaves alone
+    c = domain.tiny_case(n=10, vertcellsff=9, turbulence_model=1, iterations=1)
+    f0 = {"uk": c.uk, "vk": c.vk, "wk": c.wk, "dpm": np.zeros_like(c.uk)}
+    imb = [_imbalance(c, f0).sum()]
+    for n in (1, 4, 16):
+        imb.append(_imbalance(c, pyoracle.diagnostic(c, f0, flow_field_level=2, iterations=n)).sum())
+    assert imb[0] > imb[1] > imb[2] > imb[3], imb

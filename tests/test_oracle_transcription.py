@@ -40,3 +40,25 @@ def test_two_buildings_tall_columns():
     bl = [domain.Building(3, 4, 3, 5, 6.5), domain.Building(6, 7, 4, 6, 3.2)]
     c = domain.build_case("two", 10, 10, bl, vertcellsff=10, direction=235.0, iterations=4)
     _agree(c, 4)
+
+
+@pytest.mark.parametrize("level,model,after,terrain", [(2, 1, 3, False), (1, 1, 0, False), (2, 2, 2, False),
+                                                     (2, 1, 2, True)])
+def test_diagnostic_pass(level, model, after, terrain):
+    # DiagnosticFlowfield + final W adjustment: on the prognostic result (level 2,
+    # all four sweep orders over 10 iterations) and on the initial field (level 1)
+    bl = [domain.Building(3, 4, 3, 5, 6.5), domain.Building(6, 7, 4, 6, 3.2)]
+    c = domain.build_case("two", 10, 10, bl, vertcellsff=10, direction=235.0, turbulence_model=model, iterations=3)
+    if after:
+        r = pyoracle.run(c, iterations=after)
+        f = {k: r[k] for k in ("uk", "vk", "wk", "dpm")}
+    else:
+        f = {"uk": c.uk, "vk": c.vk, "wk": c.wk, "dpm": np.zeros_like(c.uk)}
+    if terrain:
+        c.dyk = np.float32(2.5)                      # the terrain variant's DYK on the j faces must show
+    ref = pyoracle.diagnostic(c, f, flow_field_level=level, terrain=terrain)
+    npy = np_transcription.diagnostic(c, f, level, terrain=terrain)
+    assert ref["dtime"] == npy["dtime"]
+    for k in ("uk", "vk", "wk", "dpm"):
+        bad = np.argwhere(ref[k].view(np.uint32) != npy[k].view(np.uint32))
+        assert bad.size == 0, (k, bad[:5].tolist())
