@@ -96,8 +96,16 @@ def main():
     avg_ms = pdom["ms"] / max(1, pdom["launches"])
     bytes_per_launch = ALG_BYTES[dom] * pdom["cells"] / max(1, pdom["launches"])
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
+    # HBM bytes per launch of that kernel, from the calibrated FETCH_SIZE / WRITE_SIZE
+    # counter passes committed with the profiles (same workload and kernel version)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_city400_8dir.json")
+    if os.path.exists(tpath) and args.n == 400 and args.directions == 8:
+        kt = json.load(open(tpath))["kernels"].get(KERNEL_OF[dom])
+        if kt:
+            traffic = kt["traffic_bytes"]
     roofline = {"bound": "hbm", "kernel": KERNEL_OF[dom], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "avg_launch_us": avg_ms * 1e3, "launches_per_iter": pdom["launches"] / 2,
                 "alg_bytes_per_cell": ALG_BYTES[dom], "alg_bytes_per_launch": bytes_per_launch}
 
