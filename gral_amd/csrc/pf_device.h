@@ -45,7 +45,8 @@ struct DomT {
     int tiled;                       // 2: strip-pipelined, 1: tiled wavefront, 0: one launch per anti-diagonal
     P<int> sync;                     // strip sweeps: [64] header (sticky error flag at 0), then 4 blocks
     int sync_stride;                 //   of sync_stride ints (ticket, unused, progress per strip)
-    P<float> planes;                 // [cells][16] mixing-length coefficient records
+    P<float> planes;                 // [cells][12] mixing-length coefficient records (neighbour weights, source pieces)
+    P<float> rec4;                   // [cells][4]  C, TERMP, P and e of the same cells
     P<const float> ptab[2], ttab[2]; // pressure PIMP/TERMP per column for IS=+1 / IS=-1
     P<const int4> meta;              // per column: KK, VI, KSTART | adv<<16 | vegp<<17, KKm_i | KKm_j<<16
 };

@@ -35,6 +35,19 @@ __device__ __forceinline__ bool diag_col(int t, int oa, int ob, int ia, int ib, 
 // ------------------------------------------------------------------------
 // utility
 // ------------------------------------------------------------------------
+// Workgroups are dealt to the 8 XCDs (each with its own L2) round robin by
+// their position in the grid.  For kernels whose neighbouring blocks read each
+// other's columns, xcd_block() renumbers the blocks of a launch whose x extent is
+// a multiple of 8 so that every XCD takes one contiguous range of logical
+// blocks; it is a bijection on [0, n) and returns -1 for the padding blocks.
+#define NXCD 8
+__device__ __forceinline__ int xcd_block(int n) {
+    int b = blockIdx.x, x = b % NXCD, idx = b / NXCD;
+    int per = n / NXCD, rem = n % NXCD;
+    int cnt = per + (x < rem ? 1 : 0);
+    if (idx >= cnt) return -1;
+    return x * per + min(x, rem) + idx;
+}
 __global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) p[g] = v;
@@ -136,10 +149,10 @@ __global__ void k_bc_top(const Dom *__restrict__ ds) {
 // ------------------------------------------------------------------------
 __global__ void k_divergence(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    if ((int)blockIdx.x >= (d.nii - 2) * (d.njj - 2)) return;
-
+    const int col = xcd_block((d.nii - 2) * (d.njj - 2));
+    if (col < 0) return;
     int nj = d.njj - 2;
-    int i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
+    int i = 2 + col / nj, j = 2 + col % nj;
     size_t c = c2(d, i, j);
     if (d.adv[c] != 1) return;
     int KK = d.kk[c], VI = d.vi[c];
@@ -228,10 +241,10 @@ __global__ __launch_bounds__(WAVE) void k_pressure(const Dom *__restrict__ ds, i
 // ------------------------------------------------------------------------
 __global__ void k_correction(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    if ((int)blockIdx.x >= (d.nii - 2) * (d.njj - 2)) return;
-
+    const int col = xcd_block((d.nii - 2) * (d.njj - 2));
+    if (col < 0) return;
     int nj = d.njj - 2;
-    int i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
+    int i = 2 + col / nj, j = 2 + col % nj;
     size_t c = c2(d, i, j);
     if (!(d.adv[c] == 1 || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1)) return;
     int KK = d.kk[c], VI = d.vi[c], KKiM = d.kk[c2(d, i - 1, j)], KKjM = d.kk[c2(d, i, j - 1)];
@@ -260,10 +273,10 @@ __global__ void k_correction(const Dom *__restrict__ ds) {
 // ------------------------------------------------------------------------
 __global__ void k_averages(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    if ((int)blockIdx.x >= (d.nii - 2) * (d.njj - 2)) return;
-
+    const int col = xcd_block((d.nii - 2) * (d.njj - 2));
+    if (col < 0) return;
     int nj = d.njj - 2;
-    int i = 2 + blockIdx.x / nj, j = 2 + blockIdx.x % nj;
+    int i = 2 + col / nj, j = 2 + col % nj;
     size_t c = c2(d, i, j);
     if (!(d.adv[c] == 1 || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1)) return;
     int KK = d.kk[c], VI = d.vi[c];
@@ -1058,9 +1071,13 @@ __global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t,
 // tiles were written by earlier launches.
 #define TILE 16
 
-// coefficient planes written by k_ml_coef, read by k_ml_tile
+// coefficient records written by k_ml_coef, read by the tile and strip sweeps
 // array of structures: 16 floats per cell, in the order of the four float4 loads
-enum { PL_AW = 0, PL_AS, PL_AE, PL_AN, PL_C, PL_T, PL_P, PL_E, PL_F, PL_G, PL_W, PL_C0, PL_X1, PL_X2, PL_X3, PL_X4, PL_N };
+// per cell, two records: rec12 = {AW, AS, AE, AN | f, g, w, c0 | X1, X2, X3, X4}
+// (written once by the first pass of k_ml_coef) and rec4 = {C, T, P, e} (written
+// once after the P/TERMP recurrence); each is written whole, in one pass
+#define REC12 12
+#define REC4 4
 
 __device__ __forceinline__ void sweep_to_ij(const DomG &d, int comp_is_pressure, int a, int b, int IS, int JS, int &i, int &j) {
     if (comp_is_pressure) {   // a = j1, b = i1
@@ -1115,7 +1132,9 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
     // per column c: A, B, C, e over k at sm + c*4*nkl
     int *scol = (int *)(sm + 4 * COEF_COLS * nkl);         // [COEF_COLS][8]: KSTART, KK, VI, i, j, first item, active
     int *stot = scol + 8 * COEF_COLS;
-    const int col0 = blockIdx.x * COEF_COLS;
+    const int blk = xcd_block((na * nb + COEF_COLS - 1) / COEF_COLS);
+    if (blk < 0) return;                                      // uniform over the block
+    const int col0 = blk * COEF_COLS;
     if (threadIdx.x < WAVE) {
         int c = lane, n = 0;
         if (c < COEF_COLS) {
@@ -1157,10 +1176,10 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
         MlPieces p = ml_pieces<COMP>(d, q, k);
         float *sA = sm + (size_t)c * 4 * nkl;
         sA[k] = p.A; sA[nkl + k] = p.B; sA[2 * nkl + k] = p.C; sA[3 * nkl + k] = p.e;
-        gf32 *P = planes + (c3(d, q.i, q.j) + k) * PL_N;
+        gf32 *P = planes + (c3(d, q.i, q.j) + k) * REC12;
         stg4(P, p.AW1, p.AS1, p.AE1, p.AN1);
-        stg4(P + 8, p.f, p.g, p.w, p.c0);
-        stg4(P + 12, p.X1, p.X2, p.X3, p.X4);
+        stg4(P + 4, p.f, p.g, p.w, p.c0);
+        stg4(P + 8, p.X1, p.X2, p.X3, p.X4);
     }
     lds_barrier();
     if (threadIdx.x < COEF_COLS && scol[threadIdx.x * 8 + 6]) {
@@ -1183,8 +1202,8 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
         const int c = column_of(t);
         const int k = scol[c * 8 + 0] + (t - scol[c * 8 + 5]);
         const float *sA = sm + (size_t)c * 4 * nkl;
-        gf32 *P = planes + (c3(d, scol[c * 8 + 3], scol[c * 8 + 4]) + k) * PL_N;
-        stg4(P + 4, sA[2 * nkl + k], sA[k], sA[nkl + k], sA[3 * nkl + k]);   // C, T, P, e
+        gf32 *P = d.rec4 + (c3(d, scol[c * 8 + 3], scol[c * 8 + 4]) + k) * REC4;
+        stg4(P, sA[2 * nkl + k], sA[k], sA[nkl + k], sA[3 * nkl + k]);   // C, T, P, e
     }
 }
 
@@ -1343,8 +1362,8 @@ template <int COMP>
 __device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *planes, size_t n3, int i, int j, int k, int IS, int JS,
                                             MlPre &r) {
     (void)n3;
-    const gf32 *P = planes + (c3(d, i, j) + k) * PL_N;
-    float4 v0 = ldg4(P), v1 = ldg4(P + 4), v2 = ldg4(P + 8), v3 = ldg4(P + 12);
+    const gf32 *P = planes + (c3(d, i, j) + k) * REC12;
+    float4 v0 = ldg4(P), v2 = ldg4(P + 4), v3 = ldg4(P + 8), v1 = ldg4(d.rec4 + (c3(d, i, j) + k) * REC4);
     r.aw = v0.x; r.as = v0.y; r.ae = v0.z; r.an = v0.w;
     r.c = v1.x; r.t = v1.y; r.p = v1.z; r.e = v1.w;
     r.f = v2.x; r.g = v2.y; r.w = v2.z; r.c0 = v2.w;
@@ -1701,6 +1720,10 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_tile(const Dom *__rest
 // kernel always ends; the host reports the error.
 // ------------------------------------------------------------------------
 #define SYNC_HDR 2
+// a strip publishes its progress every STRIP_PUBLISH steps (and at its end): the
+// release (L2 write-back) and the reader's acquire (L2 invalidate) are paid once
+// per STRIP_PUBLISH steps, for a lag of up to STRIP_PUBLISH-1 more steps per strip
+#define STRIP_PUBLISH 4
 #define STRIP_TIMEOUT_TICKS 200000000ull
 
 __device__ __forceinline__ int load_agent(const gi32 *p) {
@@ -1775,13 +1798,26 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict_
     Meta mn;
     int in_ = 0, jn_ = 0;
     bool inr_n = false;
+    // the metadata of a column is requested one step before its data (the data
+    // loads need it for their addresses)
+    auto meta_at = [&](int s) {
+        int l = s - r, i, j;
+        if (row_ok && l >= 0 && oa + l <= ob) {
+            sweep_to_ij(d, 0, oa + l, b, IS, JS, i, j);
+            return ldg4i(d.meta + c2(d, i, j));
+        }
+        return make_int4(0, -1, 0, 0);
+    };
+    int4 mq = meta_at(0);
     auto locate = [&](int s) {
         int l = s - r;                                        // column a = oa + l
         inr_n = row_ok && l >= 0 && oa + l <= ob;
+        const int4 m4 = mq;
+        mq = meta_at(s + 1);
         mn.adv = false;
         if (inr_n) {
             sweep_to_ij(d, 0, oa + l, b, IS, JS, in_, jn_);
-            mn = unpack_meta(ldg4i(d.meta + c2(d, in_, jn_)));
+            mn = unpack_meta(m4);
             const size_t o = c3(d, in_, jn_);
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
@@ -1847,6 +1883,10 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict_
         if (r == 0) sw.wait(s + 1 + TILE);             // the previous strip's row for column oa+s+1
         locate(s + 1);
         lds_barrier();
+        // the last wave has nothing to do while wave 0 runs the chains: it
+        // publishes the steps before this one (their stores were issued in the
+        // previous step's write-back)
+        if (r == TILE - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);
         if (r == 0) {
             const bool on = lane < TILE && sinfo[(lane & (TILE - 1)) * 5];
             const int cl = lane & (TILE - 1);
@@ -1863,7 +1903,6 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict_
             }
         }
         lds_barrier();
-        if (r == TILE - 1 && s > 0) strip_publish(mine, s, lane);   // steps 0..s-1 written
         if (act) {
             const size_t o = c3(d, i, j);
             for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) X[o + k] = xcur[k];
@@ -1909,13 +1948,24 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__res
     float xfull[2], qfull[2];
     int in_ = 0, jn_ = 0, KKn = 0, VIn = 0;
     bool inr_n = false, actn = false;
+    auto meta_at = [&](int s) {
+        int l = s - r, i, j;
+        if (row_ok && l >= 0 && oa + l <= ob) {
+            sweep_to_ij(d, 1, oa + l, b, IS, JS, i, j);
+            return ldg4i(d.meta + c2(d, i, j));
+        }
+        return make_int4(0, -1, 0, 0);
+    };
+    int4 mq = meta_at(0);
     auto locate = [&](int s) {
         int l = s - r;                                        // sweep row a = oa + l
         inr_n = row_ok && l >= 0 && oa + l <= ob;
         actn = false;
+        const int4 m4 = mq;
+        mq = meta_at(s + 1);
         if (inr_n) {
             sweep_to_ij(d, 1, oa + l, b, IS, JS, in_, jn_);
-            Meta m = unpack_meta(ldg4i(d.meta + c2(d, in_, jn_)));
+            Meta m = unpack_meta(m4);
             actn = m.adv;
             KKn = m.KK; VIn = m.VI;
             const size_t o = c3(d, in_, jn_);
@@ -1992,6 +2042,7 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__res
         if (r == 0) sw.wait(s + 1 + TILE);
         locate(s + 1);
         lds_barrier();
+        if (r == TILE - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);   // while wave 0 runs the chains
         if (r == 0) {
             const int cl = lane & (TILE - 1);
             const bool on = lane < TILE && sinfo[cl * 4];
@@ -2007,7 +2058,6 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__res
             }
         }
         lds_barrier();
-        if (r == TILE - 1 && s > 0) strip_publish(mine, s, lane);
         if (act) {
             size_t o = c3(d, i, j);
             float *qring = ring + (size_t)rs * nkl;
@@ -2256,10 +2306,11 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         default: break;
     }
     int ncols = (nii - 2) * (njj - 2);
+    const int ncol8 = (ncols + NXCD - 1) / NXCD * NXCD;      // x extent of the XCD-ordered column kernels
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(kb, njj - 1, Z), dim3(64), 0, s, dd); });
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ns, dim3(kb, nii - 1, Z), dim3(64), 0, s, dd); });
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_top, dim3((ncols + 255) / 256, 1, Z), dim3(256), 0, s, dd); });
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncols, 1, Z), dim3(64), 0, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
     size_t nrow = (size_t)nj2 * nkp;
     for (int q = 0; q < nsit; ++q) nrow = max(nrow, (size_t)4 * hd[q].sync_stride);
     L.go(FAM_STENCIL, [&] {
@@ -2300,8 +2351,8 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
                 });
         }
     }
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_correction, dim3(ncols, 1, Z), dim3(64), 0, s, dd); });
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_averages, dim3(ncols, 1, Z), dim3(64), 0, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_correction, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_averages, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_avg_edges, dim3(njj - 2, 1, Z), dim3(64), 0, s, dd); });
     size_t shm = sizeof(float) * 5 * nkp;
     // per-diagonal sweeps: sweep ranges a in [oa, nii-1] (outer), b in [ia, njj-1]
@@ -2318,7 +2369,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         for (int q = 0; q < nsit; ++q) {
             int ob = hd[q].nii - 1, ib = hd[q].njj - 1;
             int na = ob - oa + 1, nb = ib - ia + 1;
-            nblk = max(nblk, (na * nb + COEF_COLS - 1) / COEF_COLS);
+            nblk = max(nblk, ((na * nb + COEF_COLS - 1) / COEF_COLS + NXCD - 1) / NXCD * NXCD);
             nt = max(nt, (ob - oa + TILE) / TILE + (ib - ia + TILE) / TILE - 1);
         }
         size_t shm_c = sizeof(float) * 4 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
@@ -2339,7 +2390,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         for (int q = 0; q < nsit; ++q) {
             int ob = hd[q].nii - 1, ib = hd[q].njj - 1;
             int na = ob - oa + 1, nb = ib - ia + 1;
-            nblk = max(nblk, (na * nb + COEF_COLS - 1) / COEF_COLS);
+            nblk = max(nblk, ((na * nb + COEF_COLS - 1) / COEF_COLS + NXCD - 1) / NXCD * NXCD);
             nst = max(nst, (ib - ia + TILE) / TILE);
         }
         size_t shm_c = sizeof(float) * 4 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
