@@ -421,13 +421,12 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
         abp[k] = (k > 1) ? area / (0.5f * (DZK[k - 1] + DZK[k])) : area / DZK[k];
         atp[k] = area / (0.5f * (DZK[k + 1] + DZK[k]));
     }
-    // sweep kernels: strip-pipelined (default), tiled wavefront, or one launch
-    // per anti-diagonal (GRAL_PF_SWEEP = strip | tile | diag)
+    // sweep kernels: strip-pipelined (default) or one launch per anti-diagonal
+    // (GRAL_PF_SWEEP = strip | diag)
     const char *sw = std::getenv("GRAL_PF_SWEEP");
     d.tiled = 2;
-    if (sw && std::strcmp(sw, "tile") == 0) d.tiled = 1;
     if (sw && std::strcmp(sw, "diag") == 0) d.tiled = 0;
-    if (h->kadvmax + 2 > 128) d.tiled = 0;   // the tiled and strip kernels hold at most 128 levels per column
+    if (h->kadvmax + 2 > 128) d.tiled = 0;   // the strip kernels hold at most 128 levels per column
     {
         // strip synchronisation: header + one block per sweep kind (pressure, U, V, W)
         int nstrip = (std::max(h->nii, h->njj) + 15) / 16 + 1;

@@ -42,7 +42,7 @@ struct DomT {
     P<unsigned> resid;               // DeltaUMax per iteration (float bits, values >= 0)
     int resid_cap;                   // entries in resid
     // sweep kernels
-    int tiled;                       // 2: strip-pipelined, 1: tiled wavefront, 0: one launch per anti-diagonal
+    int tiled;                       // 2: strip-pipelined sweeps, 0: one launch per anti-diagonal
     P<int> sync;                     // strip sweeps: [64] header (sticky error flag at 0), then 4 blocks
     int sync_stride;                 //   of sync_stride ints (ticket, unused, progress per strip)
     P<float> planes;                 // [cells][12] mixing-length coefficient records (neighbour weights, source pieces)

@@ -1056,24 +1056,20 @@ __global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t,
 }
 
 // ------------------------------------------------------------------------
-// Tiled wavefront path (mixing-length model and pressure)
+// Strip wavefront path (mixing-length model and pressure)
 // ------------------------------------------------------------------------
-// The sweep plane (a, b) is cut into TILE x TILE tiles.  Tiles on one tile
-// anti-diagonal are independent and run as the workgroups of one launch; the
-// tile anti-diagonals run in order, one launch each.  Inside a tile, wave r
-// owns row b = b0 + r and at local step s handles column a = a0 + s - r, so
-// every local step is one anti-diagonal of the tile.  Each step is three
-// phases separated by workgroup barriers: (1) all lanes of every active wave
-// assemble the column's source terms (lanes over k) into LDS, (2) lane 0 of
-// every active wave runs the serial Thomas recurrences, (3) all lanes write
-// the new column back.  Neighbour values written by other waves in earlier
-// steps are read from global memory after the barrier; values from other
-// tiles were written by earlier launches.
+// A workgroup of TILE waves owns TILE consecutive sweep rows (b); wave r owns
+// row b0 + r and at local step s handles column a = oa + s - r, so every local
+// step is one anti-diagonal of the strip.  Each step is three phases separated
+// by workgroup barriers: (1) every active wave assembles its column's source
+// terms (lanes over k) into LDS, (2) wave 0 runs the serial Thomas recurrences
+// of all the strip's columns (lane = row), (3) the waves write their columns
+// back.  Neighbours updated in the previous step are exchanged through LDS
+// column buffers.
 #define TILE 16
 
-// coefficient records written by k_ml_coef, read by the tile and strip sweeps
-// array of structures: 16 floats per cell, in the order of the four float4 loads
-// per cell, two records: rec12 = {AW, AS, AE, AN | f, g, w, c0 | X1, X2, X3, X4}
+// coefficient records written by k_ml_coef and read by the strip sweeps; per
+// cell, two records: rec12 = {AW, AS, AE, AN | f, g, w, c0 | X1, X2, X3, X4}
 // (written once by the first pass of k_ml_coef) and rec4 = {C, T, P, e} (written
 // once after the P/TERMP recurrence); each is written whole, in one pass
 #define REC12 12
@@ -1207,12 +1203,6 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
     }
 }
 
-__device__ __forceinline__ bool tile_coords(int tau, int nta, int ntb, int &A, int &B) {
-    int amin = max(0, tau - (ntb - 1));
-    A = amin + blockIdx.x;
-    B = tau - A;
-    return A < nta && B >= 0;
-}
 
 // Serial Thomas recurrences of one column (one lane), software-pipelined: the
 // operands of the next CH levels are read from LDS while the current CH levels
@@ -1223,20 +1213,23 @@ __device__ __forceinline__ bool tile_coords(int tau, int nta, int ntb, int &A, i
 #define SLOT_PAD (2 * CH)
 __device__ __forceinline__ int lds_slot(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
 
-// The chunk loops below are unrolled over the largest column the tiled and
-// strip kernels hold (MAXCH chunks of CH levels) and leave early at the
+// The chunk loops below are unrolled over the largest column the strip
+// kernels hold (MAXCH chunks of CH levels) and leave early at the
 // wave-wide longest chain (nmax): straight-line code lets the compiler wait
 // for exactly the LDS reads a chunk needs, while the reads of the next chunk
 // are in flight.  Reads stay within CH levels past a lane's own range (the
-// slot padding); a lane that is done re-reads its first chunk, and levels
-// outside a lane's range are stored to a per-lane dummy word, so the chunks
-// are branch-free.
+// slot padding); a lane that is done re-reads its first chunk.  A lane's
+// last chunk stores up to CH-1 levels past its range (into the slot padding or
+// levels nobody reads), and chunks after its end go to a per-lane sink of CH
+// words, so the chunks are branch-free.  The back-substitution writes its
+// result to a separate slot (the old column stays readable and untouched below
+// the range).
 #define MAXCH (128 / CH)
 
 // forward: Q[t] = (D[t] + C[t]*Q[t-1]) * T[t], t = 0..n-1, Q[-1] = q0
 __device__ __forceinline__ void chain_fwd(const float *__restrict__ D, const float *__restrict__ C,
                                           const float *__restrict__ T, float *__restrict__ Q, int n, int nmax, float q0,
-                                          float *dump) {
+                                          float *sink) {
     float Qp = q0;
     float cd[CH], cc[CH], ct[CH];
 #pragma unroll
@@ -1250,10 +1243,11 @@ __device__ __forceinline__ void chain_fwd(const float *__restrict__ D, const flo
 #pragma unroll
         for (int u = 0; u < CH; ++u) { nd[u] = D[o + u]; nc[u] = C[o + u]; nt[u] = T[o + u]; }
         __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
+        float *Qw = (t < n) ? Q : sink - t;              // a lane past its end stores to its sink
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             float q = (cd[u] + cc[u] * Qp) * ct[u];
-            *((t + u < n) ? &Q[t + u] : dump) = q;
+            Qw[t + u] = q;
             Qp = q;
         }
 #pragma unroll
@@ -1263,7 +1257,8 @@ __device__ __forceinline__ void chain_fwd(const float *__restrict__ D, const flo
 
 // relaxed back-substitution downwards: X[-t] += relax*(P[-t]*X[-t+1] + Q[-t] - X[-t]), t = 0..n-1
 __device__ __forceinline__ void chain_back(const float *__restrict__ P, const float *__restrict__ Q,
-                                           float *__restrict__ X, float relax, int n, int nmax, float *dump) {
+                                           const float *__restrict__ X, float *__restrict__ Xn, float relax, int n,
+                                           int nmax, float *sink) {
     float up = X[1];
     float cp[CH], cq[CH], cx[CH];
 #pragma unroll
@@ -1277,10 +1272,11 @@ __device__ __forceinline__ void chain_back(const float *__restrict__ P, const fl
 #pragma unroll
         for (int u = 0; u < CH; ++u) { np[u] = P[-(o + u)]; nq[u] = Q[-(o + u)]; nx[u] = X[-(o + u)]; }
         __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
+        float *Xw = (t < n) ? Xn : sink + (CH - 1) + t;
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             float x = cx[u] + relax * (cp[u] * up + cq[u] - cx[u]);
-            *((t + u < n) ? &X[-(t + u)] : dump) = x;
+            Xw[-(t + u)] = x;
             up = x;
         }
 #pragma unroll
@@ -1291,7 +1287,7 @@ __device__ __forceinline__ void chain_back(const float *__restrict__ P, const fl
 // pressure, downwards from the top: Q[-t] = (A[-t]*Q[-t+1] + D[-t]) * T[-t] with
 // A (= ATP) held in Q on entry and Q[1] = qtop, t = 0..n-1
 __device__ __forceinline__ void chain_pdown(const float *__restrict__ D, const float *__restrict__ T, float *__restrict__ Q,
-                                            float qtop, int n, int nmax, float *dump) {
+                                            float qtop, int n, int nmax, float *sink) {
     float Qn = qtop;
     float ca[CH], cd[CH], ct[CH];
 #pragma unroll
@@ -1305,10 +1301,11 @@ __device__ __forceinline__ void chain_pdown(const float *__restrict__ D, const f
 #pragma unroll
         for (int u = 0; u < CH; ++u) { na[u] = Q[-(o + u)]; nd[u] = D[-(o + u)]; nt[u] = T[-(o + u)]; }
         __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
+        float *Qw = (t < n) ? Q : sink + (CH - 1) + t;
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             float q = (ca[u] * Qn + cd[u]) * ct[u];
-            *((t + u < n) ? &Q[-(t + u)] : dump) = q;
+            Qw[-(t + u)] = q;
             Qn = q;
         }
 #pragma unroll
@@ -1318,7 +1315,7 @@ __device__ __forceinline__ void chain_pdown(const float *__restrict__ D, const f
 
 // pressure, upwards: N[t] = P[t]*N[t-1] + Q[t], t = 0..n-1, N[-1] = bottom
 __device__ __forceinline__ void chain_pup(const float *__restrict__ P, const float *__restrict__ Q, float *__restrict__ N,
-                                          float bottom, int n, int nmax, float *dump) {
+                                          float bottom, int n, int nmax, float *sink) {
     float dp = bottom;
     float cp[CH], cq[CH];
 #pragma unroll
@@ -1332,10 +1329,11 @@ __device__ __forceinline__ void chain_pup(const float *__restrict__ P, const flo
 #pragma unroll
         for (int u = 0; u < CH; ++u) { np[u] = P[o + u]; nq[u] = Q[o + u]; }
         __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
+        float *Nw = (t < n) ? N : sink - t;
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             float x = cp[u] * dp + cq[u];
-            *((t + u < n) ? &N[t + u] : dump) = x;
+            Nw[t + u] = x;
             dp = x;
         }
 #pragma unroll
@@ -1355,7 +1353,7 @@ __device__ __forceinline__ int wave_max(int v, bool on) {
 struct MlPre {
     float aw, as, ae, an, e, f, g, w, c0, x1, x2, x3, x4, c, t, p;
     float x, xa, xb;      // own old value; old neighbours in the +IS (xa) and +JS (xb) directions
-    float ni, nj;         // -IS / -JS neighbours, prefetched only when they lie in an earlier tile
+    float ni, nj;         // -IS / -JS neighbours, prefetched only when another strip or the boundary holds them
 };
 
 template <int COMP>
@@ -1381,330 +1379,18 @@ __device__ __forceinline__ ColInfo meta_col(const Meta &m, int i, int j, int W) 
     return q;
 }
 
-// Mixing-length sweep on tiles.  Per local step:
-//  (1) wave r assembles the source terms D of its column (lanes over k) from
-//      values prefetched one step earlier and from the two neighbours updated
-//      earlier in this sweep, which inside the tile come from the previous
-//      step's column buffers in LDS; it stores D, C, T, P, the reset rows of Q
-//      and the whole old column in LDS, then prefetches its next column;
-//  (2) wave 0 runs the Q recurrence and the relaxed back-substitution of all
-//      the tile's active columns together, lane r = column of wave r;
-//  (3) wave r writes its column back to global memory (fire and forget: no
-//      later read in this launch depends on those stores).
-// Column buffers are double buffered by step parity; all barriers order LDS
-// only, and no global load inside the step loop waits on another one.
-template <int COMP>
-__global__ __launch_bounds__(TILE * WAVE) void k_ml_tile(const Dom *__restrict__ ds, int tau, int IS, int JS) {
-    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    const gf32 *planes = d.planes;
-
-    extern __shared__ float sm[];
-    const int nkp = d.nkp, nkl = lds_slot(nkp);
-    const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2, ob = d.nii - 1, ib = d.njj - 1;
-    const int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
-    int A, B;
-    if (!tile_coords(tau, nta, ntb, A, B)) return;     // uniform over the workgroup
-    const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int a0 = oa + A * TILE, b = ia + B * TILE + r;
-    // every per-row array is a padded slot: element k at slot + SLOT_PAD + k
-    float *sD = sm + (size_t)r * 5 * nkl + SLOT_PAD, *sC = sD + nkl, *sT = sC + nkl, *sP = sT + nkl, *sQ = sP + nkl;
-    float *xbuf = sm + (size_t)TILE * 5 * nkl + SLOT_PAD;      // [2][TILE] slots of whole columns
-    float *sdzk = sm + (size_t)TILE * 7 * nkl;                // [nkp]
-    int4 *smeta = (int4 *)(sm + (((size_t)TILE * 7 * nkl + nkp + 3) & ~(size_t)3));   // [TILE rows][TILE columns], 16-byte aligned
-    int *sinfo = (int *)(smeta + TILE * TILE);                // per row: active, KSTART, KK, VI, KKm
-    float *sdump = (float *)(sinfo + 5 * TILE);               // [WAVE] stores of the chains outside a column
-    const size_t n3 = (size_t)(d.nii + 2) * d.nj2 * nkp;
-    gf32 *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk);
-    // per-launch staging: the metadata of this wave's row and the layer thicknesses
-    if (lane < TILE) {
-        int a = a0 + lane;
-        int4 m = make_int4(0, -1, 0, 0);
-        if (a <= ob && b <= ib) {
-            int i, j;
-            sweep_to_ij(d, 0, a, b, IS, JS, i, j);
-            m = ldg4i(d.meta + c2(d, i, j));
-            m.z |= 1 << 20;                                    // in range
-        }
-        smeta[r * TILE + lane] = m;
-    }
-    for (int k = threadIdx.x; k < nkp; k += TILE * WAVE) sdzk[k] = d.dzk[k];
-    __syncthreads();
-    MlPre pre[2];
-    float xfull[2];
-    Meta mn;
-    int in_ = 0, jn_ = 0;
-    bool inr_n = false;
-    auto locate = [&](int s) {
-        int l = s - r;
-        inr_n = false;
-        mn.adv = false;
-        if (l >= 0 && l < TILE) {
-            int4 m4 = smeta[r * TILE + l];
-            inr_n = (m4.z >> 20) & 1;
-            mn = unpack_meta(m4);
-            mn.adv = mn.adv && inr_n;
-        }
-        if (inr_n) {
-            sweep_to_ij(d, 0, a0 + l, b, IS, JS, in_, jn_);
-            const size_t o = c3(d, in_, jn_);
-#pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                int k = lane + WAVE * m;
-                if (k < nkp) xfull[m] = X[o + k];
-            }
-        }
-        if (mn.adv) {
-#pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                int k = mn.KSTART + lane + WAVE * m;
-                if (k <= mn.VI) {
-                    ml_prefetch<COMP>(d, planes, n3, in_, jn_, k, IS, JS, pre[m]);
-                    if (l == 0) pre[m].ni = X[c3(d, in_ - IS, jn_) + k];
-                    if (r == 0) pre[m].nj = X[c3(d, in_, jn_ - JS) + k];
-                }
-            }
-        }
-    };
-    locate(0);
-    for (int s = 0; s < 2 * TILE - 1; ++s) {
-        const bool act = mn.adv, inr = inr_n;
-        const Meta mt = mn;
-        const int i = in_, j = jn_;
-        const ColInfo q = meta_col(mt, i, j, d.simd);
-        float *xcur = xbuf + (size_t)((s & 1) * TILE + r) * nkl;
-        const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r) * nkl;
-        const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r - 1) * nkl;
-        const bool w_in = (s - r >= 1), s_in = (r >= 1);
-        if (inr) {
-#pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                int k = lane + WAVE * m;
-                if (k < nkp) xcur[k] = xfull[m];
-            }
-        }
-        if (act) {
-            const int KKP1 = q.KK + 1;
-#pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                int k = q.KSTART + lane + WAVE * m;
-                if (k <= q.VI) {
-                    const MlPre &r_ = pre[m];
-                    // neighbours updated earlier in this sweep: -IS and -JS directions
-                    float xnew_i = w_in ? xw_lds[k] : r_.ni;
-                    float xnew_j = s_in ? xs_lds[k] : r_.nj;
-                    float xw = (IS == 1) ? xnew_i : r_.xa, xe = (IS == 1) ? r_.xa : xnew_i;
-                    float xs = (JS == 1) ? xnew_j : r_.xb, xn = (JS == 1) ? r_.xb : xnew_j;
-                    MlPieces pc;
-                    pc.AW1 = r_.aw; pc.AS1 = r_.as; pc.AE1 = r_.ae; pc.AN1 = r_.an; pc.e = r_.e; pc.f = r_.f;
-                    pc.g = r_.g; pc.w = r_.w; pc.c0 = r_.c0; pc.X1 = r_.x1; pc.X2 = r_.x2; pc.X3 = r_.x3; pc.X4 = r_.x4;
-                    float D = ml_source<COMP>(d, q, k, sdzk[k], pc, xw, xs, xe, xn, r_.x);
-                    sD[k] = D; sC[k] = r_.c; sT[k] = r_.t; sP[k] = r_.p;
-                    if (k <= KKP1) sQ[k] = D * r_.t;
-                }
-            }
-            if (lane == 0) {
-                int KKm = (COMP == 0) ? mt.KKmi : (COMP == 1) ? mt.KKmj : q.KK;
-                sinfo[r * 5 + 1] = q.KSTART; sinfo[r * 5 + 2] = q.KK; sinfo[r * 5 + 3] = q.VI; sinfo[r * 5 + 4] = KKm;
-            }
-        }
-        if (lane == 0) sinfo[r * 5] = act;
-        locate(s + 1);                                   // prefetch of the next column, in flight across phases 2-3
-        lds_barrier();
-        if (r == 0) {
-            const bool on = lane < TILE && sinfo[(lane & (TILE - 1)) * 5];
-            const int cl = lane & (TILE - 1);
-            const int KSTART = sinfo[cl * 5 + 1], KK = sinfo[cl * 5 + 2], VI = sinfo[cl * 5 + 3], KKm = sinfo[cl * 5 + 4];
-            const int kend = (COMP == 2) ? KSTART : max(KSTART, KKm + 1);
-            const int nf = VI - KK - 1, nb = VI - kend + 1;
-            const int nfmax = wave_max(nf, on), nbmax = wave_max(nb, on);
-            if (on) {
-                float *D = sm + (size_t)cl * 5 * nkl + SLOT_PAD;
-                float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl;
-                float *Xs = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
-                chain_fwd(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane);
-                chain_back(P + VI, Q + VI, Xs + VI, d.relax, nb, nbmax, sdump + lane);
-            }
-        }
-        lds_barrier();
-        if (act) {
-            const size_t o = c3(d, i, j);
-            for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) X[o + k] = xcur[k];
-        }
-        lds_barrier();
-    }
-}
-
-// Pressure sweep on tiles.  PIMP/TERMP of every column depend only on the
-// geometry (including the stale top value of the row), so they are tabulated
-// once per solve for both i directions (ptab/ttab); only the QIMP recurrence,
-// with its own stale top value from the row scratch, depends on the flow.
-// Same step structure and LDS column buffers (of DPM) as k_ml_tile.  The row
-// scratch of QIMP is global: it is read in phase 2 of the step after the one
-// that wrote it, so phase 3 ends with a full barrier.
+// prefetched per-level values of one column of the pressure sweep
 struct PrPre {
     float div, t, p, pa, pb;       // DIV, TERMP, PIMP, old DPM in the +IS (i) and +JS (j) directions
-    float ni, nj;                  // -IS / -JS neighbours when they lie in an earlier tile
+    float ni, nj;                  // -IS / -JS neighbours when another strip or the boundary holds them
 };
 
-__global__ __launch_bounds__(TILE * WAVE) void k_pressure_tile(const Dom *__restrict__ ds, int tau, int IS, int JS, int slot) {
-    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    const gf32 *ptab = d.ptab[IS == 1 ? 0 : 1], *ttab = d.ttab[IS == 1 ? 0 : 1];
-
-    extern __shared__ float sm[];
-    const int nkp = d.nkp, nkl = lds_slot(nkp);
-    const int oa = 2, ia = 2, ob = d.njj - 1, ib = d.nii - 1;
-    const int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
-    int A, B;
-    if (!tile_coords(tau, nta, ntb, A, B)) return;
-    const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int a0 = oa + A * TILE, b = ia + B * TILE + r;
-    float *sD = sm + (size_t)r * 4 * nkl + SLOT_PAD, *sQ = sD + nkl, *sT = sQ + nkl, *sP = sT + nkl;
-    float *xbuf = sm + (size_t)TILE * 4 * nkl + SLOT_PAD;      // [2][TILE] slots of whole DPM columns
-    float *sdzk = sm + (size_t)TILE * 6 * nkl, *satp = sdzk + nkp;
-    int4 *smeta = (int4 *)(sm + (((size_t)TILE * 6 * nkl + 2 * nkp + 3) & ~(size_t)3));
-    int *sinfo = (int *)(smeta + TILE * TILE);                // per row: active, KK, VI, j
-    float *sdump = (float *)(sinfo + 4 * TILE);               // [WAVE] stores of the chains outside a column
-    if (lane < TILE) {
-        int a = a0 + lane;
-        int4 m = make_int4(0, -1, 0, 0);
-        if (a <= ob && b <= ib) {
-            int i, j;
-            sweep_to_ij(d, 1, a, b, IS, JS, i, j);
-            m = ldg4i(d.meta + c2(d, i, j));
-            m.z |= 1 << 20;
-        }
-        smeta[r * TILE + lane] = m;
-    }
-    for (int k = threadIdx.x; k < nkp; k += TILE * WAVE) { sdzk[k] = d.dzk[k]; satp[k] = d.atp[k]; }
-    __syncthreads();
-    const float DXK = d.dxk, DYK = d.dyk;
-    float rmax = 0.0f;
-    PrPre pre[2];
-    float xfull[2];
-    int in_ = 0, jn_ = 0, KKn = 0, VIn = 0;
-    bool inr_n = false, actn = false;
-    auto locate = [&](int s) {
-        int l = s - r;
-        inr_n = false;
-        actn = false;
-        if (l >= 0 && l < TILE) {
-            Meta m = unpack_meta(smeta[r * TILE + l]);
-            inr_n = (smeta[r * TILE + l].z >> 20) & 1;
-            actn = m.adv && inr_n;
-            KKn = m.KK; VIn = m.VI;
-        }
-        if (inr_n) {
-            sweep_to_ij(d, 1, a0 + l, b, IS, JS, in_, jn_);
-            const size_t o = c3(d, in_, jn_);
-#pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                int k = lane + WAVE * m;
-                if (k < nkp) xfull[m] = d.dpm[o + k];
-            }
-        }
-        if (actn) {
-            size_t o = c3(d, in_, jn_);
-#pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                int k = KKn + 1 + lane + WAVE * m;
-                if (k <= VIn) {
-                    pre[m].div = d.div[o + k];
-                    pre[m].t = ttab[o + k];
-                    pre[m].p = ptab[o + k];
-                    pre[m].pa = d.dpm[c3(d, in_ + IS, jn_) + k];
-                    pre[m].pb = d.dpm[c3(d, in_, jn_ + JS) + k];
-                    if (l == 0) pre[m].nj = d.dpm[c3(d, in_, jn_ - JS) + k];
-                    if (r == 0) pre[m].ni = d.dpm[c3(d, in_ - IS, jn_) + k];
-                }
-            }
-        }
-    };
-    locate(0);
-    for (int s = 0; s < 2 * TILE - 1; ++s) {
-        const bool act = actn, inr = inr_n;
-        const int i = in_, j = jn_, KK = KKn, VI = VIn;
-        float *xcur = xbuf + (size_t)((s & 1) * TILE + r) * nkl;
-        const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r) * nkl;
-        const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r - 1) * nkl;
-        const bool w_in = (s - r >= 1), s_in = (r >= 1);
-        if (inr) {
-#pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                int k = lane + WAVE * m;
-                if (k < nkp) xcur[k] = xfull[m];
-            }
-        }
-        if (act) {
-#pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                int k = KK + 1 + lane + WAVE * m;
-                if (k <= VI) {
-                    const PrPre &r_ = pre[m];
-                    // DPM of the neighbours updated earlier in this sweep: (a-1,b) is the
-                    // -JS row neighbour, (a,b-1) the -IS column neighbour
-                    float newj = w_in ? xw_lds[k] : r_.nj;
-                    float newi = s_in ? xs_lds[k] : r_.ni;
-                    float dim_ = (IS == 1) ? newi : r_.pa, dip = (IS == 1) ? r_.pa : newi;
-                    float djm = (JS == 1) ? newj : r_.pb, djp = (JS == 1) ? r_.pb : newj;
-                    float dz = sdzk[k];
-                    sD[k] = r_.div / d.dtime + (dim_ + dip) / DXK * DYK * dz + (djm - djp) / DYK * DXK * dz;
-                    sT[k] = r_.t;
-                    sP[k] = r_.p;
-                    sQ[k] = satp[k];                  // consumed and replaced by QIMP in phase 2
-                }
-            }
-            if (lane == 0) { sinfo[r * 4 + 1] = KK; sinfo[r * 4 + 2] = VI; sinfo[r * 4 + 3] = j; }
-        }
-        if (lane == 0) sinfo[r * 4] = act;
-        locate(s + 1);
-        lds_barrier();
-        if (r == 0) {
-            const int cl = lane & (TILE - 1);
-            const bool on = lane < TILE && sinfo[cl * 4];
-            const int KK2 = sinfo[cl * 4 + 1], VI2 = sinfo[cl * 4 + 2], jr = sinfo[cl * 4 + 3];
-            const int nmax = wave_max(VI2 - KK2, on);
-            if (on) {
-                float *Dd = sm + (size_t)cl * 4 * nkl + SLOT_PAD;
-                float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl;
-                const float *O = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
-                // Q[k] holds ATP[k] on entry; the new DPM replaces D
-                chain_pdown(Dd + VI2, T + VI2, Q + VI2, d.prow_q[(size_t)jr * nkp + VI2 + 1], VI2 - KK2, nmax, sdump + lane);
-                chain_pup(P + KK2 + 1, Q + KK2 + 1, Dd + KK2 + 1, O[KK2], VI2 - KK2, nmax, sdump + lane);
-            }
-        }
-        lds_barrier();
-        if (act) {
-            size_t o = c3(d, i, j);
-            gf32 *qrow = d.prow_q + (size_t)j * nkp;
-            for (int k = KK + 1 + lane; k <= VI; k += WAVE) {
-                float temp = sD[k];
-                double qv = (double)conv_to_int((double)temp * d.frund) / d.frund;
-                float rr = (float)fabs((double)xcur[k] - qv);
-                rmax = rr > rmax ? rr : rmax;
-                xcur[k] = temp;                             // column buffer now holds the new DPM
-                d.dpm[o + k] = temp;
-                d.dpmnew[o + k] = d.dpmnew[o + k] + d.relaxp * temp;
-                qrow[k] = sQ[k];
-            }
-        }
-        __syncthreads();
-    }
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-        float other = __shfl_xor(rmax, off);
-        rmax = other > rmax ? other : rmax;
-    }
-    if (lane == 0 && rmax > 0.0f && slot >= 0 && slot < d.resid_cap) atomicMax((unsigned *)&d.resid[slot], __float_as_uint(rmax));
-}
-
 // ------------------------------------------------------------------------
-// Strip-pipelined sweeps.  One workgroup owns a strip of TILE sweep rows
-// (b) and walks the whole a range with the same per-step structure as the
-// tiled kernels (wave r handles row b0+r, column a = oa+s-r at local step
-// s).  Strip B reads the last row of strip B-1 (its new values, and for the
-// pressure sweep the QIMP row scratch) and nothing else from other strips,
-// so it only has to stay TILE steps behind strip B-1: the serial path of a
-// sweep is about (a range) + TILE * (number of strips) steps instead of the
-// (2*TILE-1) * (number of tile diagonals) of the tiled launches.
+// Strip-pipelined sweeps.  Strip B reads the last row of strip B-1 (its new
+// values, and for the pressure sweep the QIMP row scratch) and nothing else
+// from other strips, so it only has to stay TILE steps behind strip B-1: one
+// launch holds the whole sweep, whose serial path is about (a range) +
+// TILE * (number of strips) local steps.
 //
 // Synchronisation, per situation and sweep kind, in int sync[...]:
 //   [0] ticket counter: strips are numbered in the order their workgroups
@@ -1775,12 +1461,14 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict_
     const int nstrip = (ib - ia + TILE) / TILE, nsteps = (ob - oa + 1) + TILE - 1;
     const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     gi32 *sync = d.sync + SYNC_BLOCK0 + (size_t)(1 + COMP) * d.sync_stride;
-    float *sD = sm + (size_t)r * 5 * nkl + SLOT_PAD, *sC = sD + nkl, *sT = sC + nkl, *sP = sT + nkl, *sQ = sP + nkl;
-    float *xbuf = sm + (size_t)TILE * 5 * nkl + SLOT_PAD;      // [2][TILE] slots of whole columns
-    float *sdzk = sm + (size_t)TILE * 7 * nkl;                // [nkp]
+    // per row: D, C, T, P, Q and the back-substitution result Xn
+    float *sD = sm + (size_t)r * 6 * nkl + SLOT_PAD, *sC = sD + nkl, *sT = sC + nkl, *sP = sT + nkl, *sQ = sP + nkl;
+    float *sXn = sQ + nkl;
+    float *xbuf = sm + (size_t)TILE * 6 * nkl + SLOT_PAD;      // [2][TILE] slots of whole columns
+    float *sdzk = sm + (size_t)TILE * 8 * nkl;                // [nkp]
     int *sinfo = (int *)(sdzk + nkp);                         // per row: active, KSTART, KK, VI, KKm
     int *sctl = sinfo + 5 * TILE;                             // strip index of this workgroup
-    float *sdump = (float *)(sctl + 4);                       // [WAVE] stores of the chains outside a column
+    float *sdump = (float *)(sctl + 4);                       // [WAVE][CH] sinks of the chains
     if (threadIdx.x == 0) sctl[0] = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int k = threadIdx.x; k < nkp; k += TILE * WAVE) sdzk[k] = d.dzk[k];
     __syncthreads();
@@ -1879,6 +1567,9 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict_
                 sinfo[r * 5 + 1] = q.KSTART; sinfo[r * 5 + 2] = q.KK; sinfo[r * 5 + 3] = q.VI; sinfo[r * 5 + 4] = KKm;
             }
         }
+        // first level the back-substitution updates (U/V skip levels inside the
+        // building or its upstream neighbour; W updates every level)
+        const int kend = (COMP == 2) ? q.KSTART : max(q.KSTART, ((COMP == 0) ? mt.KKmi : mt.KKmj) + 1);
         if (lane == 0) sinfo[r * 5] = act;
         if (r == 0) sw.wait(s + 1 + TILE);             // the previous strip's row for column oa+s+1
         locate(s + 1);
@@ -1895,17 +1586,21 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict_
             const int nf = VI - KK - 1, nb = VI - kend + 1;
             const int nfmax = wave_max(nf, on), nbmax = wave_max(nb, on);
             if (on) {
-                float *D = sm + (size_t)cl * 5 * nkl + SLOT_PAD;
-                float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl;
-                float *Xs = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
-                chain_fwd(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane);
-                chain_back(P + VI, Q + VI, Xs + VI, d.relax, nb, nbmax, sdump + lane);
+                float *D = sm + (size_t)cl * 6 * nkl + SLOT_PAD;
+                float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl, *Xn = Q + nkl;
+                const float *Xs = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
+                chain_fwd(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane * CH);
+                chain_back(P + VI, Q + VI, Xs + VI, Xn + VI, d.relax, nb, nbmax, sdump + lane * CH);
             }
         }
         lds_barrier();
         if (act) {
             const size_t o = c3(d, i, j);
-            for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) X[o + k] = xcur[k];
+            for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) {
+                float v = (k >= kend) ? sXn[k] : xcur[k];
+                xcur[k] = v;
+                X[o + k] = v;
+            }
         }
         lds_barrier();
     }
@@ -1931,7 +1626,7 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__res
     float *sdzk = sm + (size_t)TILE * 7 * nkl, *satp = sdzk + nkp;
     int *sinfo = (int *)(satp + nkp);                         // per row: active, KK, VI, ring slot
     int *sctl = sinfo + 4 * TILE;
-    float *sdump = (float *)(sctl + 4);                       // [WAVE] stores of the chains outside a column
+    float *sdump = (float *)(sctl + 4);                       // [WAVE][CH] sinks of the chains
     if (threadIdx.x == 0) sctl[0] = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int k = threadIdx.x; k < nkp; k += TILE * WAVE) { sdzk[k] = d.dzk[k]; satp[k] = d.atp[k]; }
     __syncthreads();
@@ -2053,8 +1748,8 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__res
                 float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl;
                 const float *O = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
                 // Q[k] holds ATP[k] on entry; the new DPM replaces D
-                chain_pdown(Dd + VI2, T + VI2, Q + VI2, ring[(size_t)rs2 * nkl + VI2 + 1], VI2 - KK2, nmax, sdump + lane);
-                chain_pup(P + KK2 + 1, Q + KK2 + 1, Dd + KK2 + 1, O[KK2], VI2 - KK2, nmax, sdump + lane);
+                chain_pdown(Dd + VI2, T + VI2, Q + VI2, ring[(size_t)rs2 * nkl + VI2 + 1], VI2 - KK2, nmax, sdump + lane * CH);
+                chain_pup(P + KK2 + 1, Q + KK2 + 1, Dd + KK2 + 1, O[KK2], VI2 - KK2, nmax, sdump + lane * CH);
             }
         }
         lds_barrier();
@@ -2319,27 +2014,10 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
     if (d0.tiled == 2) {
         int nst = 0;
         for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].nii - 1 - 2 + TILE) / TILE);
-        size_t shm = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + 2 * nkp + 8 + WAVE) + sizeof(int) * (5 * TILE + 4);
+        size_t shm = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + 2 * nkp + 8 + WAVE * CH) + sizeof(int) * (5 * TILE + 4);
         L.go(FAM_PRESSURE, [&] {
             hipLaunchKernelGGL(k_pressure_strip, dim3(nst, 1, Z), dim3(TILE * WAVE), shm, s, dd, IS, JS, slot);
         });
-    } else if (d0.tiled) {
-        int nt = 0;
-        for (int q = 0; q < nsit; ++q) {
-            int nta = (hd[q].njj - 1 - 2 + TILE) / TILE, ntb = (hd[q].nii - 1 - 2 + TILE) / TILE;
-            nt = max(nt, nta + ntb - 1);
-        }
-        size_t shm = sizeof(float) * ((size_t)6 * lds_slot_h(nkp) * TILE + 2 * nkp + 8 + WAVE) + sizeof(int4) * TILE * TILE + sizeof(int) * 4 * TILE;
-        for (int tau = 0; tau < nt; ++tau) {
-            int cnt = 0;
-            for (int q = 0; q < nsit; ++q) {
-                int nta = (hd[q].njj - 1 - 2 + TILE) / TILE, ntb = (hd[q].nii - 1 - 2 + TILE) / TILE;
-                if (tau < nta + ntb - 1) cnt = max(cnt, min(nta - 1, tau) - max(0, tau - (ntb - 1)) + 1);
-            }
-            L.go(FAM_PRESSURE, [&] {
-                hipLaunchKernelGGL(k_pressure_tile, dim3(cnt, 1, Z), dim3(TILE * WAVE), shm, s, dd, tau, IS, JS, slot);
-            });
-        }
     } else {
         int oa = 2, ob = njj - 1, ia = 2, ib = nii - 1;
         size_t shm = sizeof(float) * 5 * nkp;
@@ -2364,27 +2042,6 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
                 L.go(fam, [&] { hipLaunchKernelGGL(kern, dim3(len, 1, Z), dim3(WAVE), shm, s, dd, t, is, js); });
         }
     };
-    auto tiled_sweep = [&](int fam, auto coef, auto tile, int oa, int ia, int is, int js) {
-        int nblk = 0, nt = 0;
-        for (int q = 0; q < nsit; ++q) {
-            int ob = hd[q].nii - 1, ib = hd[q].njj - 1;
-            int na = ob - oa + 1, nb = ib - ia + 1;
-            nblk = max(nblk, ((na * nb + COEF_COLS - 1) / COEF_COLS + NXCD - 1) / NXCD * NXCD);
-            nt = max(nt, (ob - oa + TILE) / TILE + (ib - ia + TILE) / TILE - 1);
-        }
-        size_t shm_c = sizeof(float) * 4 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
-        L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
-        size_t shm_t = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + nkp + 8 + WAVE) + sizeof(int4) * TILE * TILE + sizeof(int) * 5 * TILE;
-        for (int tau = 0; tau < nt; ++tau) {
-            int cnt = 0;
-            for (int q = 0; q < nsit; ++q) {
-                int ob = hd[q].nii - 1, ib = hd[q].njj - 1;
-                int nta = (ob - oa + TILE) / TILE, ntb = (ib - ia + TILE) / TILE;
-                if (tau < nta + ntb - 1) cnt = max(cnt, min(nta - 1, tau) - max(0, tau - (ntb - 1)) + 1);
-            }
-            L.go(fam, [&] { hipLaunchKernelGGL(tile, dim3(cnt, 1, Z), dim3(TILE * WAVE), shm_t, s, dd, tau, is, js); });
-        }
-    };
     auto strip_sweep = [&](int fam, auto coef, auto strip, int oa, int ia, int is, int js) {
         int nblk = 0, nst = 0;
         for (int q = 0; q < nsit; ++q) {
@@ -2395,17 +2052,13 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         }
         size_t shm_c = sizeof(float) * 4 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
         L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
-        size_t shm_t = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + nkp + 8 + WAVE) + sizeof(int) * (6 * TILE + 4);
+        size_t shm_t = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * TILE + nkp + 8 + WAVE * CH) + sizeof(int) * (6 * TILE + 4);
         L.go(fam, [&] { hipLaunchKernelGGL(strip, dim3(nst, 1, Z), dim3(TILE * WAVE), shm_t, s, dd, is, js); });
     };
     if (d0.model == 1 && d0.tiled == 2) {
         strip_sweep(FAM_U, k_ml_coef<0>, k_ml_strip<0>, 3, 2, IS, JS);
         strip_sweep(FAM_V, k_ml_coef<1>, k_ml_strip<1>, 2, 3, -IS, -JS);
         strip_sweep(FAM_W, k_ml_coef<2>, k_ml_strip<2>, 2, 2, IS, JS);
-    } else if (d0.model == 1 && d0.tiled) {
-        tiled_sweep(FAM_U, k_ml_coef<0>, k_ml_tile<0>, 3, 2, IS, JS);
-        tiled_sweep(FAM_V, k_ml_coef<1>, k_ml_tile<1>, 2, 3, -IS, -JS);
-        tiled_sweep(FAM_W, k_ml_coef<2>, k_ml_tile<2>, 2, 2, IS, JS);
     } else if (d0.model == 1) {
         sweep(FAM_U, k_momentum<0, 0>, 3, 2, IS, JS);
         sweep(FAM_V, k_momentum<1, 0>, 2, 3, -IS, -JS);

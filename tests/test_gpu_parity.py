@@ -18,7 +18,7 @@ REL_L2_TOL = 1e-4
 MAX_ABS_TOL = 1e-3
 
 
-def _compare(case, iterations, sweeps=("strip", "tile", "diag")):
+def _compare(case, iterations, sweeps=("strip", "diag")):
     from gral_amd import pf
     from oracle import pyoracle
     ref = pyoracle.run(case, iterations=iterations)
@@ -97,5 +97,5 @@ def test_config1_200_iterations():
     # the full fixed-iteration parity run of config 1 (200 SIMPLE iterations,
     # convergence bookkeeping at 100 and 200)
     c = domain.config1(turbulence_model=1, iterations=200)
-    rep = _compare(c, 200, sweeps=("strip", "tile"))
+    rep = _compare(c, 200, sweeps=("strip",))
     print(rep)
