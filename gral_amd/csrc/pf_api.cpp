@@ -359,7 +359,6 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     if (!h->masks || !h->rough || !h->fields) return fail(h, GRAL_PF_ESTATE, "prepare: masks, roughness and fields must be set first");
     if (p->turbulence_model < 0 || p->turbulence_model > 2) return fail(h, GRAL_PF_EINVAL, "prepare: turbulence_model must be 0, 1 or 2");
     if (p->simd_width != 8 && p->simd_width != 16) return fail(h, GRAL_PF_EINVAL, "prepare: simd_width must be 8 or 16");
-    if (p->simd_width == 16) return fail(h, GRAL_PF_EINVAL, "prepare: Vector512 semantics (simd_width 16) are not implemented");
     if (h->vcf < p->simd_width) return fail(h, GRAL_PF_EINVAL, "prepare: vertcellsff must be >= simd_width");
     if (p->iter_max < 1 || p->iter_min < 0 || p->iter_max > 10000000) return fail(h, GRAL_PF_EINVAL, "prepare: bad iteration limits");
     if (p->refp_i < 1 || p->refp_i > h->nii || p->refp_j < 1 || p->refp_j > h->njj) return fail(h, GRAL_PF_EINVAL, "prepare: reference point outside the grid");
@@ -429,7 +428,7 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     if (h->kadvmax + 2 > 128) d.tiled = 0;   // the strip kernels hold at most 128 levels per column
     {
         // strip synchronisation: header + one block per sweep kind (pressure, U, V, W)
-        int nstrip = (std::max(h->nii, h->njj) + 15) / 16 + 1;
+        int nstrip = (std::max(h->nii, h->njj) + STRIP_ROWS - 1) / STRIP_ROWS + 1;
         d.sync_stride = ((2 + nstrip) + 63) & ~63;
         int *fs;
         if ((rc = dalloc(h, &fs, (size_t)SYNC_BLOCK0 + 4 * (size_t)d.sync_stride))) return rc;

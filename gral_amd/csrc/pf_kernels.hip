@@ -313,18 +313,17 @@ __global__ void k_avg_edges(const Dom *__restrict__ ds) {
 // ------------------------------------------------------------------------
 // Mixing-length momentum (turbulence model 1): U/V/W-prognostic-microscale_1.cs
 // ------------------------------------------------------------------------
-// The reference evaluates a column in Vector<float> blocks of W lanes starting
-// at KSTART, KSTART+W, ..., the last block moved down to KEND-W.  Every lane
+// The reference evaluates a column in vector blocks of W lanes starting at
+// KSTART, KSTART+W, ..., the last block moved down to KEND-W.  Every lane
 // quantity is lane-local except the gate of the eddy-viscosity cross terms,
 // which tests the block start (k_block > KKART+1); block_base() returns the
 // start of the block whose lane k enters the recurrence.
-// Vector<float>.Count of the reference's default (AVX2) build; the only width
-// the host accepts
-#define SIMDW 8
+// W = 8: Vector<float> solvers (U/V/W-prognostic-microscale_1.cs, AVX2 width);
+// W = 16: the 512-bit solvers (U/V/W-prognostic-microscale_1_Vec512.cs), which
+// also fuse some multiply-adds (see v512 below).  The host accepts only 8 or 16.
 __device__ __forceinline__ int block_base(int k, int KSTART, int KEND, int W) {
-    (void)W;
-    W = SIMDW;
-    int base = KSTART + ((k - KSTART) / W) * W;
+    const int sh = (W == 16) ? 4 : 3;
+    int base = KSTART + (((k - KSTART) >> sh) << sh);
     if (base > KEND - W) base = KEND - W;
     return base;
 }
@@ -333,6 +332,7 @@ struct ColInfo {
     int i, j, KK, KSTART, VI, W;
     bool as, an, aw, ae, vegp;
     float cutk, cov, ht, ho, z0s;
+    __device__ __forceinline__ bool v512() const { return W == 16; }
 };
 
 __device__ __forceinline__ ColInfo col_info(const DomG &d, int i, int j) {
@@ -350,11 +350,13 @@ __device__ __forceinline__ ColInfo col_info(const DomG &d, int i, int j) {
     return q;
 }
 
-// mixing-length eddy viscosity (U-prognostic-microscale_1.cs:216-268), lane-local
+// mixing-length eddy viscosity (U-prognostic-microscale_1.cs:216-268), lane-local;
+// the 512-bit solvers fuse the half-cell offset into the ground height
+// (U-prognostic-microscale_1_Vec512.cs:218)
 __device__ __forceinline__ float ml_visc(const DomG &d, const ColInfo &q, int k, const gf32 *UK, const gf32 *VK) {
     if (!(k > q.KK + 1)) return 0.0f;
     float dz = d.dzk[k];
-    float zs = d.hok[k] - d.hok[q.KK] - dz * 0.5f;
+    float zs = q.v512() ? d.hok[k] - fmaf(dz, 0.5f, d.hok[q.KK]) : d.hok[k] - d.hok[q.KK] - dz * 0.5f;
     float mixL = vminf(0.071f * zs, 100.0f);
     if (q.cov > 0) mixL *= (1.0f - 0.99f * q.cov);
     float in = (UK[k + 1] - UK[k - 1]) / (2.0f * dz);
@@ -368,6 +370,15 @@ __device__ __forceinline__ float ml_visc(const DomG &d, const ColInfo &q, int k,
 __device__ __forceinline__ float plaw_f(float D, float P) {
     float in = 1.0f - 0.1f * P;
     return D * vmaxf(0.0f, in * in * in * in * in);
+}
+
+// power-law coefficient plus its upwind flux Fp = max(+-F, 0):
+// D*max(0, x^5) + Fp (U-prognostic-microscale_1.cs:345-367), or, in the 512-bit
+// solvers, FMA(D, max(0, x*((x*x)*(x*x))), Fp) (U-prognostic-microscale_1_Vec512.cs:355-377)
+__device__ __forceinline__ float plaw_coef(bool v512, float D, float P, float Fp) {
+    if (!v512) return plaw_f(D, P) + Fp;
+    float in = 1.0f - 0.1f * P;
+    return fmaf(D, vmaxf(0.0f, in * ((in * in) * (in * in))), Fp);
 }
 
 struct Coef { float A, B, C, D; };
@@ -388,12 +399,13 @@ __device__ __forceinline__ void ml_coeffs(const DomG &d, const ColInfo &q, int k
     DS = vmaxf(DS, 0.0001f); DN = vmaxf(DN, 0.0001f); DT = vmaxf(DT, 0.0001f);
     float PE = fabsf(FE / DE), PB = fabsf(FB / DB), PW = fabsf(FW / DW);
     float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
-    BIM = plaw_f(DT, PT) + vmaxf(-FT, 0.0f);
-    CIM = plaw_f(DB, PB) + vmaxf(FB, 0.0f);
-    AE1 = plaw_f(DE, PE) + vmaxf(-FE, 0.0f);
-    AW1 = plaw_f(DW, PW) + vmaxf(FW, 0.0f);
-    AS1 = plaw_f(DS, PS) + vmaxf(FS, 0.0f);
-    AN1 = plaw_f(DN, PN) + vmaxf(-FN, 0.0f);
+    const bool v = q.v512();
+    BIM = plaw_coef(v, DT, PT, vmaxf(-FT, 0.0f));
+    CIM = plaw_coef(v, DB, PB, vmaxf(FB, 0.0f));
+    AE1 = plaw_coef(v, DE, PE, vmaxf(-FE, 0.0f));
+    AW1 = plaw_coef(v, DW, PW, vmaxf(FW, 0.0f));
+    AS1 = plaw_coef(v, DS, PS, vmaxf(FS, 0.0f));
+    AN1 = plaw_coef(v, DN, PN, vmaxf(-FN, 0.0f));
     AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + d.ap0[k];
 }
 
@@ -423,7 +435,10 @@ __device__ __forceinline__ float ml_wall_uv(const DomG &d, const ColInfo &q, int
 // cell's own column, which is updated only when its own turn comes), except
 // the four neighbour values of the swept component, which enter as
 //   D = AW1*X_W + AS1*X_S + AE1*X_E + AN1*X_N + e + f + g  [- w]  [+ cross]
-// summed left to right exactly as the reference does.
+// summed left to right exactly as the reference does.  The 512-bit solvers
+// form the neighbour part as FMA(AW1,X_W,AS1*X_S) + FMA(AE1,X_E,AN1*X_N)
+// (not V without vegetation), and U without vegetation folds g into
+// f = FMA(DDPX, DYKDZK, g) (U-prognostic-microscale_1_Vec512.cs:404-421).
 struct MlPieces {
     float AW1, AS1, AE1, AN1, A, B, C;
     float e, f, g, w;          // g: Coriolis(+vegetation) term of U/V; vegetation drag (subtracted) of W
@@ -467,6 +482,7 @@ __device__ MlPieces u1_pieces(const DomG &d, const ColInfo &q, int k) {
         o.g = (d.cor * (d.ug - u) - veg * u * wh) * d.area * dz;
     } else {
         o.g = d.cor * (d.ug - u) * d.area * dz;
+        if (q.v512()) o.f = fmaf(DPMNEW_W[k] - DPMNEW[k], dykdzk, o.g);
     }
     o.w = (k == q.KK + 1) ? ml_wall_uv(d, q, k, u, UKS, UKS_W, VKS, VKS_W) : 0.0f;
     o.c0 = VIS / d.dxk;
@@ -569,9 +585,14 @@ __device__ MlPieces w1_pieces(const DomG &d, const ColInfo &q, int k) {
 template <int COMP>
 __device__ __forceinline__ float ml_source(const DomG &d, const ColInfo &q, int k, float dz, const MlPieces &o, float xw,
                                            float xs, float xe, float xn, float x) {
-    float D = o.AW1 * xw + o.AS1 * xs + o.AE1 * xe + o.AN1 * xn + o.e + o.f;
+    float D;
+    if (q.v512() && (COMP != 1 || q.vegp))
+        D = fmaf(o.AW1, xw, o.AS1 * xs) + fmaf(o.AE1, xe, o.AN1 * xn);
+    else
+        D = o.AW1 * xw + o.AS1 * xs + o.AE1 * xe + o.AN1 * xn;
+    D = D + o.e + o.f;
     if (COMP != 2) {
-        D = D + o.g;
+        if (COMP != 0 || !q.v512() || q.vegp) D = D + o.g;
         if (k == q.KK + 1) D -= o.w;
         if (ml_cross_on(q, k)) {
             if (COMP == 0) {
@@ -1058,15 +1079,17 @@ __global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t,
 // ------------------------------------------------------------------------
 // Strip wavefront path (mixing-length model and pressure)
 // ------------------------------------------------------------------------
-// A workgroup of TILE waves owns TILE consecutive sweep rows (b); wave r owns
-// row b0 + r and at local step s handles column a = oa + s - r, so every local
-// step is one anti-diagonal of the strip.  Each step is three phases separated
-// by workgroup barriers: (1) every active wave assembles its column's source
-// terms (lanes over k) into LDS, (2) wave 0 runs the serial Thomas recurrences
-// of all the strip's columns (lane = row), (3) the waves write their columns
-// back.  Neighbours updated in the previous step are exchanged through LDS
-// column buffers.
-#define TILE 16
+// A workgroup owns ROWS consecutive sweep rows (b) with one wave per row plus
+// a chain wave.  Row wave r owns row b0 + r and at local step s handles column
+// a = oa + s - r, so every local step is one anti-diagonal of the strip.  Each
+// step is three phases separated by workgroup barriers: (1) every active row
+// wave assembles its column's source terms (lanes over k) into LDS, (2) the
+// chain wave runs the serial Thomas recurrences of all the strip's columns
+// (lane = row) while the row waves prefetch their next columns, (3) the row
+// waves write their columns back.  Neighbours updated in the previous step are
+// exchanged through LDS column buffers.
+#define ROWS STRIP_ROWS
+#define STRIP_WAVES (STRIP_ROWS + 1)
 
 // coefficient records written by k_ml_coef and read by the strip sweeps; per
 // cell, two records: rec12 = {AW, AS, AE, AN | f, g, w, c0 | X1, X2, X3, X4}
@@ -1115,7 +1138,10 @@ __device__ __forceinline__ Meta unpack_meta(int4 m) {
 // compute the coefficients.  One wave then runs the P/TERMP recurrence of all
 // COEF_COLS columns at once (lane = column), and the lanes write the last part
 // of each cell's record.  Each 64-byte record is written once, as 4 float4.
-template <int COMP>
+// WFIX: the solver width when every situation of the launch has it (the
+// width-dependent arithmetic then folds at compile time), 0 to read it per
+// situation.
+template <int COMP, int WFIX>
 __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds, int IS, int JS) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     gf32 *planes = d.planes;
@@ -1168,7 +1194,8 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
     for (int t = threadIdx.x; t < total; t += 4 * WAVE) {
         const int c = column_of(t);
         const int k = scol[c * 8 + 0] + (t - scol[c * 8 + 5]);
-        const ColInfo q = col_info(d, scol[c * 8 + 3], scol[c * 8 + 4]);
+        ColInfo q = col_info(d, scol[c * 8 + 3], scol[c * 8 + 4]);
+        if (WFIX) q.W = WFIX;
         MlPieces p = ml_pieces<COMP>(d, q, k);
         float *sA = sm + (size_t)c * 4 * nkl;
         sA[k] = p.A; sA[nkl + k] = p.B; sA[2 * nkl + k] = p.C; sA[3 * nkl + k] = p.e;
@@ -1388,9 +1415,9 @@ struct PrPre {
 // ------------------------------------------------------------------------
 // Strip-pipelined sweeps.  Strip B reads the last row of strip B-1 (its new
 // values, and for the pressure sweep the QIMP row scratch) and nothing else
-// from other strips, so it only has to stay TILE steps behind strip B-1: one
+// from other strips, so it only has to stay ROWS steps behind strip B-1: one
 // launch holds the whole sweep, whose serial path is about (a range) +
-// TILE * (number of strips) local steps.
+// ROWS * (number of strips) local steps.
 //
 // Synchronisation, per situation and sweep kind, in int sync[...]:
 //   [0] ticket counter: strips are numbered in the order their workgroups
@@ -1399,8 +1426,8 @@ struct PrPre {
 //   [2 + B] steps of strip B whose stores are complete and released;
 // the blocks are cleared at the start of every iteration.  d.sync[0] is the
 // situation's sticky error flag: set when a wait exceeded the time limit.
-// Wave TILE-1 publishes its strip's progress with an agent-scope release;
-// wave 0 of the next strip polls it and acquires before reading.  Every
+// The last row wave publishes its strip's progress with an agent-scope
+// release; row 0 of the next strip polls it and acquires before reading.  Every
 // wait is bounded (STRIP_TIMEOUT_TICKS of the constant 100 MHz clock): on
 // timeout the strip sets the error flag and goes on without waiting, so the
 // kernel always ends; the host reports the error.
@@ -1444,6 +1471,13 @@ struct StripWait {
     }
 };
 
+// The sweeps' own stores are agent-scope (sc1) stores, which the L2 writes
+// through, so the L2 of the XCD holds no dirty lines from them when a strip
+// publishes (an agent-scope release writes back every dirty line of the L2).
+__device__ __forceinline__ void st_through(gf32 *p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // called by every lane of the strip's last wave: progress v (steps whose
 // stores this wave issued) becomes visible with the stores before it
 __device__ __forceinline__ void strip_publish(gi32 *mine, int v, int lane) {
@@ -1452,29 +1486,57 @@ __device__ __forceinline__ void strip_publish(gi32 *mine, int v, int lane) {
 }
 
 template <int COMP>
-__global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict__ ds, int IS, int JS) {
+__global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__restrict__ ds, int IS, int JS) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     const gf32 *planes = d.planes;
     extern __shared__ float sm[];
     const int nkp = d.nkp, nkl = lds_slot(nkp);
     const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2, ob = d.nii - 1, ib = d.njj - 1;
-    const int nstrip = (ib - ia + TILE) / TILE, nsteps = (ob - oa + 1) + TILE - 1;
-    const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int nstrip = (ib - ia + ROWS) / ROWS, nsteps = (ob - oa + 1) + ROWS - 1;
+    const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
+    const bool chainw = r == ROWS;                            // the chain wave owns no row
     gi32 *sync = d.sync + SYNC_BLOCK0 + (size_t)(1 + COMP) * d.sync_stride;
     // per row: D, C, T, P, Q and the back-substitution result Xn
     float *sD = sm + (size_t)r * 6 * nkl + SLOT_PAD, *sC = sD + nkl, *sT = sC + nkl, *sP = sT + nkl, *sQ = sP + nkl;
     float *sXn = sQ + nkl;
-    float *xbuf = sm + (size_t)TILE * 6 * nkl + SLOT_PAD;      // [2][TILE] slots of whole columns
-    float *sdzk = sm + (size_t)TILE * 8 * nkl;                // [nkp]
+    float *xbuf = sm + (size_t)ROWS * 6 * nkl + SLOT_PAD;      // [2][ROWS] slots of whole columns
+    float *sdzk = sm + (size_t)ROWS * 8 * nkl;                // [nkp]
     int *sinfo = (int *)(sdzk + nkp);                         // per row: active, KSTART, KK, VI, KKm
-    int *sctl = sinfo + 5 * TILE;                             // strip index of this workgroup
+    int *sctl = sinfo + 5 * ROWS;                             // strip index of this workgroup
     float *sdump = (float *)(sctl + 4);                       // [WAVE][CH] sinks of the chains
     if (threadIdx.x == 0) sctl[0] = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int k = threadIdx.x; k < nkp; k += TILE * WAVE) sdzk[k] = d.dzk[k];
+    for (int k = threadIdx.x; k < nkp; k += STRIP_WAVES * WAVE) sdzk[k] = d.dzk[k];
     __syncthreads();
     const int B = sctl[0];
     if (B >= nstrip) return;                                  // uniform over the workgroup
-    const int b = ia + B * TILE + r;
+    if (chainw) {
+        // The chain wave runs its own loop with the same three barriers per
+        // step as the row waves (s_barrier counts waves, wherever they are in
+        // the code), so none of the row waves' loop-carried prefetch registers
+        // is live in the recurrences.
+        for (int s = 0; s < nsteps; ++s) {
+            lds_barrier();                                    // the rows' sources are assembled
+            const int cl = lane < ROWS ? lane : 0;
+            const bool on = lane < ROWS && sinfo[cl * 5];
+            const int KSTART = sinfo[cl * 5 + 1], KK = sinfo[cl * 5 + 2], VI = sinfo[cl * 5 + 3], KKm = sinfo[cl * 5 + 4];
+            // first level the back-substitution updates (U/V skip levels inside
+            // the building or its upstream neighbour; W updates every level)
+            const int kend = (COMP == 2) ? KSTART : max(KSTART, KKm + 1);
+            const int nf = VI - KK - 1, nb = VI - kend + 1;
+            const int nfmax = wave_max(nf, on), nbmax = wave_max(nb, on);
+            if (on) {
+                float *D = sm + (size_t)cl * 6 * nkl + SLOT_PAD;
+                float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl, *Xn = Q + nkl;
+                const float *Xs = xbuf + (size_t)((s & 1) * ROWS + cl) * nkl;
+                chain_fwd(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane * CH);
+                chain_back(P + VI, Q + VI, Xs + VI, Xn + VI, d.relax, nb, nbmax, sdump + lane * CH);
+            }
+            lds_barrier();                                    // the recurrences are done
+            lds_barrier();                                    // the rows have written back
+        }
+        return;
+    }
+    const int b = ia + B * ROWS + r;
     const bool row_ok = b <= ib;
     StripWait sw{B > 0 ? sync + SYNC_HDR + B - 1 : nullptr, d.sync, B > 0 ? 0 : 0x7fffffff, nsteps,
                  wall_clock64() + STRIP_TIMEOUT_TICKS};
@@ -1520,21 +1582,34 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict_
                 if (k <= mn.VI) {
                     ml_prefetch<COMP>(d, planes, n3, in_, jn_, k, IS, JS, pre[m]);
                     if (l == 0) pre[m].ni = X[c3(d, in_ - IS, jn_) + k];
-                    if (r == 0) pre[m].nj = X[c3(d, in_, jn_ - JS) + k];
                 }
             }
         }
     };
-    if (r == 0) sw.wait(TILE);
+    // row 0's -JS neighbours lie in the previous strip's last row: row 0 issues
+    // its other look-ahead loads first, then waits for that strip, then reads them
+    auto locate_prev = [&]() {
+        if (mn.adv) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = mn.KSTART + lane + WAVE * m;
+                if (k <= mn.VI) pre[m].nj = X[c3(d, in_, jn_ - JS) + k];
+            }
+        }
+    };
     locate(0);
+    if (r == 0) {
+        sw.wait(ROWS);
+        locate_prev();
+    }
     for (int s = 0; s < nsteps; ++s) {
         const bool act = mn.adv, inr = inr_n;
         const Meta mt = mn;
         const int i = in_, j = jn_;
-        const ColInfo q = meta_col(mt, i, j, SIMDW);
-        float *xcur = xbuf + (size_t)((s & 1) * TILE + r) * nkl;
-        const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r) * nkl;
-        const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r - 1) * nkl;
+        const ColInfo q = meta_col(mt, i, j, d.simd);
+        float *xcur = xbuf + (size_t)((s & 1) * ROWS + r) * nkl;
+        const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r) * nkl;
+        const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r - 1) * nkl;
         const bool w_in = (s - r >= 1), s_in = (r >= 1);
         if (inr) {
 #pragma unroll
@@ -1567,31 +1642,21 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict_
                 sinfo[r * 5 + 1] = q.KSTART; sinfo[r * 5 + 2] = q.KK; sinfo[r * 5 + 3] = q.VI; sinfo[r * 5 + 4] = KKm;
             }
         }
-        // first level the back-substitution updates (U/V skip levels inside the
-        // building or its upstream neighbour; W updates every level)
         const int kend = (COMP == 2) ? q.KSTART : max(q.KSTART, ((COMP == 0) ? mt.KKmi : mt.KKmj) + 1);
         if (lane == 0) sinfo[r * 5] = act;
-        if (r == 0) sw.wait(s + 1 + TILE);             // the previous strip's row for column oa+s+1
-        locate(s + 1);
         lds_barrier();
-        // the last wave has nothing to do while wave 0 runs the chains: it
-        // publishes the steps before this one (their stores were issued in the
-        // previous step's write-back)
-        if (r == TILE - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);
+        // while the chain wave runs the recurrences: the last row publishes the
+        // steps before this one (their stores were issued in the previous
+        // step's write-back), and every row prefetches its next column.  The
+        // look-ahead of row r reads columns that rows r and r+1 write two steps
+        // later at the earliest (for the last row, the next strip, which waits
+        // for this one); row 0 also reads the previous strip's last row, after
+        // its wait.
+        if (r == ROWS - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);
+        locate(s + 1);
         if (r == 0) {
-            const bool on = lane < TILE && sinfo[(lane & (TILE - 1)) * 5];
-            const int cl = lane & (TILE - 1);
-            const int KSTART = sinfo[cl * 5 + 1], KK = sinfo[cl * 5 + 2], VI = sinfo[cl * 5 + 3], KKm = sinfo[cl * 5 + 4];
-            const int kend = (COMP == 2) ? KSTART : max(KSTART, KKm + 1);
-            const int nf = VI - KK - 1, nb = VI - kend + 1;
-            const int nfmax = wave_max(nf, on), nbmax = wave_max(nb, on);
-            if (on) {
-                float *D = sm + (size_t)cl * 6 * nkl + SLOT_PAD;
-                float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl, *Xn = Q + nkl;
-                const float *Xs = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
-                chain_fwd(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane * CH);
-                chain_back(P + VI, Q + VI, Xs + VI, Xn + VI, d.relax, nb, nbmax, sdump + lane * CH);
-            }
+            sw.wait(s + 1 + ROWS);
+            locate_prev();
         }
         lds_barrier();
         if (act) {
@@ -1599,40 +1664,63 @@ __global__ __launch_bounds__(TILE * WAVE) void k_ml_strip(const Dom *__restrict_
             for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) {
                 float v = (k >= kend) ? sXn[k] : xcur[k];
                 xcur[k] = v;
-                X[o + k] = v;
+                st_through(X + o + k, v);
             }
         }
         lds_barrier();
     }
-    if (r == TILE - 1) strip_publish(mine, nsteps, lane);
+    if (r == ROWS - 1) strip_publish(mine, nsteps, lane);
 }
 
 // Pressure sweep on strips.  The QIMP row scratch of sweep row a travels
-// with the wavefront through LDS (ring slot (a - oa) mod TILE): wave 0 loads
-// it from global memory (as strip B-1 left it), each wave updates it with its
-// column, and the last wave stores it back for strip B+1.
-__global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__restrict__ ds, int IS, int JS, int slot) {
+// with the wavefront through LDS (ring slot (a - oa) mod RING; the ROWS rows of
+// a step hold ROWS consecutive sweep rows): row 0 loads it from global memory
+// (as strip B-1 left it), each row updates it with its column, and the last
+// row stores it back for strip B+1.
+#define RING 16
+__global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom *__restrict__ ds, int IS, int JS, int slot) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];
     const gf32 *ptab = d.ptab[IS == 1 ? 0 : 1], *ttab = d.ttab[IS == 1 ? 0 : 1];
     extern __shared__ float sm[];
     const int nkp = d.nkp, nkl = lds_slot(nkp);
     const int oa = 2, ia = 2, ob = d.njj - 1, ib = d.nii - 1;
-    const int nstrip = (ib - ia + TILE) / TILE, nsteps = (ob - oa + 1) + TILE - 1;
-    const int r = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int nstrip = (ib - ia + ROWS) / ROWS, nsteps = (ob - oa + 1) + ROWS - 1;
+    const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
+    const bool chainw = r == ROWS;                            // the chain wave owns no row
     gi32 *sync = d.sync + SYNC_BLOCK0;
     float *sD = sm + (size_t)r * 4 * nkl + SLOT_PAD, *sQ = sD + nkl, *sT = sQ + nkl, *sP = sT + nkl;
-    float *xbuf = sm + (size_t)TILE * 4 * nkl + SLOT_PAD;      // [2][TILE] slots of whole DPM columns
-    float *ring = sm + (size_t)TILE * 6 * nkl + SLOT_PAD;      // [TILE] QIMP row scratch slots
-    float *sdzk = sm + (size_t)TILE * 7 * nkl, *satp = sdzk + nkp;
+    float *xbuf = sm + (size_t)ROWS * 4 * nkl + SLOT_PAD;      // [2][ROWS] slots of whole DPM columns
+    float *ring = sm + (size_t)ROWS * 6 * nkl + SLOT_PAD;      // [RING] QIMP row scratch slots
+    float *sdzk = sm + (size_t)(ROWS * 6 + RING) * nkl, *satp = sdzk + nkp;
     int *sinfo = (int *)(satp + nkp);                         // per row: active, KK, VI, ring slot
-    int *sctl = sinfo + 4 * TILE;
+    int *sctl = sinfo + 4 * ROWS;
     float *sdump = (float *)(sctl + 4);                       // [WAVE][CH] sinks of the chains
     if (threadIdx.x == 0) sctl[0] = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int k = threadIdx.x; k < nkp; k += TILE * WAVE) { sdzk[k] = d.dzk[k]; satp[k] = d.atp[k]; }
+    for (int k = threadIdx.x; k < nkp; k += STRIP_WAVES * WAVE) { sdzk[k] = d.dzk[k]; satp[k] = d.atp[k]; }
     __syncthreads();
     const int B = sctl[0];
     if (B >= nstrip) return;
-    const int b = ia + B * TILE + r;
+    if (chainw) {                                             // own loop, as in k_ml_strip
+        for (int s = 0; s < nsteps; ++s) {
+            lds_barrier();                                    // the rows' sources are assembled
+            const int cl = lane < ROWS ? lane : 0;
+            const bool on = lane < ROWS && sinfo[cl * 4];
+            const int KK2 = sinfo[cl * 4 + 1], VI2 = sinfo[cl * 4 + 2], rs2 = sinfo[cl * 4 + 3];
+            const int nmax = wave_max(VI2 - KK2, on);
+            if (on) {
+                float *Dd = sm + (size_t)cl * 4 * nkl + SLOT_PAD;
+                float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl;
+                const float *O = xbuf + (size_t)((s & 1) * ROWS + cl) * nkl;
+                // Q[k] holds ATP[k] on entry; the new DPM replaces D
+                chain_pdown(Dd + VI2, T + VI2, Q + VI2, ring[(size_t)rs2 * nkl + VI2 + 1], VI2 - KK2, nmax, sdump + lane * CH);
+                chain_pup(P + KK2 + 1, Q + KK2 + 1, Dd + KK2 + 1, O[KK2], VI2 - KK2, nmax, sdump + lane * CH);
+            }
+            lds_barrier();                                    // the recurrences are done
+            lds_barrier();                                    // the rows have written back
+        }
+        return;
+    }
+    const int b = ia + B * ROWS + r;
     const bool row_ok = b <= ib;
     StripWait sw{B > 0 ? sync + SYNC_HDR + B - 1 : nullptr, d.sync, B > 0 ? 0 : 0x7fffffff, nsteps,
                  wall_clock64() + STRIP_TIMEOUT_TICKS};
@@ -1669,13 +1757,6 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__res
                 int k = lane + WAVE * m2;
                 if (k < nkp) xfull[m2] = d.dpm[o + k];
             }
-            if (r == 0) {                                     // the row scratch as strip B-1 left it
-#pragma unroll
-                for (int m2 = 0; m2 < 2; ++m2) {
-                    int k = lane + WAVE * m2;
-                    if (k < nkp) qfull[m2] = d.prow_q[(size_t)jn_ * nkp + k];
-                }
-            }
         }
         if (actn) {
             size_t o = c3(d, in_, jn_);
@@ -1689,20 +1770,41 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__res
                     pre[m2].pa = d.dpm[c3(d, in_ + IS, jn_) + k];
                     pre[m2].pb = d.dpm[c3(d, in_, jn_ + JS) + k];
                     if (l == 0) pre[m2].nj = d.dpm[c3(d, in_, jn_ - JS) + k];
-                    if (r == 0) pre[m2].ni = d.dpm[c3(d, in_ - IS, jn_) + k];
                 }
             }
         }
     };
-    if (r == 0) sw.wait(TILE);
+    // row 0 reads the previous strip's last row (-IS neighbours) and the row
+    // scratch as that strip left it: after its other look-ahead loads and its wait
+    auto locate_prev = [&]() {
+        if (inr_n) {
+#pragma unroll
+            for (int m2 = 0; m2 < 2; ++m2) {
+                int k = lane + WAVE * m2;
+                if (k < nkp) qfull[m2] = d.prow_q[(size_t)jn_ * nkp + k];
+            }
+        }
+        if (actn) {
+            size_t o = c3(d, in_ - IS, jn_);
+#pragma unroll
+            for (int m2 = 0; m2 < 2; ++m2) {
+                int k = KKn + 1 + lane + WAVE * m2;
+                if (k <= VIn) pre[m2].ni = d.dpm[o + k];
+            }
+        }
+    };
     locate(0);
+    if (r == 0) {
+        sw.wait(ROWS);
+        locate_prev();
+    }
     for (int s = 0; s < nsteps; ++s) {
         const bool act = actn, inr = inr_n;
         const int i = in_, j = jn_, KK = KKn, VI = VIn;
-        const int rs = (s - r) & (TILE - 1);                  // ring slot of this wave's sweep row
-        float *xcur = xbuf + (size_t)((s & 1) * TILE + r) * nkl;
-        const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r) * nkl;
-        const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * TILE + r - 1) * nkl;
+        const int rs = (s - r) & (RING - 1);                  // ring slot of this wave's sweep row
+        float *xcur = xbuf + (size_t)((s & 1) * ROWS + r) * nkl;
+        const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r) * nkl;
+        const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r - 1) * nkl;
         const bool w_in = (s - r >= 1), s_in = (r >= 1);
         if (inr) {
 #pragma unroll
@@ -1734,23 +1836,15 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__res
             if (lane == 0) { sinfo[r * 4 + 1] = KK; sinfo[r * 4 + 2] = VI; sinfo[r * 4 + 3] = rs; }
         }
         if (lane == 0) sinfo[r * 4] = act;
-        if (r == 0) sw.wait(s + 1 + TILE);
-        locate(s + 1);
         lds_barrier();
-        if (r == TILE - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);   // while wave 0 runs the chains
+        // while the chain wave runs the recurrences: publish and prefetch as in
+        // k_ml_strip; row 0 reads the previous strip's row scratch and last row
+        // after its wait
+        if (r == ROWS - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);
+        locate(s + 1);
         if (r == 0) {
-            const int cl = lane & (TILE - 1);
-            const bool on = lane < TILE && sinfo[cl * 4];
-            const int KK2 = sinfo[cl * 4 + 1], VI2 = sinfo[cl * 4 + 2], rs2 = sinfo[cl * 4 + 3];
-            const int nmax = wave_max(VI2 - KK2, on);
-            if (on) {
-                float *Dd = sm + (size_t)cl * 4 * nkl + SLOT_PAD;
-                float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl;
-                const float *O = xbuf + (size_t)((s & 1) * TILE + cl) * nkl;
-                // Q[k] holds ATP[k] on entry; the new DPM replaces D
-                chain_pdown(Dd + VI2, T + VI2, Q + VI2, ring[(size_t)rs2 * nkl + VI2 + 1], VI2 - KK2, nmax, sdump + lane * CH);
-                chain_pup(P + KK2 + 1, Q + KK2 + 1, Dd + KK2 + 1, O[KK2], VI2 - KK2, nmax, sdump + lane * CH);
-            }
+            sw.wait(s + 1 + ROWS);
+            locate_prev();
         }
         lds_barrier();
         if (act) {
@@ -1762,19 +1856,19 @@ __global__ __launch_bounds__(TILE * WAVE) void k_pressure_strip(const Dom *__res
                 float rr = (float)fabs((double)xcur[k] - qv);
                 rmax = rr > rmax ? rr : rmax;
                 xcur[k] = temp;                             // column buffer now holds the new DPM
-                d.dpm[o + k] = temp;
-                d.dpmnew[o + k] = d.dpmnew[o + k] + d.relaxp * temp;
+                st_through(d.dpm + o + k, temp);
+                st_through(d.dpmnew + o + k, d.dpmnew[o + k] + d.relaxp * temp);
                 qring[k] = sQ[k];
             }
         }
-        if (r == TILE - 1 && inr) {                           // hand the row scratch to strip B+1
+        if (r == ROWS - 1 && inr) {                           // hand the row scratch to strip B+1
             const float *qring = ring + (size_t)rs * nkl;
             gf32 *qrow = d.prow_q + (size_t)j * nkp;
-            for (int k = lane; k < nkp; k += WAVE) qrow[k] = qring[k];
+            for (int k = lane; k < nkp; k += WAVE) st_through(qrow + k, qring[k]);
         }
         lds_barrier();
     }
-    if (r == TILE - 1) strip_publish(mine, nsteps, lane);
+    if (r == ROWS - 1) strip_publish(mine, nsteps, lane);
     for (int off = WAVE / 2; off > 0; off >>= 1) {
         float other = __shfl_xor(rmax, off);
         rmax = other > rmax ? other : rmax;
@@ -2013,10 +2107,11 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
     });
     if (d0.tiled == 2) {
         int nst = 0;
-        for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].nii - 1 - 2 + TILE) / TILE);
-        size_t shm = sizeof(float) * ((size_t)7 * lds_slot_h(nkp) * TILE + 2 * nkp + 8 + WAVE * CH) + sizeof(int) * (5 * TILE + 4);
+        for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].nii - 1 - 2 + ROWS) / ROWS);
+        size_t shm = sizeof(float) * ((size_t)(6 * ROWS + RING) * lds_slot_h(nkp) + 2 * nkp + 8 + WAVE * CH) +
+                     sizeof(int) * (5 * ROWS + 4);
         L.go(FAM_PRESSURE, [&] {
-            hipLaunchKernelGGL(k_pressure_strip, dim3(nst, 1, Z), dim3(TILE * WAVE), shm, s, dd, IS, JS, slot);
+            hipLaunchKernelGGL(k_pressure_strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm, s, dd, IS, JS, slot);
         });
     } else {
         int oa = 2, ob = njj - 1, ia = 2, ib = nii - 1;
@@ -2048,17 +2143,23 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
             int ob = hd[q].nii - 1, ib = hd[q].njj - 1;
             int na = ob - oa + 1, nb = ib - ia + 1;
             nblk = max(nblk, ((na * nb + COEF_COLS - 1) / COEF_COLS + NXCD - 1) / NXCD * NXCD);
-            nst = max(nst, (ib - ia + TILE) / TILE);
+            nst = max(nst, (ib - ia + ROWS) / ROWS);
         }
         size_t shm_c = sizeof(float) * 4 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
         L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
-        size_t shm_t = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * TILE + nkp + 8 + WAVE * CH) + sizeof(int) * (6 * TILE + 4);
-        L.go(fam, [&] { hipLaunchKernelGGL(strip, dim3(nst, 1, Z), dim3(TILE * WAVE), shm_t, s, dd, is, js); });
+        size_t shm_t = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) + sizeof(int) * (6 * ROWS + 4);
+        L.go(fam, [&] { hipLaunchKernelGGL(strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm_t, s, dd, is, js); });
     };
-    if (d0.model == 1 && d0.tiled == 2) {
-        strip_sweep(FAM_U, k_ml_coef<0>, k_ml_strip<0>, 3, 2, IS, JS);
-        strip_sweep(FAM_V, k_ml_coef<1>, k_ml_strip<1>, 2, 3, -IS, -JS);
-        strip_sweep(FAM_W, k_ml_coef<2>, k_ml_strip<2>, 2, 2, IS, JS);
+    bool all_w8 = true;
+    for (int q = 0; q < nsit; ++q) all_w8 = all_w8 && hd[q].simd == 8;
+    if (d0.model == 1 && d0.tiled == 2 && all_w8) {
+        strip_sweep(FAM_U, k_ml_coef<0, 8>, k_ml_strip<0>, 3, 2, IS, JS);
+        strip_sweep(FAM_V, k_ml_coef<1, 8>, k_ml_strip<1>, 2, 3, -IS, -JS);
+        strip_sweep(FAM_W, k_ml_coef<2, 8>, k_ml_strip<2>, 2, 2, IS, JS);
+    } else if (d0.model == 1 && d0.tiled == 2) {
+        strip_sweep(FAM_U, k_ml_coef<0, 0>, k_ml_strip<0>, 3, 2, IS, JS);
+        strip_sweep(FAM_V, k_ml_coef<1, 0>, k_ml_strip<1>, 2, 3, -IS, -JS);
+        strip_sweep(FAM_W, k_ml_coef<2, 0>, k_ml_strip<2>, 2, 2, IS, JS);
     } else if (d0.model == 1) {
         sweep(FAM_U, k_momentum<0, 0>, 3, 2, IS, JS);
         sweep(FAM_V, k_momentum<1, 0>, 2, 3, -IS, -JS);

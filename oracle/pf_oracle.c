@@ -8,6 +8,8 @@
  *     in deterministic mode (UseFixedRndSeedVal: MaxDegreeOfParallelism = 1
  *     around the pressure sweep and the momentum solvers, :806-811, :1058-1136)
  *   - U/V/W-prognostic-microscale_1.cs (Vector<float>, width 8, no FMA)
+ *   - U/V/W-prognostic-microscale_1_Vec512.cs (Vector512, width 16, the FMA
+ *     placements of those files; selected by simd_width 16)
  *   - U/V/W-prognostic-microscale_2.cs, TKE-prognostic-microscale.cs (k-eps)
  *   - U/V/W-prognostic-microscale_0.cs (no diffusion)
  *   - ProgramFunctions.cs:768-849 (FloatMax, ConvToInt, Pow2/3/5)
@@ -338,7 +340,9 @@ static void ml_column(St *s, int i, int j, lane_fn fn, void *ctx) {
 static float ml_vis(St *s, int i, int j, int k, int kk) {
     int W = s->W, KK = KK(s, i, j), KKP1 = KK + 1;
     float DZ = s->DZK[kk];
-    float zs = s->HOKART[kk] - s->HOKART[KK] - DZ * 0.5f;
+    /* the 512-bit solvers fuse the half-cell offset into the ground height
+     * (U-prognostic-microscale_1_Vec512.cs:218) */
+    float zs = (W == 16) ? s->HOKART[kk] - fmaf(DZ, 0.5f, s->HOKART[KK]) : s->HOKART[kk] - s->HOKART[KK] - DZ * 0.5f;
     bool mask_v_enable = (KKP1 >= k && KKP1 < k + W);
     float mask = (kk > KKP1) ? 1.0f : 0.0f;
     float VIS = 0;
@@ -361,6 +365,24 @@ static float ml_vis(St *s, int i, int j, int k, int kk) {
 static inline float ml_pow(float Dv, float Pv) {
     float in = 1.0f - 0.1f * Pv;
     return Dv * vmax(0.0f, in * in * in * in * in);
+}
+
+/* power-law coefficient plus its upwind flux F+ = max(+-F, 0).  Vector<float>
+ * solvers: D*max(0, x*x*x*x*x) + F+ (U-prognostic-microscale_1.cs:345-367);
+ * 512-bit solvers: FMA(D, max(0, x*((x*x)*(x*x))), F+), x = 1 - 0.1P
+ * (U-prognostic-microscale_1_Vec512.cs:355-377) */
+static inline float ml_coef(int W, float Dv, float Pv, float Fpos) {
+    if (W != 16) return ml_pow(Dv, Pv) + Fpos;
+    float in = 1.0f - 0.1f * Pv;
+    return fmaf(Dv, vmax(0.0f, in * ((in * in) * (in * in))), Fpos);
+}
+
+/* neighbour part AW*XW + AS*XS + AE*XE + AN*XN of a source term, summed left to
+ * right, or as FMA(AW,XW,AS*XS) + FMA(AE,XE,AN*XN) where the 512-bit solvers
+ * fuse it (U-prognostic-microscale_1_Vec512.cs:404-422) */
+static inline float ml_nb(bool fused, float aw, float xw, float as, float xs, float ae, float xe, float an, float xn) {
+    if (fused) return fmaf(aw, xw, as * xs) + fmaf(ae, xe, an * xn);
+    return aw * xw + as * xs + ae * xe + an * xn;
 }
 
 typedef struct { float UG; } uctx;
@@ -405,12 +427,12 @@ static void u1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
     DS = vmax(DS, 0.0001f); DN = vmax(DN, 0.0001f); DT = vmax(DT, 0.0001f);
     float PE = fabsf(FE / DE), PB = fabsf(FB / DB), PW = fabsf(FW / DW);
     float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
-    float BIM = ml_pow(DT, PT) + vmax(-FT, 0.0f);
-    float CIM = ml_pow(DB, PB) + vmax(FB, 0.0f);
-    float AE1 = ml_pow(DE, PE) + vmax(-FE, 0.0f);
-    float AW1 = ml_pow(DW, PW) + vmax(FW, 0.0f);
-    float AS1 = ml_pow(DS, PS) + vmax(FS, 0.0f);
-    float AN1 = ml_pow(DN, PN) + vmax(-FN, 0.0f);
+    float BIM = ml_coef(W, DT, PT, vmax(-FT, 0.0f));
+    float CIM = ml_coef(W, DB, PB, vmax(FB, 0.0f));
+    float AE1 = ml_coef(W, DE, PE, vmax(-FE, 0.0f));
+    float AW1 = ml_coef(W, DW, PW, vmax(FW, 0.0f));
+    float AS1 = ml_coef(W, DS, PS, vmax(FS, 0.0f));
+    float AN1 = ml_coef(W, DN, PN, vmax(-FN, 0.0f));
     float AP0 = s->AP0[kk];
     float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + AP0;
     float DDPX = DPMNEW_W[kk] - DPMNEW[kk];
@@ -421,9 +443,13 @@ static void u1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
         float in = 0.5f * (UKS_W[kk] + UKS[kk]);
         float wh = vmax(sqrtf(in * in + in2 * in2), 0.01f);
         float VEGv = s->VEG[IX3(s, i, j, kk - 1)];
-        DIM = AW1 * UK_W[kk] + AS1 * UK_S[kk] + AE1 * UK_E[kk] + AN1 * UK_N[kk] +
+        DIM = ml_nb(W == 16, AW1, UK_W[kk], AS1, UK_S[kk], AE1, UK_E[kk], AN1, UK_N[kk]) +
               AP0 * 0.5f * (UKS_W[kk] + UKS[kk]) + DDPX * DYKDZK +
               (s->Cor * (UG - UKv) - VEGv * UKv * wh) * AREAxy * DZ;
+    } else if (W == 16) {
+        /* U-prognostic-microscale_1_Vec512.cs:415-421: the pressure term is fused too */
+        DIM = ml_nb(true, AW1, UK_W[kk], AS1, UK_S[kk], AE1, UK_E[kk], AN1, UK_N[kk]) +
+              AP0 * 0.5f * (UKS_W[kk] + UKS[kk]) + fmaf(DDPX, DYKDZK, s->Cor * (UG - UKv) * AREAxy * DZ);
     } else {
         DIM = AW1 * UK_W[kk] + AS1 * UK_S[kk] + AE1 * UK_E[kk] + AN1 * UK_N[kk] +
               AP0 * 0.5f * (UKS_W[kk] + UKS[kk]) + DDPX * DYKDZK + s->Cor * (UG - UKv) * AREAxy * DZ;
@@ -504,12 +530,12 @@ static void v1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
     DS = vmax(DS, 0.0001f); DN = vmax(DN, 0.0001f); DT = vmax(DT, 0.0001f);
     float PE = fabsf(FE / DE), PB = fabsf(FB / DB), PW = fabsf(FW / DW);
     float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
-    float BIM = ml_pow(DT, PT) + vmax(-FT, 0.0f);
-    float CIM = ml_pow(DB, PB) + vmax(FB, 0.0f);
-    float AE1 = ml_pow(DE, PE) + vmax(-FE, 0.0f);
-    float AW1 = ml_pow(DW, PW) + vmax(FW, 0.0f);
-    float AS1 = ml_pow(DS, PS) + vmax(FS, 0.0f);
-    float AN1 = ml_pow(DN, PN) + vmax(-FN, 0.0f);
+    float BIM = ml_coef(W, DT, PT, vmax(-FT, 0.0f));
+    float CIM = ml_coef(W, DB, PB, vmax(FB, 0.0f));
+    float AE1 = ml_coef(W, DE, PE, vmax(-FE, 0.0f));
+    float AW1 = ml_coef(W, DW, PW, vmax(FW, 0.0f));
+    float AS1 = ml_coef(W, DS, PS, vmax(FS, 0.0f));
+    float AN1 = ml_coef(W, DN, PN, vmax(-FN, 0.0f));
     float AP0 = s->AP0[kk];
     float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + AP0;
     float DDPY = DPMNEW_S[kk] - DPMNEW[kk];
@@ -520,7 +546,8 @@ static void v1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
         float in = 0.5f * (UKS_W[kk] + UKS[kk]);
         float wh = vmax(sqrtf(in * in + in2 * in2), 0.01f);
         float VEGv = s->VEG[IX3(s, i, j, kk - 1)];
-        DIM = AW1 * VK_W[kk] + AS1 * VK_S[kk] + AE1 * VK_E[kk] + AN1 * VK_N[kk] +
+        /* fused by the 512-bit solver only here (V-prognostic-microscale_1_Vec512.cs:404-411) */
+        DIM = ml_nb(W == 16, AW1, VK_W[kk], AS1, VK_S[kk], AE1, VK_E[kk], AN1, VK_N[kk]) +
               AP0 * 0.5f * (VKS_S[kk] + VKS[kk]) + DDPY * DXKDZK +
               (s->Cor * (VG - VKv) - VEGv * VKv * wh) * AREAxy * DZ;
     } else {
@@ -601,12 +628,12 @@ static void w1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
     DS = vmax(DS, 0.0001f); DN = vmax(DN, 0.0001f); DT = vmax(DT, 0.0001f);
     float PE = fabsf(FE / DE), PB = fabsf(FB / DB), PW = fabsf(FW / DW);
     float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
-    float BIM = ml_pow(DT, PT) + vmax(-FT, 0.0f);
-    float CIM = ml_pow(DB, PB) + vmax(FB, 0.0f);
-    float AE1 = ml_pow(DE, PE) + vmax(-FE, 0.0f);
-    float AW1 = ml_pow(DW, PW) + vmax(FW, 0.0f);
-    float AS1 = ml_pow(DS, PS) + vmax(FS, 0.0f);
-    float AN1 = ml_pow(DN, PN) + vmax(-FN, 0.0f);
+    float BIM = ml_coef(W, DT, PT, vmax(-FT, 0.0f));
+    float CIM = ml_coef(W, DB, PB, vmax(FB, 0.0f));
+    float AE1 = ml_coef(W, DE, PE, vmax(-FE, 0.0f));
+    float AW1 = ml_coef(W, DW, PW, vmax(FW, 0.0f));
+    float AS1 = ml_coef(W, DS, PS, vmax(FS, 0.0f));
+    float AN1 = ml_coef(W, DN, PN, vmax(-FN, 0.0f));
     float AP0 = s->AP0[kk];
     float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + AP0;
     float DDPZ = DPMNEW[kk - 1] - DPMNEW[kk];
@@ -617,10 +644,11 @@ static void w1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
         float in = 0.5f * (UKS_W[kk] + UKS[kk]);
         float wh = vmax(sqrtf(in * in + in2 * in2), 0.01f);
         float VEGv = s->VEG[IX3(s, i, j, kk - 1)];
-        DIM = AW1 * WK_W[kk] + AS1 * WK_S[kk] + AE1 * WK_E[kk] + AN1 * WK_N[kk] +
+        /* W-prognostic-microscale_1_Vec512.cs:367-383 fuses both forms */
+        DIM = ml_nb(W == 16, AW1, WK_W[kk], AS1, WK_S[kk], AE1, WK_E[kk], AN1, WK_N[kk]) +
               AP0 * 0.5f * (WKS[kk - 1] + WKS[kk]) + DDPZ * AREAxy - VEGv * WKv * wh * AREAxy * DZ;
     } else {
-        DIM = AW1 * WK_W[kk] + AS1 * WK_S[kk] + AE1 * WK_E[kk] + AN1 * WK_N[kk] +
+        DIM = ml_nb(W == 16, AW1, WK_W[kk], AS1, WK_S[kk], AE1, WK_E[kk], AN1, WK_N[kk]) +
               AP0 * 0.5f * (WKS[kk - 1] + WKS[kk]) + DDPZ * AREAxy;
     }
     if (k > KKP1) {

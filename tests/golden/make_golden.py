@@ -49,6 +49,9 @@ CASES = [
     ("c1_m1_it4", "config1", dict(turbulence_model=1), 4, False),
     ("c1_m1_it200", "config1", dict(turbulence_model=1), 200, False),
     ("c1_m2_it4", "config1", dict(turbulence_model=2), 4, False),
+    # the 512-bit mixing-length solvers (simd_width 16)
+    ("tiny_m1_w16", "tiny_case", dict(n=9, vertcellsff=16, turbulence_model=1, simd_width=16), 3, True),
+    ("c1_m1_w16_it4", "config1", dict(turbulence_model=1, simd_width=16), 4, False),
 ]
 
 
@@ -63,7 +66,12 @@ DIAG_FIELDS = ("uk", "vk", "wk", "dpm")
 
 def make_case(gen, kwargs, iterations):
     from gral_amd import domain
-    return getattr(domain, gen)(iterations=iterations, **kwargs)
+    kw = dict(kwargs)
+    width = kw.pop("simd_width", None)
+    c = getattr(domain, gen)(iterations=iterations, **kw)
+    if width is not None:
+        c.simd_width = width
+    return c
 
 
 def crop(a, kadvmax):

@@ -7,7 +7,11 @@ src/), in a different shape:
   * the Vector<float> solvers (U/V/W-prognostic-microscale_1.cs) are executed
     block by block on width-W numpy float32 vectors, as the JIT does;
   * the scalar solvers (_0/_2, TKE) loop over k with numpy float32 scalars;
-  * FP64 sub-expressions are evaluated with Python floats.
+  * FP64 sub-expressions are evaluated with Python floats;
+  * the fused multiply-adds of the 512-bit solvers
+    (U/V/W-prognostic-microscale_1_Vec512.cs, simd_width 16) are evaluated by
+    fma32() below: exact product and error-free sum in float64, then one
+    rounding to float32 (no C library fmaf, which the C oracle uses).
 NumPy 2 treats Python scalars as weak, so every float32 literal is written as
 F(...) and every C# double promotion as float(...).
 Only for small cases: pure Python loops over columns.
@@ -45,6 +49,28 @@ def net_min(a, b):
     return a if math.copysign(1.0, a) < 0 else b
 
 
+def fma32(a, b, c):
+    """a*b+c of float32 operands with a single rounding to float32.
+
+    The product of two float32 values is exact in float64; the sum s = p + c is
+    rounded once to float64 and its exact error e is recovered by TwoSum, so
+    p + c == s + e exactly.  Rounding s to float32 can differ from rounding
+    s + e only when s is a tie between two float32 neighbours (rounding is
+    monotonic, so s never crosses one); then e decides the side."""
+    a64, b64, c64 = (np.asarray(v, dtype=np.float64) for v in (a, b, c))
+    p = a64 * b64
+    s = p + c64
+    bb = s - p
+    e = (p - (s - bb)) + (c64 - bb)
+    r = s.astype(F)
+    r64 = r.astype(np.float64)
+    other = np.nextafter(r, np.where(s > r64, F(np.inf), F(-np.inf)).astype(F))
+    tie = (s != r64) & (s == (r64 + other.astype(np.float64)) * 0.5)
+    fix = tie & (e != 0) & (np.sign(e) == np.sign(s - r64))
+    out = np.where(fix, other, r).astype(F)
+    return out if out.ndim else F(out)
+
+
 def conv_to_int(x: float) -> int:    # ProgramFunctions.cs:792-804 (double overload)
     return int(x - 0.5) if x < 0 else int(x + 0.5)
 
@@ -65,7 +91,7 @@ class Solver:
         self.VEG = None if c.veg is None else c.veg.astype(F)
         self.VEGP = None if c.veg_present is None else c.veg_present.astype(bool)
         self.model = c.turbulence_model
-        self.W = 8
+        self.W = int(getattr(c, "simd_width", 8))
         self.relax, self.relaxp = F(c.relax), F(c.relaxp)
         self.bz0 = F(c.building_z0)
         self.Cor = F(c.coriolis)
@@ -330,7 +356,10 @@ class Solver:
         DXKDZK, DYKDZK = self.DXK * D, self.DYK * D
         AREA = self.AREA
         UK, VK, WK, UKS, VKS, WKS = self.UK, self.VK, self.WK, self.UKS, self.VKS, self.WKS
-        zs = (self.HOK[ks] - self.HOK[kk]) - D * F(0.5)
+        if W == 16:                          # U-prognostic-microscale_1_Vec512.cs:218
+            zs = self.HOK[ks] - fma32(D, F(0.5), np.full(W, self.HOK[kk]))
+        else:
+            zs = (self.HOK[ks] - self.HOK[kk]) - D * F(0.5)
         VIS = np.zeros(W, dtype=F)
         mask_on = KKP1 >= k and KKP1 < k + W
         Mask = (ks > KKP1).astype(F)
@@ -398,6 +427,8 @@ class Solver:
 
         def pl(Dv, Pv, Fv):
             t = F(1) - F(0.1) * Pv
+            if W == 16:                      # U-prognostic-microscale_1_Vec512.cs:355-377
+                return fma32(Dv, vmax(z, t * ((t * t) * (t * t))), vmax(Fv, z))
             return Dv * vmax(z, t * t * t * t * t) + vmax(Fv, z)
         BIM = pl(DT, PT, -FT)
         CIM = pl(DB, PB, FB)
@@ -410,13 +441,14 @@ class Solver:
         vegp = self.VEGP is not None and self.VEGP[i, j]
         if comp == "u":
             X = UK
-            DDP = (self.DPMNEW[i - 1, j, ks] - self.DPMNEW[i, j, ks]) * DYKDZK
+            ddp, ddm = self.DPMNEW[i - 1, j, ks] - self.DPMNEW[i, j, ks], DYKDZK
         elif comp == "v":
             X = VK
-            DDP = (self.DPMNEW[i, j - 1, ks] - self.DPMNEW[i, j, ks]) * DXKDZK
+            ddp, ddm = self.DPMNEW[i, j - 1, ks] - self.DPMNEW[i, j, ks], DXKDZK
         else:
             X = WK
-            DDP = (self.DPMNEW[i, j, ks - 1] - self.DPMNEW[i, j, ks]) * AREA
+            ddp, ddm = self.DPMNEW[i, j, ks - 1] - self.DPMNEW[i, j, ks], AREA
+        DDP = ddp * ddm
         x = X[i, j, ks]
         # AP0_V * Vect_05 * (a + b) is (AP0*0.5)*(a+b)
         if comp == "u":
@@ -425,21 +457,31 @@ class Solver:
             ssum = VKS[i, j - 1, ks] + VKS[i, j, ks]
         else:
             ssum = WKS[i, j, ks - 1] + WKS[i, j, ks]
-        DIM = AW1 * X[i - 1, j, ks] + AS1 * X[i, j - 1, ks] + AE1 * X[i + 1, j, ks] + AN1 * X[i, j + 1, ks] + AP0 * F(0.5) * ssum + DDP
+        # the 512-bit solvers fuse the neighbour products pairwise, except V
+        # without vegetation (V-prognostic-microscale_1_Vec512.cs:414-419)
+        if W == 16 and (comp != "v" or vegp):
+            nb = fma32(AW1, X[i - 1, j, ks], AS1 * X[i, j - 1, ks]) + fma32(AE1, X[i + 1, j, ks], AN1 * X[i, j + 1, ks])
+        else:
+            nb = AW1 * X[i - 1, j, ks] + AS1 * X[i, j - 1, ks] + AE1 * X[i + 1, j, ks] + AN1 * X[i, j + 1, ks]
+        DIM = nb + AP0 * F(0.5) * ssum
         if comp in ("u", "v"):
             G = UG
             if vegp:
                 in2 = F(0.5) * (VKS[i - 1, j, ks] + VKS[i, j, ks])
                 in1 = F(0.5) * (UKS[i - 1, j, ks] + UKS[i, j, ks])
                 wh = vmax(np.sqrt(in1 * in1 + in2 * in2), np.full(W, F(0.01)))
-                DIM = DIM + (self.Cor * (G - x) - self.VEG[i, j, ks - 1] * x * wh) * AREA * D
+                DIM = DIM + DDP + (self.Cor * (G - x) - self.VEG[i, j, ks - 1] * x * wh) * AREA * D
+            elif W == 16 and comp == "u":    # U-prognostic-microscale_1_Vec512.cs:419-421
+                DIM = DIM + fma32(ddp, ddm, self.Cor * (G - x) * AREA * D)
             else:
-                DIM = DIM + self.Cor * (G - x) * AREA * D
-        elif vegp:
-            in2 = F(0.5) * (VKS[i, j - 1, ks] + VKS[i, j, ks])
-            in1 = F(0.5) * (UKS[i - 1, j, ks] + UKS[i, j, ks])
-            wh = vmax(np.sqrt(in1 * in1 + in2 * in2), np.full(W, F(0.01)))
-            DIM = DIM - self.VEG[i, j, ks - 1] * x * wh * AREA * D
+                DIM = DIM + DDP + self.Cor * (G - x) * AREA * D
+        else:
+            DIM = DIM + DDP
+            if vegp:
+                in2 = F(0.5) * (VKS[i, j - 1, ks] + VKS[i, j, ks])
+                in1 = F(0.5) * (UKS[i - 1, j, ks] + UKS[i, j, ks])
+                wh = vmax(np.sqrt(in1 * in1 + in2 * in2), np.full(W, F(0.01)))
+                DIM = DIM - self.VEG[i, j, ks - 1] * x * wh * AREA * D
         if comp in ("u", "v") and k <= KKP1 < k + W:
             ii = KKP1 - k
             m2 = np.zeros(W, dtype=F)
