@@ -436,14 +436,16 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
         d.sync = fs;
     }
     d.planes = nullptr;
-    d.rec4 = nullptr;
+    d.rec3 = nullptr;
+    d.wcol = nullptr;
     d.ptab[0] = d.ptab[1] = d.ttab[0] = d.ttab[1] = nullptr;
     if (d.tiled) {
         if (p->turbulence_model == 1) {
             float *pl;
-            if ((rc = dalloc(h, &pl, 16 * n3))) return rc;
-            d.planes = pl;                       // 12 floats per cell
-            d.rec4 = pl + 12 * n3;               // then 4 per cell
+            if ((rc = dalloc(h, &pl, 15 * n3 + n2))) return rc;
+            d.planes = pl;                       // 12 (U, V) or 8 (W) floats per cell
+            d.rec3 = pl + 12 * n3;               // then 3 per cell
+            d.wcol = pl + 15 * n3;               // then 1 per column
         }
         // PIMP/TERMP of the pressure TDMA for every column, in the reference's
         // row order (fresh row arrays per j row, columns in i1 order for IS=+1 and

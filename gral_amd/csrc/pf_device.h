@@ -45,8 +45,9 @@ struct DomT {
     int tiled;                       // 2: strip-pipelined sweeps, 0: one launch per anti-diagonal
     P<int> sync;                     // strip sweeps: [64] header (sticky error flag at 0), then 4 blocks
     int sync_stride;                 //   of sync_stride ints (ticket, unused, progress per strip)
-    P<float> planes;                 // [cells][12] mixing-length coefficient records (neighbour weights, source pieces)
-    P<float> rec4;                   // [cells][4]  C, TERMP, P and e of the same cells
+    P<float> planes;                 // [cells][12 (U, V) or 8 (W)] mixing-length coefficient records
+    P<float> rec3;                   // [cells][3]  C, TERMP and P of the same cells
+    P<float> wcol;                   // [columns] wall term of the U / V sweep at KKART+1
     P<const float> ptab[2], ttab[2]; // pressure PIMP/TERMP per column for IS=+1 / IS=-1
     P<const int4> meta;              // per column: KK, VI, KSTART | adv<<16 | vegp<<17, KKm_i | KKm_j<<16
 };
@@ -68,6 +69,17 @@ __device__ __forceinline__ float4 ldg4(const gf32 *p) {
 __device__ __forceinline__ void stg4(gf32 *p, float a, float b, float c, float e) {
     v4f v = {a, b, c, e};
     *(gv4f *)p = v;
+}
+// 12-byte records (p 4-byte aligned): one dwordx3 access
+typedef float v3fa __attribute__((ext_vector_type(3), aligned(4)));
+typedef v3fa __attribute__((address_space(1))) gv3f;
+__device__ __forceinline__ float3 ldg3(const gf32 *p) {
+    v3fa v = *(const gv3f *)p;
+    return make_float3(v.x, v.y, v.z);
+}
+__device__ __forceinline__ void stg3(gf32 *p, float a, float b, float c) {
+    v3fa v = {a, b, c};
+    *(gv3f *)p = v;
 }
 template <class T>
 __device__ __forceinline__ int4 ldg4i(const T *p) {

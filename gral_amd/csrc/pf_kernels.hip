@@ -1092,11 +1092,15 @@ __global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t,
 #define STRIP_WAVES (STRIP_ROWS + 1)
 
 // coefficient records written by k_ml_coef and read by the strip sweeps; per
-// cell, two records: rec12 = {AW, AS, AE, AN | f, g, w, c0 | X1, X2, X3, X4}
-// (written once by the first pass of k_ml_coef) and rec4 = {C, T, P, e} (written
-// once after the P/TERMP recurrence); each is written whole, in one pass
-#define REC12 12
-#define REC4 4
+// cell, two records, each written whole, in one pass:
+//   planes: U, V {AW, AS, AE, AN | f, g, e, c0 | X1, X2, X3, X4} (REC_UV floats),
+//           W    {AW, AS, AE, AN | f, g, e, X1}                   (REC_W floats),
+//   written by the first pass of k_ml_coef;
+//   rec3 = {C, T, P}, written after the P/TERMP recurrence.
+// The U/V wall term exists only at KKART+1 and is kept per column (wcol).
+#define REC_UV 12
+#define REC_W 8
+template <int COMP> __device__ __forceinline__ constexpr int rec_floats() { return COMP == 2 ? REC_W : REC_UV; }
 
 __device__ __forceinline__ void sweep_to_ij(const DomG &d, int comp_is_pressure, int a, int b, int IS, int JS, int &i, int &j) {
     if (comp_is_pressure) {   // a = j1, b = i1
@@ -1137,7 +1141,7 @@ __device__ __forceinline__ Meta unpack_meta(int4 m) {
 // block's 256 lanes (a lane per level, whatever the column heights), which
 // compute the coefficients.  One wave then runs the P/TERMP recurrence of all
 // COEF_COLS columns at once (lane = column), and the lanes write the last part
-// of each cell's record.  Each 64-byte record is written once, as 4 float4.
+// of each cell's record.  Each record is written once, as float4 / float3 stores.
 // WFIX: the solver width when every situation of the launch has it (the
 // width-dependent arithmetic then folds at compile time), 0 to read it per
 // situation.
@@ -1151,8 +1155,8 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
     const int lane = threadIdx.x % WAVE;
     const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2;
     const int na = d.nii - 1 - oa + 1, nb = d.njj - 1 - ia + 1;
-    // per column c: A, B, C, e over k at sm + c*4*nkl
-    int *scol = (int *)(sm + 4 * COEF_COLS * nkl);         // [COEF_COLS][8]: KSTART, KK, VI, i, j, first item, active
+    // per column c: A, B, C over k at sm + c*3*nkl
+    int *scol = (int *)(sm + 3 * COEF_COLS * nkl);         // [COEF_COLS][8]: KSTART, KK, VI, i, j, first item, active
     int *stot = scol + 8 * COEF_COLS;
     const int blk = xcd_block((na * nb + COEF_COLS - 1) / COEF_COLS);
     if (blk < 0) return;                                      // uniform over the block
@@ -1197,17 +1201,22 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
         ColInfo q = col_info(d, scol[c * 8 + 3], scol[c * 8 + 4]);
         if (WFIX) q.W = WFIX;
         MlPieces p = ml_pieces<COMP>(d, q, k);
-        float *sA = sm + (size_t)c * 4 * nkl;
-        sA[k] = p.A; sA[nkl + k] = p.B; sA[2 * nkl + k] = p.C; sA[3 * nkl + k] = p.e;
-        gf32 *P = planes + (c3(d, q.i, q.j) + k) * REC12;
+        float *sA = sm + (size_t)c * 3 * nkl;
+        sA[k] = p.A; sA[nkl + k] = p.B; sA[2 * nkl + k] = p.C;
+        gf32 *P = planes + (c3(d, q.i, q.j) + k) * rec_floats<COMP>();
         stg4(P, p.AW1, p.AS1, p.AE1, p.AN1);
-        stg4(P + 4, p.f, p.g, p.w, p.c0);
-        stg4(P + 8, p.X1, p.X2, p.X3, p.X4);
+        if (COMP == 2) {
+            stg4(P + 4, p.f, p.g, p.e, p.X1);
+        } else {
+            stg4(P + 4, p.f, p.g, p.e, p.c0);
+            stg4(P + 8, p.X1, p.X2, p.X3, p.X4);
+            if (k == q.KK + 1) d.wcol[c2(d, q.i, q.j)] = p.w;
+        }
     }
     lds_barrier();
     if (threadIdx.x < COEF_COLS && scol[threadIdx.x * 8 + 6]) {
         const int c = threadIdx.x;
-        float *__restrict__ A = sm + (size_t)c * 4 * nkl;
+        float *__restrict__ A = sm + (size_t)c * 3 * nkl;
         float *__restrict__ Bq = A + nkl;
         float *__restrict__ Cq = Bq + nkl;
         const int KSTART = scol[c * 8 + 0], KKP1 = scol[c * 8 + 1] + 1, VI = scol[c * 8 + 2];
@@ -1224,9 +1233,9 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
     for (int t = threadIdx.x; t < total; t += 4 * WAVE) {
         const int c = column_of(t);
         const int k = scol[c * 8 + 0] + (t - scol[c * 8 + 5]);
-        const float *sA = sm + (size_t)c * 4 * nkl;
-        gf32 *P = d.rec4 + (c3(d, scol[c * 8 + 3], scol[c * 8 + 4]) + k) * REC4;
-        stg4(P, sA[2 * nkl + k], sA[k], sA[nkl + k], sA[3 * nkl + k]);   // C, T, P, e
+        const float *sA = sm + (size_t)c * 3 * nkl;
+        gf32 *P = d.rec3 + (c3(d, scol[c * 8 + 3], scol[c * 8 + 4]) + k) * 3;
+        stg3(P, sA[2 * nkl + k], sA[k], sA[nkl + k]);                   // C, T, P
     }
 }
 
@@ -1378,7 +1387,7 @@ __device__ __forceinline__ int wave_max(int v, bool on) {
 // per-level values of one column that do not change during the sweep, loaded
 // one local step ahead
 struct MlPre {
-    float aw, as, ae, an, e, f, g, w, c0, x1, x2, x3, x4, c, t, p;
+    float aw, as, ae, an, e, f, g, c0, x1, x2, x3, x4, c, t, p;
     float x, xa, xb;      // own old value; old neighbours in the +IS (xa) and +JS (xb) directions
     float ni, nj;         // -IS / -JS neighbours, prefetched only when another strip or the boundary holds them
 };
@@ -1387,12 +1396,20 @@ template <int COMP>
 __device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *planes, size_t n3, int i, int j, int k, int IS, int JS,
                                             MlPre &r) {
     (void)n3;
-    const gf32 *P = planes + (c3(d, i, j) + k) * REC12;
-    float4 v0 = ldg4(P), v2 = ldg4(P + 4), v3 = ldg4(P + 8), v1 = ldg4(d.rec4 + (c3(d, i, j) + k) * REC4);
+    const gf32 *P = planes + (c3(d, i, j) + k) * rec_floats<COMP>();
+    float4 v0 = ldg4(P), v2 = ldg4(P + 4);
+    float3 v1 = ldg3(d.rec3 + (c3(d, i, j) + k) * 3);
     r.aw = v0.x; r.as = v0.y; r.ae = v0.z; r.an = v0.w;
-    r.c = v1.x; r.t = v1.y; r.p = v1.z; r.e = v1.w;
-    r.f = v2.x; r.g = v2.y; r.w = v2.z; r.c0 = v2.w;
-    r.x1 = v3.x; r.x2 = v3.y; r.x3 = v3.z; r.x4 = v3.w;
+    r.c = v1.x; r.t = v1.y; r.p = v1.z;
+    r.f = v2.x; r.g = v2.y; r.e = v2.z;
+    if (COMP == 2) {
+        r.x1 = v2.w;
+        r.c0 = r.x2 = r.x3 = r.x4 = 0.0f;
+    } else {
+        float4 v3 = ldg4(P + 8);
+        r.c0 = v2.w;
+        r.x1 = v3.x; r.x2 = v3.y; r.x3 = v3.z; r.x4 = v3.w;
+    }
     const gf32 *X = COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk;
     r.x = X[c3(d, i, j) + k];
     r.xa = X[c3(d, i + IS, j) + k];
@@ -1545,6 +1562,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
     gf32 *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk);
     MlPre pre[2];
     float xfull[2];
+    float wn = 0.0f;                                          // wall term of the located column (U, V)
     Meta mn;
     int in_ = 0, jn_ = 0;
     bool inr_n = false;
@@ -1576,6 +1594,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
             }
         }
         if (mn.adv) {
+            if (COMP != 2) wn = d.wcol[c2(d, in_, jn_)];
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
                 int k = mn.KSTART + lane + WAVE * m;
@@ -1606,6 +1625,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         const bool act = mn.adv, inr = inr_n;
         const Meta mt = mn;
         const int i = in_, j = jn_;
+        const float wcur = wn;
         const ColInfo q = meta_col(mt, i, j, d.simd);
         float *xcur = xbuf + (size_t)((s & 1) * ROWS + r) * nkl;
         const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r) * nkl;
@@ -1631,7 +1651,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                     float xs = (JS == 1) ? xnew_j : r_.xb, xn = (JS == 1) ? r_.xb : xnew_j;
                     MlPieces pc;
                     pc.AW1 = r_.aw; pc.AS1 = r_.as; pc.AE1 = r_.ae; pc.AN1 = r_.an; pc.e = r_.e; pc.f = r_.f;
-                    pc.g = r_.g; pc.w = r_.w; pc.c0 = r_.c0; pc.X1 = r_.x1; pc.X2 = r_.x2; pc.X3 = r_.x3; pc.X4 = r_.x4;
+                    pc.g = r_.g; pc.w = wcur; pc.c0 = r_.c0; pc.X1 = r_.x1; pc.X2 = r_.x2; pc.X3 = r_.x3; pc.X4 = r_.x4;
                     float D = ml_source<COMP>(d, q, k, sdzk[k], pc, xw, xs, xe, xn, r_.x);
                     sD[k] = D; sC[k] = r_.c; sT[k] = r_.t; sP[k] = r_.p;
                     if (k <= KKP1) sQ[k] = D * r_.t;
@@ -2145,7 +2165,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
             nblk = max(nblk, ((na * nb + COEF_COLS - 1) / COEF_COLS + NXCD - 1) / NXCD * NXCD);
             nst = max(nst, (ib - ia + ROWS) / ROWS);
         }
-        size_t shm_c = sizeof(float) * 4 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
+        size_t shm_c = sizeof(float) * 3 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
         L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
         size_t shm_t = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) + sizeof(int) * (6 * ROWS + 4);
         L.go(fam, [&] { hipLaunchKernelGGL(strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm_t, s, dd, is, js); });

@@ -52,7 +52,9 @@ typedef struct {
  * StdDeviationV/W, BdLayHeight, V0[1], SubDomainRefPos. */
 typedef struct {
     int32_t turbulence_model;   /* 0 no diffusion, 1 mixing length (default), 2 k-eps */
-    int32_t simd_width;         /* Vector<float>.Count semantics of the mixing-length solvers: 8 */
+    int32_t simd_width;         /* mixing-length solver width: 8 = Vector<float> (AVX2),
+                                   16 = Vector512 solvers (PrognosticFlowfield.cs:1072-1091,
+                                   U/V/W-prognostic-microscale_1_Vec512.cs) */
     int32_t iter_min, iter_max; /* IterationStepsMin/Max */
     float relax, relaxp;        /* velocity / pressure relaxation */
     float building_z0;          /* building wall roughness [m] */
