@@ -1516,10 +1516,10 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
     // per row: D, C, T, P, Q and the back-substitution result Xn
     float *sD = sm + (size_t)r * 6 * nkl + SLOT_PAD, *sC = sD + nkl, *sT = sC + nkl, *sP = sT + nkl, *sQ = sP + nkl;
     float *sXn = sQ + nkl;
-    float *xbuf = sm + (size_t)ROWS * 6 * nkl + SLOT_PAD;      // [2][ROWS] slots of whole columns
+    float *xbuf = sm + (size_t)ROWS * 6 * nkl + SLOT_PAD;      // [2][ROWS] slots of whole columns (old values)
     float *sdzk = sm + (size_t)ROWS * 8 * nkl;                // [nkp]
-    int *sinfo = (int *)(sdzk + nkp);                         // per row: active, KSTART, KK, VI, KKm
-    int *sctl = sinfo + 5 * ROWS;                             // strip index of this workgroup
+    int *sinfo = (int *)(sdzk + nkp);                         // [2][ROWS] per row: active, KSTART, KK, VI, KKm
+    int *sctl = sinfo + 10 * ROWS;                            // strip index of this workgroup
     float *sdump = (float *)(sctl + 4);                       // [WAVE][CH] sinks of the chains
     if (threadIdx.x == 0) sctl[0] = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int k = threadIdx.x; k < nkp; k += STRIP_WAVES * WAVE) sdzk[k] = d.dzk[k];
@@ -1533,9 +1533,10 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         // is live in the recurrences.
         for (int s = 0; s < nsteps; ++s) {
             lds_barrier();                                    // the rows' sources are assembled
+            const int *si = sinfo + (s & 1) * 5 * ROWS;
             const int cl = lane < ROWS ? lane : 0;
-            const bool on = lane < ROWS && sinfo[cl * 5];
-            const int KSTART = sinfo[cl * 5 + 1], KK = sinfo[cl * 5 + 2], VI = sinfo[cl * 5 + 3], KKm = sinfo[cl * 5 + 4];
+            const bool on = lane < ROWS && si[cl * 5];
+            const int KSTART = si[cl * 5 + 1], KK = si[cl * 5 + 2], VI = si[cl * 5 + 3], KKm = si[cl * 5 + 4];
             // first level the back-substitution updates (U/V skip levels inside
             // the building or its upstream neighbour; W updates every level)
             const int kend = (COMP == 2) ? KSTART : max(KSTART, KKm + 1);
@@ -1549,7 +1550,6 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                 chain_back(P + VI, Q + VI, Xs + VI, Xn + VI, d.relax, nb, nbmax, sdump + lane * CH);
             }
             lds_barrier();                                    // the recurrences are done
-            lds_barrier();                                    // the rows have written back
         }
         return;
     }
@@ -1621,6 +1621,18 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         sw.wait(ROWS);
         locate_prev();
     }
+    // The results of a step are merged on use: level k of a row's previous
+    // column is the recurrence output (sXn) where the back-substitution updated
+    // it (kend..VI), else the old value in the column buffer.  Each row stores
+    // the updated levels of its previous column at the end of the next step's
+    // assembly (before that step's publish, so progress p still means that
+    // steps < p are stored), after the assembly's waits for its prefetched
+    // loads, which therefore never wait for these write-through stores.
+    int pkend = 1 << 30, pvi = -1;                            // own previous column: updated levels
+    size_t po = 0;
+    auto store_prev = [&]() {
+        for (int k = pkend + lane; k <= pvi; k += WAVE) st_through(X + po + k, sXn[k]);
+    };
     for (int s = 0; s < nsteps; ++s) {
         const bool act = mn.adv, inr = inr_n;
         const Meta mt = mn;
@@ -1630,6 +1642,9 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         float *xcur = xbuf + (size_t)((s & 1) * ROWS + r) * nkl;
         const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r) * nkl;
         const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r - 1) * nkl;
+        const float *xs_new = sXn - (size_t)6 * nkl;         // row r-1's recurrence output
+        int *si = sinfo + (s & 1) * 5 * ROWS;
+        const int *sp = sinfo + ((s - 1) & 1) * 5 * ROWS;
         const bool w_in = (s - r >= 1), s_in = (r >= 1);
         if (inr) {
 #pragma unroll
@@ -1640,13 +1655,20 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         }
         if (act) {
             const int KKP1 = q.KK + 1;
+            // updated levels of the southern neighbour's column (row r-1, previous step)
+            int skend = 1 << 30, svi = -1;
+            if (s_in && sp[(r - 1) * 5]) {
+                const int sks = sp[(r - 1) * 5 + 1], skm = sp[(r - 1) * 5 + 4];
+                skend = (COMP == 2) ? sks : max(sks, skm + 1);
+                svi = sp[(r - 1) * 5 + 3];
+            }
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
                 int k = q.KSTART + lane + WAVE * m;
                 if (k <= q.VI) {
                     const MlPre &r_ = pre[m];
-                    float xnew_i = w_in ? xw_lds[k] : r_.ni;
-                    float xnew_j = s_in ? xs_lds[k] : r_.nj;
+                    float xnew_i = w_in ? ((k >= pkend && k <= pvi) ? sXn[k] : xw_lds[k]) : r_.ni;
+                    float xnew_j = s_in ? ((k >= skend && k <= svi) ? xs_new[k] : xs_lds[k]) : r_.nj;
                     float xw = (IS == 1) ? xnew_i : r_.xa, xe = (IS == 1) ? r_.xa : xnew_i;
                     float xs = (JS == 1) ? xnew_j : r_.xb, xn = (JS == 1) ? r_.xb : xnew_j;
                     MlPieces pc;
@@ -1659,15 +1681,19 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
             }
             if (lane == 0) {
                 int KKm = (COMP == 0) ? mt.KKmi : (COMP == 1) ? mt.KKmj : q.KK;
-                sinfo[r * 5 + 1] = q.KSTART; sinfo[r * 5 + 2] = q.KK; sinfo[r * 5 + 3] = q.VI; sinfo[r * 5 + 4] = KKm;
+                si[r * 5 + 1] = q.KSTART; si[r * 5 + 2] = q.KK; si[r * 5 + 3] = q.VI; si[r * 5 + 4] = KKm;
             }
         }
         const int kend = (COMP == 2) ? q.KSTART : max(q.KSTART, ((COMP == 0) ? mt.KKmi : mt.KKmj) + 1);
-        if (lane == 0) sinfo[r * 5] = act;
+        if (lane == 0) si[r * 5] = act;
+        store_prev();
+        pkend = act ? kend : 1 << 30;
+        pvi = act ? q.VI : -1;
+        po = c3(d, i, j);
         lds_barrier();
         // while the chain wave runs the recurrences: the last row publishes the
-        // steps before this one (their stores were issued in the previous
-        // step's write-back), and every row prefetches its next column.  The
+        // steps before this one (their stores were issued at the top of this
+        // step), and every row prefetches its next column.  The
         // look-ahead of row r reads columns that rows r and r+1 write two steps
         // later at the earliest (for the last row, the next strip, which waits
         // for this one); row 0 also reads the previous strip's last row, after
@@ -1678,17 +1704,9 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
             sw.wait(s + 1 + ROWS);
             locate_prev();
         }
-        lds_barrier();
-        if (act) {
-            const size_t o = c3(d, i, j);
-            for (int k = q.KSTART + lane; k <= q.VI; k += WAVE) {
-                float v = (k >= kend) ? sXn[k] : xcur[k];
-                xcur[k] = v;
-                st_through(X + o + k, v);
-            }
-        }
-        lds_barrier();
+        lds_barrier();                                        // the recurrence outputs are ready
     }
+    store_prev();
     if (r == ROWS - 1) strip_publish(mine, nsteps, lane);
 }
 
@@ -2167,7 +2185,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         }
         size_t shm_c = sizeof(float) * 3 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
         L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
-        size_t shm_t = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) + sizeof(int) * (6 * ROWS + 4);
+        size_t shm_t = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) + sizeof(int) * (10 * ROWS + 4);
         L.go(fam, [&] { hipLaunchKernelGGL(strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm_t, s, dd, is, js); });
     };
     bool all_w8 = true;
