@@ -1426,6 +1426,7 @@ __device__ __forceinline__ ColInfo meta_col(const Meta &m, int i, int j, int W) 
 // prefetched per-level values of one column of the pressure sweep
 struct PrPre {
     float div, t, p, pa, pb;       // DIV, TERMP, PIMP, old DPM in the +IS (i) and +JS (j) directions
+    float dn;                      // DPMNEW of the cell (read-modify-written one step later)
     float ni, nj;                  // -IS / -JS neighbours when another strip or the boundary holds them
 };
 
@@ -1726,12 +1727,13 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
     const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     const bool chainw = r == ROWS;                            // the chain wave owns no row
     gi32 *sync = d.sync + SYNC_BLOCK0;
-    float *sD = sm + (size_t)r * 4 * nkl + SLOT_PAD, *sQ = sD + nkl, *sT = sQ + nkl, *sP = sT + nkl;
-    float *xbuf = sm + (size_t)ROWS * 4 * nkl + SLOT_PAD;      // [2][ROWS] slots of whole DPM columns
-    float *ring = sm + (size_t)ROWS * 6 * nkl + SLOT_PAD;      // [RING] QIMP row scratch slots
-    float *sdzk = sm + (size_t)(ROWS * 6 + RING) * nkl, *satp = sdzk + nkp;
-    int *sinfo = (int *)(satp + nkp);                         // per row: active, KK, VI, ring slot
-    int *sctl = sinfo + 4 * ROWS;
+    // per row: D, Q, T, P and the recurrence output N (the new DPM)
+    float *sD = sm + (size_t)r * 5 * nkl + SLOT_PAD, *sQ = sD + nkl, *sT = sQ + nkl, *sP = sT + nkl, *sN = sP + nkl;
+    float *xbuf = sm + (size_t)ROWS * 5 * nkl + SLOT_PAD;      // [2][ROWS] slots of whole DPM columns (old values)
+    float *ring = sm + (size_t)ROWS * 7 * nkl + SLOT_PAD;      // [RING] QIMP row scratch slots
+    float *sdzk = sm + (size_t)(ROWS * 7 + RING) * nkl, *satp = sdzk + nkp;
+    int *sinfo = (int *)(satp + nkp);                         // [2][ROWS] per row: active, KK, VI, ring slot
+    int *sctl = sinfo + 8 * ROWS;
     float *sdump = (float *)(sctl + 4);                       // [WAVE][CH] sinks of the chains
     if (threadIdx.x == 0) sctl[0] = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int k = threadIdx.x; k < nkp; k += STRIP_WAVES * WAVE) { sdzk[k] = d.dzk[k]; satp[k] = d.atp[k]; }
@@ -1741,20 +1743,20 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
     if (chainw) {                                             // own loop, as in k_ml_strip
         for (int s = 0; s < nsteps; ++s) {
             lds_barrier();                                    // the rows' sources are assembled
+            const int *si = sinfo + (s & 1) * 4 * ROWS;
             const int cl = lane < ROWS ? lane : 0;
-            const bool on = lane < ROWS && sinfo[cl * 4];
-            const int KK2 = sinfo[cl * 4 + 1], VI2 = sinfo[cl * 4 + 2], rs2 = sinfo[cl * 4 + 3];
+            const bool on = lane < ROWS && si[cl * 4];
+            const int KK2 = si[cl * 4 + 1], VI2 = si[cl * 4 + 2], rs2 = si[cl * 4 + 3];
             const int nmax = wave_max(VI2 - KK2, on);
             if (on) {
-                float *Dd = sm + (size_t)cl * 4 * nkl + SLOT_PAD;
-                float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl;
+                float *Dd = sm + (size_t)cl * 5 * nkl + SLOT_PAD;
+                float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl, *N = P + nkl;
                 const float *O = xbuf + (size_t)((s & 1) * ROWS + cl) * nkl;
-                // Q[k] holds ATP[k] on entry; the new DPM replaces D
+                // Q[k] holds ATP[k] on entry; the new DPM goes to N
                 chain_pdown(Dd + VI2, T + VI2, Q + VI2, ring[(size_t)rs2 * nkl + VI2 + 1], VI2 - KK2, nmax, sdump + lane * CH);
-                chain_pup(P + KK2 + 1, Q + KK2 + 1, Dd + KK2 + 1, O[KK2], VI2 - KK2, nmax, sdump + lane * CH);
+                chain_pup(P + KK2 + 1, Q + KK2 + 1, N + KK2 + 1, O[KK2], VI2 - KK2, nmax, sdump + lane * CH);
             }
             lds_barrier();                                    // the recurrences are done
-            lds_barrier();                                    // the rows have written back
         }
         return;
     }
@@ -1807,6 +1809,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
                     pre[m2].p = ptab[o + k];
                     pre[m2].pa = d.dpm[c3(d, in_ + IS, jn_) + k];
                     pre[m2].pb = d.dpm[c3(d, in_, jn_ + JS) + k];
+                    pre[m2].dn = d.dpmnew[o + k];
                     if (l == 0) pre[m2].nj = d.dpm[c3(d, in_, jn_ - JS) + k];
                 }
             }
@@ -1836,6 +1839,38 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
         sw.wait(ROWS);
         locate_prev();
     }
+    // The results of a step are merged on use, as in k_ml_strip: level k of a
+    // row's previous column is the new DPM (sN) in KK+1..VI, else the old value.
+    // Each row finishes its previous column in the next step: the row scratch
+    // update (and the last row's hand-off to strip B+1) before the new sources
+    // replace sQ, the residual and the DPM/DPMNEW stores after the assembly's
+    // load waits (DPMNEW prefetched one step ahead).
+    int pkk = 0, pvi = -1, prs = 0, pj = 0;                  // own previous column (pvi < 0: none)
+    bool pinr = false;
+    size_t po = 0;
+    float dnp[2] = {0.0f, 0.0f};
+    auto finish_ring = [&]() {
+        float *qring = ring + (size_t)prs * nkl;
+        for (int k = pkk + 1 + lane; k <= pvi; k += WAVE) qring[k] = sQ[k];
+        if (r == ROWS - 1 && pinr) {                          // hand the row scratch to strip B+1
+            gf32 *qrow = d.prow_q + (size_t)pj * nkp;
+            for (int k = lane; k < nkp; k += WAVE) st_through(qrow + k, qring[k]);
+        }
+    };
+    auto finish_store = [&](const float *xold) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            int k = pkk + 1 + lane + WAVE * m;
+            if (k <= pvi) {
+                float temp = sN[k];
+                double qv = (double)conv_to_int((double)temp * d.frund) / d.frund;
+                float rr = (float)fabs((double)xold[k] - qv);
+                rmax = rr > rmax ? rr : rmax;
+                st_through(d.dpm + po + k, temp);
+                st_through(d.dpmnew + po + k, dnp[m] + d.relaxp * temp);
+            }
+        }
+    };
     for (int s = 0; s < nsteps; ++s) {
         const bool act = actn, inr = inr_n;
         const int i = in_, j = jn_, KK = KKn, VI = VIn;
@@ -1843,7 +1878,11 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
         float *xcur = xbuf + (size_t)((s & 1) * ROWS + r) * nkl;
         const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r) * nkl;
         const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r - 1) * nkl;
+        const float *xs_new = sN - (size_t)5 * nkl;          // row r-1's new DPM
+        int *si = sinfo + (s & 1) * 4 * ROWS;
+        const int *sp = sinfo + ((s - 1) & 1) * 4 * ROWS;
         const bool w_in = (s - r >= 1), s_in = (r >= 1);
+        finish_ring();
         if (inr) {
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
@@ -1855,13 +1894,19 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
             }
         }
         if (act) {
+            // updated levels of the neighbouring row's column (row r-1, previous step)
+            int slo = 1 << 30, shi = -1;
+            if (s_in && sp[(r - 1) * 4]) {
+                slo = sp[(r - 1) * 4 + 1] + 1;
+                shi = sp[(r - 1) * 4 + 2];
+            }
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
                 int k = KK + 1 + lane + WAVE * m;
                 if (k <= VI) {
                     const PrPre &r_ = pre[m];
-                    float newj = w_in ? xw_lds[k] : r_.nj;
-                    float newi = s_in ? xs_lds[k] : r_.ni;
+                    float newj = w_in ? ((k > pkk && k <= pvi) ? sN[k] : xw_lds[k]) : r_.nj;
+                    float newi = s_in ? ((k >= slo && k <= shi) ? xs_new[k] : xs_lds[k]) : r_.ni;
                     float dim_ = (IS == 1) ? newi : r_.pa, dip = (IS == 1) ? r_.pa : newi;
                     float djm = (JS == 1) ? newj : r_.pb, djp = (JS == 1) ? r_.pb : newj;
                     float dz = sdzk[k];
@@ -1871,9 +1916,12 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
                     sQ[k] = satp[k];                  // consumed and replaced by QIMP in phase 2
                 }
             }
-            if (lane == 0) { sinfo[r * 4 + 1] = KK; sinfo[r * 4 + 2] = VI; sinfo[r * 4 + 3] = rs; }
+            if (lane == 0) { si[r * 4 + 1] = KK; si[r * 4 + 2] = VI; si[r * 4 + 3] = rs; }
         }
-        if (lane == 0) sinfo[r * 4] = act;
+        if (lane == 0) si[r * 4] = act;
+        finish_store(xw_lds);
+        pkk = KK; pvi = act ? VI : -1; prs = rs; pj = j; pinr = inr; po = c3(d, i, j);
+        dnp[0] = pre[0].dn; dnp[1] = pre[1].dn;
         lds_barrier();
         // while the chain wave runs the recurrences: publish and prefetch as in
         // k_ml_strip; row 0 reads the previous strip's row scratch and last row
@@ -1884,28 +1932,10 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
             sw.wait(s + 1 + ROWS);
             locate_prev();
         }
-        lds_barrier();
-        if (act) {
-            size_t o = c3(d, i, j);
-            float *qring = ring + (size_t)rs * nkl;
-            for (int k = KK + 1 + lane; k <= VI; k += WAVE) {
-                float temp = sD[k];
-                double qv = (double)conv_to_int((double)temp * d.frund) / d.frund;
-                float rr = (float)fabs((double)xcur[k] - qv);
-                rmax = rr > rmax ? rr : rmax;
-                xcur[k] = temp;                             // column buffer now holds the new DPM
-                st_through(d.dpm + o + k, temp);
-                st_through(d.dpmnew + o + k, d.dpmnew[o + k] + d.relaxp * temp);
-                qring[k] = sQ[k];
-            }
-        }
-        if (r == ROWS - 1 && inr) {                           // hand the row scratch to strip B+1
-            const float *qring = ring + (size_t)rs * nkl;
-            gf32 *qrow = d.prow_q + (size_t)j * nkp;
-            for (int k = lane; k < nkp; k += WAVE) st_through(qrow + k, qring[k]);
-        }
-        lds_barrier();
+        lds_barrier();                                        // the recurrence outputs are ready
     }
+    finish_ring();
+    finish_store(xbuf + (size_t)(((nsteps - 1) & 1) * ROWS + r) * nkl);
     if (r == ROWS - 1) strip_publish(mine, nsteps, lane);
     for (int off = WAVE / 2; off > 0; off >>= 1) {
         float other = __shfl_xor(rmax, off);
@@ -2146,8 +2176,8 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
     if (d0.tiled == 2) {
         int nst = 0;
         for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].nii - 1 - 2 + ROWS) / ROWS);
-        size_t shm = sizeof(float) * ((size_t)(6 * ROWS + RING) * lds_slot_h(nkp) + 2 * nkp + 8 + WAVE * CH) +
-                     sizeof(int) * (5 * ROWS + 4);
+        size_t shm = sizeof(float) * ((size_t)(7 * ROWS + RING) * lds_slot_h(nkp) + 2 * nkp + 8 + WAVE * CH) +
+                     sizeof(int) * (9 * ROWS + 4);
         L.go(FAM_PRESSURE, [&] {
             hipLaunchKernelGGL(k_pressure_strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm, s, dd, IS, JS, slot);
         });
