@@ -1135,6 +1135,7 @@ __device__ __forceinline__ Meta unpack_meta(int4 m) {
     return r;
 }
 #define COEF_COLS 32
+#define SCOL 16           // words of column data per column in k_ml_coef
 // All neighbour-independent work of one mixing-length sweep for every column
 // at once.  A block of 4 waves takes COEF_COLS consecutive columns of the
 // sweep; their active levels are laid out one after another and dealt to the
@@ -1155,9 +1156,10 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
     const int lane = threadIdx.x % WAVE;
     const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2;
     const int na = d.nii - 1 - oa + 1, nb = d.njj - 1 - ia + 1;
-    // per column c: A, B, C over k at sm + c*3*nkl
-    int *scol = (int *)(sm + 3 * COEF_COLS * nkl);         // [COEF_COLS][8]: KSTART, KK, VI, i, j, first item, active
-    int *stot = scol + 8 * COEF_COLS;
+    // per column c: A, B, C over k at sm + c*3*nkl, and the column's data
+    // (read once per column here, not once per level)
+    int *scol = (int *)(sm + 3 * COEF_COLS * nkl);         // [COEF_COLS][SCOL]
+    int *stot = scol + SCOL * COEF_COLS;
     const int blk = xcd_block((na * nb + COEF_COLS - 1) / COEF_COLS);
     if (blk < 0) return;                                      // uniform over the block
     const int col0 = blk * COEF_COLS;
@@ -1172,9 +1174,16 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
                 sweep_to_ij(d, 0, oa + col / nb, ia + col % nb, IS, JS, i, j);
                 m = unpack_meta(ldg4i(d.meta + c2(d, i, j)));
             }
-            if (m.adv) n = m.VI - m.KSTART + 1;
-            scol[c * 8 + 0] = m.KSTART; scol[c * 8 + 1] = m.KK; scol[c * 8 + 2] = m.VI;
-            scol[c * 8 + 3] = i; scol[c * 8 + 4] = j; scol[c * 8 + 6] = m.adv ? 1 : 0;
+            if (m.adv) {
+                n = m.VI - m.KSTART + 1;
+                const ColInfo q = col_info(d, i, j);
+                int *sc = scol + c * SCOL;
+                sc[7] = (q.as ? 1 : 0) | (q.an ? 2 : 0) | (q.aw ? 4 : 0) | (q.ae ? 8 : 0) | (q.vegp ? 16 : 0);
+                sc[8] = __float_as_int(q.cutk); sc[9] = __float_as_int(q.cov); sc[10] = __float_as_int(q.ht);
+                sc[11] = __float_as_int(q.ho); sc[12] = __float_as_int(q.z0s);
+            }
+            scol[c * SCOL + 0] = m.KSTART; scol[c * SCOL + 1] = m.KK; scol[c * SCOL + 2] = m.VI;
+            scol[c * SCOL + 3] = i; scol[c * SCOL + 4] = j; scol[c * SCOL + 6] = m.adv ? 1 : 0;
         }
         // inclusive prefix sum of the level counts over the lanes
         int incl = n;
@@ -1182,7 +1191,7 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
             int v = __shfl_up(incl, off);
             if (lane >= off) incl += v;
         }
-        if (c < COEF_COLS) scol[c * 8 + 5] = incl - n;
+        if (c < COEF_COLS) scol[c * SCOL + 5] = incl - n;
         if (lane == WAVE - 1) stot[0] = incl;
     }
     lds_barrier();
@@ -1192,14 +1201,20 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
     auto column_of = [&](int t) {
         int lo = 0;
         for (int step = COEF_COLS / 2; step > 0; step >>= 1)
-            if (scol[(lo + step) * 8 + 5] <= t) lo += step;
+            if (scol[(lo + step) * SCOL + 5] <= t) lo += step;
         return lo;
     };
     for (int t = threadIdx.x; t < total; t += 4 * WAVE) {
         const int c = column_of(t);
-        const int k = scol[c * 8 + 0] + (t - scol[c * 8 + 5]);
-        ColInfo q = col_info(d, scol[c * 8 + 3], scol[c * 8 + 4]);
-        if (WFIX) q.W = WFIX;
+        const int *sc = scol + c * SCOL;
+        const int k = sc[0] + (t - sc[5]);
+        ColInfo q;
+        q.KSTART = sc[0]; q.KK = sc[1]; q.VI = sc[2]; q.i = sc[3]; q.j = sc[4];
+        const int fl = sc[7];
+        q.as = fl & 1; q.an = (fl >> 1) & 1; q.aw = (fl >> 2) & 1; q.ae = (fl >> 3) & 1; q.vegp = (fl >> 4) & 1;
+        q.cutk = __int_as_float(sc[8]); q.cov = __int_as_float(sc[9]); q.ht = __int_as_float(sc[10]);
+        q.ho = __int_as_float(sc[11]); q.z0s = __int_as_float(sc[12]);
+        q.W = WFIX ? WFIX : d.simd;
         MlPieces p = ml_pieces<COMP>(d, q, k);
         float *sA = sm + (size_t)c * 3 * nkl;
         sA[k] = p.A; sA[nkl + k] = p.B; sA[2 * nkl + k] = p.C;
@@ -1214,12 +1229,12 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
         }
     }
     lds_barrier();
-    if (threadIdx.x < COEF_COLS && scol[threadIdx.x * 8 + 6]) {
+    if (threadIdx.x < COEF_COLS && scol[threadIdx.x * SCOL + 6]) {
         const int c = threadIdx.x;
         float *__restrict__ A = sm + (size_t)c * 3 * nkl;
         float *__restrict__ Bq = A + nkl;
         float *__restrict__ Cq = Bq + nkl;
-        const int KSTART = scol[c * 8 + 0], KKP1 = scol[c * 8 + 1] + 1, VI = scol[c * 8 + 2];
+        const int KSTART = scol[c * SCOL + 0], KKP1 = scol[c * SCOL + 1] + 1, VI = scol[c * SCOL + 2];
         float Pp = 0;
         for (int k = KSTART; k <= VI; ++k) {
             float T = (k > KKP1) ? 1 / (A[k] - Cq[k] * Pp) : 1 / A[k];
@@ -1232,9 +1247,9 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
     lds_barrier();
     for (int t = threadIdx.x; t < total; t += 4 * WAVE) {
         const int c = column_of(t);
-        const int k = scol[c * 8 + 0] + (t - scol[c * 8 + 5]);
+        const int k = scol[c * SCOL + 0] + (t - scol[c * SCOL + 5]);
         const float *sA = sm + (size_t)c * 3 * nkl;
-        gf32 *P = d.rec3 + (c3(d, scol[c * 8 + 3], scol[c * 8 + 4]) + k) * 3;
+        gf32 *P = d.rec3 + (c3(d, scol[c * SCOL + 3], scol[c * SCOL + 4]) + k) * 3;
         stg3(P, sA[2 * nkl + k], sA[k], sA[nkl + k]);                   // C, T, P
     }
 }
@@ -2213,7 +2228,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
             nblk = max(nblk, ((na * nb + COEF_COLS - 1) / COEF_COLS + NXCD - 1) / NXCD * NXCD);
             nst = max(nst, (ib - ia + ROWS) / ROWS);
         }
-        size_t shm_c = sizeof(float) * 3 * (nkp | 1) * COEF_COLS + sizeof(int) * (8 * COEF_COLS + 4);
+        size_t shm_c = sizeof(float) * 3 * (nkp | 1) * COEF_COLS + sizeof(int) * (SCOL * COEF_COLS + 4);
         L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
         size_t shm_t = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) + sizeof(int) * (10 * ROWS + 4);
         L.go(fam, [&] { hipLaunchKernelGGL(strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm_t, s, dd, is, js); });
