@@ -32,6 +32,53 @@ ALG_BYTES_ITER = 188          # whole SIMPLE iteration, mixing-length model
 HBM_PEAK_GBS = 8000.0
 
 
+def situations_per_hour(n, directions):
+    """Flow situations per hour, batched against plain, at equal residual.
+
+    Every situation is solved to the reference's convergence test with its
+    default limits (100 to 500 iterations, stop test every 100), once as one
+    lock-step batch and once one after another.  The batch is bit-identical
+    to the plain solves, so both stop at the same iterations with the same
+    residuals (checked).  Wall time covers what a caller waits for: handle
+    creation and upload, the host prologue, the solve and the download of the
+    fields."""
+    from gral_amd import domain, pf
+
+    def cases():
+        cs = [domain.config2(direction=d, n=n) for d in directions]
+        for c in cs:
+            c.iter_min, c.iter_max = 100, 500
+        return cs
+
+    cs = cases()
+    t0 = time.perf_counter()
+    solvers = [pf.PrognosticFlowfield(c) for c in cs]
+    for sv in solvers:
+        sv.prepare()
+    with pf.Batch(solvers) as b:
+        st_b = b.solve()
+    for sv in solvers:
+        sv.download()
+        sv.close()
+    t_batch = time.perf_counter() - t0
+
+    cs = cases()
+    t0 = time.perf_counter()
+    st_p = []
+    for c in cs:
+        with pf.PrognosticFlowfield(c) as sv:
+            st_p.append(sv.solve())
+            sv.download()
+    t_plain = time.perf_counter() - t0
+    its_b = [s["iteration_loops"] for s in st_b]
+    its_p = [s["iteration_loops"] for s in st_p]
+    same = its_b == its_p and [s["delta_finish"] for s in st_b] == [s["delta_finish"] for s in st_p]
+    return {"situations": len(cs), "batched_per_hour": len(cs) / t_batch * 3600.0,
+            "plain_per_hour": len(cs) / t_plain * 3600.0, "batched_over_plain": t_plain / t_batch,
+            "batched_s": t_batch, "plain_s": t_plain, "iterations": its_b, "equal_residual": same,
+            "limits": "IterationStepsMin 100, IterationStepsMax 500, stop test every 100 iterations"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,6 +91,8 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=2)
     ap.add_argument("--no-family-profile", action="store_true",
                     help="skip the per-launch event timing (for counter-collection runs)")
+    ap.add_argument("--situations-hour", action="store_true",
+                    help="also solve every situation to convergence, batched and plain (about 1-2 minutes)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     args = ap.parse_args()
@@ -146,6 +195,8 @@ def main():
             "iteration_frac_of_hbm_roofline": value * ALG_BYTES_ITER / (HBM_PEAK_GBS * 1e9),
             "roofline": roofline, "cpu_baseline": cpu, "plain_single_situation": plain,
             "kernel_families_ms_per_iter": {k: v["ms"] / 2 for k, v in prof.items()}}
+    if args.situations_hour:
+        line["situations_per_hour"] = situations_per_hour(args.n, directions)
     print(json.dumps(line))
 
 
