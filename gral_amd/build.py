@@ -10,7 +10,9 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libgral_pf.so")
-SOURCES = [os.path.join(HERE, "csrc", "pf_kernels.hip"), os.path.join(HERE, "csrc", "pf_api.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", "pf_kernels.hip"), os.path.join(HERE, "csrc", "pf_api.cpp"),
+           os.path.join(HERE, "csrc", "pf_zip.cpp")]
+LIBS = ["-lz"]  # zlib: deflate of the flow-field files' zip container
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument"]
@@ -23,7 +25,7 @@ def hipcc():
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [hipcc()] + FLAGS + ["-x", "hip"] + SOURCES + ["-o", LIB + ".tmp"]
+    cmd = [hipcc()] + FLAGS + ["-x", "hip"] + SOURCES + LIBS + ["-o", LIB + ".tmp"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

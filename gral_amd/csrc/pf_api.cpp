@@ -38,6 +38,7 @@ struct DiagArgs {
     float *uk, *vk, *wk, *dpm, *div;
 };
 extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStream_t s);
+extern "C" int pf_launch_gff(const GffArgs *ga, hipStream_t s);
 
 namespace {
 
@@ -88,6 +89,12 @@ struct gral_pf_handle {
     std::vector<void *> dev;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // flow-field file packer: device copy of the host levels [gff_nlo, nkh) of
+    // UK, VK, WK, TURB (the levels the solver does not hold on the device)
+    void *gff_up = nullptr;
+    size_t gff_bytes = 0;
+    int gff_nlo = -1;
+    bool gff_ok = false;
     std::string err;
 
     size_t n2() const { return (size_t)(nii + 2) * (size_t)(njj + 2); }
@@ -196,6 +203,7 @@ extern "C" int gral_pf_upload_fields(gral_pf_handle *h, const float *uk, const f
     h->wk.assign(wk, wk + n3);
     h->fields = true;
     h->prepared = false;
+    h->gff_ok = false;
     return GRAL_PF_OK;
 }
 
@@ -534,6 +542,7 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
             if (h->adv[c] == 1) h->active += h->vi[c] - h->kk[c];
         }
     h->prepared = true;
+    h->gff_ok = false;
     h->done = false;
     h->host_stale = false;
     h->iter_loops = 1;
@@ -987,12 +996,146 @@ extern "C" int gral_pf_diagnostic(gral_pf_handle *h, int32_t flow_field_level, i
     cleanup();
     h->diag_dtime = dt;
     h->host_stale = false;
+    h->gff_ok = false;
     if (h->prepared) {
         // keep the solver's device state (levels below nkp) equal to the host's
         if ((rc = put_field(h, h->d.uk, h->uk)) || (rc = put_field(h, h->d.vk, h->vk)) || (rc = put_field(h, h->d.wk, h->wk)) ||
             (rc = put_field(h, h->d.dpm, h->dpm)))
             return rc;
     }
+    return GRAL_PF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Flow-field files, GFFWrite.cs: the int16 payload is packed on the device
+// from the solver's resident levels (no download of the fields) and the host
+// copy of the levels above them; the host adds the 32-byte header.
+// ---------------------------------------------------------------------------
+static size_t gff_columns(const gral_pf_handle *h, int kind) {
+    return kind == 3 ? (size_t)h->nii * h->njj : (size_t)(h->nii + 1) * (size_t)(h->njj + 1);
+}
+
+static size_t gff_size(const gral_pf_handle *h, int kind) {
+    const size_t nkk = (size_t)h->nkk;
+    if (kind == 3) return 32 + gff_columns(h, 3) * nkk * 2;
+    if (kind == 0 || kind == 1) return 32 + gff_columns(h, kind) * (nkk + 1) * 6;
+    size_t n = 32 + (size_t)h->nii * h->njj * 2;
+    for (int i = 1; i <= h->nii + 1; ++i)
+        for (int j = 1; j <= h->njj + 1; ++j) n += 6 * (size_t)(h->nkk + 2 - std::max<int>(h->kk[h->i2(i, j)], 1));
+    return n;
+}
+
+// device copy of the host's levels [nlo, nkh) of UK, VK, WK, TURB
+static int gff_upper(gral_pf_handle *h, int nlo) {
+    const int nup = h->nkh - nlo;
+    if (h->gff_ok && h->gff_nlo == nlo) return 0;
+    if (nup <= 0) { h->gff_ok = true; h->gff_nlo = nlo; return 0; }
+    const size_t n2 = h->n2(), per = n2 * (size_t)nup, bytes = 4 * per * sizeof(float);
+    if (bytes > h->gff_bytes) {
+        if (h->gff_up) (void)hipFree(h->gff_up);
+        h->gff_up = nullptr;
+        h->gff_bytes = 0;
+        if (hipMalloc(&h->gff_up, bytes) != hipSuccess) { h->gff_up = nullptr; return fail(h, GRAL_PF_ENOMEM, "gff: hipMalloc failed"); }
+        h->gff_bytes = bytes;
+    }
+    std::vector<float> tmp(4 * per, 0.0f);
+    const std::vector<float> *src[4] = {&h->uk, &h->vk, &h->wk, &h->turb};
+    const size_t n3 = n2 * (size_t)h->nkh;
+    for (int f = 0; f < 4; ++f) {
+        if (src[f]->size() != n3) continue;
+        for (size_t c = 0; c < n2; ++c)
+            std::memcpy(&tmp[f * per + c * nup], src[f]->data() + c * h->nkh + nlo, sizeof(float) * nup);
+    }
+    HIPCHK(h, hipMemcpyAsync(h->gff_up, tmp.data(), bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->gff_ok = true;
+    h->gff_nlo = nlo;
+    return 0;
+}
+
+extern "C" int gral_pf_gff_payload(gral_pf_handle *h, int32_t kind, const gral_pf_gff_meta *meta, void *out,
+                                   size_t cap, size_t *len) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (kind < 0 || kind > 3) return fail(h, GRAL_PF_EINVAL, "gff: kind must be 0, 1, 2 (writing modes) or 3 (tke)");
+    if (!meta || !len) return fail(h, GRAL_PF_EINVAL, "gff: null meta or len");
+    if (!h->masks || !h->fields) return fail(h, GRAL_PF_ESTATE, "gff: masks and fields must be set first");
+    if (kind == 3 && !(h->prepared && h->prm.turbulence_model == 2))
+        return fail(h, GRAL_PF_ESTATE, "gff: the tke file needs a prepared k-eps (turbulence_model 2) situation");
+    const size_t total = gff_size(h, kind);
+    *len = total;
+    if (!out) return GRAL_PF_OK;
+    if (cap < total) return fail(h, GRAL_PF_EINVAL, "gff: output buffer too small");
+    const int nlo = h->prepared ? std::min(h->nkp, h->nkh) : 0;
+    int rc = gff_upper(h, nlo);
+    if (rc) return rc;
+    // header: NKK (negative for modes 1 and 2), NJJ, NII, direction, speed, class, DXK, format flag
+    uint8_t *o8 = static_cast<uint8_t *>(out);
+    const int32_t hn[3] = {(kind == 1 || kind == 2) ? -h->nkk : h->nkk, h->njj, h->nii};
+    const int32_t flag = (kind == 1) ? -2 : (kind == 2) ? -3 : -1;
+    std::memcpy(o8, hn, 12);
+    std::memcpy(o8 + 12, &meta->wind_dir, 4);
+    std::memcpy(o8 + 16, &meta->wind_vel, 4);
+    std::memcpy(o8 + 20, &meta->stab_class, 4);
+    std::memcpy(o8 + 24, &h->dxk, 4);
+    std::memcpy(o8 + 28, &flag, 4);
+    // device buffers: payload body, KKART, kind-2 column offsets
+    const size_t body = total - 32, ncol = gff_columns(h, kind), n2 = h->n2();
+    const size_t o_kk = (body + 255) & ~size_t(255), o_off = (o_kk + n2 * 2 + 255) & ~size_t(255);
+    const size_t bytes = o_off + (kind == 2 ? ncol * sizeof(long long) : 0);
+    void *dbuf = nullptr;
+    if (hipMalloc(&dbuf, bytes + 256) != hipSuccess) return fail(h, GRAL_PF_ENOMEM, "gff: hipMalloc failed");
+    auto bail = [&](int code, const char *msg) { (void)hipFree(dbuf); return fail(h, code, msg); };
+    char *db = static_cast<char *>(dbuf);
+    std::vector<long long> off;
+    if (kind == 2) {
+        off.resize(ncol);
+        long long a = 0;
+        size_t q = 0;
+        for (int i = 1; i <= h->nii + 1; ++i)
+            for (int j = 1; j <= h->njj + 1; ++j) {
+                off[q++] = a;
+                a += 3LL * (h->nkk + 2 - std::max<int>(h->kk[h->i2(i, j)], 1));
+            }
+        if (hipMemcpyAsync(db + o_off, off.data(), ncol * sizeof(long long), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+            return bail(GRAL_PF_EDEVICE, "gff: upload failed");
+    }
+    if (hipMemcpyAsync(db + o_kk, h->kk.data(), n2 * 2, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        return bail(GRAL_PF_EDEVICE, "gff: upload failed");
+    GffArgs g{};
+    const float *lo[4] = {h->d.uk, h->d.vk, h->d.wk, h->d.turb};
+    const int nup = h->nkh - nlo;
+    for (int f = 0; f < 4; ++f) {
+        g.lo[f] = nlo > 0 ? lo[f] : nullptr;
+        g.up[f] = nup > 0 ? static_cast<const float *>(h->gff_up) + (size_t)f * n2 * nup : nullptr;
+    }
+    g.nlo = nlo; g.nkp = h->nkp; g.nup = nup;
+    g.nii = h->nii; g.njj = h->njj; g.nkk = h->nkk; g.nj2 = h->njj + 2;
+    g.kind = kind;
+    g.kk = reinterpret_cast<const int16_t *>(db + o_kk);
+    g.off = kind == 2 ? reinterpret_cast<const long long *>(db + o_off) : nullptr;
+    g.out = reinterpret_cast<int16_t *>(db);
+    if (pf_launch_gff(&g, h->stream)) return bail(GRAL_PF_EDEVICE, "gff: kernel launch failed");
+    if (hipMemcpyAsync(o8 + 32, db, body, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return bail(GRAL_PF_EDEVICE, "gff: download failed");
+    (void)hipFree(dbuf);
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_gff_write(gral_pf_handle *h, int32_t kind, const gral_pf_gff_meta *meta, const char *path) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!path) return fail(h, GRAL_PF_EINVAL, "gff: null path");
+    size_t len = 0;
+    int rc = gral_pf_gff_payload(h, kind, meta, nullptr, 0, &len);
+    if (rc) return rc;
+    if (len >= 0xffffffffull) return fail(h, GRAL_PF_EINVAL, "gff: payload of 4 GiB or more (zip without ZIP64)");
+    std::vector<uint8_t> buf(len);
+    if ((rc = gral_pf_gff_payload(h, kind, meta, buf.data(), len, &len))) return rc;
+    // the zip entry carries the file's own name (GFFWrite.cs:109, :177)
+    std::string entry(path);
+    const size_t slash = entry.find_last_of("/\\");
+    if (slash != std::string::npos) entry = entry.substr(slash + 1);
+    if ((rc = gral_pf_zip_write(path, entry.c_str(), buf.data(), len))) return fail(h, rc, "gff: writing the zip file failed");
     return GRAL_PF_OK;
 }
 
@@ -1010,6 +1153,7 @@ extern "C" const char *gral_pf_last_error(const gral_pf_handle *h) {
 extern "C" void gral_pf_destroy(gral_pf_handle *h) {
     if (!h) return;
     free_device(h);
+    if (h->gff_up) (void)hipFree(h->gff_up);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -87,6 +87,20 @@ __device__ __forceinline__ int4 ldg4i(const T *p) {
     return make_int4(v.x, v.y, v.z, v.w);
 }
 
+// Flow-field file payload packer (GFFWrite.cs): fields u, v, w, turb, levels
+// below nlo from the solver's device fields (stride nkp), the rest from a copy
+// of the host's upper levels (stride nup, level nlo first).
+struct GffArgs {
+    const float *lo[4];
+    const float *up[4];
+    int nlo, nkp, nup;
+    int nii, njj, nkk, nj2;
+    int kind;                         // 0, 1, 2: *.gff writing modes; 3: *.tke
+    const int16_t *kk;                // KKART [nii+2][njj+2]
+    const long long *off;             // kind 2: int16 offset of each column record after the KKART block
+    int16_t *out;                     // the payload after its 32-byte header
+};
+
 #define SYNC_BLOCK0 64               // Dom::sync: sticky error flag at 0, sweep blocks from here
 #define STRIP_ROWS 15                // sweep rows per strip workgroup (plus one chain wave)
 

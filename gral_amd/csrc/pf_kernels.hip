@@ -2272,3 +2272,92 @@ extern "C" int pf_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
     return (int)hipGetLastError();
 }
+
+// ------------------------------------------------------------------------
+// Flow-field file payloads, GFFWrite.cs: int16 of value*100 (FP32 product)
+// kind 0 (:167-225): column records (u, v, w per level), clamp then truncate;
+// kind 1 (:256-360): level-major rows u(j), v(j), w(j), clamp then truncate;
+// kind 2 (:388-510): KKART, then per column u, v, w from max(KKART,1) up,
+//                    MathF.Round (half to even) then clamp;
+// kind 3 (:79-133, *.tke): TURB, clamp then Convert.ToInt16 (half to even).
+// u and v of file level k are UK/VK[k-1], w is WK[k].  A NaN packs to 0.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ float gff_at(const GffArgs &g, int f, size_t col, int k) {
+    return k < g.nlo ? g.lo[f][col * (size_t)g.nkp + k] : g.up[f][col * (size_t)g.nup + (k - g.nlo)];
+}
+__device__ __forceinline__ int16_t gff_trunc(float v) {
+    if (v != v) return 0;                                     // NaN
+    if (v > 32767.0f) v = 32767.0f;
+    else if (v < -32768.0f) v = -32768.0f;
+    return (int16_t)(int)v;                                   // truncation toward zero
+}
+__device__ __forceinline__ int16_t gff_round(float v) {
+    if (v != v) return 0;
+    v = rintf(v);                                             // half to even
+    if (v > 32767.0f) v = 32767.0f;
+    else if (v < -32768.0f) v = -32768.0f;
+    return (int16_t)(int)v;
+}
+
+// kinds 0 and 3: one block per column (a, b), threads over levels
+__global__ void k_gff_column(GffArgs g) {
+    const int nb = (g.kind == 3) ? g.njj : g.njj + 1;
+    const int a = blockIdx.x / nb, b = blockIdx.x % nb;
+    const size_t col = (size_t)(a + 1) * g.nj2 + (b + 1);
+    if (g.kind == 3) {
+        int16_t *o = g.out + ((size_t)a * g.njj + b) * g.nkk;
+        for (int k = 1 + threadIdx.x; k <= g.nkk; k += blockDim.x) o[k - 1] = gff_round(gff_at(g, 3, col, k) * 100.0f);
+        return;
+    }
+    int16_t *o = g.out + ((size_t)a * (g.njj + 1) + b) * (size_t)(g.nkk + 1) * 3;
+    for (int kq = threadIdx.x; kq <= g.nkk; kq += blockDim.x) {
+        o[3 * kq] = gff_trunc(gff_at(g, 0, col, kq) * 100.0f);
+        o[3 * kq + 1] = gff_trunc(gff_at(g, 1, col, kq) * 100.0f);
+        o[3 * kq + 2] = gff_trunc(gff_at(g, 2, col, kq + 1) * 100.0f);
+    }
+}
+
+// kind 1: block (kq, a), threads over b; rows u(b), v(b), w(b)
+__global__ void k_gff_rows(GffArgs g) {
+    const int kq = blockIdx.y, a = blockIdx.x;
+    const int n = g.njj + 1;
+    int16_t *o = g.out + ((size_t)kq * (g.nii + 1) + a) * 3 * (size_t)n;
+    for (int b = threadIdx.x; b < n; b += blockDim.x) {
+        const size_t col = (size_t)(a + 1) * g.nj2 + (b + 1);
+        o[b] = gff_trunc(gff_at(g, 0, col, kq) * 100.0f);
+        o[n + b] = gff_trunc(gff_at(g, 1, col, kq) * 100.0f);
+        o[2 * n + b] = gff_trunc(gff_at(g, 2, col, kq + 1) * 100.0f);
+    }
+}
+
+// kind 2: block per column (a, b) of the record part; block 0 also writes KKART
+__global__ void k_gff_terrain(GffArgs g) {
+    const int a = blockIdx.x / (g.njj + 1), b = blockIdx.x % (g.njj + 1);
+    if (blockIdx.x == 0)
+        for (size_t t = threadIdx.x; t < (size_t)g.nii * g.njj; t += blockDim.x) {
+            const int ia = (int)(t / g.njj), jb = (int)(t % g.njj);
+            g.out[t] = g.kk[(size_t)(ia + 1) * g.nj2 + (jb + 1)];
+        }
+    const size_t col = (size_t)(a + 1) * g.nj2 + (b + 1);
+    const int k0 = max((int)g.kk[col], 1);
+    const int n = g.nkk + 2 - k0;
+    int16_t *o = g.out + (size_t)g.nii * g.njj + g.off[blockIdx.x];
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        o[t] = gff_round(gff_at(g, 0, col, k0 - 1 + t) * 100.0f);
+        o[n + t] = gff_round(gff_at(g, 1, col, k0 - 1 + t) * 100.0f);
+        o[2 * n + t] = gff_round(gff_at(g, 2, col, k0 + t) * 100.0f);
+    }
+}
+
+extern "C" int pf_launch_gff(const GffArgs *ga, hipStream_t s) {
+    const GffArgs g = *ga;
+    if (g.kind == 0)
+        hipLaunchKernelGGL(k_gff_column, dim3((unsigned)((g.nii + 1) * (g.njj + 1))), dim3(256), 0, s, g);
+    else if (g.kind == 3)
+        hipLaunchKernelGGL(k_gff_column, dim3((unsigned)(g.nii * g.njj)), dim3(256), 0, s, g);
+    else if (g.kind == 1)
+        hipLaunchKernelGGL(k_gff_rows, dim3((unsigned)(g.nii + 1), (unsigned)(g.nkk + 1)), dim3(256), 0, s, g);
+    else
+        hipLaunchKernelGGL(k_gff_terrain, dim3((unsigned)((g.nii + 1) * (g.njj + 1))), dim3(256), 0, s, g);
+    return (int)hipGetLastError();
+}

@@ -26,6 +26,7 @@ EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral
            "gral_pf_diagnostic",
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
+           "gral_pf_gff_payload", "gral_pf_gff_write", "gral_pf_zip_write",
            "gral_pf_last_error", "gral_pf_destroy", "gral_pf_batch_last_error", "gral_pf_batch_destroy"]
 FAMILIES = ("stencil", "pressure", "u", "v", "w", "tke", "coef")
 
@@ -62,6 +63,14 @@ class FamilyTime(C.Structure):
 _lib = None
 
 
+class GffMeta(C.Structure):
+    _fields_ = [("wind_dir", C.c_float), ("wind_vel", C.c_float), ("stab_class", C.c_int32)]
+
+
+# flow-field file kinds: GRAL_FlowFields.txt writing modes 0, 1, 2 and the *.tke file
+GFF_KINDS = {"mode0": 0, "mode1": 1, "mode2": 2, "tke": 3}
+
+
 def load(path: str = LIB_PATH):
     """Load the HIP library; raises if it has not been built."""
     global _lib
@@ -88,6 +97,10 @@ def load(path: str = LIB_PATH):
     lib.gral_pf_batch_solve.argtypes = [H, C.POINTER(Stats)]
     lib.gral_pf_batch_profile_iterations.argtypes = [H, C.c_int32, C.POINTER(FamilyTime), C.c_int32]
     lib.gral_pf_batch_last_timing.argtypes = [H, C.POINTER(Timing)]
+    lib.gral_pf_gff_payload.argtypes = [H, C.c_int32, C.POINTER(GffMeta), C.c_void_p, C.c_size_t,
+                                        C.POINTER(C.c_size_t)]
+    lib.gral_pf_gff_write.argtypes = [H, C.c_int32, C.POINTER(GffMeta), C.c_char_p]
+    lib.gral_pf_zip_write.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_size_t]
     for n in ("gral_pf_last_error", "gral_pf_batch_last_error"):
         getattr(lib, n).argtypes = [H]
         getattr(lib, n).restype = C.c_char_p
@@ -191,6 +204,23 @@ class PrognosticFlowfield:
         """The mass-conservation post-pass (DiagnosticFlowfield + final W step)."""
         self._check(self.lib.gral_pf_diagnostic(self.h, int(flow_field_level), int(bool(terrain))))
 
+    def gff_payload(self, kind, wind_dir=270.0, wind_vel=3.0, stab_class=4):
+        """The bytes GFFWrite.cs puts into a flow-field file's zip entry (kind 0,
+        1, 2: writing modes of GRAL_FlowFields.txt; 3: the *.tke file),
+        packed on the GPU."""
+        meta = GffMeta(float(wind_dir), float(wind_vel), int(stab_class))
+        n = C.c_size_t(0)
+        self._check(self.lib.gral_pf_gff_payload(self.h, int(kind), C.byref(meta), None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self._check(self.lib.gral_pf_gff_payload(self.h, int(kind), C.byref(meta), buf, n.value, C.byref(n)))
+        return buf.raw[:n.value]
+
+    def write_gff(self, path, kind, wind_dir=270.0, wind_vel=3.0, stab_class=4):
+        """GFFWrite.Write() for one file: payload packed on the GPU, written as
+        a zip archive with one entry named like the file."""
+        meta = GffMeta(float(wind_dir), float(wind_vel), int(stab_class))
+        self._check(self.lib.gral_pf_gff_write(self.h, int(kind), C.byref(meta), os.fsencode(path)))
+
     def download(self):
         shp = self.case.uk.shape
         names = ("uk", "vk", "wk", "uks", "vks", "wks", "dpm", "dpmnew", "turb", "tdiss")
@@ -273,6 +303,14 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def zip_write(path, entry, data: bytes):
+    """One-entry deflated zip archive (the container of the flow-field files)."""
+    lib = load()
+    rc = lib.gral_pf_zip_write(os.fsencode(path), entry.encode(), data, len(data))
+    if rc != 0:
+        raise RuntimeError(f"gral_pf_zip_write failed ({rc})")
 
 
 def _stats(st):

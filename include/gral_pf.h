@@ -21,6 +21,7 @@
  */
 #ifndef GRAL_PF_H
 #define GRAL_PF_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -159,6 +160,49 @@ typedef struct {
     int64_t cells;
 } gral_pf_family_time;
 int gral_pf_profile_iterations(gral_pf_handle *h, int32_t n, gral_pf_family_time *out, int32_t nfam);
+
+/* ---- Flow-field files (GFFWrite.cs) -----------------------------------
+ * GFFWrite.Write() stores each situation's wind field as a zip archive
+ * holding one entry of int16 values x100.  kind selects the format:
+ *   0  writing mode 0 (WriteCompressedGFFFiles, GFFWrite.cs:167-250):
+ *      flag -1, per column (i 1..NII+1, j 1..NJJ+1) u, v, w of each level
+ *      k 1..NKK+1, clamped then truncated;
+ *   1  writing mode 1 (WriteStrongCompressedGFFFiles, :256-382): flag -2,
+ *      header NKK negated, per level, per row i: u(j), then v(j), then w(j);
+ *   2  writing mode 2 (WriteTerrainCompressedGFFFiles, :388-537): flag -3,
+ *      header NKK negated, KKART of the inner columns, then per column u, v,
+ *      w from level max(KKART,1) up, each MathF.Round-ed (half to even);
+ *   3  the *.tke file of the k-eps model (GFFWrite.cs:79-160): flag -1,
+ *      TURB of levels 1..NKK per inner column, rounded half to even.
+ * u and v of file level k are UK/VK[k-1], w is WK[k] (GFFRead.cs:181-321
+ * reads them back).  The 32-byte header is NKK, NJJ, NII (int32), the wind
+ * direction and speed (float), the stability class (int32), DXK (float) and
+ * the flag (int32).  The payload is packed on the GPU from the fields the
+ * handle holds: after prepare, the solver's device-resident levels (no
+ * download of the fields) and the host levels above them; before prepare
+ * (upload_fields, diagnostic), the host fields. */
+typedef struct {
+    float wind_dir;             /* Program.WindDirGral */
+    float wind_vel;             /* Program.WindVelGral */
+    int32_t stab_class;         /* Program.StabClass */
+} gral_pf_gff_meta;
+
+/* The bytes of the zip entry (header + int16 body).  out = NULL: only *len
+ * (the payload size) is set.  Otherwise cap must be >= that size.  kind 3
+ * needs a prepared situation with turbulence_model 2, as GFFWrite.cs:81
+ * writes the *.tke file for the k-eps model only. */
+int gral_pf_gff_payload(gral_pf_handle *h, int32_t kind, const gral_pf_gff_meta *meta, void *out,
+                        size_t cap, size_t *len);
+
+/* GFFWrite.Write() for one file: packs the payload and writes it to path as a
+ * zip archive with one deflated entry named like the file (GFFWrite.cs:109,
+ * :177).  A file that cannot be written completely is removed (:238-246).
+ * No ZIP64: payloads of 4 GiB or more are refused. */
+int gral_pf_gff_write(gral_pf_handle *h, int32_t kind, const gral_pf_gff_meta *meta, const char *path);
+
+/* The zip container on its own: one deflated entry holding len bytes of
+ * data.  Deflate runs on host threads in independent 4 MiB chunks. */
+int gral_pf_zip_write(const char *path, const char *entry, const void *data, size_t len);
 
 const char *gral_pf_last_error(const gral_pf_handle *h);
 void gral_pf_destroy(gral_pf_handle *h);
