@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -79,6 +81,52 @@ def situations_per_hour(n, directions):
             "limits": "IterationStepsMin 100, IterationStepsMax 500, stop test every 100 iterations"}
 
 
+def gff_timing(sv, reps=5):
+    """Flow-field file writer on a prepared situation whose fields are on the
+    device: payload packed on the GPU and copied to the host (per kind), the
+    zip file written (writing mode 2, parallel deflate), and beside it the
+    oracle's numpy payload plus single-thread zlib deflate of the same data."""
+    import tempfile
+    import zipfile
+    meta = dict(wind_dir=270.0, wind_vel=3.0, stab_class=4)
+    out = {"kinds": {}}
+    sv.gff_payload(2, **meta)                 # device copy of the upper levels, once per situation
+    buf = np.empty(max(sv.gff_size(k) for k in (0, 1, 2)), dtype=np.uint8)
+    buf[:] = 0                                # pages touched before timing
+    for kind in (0, 1, 2):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            n = sv.gff_payload_into(kind, buf, **meta)
+            ts.append(time.perf_counter() - t)
+        ms = sorted(ts)[reps // 2] * 1e3
+        out["kinds"][str(kind)] = {"payload_bytes": n, "pack_and_copy_ms": ms, "gb_per_s": n / ms / 1e6}
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "00001.gff")
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            sv.write_gff(p, 2, **meta)
+            ts.append(time.perf_counter() - t)
+        out["write_mode2_ms"] = sorted(ts)[1] * 1e3
+        out["zip_bytes_mode2"] = os.path.getsize(p)
+        with zipfile.ZipFile(p) as z:
+            assert z.read("00001.gff") == sv.gff_payload(2, **meta)
+    from oracle import gff_oracle
+    f = sv.download()
+    c = sv.case
+    t = time.perf_counter()
+    ref = gff_oracle.payload(2, f["uk"], f["vk"], f["wk"], c.kkart, c.nii, c.njj, c.nkk, c.dxk, **meta)
+    t_pack = time.perf_counter() - t
+    t = time.perf_counter()
+    zlib_len = len(__import__("zlib").compress(ref, 6))
+    t_zip = time.perf_counter() - t
+    out["cpu_oracle"] = {"pack_ms": t_pack * 1e3, "deflate_ms": t_zip * 1e3, "deflated_bytes": zlib_len, "cores": 1,
+                         "sample": "writing mode 2 of the same situation: numpy payload oracle, zlib level 6"}
+    out["equal_to_oracle"] = ref == sv.gff_payload(2, **meta)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +141,8 @@ def main():
                     help="skip the per-launch event timing (for counter-collection runs)")
     ap.add_argument("--situations-hour", action="store_true",
                     help="also solve every situation to convergence, batched and plain (about 1-2 minutes)")
+    ap.add_argument("--gff", action="store_true",
+                    help="also time the flow-field file writer (GPU pack + zip) on the first situation")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     args = ap.parse_args()
@@ -137,6 +187,7 @@ def main():
     # instrumented iterations (events bracket single launches on the batch's stream)
     prof = batch.profile_families(2)
     batch.close()
+    gff = gff_timing(solvers[0]) if args.gff else None
     for sv in solvers:
         sv.close()
     sweeps = {k: v for k, v in prof.items() if k in ALG_BYTES}
@@ -195,6 +246,8 @@ def main():
             "iteration_frac_of_hbm_roofline": value * ALG_BYTES_ITER / (HBM_PEAK_GBS * 1e9),
             "roofline": roofline, "cpu_baseline": cpu, "plain_single_situation": plain,
             "kernel_families_ms_per_iter": {k: v["ms"] / 2 for k, v in prof.items()}}
+    if gff is not None:
+        line["gff_writer"] = gff
     if args.situations_hour:
         line["situations_per_hour"] = situations_per_hour(args.n, directions)
     print(json.dumps(line))

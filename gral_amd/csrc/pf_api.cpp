@@ -93,6 +93,8 @@ struct gral_pf_handle {
     // UK, VK, WK, TURB (the levels the solver does not hold on the device)
     void *gff_up = nullptr;
     size_t gff_bytes = 0;
+    void *gff_dbuf = nullptr, *gff_hbuf = nullptr;   // packed payload: device buffer, pinned host staging
+    size_t gff_dbuf_bytes = 0, gff_hbuf_bytes = 0;
     int gff_nlo = -1;
     bool gff_ok = false;
     std::string err;
@@ -1053,54 +1055,49 @@ static int gff_upper(gral_pf_handle *h, int nlo) {
     return 0;
 }
 
-extern "C" int gral_pf_gff_payload(gral_pf_handle *h, int32_t kind, const gral_pf_gff_meta *meta, void *out,
-                                   size_t cap, size_t *len) {
-    if (!h) return GRAL_PF_EINVAL;
-    if (kind < 0 || kind > 3) return fail(h, GRAL_PF_EINVAL, "gff: kind must be 0, 1, 2 (writing modes) or 3 (tke)");
-    if (!meta || !len) return fail(h, GRAL_PF_EINVAL, "gff: null meta or len");
-    if (!h->masks || !h->fields) return fail(h, GRAL_PF_ESTATE, "gff: masks and fields must be set first");
-    if (kind == 3 && !(h->prepared && h->prm.turbulence_model == 2))
-        return fail(h, GRAL_PF_ESTATE, "gff: the tke file needs a prepared k-eps (turbulence_model 2) situation");
+// Packs the payload of one file on the device: the 32-byte header into hdr,
+// the int16 body into the handle's device buffer (h->gff_dbuf); returns its
+// size in bytes through body.
+static int gff_pack(gral_pf_handle *h, int kind, const gral_pf_gff_meta *meta, uint8_t *hdr, size_t &body) {
     const size_t total = gff_size(h, kind);
-    *len = total;
-    if (!out) return GRAL_PF_OK;
-    if (cap < total) return fail(h, GRAL_PF_EINVAL, "gff: output buffer too small");
     const int nlo = h->prepared ? std::min(h->nkp, h->nkh) : 0;
     int rc = gff_upper(h, nlo);
     if (rc) return rc;
     // header: NKK (negative for modes 1 and 2), NJJ, NII, direction, speed, class, DXK, format flag
-    uint8_t *o8 = static_cast<uint8_t *>(out);
     const int32_t hn[3] = {(kind == 1 || kind == 2) ? -h->nkk : h->nkk, h->njj, h->nii};
     const int32_t flag = (kind == 1) ? -2 : (kind == 2) ? -3 : -1;
-    std::memcpy(o8, hn, 12);
-    std::memcpy(o8 + 12, &meta->wind_dir, 4);
-    std::memcpy(o8 + 16, &meta->wind_vel, 4);
-    std::memcpy(o8 + 20, &meta->stab_class, 4);
-    std::memcpy(o8 + 24, &h->dxk, 4);
-    std::memcpy(o8 + 28, &flag, 4);
-    // device buffers: payload body, KKART, kind-2 column offsets
-    const size_t body = total - 32, ncol = gff_columns(h, kind), n2 = h->n2();
+    std::memcpy(hdr, hn, 12);
+    std::memcpy(hdr + 12, &meta->wind_dir, 4);
+    std::memcpy(hdr + 16, &meta->wind_vel, 4);
+    std::memcpy(hdr + 20, &meta->stab_class, 4);
+    std::memcpy(hdr + 24, &h->dxk, 4);
+    std::memcpy(hdr + 28, &flag, 4);
+    // device buffer: body, KKART, kind-2 column offsets; kept by the handle
+    body = total - 32;
+    const size_t ncol = gff_columns(h, kind), n2 = h->n2();
     const size_t o_kk = (body + 255) & ~size_t(255), o_off = (o_kk + n2 * 2 + 255) & ~size_t(255);
     const size_t bytes = o_off + (kind == 2 ? ncol * sizeof(long long) : 0);
-    void *dbuf = nullptr;
-    if (hipMalloc(&dbuf, bytes + 256) != hipSuccess) return fail(h, GRAL_PF_ENOMEM, "gff: hipMalloc failed");
-    auto bail = [&](int code, const char *msg) { (void)hipFree(dbuf); return fail(h, code, msg); };
-    char *db = static_cast<char *>(dbuf);
+    if (bytes > h->gff_dbuf_bytes) {
+        if (h->gff_dbuf) (void)hipFree(h->gff_dbuf);
+        h->gff_dbuf = nullptr;
+        h->gff_dbuf_bytes = 0;
+        if (hipMalloc(&h->gff_dbuf, bytes) != hipSuccess) { h->gff_dbuf = nullptr; return fail(h, GRAL_PF_ENOMEM, "gff: hipMalloc failed"); }
+        h->gff_dbuf_bytes = bytes;
+    }
+    char *db = static_cast<char *>(h->gff_dbuf);
     std::vector<long long> off;
     if (kind == 2) {
         off.resize(ncol);
-        long long a = 0;
+        long long acc = 0;
         size_t q = 0;
         for (int i = 1; i <= h->nii + 1; ++i)
             for (int j = 1; j <= h->njj + 1; ++j) {
-                off[q++] = a;
-                a += 3LL * (h->nkk + 2 - std::max<int>(h->kk[h->i2(i, j)], 1));
+                off[q++] = acc;
+                acc += 3LL * (h->nkk + 2 - std::max<int>(h->kk[h->i2(i, j)], 1));
             }
-        if (hipMemcpyAsync(db + o_off, off.data(), ncol * sizeof(long long), hipMemcpyHostToDevice, h->stream) != hipSuccess)
-            return bail(GRAL_PF_EDEVICE, "gff: upload failed");
+        HIPCHK(h, hipMemcpyAsync(db + o_off, off.data(), ncol * sizeof(long long), hipMemcpyHostToDevice, h->stream));
     }
-    if (hipMemcpyAsync(db + o_kk, h->kk.data(), n2 * 2, hipMemcpyHostToDevice, h->stream) != hipSuccess)
-        return bail(GRAL_PF_EDEVICE, "gff: upload failed");
+    HIPCHK(h, hipMemcpyAsync(db + o_kk, h->kk.data(), n2 * 2, hipMemcpyHostToDevice, h->stream));
     GffArgs g{};
     const float *lo[4] = {h->d.uk, h->d.vk, h->d.wk, h->d.turb};
     const int nup = h->nkh - nlo;
@@ -1114,11 +1111,80 @@ extern "C" int gral_pf_gff_payload(gral_pf_handle *h, int32_t kind, const gral_p
     g.kk = reinterpret_cast<const int16_t *>(db + o_kk);
     g.off = kind == 2 ? reinterpret_cast<const long long *>(db + o_off) : nullptr;
     g.out = reinterpret_cast<int16_t *>(db);
-    if (pf_launch_gff(&g, h->stream)) return bail(GRAL_PF_EDEVICE, "gff: kernel launch failed");
-    if (hipMemcpyAsync(o8 + 32, db, body, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess)
-        return bail(GRAL_PF_EDEVICE, "gff: download failed");
-    (void)hipFree(dbuf);
+    if (pf_launch_gff(&g, h->stream)) return fail(h, GRAL_PF_EDEVICE, "gff: kernel launch failed");
+    // the upload sources (off, h->kk) must stay alive until the copies ran
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// pinned host staging buffer of the handle, at least n bytes
+static int gff_host(gral_pf_handle *h, size_t n) {
+    if (n <= h->gff_hbuf_bytes) return 0;
+    if (h->gff_hbuf) (void)hipHostFree(h->gff_hbuf);
+    h->gff_hbuf = nullptr;
+    h->gff_hbuf_bytes = 0;
+    if (hipHostMalloc(&h->gff_hbuf, n, hipHostMallocDefault) != hipSuccess) {
+        h->gff_hbuf = nullptr;
+        return fail(h, GRAL_PF_ENOMEM, "gff: hipHostMalloc failed");
+    }
+    h->gff_hbuf_bytes = n;
+    return 0;
+}
+
+static void gff_free(gral_pf_handle *h) {
+    if (h->gff_up) (void)hipFree(h->gff_up);
+    if (h->gff_dbuf) (void)hipFree(h->gff_dbuf);
+    if (h->gff_hbuf) (void)hipHostFree(h->gff_hbuf);
+    h->gff_up = h->gff_dbuf = h->gff_hbuf = nullptr;
+    h->gff_bytes = h->gff_dbuf_bytes = h->gff_hbuf_bytes = 0;
+}
+
+extern "C" int gral_pf_gff_payload(gral_pf_handle *h, int32_t kind, const gral_pf_gff_meta *meta, void *out,
+                                   size_t cap, size_t *len) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (kind < 0 || kind > 3) return fail(h, GRAL_PF_EINVAL, "gff: kind must be 0, 1, 2 (writing modes) or 3 (tke)");
+    if (!meta || !len) return fail(h, GRAL_PF_EINVAL, "gff: null meta or len");
+    if (!h->masks || !h->fields) return fail(h, GRAL_PF_ESTATE, "gff: masks and fields must be set first");
+    if (kind == 3 && !(h->prepared && h->prm.turbulence_model == 2))
+        return fail(h, GRAL_PF_ESTATE, "gff: the tke file needs a prepared k-eps (turbulence_model 2) situation");
+    *len = gff_size(h, kind);
+    if (!out) return GRAL_PF_OK;
+    if (cap < *len) return fail(h, GRAL_PF_EINVAL, "gff: output buffer too small");
+    uint8_t *o8 = static_cast<uint8_t *>(out);
+    size_t body = 0;
+    int rc = gff_pack(h, kind, meta, o8, body);
+    if (rc) return rc;
+    // device -> pinned staging in chunks, each copied on to the caller's buffer
+    // as soon as its transfer has completed (the two copies overlap)
+    constexpr size_t kCh = size_t(8) << 20;
+    const size_t stage = std::min(body, 4 * kCh);
+    if ((rc = gff_host(h, std::max<size_t>(stage, 1)))) return rc;
+    const size_t nch = (body + kCh - 1) / kCh;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int q = 0; q < 4; ++q)
+        if (hipEventCreateWithFlags(&ev[q], hipEventDisableTiming) != hipSuccess) {
+            for (int r = 0; r < q; ++r) (void)hipEventDestroy(ev[r]);
+            return fail(h, GRAL_PF_EDEVICE, "gff: event create failed");
+        }
+    const char *db = static_cast<const char *>(h->gff_dbuf);
+    char *hb = static_cast<char *>(h->gff_hbuf);
+    bool ok = true;
+    auto issue = [&](size_t c) {
+        const size_t o = c * kCh, n = std::min(kCh, body - o);
+        ok = ok && hipMemcpyAsync(hb + (c % 4) * kCh, db + o, n, hipMemcpyDeviceToHost, h->stream) == hipSuccess &&
+             hipEventRecord(ev[c % 4], h->stream) == hipSuccess;
+    };
+    for (size_t c = 0; c < std::min<size_t>(nch, 4); ++c) issue(c);
+    for (size_t c = 0; ok && c < nch; ++c) {
+        ok = hipEventSynchronize(ev[c % 4]) == hipSuccess;
+        if (!ok) break;
+        const size_t o = c * kCh, n = std::min(kCh, body - o);
+        std::memcpy(o8 + 32 + o, hb + (c % 4) * kCh, n);
+        if (c + 4 < nch) issue(c + 4);
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) ok = false;
+    for (int q = 0; q < 4; ++q) (void)hipEventDestroy(ev[q]);
+    if (!ok) return fail(h, GRAL_PF_EDEVICE, "gff: download failed");
     return GRAL_PF_OK;
 }
 
@@ -1129,13 +1195,18 @@ extern "C" int gral_pf_gff_write(gral_pf_handle *h, int32_t kind, const gral_pf_
     int rc = gral_pf_gff_payload(h, kind, meta, nullptr, 0, &len);
     if (rc) return rc;
     if (len >= 0xffffffffull) return fail(h, GRAL_PF_EINVAL, "gff: payload of 4 GiB or more (zip without ZIP64)");
-    std::vector<uint8_t> buf(len);
-    if ((rc = gral_pf_gff_payload(h, kind, meta, buf.data(), len, &len))) return rc;
+    // whole payload into the pinned buffer, deflated from there
+    if ((rc = gff_host(h, len))) return rc;
+    uint8_t *hb = static_cast<uint8_t *>(h->gff_hbuf);
+    size_t body = 0;
+    if ((rc = gff_pack(h, kind, meta, hb, body))) return rc;
+    HIPCHK(h, hipMemcpyAsync(hb + 32, h->gff_dbuf, body, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     // the zip entry carries the file's own name (GFFWrite.cs:109, :177)
     std::string entry(path);
     const size_t slash = entry.find_last_of("/\\");
     if (slash != std::string::npos) entry = entry.substr(slash + 1);
-    if ((rc = gral_pf_zip_write(path, entry.c_str(), buf.data(), len))) return fail(h, rc, "gff: writing the zip file failed");
+    if ((rc = gral_pf_zip_write(path, entry.c_str(), hb, len))) return fail(h, rc, "gff: writing the zip file failed");
     return GRAL_PF_OK;
 }
 
@@ -1153,7 +1224,7 @@ extern "C" const char *gral_pf_last_error(const gral_pf_handle *h) {
 extern "C" void gral_pf_destroy(gral_pf_handle *h) {
     if (!h) return;
     free_device(h);
-    if (h->gff_up) (void)hipFree(h->gff_up);
+    gff_free(h);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);

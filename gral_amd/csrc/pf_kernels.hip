@@ -2317,16 +2317,36 @@ __global__ void k_gff_column(GffArgs g) {
     }
 }
 
-// kind 1: block (kq, a), threads over b; rows u(b), v(b), w(b)
-__global__ void k_gff_rows(GffArgs g) {
-    const int kq = blockIdx.y, a = blockIdx.x;
-    const int n = g.njj + 1;
-    int16_t *o = g.out + ((size_t)kq * (g.nii + 1) + a) * 3 * (size_t)n;
-    for (int b = threadIdx.x; b < n; b += blockDim.x) {
-        const size_t col = (size_t)(a + 1) * g.nj2 + (b + 1);
-        o[b] = gff_trunc(gff_at(g, 0, col, kq) * 100.0f);
-        o[n + b] = gff_trunc(gff_at(g, 1, col, kq) * 100.0f);
-        o[2 * n + b] = gff_trunc(gff_at(g, 2, col, kq + 1) * 100.0f);
+// kind 1: block (a, 64 columns b), levels in tiles of 32: the column reads
+// (contiguous in k) are transposed through LDS into the rows of the file
+// (contiguous in b)
+#define GFF_BJ 64
+#define GFF_KT 32
+__global__ void __launch_bounds__(256) k_gff_rows(GffArgs g) {
+    __shared__ int16_t tile[3][GFF_KT][GFF_BJ + 2];
+    const int a = blockIdx.x, b0 = blockIdx.y * GFF_BJ;
+    const int n = g.njj + 1, nb = min(GFF_BJ, n - b0), nk = g.nkk + 1;
+    for (int k0 = 0; k0 < nk; k0 += GFF_KT) {
+        const int kt = min(GFF_KT, nk - k0);
+        for (int t = threadIdx.x; t < GFF_BJ * GFF_KT; t += blockDim.x) {
+            const int cb = t / GFF_KT, kk = t % GFF_KT;
+            if (cb < nb && kk < kt) {
+                const size_t col = (size_t)(a + 1) * g.nj2 + (b0 + cb + 1);
+                const int kq = k0 + kk;
+                tile[0][kk][cb] = gff_trunc(gff_at(g, 0, col, kq) * 100.0f);
+                tile[1][kk][cb] = gff_trunc(gff_at(g, 1, col, kq) * 100.0f);
+                tile[2][kk][cb] = gff_trunc(gff_at(g, 2, col, kq + 1) * 100.0f);
+            }
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < GFF_KT * 3 * GFF_BJ; t += blockDim.x) {
+            const int cb = t % GFF_BJ, cp = (t / GFF_BJ) % 3, kk = t / (3 * GFF_BJ);
+            if (cb < nb && kk < kt) {
+                const size_t row = ((size_t)(k0 + kk) * (g.nii + 1) + a) * 3 + cp;
+                g.out[row * n + b0 + cb] = tile[cp][kk][cb];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -2356,7 +2376,7 @@ extern "C" int pf_launch_gff(const GffArgs *ga, hipStream_t s) {
     else if (g.kind == 3)
         hipLaunchKernelGGL(k_gff_column, dim3((unsigned)(g.nii * g.njj)), dim3(256), 0, s, g);
     else if (g.kind == 1)
-        hipLaunchKernelGGL(k_gff_rows, dim3((unsigned)(g.nii + 1), (unsigned)(g.nkk + 1)), dim3(256), 0, s, g);
+        hipLaunchKernelGGL(k_gff_rows, dim3((unsigned)(g.nii + 1), (unsigned)((g.njj + GFF_BJ) / GFF_BJ)), dim3(256), 0, s, g);
     else
         hipLaunchKernelGGL(k_gff_terrain, dim3((unsigned)((g.nii + 1) * (g.njj + 1))), dim3(256), 0, s, g);
     return (int)hipGetLastError();

@@ -204,16 +204,29 @@ class PrognosticFlowfield:
         """The mass-conservation post-pass (DiagnosticFlowfield + final W step)."""
         self._check(self.lib.gral_pf_diagnostic(self.h, int(flow_field_level), int(bool(terrain))))
 
+    def gff_payload_into(self, kind, out, wind_dir=270.0, wind_vel=3.0, stab_class=4):
+        """Packs the payload of a flow-field file into out (a contiguous uint8
+        numpy array of at least gff_size(kind) bytes); returns its length."""
+        meta = GffMeta(float(wind_dir), float(wind_vel), int(stab_class))
+        n = C.c_size_t(0)
+        assert out.dtype == np.uint8 and out.flags["C_CONTIGUOUS"]
+        self._check(self.lib.gral_pf_gff_payload(self.h, int(kind), C.byref(meta), out.ctypes.data_as(C.c_void_p),
+                                                 out.nbytes, C.byref(n)))
+        return n.value
+
+    def gff_size(self, kind):
+        meta = GffMeta(0.0, 0.0, 0)
+        n = C.c_size_t(0)
+        self._check(self.lib.gral_pf_gff_payload(self.h, int(kind), C.byref(meta), None, 0, C.byref(n)))
+        return n.value
+
     def gff_payload(self, kind, wind_dir=270.0, wind_vel=3.0, stab_class=4):
         """The bytes GFFWrite.cs puts into a flow-field file's zip entry (kind 0,
         1, 2: writing modes of GRAL_FlowFields.txt; 3: the *.tke file),
         packed on the GPU."""
-        meta = GffMeta(float(wind_dir), float(wind_vel), int(stab_class))
-        n = C.c_size_t(0)
-        self._check(self.lib.gral_pf_gff_payload(self.h, int(kind), C.byref(meta), None, 0, C.byref(n)))
-        buf = C.create_string_buffer(n.value)
-        self._check(self.lib.gral_pf_gff_payload(self.h, int(kind), C.byref(meta), buf, n.value, C.byref(n)))
-        return buf.raw[:n.value]
+        buf = np.empty(self.gff_size(kind), dtype=np.uint8)
+        n = self.gff_payload_into(kind, buf, wind_dir, wind_vel, stab_class)
+        return buf[:n].tobytes()
 
     def write_gff(self, path, kind, wind_dir=270.0, wind_vel=3.0, stab_class=4):
         """GFFWrite.Write() for one file: payload packed on the GPU, written as
