@@ -1,0 +1,434 @@
+// Set-up of one weather situation on flat terrain, before the prognostic call:
+//   * prognostic sub-domains around the buildings (ReadGeometryAndDomains.cs:321-349),
+//   * the reference point of the sub-domains (MicroscaleTerrainSearchRefPoint.cs:27-103,
+//     MicroscaleTerrain.cs:936-963),
+//   * the initial wind field, KKART, AHK, BUI_HEIGHT and KADVMAX
+//     (MicroscaleFlat.cs:29-197) and the damping of the initial wind near
+//     obstacle walls (:199-345).
+//
+// The transcendental parts depend on one level and one Obukhov length only
+// (the power-law profile) or on one neighbour offset only (the wall damping):
+// the host evaluates them into small tables with the C library's powf, logf
+// and sqrtf, as .NET's MathF does on Linux; the kernels do the per-cell work:
+// masks, table look-ups, the neighbour searches and the reductions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gral_pf.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int setfail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+inline float net_maxf(float a, float b) {  // Math.Max(float, float)
+    if (a != b) { if (!std::isnan(a)) return b < a ? a : b; return a; }
+    return std::signbit(b) ? a : b;
+}
+
+// device scratch of one call, released on every exit path
+struct Scratch {
+    std::vector<void *> p;
+    hipStream_t s = nullptr;
+    ~Scratch() {
+        if (s) (void)hipStreamSynchronize(s);
+        for (void *q : p) (void)hipFree(q);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    template <class T>
+    T *alloc(size_t n) {
+        void *q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        p.push_back(q);
+        return static_cast<T *>(q);
+    }
+};
+
+}  // namespace
+
+#define SETCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) return setfail(GRAL_PF_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Sub-domains: every column with CUTK > 0 marks the rectangle of
+// (int)(factor*CUTK/DXK) columns in i and (int)(factor*CUTK/DYK) in j around
+// it (clipped to 1..NII x 1..NJJ).  Two passes give the same union without
+// writing every rectangle: reach[i][j1] = the largest i-reach among the
+// columns of row i whose j-range covers j1, then a forward and a backward
+// scan along i marks the rows within reach.
+// ---------------------------------------------------------------------------
+__global__ void k_sd_reach(const float *cutk, int nii, int njj, float factor, float dxk, float dyk, int *reach) {
+    const int nj2 = njj + 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nii * njj) return;
+    const int i = 1 + (int)(t / njj), j = 1 + (int)(t % njj);
+    const float c = cutk[(size_t)i * nj2 + j];
+    if (!(c > 0)) return;
+    const int ip = (int)(factor * c / dxk), jp = (int)(factor * c / dyk);
+    const int j0 = max(1, j - jp), j1 = min(njj, j + jp);
+    for (int q = j0; q <= j1; ++q) atomicMax(&reach[(size_t)i * nj2 + q], ip);
+}
+
+__global__ void k_sd_mark(const int *reach, int nii, int njj, uint8_t *adv) {
+    const int nj2 = njj + 2;
+    const int j = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j > njj) return;
+    int r = -1;
+    for (int i = 1; i <= nii; ++i) {              // a column i' <= i reaches i
+        r = max(r - 1, reach[(size_t)i * nj2 + j]);
+        if (r >= 0) adv[(size_t)i * nj2 + j] = 1;
+    }
+    r = -1;
+    for (int i = nii; i >= 1; --i) {              // a column i' >= i reaches i
+        r = max(r - 1, reach[(size_t)i * nj2 + j]);
+        if (r >= 0) adv[(size_t)i * nj2 + j] = 1;
+    }
+}
+
+extern "C" int gral_pf_subdomains(int32_t nii, int32_t njj, float dxk, float dyk, int32_t factor, const float *cutk,
+                                  uint8_t *advdom) {
+    if (nii < 1 || njj < 1 || !(dxk > 0) || !(dyk > 0) || factor < 0 || !cutk || !advdom)
+        return setfail(GRAL_PF_EINVAL, "subdomains: bad argument");
+    const size_t n2 = (size_t)(nii + 2) * (njj + 2);
+    Scratch S;
+    SETCHK(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
+    float *dc = S.alloc<float>(n2);
+    int *dr = S.alloc<int>(n2);
+    uint8_t *da = S.alloc<uint8_t>(n2);
+    if (!dc || !dr || !da) return setfail(GRAL_PF_ENOMEM, "subdomains: hipMalloc failed");
+    SETCHK(hipMemcpyAsync(dc, cutk, n2 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(da, advdom, n2, hipMemcpyHostToDevice, S.s));
+    std::vector<int> neg(n2, -1);
+    SETCHK(hipMemcpyAsync(dr, neg.data(), n2 * sizeof(int), hipMemcpyHostToDevice, S.s));
+    const size_t cols = (size_t)nii * njj;
+    hipLaunchKernelGGL(k_sd_reach, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, S.s, dc, nii, njj, (float)factor,
+                       dxk, dyk, dr);
+    hipLaunchKernelGGL(k_sd_mark, dim3((unsigned)((njj + 63) / 64)), dim3(64), 0, S.s, dr, nii, njj, da);
+    SETCHK(hipGetLastError());
+    SETCHK(hipMemcpyAsync(advdom, da, n2, hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipStreamSynchronize(S.s));
+    return GRAL_PF_OK;
+}
+
+// Spiral search from the border towards the centre for the first sub-domain
+// cell (MicroscaleTerrainSearchRefPoint.cs:27-103); (1, 1) when there is none,
+// as GetSubDomainPoint (MicroscaleTerrain.cs:936-963) sets it, and the
+// return code then says so.
+extern "C" int gral_pf_reference_point(int32_t nii, int32_t njj, const uint8_t *advdom, int32_t *ref_i, int32_t *ref_j) {
+    if (nii < 1 || njj < 1 || !advdom || !ref_i || !ref_j) return setfail(GRAL_PF_EINVAL, "reference_point: bad argument");
+    const int xmax = nii + 2, ymax = njj + 2, searchmax = std::min(xmax, ymax) / 2;
+    auto A = [&](int i, int j) { return advdom[(size_t)i * ymax + j]; };
+    int delta = 1, i = 1, j = 1;
+    bool found = false;
+    while (!found && delta < searchmax) {
+        i = delta;
+        j = delta;
+        while (j < ymax - delta && !found) { if (A(i, j) == 1) found = true; else ++j; }
+        while (i < xmax - delta && !found) { if (A(i, j) == 1) found = true; else ++i; }
+        while (j > delta && !found) { if (A(i, j) == 1) found = true; else --j; }
+        while (i > delta && !found) { if (A(i, j) == 1) found = true; else --i; }
+        ++delta;
+    }
+    if (!found) {
+        *ref_i = 1;
+        *ref_j = 1;
+        return setfail(GRAL_PF_ESTATE, "reference_point: no prognostic sub-domain");
+    }
+    *ref_i = i;
+    *ref_j = j;
+    return GRAL_PF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Initial field.  Per column (i, j) of 1..NII+1 x 1..NJJ+1, levels 1..NKK.
+// A level is blocked when its centre vertk = HOKART[k-1] + DZK[k]/2 is at or
+// below CUTK.  The reference zeroes the levels first, then for every inner
+// column sets u and v from the profile above the obstacle, zeroes the u of
+// the next column in i (v of the next in j) inside it, and copies the last
+// inner row and column into the ghost row NII+1 (u) and column NJJ+1 (v).
+// Independent of the order the columns are visited in, this leaves:
+//   u[i][j] = profile when neither (i, j) nor (i-1, j) is blocked, else 0
+//   v[i][j] = profile when neither (i, j) nor (i, j-1) is blocked, else 0
+//   u[NII+1][j] = u[NII][j], v[i][NJJ+1] = v[i][NJJ], everything else 0.
+// ---------------------------------------------------------------------------
+struct FlatArgs {
+    int nii, njj, nkk, nkh, nj2;
+    const float *vertk;                  // [nkh]
+    const float *hok;                    // [nkh]
+    const float *cutk;                   // [n2]
+    const int *cls;                      // [n2] profile class of each column, or null (class 0)
+    const float *ux, *uy;                // [classes][nkh] profile values per level
+    float *uk, *vk, *wk;                 // [n2][nkh]
+    int16_t *kkart;
+    float *ahk, *bui;
+    int *kad;                            // max KKART over columns with CUTK > 0
+};
+
+__device__ __forceinline__ float fl_u(const FlatArgs &a, int i, int j, int k) {
+    const size_t c = (size_t)i * a.nj2 + j;
+    const float v = a.vertk[k];
+    if (v <= a.cutk[c]) return 0.0f;
+    if (i > 1 && v <= a.cutk[c - a.nj2]) return 0.0f;
+    return a.ux[(size_t)(a.cls ? a.cls[c] : 0) * a.nkh + k];
+}
+__device__ __forceinline__ float fl_v(const FlatArgs &a, int i, int j, int k) {
+    const size_t c = (size_t)i * a.nj2 + j;
+    const float v = a.vertk[k];
+    if (v <= a.cutk[c]) return 0.0f;
+    if (j > 1 && v <= a.cutk[c - 1]) return 0.0f;
+    return a.uy[(size_t)(a.cls ? a.cls[c] : 0) * a.nkh + k];
+}
+
+// one wave per column
+__global__ void __launch_bounds__(64) k_flat_column(FlatArgs a) {
+    const int ncj = a.njj + 1;
+    const int i = 1 + (int)(blockIdx.x / ncj), j = 1 + (int)(blockIdx.x % ncj);
+    const size_t c = (size_t)i * a.nj2 + j, o = c * a.nkh;
+    const bool inner_i = i <= a.nii, inner_j = j <= a.njj;
+    int kb = 0;                          // highest blocked level
+    float ah = 0.0f;
+    for (int k = 1 + threadIdx.x; k <= a.nkk; k += 64) {
+        float u = 0.0f, v = 0.0f;
+        if (inner_i && inner_j) {
+            u = fl_u(a, i, j, k);
+            v = fl_v(a, i, j, k);
+            if (a.vertk[k] <= a.cutk[c]) {
+                kb = k;
+                ah = fmaxf(ah, a.hok[k]);
+            }
+        } else if (inner_j) {            // i = NII+1: copy of row NII
+            u = fl_u(a, a.nii, j, k);
+        } else if (inner_i) {            // j = NJJ+1: copy of column NJJ
+            v = fl_v(a, i, a.njj, k);
+        }
+        a.uk[o + k] = u;
+        a.vk[o + k] = v;
+        a.wk[o + k] = 0.0f;
+    }
+    if (!(inner_i && inner_j)) return;
+    for (int m = 32; m >= 1; m >>= 1) {
+        kb = max(kb, __shfl_xor(kb, m));
+        ah = fmaxf(ah, __shfl_xor(ah, m));
+    }
+    if (threadIdx.x == 0) {
+        a.kkart[c] = (int16_t)kb;
+        a.ahk[c] = ah;
+        if (kb > 0) {
+            a.bui[c] = ah;
+            if (a.cutk[c] > 0) atomicMax(a.kad, kb);
+        }
+    }
+}
+
+// Damping near walls (MicroscaleFlat.cs:201-345): for each free level above
+// KKART of a column in IGEB+1..NII-IGEB x IGEB+1..NJJ-IGEB, the first
+// obstacle (CUTK > 1 and vertk <= AHK) within IGEB cells in each of the eight
+// directions W, E, N, S, SE, NW, NE, SW multiplies u and v by the factor of
+// its distance, fac[dir class][offset] (0: along i, 1: along j, 2: diagonal;
+// 1.0 where the distance exceeds 20 m).
+struct DampArgs {
+    int nii, njj, nkk, nkh, nj2, igeb;
+    int mode;                            // 1: columns outside the sub-domains, 2: all
+    const float *vertk;
+    const float *cutk, *ahk;
+    const int16_t *kkart;
+    const uint8_t *adv;
+    const float *fac;                    // [3][igeb+1]
+    float *uk, *vk;
+};
+
+__device__ __forceinline__ bool dm_hit(const DampArgs &a, int i, int j, float v) {
+    const size_t c = (size_t)i * a.nj2 + j;
+    return a.cutk[c] > 1.0f && v <= a.ahk[c];
+}
+
+__global__ void k_flat_damp(DampArgs a) {
+    const int w = a.nii - 2 * a.igeb, hgt = a.njj - 2 * a.igeb;
+    const int col = blockIdx.x;
+    if (col >= w * hgt) return;
+    const int i = 1 + a.igeb + col / hgt, j = 1 + a.igeb + col % hgt;
+    const size_t c = (size_t)i * a.nj2 + j;
+    if (a.mode == 1 && a.adv[c] != 0) return;
+    const int kk = a.kkart[c];
+    const int G = a.igeb;
+    for (int k = max(1, kk + 1) + threadIdx.x; k <= a.nkk - 1; k += blockDim.x) {
+        const float v = a.vertk[k];
+        float ab = 1.0f;
+        int g;
+        for (g = 1; g <= G && !dm_hit(a, i - g, j, v); ++g) {}          // west
+        if (g <= G) ab *= a.fac[0 * (G + 1) + g];
+        for (g = 1; g <= G && !dm_hit(a, i + g, j, v); ++g) {}          // east
+        if (g <= G) ab *= a.fac[0 * (G + 1) + g];
+        for (g = 1; g <= G && !dm_hit(a, i, j + g, v); ++g) {}          // north
+        if (g <= G) ab *= a.fac[1 * (G + 1) + g];
+        for (g = 1; g <= G && !dm_hit(a, i, j - g, v); ++g) {}          // south
+        if (g <= G) ab *= a.fac[1 * (G + 1) + g];
+        for (g = 1; g <= G && !dm_hit(a, i + g, j + g, v); ++g) {}      // ig = i+g, jg = j+ig-i
+        if (g <= G) ab *= a.fac[2 * (G + 1) + g];
+        for (g = 1; g <= G && !dm_hit(a, i - g, j - g, v); ++g) {}      // ig = i-g, jg = j+ig-i
+        if (g <= G) ab *= a.fac[2 * (G + 1) + g];
+        for (g = 1; g <= G && !dm_hit(a, i + g, j - g, v); ++g) {}      // ig = i+g, jg = j+i-ig
+        if (g <= G) ab *= a.fac[2 * (G + 1) + g];
+        for (g = 1; g <= G && !dm_hit(a, i - g, j + g, v); ++g) {}      // ig = i-g, jg = j+i-ig
+        if (g <= G) ab *= a.fac[2 * (G + 1) + g];
+        const size_t o = c * a.nkh + k;
+        a.uk[o] *= ab;
+        a.vk[o] *= ab;
+    }
+}
+
+// Profile value of one level (MicroscaleFlat.cs:84-138): power law below the
+// first and (with one profile level) above the last measurement height,
+// linear interpolation between measurements, the top value above them.
+static void profile_level(const gral_pf_flat_met *m, float vertk, float exponent, float &ux, float &uy) {
+    const int n = m->n_profile;
+    const float *H = m->meas_height, *U = m->obs_u, *V = m->obs_v;
+    if (vertk <= H[1]) {
+        const float f = powf(vertk / H[1], exponent);
+        ux = U[1] * f;
+        uy = V[1] * f;
+    } else if (vertk > H[n]) {
+        if (n == 1) {
+            const float f = powf(vertk / H[n], exponent);
+            ux = U[n] * f;
+            uy = V[n] * f;
+        } else {
+            ux = U[n];
+            uy = V[n];
+        }
+    } else {
+        int ipo = 1;
+        for (int q = 1; q <= n; ++q)
+            if (vertk > H[q]) ipo = q + 1;
+        ux = U[ipo - 1] + (U[ipo] - U[ipo - 1]) / (H[ipo] - H[ipo - 1]) * (vertk - H[ipo - 1]);
+        uy = V[ipo - 1] + (V[ipo] - V[ipo - 1]) / (H[ipo] - H[ipo - 1]) * (vertk - H[ipo - 1]);
+    }
+}
+
+static float profile_exponent(float ob) {
+    if (ob <= 0) return net_maxf(0.35f - 0.4f * powf(std::fabs(ob), -0.15f), 0.05f);
+    return 0.56f * powf(ob, -0.15f);
+}
+
+extern "C" int gral_pf_flat_init(const gral_pf_grid *g, const gral_pf_flat_met *m, const float *cutk,
+                                 const uint8_t *advdom, float *uk, float *vk, float *wk, int16_t *kkart, float *ahk,
+                                 float *bui_height, int32_t *kadvmax) {
+    if (!g || !m || !cutk || !uk || !vk || !wk || !kkart || !ahk || !kadvmax || !g->dzk || !g->hokart)
+        return setfail(GRAL_PF_EINVAL, "flat_init: null argument");
+    if (g->nii < 1 || g->njj < 1 || g->nkk < 2 || g->vertcellsff < 0 || !(g->dxk > 0) || !(g->dyk > 0))
+        return setfail(GRAL_PF_EINVAL, "flat_init: bad grid");
+    if (m->n_profile < 1 || m->n_profile > 99 || !m->meas_height || !m->obs_u || !m->obs_v)
+        return setfail(GRAL_PF_EINVAL, "flat_init: bad meteorological profile");
+    if (m->near_wall < 0 || m->near_wall > 2 || (m->near_wall == 1 && !advdom))
+        return setfail(GRAL_PF_EINVAL, "flat_init: near_wall must be 0, 1 (needs advdom) or 2");
+    const int nii = g->nii, njj = g->njj, nkk = g->nkk, nkh = nkk + 2, nj2 = njj + 2;
+    const size_t n2 = (size_t)(nii + 2) * nj2, n3 = n2 * nkh;
+    // levels: vertk = HOKART[k-1] + DZK[k]*0.5
+    std::vector<float> vertk(nkh, 0.0f);
+    for (int k = 1; k <= nkk; ++k) vertk[k] = g->hokart[k - 1] + g->dzk[k] * 0.5f;
+    // profile classes: one per distinct Obukhov length of the inner columns
+    std::vector<float> obs;
+    std::vector<int> cls;
+    if (m->ol_column) {
+        std::vector<uint32_t> bits;
+        bits.reserve((size_t)nii * njj);
+        for (int i = 1; i <= nii; ++i)
+            for (int j = 1; j <= njj; ++j) {
+                uint32_t b;
+                std::memcpy(&b, &m->ol_column[(size_t)i * nj2 + j], 4);
+                bits.push_back(b);
+            }
+        std::sort(bits.begin(), bits.end());
+        bits.erase(std::unique(bits.begin(), bits.end()), bits.end());
+        obs.resize(bits.size());
+        for (size_t q = 0; q < bits.size(); ++q) std::memcpy(&obs[q], &bits[q], 4);
+        cls.assign(n2, 0);
+        for (int i = 1; i <= nii; ++i)
+            for (int j = 1; j <= njj; ++j) {
+                uint32_t b;
+                std::memcpy(&b, &m->ol_column[(size_t)i * nj2 + j], 4);
+                cls[(size_t)i * nj2 + j] = (int)(std::lower_bound(bits.begin(), bits.end(), b) - bits.begin());
+            }
+    } else {
+        obs.push_back(m->ob_length);
+    }
+    const size_t ncls = obs.size();
+    std::vector<float> ux(ncls * nkh, 0.0f), uy(ncls * nkh, 0.0f);
+    for (size_t q = 0; q < ncls; ++q) {
+        const float e = profile_exponent(obs[q]);
+        for (int k = 1; k <= nkk; ++k) profile_level(m, vertk[k], e, ux[q * nkh + k], uy[q * nkh + k]);
+    }
+    // wall damping factors 0.19*ln((entf+0.5)*10) per direction class and offset
+    const int igeb = std::max((int)(20.0f / g->dxk), 1);
+    std::vector<float> fac(3 * (size_t)(igeb + 1), 1.0f);
+    for (int o = 1; o <= igeb; ++o) {
+        const float dx = (float)o * g->dxk, dy = (float)o * g->dyk;
+        const float e3[3] = {std::fabs(dx), std::fabs(dy), std::sqrt(dx * dx + dy * dy)};
+        for (int q = 0; q < 3; ++q)
+            fac[q * (igeb + 1) + o] = (e3[q] <= 20.0f) ? 0.19f * logf((e3[q] + 0.5f) * 10.0f) : 1.0f;
+    }
+    // device work
+    Scratch S;
+    SETCHK(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
+    float *d_vk = S.alloc<float>(nkh), *d_hok = S.alloc<float>(nkh), *d_cut = S.alloc<float>(n2);
+    float *d_ux = S.alloc<float>(ux.size()), *d_uy = S.alloc<float>(uy.size()), *d_fac = S.alloc<float>(fac.size());
+    float *d_u = S.alloc<float>(n3), *d_v = S.alloc<float>(n3), *d_w = S.alloc<float>(n3);
+    float *d_ahk = S.alloc<float>(n2), *d_bui = S.alloc<float>(n2);
+    int16_t *d_kk = S.alloc<int16_t>(n2);
+    uint8_t *d_adv = S.alloc<uint8_t>(n2);
+    int *d_cls = cls.empty() ? nullptr : S.alloc<int>(n2);
+    int *d_kad = S.alloc<int>(1);
+    if (!d_vk || !d_hok || !d_cut || !d_ux || !d_uy || !d_fac || !d_u || !d_v || !d_w || !d_ahk || !d_bui || !d_kk ||
+        !d_adv || !d_kad || (!cls.empty() && !d_cls))
+        return setfail(GRAL_PF_ENOMEM, "flat_init: hipMalloc failed");
+    const int one = 1;
+    SETCHK(hipMemcpyAsync(d_vk, vertk.data(), nkh * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_hok, g->hokart, nkh * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_cut, cutk, n2 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_ux, ux.data(), ux.size() * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_uy, uy.data(), uy.size() * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_fac, fac.data(), fac.size() * sizeof(float), hipMemcpyHostToDevice, S.s));
+    // the fields keep the caller's values outside the levels and columns the set-up writes
+    SETCHK(hipMemcpyAsync(d_u, uk, n3 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_v, vk, n3 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_w, wk, n3 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_kk, kkart, n2 * sizeof(int16_t), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_ahk, ahk, n2 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    if (bui_height) SETCHK(hipMemcpyAsync(d_bui, bui_height, n2 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    if (advdom) SETCHK(hipMemcpyAsync(d_adv, advdom, n2, hipMemcpyHostToDevice, S.s));
+    if (d_cls) SETCHK(hipMemcpyAsync(d_cls, cls.data(), n2 * sizeof(int), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_kad, &one, sizeof(int), hipMemcpyHostToDevice, S.s));
+    FlatArgs fa{nii, njj, nkk, nkh, nj2, d_vk, d_hok, d_cut, d_cls, d_ux, d_uy, d_u, d_v, d_w, d_kk, d_ahk, d_bui, d_kad};
+    hipLaunchKernelGGL(k_flat_column, dim3((unsigned)((size_t)(nii + 1) * (njj + 1))), dim3(64), 0, S.s, fa);
+    SETCHK(hipGetLastError());
+    if (m->near_wall != 0 && nii > 2 * igeb && njj > 2 * igeb) {
+        DampArgs da{nii, njj, nkk, nkh, nj2, igeb, m->near_wall, d_vk, d_cut, d_ahk, d_kk, d_adv, d_fac, d_u, d_v};
+        hipLaunchKernelGGL(k_flat_damp, dim3((unsigned)((size_t)(nii - 2 * igeb) * (njj - 2 * igeb))), dim3(64), 0, S.s, da);
+        SETCHK(hipGetLastError());
+    }
+    int kad = 1;
+    SETCHK(hipMemcpyAsync(uk, d_u, n3 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(vk, d_v, n3 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(wk, d_w, n3 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(kkart, d_kk, n2 * sizeof(int16_t), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(ahk, d_ahk, n2 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    if (bui_height) SETCHK(hipMemcpyAsync(bui_height, d_bui, n2 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(&kad, d_kad, sizeof(int), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipStreamSynchronize(S.s));
+    *kadvmax = std::min(nkk, kad + g->vertcellsff);
+    return GRAL_PF_OK;
+}
+
+extern "C" const char *gral_pf_setup_last_error(void) { return g_err.c_str(); }

@@ -325,3 +325,92 @@ def tiny_case(n: int = 8, vertcellsff: int = 8, building=(3, 5, 4, 6, 4.5), dire
         c.advdom[1:n + 1, 1:n + 1] = 1
         c.refp = search_reference_point(c.advdom)
     return c
+
+
+# ---------------------------------------------------------------------------
+# Situations set up the way GRAL sets up flat terrain, on the GPU
+# ---------------------------------------------------------------------------
+
+# Obukhov length per stability class for the synthetic situations (an input of
+# the set-up, as Program.Ob is; GRAL derives it from the class and z0 in its
+# meteorology reader, which is not part of this path)
+STABILITY_OBUKHOV = {1: -6.0, 2: -20.0, 3: -60.0, 4: 1000.0, 5: 90.0, 6: 30.0, 7: 12.0}
+
+
+def wind_from_met(speed: float, direction_deg: float):
+    """ObsWindU/V from a meteopgt.all direction and speed (Input_pgt.cs:89-93):
+    the direction the wind blows from, in degrees, turned into radians from
+    the x axis; components in double, stored as float."""
+    rad = (270.0 - float(F32(direction_deg))) * math.pi / 180.0
+    return float(F32(speed * math.cos(F32(rad)))), float(F32(speed * math.sin(F32(rad))))
+
+
+def flat_case(name: str, nii: int, njj: int, cutk: np.ndarray, *, dx: float = 2.0, vertcellsff: int = 40,
+              speed: float = 3.0, direction: float = 270.0, stability: int = 4, anemometer: float = 10.0,
+              z0: float = 0.5, turbulence_model: int = 1, iterations: int = 200, factor: int = 15,
+              near_wall: int = 0) -> Case:
+    """A situation on flat terrain set up by the library on the GPU, as GRAL
+    does before the prognostic call: sub-domains around the buildings, the
+    reference point, the power-law initial field from one measurement height,
+    KKART, AHK and KADVMAX (``pf.subdomains``, ``pf.reference_point``,
+    ``pf.flat_init``).  ``near_wall`` = 0 is GRAL's default for the
+    prognostic approach (SubDomainDistance 1e6)."""
+    from gral_amd import pf
+    nkk, dzk, hok = vertical_grid_flat()
+    n2 = (nii + 2, njj + 2)
+    cutk = np.ascontiguousarray(cutk, dtype=F32)
+    adv = pf.subdomains(cutk, nii, njj, dx, dx, factor)
+    refp, ok = pf.reference_point(adv)
+    if not ok:
+        raise ValueError("no prognostic sub-domain in this situation")
+    u, v = wind_from_met(speed, direction)
+    ob = STABILITY_OBUKHOV[int(stability)]
+    met = {"n_profile": 1, "meas_height": [0.0, anemometer], "obs_u": [0.0, u], "obs_v": [0.0, v],
+           "ob_length": ob, "ol_column": None, "near_wall": near_wall}
+    r = pf.flat_init(nii, njj, nkk, vertcellsff, dx, dx, dzk, hok, cutk, met, advdom=adv)
+    ustar = speed * 0.4 / math.log(anemometer / z0)
+    zfield = np.full(n2, F32(z0), dtype=F32)
+    return Case(name=name, nii=nii, njj=njj, nkk=nkk, kadvmax=r["kadvmax"], vertcellsff=vertcellsff,
+                dxk=float(F32(dx)), dyk=float(F32(dx)), dzk=dzk, hokart=hok, advdom=adv, kkart=r["kkart"],
+                cutk=cutk, cov=np.zeros(n2, dtype=F32), ahk=r["ahk"], z0_wall=zfield.copy(),
+                z0_solver=zfield.copy(), ob_init=np.full(n2, F32(ob), dtype=F32),
+                ust_init=np.full(n2, F32(ustar), dtype=F32), z0_init=zfield.copy(), uk=r["uk"], vk=r["vk"],
+                wk=r["wk"], refp=refp, turbulence_model=turbulence_model, iter_min=iterations,
+                iter_max=iterations, coriolis=coriolis_parameter(47.0))
+
+
+def config3_situations(count: int = 256, seed: int = 20260003):
+    """The batched workload: ``count`` weather situations with directions
+    0-350 deg in 10 deg steps, speeds 0.5-6 m/s, stability classes 1-7, on
+    nested square windows of 200-600 cells (centred on one 600 x 600 city of
+    the config-2 generator) with 30-50 prognostic layers.  Seeded."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(count):
+        out.append({"index": q, "direction": float(10 * int(rng.integers(0, 36))),
+                    "speed": float(rng.uniform(0.5, 6.0)), "stability": int(rng.integers(1, 8)),
+                    "n": int(rng.integers(200, 601)), "vertcellsff": int(rng.integers(30, 51))})
+    return out
+
+
+_CITY600 = None
+
+
+def config3(situation: dict, iterations: int = 500) -> Case:
+    """One situation of ``config3_situations``, set up on the GPU."""
+    global _CITY600
+    if _CITY600 is None:
+        cut = np.zeros((602, 602), dtype=F32)
+        for b in city_buildings(600, 600, 2.0, 20260001):
+            sl = (slice(b.i0, b.i1 + 1), slice(b.j0, b.j1 + 1))
+            cut[sl] = np.maximum(cut[sl], F32(b.height))
+        _CITY600 = cut
+    n = situation["n"]
+    o = (600 - n) // 2
+    cutk = np.zeros((n + 2, n + 2), dtype=F32)
+    cutk[1:n + 1, 1:n + 1] = _CITY600[o + 1:o + n + 1, o + 1:o + n + 1]
+    c = flat_case(f"C3-{situation['index']}", n, n, cutk, vertcellsff=situation["vertcellsff"],
+                  speed=situation["speed"], direction=situation["direction"], stability=situation["stability"],
+                  iterations=iterations)
+    c.iter_min = min(100, iterations)
+    return c

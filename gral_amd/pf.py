@@ -27,6 +27,7 @@ EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
            "gral_pf_gff_payload", "gral_pf_gff_write", "gral_pf_zip_write",
+           "gral_pf_subdomains", "gral_pf_reference_point", "gral_pf_flat_init", "gral_pf_setup_last_error",
            "gral_pf_last_error", "gral_pf_destroy", "gral_pf_batch_last_error", "gral_pf_batch_destroy"]
 FAMILIES = ("stencil", "pressure", "u", "v", "w", "tke", "coef")
 
@@ -67,6 +68,12 @@ class GffMeta(C.Structure):
     _fields_ = [("wind_dir", C.c_float), ("wind_vel", C.c_float), ("stab_class", C.c_int32)]
 
 
+class FlatMet(C.Structure):
+    _fields_ = [("n_profile", C.c_int32), ("meas_height", C.POINTER(C.c_float)), ("obs_u", C.POINTER(C.c_float)),
+                ("obs_v", C.POINTER(C.c_float)), ("ob_length", C.c_float), ("ol_column", C.POINTER(C.c_float)),
+                ("near_wall", C.c_int32)]
+
+
 # flow-field file kinds: GRAL_FlowFields.txt writing modes 0, 1, 2 and the *.tke file
 GFF_KINDS = {"mode0": 0, "mode1": 1, "mode2": 2, "tke": 3}
 
@@ -101,6 +108,12 @@ def load(path: str = LIB_PATH):
                                         C.POINTER(C.c_size_t)]
     lib.gral_pf_gff_write.argtypes = [H, C.c_int32, C.POINTER(GffMeta), C.c_char_p]
     lib.gral_pf_zip_write.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_size_t]
+    lib.gral_pf_subdomains.argtypes = [C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_int32, FP, U8P]
+    lib.gral_pf_reference_point.argtypes = [C.c_int32, C.c_int32, U8P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    lib.gral_pf_flat_init.argtypes = [C.POINTER(Grid), C.POINTER(FlatMet), FP, U8P, FP, FP, FP, I16P, FP, FP,
+                                      C.POINTER(C.c_int32)]
+    lib.gral_pf_setup_last_error.argtypes = []
+    lib.gral_pf_setup_last_error.restype = C.c_char_p
     for n in ("gral_pf_last_error", "gral_pf_batch_last_error"):
         getattr(lib, n).argtypes = [H]
         getattr(lib, n).restype = C.c_char_p
@@ -108,7 +121,8 @@ def load(path: str = LIB_PATH):
         getattr(lib, n).argtypes = [H]
         getattr(lib, n).restype = None
     for n in EXPORTS[:-4]:
-        getattr(lib, n).restype = C.c_int
+        if n != "gral_pf_setup_last_error":
+            getattr(lib, n).restype = C.c_int
     _lib = lib
     return lib
 
@@ -324,6 +338,64 @@ def zip_write(path, entry, data: bytes):
     rc = lib.gral_pf_zip_write(os.fsencode(path), entry.encode(), data, len(data))
     if rc != 0:
         raise RuntimeError(f"gral_pf_zip_write failed ({rc})")
+
+
+def _setup_check(lib, rc, what):
+    if rc != 0:
+        msg = lib.gral_pf_setup_last_error()
+        raise RuntimeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def subdomains(cutk, nii, njj, dxk, dyk, factor=15, advdom=None):
+    """Prognostic sub-domains around the buildings (ReadGeometryAndDomains.cs:321-349), on the GPU."""
+    lib = load()
+    cut = _f32(cutk)
+    adv = np.zeros((nii + 2, njj + 2), np.uint8) if advdom is None else np.array(advdom, dtype=np.uint8, order="C")
+    _setup_check(lib, lib.gral_pf_subdomains(int(nii), int(njj), float(dxk), float(dyk), int(factor), _ptr(cut),
+                                             _ptr(adv, U8P)), "gral_pf_subdomains")
+    return adv
+
+
+def reference_point(advdom):
+    """Spiral search of the reference point (MicroscaleTerrainSearchRefPoint.cs:27-103)."""
+    lib = load()
+    adv = np.ascontiguousarray(advdom, dtype=np.uint8)
+    ri, rj = C.c_int32(0), C.c_int32(0)
+    rc = lib.gral_pf_reference_point(adv.shape[0] - 2, adv.shape[1] - 2, _ptr(adv, U8P), C.byref(ri), C.byref(rj))
+    if rc not in (0, -4):
+        _setup_check(lib, rc, "gral_pf_reference_point")
+    return (ri.value, rj.value), rc == 0
+
+
+def flat_init(nii, njj, nkk, vertcellsff, dxk, dyk, dzk, hokart, cutk, met, uk=None, vk=None, wk=None, advdom=None,
+              kkart=None, ahk=None, bui_height=None):
+    """MicroscaleFlat.Calculate() up to the flow-field call (MicroscaleFlat.cs:29-345), on the GPU.
+    met: dict(n_profile, meas_height, obs_u, obs_v, ob_length, ol_column=None, near_wall=0).
+    Returns dict(uk, vk, wk, kkart, ahk, bui_height, kadvmax)."""
+    lib = load()
+    shp3 = (nii + 2, njj + 2, nkk + 2)
+    shp2 = (nii + 2, njj + 2)
+    out = {"uk": np.zeros(shp3, np.float32) if uk is None else np.array(uk, dtype=np.float32, order="C"),
+           "vk": np.zeros(shp3, np.float32) if vk is None else np.array(vk, dtype=np.float32, order="C"),
+           "wk": np.zeros(shp3, np.float32) if wk is None else np.array(wk, dtype=np.float32, order="C"),
+           "kkart": np.zeros(shp2, np.int16) if kkart is None else np.array(kkart, dtype=np.int16, order="C"),
+           "ahk": np.zeros(shp2, np.float32) if ahk is None else np.array(ahk, dtype=np.float32, order="C"),
+           "bui_height": np.zeros(shp2, np.float32) if bui_height is None else
+           np.array(bui_height, dtype=np.float32, order="C")}
+    keep = {"dzk": _f32(dzk), "hok": _f32(hokart), "cut": _f32(cutk), "H": _f32(met["meas_height"]),
+            "U": _f32(met["obs_u"]), "V": _f32(met["obs_v"]),
+            "ol": None if met.get("ol_column") is None else _f32(met["ol_column"]),
+            "adv": None if advdom is None else np.ascontiguousarray(advdom, dtype=np.uint8)}
+    g = Grid(nii, njj, nkk, 2, vertcellsff, dxk, dyk, _ptr(keep["dzk"]), _ptr(keep["hok"]))
+    m = FlatMet(int(met["n_profile"]), _ptr(keep["H"]), _ptr(keep["U"]), _ptr(keep["V"]), float(met["ob_length"]),
+                _ptr(keep["ol"]), int(met.get("near_wall", 0)))
+    kad = C.c_int32(0)
+    _setup_check(lib, lib.gral_pf_flat_init(C.byref(g), C.byref(m), _ptr(keep["cut"]), _ptr(keep["adv"], U8P),
+                                            _ptr(out["uk"]), _ptr(out["vk"]), _ptr(out["wk"]),
+                                            _ptr(out["kkart"], I16P), _ptr(out["ahk"]), _ptr(out["bui_height"]),
+                                            C.byref(kad)), "gral_pf_flat_init")
+    out["kadvmax"] = kad.value
+    return out
 
 
 def _stats(st):

@@ -207,6 +207,55 @@ int gral_pf_zip_write(const char *path, const char *entry, const void *data, siz
 const char *gral_pf_last_error(const gral_pf_handle *h);
 void gral_pf_destroy(gral_pf_handle *h);
 
+/* ---- Set-up of a situation on flat terrain ------------------------------
+ * What GRAL computes before the prognostic call when there is no terrain,
+ * on the GPU.  These calls need no handle; their output feeds
+ * gral_pf_create (kadvmax), gral_pf_set_masks (advdom, kkart, ahk),
+ * gral_pf_upload_fields (uk, vk, wk) and gral_pf_prepare (refp_i/j).
+ * On failure gral_pf_setup_last_error() gives the message. */
+
+/* CalculateSubDomainsForPrognosticWindSimulation (ReadGeometryAndDomains.cs:321-349):
+ * sets advdom[i1][j1] = 1 within (int)(factor*CUTK/DXK) columns in i and
+ * (int)(factor*CUTK/DYK) in j of every column with CUTK > 0 (clipped to
+ * 1..NII x 1..NJJ); other entries keep their value.  factor is
+ * Program.PrognosticSubDomainFactor (15 by default). */
+int gral_pf_subdomains(int32_t nii, int32_t njj, float dxk, float dyk, int32_t factor, const float *cutk,
+                       uint8_t *advdom);
+
+/* SearchReferencePoint (MicroscaleTerrainSearchRefPoint.cs:27-103) as
+ * GetSubDomainPoint (MicroscaleTerrain.cs:936-963) calls it: the first
+ * sub-domain cell of a spiral from the border inwards.  Without any
+ * sub-domain cell it sets (1, 1) and returns GRAL_PF_ESTATE. */
+int gral_pf_reference_point(int32_t nii, int32_t njj, const uint8_t *advdom, int32_t *ref_i, int32_t *ref_j);
+
+/* Meteorological input of the initial profile: Program.MetProfileNumb,
+ * MeasurementHeight, ObsWindU/V, Ob[1][1] and OLGral. */
+typedef struct {
+    int32_t n_profile;          /* MetProfileNumb, 1..99 */
+    const float *meas_height;   /* [n_profile+1], index 0 unused */
+    const float *obs_u, *obs_v; /* [n_profile+1] wind components at those heights */
+    float ob_length;            /* Ob[1][1] */
+    const float *ol_column;     /* [nii+2][njj+2] OLGral when AdaptiveRoughnessMax > 0, else NULL */
+    int32_t near_wall;          /* damping near obstacle walls (MicroscaleFlat.cs:201-345): 0 none;
+                                   1 columns outside the sub-domains (prognostic with
+                                   SubDomainDistance < 10000; needs advdom); 2 every column
+                                   (diagnostic flow field) */
+} gral_pf_flat_met;
+
+/* MicroscaleFlat.Calculate() up to the flow-field call (MicroscaleFlat.cs:29-345):
+ * zeroes u, v, w at levels 1..NKK of columns 1..NII+1 x 1..NJJ+1, sets u, v
+ * above the obstacles from the power-law / interpolated profile, KKART and
+ * AHK of every inner column (and BUI_HEIGHT of the blocked ones, when
+ * bui_height is not NULL), damps u, v near walls as near_wall selects, and
+ * returns KADVMAX = min(NKK, max(1, max KKART over CUTK > 0) + VertCellsFF).
+ * grid->kadvmax is not read.  uk/vk/wk are in/out [nii+2][njj+2][nkk+2]:
+ * entries the reference does not write keep their values. */
+int gral_pf_flat_init(const gral_pf_grid *grid, const gral_pf_flat_met *met, const float *cutk,
+                      const uint8_t *advdom, float *uk, float *vk, float *wk, int16_t *kkart, float *ahk,
+                      float *bui_height, int32_t *kadvmax);
+
+const char *gral_pf_setup_last_error(void);
+
 /* ---- Batches of weather situations ------------------------------------
  * GRAL calls Calculate() once per weather situation, one after the other
  * (Program.cs main loop -> MicroscaleFlat.cs:381 / MicroscaleTerrain.cs:847);
