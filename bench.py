@@ -127,6 +127,55 @@ def gff_timing(sv, reps=5):
     return out
 
 
+def batched_workload(k, reps_log=sys.stderr):
+    """The batched workload generator's first k situations (directions in 10 deg
+    steps, speeds 0.5-6 m/s, stability classes 1-7, windows of 200-600 cells,
+    30-50 prognostic layers), each set up on the GPU and solved to the stop
+    test (100-500 iterations): once as one lock-step batch, once one after
+    another with the plain solver.  Wall time covers the set-up, handle
+    creation, prologue, solve and download."""
+    from gral_amd import domain, pf
+    sits = domain.config3_situations()[:k]
+
+    def log(msg):
+        print(f"[batched-workload] {msg}", file=reps_log, flush=True)
+
+    t0 = time.perf_counter()
+    solvers = []
+    for s in sits:
+        c = domain.config3(s)
+        sv = pf.PrognosticFlowfield(c)
+        sv.prepare()
+        solvers.append(sv)
+    t_setup = time.perf_counter() - t0
+    log(f"batch set up ({t_setup:.1f} s), solving {len(solvers)} situations in lock step")
+    with pf.Batch(solvers) as b:
+        st_b = b.solve()
+        tb = b.timing()
+    for sv in solvers:
+        sv.download()
+        sv.close()
+    t_batch = time.perf_counter() - t0
+    log(f"batched {t_batch:.1f} s")
+    st_p = []
+    t0 = time.perf_counter()
+    for q, s in enumerate(sits):
+        c = domain.config3(s)
+        with pf.PrognosticFlowfield(c) as sv:
+            st_p.append(sv.solve())
+            sv.download()
+        log(f"plain situation {q + 1}/{len(sits)} ({time.perf_counter() - t0:.1f} s)")
+    t_plain = time.perf_counter() - t0
+    its_b = [x["iteration_loops"] for x in st_b]
+    same = its_b == [x["iteration_loops"] for x in st_p] and \
+        [x["delta_finish"] for x in st_b] == [x["delta_finish"] for x in st_p]
+    return {"situations": len(sits), "sizes": [s["n"] for s in sits], "layers": [s["vertcellsff"] for s in sits],
+            "batched_per_hour": len(sits) / t_batch * 3600.0, "plain_per_hour": len(sits) / t_plain * 3600.0,
+            "batched_over_plain": t_plain / t_batch, "batched_s": t_batch, "plain_s": t_plain,
+            "batched_setup_s": t_setup, "batched_device_ms": tb["ms_total"], "iterations": its_b,
+            "equal_residual": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +192,8 @@ def main():
                     help="also solve every situation to convergence, batched and plain (about 1-2 minutes)")
     ap.add_argument("--gff", action="store_true",
                     help="also time the flow-field file writer (GPU pack + zip) on the first situation")
+    ap.add_argument("--batched-workload", type=int, default=0, metavar="K",
+                    help="also solve the first K situations of the batched workload, batched and plain")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     args = ap.parse_args()
@@ -248,6 +299,8 @@ def main():
             "kernel_families_ms_per_iter": {k: v["ms"] / 2 for k, v in prof.items()}}
     if gff is not None:
         line["gff_writer"] = gff
+    if args.batched_workload:
+        line["batched_workload"] = batched_workload(args.batched_workload)
     if args.situations_hour:
         line["situations_per_hour"] = situations_per_hour(args.n, directions)
     print(json.dumps(line))

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gral_pf.h"
@@ -56,6 +57,47 @@ inline double dmax_net(double a, double b) {
     return std::signbit(b) ? a : b;
 }
 
+// Host loops over columns or rows run on up to 16 threads: each element is
+// computed as in a serial loop, and the only reductions (a maximum, counts)
+// do not depend on the order.
+unsigned host_threads() {
+    static const unsigned n = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    return n;
+}
+template <class Fn>
+void par_for(long lo, long hi, Fn fn) {            // fn(t, a, b): thread t covers [a, b)
+    const long n = hi - lo;
+    if (n <= 0) return;
+    const long T = std::min<long>(host_threads(), n);
+    if (T <= 1) { fn(0, lo, hi); return; }
+    const long chunk = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    th.reserve((size_t)T - 1);
+    for (long t = 1; t < T; ++t) {
+        const long a = lo + t * chunk, b = std::min(hi, a + chunk);
+        if (a < b) th.emplace_back(fn, (int)t, a, b);
+    }
+    fn(0, lo, std::min(hi, lo + chunk));
+    for (auto &x : th) x.join();
+}
+void par_copy(void *dst, const void *src, size_t bytes) {
+    const size_t blk = size_t(1) << 22;
+    par_for(0, (long)((bytes + blk - 1) / blk), [&](int, long a, long b) {
+        const size_t o = (size_t)a * blk, e = std::min(bytes, (size_t)b * blk);
+        std::memcpy(static_cast<char *>(dst) + o, static_cast<const char *>(src) + o, e - o);
+    });
+}
+void par_fill(std::vector<float> &v, size_t n, float x) {
+    v.resize(n);
+    par_for(0, (long)((n + 1048575) >> 20), [&](int, long a, long b) {
+        std::fill(v.begin() + std::min(n, (size_t)a << 20), v.begin() + std::min(n, (size_t)b << 20), x);
+    });
+}
+void par_assign(std::vector<float> &v, const float *src, size_t n) {
+    v.resize(n);
+    par_copy(v.data(), src, n * sizeof(float));
+}
+
 }  // namespace
 
 struct gral_pf_handle {
@@ -94,6 +136,8 @@ struct gral_pf_handle {
     void *gff_up = nullptr;
     size_t gff_bytes = 0;
     void *gff_dbuf = nullptr, *gff_hbuf = nullptr;   // packed payload: device buffer, pinned host staging
+    void *stage_buf = nullptr;      // pinned staging of one device field (put_field / get_field)
+    size_t stage_bytes = 0;
     size_t gff_dbuf_bytes = 0, gff_hbuf_bytes = 0;
     int gff_nlo = -1;
     bool gff_ok = false;
@@ -200,9 +244,9 @@ extern "C" int gral_pf_upload_fields(gral_pf_handle *h, const float *uk, const f
     if (!h) return GRAL_PF_EINVAL;
     if (!uk || !vk || !wk) return fail(h, GRAL_PF_EINVAL, "upload_fields: null field");
     size_t n3 = h->n2() * (size_t)h->nkh;
-    h->uk.assign(uk, uk + n3);
-    h->vk.assign(vk, vk + n3);
-    h->wk.assign(wk, wk + n3);
+    par_assign(h->uk, uk, n3);
+    par_assign(h->vk, vk, n3);
+    par_assign(h->wk, wk, n3);
     h->fields = true;
     h->prepared = false;
     h->gff_ok = false;
@@ -225,9 +269,9 @@ static int prologue(gral_pf_handle *h, const gral_pf_params *p, std::vector<floa
     const size_t n3 = h->n2() * (size_t)h->nkh;
     const float *DZK = h->dzk.data();
     auto &UK = h->uk, &VK = h->vk, &WK = h->wk;
-    h->uks.assign(n3, 0.0f); h->vks.assign(n3, 0.0f); h->wks.assign(n3, 0.0f);
-    h->dpm.assign(n3, 0.0f); h->dpmnew.assign(n3, 0.0f);
-    if (model != 1) { h->turb.assign(n3, 0.0f); h->tdiss.assign(n3, 0.0f); }
+    par_fill(h->uks, n3, 0.0f); par_fill(h->vks, n3, 0.0f); par_fill(h->wks, n3, 0.0f);
+    par_fill(h->dpm, n3, 0.0f); par_fill(h->dpmnew, n3, 0.0f);
+    if (model != 1) { par_fill(h->turb, n3, 0.0f); par_fill(h->tdiss, n3, 0.0f); }
     else { h->turb.clear(); h->tdiss.clear(); }
     int ri = p->refp_i, rj = p->refp_j;
     float topwind = std::sqrt(p2f(UK[h->i3(ri, rj, NKK - 1)]) + p2f(VK[h->i3(ri, rj, NKK - 1)]));
@@ -239,29 +283,37 @@ static int prologue(gral_pf_handle *h, const gral_pf_params *p, std::vector<floa
     ap0.assign((size_t)h->nkp, 0.0f);
     for (int k = 1; k <= h->kadvmax; ++k) ap0[k] = area * DZK[k] / dtime;
     if (model != 1)
-        for (int i = 1; i <= NII; ++i)
-            for (int j = 1; j <= NJJ; ++j)
-                for (int k = 1; k <= NKK + 1; ++k) { h->turb[h->i3(i, j, k)] = 0.001f; h->tdiss[h->i3(i, j, k)] = 0.0000001f; }
+        par_for(1, NII + 1, [&](int, long a, long b) {
+            for (int i = (int)a; i < b; ++i)
+                for (int j = 1; j <= NJJ; ++j)
+                    for (int k = 1; k <= NKK + 1; ++k) { h->turb[h->i3(i, j, k)] = 0.001f; h->tdiss[h->i3(i, j, k)] = 0.0000001f; }
+        });
     frund = (double)((model == 2 ? 10000.0f : 8000.0f) / topwind);
     // initial cell averages on the 3x3-dilated sub-domain footprint, and GESCHW_MAX
     auto A = [&](int i, int j) { return (int)h->adv[h->i2(i, j)]; };
-    double gmax = 0;
-    for (int i = 1; i <= NII; ++i)
-        for (int j = 1; j <= NJJ; ++j) {
-            int ip = std::min(i + 1, NII + 1), jp = std::min(j + 1, NJJ + 1);
-            bool dil = A(i, j) == 1 || A(i - 1, j) == 1 || A(ip, j) == 1 || A(i, j - 1) == 1 || A(i - 1, j - 1) == 1 ||
-                       A(ip, j - 1) == 1 || A(i, jp) == 1 || A(i - 1, jp) == 1 || A(ip, jp) == 1;
-            if (!dil) continue;
-            const size_t o = h->i3(i, j, 0), oip = h->i3(i + 1, j, 0), ojp = h->i3(i, j + 1, 0);
-            for (int k = 1; k <= NKK - 1; ++k) {
-                float u = 0.5f * (UK[o + k] + UK[oip + k]);
-                float v = 0.5f * (VK[o + k] + VK[ojp + k]);
-                h->uks[o + k] = u;
-                h->vks[o + k] = v;
-                h->wks[o + k] = 0.5f * (WK[o + k] + WK[o + k + 1]);
-                gmax = dmax_net(gmax, std::sqrt((double)(p2f(u) + p2f(v))));
+    std::vector<double> gpart(host_threads(), 0.0);
+    par_for(1, NII + 1, [&](int t, long a, long b) {
+        double gmax = 0;
+        for (int i = (int)a; i < b; ++i)
+            for (int j = 1; j <= NJJ; ++j) {
+                int ip = std::min(i + 1, NII + 1), jp = std::min(j + 1, NJJ + 1);
+                bool dil = A(i, j) == 1 || A(i - 1, j) == 1 || A(ip, j) == 1 || A(i, j - 1) == 1 || A(i - 1, j - 1) == 1 ||
+                           A(ip, j - 1) == 1 || A(i, jp) == 1 || A(i - 1, jp) == 1 || A(ip, jp) == 1;
+                if (!dil) continue;
+                const size_t o = h->i3(i, j, 0), oip = h->i3(i + 1, j, 0), ojp = h->i3(i, j + 1, 0);
+                for (int k = 1; k <= NKK - 1; ++k) {
+                    float u = 0.5f * (UK[o + k] + UK[oip + k]);
+                    float v = 0.5f * (VK[o + k] + VK[ojp + k]);
+                    h->uks[o + k] = u;
+                    h->vks[o + k] = v;
+                    h->wks[o + k] = 0.5f * (WK[o + k] + WK[o + k + 1]);
+                    gmax = dmax_net(gmax, std::sqrt((double)(p2f(u) + p2f(v))));
+                }
             }
-        }
+        gpart[t] = gmax;
+    });
+    double gmax = 0;
+    for (double g : gpart) gmax = dmax_net(gmax, g);
     h->geschw_max = gmax;
     h->vi.assign(h->n2(), 0);
     for (int i = 1; i <= NII; ++i)
@@ -297,7 +349,8 @@ static int prologue(gral_pf_handle *h, const gral_pf_params *p, std::vector<floa
     ug = h->uks[h->i3(ri, rj, NKK - 1)];
     vg = h->vks[h->i3(ri, rj, NKK - 1)];
     // TKE initialisation and the ghost-row copies done in the same loop
-    for (int i = 1; i <= NII; ++i)
+    par_for(1, NII + 1, [&](int, long ia, long ib) {
+    for (int i = (int)ia; i < ib; ++i)
         for (int j = 1; j <= NJJ; ++j) {
             size_t c = h->i2(i, j);
             float ob = h->obi[c], ust = h->usti[c], rough = h->z0i[c];
@@ -315,9 +368,11 @@ static int prologue(gral_pf_handle *h, const gral_pf_params *p, std::vector<floa
                 h->turb[h->i3(i, j, k)] = (float)(0.5 * (double)(p2f(U0int) + p2f(V0int) + varw));
             }
         }
+    });
     ufx = (float)(0.4 / std::log((double)h->dxk * 0.5 / (double)p->building_z0));
     // zero velocities (and k, eps) inside buildings
-    for (int i = 1; i <= NII; ++i)
+    par_for(1, NII + 1, [&](int, long ia, long ib) {
+    for (int i = (int)ia; i < ib; ++i)
         for (int j = 1; j <= NJJ; ++j) {
             size_t c = h->i2(i, j);
             if (h->adv[c] != 1) continue;
@@ -333,27 +388,50 @@ static int prologue(gral_pf_handle *h, const gral_pf_params *p, std::vector<floa
                 h->uks[h->i3(i, j, k)] = 0; h->vks[h->i3(i, j, k)] = 0; h->wks[h->i3(i, j, k)] = 0;
             }
         }
+    });
     h->topwind = topwind;
     return 0;
 }
 
+// pinned host staging of one device field [n2][nkp], kept by the handle
+static float *stage(gral_pf_handle *h) {
+    const size_t n = h->n2() * (size_t)h->nkp * sizeof(float);
+    if (h->stage_bytes < n) {
+        if (h->stage_buf) (void)hipHostFree(h->stage_buf);
+        h->stage_buf = nullptr;
+        h->stage_bytes = 0;
+        if (hipHostMalloc(&h->stage_buf, n, hipHostMallocDefault) != hipSuccess) { h->stage_buf = nullptr; return nullptr; }
+        h->stage_bytes = n;
+    }
+    return static_cast<float *>(h->stage_buf);
+}
+
 // copy levels [0, nkp) of a host full-depth field into a device [.,.,nkp] field
 static int put_field(gral_pf_handle *h, float *dptr, const std::vector<float> &host) {
-    size_t n2 = h->n2();
-    std::vector<float> tmp(n2 * (size_t)h->nkp, 0.0f);
-    int kc = std::min(h->nkp, h->nkh);
-    for (size_t c = 0; c < n2; ++c) std::memcpy(&tmp[c * h->nkp], &host[c * h->nkh], sizeof(float) * kc);
-    HIPCHK(h, hipMemcpyAsync(dptr, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    const size_t n2 = h->n2(), n = n2 * (size_t)h->nkp;
+    float *tmp = stage(h);
+    if (!tmp) return fail(h, GRAL_PF_ENOMEM, "hipHostMalloc of the staging buffer failed");
+    const int kc = std::min(h->nkp, h->nkh);
+    par_for(0, (long)n2, [&](int, long a, long b) {
+        for (size_t c = (size_t)a; c < (size_t)b; ++c) {
+            std::memcpy(tmp + c * h->nkp, &host[c * h->nkh], sizeof(float) * kc);
+            std::fill(tmp + c * h->nkp + kc, tmp + (c + 1) * h->nkp, 0.0f);
+        }
+    });
+    HIPCHK(h, hipMemcpyAsync(dptr, tmp, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
 }
 static int get_field(gral_pf_handle *h, const float *dptr, std::vector<float> &host) {
-    size_t n2 = h->n2();
-    std::vector<float> tmp(n2 * (size_t)h->nkp);
-    HIPCHK(h, hipMemcpyAsync(tmp.data(), dptr, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    const size_t n2 = h->n2(), n = n2 * (size_t)h->nkp;
+    float *tmp = stage(h);
+    if (!tmp) return fail(h, GRAL_PF_ENOMEM, "hipHostMalloc of the staging buffer failed");
+    HIPCHK(h, hipMemcpyAsync(tmp, dptr, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    int kc = std::min(h->nkp, h->nkh);
-    for (size_t c = 0; c < n2; ++c) std::memcpy(&host[c * h->nkh], &tmp[c * h->nkp], sizeof(float) * kc);
+    const int kc = std::min(h->nkp, h->nkh);
+    par_for(0, (long)n2, [&](int, long a, long b) {
+        for (size_t c = (size_t)a; c < (size_t)b; ++c) std::memcpy(&host[c * h->nkh], tmp + c * h->nkp, sizeof(float) * kc);
+    });
     return 0;
 }
 template <class T>
@@ -460,27 +538,31 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
         // PIMP/TERMP of the pressure TDMA for every column, in the reference's
         // row order (fresh row arrays per j row, columns in i1 order for IS=+1 and
         // IS=-1), so the stale top value PIMP[VI+1] is part of the table
-        std::vector<float> tp(n3, 0.0f), tt(n3, 0.0f), prow((size_t)h->nkh + 1, 0.0f);
+        std::vector<float> tp, tt;
         for (int v = 0; v < 2; ++v) {
-            std::fill(tp.begin(), tp.end(), 0.0f);
-            std::fill(tt.begin(), tt.end(), 0.0f);
-            for (int j = 2; j <= h->njj - 1; ++j) {
-                std::fill(prow.begin(), prow.end(), 0.0f);
-                for (int i1 = 2; i1 <= h->nii - 1; ++i1) {
-                    int i = (v == 1) ? h->nii - i1 + 1 : i1;
-                    size_t c = h->i2(i, j);
-                    if (h->adv[c] != 1) continue;
-                    int KK = h->kk[c], VI = h->vi[c];
-                    size_t o = c * (size_t)h->nkp;
-                    for (int k = VI; k > KK; --k) {
-                        float TERMP = 1 / (app[k] - atp[k] * prow[k + 1]);
-                        float P = abp[k] * TERMP;
-                        prow[k] = P;
-                        tp[o + k] = P;
-                        tt[o + k] = TERMP;
+            par_fill(tp, n3, 0.0f);
+            par_fill(tt, n3, 0.0f);
+            // rows are independent: each has its own row arrays
+            par_for(2, h->njj, [&](int, long ja, long jb) {
+                std::vector<float> prow((size_t)h->nkh + 1, 0.0f);
+                for (int j = (int)ja; j < jb; ++j) {
+                    std::fill(prow.begin(), prow.end(), 0.0f);
+                    for (int i1 = 2; i1 <= h->nii - 1; ++i1) {
+                        int i = (v == 1) ? h->nii - i1 + 1 : i1;
+                        size_t c = h->i2(i, j);
+                        if (h->adv[c] != 1) continue;
+                        int KK = h->kk[c], VI = h->vi[c];
+                        size_t o = c * (size_t)h->nkp;
+                        for (int k = VI; k > KK; --k) {
+                            float TERMP = 1 / (app[k] - atp[k] * prow[k + 1]);
+                            float P = abp[k] * TERMP;
+                            prow[k] = P;
+                            tp[o + k] = P;
+                            tt[o + k] = TERMP;
+                        }
                     }
                 }
-            }
+            });
             float *dp, *dt;
             if ((rc = dalloc(h, &dp, n3)) || (rc = dalloc(h, &dt, n3)) || (rc = put_vec(h, dp, tp)) || (rc = put_vec(h, dt, tt)))
                 return rc;
@@ -731,21 +813,26 @@ extern "C" int gral_pf_finish(gral_pf_handle *h, gral_pf_stats *st) {
     int rc = sync_host(h);
     if (rc) return rc;
     // unrealistic wind speeds (PrognosticFlowfield.cs:1192-1243)
-    int nu = 0, nv = 0, nw = 0;
     float limit = (float)(10 * h->geschw_max);
-    for (int i = 1; i <= h->nii; ++i)
-        for (int j = 1; j <= h->njj; ++j) {
-            size_t c = h->i2(i, j);
-            if (h->adv[c] != 1) continue;
-            int kmax = std::min(h->nkk - 1, h->kk[c] + h->vcf);
-            for (int k = 1; k <= kmax; ++k) {
-                float u = std::fabs(h->uk[h->i3(i, j, k)]), v = std::fabs(h->vk[h->i3(i, j, k)]), w = std::fabs(h->wk[h->i3(i, j, k)]);
-                if (u > 55 || u > limit) ++nu;
-                if (v > 55 || v > limit) ++nv;
-                if (w > 20 || w > limit) ++nw;
+    std::vector<int> cnt(3 * (size_t)host_threads(), 0);
+    par_for(1, h->nii + 1, [&](int t, long a, long b) {
+        int nu = 0, nv = 0, nw = 0;
+        for (int i = (int)a; i < b; ++i)
+            for (int j = 1; j <= h->njj; ++j) {
+                size_t c = h->i2(i, j);
+                if (h->adv[c] != 1) continue;
+                int kmax = std::min(h->nkk - 1, h->kk[c] + h->vcf);
+                for (int k = 1; k <= kmax; ++k) {
+                    float u = std::fabs(h->uk[h->i3(i, j, k)]), v = std::fabs(h->vk[h->i3(i, j, k)]), w = std::fabs(h->wk[h->i3(i, j, k)]);
+                    if (u > 55 || u > limit) ++nu;
+                    if (v > 55 || v > limit) ++nv;
+                    if (w > 20 || w > limit) ++nw;
+                }
             }
-        }
-    h->unreal[0] = nu; h->unreal[1] = nv; h->unreal[2] = nw;
+        cnt[3 * t] = nu; cnt[3 * t + 1] = nv; cnt[3 * t + 2] = nw;
+    });
+    h->unreal[0] = h->unreal[1] = h->unreal[2] = 0;
+    for (size_t t = 0; t < cnt.size(); ++t) h->unreal[t % 3] += cnt[t];
     fill_stats(h, st);
     return GRAL_PF_OK;
 }
@@ -767,7 +854,7 @@ extern "C" int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, 
     size_t n3 = h->n2() * (size_t)h->nkh;
     auto cp = [&](float *dst, const std::vector<float> &src) {
         if (!dst) return;
-        if (src.size() == n3) std::memcpy(dst, src.data(), n3 * sizeof(float));
+        if (src.size() == n3) par_copy(dst, src.data(), n3 * sizeof(float));
         else std::memset(dst, 0, n3 * sizeof(float));
     };
     cp(uk, h->uk); cp(vk, h->vk); cp(wk, h->wk); cp(uks, h->uks); cp(vks, h->vks); cp(wks, h->wks);
@@ -1225,6 +1312,7 @@ extern "C" void gral_pf_destroy(gral_pf_handle *h) {
     if (!h) return;
     free_device(h);
     gff_free(h);
+    if (h->stage_buf) (void)hipHostFree(h->stage_buf);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
