@@ -247,6 +247,7 @@ struct DampArgs {
     const uint8_t *adv;
     const float *fac;                    // [3][igeb+1]
     float *uk, *vk;
+    float *wk;                           // also damped when not null (terrain)
 };
 
 __device__ __forceinline__ bool dm_hit(const DampArgs &a, int i, int j, float v) {
@@ -286,6 +287,7 @@ __global__ void k_flat_damp(DampArgs a) {
         const size_t o = c * a.nkh + k;
         a.uk[o] *= ab;
         a.vk[o] *= ab;
+        if (a.wk) a.wk[o] *= ab;
     }
 }
 
@@ -320,6 +322,19 @@ static void profile_level(const gral_pf_flat_met *m, float vertk, float exponent
 static float profile_exponent(float ob) {
     if (ob <= 0) return net_maxf(0.35f - 0.4f * powf(std::fabs(ob), -0.15f), 0.05f);
     return 0.56f * powf(ob, -0.15f);
+}
+
+// wall damping factors 0.19*ln((entf+0.5)*10) per direction class (along i,
+// along j, diagonal) and offset; 1 where the distance exceeds 20 m
+static std::vector<float> damping_table(float dxk, float dyk, int igeb) {
+    std::vector<float> fac(3 * (size_t)(igeb + 1), 1.0f);
+    for (int o = 1; o <= igeb; ++o) {
+        const float dx = (float)o * dxk, dy = (float)o * dyk;
+        const float e3[3] = {std::fabs(dx), std::fabs(dy), std::sqrt(dx * dx + dy * dy)};
+        for (int q = 0; q < 3; ++q)
+            fac[q * (igeb + 1) + o] = (e3[q] <= 20.0f) ? 0.19f * logf((e3[q] + 0.5f) * 10.0f) : 1.0f;
+    }
+    return fac;
 }
 
 extern "C" int gral_pf_flat_init(const gral_pf_grid *g, const gral_pf_flat_met *m, const float *cutk,
@@ -370,15 +385,8 @@ extern "C" int gral_pf_flat_init(const gral_pf_grid *g, const gral_pf_flat_met *
         const float e = profile_exponent(obs[q]);
         for (int k = 1; k <= nkk; ++k) profile_level(m, vertk[k], e, ux[q * nkh + k], uy[q * nkh + k]);
     }
-    // wall damping factors 0.19*ln((entf+0.5)*10) per direction class and offset
     const int igeb = std::max((int)(20.0f / g->dxk), 1);
-    std::vector<float> fac(3 * (size_t)(igeb + 1), 1.0f);
-    for (int o = 1; o <= igeb; ++o) {
-        const float dx = (float)o * g->dxk, dy = (float)o * g->dyk;
-        const float e3[3] = {std::fabs(dx), std::fabs(dy), std::sqrt(dx * dx + dy * dy)};
-        for (int q = 0; q < 3; ++q)
-            fac[q * (igeb + 1) + o] = (e3[q] <= 20.0f) ? 0.19f * logf((e3[q] + 0.5f) * 10.0f) : 1.0f;
-    }
+    const std::vector<float> fac = damping_table(g->dxk, g->dyk, igeb);
     // device work
     Scratch S;
     SETCHK(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
@@ -414,7 +422,7 @@ extern "C" int gral_pf_flat_init(const gral_pf_grid *g, const gral_pf_flat_met *
     hipLaunchKernelGGL(k_flat_column, dim3((unsigned)((size_t)(nii + 1) * (njj + 1))), dim3(64), 0, S.s, fa);
     SETCHK(hipGetLastError());
     if (m->near_wall != 0 && nii > 2 * igeb && njj > 2 * igeb) {
-        DampArgs da{nii, njj, nkk, nkh, nj2, igeb, m->near_wall, d_vk, d_cut, d_ahk, d_kk, d_adv, d_fac, d_u, d_v};
+        DampArgs da{nii, njj, nkk, nkh, nj2, igeb, m->near_wall, d_vk, d_cut, d_ahk, d_kk, d_adv, d_fac, d_u, d_v, nullptr};
         hipLaunchKernelGGL(k_flat_damp, dim3((unsigned)((size_t)(nii - 2 * igeb) * (njj - 2 * igeb))), dim3(64), 0, S.s, da);
         SETCHK(hipGetLastError());
     }
@@ -428,6 +436,339 @@ extern "C" int gral_pf_flat_init(const gral_pf_grid *g, const gral_pf_flat_met *
     SETCHK(hipMemcpyAsync(&kad, d_kad, sizeof(int), hipMemcpyDeviceToHost, S.s));
     SETCHK(hipStreamSynchronize(S.s));
     *kadvmax = std::min(nkk, kad + g->vertcellsff);
+    return GRAL_PF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Terrain (MicroscaleTerrain.cs:57-316): the GRAMM topography and wind field
+// interpolated onto the GRAL grid.  Per inner column: the GRAMM cell (indi,
+// indj) holding the column centre and the offsets xwert, ywert in it; the
+// heights of the GRAMM levels at the centre by 16-point inverse-distance
+// weighting of AHE (interior GRAMM cells) or AHE of the cell itself (border);
+// per GRAL level, the highest GRAMM level whose height plus CUTK lies below
+// the level centre; u, v, w bilinear in the horizontal and linear in the
+// vertical between that GRAMM level and the next; below it the level is
+// blocked (KKART, AHK, BUI_HEIGHT).
+// ---------------------------------------------------------------------------
+struct TerrArgs {
+    int nii, njj, nkk, nkh, nj2;
+    int nx, ny, nz, gy2, gz2, gy1, gz1;  // GRAMM sizes and the strides of AHE [nx+2][ny+2][nz+2] and the [nx+1][ny+1][nz+1] fields
+    float ddx, ddy, dxk, dyk, ahmin;
+    int di, dj;
+    double zlim;                         // 10 + (ZKO[2]-ZKO[1])*0.5
+    const double *vertk;                 // [nkh] HOKART[k-1] + DZK[k]*0.5 + AHMIN (double)
+    const float *hok;
+    const float *cutk;
+    const uint8_t *adv;
+    const float *ahe, *uw, *vw, *ww, *zsp, *ah;
+    float *uk, *vk, *wk;
+    int16_t *kkart;
+    float *ahk, *bui;
+    float *vec;                          // [n2][2] u, v of the level counted for the 10 m wind, or NaN
+    int *kad;
+};
+
+__device__ __forceinline__ float tr_p2(float x) { return x * x; }
+__device__ __forceinline__ float tr_w(float a, float b) { return 1.0f / tr_p2(tr_p2(a) + tr_p2(b) + 0.1f); }
+__device__ __forceinline__ float tr_max(float a, float b) {   // Math.Max(float, float)
+    if (a != b) { if (!__builtin_isnan(a)) return b < a ? a : b; return a; }
+    return __builtin_signbit(b) ? a : b;
+}
+
+#define TERR_NZ 256
+__global__ void __launch_bounds__(64) k_terrain_column(TerrArgs a) {
+    __shared__ float ch[TERR_NZ + 2], chp[TERR_NZ + 2];
+    const int i = 1 + (int)(blockIdx.x / a.njj), j = 1 + (int)(blockIdx.x % a.njj);
+    const size_t c = (size_t)i * a.nj2 + j, o = c * a.nkh;
+    const float DX = a.dxk, DY = a.dyk, X1 = a.ddx, Y1 = a.ddy, AHMIN = a.ahmin;
+    int indi = (int)(((double)((float)a.di + DX * (float)i) - (double)DX * 0.5) / (double)X1) + 1;
+    int indj = (int)(((double)((float)a.dj + DY * (float)j) - (double)DY * 0.5) / (double)Y1) + 1;
+    const float xw = ((DX * (float)i - DX * 0.5f + (float)a.di) - (float)(indi - 1) * X1);
+    const float yw = ((DY * (float)j - DY * 0.5f + (float)a.dj) - (float)(indj - 1) * Y1);
+    const bool interior = indi >= 2 && indi <= a.nx - 1 && indj >= 2 && indj <= a.ny - 1;
+    const int ci = min(max(indi, 1), a.nx), cj = min(max(indj, 1), a.ny);
+    const float *ahe_l = a.ahe + ((size_t)ci * a.gy2 + cj) * a.gz2;
+    if (interior) {
+        const float xd = X1 - xw, yd = Y1 - yw;
+        const float r[16] = {tr_w(xw, yw), tr_w(xw, yd), tr_w(xd, yd), tr_w(xd, yw), tr_w(X1 + xw, Y1 + yw),
+                             tr_w(xw, Y1 + yw), tr_w(xd, Y1 + yw), tr_w(X1 + xd, Y1 + yw), tr_w(X1 + xd, yw),
+                             tr_w(X1 + xd, yd), tr_w(X1 + xd, Y1 + yd), tr_w(xd, Y1 + yd), tr_w(xw, Y1 + yd),
+                             tr_w(X1 + xw, Y1 + yd), tr_w(X1 + xw, yd), tr_w(X1 + xw, yw)};
+        const int di[16] = {0, 0, 1, 1, -1, 0, 1, 2, 2, 2, 2, 1, 0, -1, -1, -1};
+        const int dj[16] = {0, 1, 1, 0, -1, -1, -1, -1, 0, 1, 2, 2, 2, 2, 1, 0};
+        float gew = r[0];
+        for (int q = 1; q < 16; ++q) gew += r[q];
+        for (int ik = 1 + threadIdx.x; ik <= a.nz; ik += 64) {
+            const int ikp = min(ik + 1, a.nz);
+            float s = 0.0f, sp = 0.0f;
+            for (int q = 0; q < 16; ++q) {
+                const float *col = a.ahe + ((size_t)(indi + di[q]) * a.gy2 + (indj + dj[q])) * a.gz2;
+                const float t = col[ik] * r[q], tp = col[ikp] * r[q];
+                s = q ? s + t : t;
+                sp = q ? sp + tp : tp;
+            }
+            ch[ik] = s / gew - AHMIN;
+            chp[ik] = sp / gew - AHMIN;
+        }
+    }
+    __syncthreads();
+    const float cut = a.cutk[c];
+    const float ch1 = interior ? ch[1] : ahe_l[1] - AHMIN;
+    // the highest GRAMM level below a GRAL level centre: returns ik (0: blocked),
+    // with the heights of that GRAMM level and the next and the height above it
+    auto search = [&](int k, float &c_h, float &c_hp, float &dz) {
+        const double vz = a.vertk[k] - (double)AHMIN;
+        for (int ik = a.nz; ik >= 1; --ik) {
+            const int ikp = min(ik + 1, a.nz);
+            c_h = interior ? ch[ik] : ahe_l[ik] - AHMIN;
+            c_hp = interior ? chp[ik] : ahe_l[ikp] - AHMIN;
+            if (vz > (double)(c_h + cut)) {
+                dz = (float)vz - (c_h + cut);
+                return ik;
+            }
+        }
+        return 0;
+    };
+    // pass 1: blocked levels (from the ground up), KKART and AHK
+    int kb = 0;
+    float ahk = AHMIN;
+    for (int k = 1 + threadIdx.x; k <= a.nkk; k += 64) {
+        float c_h, c_hp, dz;
+        if (search(k, c_h, c_hp, dz) == 0) {
+            kb = max(kb, k);
+            ahk = tr_max(a.hok[k] + AHMIN, ahk);
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        kb = max(kb, __shfl_xor(kb, m));
+        const float oa = __shfl_xor(ahk, m);
+        ahk = oa > ahk ? oa : ahk;
+    }
+    // pass 2: u, v, w of the free levels; w of the first free level is zeroed by
+    // the blocked level below it; the highest free level whose GRAMM level is
+    // within about 10 m of the ground counts for the mean wind
+    int vkk = 0;
+    float vu = 0.0f, vv = 0.0f;
+    for (int k = 1 + threadIdx.x; k <= a.nkk; k += 64) {
+        float c_h = 0.0f, c_hp = 0.0f, dz = 0.0f;
+        const int indk = search(k, c_h, c_hp, dz);
+        float u = 0.0f, v = 0.0f, w = 0.0f;
+        if (indk > 0) {
+            const int kp = min(indk + 1, a.nz);
+            const int ixm = (xw < X1 * 0.5f) ? max(ci - 1, 1) : min(ci + 1, a.nx);
+            const int iym = (yw < Y1 * 0.5f) ? max(cj - 1, 1) : min(cj + 1, a.ny);
+            const float fx = xw - X1 * 0.5f, fy = yw - Y1 * 0.5f, sx = (float)(ixm - ci), sy = (float)(iym - cj);
+            const float den = tr_max(c_hp - c_h, 0.1f);
+            auto at = [&](const float *W, int x, int y, int z) { return W[((size_t)x * a.gy1 + y) * a.gz1 + z]; };
+            auto interp = [&](const float *W) {
+                auto hx = [&](int y, int z) {
+                    const float b0 = at(W, ci, y, z);
+                    return b0 + (at(W, ixm, y, z) - b0) / X1 * sx * fx;
+                };
+                const float x1 = hx(cj, indk), x2 = hx(iym, indk), x3 = hx(cj, kp), x4 = hx(iym, kp);
+                const float x5 = x1 + (x3 - x1) / den * dz, x6 = x2 + (x4 - x2) / den * dz;
+                return x5 + (x6 - x5) / Y1 * sy * fy;
+            };
+            u = interp(a.uw);
+            v = interp(a.vw);
+            w = (k == kb + 1 && kb > 0) ? 0.0f : interp(a.ww);
+            const float zz = a.zsp[((size_t)ci * a.gy1 + cj) * a.gz1 + indk] - a.ah[(size_t)ci * a.gy1 + cj];
+            if ((double)zz < a.zlim && k > vkk) { vkk = k; vu = u; vv = v; }
+        }
+        a.uk[o + k] = u;
+        a.vk[o + k] = v;
+        a.wk[o + k] = w;
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        const int ok = __shfl_xor(vkk, m);
+        const float ou = __shfl_xor(vu, m), ov = __shfl_xor(vv, m);
+        if (ok > vkk) { vkk = ok; vu = ou; vv = ov; }
+    }
+    if (threadIdx.x == 0) {
+        a.kkart[c] = (int16_t)kb;
+        a.ahk[c] = ahk;
+        if (kb > 0 && cut > 0) a.bui[c] = tr_max(ahk - AHMIN - ch1, a.bui[c]);
+        if (a.adv[c] > 0) atomicMax(a.kad, kb);
+        a.vec[2 * c] = vkk > 0 ? vu : __builtin_nanf("");
+        a.vec[2 * c + 1] = vkk > 0 ? vv : 0.0f;
+    }
+}
+
+// u at levels up to KKART of the column at i-1, v up to KKART of the column
+// at j-1 (the face of a blocked neighbour); after every column is set
+__global__ void __launch_bounds__(64) k_terrain_faces(TerrArgs a) {
+    const int i = 1 + (int)(blockIdx.x / a.njj), j = 1 + (int)(blockIdx.x % a.njj);
+    const size_t c = (size_t)i * a.nj2 + j, o = c * a.nkh;
+    const int ki = i > 1 ? a.kkart[c - a.nj2] : 0, kj = j > 1 ? a.kkart[c - 1] : 0;
+    for (int k = 1 + threadIdx.x; k <= max(ki, kj) && k <= a.nkk; k += 64) {
+        if (k <= ki) a.uk[o + k] = 0.0f;
+        if (k <= kj) a.vk[o + k] = 0.0f;
+    }
+}
+
+// Ghost row NII+1 of u and column NJJ+1 of v.  Every free cell of row j
+// copies u[NII][j][k] into u[NII+1][j][k] when it is set: the last copy is the
+// final u[NII] when column NII is free at k, else the value u[NII][j][k] had
+// before the set-up (no column set it since), when any column before NII is
+// free at k; else the ghost keeps its value.  Likewise for v along j.
+__global__ void __launch_bounds__(64) k_terrain_ghosts(TerrArgs a, const float *u_old, const float *v_old) {
+    const int q = blockIdx.x;
+    const bool rowu = q < a.njj;                 // blocks 0..NJJ-1: u rows j; then v columns i
+    const int j = rowu ? q + 1 : a.njj, i = rowu ? a.nii : q - a.njj + 1;
+    // lowest KKART among the columns before NII (before NJJ)
+    int mn = 1 << 30;
+    const int nb = rowu ? a.nii - 1 : a.njj - 1;
+    for (int t = threadIdx.x; t < nb; t += 64) {
+        const size_t cc = rowu ? (size_t)(t + 1) * a.nj2 + j : (size_t)i * a.nj2 + (t + 1);
+        mn = min(mn, (int)a.kkart[cc]);
+    }
+    for (int m = 32; m >= 1; m >>= 1) mn = min(mn, __shfl_xor(mn, m));
+    const size_t csrc = (size_t)i * a.nj2 + j;                         // column NII (row u) or NJJ (column v)
+    const size_t cdst = rowu ? (size_t)(a.nii + 1) * a.nj2 + j : (size_t)i * a.nj2 + (a.njj + 1);
+    const int kb = a.kkart[csrc];
+    float *f = rowu ? a.uk : a.vk;
+    const float *old = rowu ? u_old + (size_t)j * a.nkh : v_old + (size_t)i * a.nkh;
+    for (int k = 1 + threadIdx.x; k <= a.nkk; k += 64) {
+        if (k > kb) f[cdst * a.nkh + k] = f[csrc * a.nkh + k];
+        else if (k > mn) f[cdst * a.nkh + k] = old[k];
+    }
+}
+
+__global__ void k_terrain_save(const float *uk, const float *vk, int nii, int njj, int nkh, float *u_old, float *v_old) {
+    const int nj2 = njj + 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nu = (size_t)nj2 * nkh, nv = (size_t)(nii + 2) * nkh;
+    if (t < nu) u_old[t] = uk[(size_t)nii * nj2 * nkh + t];                    // row NII: [j][k]
+    else if (t < nu + nv) {
+        const size_t q = t - nu, i = q / nkh, k = q % nkh;
+        v_old[q] = vk[(i * nj2 + njj) * nkh + k];                              // column NJJ: [i][k]
+    }
+}
+
+extern "C" int gral_pf_terrain_init(const gral_pf_grid *g, const gral_pf_gramm *gr, const float *cutk,
+                                    const uint8_t *advdom, int32_t near_wall, float *uk, float *vk, float *wk,
+                                    int16_t *kkart, float *ahk, float *bui_height, float *ahmin, int32_t *kadvmax,
+                                    double *wind10) {
+    if (!g || !gr || !cutk || !advdom || !uk || !vk || !wk || !kkart || !ahk || !bui_height || !ahmin || !kadvmax ||
+        !g->dzk || !g->hokart)
+        return setfail(GRAL_PF_EINVAL, "terrain_init: null argument");
+    if (g->nii < 1 || g->njj < 1 || g->nkk < 2 || g->vertcellsff < 0 || !(g->dxk > 0) || !(g->dyk > 0))
+        return setfail(GRAL_PF_EINVAL, "terrain_init: bad grid");
+    if (gr->nx < 1 || gr->ny < 1 || gr->nz < 1 || gr->nz > TERR_NZ || !(gr->ddx > 0) || !(gr->ddy > 0) || !gr->ahe ||
+        !gr->uwin || !gr->vwin || !gr->wwin || !gr->zsp || !gr->ah)
+        return setfail(GRAL_PF_EINVAL, "terrain_init: bad GRAMM grid (1 <= nz <= 256)");
+    if (near_wall < 0 || near_wall > 2) return setfail(GRAL_PF_EINVAL, "terrain_init: near_wall must be 0, 1 or 2");
+    const int nii = g->nii, njj = g->njj, nkk = g->nkk, nkh = nkk + 2, nj2 = njj + 2;
+    const size_t n2 = (size_t)(nii + 2) * nj2, n3 = n2 * nkh;
+    const int nx = gr->nx, ny = gr->ny, nz = gr->nz;
+    const size_t nahe = (size_t)(nx + 2) * (ny + 2) * (nz + 2), ngf = (size_t)(nx + 1) * (ny + 1) * (nz + 1),
+                 nah = (size_t)(nx + 1) * (ny + 1);
+    const float AHMIN = *ahmin;
+    std::vector<double> vertk(nkh, 0.0);
+    for (int k = 1; k <= nkk; ++k) vertk[k] = (double)g->hokart[k - 1] + (double)g->dzk[k] * 0.5 + (double)AHMIN;
+    Scratch S;
+    SETCHK(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
+    double *d_vz = S.alloc<double>(nkh);
+    float *d_hok = S.alloc<float>(nkh), *d_cut = S.alloc<float>(n2), *d_ahe = S.alloc<float>(nahe);
+    float *d_uw = S.alloc<float>(ngf), *d_vw = S.alloc<float>(ngf), *d_ww = S.alloc<float>(ngf), *d_zsp = S.alloc<float>(ngf);
+    float *d_ah = S.alloc<float>(nah);
+    float *d_u = S.alloc<float>(n3), *d_v = S.alloc<float>(n3), *d_w = S.alloc<float>(n3);
+    float *d_ahk = S.alloc<float>(n2), *d_bui = S.alloc<float>(n2), *d_vec = S.alloc<float>(2 * n2);
+    float *d_uo = S.alloc<float>((size_t)nj2 * nkh), *d_vo = S.alloc<float>((size_t)(nii + 2) * nkh);
+    int16_t *d_kk = S.alloc<int16_t>(n2);
+    uint8_t *d_adv = S.alloc<uint8_t>(n2);
+    int *d_kad = S.alloc<int>(1);
+    if (!d_vz || !d_hok || !d_cut || !d_ahe || !d_uw || !d_vw || !d_ww || !d_zsp || !d_ah || !d_u || !d_v || !d_w ||
+        !d_ahk || !d_bui || !d_vec || !d_uo || !d_vo || !d_kk || !d_adv || !d_kad)
+        return setfail(GRAL_PF_ENOMEM, "terrain_init: hipMalloc failed");
+    const int one = 1;
+    SETCHK(hipMemcpyAsync(d_vz, vertk.data(), nkh * sizeof(double), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_hok, g->hokart, nkh * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_cut, cutk, n2 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_adv, advdom, n2, hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_ahe, gr->ahe, nahe * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_uw, gr->uwin, ngf * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_vw, gr->vwin, ngf * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_ww, gr->wwin, ngf * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_zsp, gr->zsp, ngf * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_ah, gr->ah, nah * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_u, uk, n3 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_v, vk, n3 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_w, wk, n3 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_kk, kkart, n2 * sizeof(int16_t), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_ahk, ahk, n2 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_bui, bui_height, n2 * sizeof(float), hipMemcpyHostToDevice, S.s));
+    SETCHK(hipMemcpyAsync(d_kad, &one, sizeof(int), hipMemcpyHostToDevice, S.s));
+    TerrArgs ta{};
+    ta.nii = nii; ta.njj = njj; ta.nkk = nkk; ta.nkh = nkh; ta.nj2 = nj2;
+    ta.nx = nx; ta.ny = ny; ta.nz = nz; ta.gy2 = ny + 2; ta.gz2 = nz + 2; ta.gy1 = ny + 1; ta.gz1 = nz + 1;
+    ta.ddx = gr->ddx; ta.ddy = gr->ddy; ta.dxk = g->dxk; ta.dyk = g->dyk; ta.ahmin = AHMIN;
+    ta.di = gr->delta_i; ta.dj = gr->delta_j;
+    ta.zlim = 10.0 + (gr->zko2 - gr->zko1) * (double)0.5f;
+    ta.vertk = d_vz; ta.hok = d_hok; ta.cutk = d_cut; ta.adv = d_adv;
+    ta.ahe = d_ahe; ta.uw = d_uw; ta.vw = d_vw; ta.ww = d_ww; ta.zsp = d_zsp; ta.ah = d_ah;
+    ta.uk = d_u; ta.vk = d_v; ta.wk = d_w; ta.kkart = d_kk; ta.ahk = d_ahk; ta.bui = d_bui; ta.vec = d_vec; ta.kad = d_kad;
+    const size_t nsave = (size_t)nj2 * nkh + (size_t)(nii + 2) * nkh;
+    hipLaunchKernelGGL(k_terrain_save, dim3((unsigned)((nsave + 255) / 256)), dim3(256), 0, S.s, d_u, d_v, nii, njj, nkh,
+                       d_uo, d_vo);
+    const unsigned ncol = (unsigned)((size_t)nii * njj);
+    hipLaunchKernelGGL(k_terrain_column, dim3(ncol), dim3(64), 0, S.s, ta);
+    hipLaunchKernelGGL(k_terrain_faces, dim3(ncol), dim3(64), 0, S.s, ta);
+    hipLaunchKernelGGL(k_terrain_ghosts, dim3((unsigned)(nii + njj)), dim3(64), 0, S.s, ta, (const float *)d_uo,
+                       (const float *)d_vo);
+    SETCHK(hipGetLastError());
+    // AHMIN of the result: the lowest AHK of the inner columns (MicroscaleTerrain.cs:605-612)
+    std::vector<float> h_ahk(n2);
+    SETCHK(hipMemcpyAsync(h_ahk.data(), d_ahk, n2 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipStreamSynchronize(S.s));
+    float amin = 100000.0f;
+    for (int i = 1; i <= nii; ++i)
+        for (int j = 1; j <= njj; ++j) {
+            const float x = h_ahk[(size_t)i * nj2 + j];
+            if (amin != x) amin = (!std::isnan(amin)) ? (amin < x ? amin : x) : amin;
+            else amin = std::signbit(amin) ? amin : x;
+        }
+    const int igeb = std::max((int)(20.0f / g->dxk), 1);
+    if (near_wall != 0 && nii > 2 * igeb && njj > 2 * igeb) {
+        std::vector<float> vf(nkh, 0.0f), fac = damping_table(g->dxk, g->dyk, igeb);
+        for (int k = 1; k <= nkk; ++k) vf[k] = g->hokart[k - 1] + g->dzk[k] * 0.5f + amin;
+        float *d_vf = S.alloc<float>(nkh), *d_fac = S.alloc<float>(fac.size());
+        if (!d_vf || !d_fac) return setfail(GRAL_PF_ENOMEM, "terrain_init: hipMalloc failed");
+        SETCHK(hipMemcpyAsync(d_vf, vf.data(), nkh * sizeof(float), hipMemcpyHostToDevice, S.s));
+        SETCHK(hipMemcpyAsync(d_fac, fac.data(), fac.size() * sizeof(float), hipMemcpyHostToDevice, S.s));
+        DampArgs da{nii, njj, nkk, nkh, nj2, igeb, near_wall, d_vf, d_cut, d_ahk, d_kk, d_adv, d_fac, d_u, d_v, d_w};
+        hipLaunchKernelGGL(k_flat_damp, dim3((unsigned)((size_t)(nii - 2 * igeb) * (njj - 2 * igeb))), dim3(64), 0, S.s, da);
+        SETCHK(hipGetLastError());
+    }
+    int kad = 1;
+    std::vector<float> vec(2 * n2);
+    SETCHK(hipMemcpyAsync(uk, d_u, n3 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(vk, d_v, n3 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(wk, d_w, n3 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(kkart, d_kk, n2 * sizeof(int16_t), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(ahk, d_ahk, n2 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(bui_height, d_bui, n2 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(vec.data(), d_vec, 2 * n2 * sizeof(float), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipMemcpyAsync(&kad, d_kad, sizeof(int), hipMemcpyDeviceToHost, S.s));
+    SETCHK(hipStreamSynchronize(S.s));
+    *kadvmax = std::min(nkk, kad + g->vertcellsff);
+    *ahmin = amin;
+    if (wind10) {
+        // the mean wind near 10 m above ground, summed in column order as a serial loop does
+        double sx = 0, sy = 0;
+        long cnt = 0;
+        for (int i = 1; i <= nii; ++i)
+            for (int j = 1; j <= njj; ++j) {
+                const size_t c = (size_t)i * nj2 + j;
+                if (std::isnan(vec[2 * c])) continue;
+                sx += (double)vec[2 * c];
+                sy += (double)vec[2 * c + 1];
+                ++cnt;
+            }
+        wind10[0] = sx;
+        wind10[1] = sy;
+        wind10[2] = (double)cnt;
+    }
     return GRAL_PF_OK;
 }
 

@@ -27,7 +27,8 @@ EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
            "gral_pf_gff_payload", "gral_pf_gff_write", "gral_pf_zip_write",
-           "gral_pf_subdomains", "gral_pf_reference_point", "gral_pf_flat_init", "gral_pf_setup_last_error",
+           "gral_pf_subdomains", "gral_pf_reference_point", "gral_pf_flat_init", "gral_pf_terrain_init",
+           "gral_pf_setup_last_error",
            "gral_pf_last_error", "gral_pf_destroy", "gral_pf_batch_last_error", "gral_pf_batch_destroy"]
 FAMILIES = ("stencil", "pressure", "u", "v", "w", "tke", "coef")
 
@@ -74,6 +75,14 @@ class FlatMet(C.Structure):
                 ("near_wall", C.c_int32)]
 
 
+class Gramm(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("ddx", C.c_float), ("ddy", C.c_float),
+                ("delta_i", C.c_int32), ("delta_j", C.c_int32), ("ahe", C.POINTER(C.c_float)),
+                ("uwin", C.POINTER(C.c_float)), ("vwin", C.POINTER(C.c_float)), ("wwin", C.POINTER(C.c_float)),
+                ("zsp", C.POINTER(C.c_float)), ("ah", C.POINTER(C.c_float)), ("zko1", C.c_double),
+                ("zko2", C.c_double)]
+
+
 # flow-field file kinds: GRAL_FlowFields.txt writing modes 0, 1, 2 and the *.tke file
 GFF_KINDS = {"mode0": 0, "mode1": 1, "mode2": 2, "tke": 3}
 
@@ -112,6 +121,8 @@ def load(path: str = LIB_PATH):
     lib.gral_pf_reference_point.argtypes = [C.c_int32, C.c_int32, U8P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.gral_pf_flat_init.argtypes = [C.POINTER(Grid), C.POINTER(FlatMet), FP, U8P, FP, FP, FP, I16P, FP, FP,
                                       C.POINTER(C.c_int32)]
+    lib.gral_pf_terrain_init.argtypes = [C.POINTER(Grid), C.POINTER(Gramm), FP, U8P, C.c_int32, FP, FP, FP, I16P, FP,
+                                         FP, FP, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     lib.gral_pf_setup_last_error.argtypes = []
     lib.gral_pf_setup_last_error.restype = C.c_char_p
     for n in ("gral_pf_last_error", "gral_pf_batch_last_error"):
@@ -395,6 +406,34 @@ def flat_init(nii, njj, nkk, vertcellsff, dxk, dyk, dzk, hokart, cutk, met, uk=N
                                             _ptr(out["kkart"], I16P), _ptr(out["ahk"]), _ptr(out["bui_height"]),
                                             C.byref(kad)), "gral_pf_flat_init")
     out["kadvmax"] = kad.value
+    return out
+
+
+def terrain_init(nii, njj, nkk, vertcellsff, dxk, dyk, dzk, hokart, cutk, advdom, gramm, ahmin, uk, vk, wk, kkart,
+                 ahk, bui_height, near_wall=0):
+    """MicroscaleTerrain.Calculate() from a GRAMM field (MicroscaleTerrain.cs:57-316), on the GPU.
+    gramm: dict(nx, ny, nz, ddx, ddy, delta_i, delta_j, ahe, uwin, vwin, wwin, zsp, ah, zko1, zko2).
+    Returns dict(uk, vk, wk, kkart, ahk, bui_height, ahmin, kadvmax, wind10)."""
+    lib = load()
+    out = {"uk": np.array(uk, dtype=np.float32, order="C"), "vk": np.array(vk, dtype=np.float32, order="C"),
+           "wk": np.array(wk, dtype=np.float32, order="C"), "kkart": np.array(kkart, dtype=np.int16, order="C"),
+           "ahk": np.array(ahk, dtype=np.float32, order="C"),
+           "bui_height": np.array(bui_height, dtype=np.float32, order="C")}
+    keep = {n: _f32(gramm[n]) for n in ("ahe", "uwin", "vwin", "wwin", "zsp", "ah")}
+    keep.update(dzk=_f32(dzk), hok=_f32(hokart), cut=_f32(cutk), adv=np.ascontiguousarray(advdom, dtype=np.uint8))
+    g = Grid(nii, njj, nkk, 2, vertcellsff, dxk, dyk, _ptr(keep["dzk"]), _ptr(keep["hok"]))
+    gr = Gramm(int(gramm["nx"]), int(gramm["ny"]), int(gramm["nz"]), float(gramm["ddx"]), float(gramm["ddy"]),
+               int(gramm["delta_i"]), int(gramm["delta_j"]), _ptr(keep["ahe"]), _ptr(keep["uwin"]),
+               _ptr(keep["vwin"]), _ptr(keep["wwin"]), _ptr(keep["zsp"]), _ptr(keep["ah"]), float(gramm["zko1"]),
+               float(gramm["zko2"]))
+    am = C.c_float(float(ahmin))
+    kad = C.c_int32(0)
+    w10 = (C.c_double * 3)()
+    _setup_check(lib, lib.gral_pf_terrain_init(C.byref(g), C.byref(gr), _ptr(keep["cut"]), _ptr(keep["adv"], U8P),
+                                               int(near_wall), _ptr(out["uk"]), _ptr(out["vk"]), _ptr(out["wk"]),
+                                               _ptr(out["kkart"], I16P), _ptr(out["ahk"]), _ptr(out["bui_height"]),
+                                               C.byref(am), C.byref(kad), w10), "gral_pf_terrain_init")
+    out.update(ahmin=np.float32(am.value), kadvmax=kad.value, wind10=(w10[0], w10[1], int(w10[2])))
     return out
 
 

@@ -254,6 +254,36 @@ int gral_pf_flat_init(const gral_pf_grid *grid, const gral_pf_flat_met *met, con
                       const uint8_t *advdom, float *uk, float *vk, float *wk, int16_t *kkart, float *ahk,
                       float *bui_height, int32_t *kadvmax);
 
+/* The GRAMM field a terrain situation starts from: Program.NX/NY/NZ,
+ * DDX[1]/DDY[1], the GRAL grid's offset in the GRAMM grid, AHE, UWIN/VWIN/WWIN,
+ * ZSP, AH and ZKO[1..2] (ProgramFunctions.cs:420-440 gives the shapes). */
+typedef struct {
+    int32_t nx, ny, nz;         /* GRAMM cells */
+    float ddx, ddy;             /* DDX[1], DDY[1] */
+    int32_t delta_i, delta_j;   /* IKOOAGRAL - GrammWest, JKOOAGRAL - GrammSouth */
+    const float *ahe;           /* [nx+2][ny+2][nz+2] heights of the GRAMM cell corners */
+    const float *uwin, *vwin, *wwin; /* [nx+1][ny+1][nz+1] GRAMM wind */
+    const float *zsp;           /* [nx+1][ny+1][nz+1] GRAMM cell-centre heights */
+    const float *ah;            /* [nx+1][ny+1] GRAMM surface height */
+    double zko1, zko2;          /* ZKO[1], ZKO[2] */
+} gral_pf_gramm;
+
+/* MicroscaleTerrain.Calculate() with GRAMM topography up to the flow-field
+ * call (MicroscaleTerrain.cs:57-316, :600-612, damping :664-832): for every
+ * inner column the GRAMM level heights at the column centre (16-point
+ * inverse-distance weighting of AHE), u, v, w interpolated from the GRAMM
+ * wind above terrain plus CUTK, KKART, AHK, BUI_HEIGHT (of building columns),
+ * the ghost copies, KADVMAX over the sub-domain columns, the new AHMIN (in/out:
+ * the value before the call goes in) and the damping near walls of u, v and w
+ * (near_wall as for gral_pf_flat_init).  wind10 (may be NULL) receives the sums
+ * of u and v and their count at about 10 m above ground (the mean GRAMM wind
+ * near the ground, :240-249, :614-628), summed in column order.  All arrays
+ * are in/out; entries the reference does not write keep their values. */
+int gral_pf_terrain_init(const gral_pf_grid *grid, const gral_pf_gramm *gramm, const float *cutk,
+                         const uint8_t *advdom, int32_t near_wall, float *uk, float *vk, float *wk,
+                         int16_t *kkart, float *ahk, float *bui_height, float *ahmin, int32_t *kadvmax,
+                         double *wind10);
+
 const char *gral_pf_setup_last_error(void);
 
 /* ---- Batches of weather situations ------------------------------------
