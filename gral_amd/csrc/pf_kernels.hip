@@ -70,20 +70,30 @@ __global__ void k_zero_rows(const Dom *__restrict__ ds, int both) {
 // Sub-domain boundary conditions, PrognosticFlowfield.cs:595-714
 // ------------------------------------------------------------------------
 // E/W: each row j is independent; within a row the reference scans i upwards
-// and later i read what earlier i wrote, so one thread per (j,k) scans i.
+// and later i read what earlier i wrote, so one thread per (j,k) scans i.  One
+// block per row: the row's ADVDOM and VerticalIndex are staged in LDS first,
+// so the scan reads them from LDS.
 __global__ void k_bc_ew(const Dom *__restrict__ ds) {
+    extern __shared__ int bc_row[];                                                  // [n] VI, then [n] ADVDOM bytes
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     if ((int)blockIdx.y >= d.njj - 1) return;
 
     int j = blockIdx.y + 2;                   // 2..NJJ
-    int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    if (k > d.kadvmax) return;
+    const int n = d.nii + 2;
+    int *svi = bc_row;
+    uint8_t *sadv = reinterpret_cast<uint8_t *>(bc_row + n);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const size_t c = c2(d, i, j);
+        svi[i] = d.vi[c];
+        sadv[i] = d.adv[c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x + 1; k <= d.kadvmax; k += blockDim.x)
     for (int i = 3; i < d.nii; ++i) {
-        size_t c = c2(d, i, j);
-        if (d.adv[c] != 1) continue;
-        if (k > d.vi[c]) continue;
-        bool iM = (d.adv[c2(d, i - 1, j)] < 1) || (i == 3);
-        bool iP = (d.adv[c2(d, i + 1, j)] < 1) || (i == d.nii - 1);
+        if (sadv[i] != 1) continue;
+        if (k > svi[i]) continue;
+        bool iM = (sadv[i - 1] < 1) || (i == 3);
+        bool iP = (sadv[i + 1] < 1) || (i == d.nii - 1);
         if (iM) {
             float u = d.uk[c3(d, i - 2, j) + k];
             if (u >= 0) d.uk[c3(d, i - 1, j) + k] = u;
@@ -100,18 +110,26 @@ __global__ void k_bc_ew(const Dom *__restrict__ ds) {
 }
 
 __global__ void k_bc_ns(const Dom *__restrict__ ds) {
+    extern __shared__ int bc_row[];                                                  // [n] VI, then [n] ADVDOM bytes
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     if ((int)blockIdx.y >= d.nii - 1) return;
 
     int i = blockIdx.y + 2;                   // 2..NII
-    int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    if (k > d.kadvmax) return;
+    const int n = d.njj + 2;
+    int *svi = bc_row;
+    uint8_t *sadv = reinterpret_cast<uint8_t *>(bc_row + n);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const size_t c = c2(d, i, j);
+        svi[j] = d.vi[c];
+        sadv[j] = d.adv[c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x + 1; k <= d.kadvmax; k += blockDim.x)
     for (int j = 3; j <= d.njj - 1; ++j) {
-        size_t c = c2(d, i, j);
-        if (d.adv[c] != 1) continue;
-        if (k > d.vi[c]) continue;
-        bool jM = (d.adv[c2(d, i, j - 1)] < 1) || (j == 3);
-        bool jP = (d.adv[c2(d, i, j + 1)] < 1) || (j == d.njj - 1);
+        if (sadv[j] != 1) continue;
+        if (k > svi[j]) continue;
+        bool jM = (sadv[j - 1] < 1) || (j == 3);
+        bool jP = (sadv[j + 1] < 1) || (j == d.njj - 1);
         if (jM) {
             float v = d.vk[c3(d, i, j - 2) + k];
             if (v >= 0) d.vk[c3(d, i, j - 1) + k] = v;
@@ -2178,10 +2196,12 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         default: break;
     }
     int ncols = (nii - 2) * (njj - 2);
-    const int ncol8 = (ncols + NXCD - 1) / NXCD * NXCD;      // x extent of the XCD-ordered column kernels
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(kb, njj - 1, Z), dim3(64), 0, s, dd); });
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ns, dim3(kb, nii - 1, Z), dim3(64), 0, s, dd); });
+    const size_t bc_shm = (size_t)(max(nii, njj) + 2) * (sizeof(int) + 1);
+    const unsigned bc_threads = (unsigned)min(64 * kb, 1024);
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(1, njj - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ns, dim3(1, nii - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_top, dim3((ncols + 255) / 256, 1, Z), dim3(256), 0, s, dd); });
+    const int ncol8 = (ncols + NXCD - 1) / NXCD * NXCD;      // x extent of the XCD-ordered column kernels
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
     size_t nrow = (size_t)nj2 * nkp;
     for (int q = 0; q < nsit; ++q) nrow = max(nrow, (size_t)4 * hd[q].sync_stride);
