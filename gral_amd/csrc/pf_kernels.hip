@@ -48,6 +48,10 @@ __device__ __forceinline__ int xcd_block(int n) {
     if (idx >= cnt) return -1;
     return x * per + min(x, rem) + idx;
 }
+// The column kernels give a wave COLS_PER_WAVE consecutive columns (in j), in
+// the XCD order above: a column is a few loads per level, so with one column
+// per wave the launch would be bound by the rate at which waves are created.
+#define COLS_PER_WAVE 4
 __global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) p[g] = v;
@@ -143,23 +147,19 @@ __global__ void k_bc_ns(const Dom *__restrict__ ds) {
             d.wk[c3(d, i, j + 1) + k] = d.wk[c3(d, i, j) + k];
         }
     }
-}
-
-// top copy VI -> VI+1 (thread per column i in [2,NII-1], j in [2,NJJ-1])
-__global__ void k_bc_top(const Dom *__restrict__ ds) {
-    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    int ni = d.nii - 2, nj = d.njj - 2;
-    if (g >= ni * nj) return;
-    int i = 2 + g / nj, j = 2 + g % nj;
-    size_t c = c2(d, i, j);
-    if (d.adv[c] != 1) return;
-    int k = d.vi[c];
-    size_t o = c3(d, i, j);
-    d.wk[o + k + 1] = d.wk[o + k];
-    d.uk[o + k + 1] = d.uk[o + k];
-    d.vk[o + k + 1] = d.vk[o + k];
+    // top of the sub-domains, VI -> VI+1 (PrognosticFlowfield.cs:698-714): the
+    // N/S pass of this row has written all it writes of the row's columns, and
+    // no other row writes them
+    if (i > d.nii - 1) return;
+    __syncthreads();
+    for (int j = 2 + threadIdx.x; j <= d.njj - 1; j += blockDim.x) {
+        if (sadv[j] != 1) continue;
+        const int k = svi[j];
+        const size_t o = c3(d, i, j);
+        d.wk[o + k + 1] = d.wk[o + k];
+        d.uk[o + k + 1] = d.uk[o + k];
+        d.vk[o + k + 1] = d.vk[o + k];
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -167,12 +167,14 @@ __global__ void k_bc_top(const Dom *__restrict__ ds) {
 // ------------------------------------------------------------------------
 __global__ void k_divergence(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    const int col = xcd_block((d.nii - 2) * (d.njj - 2));
-    if (col < 0) return;
+    const int ncols = (d.nii - 2) * (d.njj - 2);
+    const int grp = xcd_block((ncols + COLS_PER_WAVE - 1) / COLS_PER_WAVE);
+    if (grp < 0) return;
+    for (int col = grp * COLS_PER_WAVE; col < min(ncols, (grp + 1) * COLS_PER_WAVE); ++col) {
     int nj = d.njj - 2;
     int i = 2 + col / nj, j = 2 + col % nj;
     size_t c = c2(d, i, j);
-    if (d.adv[c] != 1) return;
+    if (d.adv[c] != 1) continue;
     int KK = d.kk[c], VI = d.vi[c];
     size_t o = c3(d, i, j), oip = c3(d, i + 1, j), ojp = c3(d, i, j + 1);
     for (int k = KK + 1 + threadIdx.x; k <= VI; k += blockDim.x) {
@@ -184,6 +186,7 @@ __global__ void k_divergence(const Dom *__restrict__ ds) {
         else v = t1 + t2 - d.wk[o + k + 1] * d.area;
         d.div[o + k] = v;
         d.dpm[o + k] = 0;
+    }
     }
 }
 
@@ -259,12 +262,14 @@ __global__ __launch_bounds__(WAVE) void k_pressure(const Dom *__restrict__ ds, i
 // ------------------------------------------------------------------------
 __global__ void k_correction(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    const int col = xcd_block((d.nii - 2) * (d.njj - 2));
-    if (col < 0) return;
+    const int ncols = (d.nii - 2) * (d.njj - 2);
+    const int grp = xcd_block((ncols + COLS_PER_WAVE - 1) / COLS_PER_WAVE);
+    if (grp < 0) return;
+    for (int col = grp * COLS_PER_WAVE; col < min(ncols, (grp + 1) * COLS_PER_WAVE); ++col) {
     int nj = d.njj - 2;
     int i = 2 + col / nj, j = 2 + col % nj;
     size_t c = c2(d, i, j);
-    if (!(d.adv[c] == 1 || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1)) return;
+    if (!(d.adv[c] == 1 || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1)) continue;
     int KK = d.kk[c], VI = d.vi[c], KKiM = d.kk[c2(d, i - 1, j)], KKjM = d.kk[c2(d, i, j - 1)];
     size_t o = c3(d, i, j), oim = c3(d, i - 1, j), ojm = c3(d, i, j - 1);
     for (int k = 1 + threadIdx.x; k <= VI; k += blockDim.x) {
@@ -284,6 +289,7 @@ __global__ void k_correction(const Dom *__restrict__ ds) {
             d.wk[o + k] = 0.0f;
         }
     }
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -291,12 +297,14 @@ __global__ void k_correction(const Dom *__restrict__ ds) {
 // ------------------------------------------------------------------------
 __global__ void k_averages(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    const int col = xcd_block((d.nii - 2) * (d.njj - 2));
-    if (col < 0) return;
+    const int ncols = (d.nii - 2) * (d.njj - 2);
+    const int grp = xcd_block((ncols + COLS_PER_WAVE - 1) / COLS_PER_WAVE);
+    if (grp < 0) return;
+    for (int col = grp * COLS_PER_WAVE; col < min(ncols, (grp + 1) * COLS_PER_WAVE); ++col) {
     int nj = d.njj - 2;
     int i = 2 + col / nj, j = 2 + col % nj;
     size_t c = c2(d, i, j);
-    if (!(d.adv[c] == 1 || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1)) return;
+    if (!(d.adv[c] == 1 || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1)) continue;
     int KK = d.kk[c], VI = d.vi[c];
     int KSTART = (d.cutk[c] == 0) ? KK + 1 : 1;
     size_t o = c3(d, i, j), oip = c3(d, i + 1, j), ojp = c3(d, i, j + 1);
@@ -311,6 +319,7 @@ __global__ void k_averages(const Dom *__restrict__ ds) {
         if (KK + 1 < k) d.wks[o + k] = 0.5f * (d.wk[o + k] + d.wk[o + k + 1]);
         if (copy_s) d.uks[os + k] = d.uks[o + k];
         if (copy_n) d.uks[on + k] = d.uks[o + k];
+    }
     }
 }
 
@@ -2200,8 +2209,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
     const unsigned bc_threads = (unsigned)min(64 * kb, 1024);
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(1, njj - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ns, dim3(1, nii - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_top, dim3((ncols + 255) / 256, 1, Z), dim3(256), 0, s, dd); });
-    const int ncol8 = (ncols + NXCD - 1) / NXCD * NXCD;      // x extent of the XCD-ordered column kernels
+    const int ncol8 = ((ncols + COLS_PER_WAVE - 1) / COLS_PER_WAVE + NXCD - 1) / NXCD * NXCD;  // x extent of the column kernels
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
     size_t nrow = (size_t)nj2 * nkp;
     for (int q = 0; q < nsit; ++q) nrow = max(nrow, (size_t)4 * hd[q].sync_stride);
