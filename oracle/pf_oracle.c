@@ -21,6 +21,9 @@
  * maxps/minps semantics (a > b ? a : b); Math.Max/Min(float) are the
  * IEEE-754:2019 maximum/minimum of .NET Core.
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "pf_oracle.h"
 #include <math.h>
 #include <stdlib.h>
@@ -65,6 +68,7 @@ typedef struct {
     float *P1, *Q1, *P2, *Q2;         /* column scratch (PIMU/QIMU, PIMTDISS...) */
     orc_capture *cap;                 /* optional checking hook */
     int iter, comp;                   /* current SIMPLE iteration and sweep (for the hook) */
+    int nthreads;                     /* > 1: the timing mode of orc_calculate_threads */
 } St;
 
 static bool cap_on(const St *s, int comp, int i, int j) {
@@ -152,6 +156,7 @@ static void boundary_conditions(St *s) {
 
 /* Mass divergence, PrognosticFlowfield.cs:723-758 */
 static void divergence(St *s) {
+#pragma omp parallel for schedule(static) if (s->nthreads > 1) num_threads(s->nthreads > 1 ? s->nthreads : 1)
     for (int i = 2; i < s->NII; ++i)
         for (int j = 2; j < s->NJJ; ++j) {
             if (ADV(s, i, j) != 1) continue;
@@ -169,17 +174,30 @@ static void divergence(St *s) {
         }
 }
 
+static float pressure_row(St *s, int IS, int JS, int j1, float *PIMP, float *QIMP);
+
 /* Pressure correction GS-TDMA, PrognosticFlowfield.cs:804-901.
  * PIMP/QIMP are per j-row (fresh zeros for each row), reused across the
  * row's columns: the stale top value PIMP[VI+1] is reproduced on purpose. */
 static float pressure(St *s, int IS, int JS) {
-    int NII = s->NII, NJJ = s->NJJ;
-    float DXK = s->DXK, DYK = s->DYK, AREAxy = s->AREAxy, DTIME = s->DTIME;
-    const float *DZK = s->DZK;
+    int NJJ = s->NJJ;
     float DeltaUMax = 0;
     float *PIMP = (float *)calloc((size_t)s->NKK + 2, sizeof(float));
     float *QIMP = (float *)calloc((size_t)s->NKK + 2, sizeof(float));
     for (int j1 = 2; j1 < NJJ; ++j1) {
+        float DeltaUMaxIntern = pressure_row(s, IS, JS, j1, PIMP, QIMP);
+        if (DeltaUMaxIntern > DeltaUMax) DeltaUMax = DeltaUMaxIntern;
+    }
+    free(PIMP); free(QIMP);
+    return DeltaUMax;
+}
+
+/* one j row of the pressure sweep with fresh row arrays PIMP/QIMP */
+static float pressure_row(St *s, int IS, int JS, int j1, float *PIMP, float *QIMP) {
+    int NII = s->NII, NJJ = s->NJJ;
+    float DXK = s->DXK, DYK = s->DYK, AREAxy = s->AREAxy, DTIME = s->DTIME;
+    const float *DZK = s->DZK;
+    {
         memset(PIMP, 0, sizeof(float) * ((size_t)s->NKK + 2));
         memset(QIMP, 0, sizeof(float) * ((size_t)s->NKK + 2));
         float DeltaUMaxIntern = 0;
@@ -215,15 +233,14 @@ static float pressure(St *s, int IS, int JS) {
             }
             cap_column(s, 3, i, j, DPM, true);
         }
-        if (DeltaUMaxIntern > DeltaUMax) DeltaUMax = DeltaUMaxIntern;
+        return DeltaUMaxIntern;
     }
-    free(PIMP); free(QIMP);
-    return DeltaUMax;
 }
 
 /* Velocity correction with FP64 quantisation, PrognosticFlowfield.cs:904-949 */
 static void correction(St *s) {
     double Frund = s->Frund;
+#pragma omp parallel for schedule(static) if (s->nthreads > 1) num_threads(s->nthreads > 1 ? s->nthreads : 1)
     for (int i = 2; i < s->NII; ++i)
         for (int j = 2; j <= s->NJJ - 1; ++j) {
             if (!(ADV(s, i, j) == 1 || ADV(s, i - 1, j) == 1 || ADV(s, i, j - 1) == 1)) continue;
@@ -254,6 +271,7 @@ static void correction(St *s) {
 /* Cell-centred averages and edge copies, PrognosticFlowfield.cs:958-1039 */
 static void averages(St *s) {
     int NII = s->NII, NJJ = s->NJJ;
+#pragma omp parallel for schedule(static) if (s->nthreads > 1) num_threads(s->nthreads > 1 ? s->nthreads : 1)
     for (int i = 2; i < NII; ++i)
         for (int j = 2; j <= NJJ - 1; ++j) {
             if (!(ADV(s, i, j) == 1 || ADV(s, i - 1, j) == 1 || ADV(s, i, j - 1) == 1)) continue;
@@ -665,10 +683,14 @@ static void w1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
 }
 
 /* U_PrognosticMicroscaleV1.Calculate(IS, JS): i1 in [3,NII-1], j1 in [2,NJJ-1] */
+static void u1_row(St *s, int IS, int JS, int i1);
 static void u1_sweep(St *s, int IS, int JS) {
     s->comp = 0;
+    for (int i1 = 3; i1 < s->NII; ++i1) u1_row(s, IS, JS, i1);
+}
+static void u1_row(St *s, int IS, int JS, int i1) {
     uctx c = { s->UG };
-    for (int i1 = 3; i1 < s->NII; ++i1) {
+    {
         int i = (IS == -1) ? s->NII - i1 + 1 : i1;
         for (int j1 = 2; j1 <= s->NJJ - 1; ++j1) {
             int j = (JS == -1) ? s->NJJ - j1 + 1 : j1;
@@ -687,10 +709,14 @@ static void u1_sweep(St *s, int IS, int JS) {
 }
 
 /* V_PrognosticMicroscaleV1.Calculate(IS, JS): i1 in [2,NII-1], j1 in [3,NJJ-1] */
+static void v1_row(St *s, int IS, int JS, int i1);
 static void v1_sweep(St *s, int IS, int JS) {
-    uctx c = { s->VG };
     s->comp = 1;
-    for (int i1 = 2; i1 < s->NII; ++i1) {
+    for (int i1 = 2; i1 < s->NII; ++i1) v1_row(s, IS, JS, i1);
+}
+static void v1_row(St *s, int IS, int JS, int i1) {
+    uctx c = { s->VG };
+    {
         int i = (IS == -1) ? s->NII - i1 + 1 : i1;
         for (int j1 = 3; j1 <= s->NJJ - 1; ++j1) {
             int j = (JS == -1) ? s->NJJ - j1 + 1 : j1;
@@ -709,9 +735,13 @@ static void v1_sweep(St *s, int IS, int JS) {
 }
 
 /* W_PrognosticMicroscaleV1.Calculate(IS, JS): i1 in [2,NII-1], j1 in [2,NJJ-1] */
+static void w1_row(St *s, int IS, int JS, int i1);
 static void w1_sweep(St *s, int IS, int JS) {
     s->comp = 2;
-    for (int i1 = 2; i1 < s->NII; ++i1) {
+    for (int i1 = 2; i1 < s->NII; ++i1) w1_row(s, IS, JS, i1);
+}
+static void w1_row(St *s, int IS, int JS, int i1) {
+    {
         int i = (IS == -1) ? s->NII - i1 + 1 : i1;
         for (int j1 = 2; j1 < s->NJJ; ++j1) {
             int j = (JS == -1) ? s->NJJ - j1 + 1 : j1;
@@ -725,6 +755,60 @@ static void w1_sweep(St *s, int IS, int JS) {
                 WK[k] += s->relax * (s->P1[k] * WK[k + 1] + s->Q1[k] - WK[k]);
             cap_column(s, 2, i, j, WK, true);
         }
+    }
+}
+
+/* ---- timing mode: the reference's default multi-threaded schedule ------
+ * Not deterministic and never compared: the rows of a sweep are dealt to
+ * threads that update the fields in place while their neighbours do
+ * (U-prognostic-microscale_1.cs:59-61 partitions i; the pressure sweep runs
+ * its j rows in parallel, PrognosticFlowfield.cs:806-811), and U and V run
+ * at the same time (PrognosticFlowfield.cs:1097-1101).  Each thread has its
+ * own column scratch. */
+static void par_scratch(St *t, const St *s) {
+    *t = *s;
+    t->cap = NULL;
+    t->P1 = (float *)calloc((size_t)s->NKP, sizeof(float)); t->Q1 = (float *)calloc((size_t)s->NKP, sizeof(float));
+    t->P2 = (float *)calloc((size_t)s->NKP, sizeof(float)); t->Q2 = (float *)calloc((size_t)s->NKP, sizeof(float));
+}
+static void par_free(St *t) { free(t->P1); free(t->Q1); free(t->P2); free(t->Q2); }
+
+static float pressure_par(St *s, int IS, int JS) {
+    float DeltaUMax = 0;
+#pragma omp parallel num_threads(s->nthreads)
+    {
+        float *PIMP = (float *)calloc((size_t)s->NKK + 2, sizeof(float));
+        float *QIMP = (float *)calloc((size_t)s->NKK + 2, sizeof(float));
+        float mine = 0;
+#pragma omp for schedule(static)
+        for (int j1 = 2; j1 < s->NJJ; ++j1) {
+            float d = pressure_row(s, IS, JS, j1, PIMP, QIMP);
+            if (d > mine) mine = d;
+        }
+#pragma omp critical
+        if (mine > DeltaUMax) DeltaUMax = mine;
+        free(PIMP); free(QIMP);
+    }
+    return DeltaUMax;
+}
+
+static void ml_sweeps_par(St *s, int IS, int JS) {
+    const int nu = s->NII - 3, nv = s->NII - 2;          /* U rows i1 = 3..NII-1, V rows i1 = 2..NII-1 */
+#pragma omp parallel num_threads(s->nthreads)
+    {
+        St t;
+        par_scratch(&t, s);
+        const int nmax = nu > nv ? nu : nv;
+#pragma omp for schedule(dynamic, 4)
+        for (int q = 0; q < 2 * nmax; ++q) {              /* U and V rows interleaved */
+            const int r = q / 2;
+            if (q % 2 == 0) { if (r < nu) u1_row(&t, IS, JS, 3 + r); }
+            else if (r < nv) v1_row(&t, -IS, -JS, 2 + r);
+        }
+#pragma omp barrier
+#pragma omp for schedule(static)
+        for (int i1 = 2; i1 < s->NII; ++i1) w1_row(&t, IS, JS, i1);
+        par_free(&t);
     }
 }
 
@@ -1134,19 +1218,25 @@ float orc_powlaw_vector(float d, float peclet) { return ml_pow(d, peclet); }
 float orc_powlaw_scalar(float d, float peclet) { return pow_d(d, peclet); }
 int orc_conv_to_int(double x) { return conv_to_int_d(x); }
 
-static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap);
+static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap,
+                     int nthreads);
+
+int orc_calculate_threads(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, int nthreads) {
+    return calculate(p, in, f, st, NULL, nthreads < 1 ? 1 : nthreads);
+}
 
 int orc_calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st) {
-    return calculate(p, in, f, st, NULL);
+    return calculate(p, in, f, st, NULL, 1);
 }
 
 int orc_calculate_capture(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap) {
     if (cap && (!cap->a || !cap->b || !cap->c || !cap->d || !cap->x_old || !cap->x_new)) return -7;
     if (cap) cap->captured = 0;
-    return calculate(p, in, f, st, cap);
+    return calculate(p, in, f, st, cap, 1);
 }
 
-static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap) {
+static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap,
+                     int nthreads) {
     if (!p || !in || !f || !st) return -1;
     if (p->simd_width != 8 && p->simd_width != 16) return -2;
     if (p->turbulence_model < 0 || p->turbulence_model > 2) return -3;
@@ -1155,6 +1245,7 @@ static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, o
     St S, *s = &S;
     memset(s, 0, sizeof(S));
     s->cap = cap;
+    s->nthreads = nthreads;
     s->NII = p->nii; s->NJJ = p->njj; s->NKK = p->nkk; s->KADVMAX = p->kadvmax; s->VCF = p->vertcellsff;
     s->NJ2 = p->njj + 2; s->NKP = p->nkk + 2; s->W = p->simd_width; s->model = p->turbulence_model;
     s->DXK = p->dxk; s->DYK = p->dyk; s->relax = p->relax; s->relaxp = p->relaxp;
@@ -1292,13 +1383,15 @@ static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, o
             case 2: IS = 1; JS = -1; break;
             default: break;
         }
-        float DeltaUMax = pressure(s, IS, JS);
+        float DeltaUMax = (s->nthreads > 1) ? pressure_par(s, IS, JS) : pressure(s, IS, JS);
         correction(s);
         averages(s);
         if (s->model == 0) {
             uv2_sweep(s, IS, JS, 0, 0);
             uv2_sweep(s, IS, JS, 1, 0);
             w2_sweep(s, IS, JS, 0);
+        } else if (s->model == 1 && s->nthreads > 1) {
+            ml_sweeps_par(s, IS, JS);
         } else if (s->model == 1) {
             u1_sweep(s, IS, JS);
             v1_sweep(s, -IS, -JS);

@@ -72,6 +72,12 @@ typedef struct {
  * Returns 0 on success, negative on invalid input. */
 int orc_calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st);
 
+/* Timing mode: the same solve with the reference's default multi-threaded
+ * schedule (rows of each sweep dealt to nthreads threads updating the fields
+ * in place, U and V at the same time).  Not deterministic; used only as the
+ * CPU baseline of the benchmark, never as a checker. */
+int orc_calculate_threads(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, int nthreads);
+
 /* DiagnosticFlowfield.Calculate() (DiagnosticFlowfield.cs:24-436) followed by
  * the final mass adjustment of the vertical velocity (MicroscaleFlat.cs:388-462,
  * or MicroscaleTerrain.cs:862-933 with terrain = 1).

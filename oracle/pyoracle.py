@@ -77,6 +77,9 @@ def load(build_if_missing: bool = True):
     _lib.orc_calculate_capture.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcInputs), C.POINTER(OrcFields),
                                            C.POINTER(OrcStats), C.POINTER(OrcCapture)]
     _lib.orc_calculate_capture.restype = C.c_int
+    _lib.orc_calculate_threads.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcInputs), C.POINTER(OrcFields),
+                                           C.POINTER(OrcStats), C.c_int]
+    _lib.orc_calculate_threads.restype = C.c_int
     for n in ("orc_powlaw_vector", "orc_powlaw_scalar"):
         getattr(_lib, n).argtypes = [C.c_float, C.c_float]
         getattr(_lib, n).restype = C.c_float
@@ -94,10 +97,12 @@ def _p(a, t=FP):
     return a.ctypes.data_as(t)
 
 
-def run(case, iterations=None, simd_width=None, keep_history=True, capture=None):
+def run(case, iterations=None, simd_width=None, keep_history=True, capture=None, threads=1):
     """Run PrognosticFlowfield.Calculate on ``case`` (a gral_amd.domain.Case)
     with the deterministic CPU oracle.  Returns a dict of output fields and stats.
-    The input case is not modified."""
+    The input case is not modified.  threads > 1 selects the timing mode: the
+    reference's default multi-threaded schedule, not deterministic (CPU
+    baseline only, never a checker)."""
     lib = load()
     c = case
     nit = c.iter_max if iterations is None else iterations
@@ -133,6 +138,8 @@ def run(case, iterations=None, simd_width=None, keep_history=True, capture=None)
         capbuf = {n: np.zeros(nk, dtype=np.float32) for n in ("a", "b", "c", "d", "x_old", "x_new")}
         cap = OrcCapture(comp, ci, cj, cit, *[_p(capbuf[n]) for n in ("a", "b", "c", "d", "x_old", "x_new")], 0)
         rc = lib.orc_calculate_capture(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st), C.byref(cap))
+    elif threads > 1:
+        rc = lib.orc_calculate_threads(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st), int(threads))
     else:
         rc = lib.orc_calculate(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st))
     if rc != 0:
