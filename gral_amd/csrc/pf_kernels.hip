@@ -1495,8 +1495,12 @@ struct PrPre {
 #define SYNC_HDR 2
 // a strip publishes its progress every STRIP_PUBLISH steps (and at its end): the
 // release (L2 write-back) and the reader's acquire (L2 invalidate) are paid once
-// per STRIP_PUBLISH steps, for a lag of up to STRIP_PUBLISH-1 more steps per strip
+// per STRIP_PUBLISH steps, for a lag of up to STRIP_PUBLISH-1 more steps per strip.
+// Measured on the 8-direction city batch (ms per launch, publishing every 1 / 2 /
+// 4 / 6 / 8 steps): U and V sweeps 7.6-8.2 / 5.4-5.5 / 4.29 / 4.10 / 4.1, W 7.7 /
+// 5.2 / 4.23 / 4.29 / 4.47, pressure 6.9 / 4.6 / 3.87 / 3.95 / 3.94.
 #define STRIP_PUBLISH 4
+template <int COMP> __device__ __forceinline__ constexpr int ml_publish() { return COMP == 2 ? 4 : 6; }
 #define STRIP_TIMEOUT_TICKS 200000000ull
 
 __device__ __forceinline__ int load_agent(const gi32 *p) {
@@ -1741,7 +1745,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         // later at the earliest (for the last row, the next strip, which waits
         // for this one); row 0 also reads the previous strip's last row, after
         // its wait.
-        if (r == ROWS - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);
+        if (r == ROWS - 1 && s > 0 && s % ml_publish<COMP>() == 0) strip_publish(mine, s, lane);
         locate(s + 1);
         if (r == 0) {
             sw.wait(s + 1 + ROWS);
