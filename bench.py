@@ -27,10 +27,14 @@ sys.path.insert(0, ROOT)
 
 # algorithmic FP32 bytes per active cell for one pass of each sweep family:
 # every array the pass reads or writes counted once (pressure: DIV, DPM, DPMNEW
-# read; DPM, DPMNEW written; momentum (mixing length): 7 arrays read, 1 written)
-ALG_BYTES = {"pressure": 5 * 4, "u": 8 * 4, "v": 8 * 4, "w": 8 * 4}
+# read; DPM, DPMNEW written; momentum (mixing length): 7 arrays read, 1 written;
+# k-eps momentum: 7 read (TURB instead of VEG), 1 written; TKE: 5 read, 2 written)
+ALG_BYTES = {"pressure": 5 * 4, "u": 8 * 4, "v": 8 * 4, "w": 8 * 4, "tke": 7 * 4}
 KERNEL_OF = {"pressure": "k_pressure_strip", "u": "k_ml_strip<0>", "v": "k_ml_strip<1>", "w": "k_ml_strip<2>"}
+KERNEL_OF_KEPS = {"pressure": "k_pressure_strip", "u": "k_momentum<0, 1>", "v": "k_momentum<1, 1>",
+                  "w": "k_momentum<2, 1>", "tke": "k_tke"}
 ALG_BYTES_ITER = 188          # whole SIMPLE iteration, mixing-length model
+ALG_BYTES_ITER_KEPS = 212     # k-eps model (53 array accesses)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -194,6 +198,8 @@ def main():
                     help="also time the flow-field file writer (GPU pack + zip) on the first situation")
     ap.add_argument("--batched-workload", type=int, default=0, metavar="K",
                     help="also solve the first K situations of the batched workload, batched and plain")
+    ap.add_argument("--turbulence-model", type=int, default=1, choices=(1, 2),
+                    help="1: mixing length (the benchmark's model), 2: k-eps")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     args = ap.parse_args()
@@ -216,7 +222,10 @@ def main():
 
     total = args.warmup + args.steps + 16
     directions = [45.0 * q for q in range(args.directions)]
-    cases = [domain.config2(direction=d, iterations=total, n=args.n) for d in directions]
+    model = args.turbulence_model
+    kernel_of = KERNEL_OF if model == 1 else KERNEL_OF_KEPS
+    alg_iter = ALG_BYTES_ITER if model == 1 else ALG_BYTES_ITER_KEPS
+    cases = [domain.config2(direction=d, iterations=total, n=args.n, turbulence_model=model) for d in directions]
     solvers = [pf.PrognosticFlowfield(c) for c in cases]
     for sv in solvers:
         sv.prepare(iterations=total)
@@ -251,11 +260,11 @@ def main():
     # counter passes committed with the profiles (same workload and kernel version)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_city400_8dir.json")
-    if os.path.exists(tpath) and args.n == 400 and args.directions == 8:
+    if os.path.exists(tpath) and args.n == 400 and args.directions == 8 and model == 1:
         kt = json.load(open(tpath))["kernels"].get(KERNEL_OF[dom])
         if kt:
             traffic = kt["traffic_bytes"]
-    roofline = {"bound": "hbm", "kernel": KERNEL_OF[dom], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    roofline = {"bound": "hbm", "kernel": kernel_of[dom], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "avg_launch_us": avg_ms * 1e3, "launches_per_iter": pdom["launches"] / 2,
                 "alg_bytes_per_cell": ALG_BYTES[dom], "alg_bytes_per_launch": bytes_per_launch}
@@ -263,7 +272,7 @@ def main():
     # the plain single-situation solve (270 deg), timed the same way
     plain = None
     if not args.no_plain:
-        c270 = domain.config2(direction=270.0, iterations=total, n=args.n)
+        c270 = domain.config2(direction=270.0, iterations=total, n=args.n, turbulence_model=model)
         with pf.PrognosticFlowfield(c270) as sv:
             sv.prepare(iterations=total)
             if args.warmup:
@@ -274,7 +283,7 @@ def main():
                  "direction_deg": 270.0, "active_cells": tp["active_cells"]}
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and model == 1:   # the timing mode threads the mixing-length sweeps only
         from oracle import pyoracle
         c0 = domain.config2(direction=270.0, iterations=args.cpu_iterations, n=args.n)
 
@@ -304,11 +313,11 @@ def main():
             "config": {"workload": f"city block {cases[0].nii}x{cases[0].njj}, dx=2 m, 40 prognostic layers, "
                                    f"30% buildings (seed 20260001), {len(cases)} wind directions "
                                    f"({', '.join(f'{d:g}' for d in directions)} deg) as one lock-step batch, "
-                                   f"plain solver per situation, mixing-length model",
+                                   f"plain solver per situation, {'mixing-length' if model == 1 else 'k-eps'} model",
                        "situations": len(cases), "active_cells": tm["active_cells"], "kadvmax": cases[0].kadvmax,
                        "parallelism": "single GPU"},
-            "wall_s_timed": wall, "alg_bytes_per_cell_iter": ALG_BYTES_ITER,
-            "iteration_frac_of_hbm_roofline": value * ALG_BYTES_ITER / (HBM_PEAK_GBS * 1e9),
+            "wall_s_timed": wall, "alg_bytes_per_cell_iter": alg_iter,
+            "iteration_frac_of_hbm_roofline": value * alg_iter / (HBM_PEAK_GBS * 1e9),
             "roofline": roofline, "cpu_baseline": cpu, "plain_single_situation": plain,
             "kernel_families_ms_per_iter": {k: v["ms"] / 2 for k, v in prof.items()}}
     if gff is not None:
