@@ -516,11 +516,12 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     if (h->kadvmax + 2 > 128) d.tiled = 0;   // the strip kernels hold at most 128 levels per column
     {
         // strip synchronisation: header + one block per sweep kind (pressure, U, V, W)
-        int nstrip = (std::max(h->nii, h->njj) + STRIP_ROWS - 1) / STRIP_ROWS + 1;
+        const int rows = std::min(STRIP_ROWS, TKE_ROWS);
+        int nstrip = (std::max(h->nii, h->njj) + rows - 1) / rows + 1;
         d.sync_stride = ((2 + nstrip) + 63) & ~63;
         int *fs;
-        if ((rc = dalloc(h, &fs, (size_t)SYNC_BLOCK0 + 4 * (size_t)d.sync_stride))) return rc;
-        HIPCHK(h, hipMemsetAsync(fs, 0, sizeof(int) * ((size_t)SYNC_BLOCK0 + 4 * (size_t)d.sync_stride), h->stream));
+        if ((rc = dalloc(h, &fs, (size_t)SYNC_BLOCK0 + 5 * (size_t)d.sync_stride))) return rc;
+        HIPCHK(h, hipMemsetAsync(fs, 0, sizeof(int) * ((size_t)SYNC_BLOCK0 + 5 * (size_t)d.sync_stride), h->stream));
         d.sync = fs;
     }
     d.planes = nullptr;
@@ -528,6 +529,11 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     d.wcol = nullptr;
     d.ptab[0] = d.ptab[1] = d.ttab[0] = d.ttab[1] = nullptr;
     if (d.tiled) {
+        if (p->turbulence_model == 2) {
+            float *pl;                           // TKE records of the strip-pipelined TKE sweep
+            if ((rc = dalloc(h, &pl, 20 * n3))) return rc;
+            d.planes = pl;
+        }
         if (p->turbulence_model == 1) {
             float *pl;
             if ((rc = dalloc(h, &pl, 15 * n3 + n2))) return rc;

@@ -103,6 +103,7 @@ struct GffArgs {
 
 #define SYNC_BLOCK0 64               // Dom::sync: sticky error flag at 0, sweep blocks from here
 #define STRIP_ROWS 15                // sweep rows per strip workgroup (plus one chain wave)
+#define TKE_ROWS 8                   // sweep rows per workgroup of the strip-pipelined TKE sweep
 
 __device__ __forceinline__ size_t c2(const DomG &d, int i, int j) { return (size_t)i * (size_t)d.nj2 + (size_t)j; }
 __device__ __forceinline__ size_t c3(const DomG &d, int i, int j) { return c2(d, i, j) * (size_t)d.nkp; }

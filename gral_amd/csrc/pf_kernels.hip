@@ -67,7 +67,7 @@ __global__ void k_zero_rows(const Dom *__restrict__ ds, int both) {
         d.prow_q[g] = 0.0f;
         if (both) d.prow_p[g] = 0.0f;
     }
-    if (d.sync && g < (size_t)4 * d.sync_stride) d.sync[SYNC_BLOCK0 + g] = 0;
+    if (d.sync && g < (size_t)5 * d.sync_stride) d.sync[SYNC_BLOCK0 + g] = 0;
 }
 
 // ------------------------------------------------------------------------
@@ -931,6 +931,155 @@ __global__ __launch_bounds__(WAVE) void k_momentum(const Dom *__restrict__ ds, i
 // ------------------------------------------------------------------------
 // TKE / dissipation (k-eps), TKE-prognostic-microscale.cs:25-352
 // ------------------------------------------------------------------------
+// Per-column pointers and neighbour data of the TKE sweep, and the per-level
+// coefficients (TKE-prognostic-microscale.cs:100-337) that the sweep's
+// in-place neighbours (TURB, TDISS of the four neighbour columns) do not touch.
+struct TkeCol {
+    size_t o, c;
+    const gf32 *TURB, *TDISS, *UK, *VK, *WK;
+    const gf32 *UKip, *UKjp, *UKipjp, *UKjm, *UKipjm, *VKjp, *VKip, *VKipjp, *VKim, *VKimjp, *WKip, *WKim, *WKjp, *WKjm;
+    int kkip, kkim, kkjp, kkjm;
+    bool cip, cim, cjp, cjm;
+};
+__device__ __forceinline__ TkeCol tke_col(const DomG &d, int i, int j) {
+    TkeCol t;
+    t.c = c2(d, i, j);
+    t.o = c3(d, i, j);
+    t.TURB = d.turb + t.o; t.TDISS = d.tdiss + t.o;
+    t.UK = d.uk + t.o; t.VK = d.vk + t.o; t.WK = d.wk + t.o;
+    t.UKip = d.uk + c3(d, i + 1, j); t.UKjp = d.uk + c3(d, i, j + 1); t.UKipjp = d.uk + c3(d, i + 1, j + 1);
+    t.UKjm = d.uk + c3(d, i, j - 1); t.UKipjm = d.uk + c3(d, i + 1, j - 1);
+    t.VKjp = d.vk + c3(d, i, j + 1); t.VKip = d.vk + c3(d, i + 1, j); t.VKipjp = d.vk + c3(d, i + 1, j + 1);
+    t.VKim = d.vk + c3(d, i - 1, j); t.VKimjp = d.vk + c3(d, i - 1, j + 1);
+    t.WKip = d.wk + c3(d, i + 1, j); t.WKim = d.wk + c3(d, i - 1, j);
+    t.WKjp = d.wk + c3(d, i, j + 1); t.WKjm = d.wk + c3(d, i, j - 1);
+    t.kkip = d.kk[c2(d, i + 1, j)]; t.kkim = d.kk[c2(d, i - 1, j)];
+    t.kkjp = d.kk[c2(d, i, j + 1)]; t.kkjm = d.kk[c2(d, i, j - 1)];
+    t.cip = d.cutk[c2(d, i + 1, j)] > 0; t.cim = d.cutk[c2(d, i - 1, j)] > 0;
+    t.cjp = d.cutk[c2(d, i, j + 1)] > 0; t.cjm = d.cutk[c2(d, i, j - 1)] > 0;
+    return t;
+}
+
+struct TkeCell {
+    float A, B1, C1, B2, C2, AW1, AS1, AE1, AN1;
+    float w1, PRO, PBO, w2, EPS;      // ap0*TURB, production, buoyancy, ap0*TDISS, eps source
+    int wf;                           // wall function replaces the back-substitution
+    float wt, we;
+};
+
+__device__ __forceinline__ TkeCell tke_cell(const DomG &d, const ColInfo &q, const TkeCol &t, int k) {
+    const float DXK = d.dxk, DYK = d.dyk, AREA = d.area, DXK_Rez = 1 / DXK, DYK_Rez = 1 / DYK;
+    const float Ceps = 1, Ceps1 = 1.44f, Ceps2 = 1.92f;
+    const int KK = q.KK;
+    const gf32 *TURB = t.TURB, *TDISS = t.TDISS, *UK = t.UK, *VK = t.VK, *WK = t.WK;
+    const gf32 *UKip = t.UKip, *UKjp = t.UKjp, *UKipjp = t.UKipjp, *UKjm = t.UKjm, *UKipjm = t.UKipjm;
+    const gf32 *VKjp = t.VKjp, *VKip = t.VKip, *VKipjp = t.VKipjp, *VKim = t.VKim, *VKimjp = t.VKimjp;
+    const gf32 *WKip = t.WKip, *WKim = t.WKim, *WKjp = t.WKjp, *WKjm = t.WKjm;
+    TkeCell r;
+    float dz = d.dzk[k];
+    float dxkdzk = DXK * dz, dykdzk = DYK * dz, DZK_Rez = 1 / dz;
+    float zs = d.hok[k] - d.hok[KK] - dz * 0.5f;
+    float VIS = sqrtf(TURB[k]) * zs * 0.09f;
+    VIS = mminf(VIS, 15.0f);
+    float DE = mmaxf(VIS, d.vishmin) * dykdzk * DXK_Rez, DW = DE;
+    float DS = mmaxf(VIS, d.vishmin) * dxkdzk * DYK_Rez, DN = DS;
+    float DT = mmaxf(VIS, d.vishmin) * AREA * DZK_Rez, DB = 0;
+    float DTEPS = mmaxf(VIS / Ceps, d.visvmin) * AREA * DZK_Rez, DBEPS = 0;
+    if (k > KK + 1) { DB = DT; DBEPS = DTEPS; }
+    if (q.as) DS = 0; if (q.an) DN = 0; if (q.aw) DW = 0; if (q.ae) DE = 0;
+    float FE = UKip[k] * dykdzk, FW = UK[k] * dykdzk, FS = VK[k] * dxkdzk, FN = VKjp[k] * dxkdzk;
+    float FT = WK[k + 1] * AREA, FB = 0;
+    if (k > KK + 1) FB = WK[k] * AREA;
+    DE = mmaxf(DE, 0.0001f); DB = mmaxf(DB, 0.0001f); DW = mmaxf(DW, 0.0001f);
+    DS = mmaxf(DS, 0.0001f); DN = mmaxf(DN, 0.0001f); DT = mmaxf(DT, 0.0001f);
+    DBEPS = mmaxf(DBEPS, 0.0001f); DTEPS = mmaxf(DTEPS, 0.0001f);
+    float PE = fabsf(FE / DE), PB = fabsf(FB / DB), PW = fabsf(FW / DW);
+    float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
+    float PBEPS = fabsf(FN / DBEPS);   // the reference uses FN here
+    float PTEPS = fabsf(FT / DTEPS);
+    float x;
+    x = 1 - 0.1f * PT; float BIM = DT * mmaxf(0, x * x * x * x * x) + mmaxf(-FT, 0);
+    x = 1 - 0.1f * PB; float CIM = DB * mmaxf(0, x * x * x * x * x) + mmaxf(FB, 0);
+    x = 1 - 0.1f * PE; float AE1 = DE * mmaxf(0, x * x * x * x * x) + mmaxf(-FE, 0);
+    x = 1 - 0.1f * PW; float AW1 = DW * mmaxf(0, x * x * x * x * x) + mmaxf(FW, 0);
+    x = 1 - 0.1f * PS; float AS1 = DS * mmaxf(0, x * x * x * x * x) + mmaxf(FS, 0);
+    x = 1 - 0.1f * PN; float AN1 = DN * mmaxf(0, x * x * x * x * x) + mmaxf(-FN, 0);
+    // the reference weights the eps system with DT/DB, not DTEPS/DBEPS
+    x = 1 - 0.1f * PTEPS; float BIMEPS = DT * mmaxf(0, x * x * x * x * x) + mmaxf(-FT, 0);
+    x = 1 - 0.1f * PBEPS; float CIMEPS = DB * mmaxf(0, x * x * x * x * x) + mmaxf(FB, 0);
+    float ap0 = d.ap0[k];
+    float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + ap0;
+    float UX1 = UKip[k], UX2 = UK[k];
+    float VX1 = 0.5f * (VKip[k] + VKipjp[k]), VX2 = 0.5f * (VKim[k] + VKimjp[k]);
+    float WX1 = 0.5f * (WKip[k] + WKip[k + 1]), WX2 = 0.5f * (WKim[k] + WKim[k + 1]);
+    float UY1 = 0.5f * (UKjp[k] + UKipjp[k]), UY2 = 0.5f * (UKjm[k] + UKipjm[k]);
+    float VY1 = VKjp[k], VY2 = VK[k];
+    float WY1 = 0.5f * (WKjp[k] + WKjp[k + 1]), WY2 = 0.5f * (WKjm[k] + WKjm[k + 1]);
+    float DUDX = (UX1 - UX2) * DXK_Rez;
+    float DVDX = (VX1 - VX2) * DXK_Rez * 0.5f;
+    float DWDX = (WX1 - WX2) * DXK_Rez * 0.5f;
+    float DUDY = (UY1 - UY2) * DYK_Rez * 0.5f;
+    float DVDY = (VY1 - VY2) * DYK_Rez;
+    float DWDY = (WY1 - WY2) * DYK_Rez * 0.5f;
+    float DUDZ = UK[k + 1] * DZK_Rez * 0.5f;
+    float DVDZ = VK[k + 1] * DZK_Rez * 0.5f;
+    float DWDZ = WK[k + 1] * DZK_Rez * 0.5f;
+    if (k > KK + 1) {
+        DUDZ = (UK[k + 1] - UK[k - 1]) * DZK_Rez * 0.5f;
+        DVDZ = (VK[k + 1] - VK[k - 1]) * DZK_Rez * 0.5f;
+        DWDZ = (WK[k + 1] - WK[k - 1]) * DZK_Rez * 0.5f;
+    }
+    float s1 = DUDY + DVDX, s2 = DUDZ + DWDX, s3 = DVDZ + DWDY;
+    float PROTURB = VIS * AREA * dz *
+                    (2.0f * (DUDX * DUDX) + 2.0f * (DVDY * DVDY) + 2.0f * (DWDZ * DWDZ) + s1 * s1 + s2 * s2 + s3 * s3 - TDISS[k]);
+    float PROBOU = (float)((double)(-d.potdtdz) * 9.81 / 283.0 * (double)VIS * 1.35 * (double)AREA * (double)dz);
+    r.A = AIM; r.B1 = BIM; r.C1 = CIM; r.B2 = BIMEPS; r.C2 = CIMEPS;
+    r.AW1 = AW1; r.AS1 = AS1; r.AE1 = AE1; r.AN1 = AN1;
+    r.w1 = ap0 * TURB[k]; r.PRO = PROTURB; r.PBO = PROBOU;
+    r.w2 = ap0 * TDISS[k];
+    r.EPS = TDISS[k] / TURB[k] * (Ceps1 * PROTURB - Ceps2 * TDISS[k]);
+    // wall functions of the back-substitution (they read only UK/VK/WK)
+    int wf = 0; float wt = 0, we = 0;
+    if (KK < k) {
+        float us;
+        if ((k == KK + 1) && (q.cutk == 0)) {
+            wf = 1;
+            const size_t cc = t.c;
+            if (d.tke_br[cc]) {
+                double s = sqrt((0.5 * (double)(UK[k + 1] + UKip[k + 1])) * (0.5 * (double)(UK[k + 1] + UKip[k + 1])) +
+                                (0.5 * (double)(VK[k + 1] + VKjp[k + 1])) * (0.5 * (double)(VK[k + 1] + VKjp[k + 1])));
+                us = (float)(d.tke_a[cc] * s);
+            } else {
+                double s = sqrt((0.5 * (double)(UK[k + 1] + UKip[k])) * (0.5 * (double)(UK[k + 1] + UKip[k])) +
+                                (0.5 * (double)(VK[k] + VKjp[k])) * (0.5 * (double)(VK[k] + VKjp[k])));
+                us = (float)(d.tke_a[cc] * s);
+            }
+            wt = (float)(3.33 * (double)(us * us));
+            we = (float)((double)(us * us * us) / d.tke_den[cc]);
+        } else if ((k == KK + 1) && (q.cutk > 0)) {
+            wf = 1;
+            double a1 = 0.5 * (double)(UK[k] + UKip[k]), b1 = 0.5 * (double)(VK[k] + VKjp[k]);
+            us = q.ho * (float)sqrt(a1 * a1 + b1 * b1);
+            wt = (float)(3.33 * (double)(us * us));
+            we = (float)((double)(us * us * us) / ((double)dz * 0.5 * 0.4));
+        } else if (((k == t.kkip + 1) && t.cip) || ((k == t.kkim + 1) && t.cim)) {
+            wf = 1;
+            double a1 = 0.5 * (double)(WK[k] + WK[k + 1]), b1 = 0.5 * (double)(VK[k] + VKjp[k]);
+            us = (float)((double)d.ufx * sqrt(a1 * a1 + b1 * b1));
+            wt = (float)(3.33 * (double)(us * us));
+            we = (float)((double)(us * us * us) / ((double)d.bz0 * 0.4));
+        } else if (((k == t.kkjp + 1) && t.cjp) || ((k == t.kkjm + 1) && t.cjm)) {
+            wf = 1;
+            double a1 = 0.5 * (double)(WK[k] + WK[k + 1]), b1 = 0.5 * (double)(UK[k] + UKip[k]);
+            us = (float)((double)d.ufx * sqrt(a1 * a1 + b1 * b1));
+            wt = (float)(3.33 * (double)(us * us));
+            we = (float)((double)(us * us * us) / ((double)d.bz0 * 0.4));
+        }
+    }
+    r.wf = wf; r.wt = wt; r.we = we;
+    return r;
+}
+
 __global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t, int IS, int JS) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
 
@@ -946,126 +1095,20 @@ __global__ __launch_bounds__(WAVE) void k_tke(const Dom *__restrict__ ds, int t,
     float *sA = sm, *sB1 = sA + nkp, *sC1 = sB1 + nkp, *sD1 = sC1 + nkp, *sB2 = sD1 + nkp, *sC2 = sB2 + nkp,
           *sD2 = sC2 + nkp, *sT = sD2 + nkp, *sE = sT + nkp, *sWT = sE + nkp, *sWE = sWT + nkp;
     int *sWF = (int *)(sWE + nkp);
-    const float DXK = d.dxk, DYK = d.dyk, AREA = d.area, DXK_Rez = 1 / DXK, DYK_Rez = 1 / DYK;
-    const float Ceps = 1, Ceps1 = 1.44f, Ceps2 = 1.92f;
-    size_t o = c3(d, i, j);
-    const gf32 *TURB = d.turb + o, *TDISS = d.tdiss + o;
+    const TkeCol tc = tke_col(d, i, j);
+    size_t o = tc.o;
+    const gf32 *TURB = tc.TURB, *TDISS = tc.TDISS;
     const gf32 *TURBim = d.turb + c3(d, i - 1, j), *TURBip = d.turb + c3(d, i + 1, j);
     const gf32 *TURBjm = d.turb + c3(d, i, j - 1), *TURBjp = d.turb + c3(d, i, j + 1);
     const gf32 *TDISSim = d.tdiss + c3(d, i - 1, j), *TDISSip = d.tdiss + c3(d, i + 1, j);
     const gf32 *TDISSjm = d.tdiss + c3(d, i, j - 1), *TDISSjp = d.tdiss + c3(d, i, j + 1);
-    const gf32 *UK = d.uk + o, *VK = d.vk + o, *WK = d.wk + o;
-    const gf32 *UKip = d.uk + c3(d, i + 1, j), *UKjp = d.uk + c3(d, i, j + 1), *UKipjp = d.uk + c3(d, i + 1, j + 1);
-    const gf32 *UKjm = d.uk + c3(d, i, j - 1), *UKipjm = d.uk + c3(d, i + 1, j - 1);
-    const gf32 *VKjp = d.vk + c3(d, i, j + 1), *VKip = d.vk + c3(d, i + 1, j), *VKipjp = d.vk + c3(d, i + 1, j + 1);
-    const gf32 *VKim = d.vk + c3(d, i - 1, j), *VKimjp = d.vk + c3(d, i - 1, j + 1);
-    const gf32 *WKip = d.wk + c3(d, i + 1, j), *WKim = d.wk + c3(d, i - 1, j);
-    const gf32 *WKjp = d.wk + c3(d, i, j + 1), *WKjm = d.wk + c3(d, i, j - 1);
     int KK = q.KK;
-    int kkip = d.kk[c2(d, i + 1, j)], kkim = d.kk[c2(d, i - 1, j)], kkjp = d.kk[c2(d, i, j + 1)], kkjm = d.kk[c2(d, i, j - 1)];
-    bool cip = d.cutk[c2(d, i + 1, j)] > 0, cim = d.cutk[c2(d, i - 1, j)] > 0;
-    bool cjp = d.cutk[c2(d, i, j + 1)] > 0, cjm = d.cutk[c2(d, i, j - 1)] > 0;
     for (int k = q.KSTART + tid; k <= q.VI; k += WAVE) {
-        float dz = d.dzk[k];
-        float dxkdzk = DXK * dz, dykdzk = DYK * dz, DZK_Rez = 1 / dz;
-        float zs = d.hok[k] - d.hok[KK] - dz * 0.5f;
-        float VIS = sqrtf(TURB[k]) * zs * 0.09f;
-        VIS = mminf(VIS, 15.0f);
-        float DE = mmaxf(VIS, d.vishmin) * dykdzk * DXK_Rez, DW = DE;
-        float DS = mmaxf(VIS, d.vishmin) * dxkdzk * DYK_Rez, DN = DS;
-        float DT = mmaxf(VIS, d.vishmin) * AREA * DZK_Rez, DB = 0;
-        float DTEPS = mmaxf(VIS / Ceps, d.visvmin) * AREA * DZK_Rez, DBEPS = 0;
-        if (k > KK + 1) { DB = DT; DBEPS = DTEPS; }
-        if (q.as) DS = 0; if (q.an) DN = 0; if (q.aw) DW = 0; if (q.ae) DE = 0;
-        float FE = UKip[k] * dykdzk, FW = UK[k] * dykdzk, FS = VK[k] * dxkdzk, FN = VKjp[k] * dxkdzk;
-        float FT = WK[k + 1] * AREA, FB = 0;
-        if (k > KK + 1) FB = WK[k] * AREA;
-        DE = mmaxf(DE, 0.0001f); DB = mmaxf(DB, 0.0001f); DW = mmaxf(DW, 0.0001f);
-        DS = mmaxf(DS, 0.0001f); DN = mmaxf(DN, 0.0001f); DT = mmaxf(DT, 0.0001f);
-        DBEPS = mmaxf(DBEPS, 0.0001f); DTEPS = mmaxf(DTEPS, 0.0001f);
-        float PE = fabsf(FE / DE), PB = fabsf(FB / DB), PW = fabsf(FW / DW);
-        float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
-        float PBEPS = fabsf(FN / DBEPS);   // the reference uses FN here
-        float PTEPS = fabsf(FT / DTEPS);
-        float x;
-        x = 1 - 0.1f * PT; float BIM = DT * mmaxf(0, x * x * x * x * x) + mmaxf(-FT, 0);
-        x = 1 - 0.1f * PB; float CIM = DB * mmaxf(0, x * x * x * x * x) + mmaxf(FB, 0);
-        x = 1 - 0.1f * PE; float AE1 = DE * mmaxf(0, x * x * x * x * x) + mmaxf(-FE, 0);
-        x = 1 - 0.1f * PW; float AW1 = DW * mmaxf(0, x * x * x * x * x) + mmaxf(FW, 0);
-        x = 1 - 0.1f * PS; float AS1 = DS * mmaxf(0, x * x * x * x * x) + mmaxf(FS, 0);
-        x = 1 - 0.1f * PN; float AN1 = DN * mmaxf(0, x * x * x * x * x) + mmaxf(-FN, 0);
-        // the reference weights the eps system with DT/DB, not DTEPS/DBEPS
-        x = 1 - 0.1f * PTEPS; float BIMEPS = DT * mmaxf(0, x * x * x * x * x) + mmaxf(-FT, 0);
-        x = 1 - 0.1f * PBEPS; float CIMEPS = DB * mmaxf(0, x * x * x * x * x) + mmaxf(FB, 0);
-        float ap0 = d.ap0[k];
-        float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + ap0;
-        float UX1 = UKip[k], UX2 = UK[k];
-        float VX1 = 0.5f * (VKip[k] + VKipjp[k]), VX2 = 0.5f * (VKim[k] + VKimjp[k]);
-        float WX1 = 0.5f * (WKip[k] + WKip[k + 1]), WX2 = 0.5f * (WKim[k] + WKim[k + 1]);
-        float UY1 = 0.5f * (UKjp[k] + UKipjp[k]), UY2 = 0.5f * (UKjm[k] + UKipjm[k]);
-        float VY1 = VKjp[k], VY2 = VK[k];
-        float WY1 = 0.5f * (WKjp[k] + WKjp[k + 1]), WY2 = 0.5f * (WKjm[k] + WKjm[k + 1]);
-        float DUDX = (UX1 - UX2) * DXK_Rez;
-        float DVDX = (VX1 - VX2) * DXK_Rez * 0.5f;
-        float DWDX = (WX1 - WX2) * DXK_Rez * 0.5f;
-        float DUDY = (UY1 - UY2) * DYK_Rez * 0.5f;
-        float DVDY = (VY1 - VY2) * DYK_Rez;
-        float DWDY = (WY1 - WY2) * DYK_Rez * 0.5f;
-        float DUDZ = UK[k + 1] * DZK_Rez * 0.5f;
-        float DVDZ = VK[k + 1] * DZK_Rez * 0.5f;
-        float DWDZ = WK[k + 1] * DZK_Rez * 0.5f;
-        if (k > KK + 1) {
-            DUDZ = (UK[k + 1] - UK[k - 1]) * DZK_Rez * 0.5f;
-            DVDZ = (VK[k + 1] - VK[k - 1]) * DZK_Rez * 0.5f;
-            DWDZ = (WK[k + 1] - WK[k - 1]) * DZK_Rez * 0.5f;
-        }
-        float s1 = DUDY + DVDX, s2 = DUDZ + DWDX, s3 = DVDZ + DWDY;
-        float PROTURB = VIS * AREA * dz *
-                        (2.0f * (DUDX * DUDX) + 2.0f * (DVDY * DVDY) + 2.0f * (DWDZ * DWDZ) + s1 * s1 + s2 * s2 + s3 * s3 - TDISS[k]);
-        float PROBOU = (float)((double)(-d.potdtdz) * 9.81 / 283.0 * (double)VIS * 1.35 * (double)AREA * (double)dz);
-        float DIMTURB = AW1 * TURBim[k] + AS1 * TURBjm[k] + AE1 * TURBip[k] + AN1 * TURBjp[k] + ap0 * TURB[k] + PROTURB + PROBOU;
-        float DIMTDISS = AW1 * TDISSim[k] + AS1 * TDISSjm[k] + AE1 * TDISSip[k] + AN1 * TDISSjp[k] + ap0 * TDISS[k] +
-                         (TDISS[k] / TURB[k] * (Ceps1 * PROTURB - Ceps2 * TDISS[k]));
-        sA[k] = AIM; sB1[k] = BIM; sC1[k] = CIM; sD1[k] = DIMTURB; sB2[k] = BIMEPS; sC2[k] = CIMEPS; sD2[k] = DIMTDISS;
-        // wall functions of the back-substitution (they read only UK/VK/WK)
-        int wf = 0; float wt = 0, we = 0;
-        if (KK < k) {
-            float us;
-            if ((k == KK + 1) && (q.cutk == 0)) {
-                wf = 1;
-                size_t cc = c2(d, i, j);
-                if (d.tke_br[cc]) {
-                    double s = sqrt((0.5 * (double)(UK[k + 1] + UKip[k + 1])) * (0.5 * (double)(UK[k + 1] + UKip[k + 1])) +
-                                    (0.5 * (double)(VK[k + 1] + VKjp[k + 1])) * (0.5 * (double)(VK[k + 1] + VKjp[k + 1])));
-                    us = (float)(d.tke_a[cc] * s);
-                } else {
-                    double s = sqrt((0.5 * (double)(UK[k + 1] + UKip[k])) * (0.5 * (double)(UK[k + 1] + UKip[k])) +
-                                    (0.5 * (double)(VK[k] + VKjp[k])) * (0.5 * (double)(VK[k] + VKjp[k])));
-                    us = (float)(d.tke_a[cc] * s);
-                }
-                wt = (float)(3.33 * (double)(us * us));
-                we = (float)((double)(us * us * us) / d.tke_den[cc]);
-            } else if ((k == KK + 1) && (q.cutk > 0)) {
-                wf = 1;
-                double a1 = 0.5 * (double)(UK[k] + UKip[k]), b1 = 0.5 * (double)(VK[k] + VKjp[k]);
-                us = q.ho * (float)sqrt(a1 * a1 + b1 * b1);
-                wt = (float)(3.33 * (double)(us * us));
-                we = (float)((double)(us * us * us) / ((double)dz * 0.5 * 0.4));
-            } else if (((k == kkip + 1) && cip) || ((k == kkim + 1) && cim)) {
-                wf = 1;
-                double a1 = 0.5 * (double)(WK[k] + WK[k + 1]), b1 = 0.5 * (double)(VK[k] + VKjp[k]);
-                us = (float)((double)d.ufx * sqrt(a1 * a1 + b1 * b1));
-                wt = (float)(3.33 * (double)(us * us));
-                we = (float)((double)(us * us * us) / ((double)d.bz0 * 0.4));
-            } else if (((k == kkjp + 1) && cjp) || ((k == kkjm + 1) && cjm)) {
-                wf = 1;
-                double a1 = 0.5 * (double)(WK[k] + WK[k + 1]), b1 = 0.5 * (double)(UK[k] + UKip[k]);
-                us = (float)((double)d.ufx * sqrt(a1 * a1 + b1 * b1));
-                wt = (float)(3.33 * (double)(us * us));
-                we = (float)((double)(us * us * us) / ((double)d.bz0 * 0.4));
-            }
-        }
-        sWF[k] = wf; sWT[k] = wt; sWE[k] = we;
+        const TkeCell e = tke_cell(d, q, tc, k);
+        float DIMTURB = e.AW1 * TURBim[k] + e.AS1 * TURBjm[k] + e.AE1 * TURBip[k] + e.AN1 * TURBjp[k] + e.w1 + e.PRO + e.PBO;
+        float DIMTDISS = e.AW1 * TDISSim[k] + e.AS1 * TDISSjm[k] + e.AE1 * TDISSip[k] + e.AN1 * TDISSjp[k] + e.w2 + e.EPS;
+        sA[k] = e.A; sB1[k] = e.B1; sC1[k] = e.C1; sD1[k] = DIMTURB; sB2[k] = e.B2; sC2[k] = e.C2; sD2[k] = DIMTDISS;
+        sWF[k] = e.wf; sWT[k] = e.wt; sWE[k] = e.we;
     }
     for (int k = q.KSTART + tid; k <= q.VI + 1; k += WAVE) { sT[k] = TURB[k]; sE[k] = TDISS[k]; }
     __syncthreads();
@@ -2148,6 +2191,180 @@ extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStrea
 // ------------------------------------------------------------------------
 // host-side launchers
 // ------------------------------------------------------------------------
+// ------------------------------------------------------------------------
+// Strip-pipelined TKE sweep (TKE-prognostic-microscale.cs:25-352).
+// k_tke_coef writes, for every active cell above KKART, what the sweep's
+// in-place neighbours do not touch: the neighbour coefficients AW..AN, the
+// remaining source terms, the halves of the two Thomas recurrences that need
+// no source (P and its denominator, per level), and the wall function.  The
+// strip kernel then only assembles the neighbour terms, runs the two Q
+// recurrences and the back-substitution, and writes back.  Record per cell
+// (TKE_REC floats): {AW, AS, AE, AN | ap0*TURB, PROTURB, PROBOU, ap0*TDISS |
+// EPS source, C1, den1, P1 | C2, den2, P2, wall flag | wall TURB, wall TDISS, -, -}.
+// ------------------------------------------------------------------------
+#define TKE_REC 20
+
+__global__ __launch_bounds__(WAVE) void k_tke_coef(const Dom *__restrict__ ds) {
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
+    extern __shared__ float sm[];
+    const int nkp = d.nkp, tid = threadIdx.x;
+    const int col = xcd_block((d.nii - 2) * (d.njj - 2));
+    if (col < 0) return;
+    const int nj = d.njj - 2, i = 2 + col / nj, j = 2 + col % nj;
+    if (d.adv[c2(d, i, j)] != 1) return;
+    const ColInfo q = col_info(d, i, j);
+    const TkeCol tc = tke_col(d, i, j);
+    float *sA = sm, *sB1 = sA + nkp, *sC1 = sB1 + nkp, *sB2 = sC1 + nkp, *sC2 = sB2 + nkp;
+    gf32 *R = d.planes + tc.o * TKE_REC;
+    for (int k = q.KK + 1 + tid; k <= q.VI; k += WAVE) {
+        const TkeCell e = tke_cell(d, q, tc, k);
+        gf32 *r = R + (size_t)k * TKE_REC;
+        stg4(r, e.AW1, e.AS1, e.AE1, e.AN1);
+        stg4(r + 4, e.w1, e.PRO, e.PBO, e.w2);
+        r[8] = e.EPS; r[9] = e.C1; r[12] = e.C2;
+        r[15] = e.wf ? 1.0f : 0.0f; r[16] = e.wt; r[17] = e.we;
+        sA[k] = e.A; sB1[k] = e.B1; sC1[k] = e.C1; sB2[k] = e.B2; sC2[k] = e.C2;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // P = B/(A - C*P_prev) above KKART+1, B/A at KKART+1 (the recurrence
+        // restarts there; the levels below it do not reach the ones above)
+        float P1 = 0, P2 = 0;
+        for (int k = q.KK + 1; k <= q.VI; ++k) {
+            const float A = sA[k];
+            float den1 = A, den2 = A;
+            if (k > q.KK + 1) { den1 = A - sC1[k] * P1; den2 = A - sC2[k] * P2; }
+            P1 = sB1[k] / den1;
+            P2 = sB2[k] / den2;
+            gf32 *r = R + (size_t)k * TKE_REC;
+            r[10] = den1; r[11] = P1; r[13] = den2; r[14] = P2;
+        }
+    }
+}
+
+// Strip sweep: TKE_ROWS row waves, no chain wave (TKE_ROWS = 8 keeps the
+// thirteen per-row level arrays within 64 KB of LDS).  Row wave r owns sweep row
+// b0 + r (j direction) and at step s handles column a = 2 + s - r; each wave
+// assembles its column's sources (lanes over levels), runs the column's two
+// recurrences on lane 0 and writes the column back.  The neighbours updated
+// earlier in the sweep come from LDS (the same row's previous column, the
+// previous row's column of the previous step) or, for row 0, from the
+// previous strip's published last row; the others are read from memory,
+// where no wave of the sweep has written them yet.
+#define TKE_SLOTS 13                 // per row: D1, D2, C1, den1, P1, C2, den2, P2, wf, wt, we, T, E
+__global__ __launch_bounds__(TKE_ROWS * WAVE) void k_tke_strip(const Dom *__restrict__ ds, int IS, int JS) {
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
+    extern __shared__ float sm[];
+    const int nkp = d.nkp, nkl = lds_slot(nkp);
+    const int oa = 2, ia = 2, ob = d.nii - 1, ib = d.njj - 1;
+    const int nstrip = (ib - ia + TKE_ROWS) / TKE_ROWS, nsteps = (ob - oa + 1) + TKE_ROWS - 1;
+    const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
+    gi32 *sync = d.sync + SYNC_BLOCK0 + (size_t)4 * d.sync_stride;
+    float *row = sm + (size_t)r * TKE_SLOTS * nkl;
+    float *sD1 = row, *sD2 = row + nkl, *sC1 = row + 2 * nkl, *sN1 = row + 3 * nkl, *sP1 = row + 4 * nkl;
+    float *sC2 = row + 5 * nkl, *sN2 = row + 6 * nkl, *sP2 = row + 7 * nkl, *sWF = row + 8 * nkl;
+    float *sWT = row + 9 * nkl, *sWE = row + 10 * nkl, *sTo = row + 11 * nkl, *sEo = row + 12 * nkl;
+    float *newT = sm + (size_t)TKE_ROWS * TKE_SLOTS * nkl;        // [2][TKE_ROWS] new TURB columns
+    float *newE = newT + (size_t)2 * TKE_ROWS * nkl;              // [2][TKE_ROWS] new TDISS columns
+    int *sinfo = (int *)(newE + (size_t)2 * TKE_ROWS * nkl);      // [2][TKE_ROWS] valid, lo, hi
+    int *sctl = sinfo + 6 * TKE_ROWS;
+    if (threadIdx.x == 0) sctl[0] = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int B = sctl[0];
+    if (B >= nstrip) return;                                  // uniform over the workgroup
+    const int b = ia + B * TKE_ROWS + r;
+    const bool row_ok = b <= ib;
+    StripWait sw{B > 0 ? sync + SYNC_HDR + B - 1 : nullptr, d.sync, B > 0 ? 0 : 0x7fffffff, nsteps,
+                 wall_clock64() + STRIP_TIMEOUT_TICKS};
+    gi32 *mine = sync + SYNC_HDR + B;
+    const int j = row_ok ? ((JS == -1) ? d.njj - b + 1 : b) : 0;
+    for (int s = 0; s < nsteps; ++s) {
+        const int par = s & 1, prv = par ^ 1;
+        int *info = sinfo + (par * TKE_ROWS + r) * 3;
+        const int l = s - r, a = oa + l;
+        bool on = row_ok && l >= 0 && a <= ob;
+        int i = 0;
+        size_t c = 0;
+        if (on) {
+            i = (IS == -1) ? d.nii - a + 1 : a;
+            c = c2(d, i, j);
+            on = d.adv[c] == 1;
+        }
+        if (r == 0 && row_ok && l >= 0 && a <= ob) sw.wait(s + TKE_ROWS);   // the previous strip's row b0-1 at column a
+        if (on) {
+            const int KK = d.kk[c], VI = d.vi[c], lo = KK + 1;
+            const size_t o = c3(d, i, j);
+            const gf32 *R = d.planes + o * TKE_REC;
+            // sources of the four neighbours: the one before in i (same row, step
+            // s-1) and before in j (row r-1, step s-1) are new; the others old
+            const int *pa = sinfo + (prv * TKE_ROWS + r) * 3;             // this row, previous column
+            const int *pb = r > 0 ? sinfo + (prv * TKE_ROWS + r - 1) * 3 : nullptr;
+            const bool va = l > 0 && pa[0], vb = pb && pb[0];
+            const float *nTa = newT + (size_t)(prv * TKE_ROWS + r) * nkl, *nEa = newE + (size_t)(prv * TKE_ROWS + r) * nkl;
+            const float *nTb = newT + (size_t)(prv * TKE_ROWS + (r > 0 ? r - 1 : 0)) * nkl;
+            const float *nEb = newE + (size_t)(prv * TKE_ROWS + (r > 0 ? r - 1 : 0)) * nkl;
+            const size_t oim = c3(d, i - 1, j), oip = c3(d, i + 1, j), ojm = c3(d, i, j - 1), ojp = c3(d, i, j + 1);
+            const size_t oa_ = (IS == 1) ? oim : oip, ob_ = (JS == 1) ? ojm : ojp;   // new-neighbour columns
+            for (int k = lo + lane; k <= VI; k += WAVE) {
+                const gf32 *rr = R + (size_t)k * TKE_REC;
+                const float4 c0 = ldg4(rr), c1 = ldg4(rr + 4), c2v = ldg4(rr + 8), c3v = ldg4(rr + 12), c4 = ldg4(rr + 16);
+                // neighbour TURB/TDISS: memory unless updated earlier in this sweep
+                float tA = d.turb[oa_ + k], eA = d.tdiss[oa_ + k], tB = d.turb[ob_ + k], eB = d.tdiss[ob_ + k];
+                if (va && k >= pa[1] && k <= pa[2]) { tA = nTa[k]; eA = nEa[k]; }
+                if (vb && k >= pb[1] && k <= pb[2]) { tB = nTb[k]; eB = nEb[k]; }
+                const float tAo = d.turb[((IS == 1) ? oip : oim) + k], eAo = d.tdiss[((IS == 1) ? oip : oim) + k];
+                const float tBo = d.turb[((JS == 1) ? ojp : ojm) + k], eBo = d.tdiss[((JS == 1) ? ojp : ojm) + k];
+                const float tW = (IS == 1) ? tA : tAo, tE = (IS == 1) ? tAo : tA;
+                const float tS = (JS == 1) ? tB : tBo, tN = (JS == 1) ? tBo : tB;
+                const float eW = (IS == 1) ? eA : eAo, eE = (IS == 1) ? eAo : eA;
+                const float eS = (JS == 1) ? eB : eBo, eN = (JS == 1) ? eBo : eB;
+                sD1[k] = c0.x * tW + c0.y * tS + c0.z * tE + c0.w * tN + c1.x + c1.y + c1.z;
+                sD2[k] = c0.x * eW + c0.y * eS + c0.z * eE + c0.w * eN + c1.w + c2v.x;
+                sC1[k] = c2v.y; sN1[k] = c2v.z; sP1[k] = c2v.w;
+                sC2[k] = c3v.x; sN2[k] = c3v.y; sP2[k] = c3v.z; sWF[k] = c3v.w;
+                sWT[k] = c4.x; sWE[k] = c4.y;
+            }
+            for (int k = lo + lane; k <= VI + 1; k += WAVE) { sTo[k] = d.turb[o + k]; sEo[k] = d.tdiss[o + k]; }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            float *oT = newT + (size_t)(par * TKE_ROWS + r) * nkl, *oE = newE + (size_t)(par * TKE_ROWS + r) * nkl;
+            if (lane == 0) {
+                float Q1 = 0, Q2 = 0;
+                for (int k = lo; k <= VI; ++k) {
+                    float q1, q2;
+                    if (k > lo) { q1 = (sD1[k] + sC1[k] * Q1) / sN1[k]; q2 = (sD2[k] + sC2[k] * Q2) / sN2[k]; }
+                    else { q1 = sD1[k] / sN1[k]; q2 = sD2[k] / sN2[k]; }
+                    sD1[k] = q1; sD2[k] = q2;
+                    Q1 = q1; Q2 = q2;
+                }
+                float tup = sTo[VI + 1], eup = sEo[VI + 1];
+                for (int k = VI; k >= lo; --k) {
+                    float tv, ev;
+                    if (sWF[k] != 0.0f) { tv = sWT[k]; ev = sWE[k]; }
+                    else { tv = sP1[k] * tup + sD1[k]; ev = sP2[k] * eup + sD2[k]; }
+                    tv = (float)mmaxd((double)tv, 0.001);
+                    ev = (float)mmaxd((double)ev, 0.0000001);
+                    oT[k] = tv; oE[k] = ev;
+                    tup = tv; eup = ev;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            gf32 *TW = d.turb + o, *EW = d.tdiss + o;
+            for (int k = lo + lane; k <= VI; k += WAVE) { st_through(TW + k, oT[k]); st_through(EW + k, oE[k]); }
+            if (lane == 0) { info[0] = 1; info[1] = lo; info[2] = VI; }
+        } else if (lane == 0) {
+            info[0] = 0;
+        }
+        // the last row publishes the strip's progress (s + 1 steps stored)
+        if (r == TKE_ROWS - 1 && (s + 1) % STRIP_PUBLISH == 0) strip_publish(mine, s + 1, lane);
+        __syncthreads();
+    }
+    if (r == TKE_ROWS - 1) strip_publish(mine, nsteps, lane);
+}
+
 static inline int lds_slot_h(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
 // longest diagonal t over the situations; the pressure sweep runs a over j
 // (outer) and b over i, the momentum sweeps a over i and b over j
@@ -2216,7 +2433,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
     const int ncol8 = ((ncols + COLS_PER_WAVE - 1) / COLS_PER_WAVE + NXCD - 1) / NXCD * NXCD;  // x extent of the column kernels
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
     size_t nrow = (size_t)nj2 * nkp;
-    for (int q = 0; q < nsit; ++q) nrow = max(nrow, (size_t)4 * hd[q].sync_stride);
+    for (int q = 0; q < nsit; ++q) nrow = max(nrow, (size_t)5 * hd[q].sync_stride);
     L.go(FAM_STENCIL, [&] {
         hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((nrow + 255) / 256), 1, Z), dim3(256), 0, s, dd, d0.tiled ? 0 : 1);
     });
@@ -2283,12 +2500,26 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         sweep(FAM_U, k_momentum<0, 1>, 3, 2, IS, JS);
         sweep(FAM_V, k_momentum<1, 1>, 2, 3, IS, JS);
         sweep(FAM_W, k_momentum<2, 1>, 2, 2, IS, JS);
-        size_t shm_t = sizeof(float) * 12 * nkp;
-        int nd = ndiag(2, nii - 1, 2, njj - 1);
-        for (int t = 0; t < nd; ++t) {
-            int len = diag_len_max(hd, nsit, t, 2, 2, false);
-            if (len > 0)
-                L.go(FAM_TKE, [&] { hipLaunchKernelGGL(k_tke, dim3(len, 1, Z), dim3(WAVE), shm_t, s, dd, t, IS, JS); });
+        if (d0.tiled == 2) {
+            // strip-pipelined TKE: the neighbour-independent pre-pass, then the strips
+            L.go(FAM_TKE, [&] {
+                hipLaunchKernelGGL(k_tke_coef, dim3(ncol8 * COLS_PER_WAVE, 1, Z), dim3(WAVE), sizeof(float) * 5 * nkp, s, dd);
+            });
+            int nst = 0;
+            for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].njj - 1 - 2 + TKE_ROWS) / TKE_ROWS);
+            const size_t shm_k =
+                sizeof(float) * (size_t)(TKE_SLOTS + 4) * TKE_ROWS * lds_slot_h(nkp) + sizeof(int) * (6 * TKE_ROWS + 4);
+            L.go(FAM_TKE, [&] {
+                hipLaunchKernelGGL(k_tke_strip, dim3(nst, 1, Z), dim3(TKE_ROWS * WAVE), shm_k, s, dd, IS, JS);
+            });
+        } else {
+            size_t shm_t = sizeof(float) * 12 * nkp;
+            int nd = ndiag(2, nii - 1, 2, njj - 1);
+            for (int t = 0; t < nd; ++t) {
+                int len = diag_len_max(hd, nsit, t, 2, 2, false);
+                if (len > 0)
+                    L.go(FAM_TKE, [&] { hipLaunchKernelGGL(k_tke, dim3(len, 1, Z), dim3(WAVE), shm_t, s, dd, t, IS, JS); });
+            }
         }
     } else {
         sweep(FAM_U, k_momentum<0, 2>, 3, 2, IS, JS);
