@@ -530,9 +530,14 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     d.ptab[0] = d.ptab[1] = d.ttab[0] = d.ttab[1] = nullptr;
     if (d.tiled) {
         if (p->turbulence_model == 2) {
-            float *pl;                           // TKE records of the strip-pipelined TKE sweep
-            if ((rc = dalloc(h, &pl, 20 * n3))) return rc;
+            // the momentum sweeps' records (12 or 8 floats per cell, then
+            // {C, denominator, P} at 12..14) and, in turn, the TKE sweep's 20
+            // floats per cell share the planes; the U/V wall terms follow
+            float *pl;
+            if ((rc = dalloc(h, &pl, 20 * n3 + n2))) return rc;
             d.planes = pl;
+            d.rec3 = pl + 12 * n3;
+            d.wcol = pl + 20 * n3;
         }
         if (p->turbulence_model == 1) {
             float *pl;

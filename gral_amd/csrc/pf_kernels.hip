@@ -699,12 +699,17 @@ __device__ __forceinline__ float ke_wall_uv(const DomG &d, const ColInfo &q, int
     return vel / wh * (us * us) * d.area;
 }
 
+// Everything of one k-eps (DIFF) or no-diffusion cell of U/V but the four
+// neighbour values of the swept component (U-prognostic-microscale_2.cs:100-260,
+// V-prognostic-microscale_2.cs, _0.cs): the coefficients, e = the old-velocity
+// sum of the transient term (its double product with AP0 is formed with the
+// neighbour part), f = the pressure term, g = the Coriolis term, w = the wall
+// term at KKART+1, c0 = VIS/DXK and X1..X4 the neighbour-free gradients of the
+// cross-diffusion term above KKART+1.
 template <bool IS_V, bool DIFF>
-__device__ Coef ke_uv_cell(const DomG &d, const ColInfo &q, int k) {
+__device__ MlPieces ke_uv_pieces(const DomG &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
     const gf32 *X = (IS_V ? d.vk : d.uk) + c3(d, i, j);
-    const gf32 *Xim = (IS_V ? d.vk : d.uk) + c3(d, i - 1, j), *Xip = (IS_V ? d.vk : d.uk) + c3(d, i + 1, j);
-    const gf32 *Xjm = (IS_V ? d.vk : d.uk) + c3(d, i, j - 1), *Xjp = (IS_V ? d.vk : d.uk) + c3(d, i, j + 1);
     const gf32 *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
     const gf32 *UKSim = d.uks + c3(d, i - 1, j), *VKSim = d.vks + c3(d, i - 1, j);
     float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
@@ -740,6 +745,7 @@ __device__ Coef ke_uv_cell(const DomG &d, const ColInfo &q, int k) {
         FT = 0.25f * (WKSjm[k] + WKS[k] + WKSjm[k + 1] + WKS[k + 1]) * d.area;
         if (k > q.KK + 1) FB = 0.25f * (WKSjm[k] + WKS[k] + WKSjm[k - 1] + WKS[k - 1]) * d.area;
     }
+    MlPieces o;
     float BIM, CIM, AE1, AW1, AS1, AN1;
     if (DIFF) {
         DE = vmaxf(DE, 0.0001f); DB = vmaxf(DB, 0.0001f); DW = vmaxf(DW, 0.0001f);
@@ -757,56 +763,49 @@ __device__ Coef ke_uv_cell(const DomG &d, const ColInfo &q, int k) {
         AW1 = vmaxf(FW, 0.0f); AS1 = vmaxf(FS, 0.0f); AN1 = vmaxf(-FN, 0.0f);
     }
     float ap0 = d.ap0[k];
-    float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + ap0;
+    o.A = BIM + CIM + AW1 + AS1 + AE1 + AN1 + ap0;
+    o.B = BIM; o.C = CIM; o.AW1 = AW1; o.AS1 = AS1; o.AE1 = AE1; o.AN1 = AN1;
     const gf32 *DPMNEW = d.dpmnew + c3(d, i, j);
     const gf32 *DPMNEWm = d.dpmnew + (IS_V ? c3(d, i, j - 1) : c3(d, i - 1, j));
     float ddp = DPMNEWm[k] - DPMNEW[k];
     float x = X[k];
-    float partial = AW1 * Xim[k] + AS1 * Xjm[k] + AE1 * Xip[k] + AN1 * Xjp[k];
-    float ssum = IS_V ? (d.vks[c3(d, i, j - 1) + k] + VKS[k]) : (UKSim[k] + UKS[k]);
-    double t5 = (double)ap0 * 0.5 * (double)ssum;
-    float t6 = ddp * (IS_V ? dxkdzk : dykdzk);
-    float t7 = d.cor * ((IS_V ? d.vg : d.ug) - x) * d.area * dz;
-    float D = (float)((double)partial + t5 + (double)t6 + (double)t7);
-    if (k == q.KK + 1) D -= ke_wall_uv(d, q, k, x, UKSim, UKS, VKSim, VKS);
+    o.e = IS_V ? (d.vks[c3(d, i, j - 1) + k] + VKS[k]) : (UKSim[k] + UKS[k]);
+    o.f = ddp * (IS_V ? dxkdzk : dykdzk);
+    o.g = d.cor * ((IS_V ? d.vg : d.ug) - x) * d.area * dz;
+    o.w = (k == q.KK + 1) ? ke_wall_uv(d, q, k, x, UKSim, UKS, VKSim, VKS) : 0.0f;
+    o.c0 = o.X1 = o.X2 = o.X3 = o.X4 = 0.0f;
     if (DIFF && k > q.KK + 1) {
-        float ADD;
+        o.c0 = VIS / d.dxk;
         if (!IS_V) {
             const gf32 *VKSipjp = d.vks + c3(d, i + 1, j + 1), *VKSjp = d.vks + c3(d, i, j + 1);
             const gf32 *VKSimjp = d.vks + c3(d, i - 1, j + 1), *VKSipjm = d.vks + c3(d, i + 1, j - 1);
             const gf32 *VKSjm = d.vks + c3(d, i, j - 1), *VKSimjm = d.vks + c3(d, i - 1, j - 1);
             const gf32 *WKip = d.wk + c3(d, i + 1, j), *WKSim = d.wks + c3(d, i - 1, j);
-            float DUDXE = (Xip[k] - x) * dykdzk;
-            float DUDXW = (x - Xim[k]) * dykdzk;
-            float DVDXN = (0.5f * (VKSipjp[k] + VKSjp[k]) - 0.5f * (VKSjp[k] + VKSimjp[k])) * dxkdzk;
-            float DVDXS = (0.5f * (VKSipjm[k] + VKSjm[k]) - 0.5f * (VKSjm[k] + VKSimjm[k])) * dxkdzk;
-            float DWDXT = (0.5f * (WKip[k] + WKS[k]) - 0.5f * (WKS[k] + WKSim[k])) * d.area;
-            float DWDXB = (0.5f * (WKip[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKSim[k - 1])) * d.area;
-            ADD = VIS / d.dxk * (DUDXE - DUDXW + DVDXN - DVDXS + DWDXT - DWDXB);
+            o.X1 = (0.5f * (VKSipjp[k] + VKSjp[k]) - 0.5f * (VKSjp[k] + VKSimjp[k])) * dxkdzk;          // DVDXN
+            o.X2 = (0.5f * (VKSipjm[k] + VKSjm[k]) - 0.5f * (VKSjm[k] + VKSimjm[k])) * dxkdzk;          // DVDXS
+            o.X3 = (0.5f * (WKip[k] + WKS[k]) - 0.5f * (WKS[k] + WKSim[k])) * d.area;                   // DWDXT
+            o.X4 = (0.5f * (WKip[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKSim[k - 1])) * d.area;   // DWDXB
         } else {
             const gf32 *UKSipjp = d.uks + c3(d, i + 1, j + 1), *UKSip = d.uks + c3(d, i + 1, j);
             const gf32 *UKSipjm = d.uks + c3(d, i + 1, j - 1), *UKSimjp = d.uks + c3(d, i - 1, j + 1);
             const gf32 *UKSimjm = d.uks + c3(d, i - 1, j - 1);
             const gf32 *WKjp = d.wk + c3(d, i, j + 1), *WKSjm = d.wks + c3(d, i, j - 1);
-            float DVDYN = (Xjp[k] - x) * dxkdzk;
-            float DVDYS = (x - Xjm[k]) * dxkdzk;
-            float DUDYE = (0.5f * (UKSipjp[k] + UKSip[k]) - 0.5f * (UKSip[k] + UKSipjm[k])) * dxkdzk;
-            float DUDYW = (0.5f * (UKSimjp[k] + UKSim[k]) - 0.5f * (UKSim[k] + UKSimjm[k])) * dxkdzk;
-            float DWDYT = (0.5f * (WKjp[k] + WKS[k]) - 0.5f * (WKS[k] + WKSjm[k])) * d.area;
-            float DWDYB = (0.5f * (WKjp[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKSjm[k - 1])) * d.area;
-            ADD = VIS / d.dxk * (DUDYE - DUDYW + DVDYN - DVDYS + DWDYT - DWDYB);
+            o.X1 = (0.5f * (UKSipjp[k] + UKSip[k]) - 0.5f * (UKSip[k] + UKSipjm[k])) * dxkdzk;          // DUDYE
+            o.X2 = (0.5f * (UKSimjp[k] + UKSim[k]) - 0.5f * (UKSim[k] + UKSimjm[k])) * dxkdzk;          // DUDYW
+            o.X3 = (0.5f * (WKjp[k] + WKS[k]) - 0.5f * (WKS[k] + WKSjm[k])) * d.area;                   // DWDYT
+            o.X4 = (0.5f * (WKjp[k - 1] + WKS[k - 1]) - 0.5f * (WKS[k - 1] + WKSjm[k - 1])) * d.area;   // DWDYB
         }
-        D += ADD;
     }
-    return Coef{AIM, BIM, CIM, D};
+    return o;
 }
 
+// W of the k-eps / no-diffusion solvers (W-prognostic-microscale_2.cs, _0.cs):
+// e = the old-velocity sum of the transient term, f = the pressure term, X1 =
+// the whole cross-diffusion term above KKART+1 (it reads no neighbour of W).
 template <bool DIFF>
-__device__ Coef ke_w_cell(const DomG &d, const ColInfo &q, int k) {
+__device__ MlPieces ke_w_pieces(const DomG &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
     const gf32 *WK = d.wk + c3(d, i, j);
-    const gf32 *WKim = d.wk + c3(d, i - 1, j), *WKip = d.wk + c3(d, i + 1, j);
-    const gf32 *WKjm = d.wk + c3(d, i, j - 1), *WKjp = d.wk + c3(d, i, j + 1);
     const gf32 *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
     const gf32 *UKSim = d.uks + c3(d, i - 1, j), *UKSip = d.uks + c3(d, i + 1, j);
     const gf32 *VKSjm = d.vks + c3(d, i, j - 1), *VKSjp = d.vks + c3(d, i, j + 1);
@@ -845,11 +844,13 @@ __device__ Coef ke_w_cell(const DomG &d, const ColInfo &q, int k) {
         BIM = vmaxf(-FT, 0.0f); CIM = vmaxf(FB, 0.0f); AE1 = vmaxf(-FE, 0.0f);
         AW1 = vmaxf(FW, 0.0f); AS1 = vmaxf(FS, 0.0f); AN1 = vmaxf(-FN, 0.0f);
     }
+    MlPieces o;
     float ap0 = d.ap0[k];
-    float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + ap0;
-    float ddpz = DPMNEW[k - 1] - DPMNEW[k];
-    float partial = AW1 * WKim[k] + AS1 * WKjm[k] + AE1 * WKip[k] + AN1 * WKjp[k];
-    float D = (float)((double)partial + (double)ap0 * 0.5 * (double)(WKS[k - 1] + WKS[k]) + (double)(ddpz * d.area));
+    o.A = BIM + CIM + AW1 + AS1 + AE1 + AN1 + ap0;
+    o.B = BIM; o.C = CIM; o.AW1 = AW1; o.AS1 = AS1; o.AE1 = AE1; o.AN1 = AN1;
+    o.e = WKS[k - 1] + WKS[k];
+    o.f = (DPMNEW[k - 1] - DPMNEW[k]) * d.area;
+    o.g = o.w = o.c0 = o.X1 = o.X2 = o.X3 = o.X4 = 0.0f;
     if (DIFF && k > q.KK + 1) {
         float DUDZE = (0.5f * (UKS[k + 1] + uks) - 0.5f * (uks + uks_m)) * dykdzk;
         float DUDZW = (0.5f * (UKSim[k + 1] + UKSim[k]) - 0.5f * (UKSim[k] + UKSim[k - 1])) * dykdzk;
@@ -857,9 +858,50 @@ __device__ Coef ke_w_cell(const DomG &d, const ColInfo &q, int k) {
         float DVDZS = (0.5f * (VKSjm[k + 1] + VKSjm[k]) - 0.5f * (VKSjm[k] + VKSjm[k - 1])) * dxkdzk;
         float DWDZT = (WK[k + 1] - WK[k]) * d.area;
         float DWDZB = (WK[k] - WK[k - 1]) * d.area;
-        D += VIS / dz * (DUDZE - DUDZW + DVDZN - DVDZS + DWDZT - DWDZB);
+        o.X1 = VIS / dz * (DUDZE - DUDZW + DVDZN - DVDZS + DWDZT - DWDZB);
     }
-    return Coef{AIM, BIM, CIM, D};
+    return o;
+}
+
+// D of a k-eps / no-diffusion cell from its pieces, the four neighbour values
+// and the cell's own old value x: the neighbour sum in float, then the
+// transient, pressure and Coriolis terms added in double in the reference's
+// order, rounded to float; the wall term (U, V at KKART+1) and the
+// cross-diffusion term follow in float.
+template <int COMP, bool DIFF>
+__device__ __forceinline__ float ke_source(const DomG &d, const ColInfo &q, int k, float dz, float ap0, const MlPieces &o,
+                                           float xw, float xs, float xe, float xn, float x) {
+    float partial = o.AW1 * xw + o.AS1 * xs + o.AE1 * xe + o.AN1 * xn;
+    double t5 = (double)ap0 * 0.5 * (double)o.e;
+    if (COMP == 2) {
+        float D = (float)((double)partial + t5 + (double)o.f);
+        if (DIFF && k > q.KK + 1) D += o.X1;
+        return D;
+    }
+    float D = (float)((double)partial + t5 + (double)o.f + (double)o.g);
+    if (k == q.KK + 1) D -= o.w;
+    if (DIFF && k > q.KK + 1) {
+        if (COMP == 0) {
+            float dykdzk = d.dyk * dz;
+            float DUDXE = (xe - x) * dykdzk;
+            float DUDXW = (x - xw) * dykdzk;
+            D += o.c0 * (DUDXE - DUDXW + o.X1 - o.X2 + o.X3 - o.X4);
+        } else {
+            float dxkdzk = d.dxk * dz;
+            float DVDYN = (xn - x) * dxkdzk;
+            float DVDYS = (x - xs) * dxkdzk;
+            D += o.c0 * (o.X1 - o.X2 + DVDYN - DVDYS + o.X3 - o.X4);
+        }
+    }
+    return D;
+}
+
+template <int COMP, bool DIFF>
+__device__ Coef ke_cell(const DomG &d, const ColInfo &q, int k) {
+    MlPieces o = (COMP == 2) ? ke_w_pieces<DIFF>(d, q, k) : ke_uv_pieces<COMP == 1, DIFF>(d, q, k);
+    float xw, xs, xe, xn, x;
+    ml_neighbours<COMP>(d, q.i, q.j, k, xw, xs, xe, xn, x);
+    return Coef{o.A, o.B, o.C, ke_source<COMP, DIFF>(d, q, k, d.dzk[k], d.ap0[k], o, xw, xs, xe, xn, x)};
 }
 
 // ------------------------------------------------------------------------
@@ -886,10 +928,7 @@ __global__ __launch_bounds__(WAVE) void k_momentum(const Dom *__restrict__ ds, i
     for (int k = q.KSTART + tid; k <= q.VI; k += WAVE) {
         Coef cf;
         if (KIND == 0) cf = (COMP == 0) ? u1_cell(d, q, k) : (COMP == 1) ? v1_cell(d, q, k) : w1_cell(d, q, k);
-        else if (KIND == 1) cf = (COMP == 0) ? ke_uv_cell<false, true>(d, q, k)
-                               : (COMP == 1) ? ke_uv_cell<true, true>(d, q, k) : ke_w_cell<true>(d, q, k);
-        else cf = (COMP == 0) ? ke_uv_cell<false, false>(d, q, k)
-                  : (COMP == 1) ? ke_uv_cell<true, false>(d, q, k) : ke_w_cell<false>(d, q, k);
+        else cf = ke_cell<COMP, KIND == 1>(d, q, k);
         sA[k] = cf.A; sB[k] = cf.B; sC[k] = cf.C; sD[k] = cf.D;
     }
     for (int k = q.KSTART + tid; k <= q.VI + 1; k += WAVE) sX[k] = X[k];
@@ -1215,8 +1254,10 @@ __device__ __forceinline__ Meta unpack_meta(int4 m) {
 // of each cell's record.  Each record is written once, as float4 / float3 stores.
 // WFIX: the solver width when every situation of the launch has it (the
 // width-dependent arithmetic then folds at compile time), 0 to read it per
-// situation.
-template <int COMP, int WFIX>
+// situation.  KIND 0: mixing length (rec3 = {C, TERMP, P}); KIND 1: k-eps, whose
+// solvers divide by the TDMA denominator instead of multiplying by its
+// reciprocal (rec3 = {C, denominator, P}).
+template <int COMP, int WFIX, int KIND = 0>
 __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds, int IS, int JS) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     gf32 *planes = d.planes;
@@ -1285,7 +1326,9 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
         q.cutk = __int_as_float(sc[8]); q.cov = __int_as_float(sc[9]); q.ht = __int_as_float(sc[10]);
         q.ho = __int_as_float(sc[11]); q.z0s = __int_as_float(sc[12]);
         q.W = WFIX ? WFIX : d.simd;
-        MlPieces p = ml_pieces<COMP>(d, q, k);
+        MlPieces p;
+        if constexpr (KIND == 1) p = (COMP == 2) ? ke_w_pieces<true>(d, q, k) : ke_uv_pieces<COMP == 1, true>(d, q, k);
+        else p = ml_pieces<COMP>(d, q, k);
         float *sA = sm + (size_t)c * 3 * nkl;
         sA[k] = p.A; sA[nkl + k] = p.B; sA[2 * nkl + k] = p.C;
         gf32 *P = planes + (c3(d, q.i, q.j) + k) * rec_floats<COMP>();
@@ -1307,8 +1350,14 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
         const int KSTART = scol[c * SCOL + 0], KKP1 = scol[c * SCOL + 1] + 1, VI = scol[c * SCOL + 2];
         float Pp = 0;
         for (int k = KSTART; k <= VI; ++k) {
-            float T = (k > KKP1) ? 1 / (A[k] - Cq[k] * Pp) : 1 / A[k];
-            float P = Bq[k] * T;
+            float T, P;
+            if (KIND == 1) {
+                T = (k > KKP1) ? A[k] - Cq[k] * Pp : A[k];
+                P = Bq[k] / T;
+            } else {
+                T = (k > KKP1) ? 1 / (A[k] - Cq[k] * Pp) : 1 / A[k];
+                P = Bq[k] * T;
+            }
             A[k] = T;          // T replaces A, P replaces B
             Bq[k] = P;
             Pp = P;
@@ -1320,7 +1369,7 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
         const int k = scol[c * SCOL + 0] + (t - scol[c * SCOL + 5]);
         const float *sA = sm + (size_t)c * 3 * nkl;
         gf32 *P = d.rec3 + (c3(d, scol[c * SCOL + 3], scol[c * SCOL + 4]) + k) * 3;
-        stg3(P, sA[2 * nkl + k], sA[k], sA[nkl + k]);                   // C, T, P
+        stg3(P, sA[2 * nkl + k], sA[k], sA[nkl + k]);                   // C, T (or the denominator), P
     }
 }
 
@@ -1348,6 +1397,7 @@ __device__ __forceinline__ int lds_slot(int nkp) { return (nkp + 2 * SLOT_PAD) |
 #define MAXCH (128 / CH)
 
 // forward: Q[t] = (D[t] + C[t]*Q[t-1]) * T[t], t = 0..n-1, Q[-1] = q0
+template <bool DIV>
 __device__ __forceinline__ void chain_fwd(const float *__restrict__ D, const float *__restrict__ C,
                                           const float *__restrict__ T, float *__restrict__ Q, int n, int nmax, float q0,
                                           float *sink) {
@@ -1367,7 +1417,7 @@ __device__ __forceinline__ void chain_fwd(const float *__restrict__ D, const flo
         float *Qw = (t < n) ? Q : sink - t;              // a lane past its end stores to its sink
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            float q = (cd[u] + cc[u] * Qp) * ct[u];
+            float q = DIV ? (cd[u] + cc[u] * Qp) / ct[u] : (cd[u] + cc[u] * Qp) * ct[u];
             Qw[t + u] = q;
             Qp = q;
         }
@@ -1475,9 +1525,10 @@ struct MlPre {
     float aw, as, ae, an, e, f, g, c0, x1, x2, x3, x4, c, t, p;
     float x, xa, xb;      // own old value; old neighbours in the +IS (xa) and +JS (xb) directions
     float ni, nj;         // -IS / -JS neighbours, prefetched only when another strip or the boundary holds them
+    float ap0;            // transient coefficient of the level (k-eps)
 };
 
-template <int COMP>
+template <int COMP, int KIND>
 __device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *planes, size_t n3, int i, int j, int k, int IS, int JS,
                                             MlPre &r) {
     (void)n3;
@@ -1499,6 +1550,7 @@ __device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *planes, s
     r.x = X[c3(d, i, j) + k];
     r.xa = X[c3(d, i + IS, j) + k];
     r.xb = X[c3(d, i, j + JS) + k];
+    if (KIND == 1) r.ap0 = d.ap0[k];
 }
 
 __device__ __forceinline__ ColInfo meta_col(const Meta &m, int i, int j, int W) {
@@ -1592,7 +1644,9 @@ __device__ __forceinline__ void strip_publish(gi32 *mine, int v, int lane) {
     if (lane == 0) store_agent(mine, v);
 }
 
-template <int COMP>
+// KIND 0: mixing length; KIND 1: k-eps (sources by ke_source, the TDMA divides
+// by the denominators k_ml_coef<.., 1> stored).
+template <int COMP, int KIND = 0>
 __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__restrict__ ds, int IS, int JS) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     const gf32 *planes = d.planes;
@@ -1636,7 +1690,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                 float *D = sm + (size_t)cl * 6 * nkl + SLOT_PAD;
                 float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl, *Xn = Q + nkl;
                 const float *Xs = xbuf + (size_t)((s & 1) * ROWS + cl) * nkl;
-                chain_fwd(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane * CH);
+                chain_fwd<KIND == 1>(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane * CH);
                 chain_back(P + VI, Q + VI, Xs + VI, Xn + VI, d.relax, nb, nbmax, sdump + lane * CH);
             }
             lds_barrier();                                    // the recurrences are done
@@ -1689,7 +1743,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
             for (int m = 0; m < 2; ++m) {
                 int k = mn.KSTART + lane + WAVE * m;
                 if (k <= mn.VI) {
-                    ml_prefetch<COMP>(d, planes, n3, in_, jn_, k, IS, JS, pre[m]);
+                    ml_prefetch<COMP, KIND>(d, planes, n3, in_, jn_, k, IS, JS, pre[m]);
                     if (l == 0) pre[m].ni = X[c3(d, in_ - IS, jn_) + k];
                 }
             }
@@ -1764,9 +1818,11 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                     MlPieces pc;
                     pc.AW1 = r_.aw; pc.AS1 = r_.as; pc.AE1 = r_.ae; pc.AN1 = r_.an; pc.e = r_.e; pc.f = r_.f;
                     pc.g = r_.g; pc.w = wcur; pc.c0 = r_.c0; pc.X1 = r_.x1; pc.X2 = r_.x2; pc.X3 = r_.x3; pc.X4 = r_.x4;
-                    float D = ml_source<COMP>(d, q, k, sdzk[k], pc, xw, xs, xe, xn, r_.x);
+                    float D;
+                    if constexpr (KIND == 1) D = ke_source<COMP, true>(d, q, k, sdzk[k], r_.ap0, pc, xw, xs, xe, xn, r_.x);
+                    else D = ml_source<COMP>(d, q, k, sdzk[k], pc, xw, xs, xe, xn, r_.x);
                     sD[k] = D; sC[k] = r_.c; sT[k] = r_.t; sP[k] = r_.p;
-                    if (k <= KKP1) sQ[k] = D * r_.t;
+                    if (k <= KKP1) sQ[k] = (KIND == 1) ? D / r_.t : D * r_.t;
                 }
             }
             if (lane == 0) {
@@ -2497,9 +2553,16 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         sweep(FAM_V, k_momentum<1, 0>, 2, 3, -IS, -JS);
         sweep(FAM_W, k_momentum<2, 0>, 2, 2, IS, JS);
     } else if (d0.model == 2) {
-        sweep(FAM_U, k_momentum<0, 1>, 3, 2, IS, JS);
-        sweep(FAM_V, k_momentum<1, 1>, 2, 3, IS, JS);
-        sweep(FAM_W, k_momentum<2, 1>, 2, 2, IS, JS);
+        if (d0.tiled == 2) {
+            // the k-eps V solver sweeps in the same directions as U and W
+            strip_sweep(FAM_U, k_ml_coef<0, 0, 1>, k_ml_strip<0, 1>, 3, 2, IS, JS);
+            strip_sweep(FAM_V, k_ml_coef<1, 0, 1>, k_ml_strip<1, 1>, 2, 3, IS, JS);
+            strip_sweep(FAM_W, k_ml_coef<2, 0, 1>, k_ml_strip<2, 1>, 2, 2, IS, JS);
+        } else {
+            sweep(FAM_U, k_momentum<0, 1>, 3, 2, IS, JS);
+            sweep(FAM_V, k_momentum<1, 1>, 2, 3, IS, JS);
+            sweep(FAM_W, k_momentum<2, 1>, 2, 2, IS, JS);
+        }
         if (d0.tiled == 2) {
             // strip-pipelined TKE: the neighbour-independent pre-pass, then the strips
             L.go(FAM_TKE, [&] {
