@@ -31,8 +31,8 @@ sys.path.insert(0, ROOT)
 # k-eps momentum: 7 read (TURB instead of VEG), 1 written; TKE: 5 read, 2 written)
 ALG_BYTES = {"pressure": 5 * 4, "u": 8 * 4, "v": 8 * 4, "w": 8 * 4, "tke": 7 * 4}
 KERNEL_OF = {"pressure": "k_pressure_strip", "u": "k_ml_strip<0>", "v": "k_ml_strip<1>", "w": "k_ml_strip<2>"}
-KERNEL_OF_KEPS = {"pressure": "k_pressure_strip", "u": "k_momentum<0, 1>", "v": "k_momentum<1, 1>",
-                  "w": "k_momentum<2, 1>", "tke": "k_tke"}
+KERNEL_OF_KEPS = {"pressure": "k_pressure_strip", "u": "k_ml_strip<0, 1>", "v": "k_ml_strip<1, 1>",
+                  "w": "k_ml_strip<2, 1>", "tke": "k_tke_strip"}
 ALG_BYTES_ITER = 188          # whole SIMPLE iteration, mixing-length model
 ALG_BYTES_ITER_KEPS = 212     # k-eps model (53 array accesses)
 HBM_PEAK_GBS = 8000.0

@@ -516,7 +516,7 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     if (h->kadvmax + 2 > 128) d.tiled = 0;   // the strip kernels hold at most 128 levels per column
     {
         // strip synchronisation: header + one block per sweep kind (pressure, U, V, W)
-        const int rows = std::min(STRIP_ROWS, TKE_ROWS);
+        const int rows = STRIP_ROWS;
         int nstrip = (std::max(h->nii, h->njj) + rows - 1) / rows + 1;
         d.sync_stride = ((2 + nstrip) + 63) & ~63;
         int *fs;

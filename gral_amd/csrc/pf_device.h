@@ -70,6 +70,12 @@ __device__ __forceinline__ void stg4(gf32 *p, float a, float b, float c, float e
     v4f v = {a, b, c, e};
     *(gv4f *)p = v;
 }
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef v2f __attribute__((address_space(1))) gv2f;
+__device__ __forceinline__ float2 ldg2(const gf32 *p) {     // p 8-byte aligned
+    v2f v = *(const gv2f *)p;
+    return make_float2(v.x, v.y);
+}
 // 12-byte records (p 4-byte aligned): one dwordx3 access
 typedef float v3fa __attribute__((ext_vector_type(3), aligned(4)));
 typedef v3fa __attribute__((address_space(1))) gv3f;
@@ -103,7 +109,6 @@ struct GffArgs {
 
 #define SYNC_BLOCK0 64               // Dom::sync: sticky error flag at 0, sweep blocks from here
 #define STRIP_ROWS 15                // sweep rows per strip workgroup (plus one chain wave)
-#define TKE_ROWS 8                   // sweep rows per workgroup of the strip-pipelined TKE sweep
 
 __device__ __forceinline__ size_t c2(const DomG &d, int i, int j) { return (size_t)i * (size_t)d.nj2 + (size_t)j; }
 __device__ __forceinline__ size_t c3(const DomG &d, int i, int j) { return c2(d, i, j) * (size_t)d.nkp; }

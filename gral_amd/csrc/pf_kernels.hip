@@ -2298,127 +2298,237 @@ __global__ __launch_bounds__(WAVE) void k_tke_coef(const Dom *__restrict__ ds) {
     }
 }
 
-// Strip sweep: TKE_ROWS row waves, no chain wave (TKE_ROWS = 8 keeps the
-// thirteen per-row level arrays within 64 KB of LDS).  Row wave r owns sweep row
-// b0 + r (j direction) and at step s handles column a = 2 + s - r; each wave
-// assembles its column's sources (lanes over levels), runs the column's two
-// recurrences on lane 0 and writes the column back.  The neighbours updated
-// earlier in the sweep come from LDS (the same row's previous column, the
-// previous row's column of the previous step) or, for row 0, from the
-// previous strip's published last row; the others are read from memory,
-// where no wave of the sweep has written them yet.
-#define TKE_SLOTS 13                 // per row: D1, D2, C1, den1, P1, C2, den2, P2, wf, wt, we, T, E
-__global__ __launch_bounds__(TKE_ROWS * WAVE) void k_tke_strip(const Dom *__restrict__ ds, int IS, int JS) {
+// Relaxation-free back-substitution of the TKE sweep, downwards from level
+// VI (pointers at VI): X[-t] = max(P*X[-t+1] + Q, floor), or the wall value
+// where the wall flag is set, t = 0..n-1, X[1] = top (the old value above VI).
+// The clamp is the reference's double-precision Math.Max.
+__device__ __forceinline__ void chain_tke_back(const float *__restrict__ P, const float *__restrict__ Q,
+                                               const float *__restrict__ WF, const float *__restrict__ WV,
+                                               float *__restrict__ X, float top, double floor_, int n, int nmax,
+                                               float *sink) {
+    float up = top;
+    float cp[CH], cq[CH], cf[CH], cv[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) { cp[u] = P[-u]; cq[u] = Q[-u]; cf[u] = WF[-u]; cv[u] = WV[-u]; }
+#pragma unroll
+    for (int h = 0; h < MAXCH; ++h) {
+        const int t = h * CH;
+        if (t >= nmax) break;
+        float np[CH], nq[CH], nf[CH], nv[CH];
+        const int o = (t + CH < n) ? t + CH : 0;
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            np[u] = P[-(o + u)]; nq[u] = Q[-(o + u)]; nf[u] = WF[-(o + u)]; nv[u] = WV[-(o + u)];
+        }
+        __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
+        float *Xw = (t < n) ? X : sink + (CH - 1) + t;
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            float x = (cf[u] != 0.0f) ? cv[u] : cp[u] * up + cq[u];
+            x = (float)mmaxd((double)x, floor_);
+            Xw[-(t + u)] = x;
+            up = x;
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) { cp[u] = np[u]; cq[u] = nq[u]; cf[u] = nf[u]; cv[u] = nv[u]; }
+    }
+}
+
+// per-level values of one column of the TKE sweep, loaded one local step ahead
+struct TkePre {
+    float4 c0, c1, c2, c3;          // record words 0..15
+    float2 c4;                      // wall TURB, wall TDISS
+    float ta, ea, tb, eb;           // old TURB/TDISS in the +IS (a) and +JS (b) directions
+    float nit, nie, njt, nje;       // -IS / -JS neighbours as memory holds them
+};
+
+// Strip sweep of TURB and TDISS, laid out as k_ml_strip: ROWS row waves and a
+// chain wave.  Row wave r owns sweep row b0 + r (j direction) and at step s
+// handles column a = 2 + s - r: it assembles the column's two sources from the
+// record (k_tke_coef) and the four neighbours (lanes over levels) and lays the
+// recurrence operands out in LDS; the chain wave then runs the Q recurrences and
+// the back-substitution of every row at once (lane r: TURB of row r, lane
+// ROWS + r: TDISS of row r), while the row waves load the next column.  The
+// neighbours updated earlier in the sweep (the previous column of the same
+// row, the previous row's column of the previous step) are merged on use from
+// the recurrence outputs in LDS over the levels they updated; their other
+// levels, and row 0's neighbours in the previous strip, come from memory.
+// LDS per row: D1, D2, C1, den1, P1, C2, den2, P2, wall flag, wall TURB,
+// wall TDISS, Q1, Q2, new TURB, new TDISS.
+#define TKE_SLOTS 15
+__global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__restrict__ ds, int IS, int JS) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     extern __shared__ float sm[];
     const int nkp = d.nkp, nkl = lds_slot(nkp);
     const int oa = 2, ia = 2, ob = d.nii - 1, ib = d.njj - 1;
-    const int nstrip = (ib - ia + TKE_ROWS) / TKE_ROWS, nsteps = (ob - oa + 1) + TKE_ROWS - 1;
+    const int nstrip = (ib - ia + ROWS) / ROWS, nsteps = (ob - oa + 1) + ROWS - 1;
     const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
+    const bool chainw = r == ROWS;
     gi32 *sync = d.sync + SYNC_BLOCK0 + (size_t)4 * d.sync_stride;
-    float *row = sm + (size_t)r * TKE_SLOTS * nkl;
-    float *sD1 = row, *sD2 = row + nkl, *sC1 = row + 2 * nkl, *sN1 = row + 3 * nkl, *sP1 = row + 4 * nkl;
-    float *sC2 = row + 5 * nkl, *sN2 = row + 6 * nkl, *sP2 = row + 7 * nkl, *sWF = row + 8 * nkl;
-    float *sWT = row + 9 * nkl, *sWE = row + 10 * nkl, *sTo = row + 11 * nkl, *sEo = row + 12 * nkl;
-    float *newT = sm + (size_t)TKE_ROWS * TKE_SLOTS * nkl;        // [2][TKE_ROWS] new TURB columns
-    float *newE = newT + (size_t)2 * TKE_ROWS * nkl;              // [2][TKE_ROWS] new TDISS columns
-    int *sinfo = (int *)(newE + (size_t)2 * TKE_ROWS * nkl);      // [2][TKE_ROWS] valid, lo, hi
-    int *sctl = sinfo + 6 * TKE_ROWS;
+    float *rowb = sm + (size_t)r * TKE_SLOTS * nkl + SLOT_PAD;
+    float *sD1 = rowb, *sD2 = rowb + nkl, *sC1 = rowb + 2 * nkl, *sN1 = rowb + 3 * nkl, *sP1 = rowb + 4 * nkl;
+    float *sC2 = rowb + 5 * nkl, *sN2 = rowb + 6 * nkl, *sP2 = rowb + 7 * nkl, *sWF = rowb + 8 * nkl;
+    float *sWT = rowb + 9 * nkl, *sWE = rowb + 10 * nkl, *sQ1 = rowb + 11 * nkl, *sQ2 = rowb + 12 * nkl;
+    float *sOT = rowb + 13 * nkl, *sOE = rowb + 14 * nkl;
+    float *sdump = sm + (size_t)ROWS * TKE_SLOTS * nkl;          // [WAVE][CH] sinks of the chains
+    int *sinfo = (int *)(sdump + WAVE * CH);                    // [2][ROWS] per row: active, lo, VI, top TURB, top TDISS
+    int *sctl = sinfo + 10 * ROWS;
     if (threadIdx.x == 0) sctl[0] = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int B = sctl[0];
     if (B >= nstrip) return;                                  // uniform over the workgroup
-    const int b = ia + B * TKE_ROWS + r;
+    if (chainw) {
+        for (int s = 0; s < nsteps; ++s) {
+            lds_barrier();                                    // the rows' sources are assembled
+            const int *si = sinfo + (s & 1) * 5 * ROWS;
+            const int f = lane >= ROWS ? 1 : 0, cl = lane < 2 * ROWS ? lane - f * ROWS : 0;
+            const bool on = lane < 2 * ROWS && si[cl * 5];
+            const int lo = si[cl * 5 + 1], VI = si[cl * 5 + 2];
+            const float top = __int_as_float(si[cl * 5 + 3 + f]);
+            const int nfmax = wave_max(VI - lo, on), nbmax = wave_max(VI - lo + 1, on);
+            if (on) {
+                float *R = sm + (size_t)cl * TKE_SLOTS * nkl + SLOT_PAD;
+                float *Dv = R + (size_t)f * nkl, *Cv = R + (size_t)(2 + 3 * f) * nkl, *Nv = R + (size_t)(3 + 3 * f) * nkl;
+                float *Pv = R + (size_t)(4 + 3 * f) * nkl, *WF = R + 8 * (size_t)nkl, *WV = R + (size_t)(9 + f) * nkl;
+                float *Qv = R + (size_t)(11 + f) * nkl, *Ov = R + (size_t)(13 + f) * nkl;
+                chain_fwd<true>(Dv + lo + 1, Cv + lo + 1, Nv + lo + 1, Qv + lo + 1, VI - lo, nfmax, Qv[lo], sdump + lane * CH);
+                chain_tke_back(Pv + VI, Qv + VI, WF + VI, WV + VI, Ov + VI, top, f ? 0.0000001 : 0.001, VI - lo + 1,
+                               nbmax, sdump + lane * CH);
+            }
+            lds_barrier();                                    // the recurrences are done
+        }
+        return;
+    }
+    const int b = ia + B * ROWS + r;
     const bool row_ok = b <= ib;
     StripWait sw{B > 0 ? sync + SYNC_HDR + B - 1 : nullptr, d.sync, B > 0 ? 0 : 0x7fffffff, nsteps,
                  wall_clock64() + STRIP_TIMEOUT_TICKS};
     gi32 *mine = sync + SYNC_HDR + B;
-    const int j = row_ok ? ((JS == -1) ? d.njj - b + 1 : b) : 0;
-    for (int s = 0; s < nsteps; ++s) {
-        const int par = s & 1, prv = par ^ 1;
-        int *info = sinfo + (par * TKE_ROWS + r) * 3;
-        const int l = s - r, a = oa + l;
-        bool on = row_ok && l >= 0 && a <= ob;
-        int i = 0;
-        size_t c = 0;
-        if (on) {
-            i = (IS == -1) ? d.nii - a + 1 : a;
-            c = c2(d, i, j);
-            on = d.adv[c] == 1;
+    TkePre pre[2];
+    Meta mn;
+    int in_ = 0, jn_ = 0;
+    bool inr_n = false;
+    float topt = 0.0f, tope = 0.0f;
+    auto meta_at = [&](int s) {
+        int l = s - r, i, j;
+        if (row_ok && l >= 0 && oa + l <= ob) {
+            sweep_to_ij(d, 0, oa + l, b, IS, JS, i, j);
+            return ldg4i(d.meta + c2(d, i, j));
         }
-        if (r == 0 && row_ok && l >= 0 && a <= ob) sw.wait(s + TKE_ROWS);   // the previous strip's row b0-1 at column a
-        if (on) {
-            const int KK = d.kk[c], VI = d.vi[c], lo = KK + 1;
-            const size_t o = c3(d, i, j);
-            const gf32 *R = d.planes + o * TKE_REC;
-            // sources of the four neighbours: the one before in i (same row, step
-            // s-1) and before in j (row r-1, step s-1) are new; the others old
-            const int *pa = sinfo + (prv * TKE_ROWS + r) * 3;             // this row, previous column
-            const int *pb = r > 0 ? sinfo + (prv * TKE_ROWS + r - 1) * 3 : nullptr;
-            const bool va = l > 0 && pa[0], vb = pb && pb[0];
-            const float *nTa = newT + (size_t)(prv * TKE_ROWS + r) * nkl, *nEa = newE + (size_t)(prv * TKE_ROWS + r) * nkl;
-            const float *nTb = newT + (size_t)(prv * TKE_ROWS + (r > 0 ? r - 1 : 0)) * nkl;
-            const float *nEb = newE + (size_t)(prv * TKE_ROWS + (r > 0 ? r - 1 : 0)) * nkl;
-            const size_t oim = c3(d, i - 1, j), oip = c3(d, i + 1, j), ojm = c3(d, i, j - 1), ojp = c3(d, i, j + 1);
-            const size_t oa_ = (IS == 1) ? oim : oip, ob_ = (JS == 1) ? ojm : ojp;   // new-neighbour columns
-            for (int k = lo + lane; k <= VI; k += WAVE) {
-                const gf32 *rr = R + (size_t)k * TKE_REC;
-                const float4 c0 = ldg4(rr), c1 = ldg4(rr + 4), c2v = ldg4(rr + 8), c3v = ldg4(rr + 12), c4 = ldg4(rr + 16);
-                // neighbour TURB/TDISS: memory unless updated earlier in this sweep
-                float tA = d.turb[oa_ + k], eA = d.tdiss[oa_ + k], tB = d.turb[ob_ + k], eB = d.tdiss[ob_ + k];
-                if (va && k >= pa[1] && k <= pa[2]) { tA = nTa[k]; eA = nEa[k]; }
-                if (vb && k >= pb[1] && k <= pb[2]) { tB = nTb[k]; eB = nEb[k]; }
-                const float tAo = d.turb[((IS == 1) ? oip : oim) + k], eAo = d.tdiss[((IS == 1) ? oip : oim) + k];
-                const float tBo = d.turb[((JS == 1) ? ojp : ojm) + k], eBo = d.tdiss[((JS == 1) ? ojp : ojm) + k];
-                const float tW = (IS == 1) ? tA : tAo, tE = (IS == 1) ? tAo : tA;
-                const float tS = (JS == 1) ? tB : tBo, tN = (JS == 1) ? tBo : tB;
-                const float eW = (IS == 1) ? eA : eAo, eE = (IS == 1) ? eAo : eA;
-                const float eS = (JS == 1) ? eB : eBo, eN = (JS == 1) ? eBo : eB;
-                sD1[k] = c0.x * tW + c0.y * tS + c0.z * tE + c0.w * tN + c1.x + c1.y + c1.z;
-                sD2[k] = c0.x * eW + c0.y * eS + c0.z * eE + c0.w * eN + c1.w + c2v.x;
-                sC1[k] = c2v.y; sN1[k] = c2v.z; sP1[k] = c2v.w;
-                sC2[k] = c3v.x; sN2[k] = c3v.y; sP2[k] = c3v.z; sWF[k] = c3v.w;
-                sWT[k] = c4.x; sWE[k] = c4.y;
-            }
-            for (int k = lo + lane; k <= VI + 1; k += WAVE) { sTo[k] = d.turb[o + k]; sEo[k] = d.tdiss[o + k]; }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            float *oT = newT + (size_t)(par * TKE_ROWS + r) * nkl, *oE = newE + (size_t)(par * TKE_ROWS + r) * nkl;
-            if (lane == 0) {
-                float Q1 = 0, Q2 = 0;
-                for (int k = lo; k <= VI; ++k) {
-                    float q1, q2;
-                    if (k > lo) { q1 = (sD1[k] + sC1[k] * Q1) / sN1[k]; q2 = (sD2[k] + sC2[k] * Q2) / sN2[k]; }
-                    else { q1 = sD1[k] / sN1[k]; q2 = sD2[k] / sN2[k]; }
-                    sD1[k] = q1; sD2[k] = q2;
-                    Q1 = q1; Q2 = q2;
-                }
-                float tup = sTo[VI + 1], eup = sEo[VI + 1];
-                for (int k = VI; k >= lo; --k) {
-                    float tv, ev;
-                    if (sWF[k] != 0.0f) { tv = sWT[k]; ev = sWE[k]; }
-                    else { tv = sP1[k] * tup + sD1[k]; ev = sP2[k] * eup + sD2[k]; }
-                    tv = (float)mmaxd((double)tv, 0.001);
-                    ev = (float)mmaxd((double)ev, 0.0000001);
-                    oT[k] = tv; oE[k] = ev;
-                    tup = tv; eup = ev;
+        return make_int4(0, -1, 0, 0);
+    };
+    int4 mq = meta_at(0);
+    auto locate = [&](int s) {
+        int l = s - r;
+        inr_n = row_ok && l >= 0 && oa + l <= ob;
+        const int4 m4 = mq;
+        mq = meta_at(s + 1);
+        mn.adv = false;
+        if (inr_n) {
+            sweep_to_ij(d, 0, oa + l, b, IS, JS, in_, jn_);
+            mn = unpack_meta(m4);
+        }
+        if (mn.adv) {
+            const size_t o = c3(d, in_, jn_);
+            topt = d.turb[o + mn.VI + 1];
+            tope = d.tdiss[o + mn.VI + 1];
+            const size_t oa_ = c3(d, in_ + IS, jn_), ob_ = c3(d, in_, jn_ + JS), oi = c3(d, in_ - IS, jn_);
+            const size_t oj = c3(d, in_, jn_ - JS);
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = mn.KK + 1 + lane + WAVE * m;
+                if (k <= mn.VI) {
+                    TkePre &p = pre[m];
+                    const gf32 *rr = d.planes + (o + k) * TKE_REC;
+                    p.c0 = ldg4(rr); p.c1 = ldg4(rr + 4); p.c2 = ldg4(rr + 8); p.c3 = ldg4(rr + 12); p.c4 = ldg2(rr + 16);
+                    p.ta = d.turb[oa_ + k]; p.ea = d.tdiss[oa_ + k];
+                    p.tb = d.turb[ob_ + k]; p.eb = d.tdiss[ob_ + k];
+                    p.nit = d.turb[oi + k]; p.nie = d.tdiss[oi + k];
+                    if (r > 0) { p.njt = d.turb[oj + k]; p.nje = d.tdiss[oj + k]; }
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            gf32 *TW = d.turb + o, *EW = d.tdiss + o;
-            for (int k = lo + lane; k <= VI; k += WAVE) { st_through(TW + k, oT[k]); st_through(EW + k, oE[k]); }
-            if (lane == 0) { info[0] = 1; info[1] = lo; info[2] = VI; }
-        } else if (lane == 0) {
-            info[0] = 0;
         }
-        // the last row publishes the strip's progress (s + 1 steps stored)
-        if (r == TKE_ROWS - 1 && (s + 1) % STRIP_PUBLISH == 0) strip_publish(mine, s + 1, lane);
-        __syncthreads();
+    };
+    // row 0's -JS neighbours lie in the previous strip's last row: read after the wait
+    auto locate_prev = [&]() {
+        if (mn.adv) {
+            const size_t oj = c3(d, in_, jn_ - JS);
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = mn.KK + 1 + lane + WAVE * m;
+                if (k <= mn.VI) { pre[m].njt = d.turb[oj + k]; pre[m].nje = d.tdiss[oj + k]; }
+            }
+        }
+    };
+    locate(0);
+    if (r == 0) {
+        sw.wait(ROWS);
+        locate_prev();
     }
-    if (r == TKE_ROWS - 1) strip_publish(mine, nsteps, lane);
+    // the outputs of a row's previous column (levels plo..pvi) are stored during
+    // the next step's assembly, before that step's publish
+    int plo = 1 << 30, pvi = -1;
+    size_t po = 0;
+    auto store_prev = [&]() {
+        gf32 *TW = d.turb + po, *EW = d.tdiss + po;
+        for (int k = plo + lane; k <= pvi; k += WAVE) { st_through(TW + k, sOT[k]); st_through(EW + k, sOE[k]); }
+    };
+    for (int s = 0; s < nsteps; ++s) {
+        const bool act = mn.adv;
+        const Meta mt = mn;
+        const int i = in_, j = jn_;
+        int *si = sinfo + (s & 1) * 5 * ROWS;
+        const int *sp = sinfo + ((s - 1) & 1) * 5 * ROWS;
+        if (act) {
+            const int lo = mt.KK + 1, VI = mt.VI;
+            // levels of the two new neighbours that hold this sweep's values
+            const bool w_in = s - r >= 1;
+            int slo = 1 << 30, svi = -1;
+            if (r >= 1 && sp[(r - 1) * 5]) { slo = sp[(r - 1) * 5 + 1]; svi = sp[(r - 1) * 5 + 2]; }
+            const float *sOTs = sOT - (size_t)TKE_SLOTS * nkl, *sOEs = sOE - (size_t)TKE_SLOTS * nkl;
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                int k = lo + lane + WAVE * m;
+                if (k <= VI) {
+                    const TkePre &p = pre[m];
+                    float tA = p.nit, eA = p.nie, tB = p.njt, eB = p.nje;
+                    if (w_in && k >= plo && k <= pvi) { tA = sOT[k]; eA = sOE[k]; }
+                    if (k >= slo && k <= svi) { tB = sOTs[k]; eB = sOEs[k]; }
+                    const float tW = (IS == 1) ? tA : p.ta, tE = (IS == 1) ? p.ta : tA;
+                    const float tS = (JS == 1) ? tB : p.tb, tN = (JS == 1) ? p.tb : tB;
+                    const float eW = (IS == 1) ? eA : p.ea, eE = (IS == 1) ? p.ea : eA;
+                    const float eS = (JS == 1) ? eB : p.eb, eN = (JS == 1) ? p.eb : eB;
+                    const float4 c0 = p.c0, c1 = p.c1, c2v = p.c2, c3v = p.c3;
+                    const float D1 = c0.x * tW + c0.y * tS + c0.z * tE + c0.w * tN + c1.x + c1.y + c1.z;
+                    const float D2 = c0.x * eW + c0.y * eS + c0.z * eE + c0.w * eN + c1.w + c2v.x;
+                    sD1[k] = D1; sD2[k] = D2;
+                    sC1[k] = c2v.y; sN1[k] = c2v.z; sP1[k] = c2v.w;
+                    sC2[k] = c3v.x; sN2[k] = c3v.y; sP2[k] = c3v.z; sWF[k] = c3v.w;
+                    sWT[k] = p.c4.x; sWE[k] = p.c4.y;
+                    if (k == lo) { sQ1[k] = D1 / c2v.z; sQ2[k] = D2 / c3v.y; }
+                }
+            }
+            if (lane == 0) {
+                si[r * 5 + 1] = lo; si[r * 5 + 2] = VI;
+                si[r * 5 + 3] = __float_as_int(topt); si[r * 5 + 4] = __float_as_int(tope);
+            }
+        }
+        if (lane == 0) si[r * 5] = act;
+        store_prev();
+        plo = act ? mt.KK + 1 : 1 << 30;
+        pvi = act ? mt.VI : -1;
+        po = c3(d, i, j);
+        lds_barrier();
+        if (r == ROWS - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);
+        locate(s + 1);
+        if (r == 0) {
+            sw.wait(s + 1 + ROWS);
+            locate_prev();
+        }
+        lds_barrier();                                        // the recurrence outputs are ready
+    }
+    store_prev();
+    if (r == ROWS - 1) strip_publish(mine, nsteps, lane);
 }
 
 static inline int lds_slot_h(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
@@ -2565,15 +2675,21 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         }
         if (d0.tiled == 2) {
             // strip-pipelined TKE: the neighbour-independent pre-pass, then the strips
-            L.go(FAM_TKE, [&] {
+            L.go(FAM_COEF, [&] {
                 hipLaunchKernelGGL(k_tke_coef, dim3(ncol8 * COLS_PER_WAVE, 1, Z), dim3(WAVE), sizeof(float) * 5 * nkp, s, dd);
             });
             int nst = 0;
-            for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].njj - 1 - 2 + TKE_ROWS) / TKE_ROWS);
-            const size_t shm_k =
-                sizeof(float) * (size_t)(TKE_SLOTS + 4) * TKE_ROWS * lds_slot_h(nkp) + sizeof(int) * (6 * TKE_ROWS + 4);
+            for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].njj - 1 - 2 + ROWS) / ROWS);
+            const size_t shm_k = sizeof(float) * ((size_t)TKE_SLOTS * ROWS * lds_slot_h(nkp) + WAVE * CH) + sizeof(int) * (10 * ROWS + 4);
+            // above 64 KB of LDS (up to about 132 KB at 128 levels); the device's limit is 160 KB
+            static size_t shm_set = 0;
+            if (shm_k > shm_set) {
+                if (hipFuncSetAttribute((const void *)k_tke_strip, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_k) != hipSuccess)
+                    return (int)hipErrorInvalidValue;
+                shm_set = shm_k;
+            }
             L.go(FAM_TKE, [&] {
-                hipLaunchKernelGGL(k_tke_strip, dim3(nst, 1, Z), dim3(TKE_ROWS * WAVE), shm_k, s, dd, IS, JS);
+                hipLaunchKernelGGL(k_tke_strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm_k, s, dd, IS, JS);
             });
         } else {
             size_t shm_t = sizeof(float) * 12 * nkp;
