@@ -980,7 +980,7 @@ struct TkeCol {
     int kkip, kkim, kkjp, kkjm;
     bool cip, cim, cjp, cjm;
 };
-__device__ __forceinline__ TkeCol tke_col(const DomG &d, int i, int j) {
+__device__ __forceinline__ TkeCol tke_col_ptrs(const DomG &d, int i, int j) {
     TkeCol t;
     t.c = c2(d, i, j);
     t.o = c3(d, i, j);
@@ -992,6 +992,10 @@ __device__ __forceinline__ TkeCol tke_col(const DomG &d, int i, int j) {
     t.VKim = d.vk + c3(d, i - 1, j); t.VKimjp = d.vk + c3(d, i - 1, j + 1);
     t.WKip = d.wk + c3(d, i + 1, j); t.WKim = d.wk + c3(d, i - 1, j);
     t.WKjp = d.wk + c3(d, i, j + 1); t.WKjm = d.wk + c3(d, i, j - 1);
+    return t;
+}
+__device__ __forceinline__ TkeCol tke_col(const DomG &d, int i, int j) {
+    TkeCol t = tke_col_ptrs(d, i, j);
     t.kkip = d.kk[c2(d, i + 1, j)]; t.kkim = d.kk[c2(d, i - 1, j)];
     t.kkjp = d.kk[c2(d, i, j + 1)]; t.kkjm = d.kk[c2(d, i, j - 1)];
     t.cip = d.cutk[c2(d, i + 1, j)] > 0; t.cim = d.cutk[c2(d, i - 1, j)] > 0;
@@ -2256,45 +2260,116 @@ extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStrea
 // strip kernel then only assembles the neighbour terms, runs the two Q
 // recurrences and the back-substitution, and writes back.  Record per cell
 // (TKE_REC floats): {AW, AS, AE, AN | ap0*TURB, PROTURB, PROBOU, ap0*TDISS |
-// EPS source, C1, den1, P1 | C2, den2, P2, wall flag | wall TURB, wall TDISS, -, -}.
+// EPS source, C1, C2, wall flag | den1, P1, den2, P2 | wall TURB, wall TDISS, -, -}.
 // ------------------------------------------------------------------------
 #define TKE_REC 20
+#define TKE_SCOL 16       // ints of column data per column in k_tke_coef
 
-__global__ __launch_bounds__(WAVE) void k_tke_coef(const Dom *__restrict__ ds) {
+// A block of 4 waves takes COEF_COLS consecutive columns (j fastest); their
+// levels KKART+1..VI are laid out one after another and dealt to the 256
+// lanes, which compute the cells (tke_cell) and write the first three float4
+// of each record and the wall values.  One wave then runs the two P
+// recurrences of all COEF_COLS columns at once (lane = column), and the lanes
+// write the fourth float4.
+__global__ __launch_bounds__(4 * WAVE) void k_tke_coef(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     extern __shared__ float sm[];
-    const int nkp = d.nkp, tid = threadIdx.x;
-    const int col = xcd_block((d.nii - 2) * (d.njj - 2));
-    if (col < 0) return;
-    const int nj = d.njj - 2, i = 2 + col / nj, j = 2 + col % nj;
-    if (d.adv[c2(d, i, j)] != 1) return;
-    const ColInfo q = col_info(d, i, j);
-    const TkeCol tc = tke_col(d, i, j);
-    float *sA = sm, *sB1 = sA + nkp, *sC1 = sB1 + nkp, *sB2 = sC1 + nkp, *sC2 = sB2 + nkp;
-    gf32 *R = d.planes + tc.o * TKE_REC;
-    for (int k = q.KK + 1 + tid; k <= q.VI; k += WAVE) {
+    const int nkp = d.nkp, nkl = lds_stride(nkp);
+    const int lane = threadIdx.x % WAVE;
+    const int na = d.nii - 2, nb = d.njj - 2;
+    // per column c: A, B1, C1, B2, C2 over k at sm + c*5*nkl, and the column's data
+    int *scol = (int *)(sm + 5 * COEF_COLS * nkl);        // [COEF_COLS][TKE_SCOL]
+    int *stot = scol + TKE_SCOL * COEF_COLS;
+    const int blk = xcd_block((na * nb + COEF_COLS - 1) / COEF_COLS);
+    if (blk < 0) return;                                      // uniform over the block
+    const int col0 = blk * COEF_COLS;
+    if (threadIdx.x < WAVE) {
+        int c = lane, n = 0;
+        if (c < COEF_COLS) {
+            int *sc = scol + c * TKE_SCOL;
+            const int col = col0 + c;
+            int act = 0;
+            if (col < na * nb) {
+                const int i = 2 + col / nb, j = 2 + col % nb;
+                if (d.adv[c2(d, i, j)] == 1) {
+                    const ColInfo q = col_info(d, i, j);
+                    const TkeCol t = tke_col(d, i, j);
+                    act = 1;
+                    n = q.VI - q.KK;
+                    sc[0] = q.KK; sc[1] = q.VI; sc[2] = i; sc[3] = j;
+                    sc[6] = (q.as ? 1 : 0) | (q.an ? 2 : 0) | (q.aw ? 4 : 0) | (q.ae ? 8 : 0) | (t.cip ? 16 : 0) |
+                            (t.cim ? 32 : 0) | (t.cjp ? 64 : 0) | (t.cjm ? 128 : 0);
+                    sc[7] = t.kkip; sc[8] = t.kkim; sc[9] = t.kkjp; sc[10] = t.kkjm;
+                    sc[11] = __float_as_int(q.cutk); sc[12] = __float_as_int(q.ho);
+                }
+            }
+            sc[5] = act;
+        }
+        // inclusive prefix sum of the level counts over the lanes
+        int incl = n;
+        for (int off = 1; off < WAVE; off <<= 1) {
+            int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (c < COEF_COLS) scol[c * TKE_SCOL + 4] = incl - n;
+        if (lane == WAVE - 1) stot[0] = incl;
+    }
+    lds_barrier();
+    const int total = stot[0];
+    auto column_of = [&](int t) {
+        int lo = 0;
+        for (int step = COEF_COLS / 2; step > 0; step >>= 1)
+            if (scol[(lo + step) * TKE_SCOL + 4] <= t) lo += step;
+        return lo;
+    };
+    for (int t = threadIdx.x; t < total; t += 4 * WAVE) {
+        const int c = column_of(t);
+        const int *sc = scol + c * TKE_SCOL;
+        const int k = sc[0] + 1 + (t - sc[4]);
+        ColInfo q;
+        q.KK = sc[0]; q.VI = sc[1]; q.i = sc[2]; q.j = sc[3]; q.KSTART = q.KK + 1; q.W = d.simd;
+        const int fl = sc[6];
+        q.as = fl & 1; q.an = (fl >> 1) & 1; q.aw = (fl >> 2) & 1; q.ae = (fl >> 3) & 1; q.vegp = false;
+        q.cutk = __int_as_float(sc[11]); q.ho = __int_as_float(sc[12]); q.cov = q.ht = q.z0s = 0.0f;
+        TkeCol tc = tke_col_ptrs(d, q.i, q.j);
+        tc.cip = (fl >> 4) & 1; tc.cim = (fl >> 5) & 1; tc.cjp = (fl >> 6) & 1; tc.cjm = (fl >> 7) & 1;
+        tc.kkip = sc[7]; tc.kkim = sc[8]; tc.kkjp = sc[9]; tc.kkjm = sc[10];
         const TkeCell e = tke_cell(d, q, tc, k);
-        gf32 *r = R + (size_t)k * TKE_REC;
+        gf32 *r = d.planes + (tc.o + k) * TKE_REC;
         stg4(r, e.AW1, e.AS1, e.AE1, e.AN1);
         stg4(r + 4, e.w1, e.PRO, e.PBO, e.w2);
-        r[8] = e.EPS; r[9] = e.C1; r[12] = e.C2;
-        r[15] = e.wf ? 1.0f : 0.0f; r[16] = e.wt; r[17] = e.we;
-        sA[k] = e.A; sB1[k] = e.B1; sC1[k] = e.C1; sB2[k] = e.B2; sC2[k] = e.C2;
+        stg4(r + 8, e.EPS, e.C1, e.C2, e.wf ? 1.0f : 0.0f);
+        r[16] = e.wt; r[17] = e.we;
+        float *sA = sm + (size_t)c * 5 * nkl;
+        sA[k] = e.A; sA[nkl + k] = e.B1; sA[2 * nkl + k] = e.C1; sA[3 * nkl + k] = e.B2; sA[4 * nkl + k] = e.C2;
     }
-    __syncthreads();
-    if (tid == 0) {
+    lds_barrier();
+    if (threadIdx.x < COEF_COLS && scol[threadIdx.x * TKE_SCOL + 5]) {
         // P = B/(A - C*P_prev) above KKART+1, B/A at KKART+1 (the recurrence
-        // restarts there; the levels below it do not reach the ones above)
+        // restarts there; the levels below it do not reach the ones above);
+        // den1, P1, den2, P2 replace A, B1, C1, B2
+        const int c = threadIdx.x;
+        float *__restrict__ A = sm + (size_t)c * 5 * nkl;
+        float *__restrict__ B1 = A + nkl, *__restrict__ C1 = B1 + nkl, *__restrict__ B2 = C1 + nkl;
+        float *__restrict__ C2 = B2 + nkl;
+        const int lo = scol[c * TKE_SCOL + 0] + 1, VI = scol[c * TKE_SCOL + 1];
         float P1 = 0, P2 = 0;
-        for (int k = q.KK + 1; k <= q.VI; ++k) {
-            const float A = sA[k];
-            float den1 = A, den2 = A;
-            if (k > q.KK + 1) { den1 = A - sC1[k] * P1; den2 = A - sC2[k] * P2; }
-            P1 = sB1[k] / den1;
-            P2 = sB2[k] / den2;
-            gf32 *r = R + (size_t)k * TKE_REC;
-            r[10] = den1; r[11] = P1; r[13] = den2; r[14] = P2;
+        for (int k = lo; k <= VI; ++k) {
+            const float a = A[k];
+            float den1 = a, den2 = a;
+            if (k > lo) { den1 = a - C1[k] * P1; den2 = a - C2[k] * P2; }
+            P1 = B1[k] / den1;
+            P2 = B2[k] / den2;
+            A[k] = den1; B1[k] = P1; C1[k] = den2; B2[k] = P2;
         }
+    }
+    lds_barrier();
+    for (int t = threadIdx.x; t < total; t += 4 * WAVE) {
+        const int c = column_of(t);
+        const int *sc = scol + c * TKE_SCOL;
+        const int k = sc[0] + 1 + (t - sc[4]);
+        const float *sA = sm + (size_t)c * 5 * nkl;
+        stg4(d.planes + (c3(d, sc[2], sc[3]) + k) * TKE_REC + 12, sA[k], sA[nkl + k], sA[2 * nkl + k], sA[3 * nkl + k]);
     }
 }
 
@@ -2502,10 +2577,10 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
                     const float D1 = c0.x * tW + c0.y * tS + c0.z * tE + c0.w * tN + c1.x + c1.y + c1.z;
                     const float D2 = c0.x * eW + c0.y * eS + c0.z * eE + c0.w * eN + c1.w + c2v.x;
                     sD1[k] = D1; sD2[k] = D2;
-                    sC1[k] = c2v.y; sN1[k] = c2v.z; sP1[k] = c2v.w;
-                    sC2[k] = c3v.x; sN2[k] = c3v.y; sP2[k] = c3v.z; sWF[k] = c3v.w;
+                    sC1[k] = c2v.y; sN1[k] = c3v.x; sP1[k] = c3v.y;
+                    sC2[k] = c2v.z; sN2[k] = c3v.z; sP2[k] = c3v.w; sWF[k] = c2v.w;
                     sWT[k] = p.c4.x; sWE[k] = p.c4.y;
-                    if (k == lo) { sQ1[k] = D1 / c2v.z; sQ2[k] = D2 / c3v.y; }
+                    if (k == lo) { sQ1[k] = D1 / c3v.x; sQ2[k] = D2 / c3v.z; }
                 }
             }
             if (lane == 0) {
@@ -2532,6 +2607,13 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
 }
 
 static inline int lds_slot_h(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
+// raises a kernel's dynamic LDS limit to bytes once (done: the limit set so far)
+static inline int lds_limit(const void *kern, size_t bytes, size_t &done) {
+    if (bytes <= done) return 0;
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return 1;
+    done = bytes;
+    return 0;
+}
 // longest diagonal t over the situations; the pressure sweep runs a over j
 // (outer) and b over i, the momentum sweeps a over i and b over j
 static inline int diag_len_max(const Dom *hd, int nsit, int t, int oa, int ia, bool pressure) {
@@ -2675,19 +2757,19 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         }
         if (d0.tiled == 2) {
             // strip-pipelined TKE: the neighbour-independent pre-pass, then the strips
-            L.go(FAM_COEF, [&] {
-                hipLaunchKernelGGL(k_tke_coef, dim3(ncol8 * COLS_PER_WAVE, 1, Z), dim3(WAVE), sizeof(float) * 5 * nkp, s, dd);
-            });
+            int nblk_t = 0;
+            for (int q = 0; q < nsit; ++q)
+                nblk_t = max(nblk_t, (((hd[q].nii - 2) * (hd[q].njj - 2) + COEF_COLS - 1) / COEF_COLS + NXCD - 1) / NXCD * NXCD);
+            const size_t shm_c = sizeof(float) * 5 * (nkp | 1) * COEF_COLS + sizeof(int) * (TKE_SCOL * COEF_COLS + 4);
+            static size_t shm_cset = 0;                       // above 64 KB from 100 levels on
+            if (lds_limit((const void *)k_tke_coef, shm_c, shm_cset)) return (int)hipErrorInvalidValue;
+            L.go(FAM_COEF, [&] { hipLaunchKernelGGL(k_tke_coef, dim3(nblk_t, 1, Z), dim3(4 * WAVE), shm_c, s, dd); });
             int nst = 0;
             for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].njj - 1 - 2 + ROWS) / ROWS);
             const size_t shm_k = sizeof(float) * ((size_t)TKE_SLOTS * ROWS * lds_slot_h(nkp) + WAVE * CH) + sizeof(int) * (10 * ROWS + 4);
             // above 64 KB of LDS (up to about 132 KB at 128 levels); the device's limit is 160 KB
             static size_t shm_set = 0;
-            if (shm_k > shm_set) {
-                if (hipFuncSetAttribute((const void *)k_tke_strip, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_k) != hipSuccess)
-                    return (int)hipErrorInvalidValue;
-                shm_set = shm_k;
-            }
+            if (lds_limit((const void *)k_tke_strip, shm_k, shm_set)) return (int)hipErrorInvalidValue;
             L.go(FAM_TKE, [&] {
                 hipLaunchKernelGGL(k_tke_strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm_k, s, dd, IS, JS);
             });
