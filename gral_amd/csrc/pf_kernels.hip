@@ -13,6 +13,7 @@
 //                      parallel (lanes over k), stages them in LDS, and one lane
 //                      runs the Thomas recurrences, which are serial in k.
 // This replays the reference's deterministic (single-thread) order exactly.
+#include <mutex>
 #include "pf_device.h"
 
 #define WAVE 64
@@ -1383,7 +1384,7 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
 // are computed.  Pointers are relative to the first level of the recurrence and
 // the loops run over n levels; the LDS slots carry 2*CH floats of padding on
 // both sides, so the look-ahead reads need no clamping (their values are unused).
-#define CH 4
+#define CH 8
 #define SLOT_PAD (2 * CH)
 __device__ __forceinline__ int lds_slot(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
 
@@ -2607,11 +2608,23 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
 }
 
 static inline int lds_slot_h(int nkp) { return (nkp + 2 * SLOT_PAD) | 1; }
-// raises a kernel's dynamic LDS limit to bytes once (done: the limit set so far)
-static inline int lds_limit(const void *kern, size_t bytes, size_t &done) {
-    if (bytes <= done) return 0;
+// raises a kernel's dynamic LDS limit to bytes (above the default 64 KB; the
+// device's is 160 KB) the first time a launch needs it; 1 on failure
+static inline int lds_limit(const void *kern, size_t bytes) {
+    static std::mutex mu;
+    static const void *keys[32];
+    static size_t set[32];
+    static int n = 0;
+    if (bytes <= 65536) return 0;
+    std::lock_guard<std::mutex> g(mu);
+    int q = 0;
+    while (q < n && keys[q] != kern) ++q;
+    if (q < n && bytes <= set[q]) return 0;
+    if (q == n && n == 32) return 1;
     if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return 1;
-    done = bytes;
+    keys[q] = kern;
+    set[q] = bytes;
+    if (q == n) ++n;
     return 0;
 }
 // longest diagonal t over the situations; the pressure sweep runs a over j
@@ -2690,6 +2703,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].nii - 1 - 2 + ROWS) / ROWS);
         size_t shm = sizeof(float) * ((size_t)(7 * ROWS + RING) * lds_slot_h(nkp) + 2 * nkp + 8 + WAVE * CH) +
                      sizeof(int) * (9 * ROWS + 4);
+        if (lds_limit((const void *)k_pressure_strip, shm)) return (int)hipErrorInvalidValue;
         L.go(FAM_PRESSURE, [&] {
             hipLaunchKernelGGL(k_pressure_strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm, s, dd, IS, JS, slot);
         });
@@ -2717,6 +2731,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
                 L.go(fam, [&] { hipLaunchKernelGGL(kern, dim3(len, 1, Z), dim3(WAVE), shm, s, dd, t, is, js); });
         }
     };
+    int lds_err = 0;
     auto strip_sweep = [&](int fam, auto coef, auto strip, int oa, int ia, int is, int js) {
         int nblk = 0, nst = 0;
         for (int q = 0; q < nsit; ++q) {
@@ -2728,6 +2743,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         size_t shm_c = sizeof(float) * 3 * (nkp | 1) * COEF_COLS + sizeof(int) * (SCOL * COEF_COLS + 4);
         L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
         size_t shm_t = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) + sizeof(int) * (10 * ROWS + 4);
+        if (lds_limit((const void *)strip, shm_t)) { lds_err = 1; return; }
         L.go(fam, [&] { hipLaunchKernelGGL(strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm_t, s, dd, is, js); });
     };
     bool all_w8 = true;
@@ -2761,15 +2777,12 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
             for (int q = 0; q < nsit; ++q)
                 nblk_t = max(nblk_t, (((hd[q].nii - 2) * (hd[q].njj - 2) + COEF_COLS - 1) / COEF_COLS + NXCD - 1) / NXCD * NXCD);
             const size_t shm_c = sizeof(float) * 5 * (nkp | 1) * COEF_COLS + sizeof(int) * (TKE_SCOL * COEF_COLS + 4);
-            static size_t shm_cset = 0;                       // above 64 KB from 100 levels on
-            if (lds_limit((const void *)k_tke_coef, shm_c, shm_cset)) return (int)hipErrorInvalidValue;
+            if (lds_limit((const void *)k_tke_coef, shm_c)) return (int)hipErrorInvalidValue;
             L.go(FAM_COEF, [&] { hipLaunchKernelGGL(k_tke_coef, dim3(nblk_t, 1, Z), dim3(4 * WAVE), shm_c, s, dd); });
             int nst = 0;
             for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].njj - 1 - 2 + ROWS) / ROWS);
             const size_t shm_k = sizeof(float) * ((size_t)TKE_SLOTS * ROWS * lds_slot_h(nkp) + WAVE * CH) + sizeof(int) * (10 * ROWS + 4);
-            // above 64 KB of LDS (up to about 132 KB at 128 levels); the device's limit is 160 KB
-            static size_t shm_set = 0;
-            if (lds_limit((const void *)k_tke_strip, shm_k, shm_set)) return (int)hipErrorInvalidValue;
+            if (lds_limit((const void *)k_tke_strip, shm_k)) return (int)hipErrorInvalidValue;
             L.go(FAM_TKE, [&] {
                 hipLaunchKernelGGL(k_tke_strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm_k, s, dd, IS, JS);
             });
@@ -2788,6 +2801,7 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         sweep(FAM_W, k_momentum<2, 2>, 2, 2, IS, JS);
     }
     if (launches) *launches += L.n;
+    if (lds_err) return (int)hipErrorInvalidValue;
     return (int)hipGetLastError();
 }
 

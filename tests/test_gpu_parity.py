@@ -59,6 +59,17 @@ def test_tiny_building(model, iterations):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("model", [1, 2])
+@pytest.mark.parametrize("vcf", [110, 122])
+def test_tall_columns(model, vcf):
+    # KADVMAX 114 and 126 (the tallest column the strip kernels hold): the
+    # strips' LDS slots then exceed the default 64 KB of dynamic LDS
+    c = domain.tiny_case(n=9, vertcellsff=vcf, turbulence_model=model, iterations=3)
+    assert c.kadvmax in (114, 126)
+    _compare(c, 3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", [1, 2])
 def test_tiny_vegetation(model):
     c = domain.tiny_case(n=10, vertcellsff=9, turbulence_model=model, iterations=4, veg=True)
     _compare(c, 4)
