@@ -449,6 +449,12 @@ extern "C" int gral_pf_flat_init(const gral_pf_grid *g, const gral_pf_flat_met *
 // the level centre; u, v, w bilinear in the horizontal and linear in the
 // vertical between that GRAMM level and the next; below it the level is
 // blocked (KKART, AHK, BUI_HEIGHT).
+// topo = 1: the variant with the original GRAL topography (:318-598): the cell
+// index and the level centre in float (above AHKOriMin), a level blocked when
+// its centre is at or below AHKOri + CUTK, the interpolated GRAMM heights
+// absolute in the interior (relative to AHMIN at the GRAMM border, as the
+// reference has it), GRAMM level 1 with level 0 when no GRAMM level lies
+// below the centre.
 // ---------------------------------------------------------------------------
 struct TerrArgs {
     int nii, njj, nkk, nkh, nj2;
@@ -466,6 +472,12 @@ struct TerrArgs {
     float *ahk, *bui;
     float *vec;                          // [n2][2] u, v of the level counted for the 10 m wind, or NaN
     int *kad;
+    int topo;
+    float aom;                           // AHKOriMin (topo)
+    const float *ahko;                   // [n2] AHKOri (topo)
+    const float *vtf;                    // [nkh] HOKART[k-1] + DZK[k]*0.5f + AHKOriMin (float, topo)
+    const float *dzk;                    // [nkh] (topo)
+    int *vlev;                           // [n2] level counted for the 10 m wind, 0: none (topo)
 };
 
 __device__ __forceinline__ float tr_p2(float x) { return x * x; }
@@ -481,8 +493,14 @@ __global__ void __launch_bounds__(64) k_terrain_column(TerrArgs a) {
     const int i = 1 + (int)(blockIdx.x / a.njj), j = 1 + (int)(blockIdx.x % a.njj);
     const size_t c = (size_t)i * a.nj2 + j, o = c * a.nkh;
     const float DX = a.dxk, DY = a.dyk, X1 = a.ddx, Y1 = a.ddy, AHMIN = a.ahmin;
-    int indi = (int)(((double)((float)a.di + DX * (float)i) - (double)DX * 0.5) / (double)X1) + 1;
-    int indj = (int)(((double)((float)a.dj + DY * (float)j) - (double)DY * 0.5) / (double)Y1) + 1;
+    int indi, indj;
+    if (a.topo) {
+        indi = (int)((((float)a.di + DX * (float)i) - DX * 0.5f) / X1) + 1;
+        indj = (int)((((float)a.dj + DY * (float)j) - DY * 0.5f) / Y1) + 1;
+    } else {
+        indi = (int)(((double)((float)a.di + DX * (float)i) - (double)DX * 0.5) / (double)X1) + 1;
+        indj = (int)(((double)((float)a.dj + DY * (float)j) - (double)DY * 0.5) / (double)Y1) + 1;
+    }
     const float xw = ((DX * (float)i - DX * 0.5f + (float)a.di) - (float)(indi - 1) * X1);
     const float yw = ((DY * (float)j - DY * 0.5f + (float)a.dj) - (float)(indj - 1) * Y1);
     const bool interior = indi >= 2 && indi <= a.nx - 1 && indj >= 2 && indj <= a.ny - 1;
@@ -507,8 +525,8 @@ __global__ void __launch_bounds__(64) k_terrain_column(TerrArgs a) {
                 s = q ? s + t : t;
                 sp = q ? sp + tp : tp;
             }
-            ch[ik] = s / gew - AHMIN;
-            chp[ik] = sp / gew - AHMIN;
+            ch[ik] = a.topo ? s / gew : s / gew - AHMIN;
+            chp[ik] = a.topo ? sp / gew : sp / gew - AHMIN;
         }
     }
     __syncthreads();
@@ -516,7 +534,23 @@ __global__ void __launch_bounds__(64) k_terrain_column(TerrArgs a) {
     const float ch1 = interior ? ch[1] : ahe_l[1] - AHMIN;
     // the highest GRAMM level below a GRAL level centre: returns ik (0: blocked),
     // with the heights of that GRAMM level and the next and the height above it
+    const float ground = a.topo ? a.ahko[c] + cut : 0.0f;
     auto search = [&](int k, float &c_h, float &c_hp, float &dz) {
+        if (a.topo) {                    // -1: no GRAMM level below the centre
+            const float vf = a.vtf[k];
+            if (vf <= ground) return 0;
+            for (int ik = a.nz; ik >= 1; --ik) {
+                const int ikp = min(ik + 1, a.nz);
+                c_h = interior ? ch[ik] : ahe_l[ik] - AHMIN;
+                c_hp = interior ? chp[ik] : ahe_l[ikp] - AHMIN;
+                if (vf > c_h + cut) {
+                    dz = vf - (c_h + cut);
+                    return ik;
+                }
+            }
+            dz = 0.0f;
+            return -1;
+        }
         const double vz = a.vertk[k] - (double)AHMIN;
         for (int ik = a.nz; ik >= 1; --ik) {
             const int ikp = min(ik + 1, a.nz);
@@ -531,18 +565,36 @@ __global__ void __launch_bounds__(64) k_terrain_column(TerrArgs a) {
     };
     // pass 1: blocked levels (from the ground up), KKART and AHK
     int kb = 0;
-    float ahk = AHMIN;
+    const float base = a.topo ? a.aom : AHMIN;
+    float ahk = base;
     for (int k = 1 + threadIdx.x; k <= a.nkk; k += 64) {
         float c_h, c_hp, dz;
         if (search(k, c_h, c_hp, dz) == 0) {
             kb = max(kb, k);
-            ahk = tr_max(a.hok[k] + AHMIN, ahk);
+            ahk = tr_max(a.hok[k] + base, ahk);
         }
     }
     for (int m = 32; m >= 1; m >>= 1) {
         kb = max(kb, __shfl_xor(kb, m));
         const float oa = __shfl_xor(ahk, m);
         ahk = oa > ahk ? oa : ahk;
+    }
+    // topo: BUI_HEIGHT of a building column from the blocked levels at or above
+    // AHKOri (k > 1): AHK minus the bottom of the level, the largest one
+    float bmax = -__builtin_inff();
+    bool bany = false;
+    if (a.topo && cut > 0) {
+        for (int k = 2 + threadIdx.x; k <= kb; k += 64)
+            if (a.vtf[k] >= a.ahko[c]) {
+                const float cand = ahk - (a.hok[k - 1] + a.aom);
+                bmax = bany ? tr_max(cand, bmax) : cand;
+                bany = true;
+            }
+        for (int m = 32; m >= 1; m >>= 1) {
+            const float ob = __shfl_xor(bmax, m);
+            const bool oy = __shfl_xor((int)bany, m) != 0;
+            if (oy) { bmax = bany ? (ob > bmax ? ob : bmax) : ob; bany = true; }
+        }
     }
     // pass 2: u, v, w of the free levels; w of the first free level is zeroed by
     // the blocked level below it; the highest free level whose GRAMM level is
@@ -553,8 +605,9 @@ __global__ void __launch_bounds__(64) k_terrain_column(TerrArgs a) {
         float c_h = 0.0f, c_hp = 0.0f, dz = 0.0f;
         const int indk = search(k, c_h, c_hp, dz);
         float u = 0.0f, v = 0.0f, w = 0.0f;
-        if (indk > 0) {
-            const int kp = min(indk + 1, a.nz);
+        if (indk != 0) {
+            const int ik0 = indk > 0 ? indk : 1;
+            const int kp = indk > 0 ? min(indk + 1, a.nz) : 0;
             const int ixm = (xw < X1 * 0.5f) ? max(ci - 1, 1) : min(ci + 1, a.nx);
             const int iym = (yw < Y1 * 0.5f) ? max(cj - 1, 1) : min(cj + 1, a.ny);
             const float fx = xw - X1 * 0.5f, fy = yw - Y1 * 0.5f, sx = (float)(ixm - ci), sy = (float)(iym - cj);
@@ -565,15 +618,19 @@ __global__ void __launch_bounds__(64) k_terrain_column(TerrArgs a) {
                     const float b0 = at(W, ci, y, z);
                     return b0 + (at(W, ixm, y, z) - b0) / X1 * sx * fx;
                 };
-                const float x1 = hx(cj, indk), x2 = hx(iym, indk), x3 = hx(cj, kp), x4 = hx(iym, kp);
+                const float x1 = hx(cj, ik0), x2 = hx(iym, ik0), x3 = hx(cj, kp), x4 = hx(iym, kp);
                 const float x5 = x1 + (x3 - x1) / den * dz, x6 = x2 + (x4 - x2) / den * dz;
                 return x5 + (x6 - x5) / Y1 * sy * fy;
             };
             u = interp(a.uw);
             v = interp(a.vw);
             w = (k == kb + 1 && kb > 0) ? 0.0f : interp(a.ww);
-            const float zz = a.zsp[((size_t)ci * a.gy1 + cj) * a.gz1 + indk] - a.ah[(size_t)ci * a.gy1 + cj];
-            if ((double)zz < a.zlim && k > vkk) { vkk = k; vu = u; vv = v; }
+            if (a.topo) {
+                if (a.vtf[k] < (a.ahko[c] + 10.0f) + 0.5f * a.dzk[k] && k > vkk) vkk = k;
+            } else {
+                const float zz = a.zsp[((size_t)ci * a.gy1 + cj) * a.gz1 + indk] - a.ah[(size_t)ci * a.gy1 + cj];
+                if ((double)zz < a.zlim && k > vkk) { vkk = k; vu = u; vv = v; }
+            }
         }
         a.uk[o + k] = u;
         a.vk[o + k] = v;
@@ -587,7 +644,12 @@ __global__ void __launch_bounds__(64) k_terrain_column(TerrArgs a) {
     if (threadIdx.x == 0) {
         a.kkart[c] = (int16_t)kb;
         a.ahk[c] = ahk;
-        if (kb > 0 && cut > 0) a.bui[c] = tr_max(ahk - AHMIN - ch1, a.bui[c]);
+        if (a.topo) {
+            if (bany) a.bui[c] = tr_max(bmax, a.bui[c]);
+            a.vlev[c] = vkk;
+        } else if (kb > 0 && cut > 0) {
+            a.bui[c] = tr_max(ahk - AHMIN - ch1, a.bui[c]);
+        }
         if (a.adv[c] > 0) atomicMax(a.kad, kb);
         a.vec[2 * c] = vkk > 0 ? vu : __builtin_nanf("");
         a.vec[2 * c + 1] = vkk > 0 ? vv : 0.0f;
@@ -634,6 +696,23 @@ __global__ void __launch_bounds__(64) k_terrain_ghosts(TerrArgs a, const float *
     }
 }
 
+// topo: the ghosts are copied while column i = 1 (j = 1) is set, in the serial
+// order: at the levels where that column is free, u[NII+1][j] takes the value
+// u[NII][j] had before the set-up (its final value when NII = 1); elsewhere the
+// ghost keeps its own.  Likewise v along j.
+__global__ void __launch_bounds__(64) k_topo_ghosts(TerrArgs a, const float *u_old, const float *v_old) {
+    const int q = blockIdx.x;
+    const bool rowu = q < a.njj;
+    const int j = rowu ? q + 1 : 1, i = rowu ? 1 : q - a.njj + 1;
+    const int kfree = a.kkart[(size_t)i * a.nj2 + j];                 // column (1, j) or (i, 1)
+    const size_t cdst = rowu ? (size_t)(a.nii + 1) * a.nj2 + j : (size_t)i * a.nj2 + (a.njj + 1);
+    const bool own = rowu ? a.nii == 1 : a.njj == 1;
+    const size_t cown = rowu ? (size_t)a.nii * a.nj2 + j : (size_t)i * a.nj2 + a.njj;
+    float *f = rowu ? a.uk : a.vk;
+    const float *old = rowu ? u_old + (size_t)j * a.nkh : v_old + (size_t)i * a.nkh;
+    for (int k = kfree + 1 + threadIdx.x; k <= a.nkk; k += 64) f[cdst * a.nkh + k] = own ? f[cown * a.nkh + k] : old[k];
+}
+
 __global__ void k_terrain_save(const float *uk, const float *vk, int nii, int njj, int nkh, float *u_old, float *v_old) {
     const int nj2 = njj + 2;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -645,17 +724,28 @@ __global__ void k_terrain_save(const float *uk, const float *vk, int nii, int nj
     }
 }
 
-extern "C" int gral_pf_terrain_init(const gral_pf_grid *g, const gral_pf_gramm *gr, const float *cutk,
-                                    const uint8_t *advdom, int32_t near_wall, float *uk, float *vk, float *wk,
-                                    int16_t *kkart, float *ahk, float *bui_height, float *ahmin, int32_t *kadvmax,
-                                    double *wind10) {
+// topo: u, v of the level counted for the 10 m wind, after the faces (the
+// reference counts the values its face rules left) and before any damping
+__global__ void k_topo_wind(TerrArgs a) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)a.nii * a.njj) return;
+    const int i = 1 + (int)(t / a.njj), j = 1 + (int)(t % a.njj);
+    const size_t c = (size_t)i * a.nj2 + j;
+    const int lv = a.vlev[c];
+    a.vec[2 * c] = lv > 0 ? a.uk[c * a.nkh + lv] : __builtin_nanf("");
+    a.vec[2 * c + 1] = lv > 0 ? a.vk[c * a.nkh + lv] : 0.0f;
+}
+
+static int terrain_run(const gral_pf_grid *g, const gral_pf_gramm *gr, int topo, const float *ahko, float aom,
+                       const float *cutk, const uint8_t *advdom, int32_t near_wall, float *uk, float *vk, float *wk,
+                       int16_t *kkart, float *ahk, float *bui_height, float *ahmin, int32_t *kadvmax, double *wind10) {
     if (!g || !gr || !cutk || !advdom || !uk || !vk || !wk || !kkart || !ahk || !bui_height || !ahmin || !kadvmax ||
-        !g->dzk || !g->hokart)
+        !g->dzk || !g->hokart || (topo && !ahko))
         return setfail(GRAL_PF_EINVAL, "terrain_init: null argument");
     if (g->nii < 1 || g->njj < 1 || g->nkk < 2 || g->vertcellsff < 0 || !(g->dxk > 0) || !(g->dyk > 0))
         return setfail(GRAL_PF_EINVAL, "terrain_init: bad grid");
     if (gr->nx < 1 || gr->ny < 1 || gr->nz < 1 || gr->nz > TERR_NZ || !(gr->ddx > 0) || !(gr->ddy > 0) || !gr->ahe ||
-        !gr->uwin || !gr->vwin || !gr->wwin || !gr->zsp || !gr->ah)
+        !gr->uwin || !gr->vwin || !gr->wwin || (!topo && (!gr->zsp || !gr->ah)))
         return setfail(GRAL_PF_EINVAL, "terrain_init: bad GRAMM grid (1 <= nz <= 256)");
     if (near_wall < 0 || near_wall > 2) return setfail(GRAL_PF_EINVAL, "terrain_init: near_wall must be 0, 1 or 2");
     const int nii = g->nii, njj = g->njj, nkk = g->nkk, nkh = nkk + 2, nj2 = njj + 2;
@@ -665,21 +755,29 @@ extern "C" int gral_pf_terrain_init(const gral_pf_grid *g, const gral_pf_gramm *
                  nah = (size_t)(nx + 1) * (ny + 1);
     const float AHMIN = *ahmin;
     std::vector<double> vertk(nkh, 0.0);
-    for (int k = 1; k <= nkk; ++k) vertk[k] = (double)g->hokart[k - 1] + (double)g->dzk[k] * 0.5 + (double)AHMIN;
+    std::vector<float> vtf(nkh, 0.0f);
+    for (int k = 1; k <= nkk; ++k) {
+        vertk[k] = (double)g->hokart[k - 1] + (double)g->dzk[k] * 0.5 + (double)AHMIN;
+        vtf[k] = (g->hokart[k - 1] + g->dzk[k] * 0.5f) + aom;
+    }
     Scratch S;
     SETCHK(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
     double *d_vz = S.alloc<double>(nkh);
     float *d_hok = S.alloc<float>(nkh), *d_cut = S.alloc<float>(n2), *d_ahe = S.alloc<float>(nahe);
-    float *d_uw = S.alloc<float>(ngf), *d_vw = S.alloc<float>(ngf), *d_ww = S.alloc<float>(ngf), *d_zsp = S.alloc<float>(ngf);
-    float *d_ah = S.alloc<float>(nah);
+    float *d_uw = S.alloc<float>(ngf), *d_vw = S.alloc<float>(ngf), *d_ww = S.alloc<float>(ngf);
+    float *d_zsp = topo ? nullptr : S.alloc<float>(ngf), *d_ah = topo ? nullptr : S.alloc<float>(nah);
+    float *d_ahko = topo ? S.alloc<float>(n2) : nullptr, *d_vtf = topo ? S.alloc<float>(nkh) : nullptr;
+    float *d_dzk = topo ? S.alloc<float>(nkh) : nullptr;
+    int *d_vlev = topo ? S.alloc<int>(n2) : nullptr;
     float *d_u = S.alloc<float>(n3), *d_v = S.alloc<float>(n3), *d_w = S.alloc<float>(n3);
     float *d_ahk = S.alloc<float>(n2), *d_bui = S.alloc<float>(n2), *d_vec = S.alloc<float>(2 * n2);
     float *d_uo = S.alloc<float>((size_t)nj2 * nkh), *d_vo = S.alloc<float>((size_t)(nii + 2) * nkh);
     int16_t *d_kk = S.alloc<int16_t>(n2);
     uint8_t *d_adv = S.alloc<uint8_t>(n2);
     int *d_kad = S.alloc<int>(1);
-    if (!d_vz || !d_hok || !d_cut || !d_ahe || !d_uw || !d_vw || !d_ww || !d_zsp || !d_ah || !d_u || !d_v || !d_w ||
-        !d_ahk || !d_bui || !d_vec || !d_uo || !d_vo || !d_kk || !d_adv || !d_kad)
+    if (!d_vz || !d_hok || !d_cut || !d_ahe || !d_uw || !d_vw || !d_ww || (!topo && (!d_zsp || !d_ah)) || !d_u || !d_v ||
+        !d_w || !d_ahk || !d_bui || !d_vec || !d_uo || !d_vo || !d_kk || !d_adv || !d_kad ||
+        (topo && (!d_ahko || !d_vtf || !d_dzk || !d_vlev)))
         return setfail(GRAL_PF_ENOMEM, "terrain_init: hipMalloc failed");
     const int one = 1;
     SETCHK(hipMemcpyAsync(d_vz, vertk.data(), nkh * sizeof(double), hipMemcpyHostToDevice, S.s));
@@ -690,8 +788,14 @@ extern "C" int gral_pf_terrain_init(const gral_pf_grid *g, const gral_pf_gramm *
     SETCHK(hipMemcpyAsync(d_uw, gr->uwin, ngf * sizeof(float), hipMemcpyHostToDevice, S.s));
     SETCHK(hipMemcpyAsync(d_vw, gr->vwin, ngf * sizeof(float), hipMemcpyHostToDevice, S.s));
     SETCHK(hipMemcpyAsync(d_ww, gr->wwin, ngf * sizeof(float), hipMemcpyHostToDevice, S.s));
-    SETCHK(hipMemcpyAsync(d_zsp, gr->zsp, ngf * sizeof(float), hipMemcpyHostToDevice, S.s));
-    SETCHK(hipMemcpyAsync(d_ah, gr->ah, nah * sizeof(float), hipMemcpyHostToDevice, S.s));
+    if (topo) {
+        SETCHK(hipMemcpyAsync(d_ahko, ahko, n2 * sizeof(float), hipMemcpyHostToDevice, S.s));
+        SETCHK(hipMemcpyAsync(d_vtf, vtf.data(), nkh * sizeof(float), hipMemcpyHostToDevice, S.s));
+        SETCHK(hipMemcpyAsync(d_dzk, g->dzk, nkh * sizeof(float), hipMemcpyHostToDevice, S.s));
+    } else {
+        SETCHK(hipMemcpyAsync(d_zsp, gr->zsp, ngf * sizeof(float), hipMemcpyHostToDevice, S.s));
+        SETCHK(hipMemcpyAsync(d_ah, gr->ah, nah * sizeof(float), hipMemcpyHostToDevice, S.s));
+    }
     SETCHK(hipMemcpyAsync(d_u, uk, n3 * sizeof(float), hipMemcpyHostToDevice, S.s));
     SETCHK(hipMemcpyAsync(d_v, vk, n3 * sizeof(float), hipMemcpyHostToDevice, S.s));
     SETCHK(hipMemcpyAsync(d_w, wk, n3 * sizeof(float), hipMemcpyHostToDevice, S.s));
@@ -708,14 +812,21 @@ extern "C" int gral_pf_terrain_init(const gral_pf_grid *g, const gral_pf_gramm *
     ta.vertk = d_vz; ta.hok = d_hok; ta.cutk = d_cut; ta.adv = d_adv;
     ta.ahe = d_ahe; ta.uw = d_uw; ta.vw = d_vw; ta.ww = d_ww; ta.zsp = d_zsp; ta.ah = d_ah;
     ta.uk = d_u; ta.vk = d_v; ta.wk = d_w; ta.kkart = d_kk; ta.ahk = d_ahk; ta.bui = d_bui; ta.vec = d_vec; ta.kad = d_kad;
+    ta.topo = topo; ta.aom = aom; ta.ahko = d_ahko; ta.vtf = d_vtf; ta.dzk = d_dzk; ta.vlev = d_vlev;
     const size_t nsave = (size_t)nj2 * nkh + (size_t)(nii + 2) * nkh;
     hipLaunchKernelGGL(k_terrain_save, dim3((unsigned)((nsave + 255) / 256)), dim3(256), 0, S.s, d_u, d_v, nii, njj, nkh,
                        d_uo, d_vo);
     const unsigned ncol = (unsigned)((size_t)nii * njj);
     hipLaunchKernelGGL(k_terrain_column, dim3(ncol), dim3(64), 0, S.s, ta);
     hipLaunchKernelGGL(k_terrain_faces, dim3(ncol), dim3(64), 0, S.s, ta);
-    hipLaunchKernelGGL(k_terrain_ghosts, dim3((unsigned)(nii + njj)), dim3(64), 0, S.s, ta, (const float *)d_uo,
-                       (const float *)d_vo);
+    if (topo) {
+        hipLaunchKernelGGL(k_topo_ghosts, dim3((unsigned)(nii + njj)), dim3(64), 0, S.s, ta, (const float *)d_uo,
+                           (const float *)d_vo);
+        hipLaunchKernelGGL(k_topo_wind, dim3((ncol + 255) / 256), dim3(256), 0, S.s, ta);
+    } else {
+        hipLaunchKernelGGL(k_terrain_ghosts, dim3((unsigned)(nii + njj)), dim3(64), 0, S.s, ta, (const float *)d_uo,
+                           (const float *)d_vo);
+    }
     SETCHK(hipGetLastError());
     // AHMIN of the result: the lowest AHK of the inner columns (MicroscaleTerrain.cs:605-612)
     std::vector<float> h_ahk(n2);
@@ -754,22 +865,46 @@ extern "C" int gral_pf_terrain_init(const gral_pf_grid *g, const gral_pf_gramm *
     *kadvmax = std::min(nkk, kad + g->vertcellsff);
     *ahmin = amin;
     if (wind10) {
-        // the mean wind near 10 m above ground, summed in column order as a serial loop does
+        // the mean wind near 10 m above ground: per row i a sum over j (u in
+        // double; v in double, in float for the topography variant), the row
+        // sums added in i order (MicroscaleTerrain.cs:67, :311-313, :329, :593-595)
         double sx = 0, sy = 0;
         long cnt = 0;
-        for (int i = 1; i <= nii; ++i)
+        for (int i = 1; i <= nii; ++i) {
+            double lx = 0, ly = 0;
+            float lyf = 0.0f;
             for (int j = 1; j <= njj; ++j) {
                 const size_t c = (size_t)i * nj2 + j;
                 if (std::isnan(vec[2 * c])) continue;
-                sx += (double)vec[2 * c];
-                sy += (double)vec[2 * c + 1];
+                lx += (double)vec[2 * c];
+                if (topo) lyf = lyf + vec[2 * c + 1];
+                else ly += (double)vec[2 * c + 1];
                 ++cnt;
             }
+            sx += lx;
+            sy += topo ? (double)lyf : ly;
+        }
         wind10[0] = sx;
         wind10[1] = sy;
         wind10[2] = (double)cnt;
     }
     return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_terrain_init(const gral_pf_grid *g, const gral_pf_gramm *gr, const float *cutk,
+                                    const uint8_t *advdom, int32_t near_wall, float *uk, float *vk, float *wk,
+                                    int16_t *kkart, float *ahk, float *bui_height, float *ahmin, int32_t *kadvmax,
+                                    double *wind10) {
+    return terrain_run(g, gr, 0, nullptr, 0.0f, cutk, advdom, near_wall, uk, vk, wk, kkart, ahk, bui_height, ahmin,
+                       kadvmax, wind10);
+}
+
+extern "C" int gral_pf_terrain_init_topo(const gral_pf_grid *g, const gral_pf_gramm *gramm, const float *ahk_ori,
+                                         float ahk_ori_min, const float *cutk, const uint8_t *advdom, int32_t near_wall,
+                                         float *uk, float *vk, float *wk, int16_t *kkart, float *ahk, float *bui_height,
+                                         float *ahmin, int32_t *kadvmax, double *wind10) {
+    return terrain_run(g, gramm, 1, ahk_ori, ahk_ori_min, cutk, advdom, near_wall, uk, vk, wk, kkart, ahk, bui_height,
+                       ahmin, kadvmax, wind10);
 }
 
 extern "C" const char *gral_pf_setup_last_error(void) { return g_err.c_str(); }

@@ -28,7 +28,7 @@ EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
            "gral_pf_gff_payload", "gral_pf_gff_write", "gral_pf_zip_write",
            "gral_pf_subdomains", "gral_pf_reference_point", "gral_pf_flat_init", "gral_pf_terrain_init",
-           "gral_pf_setup_last_error",
+           "gral_pf_terrain_init_topo", "gral_pf_setup_last_error",
            "gral_pf_last_error", "gral_pf_destroy", "gral_pf_batch_last_error", "gral_pf_batch_destroy"]
 FAMILIES = ("stencil", "pressure", "u", "v", "w", "tke", "coef")
 
@@ -123,6 +123,9 @@ def load(path: str = LIB_PATH):
                                       C.POINTER(C.c_int32)]
     lib.gral_pf_terrain_init.argtypes = [C.POINTER(Grid), C.POINTER(Gramm), FP, U8P, C.c_int32, FP, FP, FP, I16P, FP,
                                          FP, FP, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    lib.gral_pf_terrain_init_topo.argtypes = [C.POINTER(Grid), C.POINTER(Gramm), FP, C.c_float, FP, U8P, C.c_int32,
+                                              FP, FP, FP, I16P, FP, FP, FP, C.POINTER(C.c_int32),
+                                              C.POINTER(C.c_double)]
     lib.gral_pf_setup_last_error.argtypes = []
     lib.gral_pf_setup_last_error.restype = C.c_char_p
     for n in ("gral_pf_last_error", "gral_pf_batch_last_error"):
@@ -433,6 +436,35 @@ def terrain_init(nii, njj, nkk, vertcellsff, dxk, dyk, dzk, hokart, cutk, advdom
                                                int(near_wall), _ptr(out["uk"]), _ptr(out["vk"]), _ptr(out["wk"]),
                                                _ptr(out["kkart"], I16P), _ptr(out["ahk"]), _ptr(out["bui_height"]),
                                                C.byref(am), C.byref(kad), w10), "gral_pf_terrain_init")
+    out.update(ahmin=np.float32(am.value), kadvmax=kad.value, wind10=(w10[0], w10[1], int(w10[2])))
+    return out
+
+
+def terrain_init_topo(nii, njj, nkk, vertcellsff, dxk, dyk, dzk, hokart, cutk, advdom, gramm, ahmin, ahk_ori,
+                      ahk_ori_min, uk, vk, wk, kkart, ahk, bui_height, near_wall=0):
+    """The terrain set-up with the original GRAL topography (MicroscaleTerrain.cs:318-598), on the GPU.
+    gramm as for terrain_init (zsp, ah, zko not read); ahk_ori [nii+2][njj+2], ahk_ori_min its minimum
+    as GRAL reads it.  Returns dict(uk, vk, wk, kkart, ahk, bui_height, ahmin, kadvmax, wind10)."""
+    lib = load()
+    out = {"uk": np.array(uk, dtype=np.float32, order="C"), "vk": np.array(vk, dtype=np.float32, order="C"),
+           "wk": np.array(wk, dtype=np.float32, order="C"), "kkart": np.array(kkart, dtype=np.int16, order="C"),
+           "ahk": np.array(ahk, dtype=np.float32, order="C"),
+           "bui_height": np.array(bui_height, dtype=np.float32, order="C")}
+    keep = {n: _f32(gramm[n]) for n in ("ahe", "uwin", "vwin", "wwin")}
+    keep.update(dzk=_f32(dzk), hok=_f32(hokart), cut=_f32(cutk), adv=np.ascontiguousarray(advdom, dtype=np.uint8),
+                ori=_f32(ahk_ori))
+    g = Grid(nii, njj, nkk, 2, vertcellsff, dxk, dyk, _ptr(keep["dzk"]), _ptr(keep["hok"]))
+    gr = Gramm(int(gramm["nx"]), int(gramm["ny"]), int(gramm["nz"]), float(gramm["ddx"]), float(gramm["ddy"]),
+               int(gramm["delta_i"]), int(gramm["delta_j"]), _ptr(keep["ahe"]), _ptr(keep["uwin"]),
+               _ptr(keep["vwin"]), _ptr(keep["wwin"]), None, None, 0.0, 0.0)
+    am = C.c_float(float(ahmin))
+    kad = C.c_int32(0)
+    w10 = (C.c_double * 3)()
+    _setup_check(lib, lib.gral_pf_terrain_init_topo(C.byref(g), C.byref(gr), _ptr(keep["ori"]), float(ahk_ori_min),
+                                                    _ptr(keep["cut"]), _ptr(keep["adv"], U8P), int(near_wall),
+                                                    _ptr(out["uk"]), _ptr(out["vk"]), _ptr(out["wk"]),
+                                                    _ptr(out["kkart"], I16P), _ptr(out["ahk"]), _ptr(out["bui_height"]),
+                                                    C.byref(am), C.byref(kad), w10), "gral_pf_terrain_init_topo")
     out.update(ahmin=np.float32(am.value), kadvmax=kad.value, wind10=(w10[0], w10[1], int(w10[2])))
     return out
 

@@ -284,6 +284,23 @@ int gral_pf_terrain_init(const gral_pf_grid *grid, const gral_pf_gramm *gramm, c
                          int16_t *kkart, float *ahk, float *bui_height, float *ahmin, int32_t *kadvmax,
                          double *wind10);
 
+/* The same set-up with the original GRAL topography (GRAL_topofile.txt),
+ * MicroscaleTerrain.cs:318-598 with the common tail :600-612 and damping
+ * :664-832: ahk_ori [nii+2][njj+2] is Program.AHKOri and ahk_ori_min
+ * Program.AHKOriMin.  A level is blocked when its centre (float,
+ * HOKART[k-1] + DZK[k]/2 + AHKOriMin) is at or below AHKOri + CUTK; the free
+ * levels take the interpolated GRAMM wind as the reference's variant does
+ * (absolute GRAMM heights in the interior, relative to the incoming *ahmin at
+ * the GRAMM border; GRAMM level 1 with level 0 when none lies below).  AHK
+ * starts at AHKOriMin; BUI_HEIGHT of building columns comes from the blocked
+ * levels at or above AHKOri; the ghosts are the values the reference's serial
+ * order leaves; wind10 counts the highest free level below AHKOri + 10 m +
+ * DZK/2.  gramm->zsp, ah and zko are not read. */
+int gral_pf_terrain_init_topo(const gral_pf_grid *grid, const gral_pf_gramm *gramm, const float *ahk_ori,
+                              float ahk_ori_min, const float *cutk, const uint8_t *advdom, int32_t near_wall,
+                              float *uk, float *vk, float *wk, int16_t *kkart, float *ahk, float *bui_height,
+                              float *ahmin, int32_t *kadvmax, double *wind10);
+
 const char *gral_pf_setup_last_error(void);
 
 /* ---- Batches of weather situations ------------------------------------

@@ -72,3 +72,31 @@ def test_terrain_situation_solved():
                     uk=r["uk"], vk=r["vk"], wk=r["wk"], refp=refp, iter_min=3, iter_max=3)
     print(f"[terrain] solve the GPU-set-up terrain situation kadvmax={c.kadvmax} refp={refp}", flush=True)
     _compare(c, 3, sweeps=("strip",))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("delta,hill,near_wall,low", [((25, 15), 30.0, 0, False), ((5, 3), 45.0, 1, False),
+                                                      ((25, 15), 0.0, 2, True), ((61, 47), 30.0, 0, True)])
+def test_terrain_init_topo_matches_oracle(delta, hill, near_wall, low):
+    # the original-GRAL-topography variant; low: a GRAL topography partly below
+    # the GRAMM surface, so that free levels lie under every GRAMM level
+    from gral_amd import pf
+    nii, njj, nkk = 44, 38, 26
+    dzk, hok, gr, cutk, adv, pre = _inputs(5, nii, njj, nkk, delta, hill)
+    r = np.random.default_rng(17)
+    ahko = (F(100.0) + r.uniform(-25.0 if low else 0.0, 15.0, (nii + 2, njj + 2))).astype(F)
+    aom = F(ahko[1:nii + 1, 1:njj + 1].min())
+    print(f"[terrain-topo] {nii}x{njj}x{nkk} delta={delta} hill={hill} near_wall={near_wall} low={low}", flush=True)
+    want = T.terrain_init_topo(nii, njj, nkk, 6, 2.0, 2.0, dzk, hok, cutk, adv, gr, pre["uk"], pre["vk"], pre["wk"],
+                               pre["kkart"], pre["ahk"], pre["bui"], F(100.0), ahko, aom, near_wall=near_wall)
+    got = pf.terrain_init_topo(nii, njj, nkk, 6, 2.0, 2.0, dzk, hok, cutk, adv, gr, F(100.0), ahko, aom, pre["uk"],
+                               pre["vk"], pre["wk"], pre["kkart"], pre["ahk"], pre["bui"], near_wall=near_wall)
+    for name, a, b in zip(("uk", "vk", "wk", "kkart", "ahk", "bui_height"), want[:6],
+                          (got["uk"], got["vk"], got["wk"], got["kkart"], got["ahk"], got["bui_height"])):
+        view = np.uint32 if a.dtype == F else np.uint16
+        bad = np.argwhere(a.view(view) != b.view(view))
+        assert bad.size == 0, (name, bad[:5].tolist())
+    assert got["ahmin"] == want[6] and got["kadvmax"] == want[7]
+    assert got["wind10"][2] == want[8][2] and got["wind10"][0] == want[8][0] and got["wind10"][1] == want[8][1]
+    assert want[8][2] > 0
+
