@@ -539,7 +539,7 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
             d.rec3 = pl + 12 * n3;
             d.wcol = pl + 20 * n3;
         }
-        if (p->turbulence_model == 1) {
+        if (p->turbulence_model != 2) {          // mixing length and no diffusion
             float *pl;
             if ((rc = dalloc(h, &pl, 15 * n3 + n2))) return rc;
             d.planes = pl;                       // 12 (U, V) or 8 (W) floats per cell

@@ -1259,9 +1259,9 @@ __device__ __forceinline__ Meta unpack_meta(int4 m) {
 // of each cell's record.  Each record is written once, as float4 / float3 stores.
 // WFIX: the solver width when every situation of the launch has it (the
 // width-dependent arithmetic then folds at compile time), 0 to read it per
-// situation.  KIND 0: mixing length (rec3 = {C, TERMP, P}); KIND 1: k-eps, whose
-// solvers divide by the TDMA denominator instead of multiplying by its
-// reciprocal (rec3 = {C, denominator, P}).
+// situation.  KIND 0: mixing length (rec3 = {C, TERMP, P}); KIND 1: k-eps and
+// KIND 2: no diffusion, whose solvers divide by the TDMA denominator instead
+// of multiplying by its reciprocal (rec3 = {C, denominator, P}).
 template <int COMP, int WFIX, int KIND = 0>
 __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds, int IS, int JS) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
@@ -1332,7 +1332,8 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
         q.ho = __int_as_float(sc[11]); q.z0s = __int_as_float(sc[12]);
         q.W = WFIX ? WFIX : d.simd;
         MlPieces p;
-        if constexpr (KIND == 1) p = (COMP == 2) ? ke_w_pieces<true>(d, q, k) : ke_uv_pieces<COMP == 1, true>(d, q, k);
+        if constexpr (KIND != 0)
+            p = (COMP == 2) ? ke_w_pieces<KIND == 1>(d, q, k) : ke_uv_pieces<COMP == 1, KIND == 1>(d, q, k);
         else p = ml_pieces<COMP>(d, q, k);
         float *sA = sm + (size_t)c * 3 * nkl;
         sA[k] = p.A; sA[nkl + k] = p.B; sA[2 * nkl + k] = p.C;
@@ -1356,7 +1357,7 @@ __global__ __launch_bounds__(4 * WAVE) void k_ml_coef(const Dom *__restrict__ ds
         float Pp = 0;
         for (int k = KSTART; k <= VI; ++k) {
             float T, P;
-            if (KIND == 1) {
+            if (KIND != 0) {
                 T = (k > KKP1) ? A[k] - Cq[k] * Pp : A[k];
                 P = Bq[k] / T;
             } else {
@@ -1555,7 +1556,7 @@ __device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *planes, s
     r.x = X[c3(d, i, j) + k];
     r.xa = X[c3(d, i + IS, j) + k];
     r.xb = X[c3(d, i, j + JS) + k];
-    if (KIND == 1) r.ap0 = d.ap0[k];
+    if (KIND != 0) r.ap0 = d.ap0[k];
 }
 
 __device__ __forceinline__ ColInfo meta_col(const Meta &m, int i, int j, int W) {
@@ -1649,8 +1650,8 @@ __device__ __forceinline__ void strip_publish(gi32 *mine, int v, int lane) {
     if (lane == 0) store_agent(mine, v);
 }
 
-// KIND 0: mixing length; KIND 1: k-eps (sources by ke_source, the TDMA divides
-// by the denominators k_ml_coef<.., 1> stored).
+// KIND 0: mixing length; KIND 1: k-eps, KIND 2: no diffusion (sources by
+// ke_source, the TDMA divides by the denominators k_ml_coef<.., KIND> stored).
 template <int COMP, int KIND = 0>
 __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__restrict__ ds, int IS, int JS) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
@@ -1695,7 +1696,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                 float *D = sm + (size_t)cl * 6 * nkl + SLOT_PAD;
                 float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl, *Xn = Q + nkl;
                 const float *Xs = xbuf + (size_t)((s & 1) * ROWS + cl) * nkl;
-                chain_fwd<KIND == 1>(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane * CH);
+                chain_fwd<KIND != 0>(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane * CH);
                 chain_back(P + VI, Q + VI, Xs + VI, Xn + VI, d.relax, nb, nbmax, sdump + lane * CH);
             }
             lds_barrier();                                    // the recurrences are done
@@ -1824,10 +1825,10 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                     pc.AW1 = r_.aw; pc.AS1 = r_.as; pc.AE1 = r_.ae; pc.AN1 = r_.an; pc.e = r_.e; pc.f = r_.f;
                     pc.g = r_.g; pc.w = wcur; pc.c0 = r_.c0; pc.X1 = r_.x1; pc.X2 = r_.x2; pc.X3 = r_.x3; pc.X4 = r_.x4;
                     float D;
-                    if constexpr (KIND == 1) D = ke_source<COMP, true>(d, q, k, sdzk[k], r_.ap0, pc, xw, xs, xe, xn, r_.x);
+                    if constexpr (KIND != 0) D = ke_source<COMP, KIND == 1>(d, q, k, sdzk[k], r_.ap0, pc, xw, xs, xe, xn, r_.x);
                     else D = ml_source<COMP>(d, q, k, sdzk[k], pc, xw, xs, xe, xn, r_.x);
                     sD[k] = D; sC[k] = r_.c; sT[k] = r_.t; sP[k] = r_.p;
-                    if (k <= KKP1) sQ[k] = (KIND == 1) ? D / r_.t : D * r_.t;
+                    if (k <= KKP1) sQ[k] = (KIND != 0) ? D / r_.t : D * r_.t;
                 }
             }
             if (lane == 0) {
@@ -2795,6 +2796,10 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
                     L.go(FAM_TKE, [&] { hipLaunchKernelGGL(k_tke, dim3(len, 1, Z), dim3(WAVE), shm_t, s, dd, t, IS, JS); });
             }
         }
+    } else if (d0.tiled == 2) {
+        strip_sweep(FAM_U, k_ml_coef<0, 0, 2>, k_ml_strip<0, 2>, 3, 2, IS, JS);
+        strip_sweep(FAM_V, k_ml_coef<1, 0, 2>, k_ml_strip<1, 2>, 2, 3, IS, JS);
+        strip_sweep(FAM_W, k_ml_coef<2, 0, 2>, k_ml_strip<2, 2>, 2, 2, IS, JS);
     } else {
         sweep(FAM_U, k_momentum<0, 2>, 3, 2, IS, JS);
         sweep(FAM_V, k_momentum<1, 2>, 2, 3, IS, JS);
