@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 # read; DPM, DPMNEW written; momentum (mixing length): 7 arrays read, 1 written;
 # k-eps momentum: 7 read (TURB instead of VEG), 1 written; TKE: 5 read, 2 written)
 ALG_BYTES = {"pressure": 5 * 4, "u": 8 * 4, "v": 8 * 4, "w": 8 * 4, "tke": 7 * 4}
-KERNEL_OF = {"pressure": "k_pressure_strip", "u": "k_ml_strip<0>", "v": "k_ml_strip<1>", "w": "k_ml_strip<2>"}
+KERNEL_OF = {"pressure": "k_pressure_strip", "u": "k_ml_strip<0, 0>", "v": "k_ml_strip<1, 0>", "w": "k_ml_strip<2, 0>"}
 KERNEL_OF_KEPS = {"pressure": "k_pressure_strip", "u": "k_ml_strip<0, 1>", "v": "k_ml_strip<1, 1>",
                   "w": "k_ml_strip<2, 1>", "tke": "k_tke_strip"}
 ALG_BYTES_ITER = 188          # whole SIMPLE iteration, mixing-length model
