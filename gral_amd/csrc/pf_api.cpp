@@ -40,6 +40,7 @@ struct DiagArgs {
 };
 extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStream_t s);
 extern "C" int pf_launch_gff(const GffArgs *ga, hipStream_t s);
+extern "C" int pf_launch_wind(const WindArgs *wa, hipStream_t s);
 
 namespace {
 
@@ -136,6 +137,8 @@ struct gral_pf_handle {
     void *gff_up = nullptr;
     size_t gff_bytes = 0;
     void *gff_dbuf = nullptr, *gff_hbuf = nullptr;   // packed payload: device buffer, pinned host staging
+    void *pw_buf = nullptr;         // particle wind: the particles' inputs and outputs on the device
+    size_t pw_bytes = 0;
     void *stage_buf = nullptr;      // pinned staging of one device field (put_field / get_field)
     size_t stage_bytes = 0;
     size_t gff_dbuf_bytes = 0, gff_hbuf_bytes = 0;
@@ -1235,6 +1238,81 @@ static void gff_free(gral_pf_handle *h) {
     if (h->gff_hbuf) (void)hipHostFree(h->gff_hbuf);
     h->gff_up = h->gff_dbuf = h->gff_hbuf = nullptr;
     h->gff_bytes = h->gff_dbuf_bytes = h->gff_hbuf_bytes = 0;
+    if (h->pw_buf) (void)hipFree(h->pw_buf);
+    h->pw_buf = nullptr;
+    h->pw_bytes = 0;
+}
+
+// The particle model's wind at n particle positions on the current fields
+// (Zeitschleife.IntWindCalculate, Intwind.cs:31-170).
+extern "C" int gral_pf_wind_at_particles(gral_pf_handle *h, int32_t topo, int32_t ikooagral, int32_t jkooagral,
+                                         float ahmin, int32_t n, const int32_t *cell_i, const int32_t *cell_j,
+                                         const float *ahint, const double *x, const double *y, const float *z,
+                                         float *ux, float *uy, float *uz, int32_t *cell_k) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "wind_at_particles: call prepare first");
+    if (topo != 0 && topo != 1) return fail(h, GRAL_PF_EINVAL, "wind_at_particles: topo must be 0 (flat, buildings) or 1 (terrain)");
+    if (n < 0) return fail(h, GRAL_PF_EINVAL, "wind_at_particles: n < 0");
+    if (n == 0) return GRAL_PF_OK;
+    if (!cell_i || !cell_j || !ahint || !x || !y || !z || !ux || !uy || !uz || !cell_k)
+        return fail(h, GRAL_PF_EINVAL, "wind_at_particles: null argument");
+    for (int32_t p = 0; p < n; ++p)
+        if (cell_i[p] < 1 || cell_i[p] > h->nii || cell_j[p] < 1 || cell_j[p] > h->njj)
+            return fail(h, GRAL_PF_EINVAL, "wind_at_particles: particle " + std::to_string(p) + " outside the cells 1..NII x 1..NJJ");
+    const int nlo = std::min(h->nkp, h->nkh);
+    int rc = gff_upper(h, nlo);
+    if (rc) return rc;
+    const size_t nn = (size_t)n, nk = (size_t)h->nkh;
+    // layout: x, y (double), then HOKART, DZK, ahint, z, ux, uy, uz (float), cell_i, cell_j, cell_k (int)
+    const size_t o_y = 8 * nn, o_hok = 16 * nn, o_dzk = o_hok + 4 * nk, o_ah = o_dzk + 4 * nk, o_z = o_ah + 4 * nn,
+                 o_ux = o_z + 4 * nn, o_uy = o_ux + 4 * nn, o_uz = o_uy + 4 * nn, o_ci = o_uz + 4 * nn,
+                 o_cj = o_ci + 4 * nn, o_ck = o_cj + 4 * nn, bytes = o_ck + 4 * nn;
+    if (bytes > h->pw_bytes) {
+        if (h->pw_buf) (void)hipFree(h->pw_buf);
+        h->pw_buf = nullptr;
+        h->pw_bytes = 0;
+        if (hipMalloc(&h->pw_buf, bytes) != hipSuccess) { h->pw_buf = nullptr; return fail(h, GRAL_PF_ENOMEM, "wind_at_particles: hipMalloc failed"); }
+        h->pw_bytes = bytes;
+    }
+    char *db = static_cast<char *>(h->pw_buf);
+    HIPCHK(h, hipMemcpyAsync(db, x, 8 * nn, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(db + o_y, y, 8 * nn, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(db + o_hok, h->hok.data(), 4 * nk, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(db + o_dzk, h->dzk.data(), 4 * nk, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(db + o_ah, ahint, 4 * nn, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(db + o_z, z, 4 * nn, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(db + o_ci, cell_i, 4 * nn, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(db + o_cj, cell_j, 4 * nn, hipMemcpyHostToDevice, h->stream));
+    WindArgs a{};
+    const float *lo[3] = {h->d.uk, h->d.vk, h->d.wk};
+    const int nup = h->nkh - nlo;
+    const size_t n2 = h->n2();
+    for (int f = 0; f < 3; ++f) {
+        a.lo[f] = lo[f];
+        a.up[f] = nup > 0 ? static_cast<const float *>(h->gff_up) + (size_t)f * n2 * nup : nullptr;
+    }
+    a.nlo = nlo; a.nkp = h->nkp; a.nup = nup; a.nj2 = h->njj + 2; a.nkk = h->nkk;
+    a.topo = topo; a.dxk = h->dxk; a.dyk = h->dyk; a.ahmin = ahmin; a.ikoo = ikooagral; a.jkoo = jkooagral;
+    a.hok = reinterpret_cast<const float *>(db + o_hok);
+    a.dzk = reinterpret_cast<const float *>(db + o_dzk);
+    a.n = n;
+    a.ci = reinterpret_cast<const int32_t *>(db + o_ci);
+    a.cj = reinterpret_cast<const int32_t *>(db + o_cj);
+    a.ahint = reinterpret_cast<const float *>(db + o_ah);
+    a.z = reinterpret_cast<const float *>(db + o_z);
+    a.x = reinterpret_cast<const double *>(db);
+    a.y = reinterpret_cast<const double *>(db + o_y);
+    a.ux = reinterpret_cast<float *>(db + o_ux);
+    a.uy = reinterpret_cast<float *>(db + o_uy);
+    a.uz = reinterpret_cast<float *>(db + o_uz);
+    a.ck = reinterpret_cast<int32_t *>(db + o_ck);
+    if (pf_launch_wind(&a, h->stream)) return fail(h, GRAL_PF_EDEVICE, "wind_at_particles: kernel launch failed");
+    HIPCHK(h, hipMemcpyAsync(ux, db + o_ux, 4 * nn, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(uy, db + o_uy, 4 * nn, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(uz, db + o_uz, 4 * nn, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(cell_k, db + o_ck, 4 * nn, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return GRAL_PF_OK;
 }
 
 extern "C" int gral_pf_gff_payload(gral_pf_handle *h, int32_t kind, const gral_pf_gff_meta *meta, void *out,

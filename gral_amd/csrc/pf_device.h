@@ -107,6 +107,25 @@ struct GffArgs {
     int16_t *out;                     // the payload after its 32-byte header
 };
 
+// Wind at particle positions on the solver's fields (Intwind.cs:31-170): one
+// thread per particle; levels below nlo from the device fields (stride nkp),
+// the ones above from the device copy of the host's upper levels (stride nup).
+struct WindArgs {
+    const float *lo[3];               // UK, VK, WK
+    const float *up[3];
+    int nlo, nkp, nup, nj2, nkk;
+    int topo;                         // 1: terrain, 0: flat terrain with buildings
+    float dxk, dyk, ahmin;
+    int ikoo, jkoo;                   // IKOOAGRAL, JKOOAGRAL
+    const float *hok, *dzk;           // HOKART, DZK [nkk+2]
+    int n;
+    const int32_t *ci, *cj;           // flow-field cell of each particle (FFCellX, FFCellY)
+    const float *ahint, *z;
+    const double *x, *y;
+    float *ux, *uy, *uz;
+    int32_t *ck;                      // IndexK
+};
+
 #define SYNC_BLOCK0 64               // Dom::sync: sticky error flag at 0, sweep blocks from here
 #define STRIP_ROWS 15                // sweep rows per strip workgroup (plus one chain wave)
 

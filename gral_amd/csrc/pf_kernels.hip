@@ -2912,6 +2912,62 @@ __global__ void k_gff_terrain(GffArgs g) {
     }
 }
 
+// ------------------------------------------------------------------------
+// The particle model's wind at a particle (Zeitschleife.IntWindCalculate,
+// Intwind.cs:31-170): the level from HOKART (terrain: BinarySearch of the
+// height above AHMIN, Intwind.cs:174-210; flat terrain with buildings: the
+// first level whose top is at or above the particle), u and v linear in x and
+// y across the cell faces (in double: the particle coordinate is a double),
+// w linear in z between the level's bottom and top faces.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ float wind_field(const WindArgs &a, int f, int i, int j, int k) {
+    const size_t c = (size_t)i * a.nj2 + j;
+    return k < a.nlo ? a.lo[f][c * a.nkp + k] : a.up[f][c * a.nup + (k - a.nlo)];
+}
+
+__global__ __launch_bounds__(256) void k_wind_at_particles(WindArgs a) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const int i = a.ci[p], j = a.cj[p];
+    const float z = a.z[p];
+    float f1 = 0.0f;
+    int k;
+    if (a.topo) {
+        f1 = z - a.ahmin;
+        int low = 1, high = a.nkk;
+        const int q = high >> 3;
+        if (a.hok[q] >= f1) high = q;
+        else low = q;
+        while (low <= high) {
+            const int mid = (low + high) >> 1;
+            if (a.hok[mid] >= f1) high = mid - 1;
+            else low = mid + 1;
+        }
+        k = min(low, a.nkk);
+    } else {
+        k = 1;
+        while (k <= a.nkk && !(z <= a.hok[k])) ++k;
+        k = min(k, a.nkk);
+    }
+    const float DXK = a.dxk, DYK = a.dyk;
+    const float u1 = wind_field(a, 0, i, j, k), u2 = wind_field(a, 0, i + 1, j, k);
+    const float ox = (float)a.ikoo + (float)(i - 1) * DXK;
+    a.ux[p] = (float)((double)u1 + (double)((u2 - u1) / DXK) * (a.x[p] - (double)ox));
+    const float v1 = wind_field(a, 1, i, j, k), v2 = wind_field(a, 1, i, j + 1, k);
+    const float oy = (float)a.jkoo + (float)(j - 1) * DYK;
+    a.uy[p] = (float)((double)v1 + (double)((v2 - v1) / DYK) * (a.y[p] - (double)oy));
+    const float w1 = wind_field(a, 2, i, j, k), w2 = wind_field(a, 2, i, j, k + 1);
+    const float dz = a.topo ? f1 - a.hok[k - 1] : (z + a.ahint[p]) - a.hok[k - 1];
+    a.uz[p] = w1 + ((w2 - w1) / a.dzk[k]) * dz;
+    a.ck[p] = k;
+}
+
+extern "C" int pf_launch_wind(const WindArgs *wa, hipStream_t s) {
+    if (wa->n <= 0) return 0;
+    hipLaunchKernelGGL(k_wind_at_particles, dim3((unsigned)((wa->n + 255) / 256)), dim3(256), 0, s, *wa);
+    return (int)hipGetLastError();
+}
+
 extern "C" int pf_launch_gff(const GffArgs *ga, hipStream_t s) {
     const GffArgs g = *ga;
     if (g.kind == 0)

@@ -26,7 +26,7 @@ EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral
            "gral_pf_diagnostic",
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
-           "gral_pf_gff_payload", "gral_pf_gff_write", "gral_pf_zip_write",
+           "gral_pf_gff_payload", "gral_pf_gff_write", "gral_pf_zip_write", "gral_pf_wind_at_particles",
            "gral_pf_subdomains", "gral_pf_reference_point", "gral_pf_flat_init", "gral_pf_terrain_init",
            "gral_pf_terrain_init_topo", "gral_pf_setup_last_error",
            "gral_pf_last_error", "gral_pf_destroy", "gral_pf_batch_last_error", "gral_pf_batch_destroy"]
@@ -128,6 +128,9 @@ def load(path: str = LIB_PATH):
                                               C.POINTER(C.c_double)]
     lib.gral_pf_setup_last_error.argtypes = []
     lib.gral_pf_setup_last_error.restype = C.c_char_p
+    I32P, F64P = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    lib.gral_pf_wind_at_particles.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_int32, I32P, I32P, FP,
+                                              F64P, F64P, FP, FP, FP, FP, I32P]
     for n in ("gral_pf_last_error", "gral_pf_batch_last_error"):
         getattr(lib, n).argtypes = [H]
         getattr(lib, n).restype = C.c_char_p
@@ -261,6 +264,21 @@ class PrognosticFlowfield:
         a zip archive with one entry named like the file."""
         meta = GffMeta(float(wind_dir), float(wind_vel), int(stab_class))
         self._check(self.lib.gral_pf_gff_write(self.h, int(kind), C.byref(meta), os.fsencode(path)))
+
+    def wind_at_particles(self, topo, ikooagral, jkooagral, ahmin, cell_i, cell_j, ahint, x, y, z):
+        """The particle model's wind at particle positions on the current fields
+        (Intwind.cs:31-170) through gral_pf_wind_at_particles; returns (ux, uy, uz, index_k)."""
+        ci, cj = np.ascontiguousarray(cell_i, np.int32), np.ascontiguousarray(cell_j, np.int32)
+        ah, zz = np.ascontiguousarray(ahint, np.float32), np.ascontiguousarray(z, np.float32)
+        xx, yy = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64)
+        n = len(ci)
+        ux, uy, uz = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+        ck = np.zeros(n, np.int32)
+        I32P, F64P = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+        self._check(self.lib.gral_pf_wind_at_particles(
+            self.h, int(topo), int(ikooagral), int(jkooagral), float(ahmin), n, _ptr(ci, I32P), _ptr(cj, I32P),
+            _ptr(ah), _ptr(xx, F64P), _ptr(yy, F64P), _ptr(zz), _ptr(ux), _ptr(uy), _ptr(uz), _ptr(ck, I32P)))
+        return ux, uy, uz, ck
 
     def download(self):
         shp = self.case.uk.shape

@@ -303,6 +303,21 @@ int gral_pf_terrain_init_topo(const gral_pf_grid *grid, const gral_pf_gramm *gra
 
 const char *gral_pf_setup_last_error(void);
 
+/* ---- The particle model's wind on the flow field ----------------------
+ * Zeitschleife.IntWindCalculate (Intwind.cs:31-170) for n particles on the
+ * handle's current fields (after gral_pf_iterate/solve or a batch): for each
+ * particle in flow-field cell (cell_i, cell_j) = (FFCellX, FFCellY) at
+ * (x, y, z) with the cell's ground height ahint (AHint), returns UXint,
+ * UYint, UZint and IndexK.  topo 1: terrain (Program.Topo ==
+ * TerrainAvailable; the level from BinarySearch of z - ahmin, ahmin =
+ * Program.AHMIN); topo 0: flat terrain with buildings (the first level whose
+ * top is at or above z).  The flat branch without buildings reads no flow
+ * field and is not provided.  Cells must lie in 1..NII x 1..NJJ. */
+int gral_pf_wind_at_particles(gral_pf_handle *h, int32_t topo, int32_t ikooagral, int32_t jkooagral, float ahmin,
+                              int32_t n, const int32_t *cell_i, const int32_t *cell_j, const float *ahint,
+                              const double *x, const double *y, const float *z, float *ux, float *uy, float *uz,
+                              int32_t *cell_k);
+
 /* ---- Batches of weather situations ------------------------------------
  * GRAL calls Calculate() once per weather situation, one after the other
  * (Program.cs main loop -> MicroscaleFlat.cs:381 / MicroscaleTerrain.cs:847);
