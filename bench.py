@@ -38,6 +38,34 @@ ALG_BYTES_ITER_KEPS = 212     # k-eps model (53 array accesses)
 HBM_PEAK_GBS = 8000.0
 
 
+def particle_wind(n, cells):
+    """The particle model's wind at n particles spread uniformly over the
+    solved city field (terrain branch, every height of the column), five calls
+    through the C ABI.  The rate includes the host-to-device and back copies of
+    the particles (48 B each way); the kernel alone is timed by rocprofv3."""
+    from gral_amd import domain, pf
+    c = domain.config2(direction=270.0, iterations=4, n=cells)
+    r = np.random.default_rng(7)
+    ikoo, jkoo, ahmin = 600000, 5200000, 100.0
+    ci = r.integers(1, c.nii + 1, n).astype(np.int32)
+    cj = r.integers(1, c.njj + 1, n).astype(np.int32)
+    x = ikoo + (ci - 1 + r.uniform(0, 1, n)) * c.dxk
+    y = jkoo + (cj - 1 + r.uniform(0, 1, n)) * c.dyk
+    z = (r.uniform(0.0, float(c.hokart[c.nkk]), n) + ahmin).astype(np.float32)
+    ah = np.zeros(n, np.float32)
+    with pf.PrognosticFlowfield(c) as sv:
+        sv.prepare(iterations=4)
+        sv.iterate(2)
+        sv.wind_at_particles(1, ikoo, jkoo, ahmin, ci, cj, ah, x, y, z)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            sv.wind_at_particles(1, ikoo, jkoo, ahmin, ci, cj, ah, x, y, z)
+        wall = (time.perf_counter() - t0) / 5
+    return {"metric": "particle wind evaluations/s (with host transfers)", "value": n / wall, "particles": n,
+            "ms_per_call": wall * 1e3, "workload": f"city block {cells}x{cells}, 270 deg, fields after 2 iterations, "
+            "particles uniform over cells and heights", "alg_bytes_per_particle": 24 + 32 + 16}
+
+
 def situations_per_hour(n, directions):
     """Flow situations per hour, batched against plain, at equal residual.
 
@@ -200,6 +228,9 @@ def main():
                     help="also solve the first K situations of the batched workload, batched and plain")
     ap.add_argument("--turbulence-model", type=int, default=1, choices=(1, 2),
                     help="1: mixing length (the benchmark's model), 2: k-eps")
+    ap.add_argument("--particle-wind", type=int, default=0, metavar="N",
+                    help="only evaluate the particle model's wind (gral_pf_wind_at_particles) at N particles on the "
+                         "solved 270 deg city field, 5 calls, and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     args = ap.parse_args()
@@ -217,6 +248,9 @@ def main():
         print(json.dumps({"error": "this solver runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.particle_wind:
+        print(json.dumps(particle_wind(args.particle_wind, args.n)))
+        return
 
     from gral_amd import domain, pf
 
