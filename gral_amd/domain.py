@@ -414,3 +414,94 @@ def config3(situation: dict, iterations: int = 500) -> Case:
                   iterations=iterations)
     c.iter_min = min(100, iterations)
     return c
+
+
+# ---------------------------------------------------------------------------
+# Terrain-following situations from a synthetic GRAMM field (the terrain
+# configuration: seeded Gaussian hills, buildings, vegetation, k-eps)
+# ---------------------------------------------------------------------------
+
+def gramm_hills(extent_x: float, extent_y: float, ddx: float = 50.0, nz: int = 15, seed: int = 20260005,
+                hills: int = 12, speed: float = 3.0, direction: float = 270.0, margin: int = 2) -> dict:
+    """A GRAMM field over ``hills`` seeded Gaussian hills (amplitudes 20-150 m,
+    widths 100-800 m) on 100 m of ground: terrain-following levels, a
+    logarithmic wind of ``speed`` at 10 m from ``direction``.  The GRAL domain
+    starts ``margin`` GRAMM cells inside the field.  Shapes as gral_pf_gramm."""
+    rng = np.random.default_rng(seed)
+    nx = int(math.ceil(extent_x / ddx)) + 2 * margin
+    ny = int(math.ceil(extent_y / ddx)) + 2 * margin
+    x = (np.arange(nx + 2) - 1 - margin) * ddx                     # corner coordinates in the GRAL frame
+    y = (np.arange(ny + 2) - 1 - margin) * ddx
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    surf = np.full(X.shape, 100.0)
+    for _ in range(hills):
+        cx, cy = rng.uniform(0, extent_x), rng.uniform(0, extent_y)
+        amp, wid = rng.uniform(20.0, 150.0), rng.uniform(100.0, 800.0)
+        surf += amp * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2.0 * wid ** 2))
+    lev = np.cumsum(np.r_[0.0, 0.0, 10.0 * 1.25 ** np.arange(nz)])[:nz + 2]
+    ahe = (surf[:, :, None] + lev[None, None, :]).astype(F32)
+    zsp = (0.5 * (ahe[:nx + 1, :ny + 1, :nz + 1] + ahe[:nx + 1, :ny + 1, 1:nz + 2])).astype(F32)
+    ah = ahe[:nx + 1, :ny + 1, 1].astype(F32)
+    zc = np.maximum(0.5 * (lev[:nz + 1] + lev[1:nz + 2]), 0.5)
+    prof = speed * np.log(zc / 0.5) / math.log(10.0 / 0.5)
+    u0, v0 = wind_components(1.0, direction)
+    u = np.broadcast_to((prof * u0).astype(F32), (nx + 1, ny + 1, nz + 1)).astype(F32)
+    v = np.broadcast_to((prof * v0).astype(F32), (nx + 1, ny + 1, nz + 1)).astype(F32)
+    w = np.zeros((nx + 1, ny + 1, nz + 1), F32)
+    return {"nx": nx, "ny": ny, "nz": nz, "ddx": ddx, "ddy": ddx, "delta_i": int(margin * ddx),
+            "delta_j": int(margin * ddx), "ahe": ahe, "uwin": u, "vwin": v, "wwin": w, "zsp": zsp, "ah": ah,
+            "zko1": 0.0, "zko2": float(lev[2] - lev[1])}
+
+
+def config5(direction: float = 270.0, speed: float = 3.0, stability: int = 4, n: int = 1000, dx: float = 4.0,
+            iterations: int = 4, seed: int = 20260005, vertcellsff: int = 10) -> Case:
+    """The terrain configuration without its adjoint: an n x n GRAL grid (dx =
+    4 m, 30-40 stretched levels) over 12 seeded Gaussian hills of a GRAMM
+    field, random cuboid buildings at 15 % plan density, vegetation patches
+    with leaf-area density 0.5-2 m^2/m^3, k-eps; set up on the GPU as GRAL's
+    terrain path does (``pf.subdomains``, ``pf.terrain_init``,
+    ``pf.reference_point``).  The terrain, buildings and vegetation depend on
+    ``seed`` only, the wind on ``direction``/``speed``/``stability``."""
+    from gral_amd import pf
+    gr = gramm_hills(n * dx, n * dx, seed=seed, speed=speed, direction=direction)
+    # stretched levels from 2 m up to 250 m above the highest ground
+    relief = float(gr["ah"].max() - gr["ah"].min())
+    nkk, dzk, hok = vertical_grid_flat(2.0, 1.1, relief + 250.0)
+    n2 = (n + 2, n + 2)
+    rng = np.random.default_rng(seed + 1)
+    bl = city_buildings(n, n, dx, seed + 2, density=0.15, fmin=8.0, fmax=40.0, hmin=6.0, hmax=40.0, margin=3)
+    cutk = np.zeros(n2, F32)
+    for b in bl:
+        sl = (slice(max(1, b.i0), min(n, b.i1) + 1), slice(max(1, b.j0), min(n, b.j1) + 1))
+        cutk[sl] = np.maximum(cutk[sl], F32(b.height))
+    # vegetation patches off the buildings: COV 0.3-0.9, LAD 0.5-2 in the lowest 5 free levels
+    cov = np.zeros(n2, F32)
+    for _ in range(max(1, n * n // 4000)):
+        w, h = int(rng.integers(5, 30)), int(rng.integers(5, 30))
+        i0, j0 = int(rng.integers(1, n - w)), int(rng.integers(1, n - h))
+        cov[i0:i0 + w, j0:j0 + h] = F32(rng.uniform(0.3, 0.9))
+    cov[cutk > 0] = 0
+    adv = pf.subdomains(cutk, n, n, dx, dx, 15)
+    adv[1:n + 1, 1:n + 1] |= (cov[1:n + 1, 1:n + 1] > 0).astype(np.uint8)
+    shp3 = (n + 2, n + 2, nkk + 2)
+    z3 = np.zeros(shp3, F32)
+    ahmin = F32(gr["ah"].min())
+    r = pf.terrain_init(n, n, nkk, vertcellsff, dx, dx, dzk, hok, cutk, adv, gr, ahmin, z3, z3, z3,
+                        np.zeros(n2, np.int16), np.zeros(n2, F32), np.zeros(n2, F32))
+    kk = r["kkart"]
+    lad = rng.uniform(0.5, 2.0, shp3).astype(F32)
+    k = np.arange(nkk + 2)[None, None, :]
+    canopy = (k > kk[:, :, None]) & (k <= kk[:, :, None] + 5) & (cov[:, :, None] > 0)
+    veg = np.where(canopy, F32(0.3) * cov[:, :, None] ** 3 * lad, F32(0)).astype(F32)
+    refp, ok = pf.reference_point(adv)
+    if not ok:
+        raise ValueError("no prognostic sub-domain in this situation")
+    ob = STABILITY_OBUKHOV[int(stability)]
+    z0 = np.full(n2, F32(0.5), F32)
+    ustar = speed * 0.4 / math.log(10.0 / 0.5)
+    return Case(name=f"terrain{n}_{int(direction)}", nii=n, njj=n, nkk=nkk, kadvmax=r["kadvmax"],
+                vertcellsff=vertcellsff, dxk=float(F32(dx)), dyk=float(F32(dx)), dzk=dzk, hokart=hok, advdom=adv,
+                kkart=kk, cutk=cutk, cov=cov, ahk=r["ahk"], z0_wall=z0.copy(), z0_solver=z0.copy(),
+                ob_init=np.full(n2, F32(ob), F32), ust_init=np.full(n2, F32(ustar), F32), z0_init=z0.copy(),
+                uk=r["uk"], vk=r["vk"], wk=r["wk"], refp=refp, veg=veg, veg_present=(cov > 0).astype(np.uint8),
+                turbulence_model=2, iter_min=iterations, iter_max=iterations, coriolis=coriolis_parameter(47.0))
