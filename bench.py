@@ -293,9 +293,10 @@ def main():
     # HBM bytes per launch of that kernel, from the calibrated FETCH_SIZE / WRITE_SIZE
     # counter passes committed with the profiles (same workload and kernel version)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_city400_8dir.json")
-    if os.path.exists(tpath) and args.n == 400 and args.directions == 8 and model == 1:
-        kt = json.load(open(tpath))["kernels"].get(KERNEL_OF[dom])
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_city400_8dir.json" if model == 1 else
+                         "hbm_traffic_city400_8dir_keps.json")
+    if os.path.exists(tpath) and args.n == 400 and args.directions == 8:
+        kt = json.load(open(tpath))["kernels"].get(kernel_of[dom])
         if kt:
             traffic = kt["traffic_bytes"]
     roofline = {"bound": "hbm", "kernel": kernel_of[dom], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
