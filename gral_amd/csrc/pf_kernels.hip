@@ -2261,10 +2261,13 @@ extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStrea
 // no source (P and its denominator, per level), and the wall function.  The
 // strip kernel then only assembles the neighbour terms, runs the two Q
 // recurrences and the back-substitution, and writes back.  Record per cell
-// (TKE_REC floats): {AW, AS, AE, AN | ap0*TURB, PROTURB, PROBOU, ap0*TDISS |
-// EPS source, C1, C2, wall flag | den1, P1, den2, P2 | wall TURB, wall TDISS, -, -}.
+// (TKE_REC floats, written whole by the first pass): {AW, AS, AE, AN |
+// ap0*TURB, PROTURB, PROBOU, ap0*TDISS | EPS source, C1, C2, wall flag | wall
+// TURB, wall TDISS, 0, 0}; after the records (at TKE_REC*n3 floats), per cell
+// {den1, P1, den2, P2}, written whole by the last pass.
 // ------------------------------------------------------------------------
-#define TKE_REC 20
+#define TKE_REC 16
+__device__ __forceinline__ size_t tke_rec4(const DomG &d) { return (size_t)TKE_REC * (d.nii + 2) * d.nj2 * d.nkp; }
 #define TKE_SCOL 16       // ints of column data per column in k_tke_coef
 
 // A block of 4 waves takes COEF_COLS consecutive columns (j fastest); their
@@ -2341,7 +2344,7 @@ __global__ __launch_bounds__(4 * WAVE) void k_tke_coef(const Dom *__restrict__ d
         stg4(r, e.AW1, e.AS1, e.AE1, e.AN1);
         stg4(r + 4, e.w1, e.PRO, e.PBO, e.w2);
         stg4(r + 8, e.EPS, e.C1, e.C2, e.wf ? 1.0f : 0.0f);
-        r[16] = e.wt; r[17] = e.we;
+        stg4(r + 12, e.wt, e.we, 0.0f, 0.0f);
         float *sA = sm + (size_t)c * 5 * nkl;
         sA[k] = e.A; sA[nkl + k] = e.B1; sA[2 * nkl + k] = e.C1; sA[3 * nkl + k] = e.B2; sA[4 * nkl + k] = e.C2;
     }
@@ -2371,7 +2374,7 @@ __global__ __launch_bounds__(4 * WAVE) void k_tke_coef(const Dom *__restrict__ d
         const int *sc = scol + c * TKE_SCOL;
         const int k = sc[0] + 1 + (t - sc[4]);
         const float *sA = sm + (size_t)c * 5 * nkl;
-        stg4(d.planes + (c3(d, sc[2], sc[3]) + k) * TKE_REC + 12, sA[k], sA[nkl + k], sA[2 * nkl + k], sA[3 * nkl + k]);
+        stg4(d.planes + tke_rec4(d) + (c3(d, sc[2], sc[3]) + k) * 4, sA[k], sA[nkl + k], sA[2 * nkl + k], sA[3 * nkl + k]);
     }
 }
 
@@ -2413,7 +2416,8 @@ __device__ __forceinline__ void chain_tke_back(const float *__restrict__ P, cons
 
 // per-level values of one column of the TKE sweep, loaded one local step ahead
 struct TkePre {
-    float4 c0, c1, c2, c3;          // record words 0..15
+    float4 c0, c1, c2;              // record words 0..11
+    float4 c3;                      // den1, P1, den2, P2
     float2 c4;                      // wall TURB, wall TDISS
     float ta, ea, tb, eb;           // old TURB/TDISS in the +IS (a) and +JS (b) directions
     float nit, nie, njt, nje;       // -IS / -JS neighbours as memory holds them
@@ -2517,7 +2521,8 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
                 if (k <= mn.VI) {
                     TkePre &p = pre[m];
                     const gf32 *rr = d.planes + (o + k) * TKE_REC;
-                    p.c0 = ldg4(rr); p.c1 = ldg4(rr + 4); p.c2 = ldg4(rr + 8); p.c3 = ldg4(rr + 12); p.c4 = ldg2(rr + 16);
+                    p.c0 = ldg4(rr); p.c1 = ldg4(rr + 4); p.c2 = ldg4(rr + 8); p.c4 = ldg2(rr + 12);
+                    p.c3 = ldg4(d.planes + tke_rec4(d) + (o + k) * 4);
                     p.ta = d.turb[oa_ + k]; p.ea = d.tdiss[oa_ + k];
                     p.tb = d.turb[ob_ + k]; p.eb = d.tdiss[ob_ + k];
                     p.nit = d.turb[oi + k]; p.nie = d.tdiss[oi + k];
