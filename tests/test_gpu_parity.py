@@ -3,8 +3,9 @@
 The oracle restates the reference's single-thread (UseFixedRndSeedVal) order,
 FP32 without FMA, FP64 quantisation.  The HIP path replays the same order as
 anti-diagonal wavefronts with the same arithmetic, so the bar here is bit
-equality of every field; the north-star tolerance (relative L2 <= 1e-4,
-max-abs <= 1e-3 m/s) is asserted as well so a failure says how far off it is.
+equality of every field; the floating-point tolerance of the target (relative
+L2 <= 1e-4, max-abs <= 1e-3 m/s) is asserted as well so a failure says how far
+off it is.
 """
 import os
 
