@@ -41,6 +41,7 @@ struct DiagArgs {
 extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStream_t s);
 extern "C" int pf_launch_gff(const GffArgs *ga, hipStream_t s);
 extern "C" int pf_launch_wind(const WindArgs *wa, hipStream_t s);
+extern "C" int pf_launch_mass(const Dom *dd, int nsit, int nblk, double *partial, double *out, hipStream_t s);
 
 namespace {
 
@@ -112,6 +113,15 @@ struct gral_pf_handle {
     std::vector<float> cutk, cov, ahk, veg;
     std::vector<float> z0w, z0s, obi, usti, z0i;
     bool masks = false, rough = false, fields = false, has_veg = false;
+    // meteorological profile of the TKE initialisation: Program.IStatistics,
+    // MetProfileNumb, MeasurementHeight[], U0[], V0[] (index 0 unused)
+    int istat = 4, nprof = 0;
+    std::vector<float> mh, u0, v0;
+    size_t dev_bytes = 0;           // device bytes allocated by prepare
+    double *mass_buf = nullptr;     // FP64 mass imbalance: [mass_nblk][2] block partials, then the [2] result
+    int mass_nblk = 0;
+    double mass[2] = {0, 0};        // sum |DIV|, sum |face flux| at the last finish / mass_imbalance call
+    size_t dev_cap = 0;             // optional cap (GRAL_PF_DEVICE_MB), 0 = none
     std::vector<float> uk, vk, wk, uks, vks, wks, dpm, dpmnew, turb, tdiss;
     std::vector<int> vi;
     // run state
@@ -166,9 +176,13 @@ static int fail(gral_pf_handle *h, int code, const std::string &msg) {
 template <class T>
 static int dalloc(gral_pf_handle *h, T **p, size_t n) {
     void *q = nullptr;
-    hipError_t e = hipMalloc(&q, n * sizeof(T) + 256);
+    const size_t bytes = n * sizeof(T) + 256;
+    if (h->dev_cap && h->dev_bytes + bytes > h->dev_cap)
+        return fail(h, GRAL_PF_ENOMEM, "device memory budget (GRAL_PF_DEVICE_MB) exceeded");
+    hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return fail(h, GRAL_PF_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     h->dev.push_back(q);
+    h->dev_bytes += bytes;
     *p = (T *)q;
     return 0;
 }
@@ -190,6 +204,9 @@ extern "C" int gral_pf_create(const gral_pf_grid *g, gral_pf_handle **out) {
     h->nkp = ((g->kadvmax + 2) + 3) & ~3;
     h->dzk.assign(g->dzk, g->dzk + h->nkh);
     h->hok.assign(g->hokart, g->hokart + h->nkh);
+    // optional per-handle budget of device memory in MiB (prepare fails with
+    // GRAL_PF_ENOMEM beyond it), e.g. to keep room for other situations
+    if (const char *cap = std::getenv("GRAL_PF_DEVICE_MB")) h->dev_cap = (size_t)std::strtoull(cap, nullptr, 10) << 20;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
         gral_pf_destroy(h);
@@ -202,6 +219,7 @@ extern "C" int gral_pf_create(const gral_pf_grid *g, gral_pf_handle **out) {
 extern "C" int gral_pf_set_masks(gral_pf_handle *h, const uint8_t *advdom, const int16_t *kkart, const float *cutk,
                                  const float *cov, const float *ahk, const float *veg, const uint8_t *veg_present) {
     if (!h) return GRAL_PF_EINVAL;
+    h->prepared = false;
     if (!advdom || !kkart || !cutk || !cov || !ahk) return fail(h, GRAL_PF_EINVAL, "set_masks: null mask array");
     if ((veg == nullptr) != (veg_present == nullptr))
         return fail(h, GRAL_PF_EINVAL, "set_masks: veg and veg_present must both be given or both be NULL");
@@ -231,6 +249,7 @@ extern "C" int gral_pf_set_masks(gral_pf_handle *h, const uint8_t *advdom, const
 extern "C" int gral_pf_set_roughness(gral_pf_handle *h, const float *z0_wall, const float *z0_solver, const float *ob_init,
                                      const float *ust_init, const float *z0_init) {
     if (!h) return GRAL_PF_EINVAL;
+    h->prepared = false;
     if (!z0_wall || !z0_solver || !ob_init || !ust_init || !z0_init) return fail(h, GRAL_PF_EINVAL, "set_roughness: null array");
     size_t n2 = h->n2();
     h->z0w.assign(z0_wall, z0_wall + n2);
@@ -243,8 +262,32 @@ extern "C" int gral_pf_set_roughness(gral_pf_handle *h, const float *z0_wall, co
     return GRAL_PF_OK;
 }
 
+extern "C" int gral_pf_set_met_profile(gral_pf_handle *h, int32_t istatistics, int32_t n_profile,
+                                       const float *meas_height, const float *u0, const float *v0) {
+    if (!h) return GRAL_PF_EINVAL;
+    h->prepared = false;
+    if (istatistics < 0 || istatistics > 4) return fail(h, GRAL_PF_EINVAL, "set_met_profile: IStatistics must be 0..4");
+    if (istatistics == 0 || istatistics == 3) {
+        if (n_profile < 1 || n_profile > 50) return fail(h, GRAL_PF_EINVAL, "set_met_profile: n_profile must be 1..50");
+        if (!meas_height || !u0 || !v0) return fail(h, GRAL_PF_EINVAL, "set_met_profile: null profile array");
+        for (int q = 2; q <= n_profile; ++q)
+            if (!(meas_height[q] > meas_height[q - 1]))
+                return fail(h, GRAL_PF_EINVAL, "set_met_profile: measurement heights must increase");
+        h->mh.assign(meas_height, meas_height + n_profile + 1);
+        h->u0.assign(u0, u0 + n_profile + 1);
+        h->v0.assign(v0, v0 + n_profile + 1);
+        h->nprof = n_profile;
+    } else {
+        h->mh.clear(); h->u0.clear(); h->v0.clear();
+        h->nprof = 0;
+    }
+    h->istat = istatistics;
+    return GRAL_PF_OK;
+}
+
 extern "C" int gral_pf_upload_fields(gral_pf_handle *h, const float *uk, const float *vk, const float *wk) {
     if (!h) return GRAL_PF_EINVAL;
+    h->prepared = false;
     if (!uk || !vk || !wk) return fail(h, GRAL_PF_EINVAL, "upload_fields: null field");
     size_t n3 = h->n2() * (size_t)h->nkh;
     par_assign(h->uk, uk, n3);
@@ -259,6 +302,9 @@ extern "C" int gral_pf_upload_fields(gral_pf_handle *h, const float *uk, const f
 static void free_device(gral_pf_handle *h) {
     for (void *p : h->dev) (void)hipFree(p);
     h->dev.clear();
+    h->dev_bytes = 0;
+    h->dd = nullptr;
+    h->d = Dom{};
 }
 
 // ---------------------------------------------------------------------------
@@ -357,14 +403,34 @@ static int prologue(gral_pf_handle *h, const gral_pf_params *p, std::vector<floa
         for (int j = 1; j <= NJJ; ++j) {
             size_t c = h->i2(i, j);
             float ob = h->obi[c], ust = h->usti[c], rough = h->z0i[c];
+            const float ground = h->ahk[c] - h->hok[h->kk[c]];
             for (int k = 1; k <= NKK - 1; ++k) {
                 if (i == 1) UK[h->i3(NII + 1, j, k)] = UK[h->i3(NII, j, k)];
                 if (j == 1) VK[h->i3(i, NJJ + 1, k)] = VK[h->i3(i, NJJ, k)];
                 if (model == 1) continue;
-                float V0int = p->v0_1;
-                double wh = dmax_net(std::sqrt((double)(p2f(UK[h->i3(i, j, k)]) + p2f(VK[h->i3(i, j, k)]))), 0.01);
-                float U0int = (float)(wh * (0.2 * std::pow(wh, -0.9) + 0.32 * (double)rough + 0.18));
-                U0int = (float)dmax_net((double)U0int, 0.3) * p->std_dev_v;
+                float V0int = p->v0_1, U0int;
+                if (h->istat == 0 || h->istat == 3) {
+                    // observed sigma_u, sigma_v over the measurement heights (IStatistics
+                    // MeteoZR / MeteoSonic, PrognosticFlowfield.cs:456-481); V0int takes the
+                    // slope of U0 between the two heights, as the reference does
+                    const float zh = h->hok[k] - ground;
+                    const float *MH = h->mh.data(), *U0 = h->u0.data(), *V0 = h->v0.data();
+                    const int np = h->nprof;
+                    if (zh <= MH[1]) { U0int = U0[1]; V0int = V0[1]; }
+                    else if (zh >= MH[np]) { U0int = U0[np]; V0int = V0[np]; }
+                    else {
+                        int ipo = 1;
+                        for (int q = 1; q <= np; ++q)
+                            if (zh > MH[q]) ipo = q + 1;
+                        const float slope = (U0[ipo] - U0[ipo - 1]) / (MH[ipo] - MH[ipo - 1]);
+                        U0int = U0[ipo - 1] + slope * (zh - MH[ipo - 1]);
+                        V0int = V0[ipo - 1] + slope * (zh - MH[ipo - 1]);
+                    }
+                } else {
+                    double wh = dmax_net(std::sqrt((double)(p2f(UK[h->i3(i, j, k)]) + p2f(VK[h->i3(i, j, k)]))), 0.01);
+                    U0int = (float)(wh * (0.2 * std::pow(wh, -0.9) + 0.32 * (double)rough + 0.18));
+                    U0int = (float)dmax_net((double)U0int, 0.3) * p->std_dev_v;
+                }
                 float varw;
                 if (ob < 0) varw = p2f(ust) * p2f(1.15f + 0.1f * powf(p->bdlayheight / (-ob), 0.67f)) * p2f(p->std_dev_w);
                 else varw = p2f(ust * p->std_dev_w * 1.25f);
@@ -394,6 +460,15 @@ static int prologue(gral_pf_handle *h, const gral_pf_params *p, std::vector<floa
     });
     h->topwind = topwind;
     return 0;
+}
+
+// blocks of the first stage of the mass-imbalance reduction: about 16 columns
+// per block, at most MASS_MAX_BLOCKS; fixed by the grid, so repeated calls
+// reduce in the same order
+static const int MASS_MAX_BLOCKS = 1024;
+static int mass_blocks(int nii, int njj) {
+    const long ncols = (long)(nii - 2) * (long)(njj - 2);
+    return (int)std::max(1L, std::min<long>(MASS_MAX_BLOCKS, (ncols + 15) / 16));
 }
 
 // pinned host staging of one device field [n2][nkp], kept by the handle
@@ -446,6 +521,12 @@ static int put_vec(gral_pf_handle *h, T *dptr, const std::vector<T> &v) {
 
 extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     if (!h) return GRAL_PF_EINVAL;
+    // a failed prepare leaves the handle unprepared: the device state of an
+    // earlier prepare is released below, so nothing may run on it afterwards
+    h->prepared = false;
+    h->done = false;
+    h->host_stale = false;
+    h->gff_ok = false;
     if (!p) return fail(h, GRAL_PF_EINVAL, "prepare: null params");
     if (!h->masks || !h->rough || !h->fields) return fail(h, GRAL_PF_ESTATE, "prepare: masks, roughness and fields must be set first");
     if (p->turbulence_model < 0 || p->turbulence_model > 2) return fail(h, GRAL_PF_EINVAL, "prepare: turbulence_model must be 0, 1 or 2");
@@ -502,6 +583,9 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     if ((rc = dalloc(h, &fres, (size_t)p->iter_max + 1))) return rc;
     d.resid = fres;
     d.resid_cap = p->iter_max + 1;
+    h->mass_nblk = mass_blocks(h->nii, h->njj);
+    if ((rc = dalloc(h, &h->mass_buf, 2 * (size_t)h->mass_nblk + 2))) return rc;
+    h->mass[0] = h->mass[1] = 0;
     // per-level pressure coefficients (PrognosticFlowfield.cs:855-866), column independent
     std::vector<float> app(h->nkp, 0.0f), abp(h->nkp, 0.0f), atp(h->nkp, 0.0f);
     const float *DZK = h->dzk.data();
@@ -666,6 +750,29 @@ static void fill_stats(gral_pf_handle *h, gral_pf_stats *st) {
     st->topwind = h->topwind; st->dtime = h->d.dtime; st->ug = h->d.ug; st->vg = h->d.vg;
     st->frund = h->d.frund; st->geschw_max = h->geschw_max;
     st->last_delta = h->last_delta;
+    st->mass_div = h->mass[0];
+    st->mass_flux = h->mass[1];
+}
+
+// FP64 sums of |DIV| and of the absolute face fluxes over the active cells of
+// the device-resident fields; partial is scratch of 2*mass_nblk doubles, out
+// of 2 (device memory)
+static int mass_run(gral_pf_handle *h, double *partial, double *out) {
+    if (pf_launch_mass(h->dd, 1, h->mass_nblk, partial, out, h->stream))
+        return fail(h, GRAL_PF_EDEVICE, "mass imbalance: kernel launch failed");
+    HIPCHK(h, hipMemcpyAsync(h->mass, out, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int gral_pf_mass_imbalance(gral_pf_handle *h, double *sum_abs_div, double *sum_abs_flux) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "mass_imbalance: call prepare first");
+    int rc = mass_run(h, h->mass_buf, h->mass_buf + 2 * (size_t)h->mass_nblk);
+    if (rc) return rc;
+    if (sum_abs_div) *sum_abs_div = h->mass[0];
+    if (sum_abs_flux) *sum_abs_flux = h->mass[1];
+    return GRAL_PF_OK;
 }
 
 // Convergence bookkeeping of one finished iteration (PrognosticFlowfield.cs:1144-1178).
@@ -824,7 +931,9 @@ static int sync_host(gral_pf_handle *h) {
 extern "C" int gral_pf_finish(gral_pf_handle *h, gral_pf_stats *st) {
     if (!h) return GRAL_PF_EINVAL;
     if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "finish: call prepare first");
-    int rc = sync_host(h);
+    int rc = mass_run(h, h->mass_buf, h->mass_buf + 2 * (size_t)h->mass_nblk);
+    if (rc) return rc;
+    rc = sync_host(h);
     if (rc) return rc;
     // unrealistic wind speeds (PrognosticFlowfield.cs:1192-1243)
     float limit = (float)(10 * h->geschw_max);

@@ -2692,6 +2692,30 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         case 2: IS = 1; JS = -1; break;
         default: break;
     }
+    // every strip kernel this iteration launches gets its dynamic-LDS limit
+    // before the first launch, so a refusal leaves the fields untouched
+    if (d0.tiled == 2) {
+        const size_t shm_p = sizeof(float) * ((size_t)(7 * ROWS + RING) * lds_slot_h(nkp) + 2 * nkp + 8 + WAVE * CH) +
+                             sizeof(int) * (9 * ROWS + 4);
+        const size_t shm_m = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) +
+                             sizeof(int) * (10 * ROWS + 4);
+        int bad = lds_limit((const void *)k_pressure_strip, shm_p);
+        if (d0.model == 1) {
+            bad |= lds_limit((const void *)k_ml_strip<0>, shm_m) | lds_limit((const void *)k_ml_strip<1>, shm_m) |
+                   lds_limit((const void *)k_ml_strip<2>, shm_m);
+        } else if (d0.model == 2) {
+            bad |= lds_limit((const void *)k_ml_strip<0, 1>, shm_m) | lds_limit((const void *)k_ml_strip<1, 1>, shm_m) |
+                   lds_limit((const void *)k_ml_strip<2, 1>, shm_m);
+            bad |= lds_limit((const void *)k_tke_coef,
+                             sizeof(float) * 5 * (nkp | 1) * COEF_COLS + sizeof(int) * (TKE_SCOL * COEF_COLS + 4));
+            bad |= lds_limit((const void *)k_tke_strip, sizeof(float) * ((size_t)TKE_SLOTS * ROWS * lds_slot_h(nkp) + WAVE * CH) +
+                                                            sizeof(int) * (10 * ROWS + 4));
+        } else {
+            bad |= lds_limit((const void *)k_ml_strip<0, 2>, shm_m) | lds_limit((const void *)k_ml_strip<1, 2>, shm_m) |
+                   lds_limit((const void *)k_ml_strip<2, 2>, shm_m);
+        }
+        if (bad) return (int)hipErrorInvalidValue;
+    }
     int ncols = (nii - 2) * (njj - 2);
     const size_t bc_shm = (size_t)(max(nii, njj) + 2) * (sizeof(int) + 1);
     const unsigned bc_threads = (unsigned)min(64 * kb, 1024);
@@ -2983,5 +3007,86 @@ extern "C" int pf_launch_gff(const GffArgs *ga, hipStream_t s) {
         hipLaunchKernelGGL(k_gff_rows, dim3((unsigned)(g.nii + 1), (unsigned)((g.njj + GFF_BJ) / GFF_BJ)), dim3(256), 0, s, g);
     else
         hipLaunchKernelGGL(k_gff_terrain, dim3((unsigned)((g.nii + 1) * (g.njj + 1))), dim3(256), 0, s, g);
+    return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// FP64 mass imbalance of the current velocity field: over the active cells
+// (ADVDOM = 1, 2 <= i <= NII-1, 2 <= j <= NJJ-1, KKART < k <= VerticalIndex)
+// the sum of |DIV|, with DIV evaluated in FP32 exactly as the divergence pass
+// forms it (PrognosticFlowfield.cs:723-758, the bottom flux dropped at
+// KKART+1), and the sum of the absolute fluxes through the cell faces the
+// same expression uses.  Each FP32 term is widened to double before it is
+// accumulated.  Stage 1: a fixed range of columns per block, one column per
+// wave with the lanes over k (coalesced), wavefront butterfly reductions in
+// double, one partial pair per block.  Stage 2: one block per situation sums
+// the partials.  The partition and the reduction trees are fixed, so the
+// result is the same bits on every run.
+// ------------------------------------------------------------------------
+#define MASS_THREADS 256
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+__device__ __forceinline__ void block_sum2_f64(double &a, double &b) {
+    __shared__ double red[MASS_THREADS / WAVE][2];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    a = wave_sum_f64(a);
+    b = wave_sum_f64(b);
+    if (lane == 0) { red[w][0] = a; red[w][1] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = 0; b = 0;
+        for (int q = 0; q < MASS_THREADS / WAVE; ++q) { a += red[q][0]; b += red[q][1]; }
+    }
+}
+__global__ __launch_bounds__(MASS_THREADS) void k_mass_partial(const Dom *__restrict__ ds, double *__restrict__ partial) {
+    const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];
+    const int nj = d.njj - 2, ncols = (d.nii - 2) * nj;
+    const int per = (ncols + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int c0 = (int)blockIdx.x * per, c1 = min(ncols, c0 + per);
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    double sd = 0, sf = 0;
+    for (int col = c0 + w; col < c1; col += MASS_THREADS / WAVE) {
+        const int i = 2 + col / nj, j = 2 + col % nj;
+        const size_t c = c2(d, i, j);
+        if (d.adv[c] != 1) continue;
+        const int KK = d.kk[c], VI = d.vi[c];
+        const size_t o = c3(d, i, j), oip = c3(d, i + 1, j), ojp = c3(d, i, j + 1);
+        for (int k = KK + 1 + lane; k <= VI; k += WAVE) {
+            const float dz = d.dzk[k];
+            const float u0 = d.uk[o + k], u1 = d.uk[oip + k], v0 = d.vk[o + k], v1 = d.vk[ojp + k];
+            const float w0 = d.wk[o + k], w1 = d.wk[o + k + 1];
+            const float t1 = (u0 - u1) * d.dyk * dz;
+            const float t2 = (v0 - v1) * d.dxk * dz;
+            const float dv = (k > KK + 1) ? t1 + t2 + (w0 - w1) * d.area : t1 + t2 - w1 * d.area;
+            sd += (double)fabsf(dv);
+            double f = (double)fabsf(u0 * d.dyk * dz) + (double)fabsf(u1 * d.dyk * dz) +
+                       (double)fabsf(v0 * d.dxk * dz) + (double)fabsf(v1 * d.dxk * dz) + (double)fabsf(w1 * d.area);
+            if (k > KK + 1) f += (double)fabsf(w0 * d.area);
+            sf += f;
+        }
+    }
+    block_sum2_f64(sd, sf);
+    if (threadIdx.x == 0) {
+        double *p = partial + 2 * ((size_t)blockIdx.z * gridDim.x + blockIdx.x);
+        p[0] = sd;
+        p[1] = sf;
+    }
+}
+__global__ __launch_bounds__(MASS_THREADS) void k_mass_final(const double *__restrict__ partial, int nblk,
+                                                             double *__restrict__ out) {
+    const double *p = partial + 2 * (size_t)blockIdx.x * nblk;
+    double sd = 0, sf = 0;
+    for (int b = threadIdx.x; b < nblk; b += MASS_THREADS) { sd += p[2 * b]; sf += p[2 * b + 1]; }
+    block_sum2_f64(sd, sf);
+    if (threadIdx.x == 0) { out[2 * blockIdx.x] = sd; out[2 * blockIdx.x + 1] = sf; }
+}
+// partial: [nsit][nblk][2] doubles of scratch; out: [nsit][2] (sum |DIV|, sum |face flux|)
+extern "C" int pf_launch_mass(const Dom *dd, int nsit, int nblk, double *partial, double *out, hipStream_t s) {
+    if (nsit < 1 || nblk < 1) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mass_partial, dim3((unsigned)nblk, 1, (unsigned)nsit), dim3(MASS_THREADS), 0, s, dd, partial);
+    hipLaunchKernelGGL(k_mass_final, dim3((unsigned)nsit), dim3(MASS_THREADS), 0, s, partial, nblk, out);
     return (int)hipGetLastError();
 }

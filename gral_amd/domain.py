@@ -143,6 +143,13 @@ class Case:
     std_dev_w: float = 1.0
     bdlayheight: float = 800.0
     v0_1: float = 0.0
+    # Program.IStatistics and the measured profile of sigma_u/sigma_v
+    # (MeasurementHeight, U0, V0 as [n_profile+1] arrays, index 0 unused) that
+    # the TKE initialisation interpolates for IStatistics 0 (zr) and 3 (sonic)
+    istatistics: int = 4
+    meas_height: Optional[np.ndarray] = None
+    u0: Optional[np.ndarray] = None
+    v0: Optional[np.ndarray] = None
 
     def active_cells(self) -> int:
         """Cells advanced by one SIMPLE iteration: ADVDOM(i,j)=1, 2<=i<=NII-1,

@@ -22,7 +22,7 @@ U8P = C.POINTER(C.c_uint8)
 I16P = C.POINTER(C.c_int16)
 
 EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral_pf_upload_fields",
-           "gral_pf_prepare", "gral_pf_iterate", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
+           "gral_pf_set_met_profile", "gral_pf_prepare", "gral_pf_iterate", "gral_pf_mass_imbalance", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
            "gral_pf_diagnostic",
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
@@ -50,7 +50,7 @@ class Stats(C.Structure):
                 ("delta_finish", C.c_float), ("convergence_problem", C.c_int32), ("unreal_u", C.c_int32),
                 ("unreal_v", C.c_int32), ("unreal_w", C.c_int32), ("topwind", C.c_float), ("dtime", C.c_float),
                 ("ug", C.c_float), ("vg", C.c_float), ("frund", C.c_double), ("geschw_max", C.c_double),
-                ("last_delta", C.c_double)]
+                ("last_delta", C.c_double), ("mass_div", C.c_double), ("mass_flux", C.c_double)]
 
 
 class Timing(C.Structure):
@@ -100,6 +100,8 @@ def load(path: str = LIB_PATH):
     lib.gral_pf_set_masks.argtypes = [H, U8P, I16P, FP, FP, FP, FP, U8P]
     lib.gral_pf_set_roughness.argtypes = [H, FP, FP, FP, FP, FP]
     lib.gral_pf_upload_fields.argtypes = [H, FP, FP, FP]
+    lib.gral_pf_set_met_profile.argtypes = [H, C.c_int32, C.c_int32, FP, FP, FP]
+    lib.gral_pf_mass_imbalance.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.gral_pf_prepare.argtypes = [H, C.POINTER(Params)]
     lib.gral_pf_iterate.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(Stats)]
     lib.gral_pf_finish.argtypes = [H, C.POINTER(Stats)]
@@ -181,6 +183,12 @@ class PrognosticFlowfield:
         k.update(z0w=_f32(c.z0_wall), z0s=_f32(c.z0_solver), obi=_f32(c.ob_init), usti=_f32(c.ust_init), z0i=_f32(c.z0_init))
         self._check(self.lib.gral_pf_set_roughness(h, _ptr(k["z0w"]), _ptr(k["z0s"]), _ptr(k["obi"]), _ptr(k["usti"]),
                                                    _ptr(k["z0i"])))
+        if c.istatistics in (0, 3):
+            k.update(mh=_f32(c.meas_height), u0=_f32(c.u0), v0=_f32(c.v0))
+            self._check(self.lib.gral_pf_set_met_profile(h, int(c.istatistics), len(k["mh"]) - 1, _ptr(k["mh"]),
+                                                         _ptr(k["u0"]), _ptr(k["v0"])))
+        elif c.istatistics != 4:
+            self._check(self.lib.gral_pf_set_met_profile(h, int(c.istatistics), 0, None, None, None))
         uk, vk, wk = _f32(c.uk), _f32(c.vk), _f32(c.wk)
         self._check(self.lib.gral_pf_upload_fields(h, _ptr(uk), _ptr(vk), _ptr(wk)))
 
@@ -204,6 +212,13 @@ class PrognosticFlowfield:
         st = Stats()
         self._check(self.lib.gral_pf_iterate(self.h, int(n), int(bool(check_convergence)), C.byref(st)))
         return _stats(st)
+
+    def mass_imbalance(self):
+        """(sum |DIV|, sum |face flux|) over the active cells of the current
+        device fields, reduced in FP64 on the GPU."""
+        a, b = C.c_double(0), C.c_double(0)
+        self._check(self.lib.gral_pf_mass_imbalance(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def finish(self):
         st = Stats()
@@ -497,4 +512,25 @@ def run(case, iterations=None):
         stats = pf.solve(iterations)
         out = pf.download()
     out["stats"] = stats
+    return out
+
+
+def run_batch(cases, iterations):
+    """Convenience: advance ``cases`` as one lock-step batch for ``iterations``
+    SIMPLE iterations (fixed count); returns one dict of fields + 'stats' per case."""
+    solvers = [PrognosticFlowfield(c) for c in cases]
+    try:
+        for s in solvers:
+            s.prepare(iterations)
+        with Batch(solvers) as b:
+            b.iterate(iterations)
+        out = []
+        for s in solvers:
+            st = s.finish()
+            f = s.download()
+            f["stats"] = st
+            out.append(f)
+    finally:
+        for s in solvers:
+            s.close()
     return out

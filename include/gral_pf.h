@@ -79,6 +79,9 @@ typedef struct {
     float topwind, dtime, ug, vg;
     double frund, geschw_max;
     double last_delta;          /* last value of DeltaMean/topwind^2 tested (every 100 iterations) */
+    double mass_div;            /* FP64 sum of |DIV| over the active cells at finish (gral_pf_mass_imbalance) */
+    double mass_flux;           /* FP64 sum of |face flux| of the same cells; mass_div/mass_flux is the
+                                   relative mass imbalance */
 } gral_pf_stats;
 
 /* Replaces the array allocations of PrognosticFlowfield.cs:45-106 (device side). */
@@ -98,6 +101,17 @@ int gral_pf_set_masks(gral_pf_handle *h, const uint8_t *advdom, const int16_t *k
 int gral_pf_set_roughness(gral_pf_handle *h, const float *z0_wall, const float *z0_solver,
                           const float *ob_init, const float *ust_init, const float *z0_init);
 
+/* Meteorological input of the TKE initialisation (turbulence models 0 and 2):
+ * Program.IStatistics and, for IStatistics 0 (MeteoZR) or 3 (MeteoSonic),
+ * MetProfileNumb and the measured standard deviations U0[]/V0[] at
+ * MeasurementHeight[] (arrays [n_profile+1], index 0 unused, heights
+ * increasing), which PrognosticFlowfield.cs:456-481 interpolates linearly in
+ * height above ground (V0 with the slope of U0, as the reference does).
+ * Other IStatistics values (4 by default, when this is never called) use the
+ * wind-speed estimate (:482-487) with gral_pf_params.v0_1 as V0[1]. */
+int gral_pf_set_met_profile(gral_pf_handle *h, int32_t istatistics, int32_t n_profile, const float *meas_height,
+                            const float *u0, const float *v0);
+
 /* Initial wind field Program.UK/VK/WK (the state MicroscaleFlat/MicroscaleTerrain
  * leave before the prognostic call). */
 int gral_pf_upload_fields(gral_pf_handle *h, const float *uk, const float *vk, const float *wk);
@@ -110,6 +124,16 @@ int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p);
  * device-resident state.  check_convergence=1 applies the reference's stop
  * test every 100 iterations; 0 runs exactly n iterations (or until iter_max). */
 int gral_pf_iterate(gral_pf_handle *h, int32_t n, int32_t check_convergence, gral_pf_stats *st);
+
+/* Mass imbalance of the device-resident velocity field, reduced in FP64 on
+ * the GPU: *sum_abs_div = sum over the active cells (ADVDOM = 1, 2 <= i <= NII-1,
+ * 2 <= j <= NJJ-1, KKART < k <= VerticalIndex) of |DIV|, DIV formed in FP32 as
+ * the divergence pass does (PrognosticFlowfield.cs:723-758); *sum_abs_flux =
+ * sum over the same cells of the absolute fluxes through the faces that DIV
+ * combines.  The reference prints no such figure; it is the FP64 residual
+ * reported beside its convergence measure.  Deterministic: the same fields
+ * give the same bits.  gral_pf_finish stores both in gral_pf_stats. */
+int gral_pf_mass_imbalance(gral_pf_handle *h, double *sum_abs_div, double *sum_abs_flux);
 
 /* Epilogue checks (PrognosticFlowfield.cs:1187-1245) and transfer back to host. */
 int gral_pf_finish(gral_pf_handle *h, gral_pf_stats *st);

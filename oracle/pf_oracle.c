@@ -1332,12 +1332,35 @@ static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, o
         for (int j = 1; j <= NJJ; ++j) {
             float ObLength = in->ob_init[IX2(s, i, j)], ust = in->ust_init[IX2(s, i, j)], rough = in->z0_init[IX2(s, i, j)];
             for (int k = 1; k <= NKK - 1; ++k) {
+                int ipo = 1;
                 float V0int = p->v0_1;
                 if (i == 1) s->UK[IX3(s, NII + 1, j, k)] = s->UK[IX3(s, NII, j, k)];
                 if (j == 1) s->VK[IX3(s, i, NJJ + 1, k)] = s->VK[IX3(s, i, NJJ, k)];
-                double windhilf = mmaxd_(sqrt((double)(p2f(s->UK[IX3(s, i, j, k)]) + p2f(s->VK[IX3(s, i, j, k)]))), 0.01);
-                float U0int = (float)(windhilf * (0.2 * pow(windhilf, -0.9) + 0.32 * (double)rough + 0.18));
-                U0int = (float)mmaxd_((double)U0int, 0.3) * p->std_dev_v;
+                /* PrognosticFlowfield.cs:452 */
+                float zhilf = s->HOKART[k] - (in->ahk[IX2(s, i, j)] - s->HOKART[KK(s, i, j)]);
+                float U0int = 0;
+                if (in->istatistics == 0 || in->istatistics == 3) {
+                    /* PrognosticFlowfield.cs:456-481: measured sigma profile (MeteoZR / MeteoSonic) */
+                    const float *MH = in->meas_height, *U0 = in->u0, *V0 = in->v0;
+                    int NP = in->n_profile;
+                    if (zhilf <= MH[1]) {
+                        U0int = U0[1];
+                        V0int = V0[1];
+                    } else if (zhilf >= MH[NP]) {
+                        U0int = U0[NP];
+                        V0int = V0[NP];
+                    } else {
+                        for (int iprof = 1; iprof <= NP; iprof++)
+                            if (zhilf > MH[iprof]) ipo = iprof + 1;
+                        U0int = U0[ipo - 1] + (U0[ipo] - U0[ipo - 1]) / (MH[ipo] - MH[ipo - 1]) * (zhilf - MH[ipo - 1]);
+                        /* the reference interpolates V0 with the slope of U0 (:478-479) */
+                        V0int = V0[ipo - 1] + (U0[ipo] - U0[ipo - 1]) / (MH[ipo] - MH[ipo - 1]) * (zhilf - MH[ipo - 1]);
+                    }
+                } else {
+                    double windhilf = mmaxd_(sqrt((double)(p2f(s->UK[IX3(s, i, j, k)]) + p2f(s->VK[IX3(s, i, j, k)]))), 0.01);
+                    U0int = (float)(windhilf * (0.2 * pow(windhilf, -0.9) + 0.32 * (double)rough + 0.18));
+                    U0int = (float)mmaxd_((double)U0int, 0.3) * p->std_dev_v;
+                }
                 float varw;
                 if (ObLength < 0)
                     varw = p2f(ust) * p2f(1.15f + 0.1f * powf(p->bdlayheight / (-ObLength), 0.67f)) * p2f(p->std_dev_w);

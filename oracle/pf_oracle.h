@@ -48,6 +48,10 @@ typedef struct {
     const float *ob_init, *ust_init, *z0_init;  /* TURB init (PrognosticFlowfield.cs:415-436) */
     const float *veg;           /* [nii+2][njj+2][nkk+2] or NULL */
     const uint8_t *veg_present; /* [nii+2][njj+2]: VEG[i][j] != null, or NULL */
+    int istatistics;            /* Program.IStatistics: 0 MeteoZR / 3 MeteoSonic select the measured
+                                   profile in the TKE initialisation (PrognosticFlowfield.cs:456-481) */
+    int n_profile;              /* Program.MetProfileNumb */
+    const float *meas_height, *u0, *v0;  /* [n_profile+1] MeasurementHeight, U0, V0 (index 0 unused) */
 } orc_inputs;
 
 typedef struct {

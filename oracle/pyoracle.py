@@ -16,49 +16,57 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_pf.so")
 
 
-class OrcParams(C.Structure):
-    _fields_ = [("nii", C.c_int), ("njj", C.c_int), ("nkk", C.c_int), ("kadvmax", C.c_int),
-                ("vertcellsff", C.c_int), ("dxk", C.c_float), ("dyk", C.c_float),
-                ("turbulence_model", C.c_int), ("simd_width", C.c_int), ("iter_min", C.c_int),
-                ("iter_max", C.c_int), ("relax", C.c_float), ("relaxp", C.c_float),
-                ("building_z0", C.c_float), ("coriolis", C.c_float), ("potdtdz", C.c_float),
-                ("std_dev_v", C.c_float), ("std_dev_w", C.c_float), ("bdlayheight", C.c_float),
-                ("v0_1", C.c_float), ("refp_i", C.c_int), ("refp_j", C.c_int)]
+def _structs(ft):
+    """The oracle's structures with ``ft`` as the floating-point type: c_float
+    for liboracle_pf.so, c_double for the FP64 shadow liboracle_pf64.so."""
+    FPt = C.POINTER(ft)
+
+    class OrcParams(C.Structure):
+        _fields_ = [("nii", C.c_int), ("njj", C.c_int), ("nkk", C.c_int), ("kadvmax", C.c_int),
+                    ("vertcellsff", C.c_int), ("dxk", ft), ("dyk", ft),
+                    ("turbulence_model", C.c_int), ("simd_width", C.c_int), ("iter_min", C.c_int),
+                    ("iter_max", C.c_int), ("relax", ft), ("relaxp", ft),
+                    ("building_z0", ft), ("coriolis", ft), ("potdtdz", ft),
+                    ("std_dev_v", ft), ("std_dev_w", ft), ("bdlayheight", ft),
+                    ("v0_1", ft), ("refp_i", C.c_int), ("refp_j", C.c_int)]
+
+    class OrcInputs(C.Structure):
+        _fields_ = [("dzk", FPt), ("hokart", FPt), ("advdom", U8P), ("kkart", I16P), ("cutk", FPt),
+                    ("cov", FPt), ("ahk", FPt), ("z0_wall", FPt), ("z0_solver", FPt), ("ob_init", FPt),
+                    ("ust_init", FPt), ("z0_init", FPt), ("veg", FPt), ("veg_present", U8P),
+                    ("istatistics", C.c_int), ("n_profile", C.c_int), ("meas_height", FPt), ("u0", FPt),
+                    ("v0", FPt)]
+
+    class OrcFields(C.Structure):
+        _fields_ = [("uk", FPt), ("vk", FPt), ("wk", FPt), ("uks", FPt), ("vks", FPt), ("wks", FPt),
+                    ("dpm", FPt), ("dpmnew", FPt), ("div", FPt), ("turb", FPt), ("tdiss", FPt)]
+
+    class OrcStats(C.Structure):
+        _fields_ = [("iteration_loops", C.c_int), ("iterations_run", C.c_int),
+                    ("delta_first", ft), ("delta_finish", ft), ("topwind", ft),
+                    ("dtime", ft), ("ug", ft), ("vg", ft),
+                    ("ustern_factor_x", ft), ("frund", C.c_double), ("geschw_max", C.c_double),
+                    ("unreal_u", C.c_int), ("unreal_v", C.c_int), ("unreal_w", C.c_int),
+                    ("convergence_problem", C.c_int), ("delta_umax_hist", FPt)]
+
+    class OrcCapture(C.Structure):
+        _fields_ = [("comp", C.c_int), ("i", C.c_int), ("j", C.c_int), ("iteration", C.c_int),
+                    ("a", FPt), ("b", FPt), ("c", FPt), ("d", FPt), ("x_old", FPt), ("x_new", FPt),
+                    ("captured", C.c_int)]
+
+    class OrcDiagParams(C.Structure):
+        _fields_ = [("nii", C.c_int), ("njj", C.c_int), ("nkk", C.c_int), ("kadvmax", C.c_int), ("dxk", ft),
+                    ("dyk", ft), ("flow_field_level", C.c_int), ("iterations", C.c_int), ("terrain", C.c_int)]
+
+    return OrcParams, OrcInputs, OrcFields, OrcStats, OrcCapture, OrcDiagParams
 
 
 FP = C.POINTER(C.c_float)
 U8P = C.POINTER(C.c_uint8)
 I16P = C.POINTER(C.c_int16)
-
-
-class OrcInputs(C.Structure):
-    _fields_ = [("dzk", FP), ("hokart", FP), ("advdom", U8P), ("kkart", I16P), ("cutk", FP),
-                ("cov", FP), ("ahk", FP), ("z0_wall", FP), ("z0_solver", FP), ("ob_init", FP),
-                ("ust_init", FP), ("z0_init", FP), ("veg", FP), ("veg_present", U8P)]
-
-
-class OrcFields(C.Structure):
-    _fields_ = [("uk", FP), ("vk", FP), ("wk", FP), ("uks", FP), ("vks", FP), ("wks", FP),
-                ("dpm", FP), ("dpmnew", FP), ("div", FP), ("turb", FP), ("tdiss", FP)]
-
-
-class OrcStats(C.Structure):
-    _fields_ = [("iteration_loops", C.c_int), ("iterations_run", C.c_int),
-                ("delta_first", C.c_float), ("delta_finish", C.c_float), ("topwind", C.c_float),
-                ("dtime", C.c_float), ("ug", C.c_float), ("vg", C.c_float),
-                ("ustern_factor_x", C.c_float), ("frund", C.c_double), ("geschw_max", C.c_double),
-                ("unreal_u", C.c_int), ("unreal_v", C.c_int), ("unreal_w", C.c_int),
-                ("convergence_problem", C.c_int), ("delta_umax_hist", FP)]
-
-
-class OrcCapture(C.Structure):
-    _fields_ = [("comp", C.c_int), ("i", C.c_int), ("j", C.c_int), ("iteration", C.c_int),
-                ("a", FP), ("b", FP), ("c", FP), ("d", FP), ("x_old", FP), ("x_new", FP), ("captured", C.c_int)]
-
-
-class OrcDiagParams(C.Structure):
-    _fields_ = [("nii", C.c_int), ("njj", C.c_int), ("nkk", C.c_int), ("kadvmax", C.c_int), ("dxk", C.c_float),
-                ("dyk", C.c_float), ("flow_field_level", C.c_int), ("iterations", C.c_int), ("terrain", C.c_int)]
+OrcParams, OrcInputs, OrcFields, OrcStats, OrcCapture, OrcDiagParams = _structs(C.c_float)
+_structs_f32 = (OrcParams, OrcInputs, OrcFields, OrcStats)
+_S64 = None
 
 
 _lib = None
@@ -90,6 +98,27 @@ def load(build_if_missing: bool = True):
     return _lib
 
 
+def load64(build_if_missing: bool = True):
+    """The FP64 shadow (liboracle_pf64.so): the same restatement with every
+    float widened to double; for error statements only, never a checker."""
+    global _lib64, _S64
+    if _lib64 is not None:
+        return _lib64
+    path = os.path.join(HERE, "liboracle_pf64.so")
+    if not os.path.exists(path) and build_if_missing:
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    _S64 = _structs(C.c_double)
+    P, I, F, S = _S64[:4]
+    lib = C.CDLL(path)
+    lib.orc_calculate.argtypes = [C.POINTER(P), C.POINTER(I), C.POINTER(F), C.POINTER(S)]
+    lib.orc_calculate.restype = C.c_int
+    _lib64 = lib
+    return lib
+
+
+_lib64 = None
+
+
 def _p(a, t=FP):
     if a is None:
         return C.cast(None, t)
@@ -97,13 +126,22 @@ def _p(a, t=FP):
     return a.ctypes.data_as(t)
 
 
-def run(case, iterations=None, simd_width=None, keep_history=True, capture=None, threads=1):
+def run(case, iterations=None, simd_width=None, keep_history=True, capture=None, threads=1, fp64=False):
     """Run PrognosticFlowfield.Calculate on ``case`` (a gral_amd.domain.Case)
     with the deterministic CPU oracle.  Returns a dict of output fields and stats.
     The input case is not modified.  threads > 1 selects the timing mode: the
     reference's default multi-threaded schedule, not deterministic (CPU
-    baseline only, never a checker)."""
-    lib = load()
+    baseline only, never a checker).  fp64 runs the FP64 shadow instead: the
+    same algorithm in double precision (fields come back as float64)."""
+    if fp64:
+        lib = load64()
+        OrcParams, OrcInputs, OrcFields, OrcStats = _S64[:4]
+        dt, FPt = np.float64, C.POINTER(C.c_double)
+        assert capture is None and threads == 1
+    else:
+        lib = load()
+        OrcParams, OrcInputs, OrcFields, OrcStats = _structs_f32
+        dt, FPt = np.float32, FP
     c = case
     nit = c.iter_max if iterations is None else iterations
     prm = OrcParams(c.nii, c.njj, c.nkk, c.kadvmax, c.vertcellsff, c.dxk, c.dyk, c.turbulence_model,
@@ -111,26 +149,32 @@ def run(case, iterations=None, simd_width=None, keep_history=True, capture=None,
                     c.iter_min if iterations is None else iterations, nit, c.relax, c.relaxp,
                     c.building_z0, c.coriolis, c.potdtdz, c.std_dev_v, c.std_dev_w, c.bdlayheight,
                     c.v0_1, int(c.refp[0]), int(c.refp[1]))
-    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=dt)
+    pf_ = lambda a: _p(a, FPt)
     keep = dict(dzk=f32(c.dzk), hokart=f32(c.hokart), advdom=np.ascontiguousarray(c.advdom, dtype=np.uint8),
                 kkart=np.ascontiguousarray(c.kkart, dtype=np.int16), cutk=f32(c.cutk), cov=f32(c.cov),
                 ahk=f32(c.ahk), z0_wall=f32(c.z0_wall), z0_solver=f32(c.z0_solver), ob_init=f32(c.ob_init),
                 ust_init=f32(c.ust_init), z0_init=f32(c.z0_init),
                 veg=None if c.veg is None else f32(c.veg),
                 veg_present=None if c.veg_present is None else np.ascontiguousarray(c.veg_present, dtype=np.uint8))
-    inp = OrcInputs(_p(keep["dzk"]), _p(keep["hokart"]), _p(keep["advdom"], U8P), _p(keep["kkart"], I16P),
-                    _p(keep["cutk"]), _p(keep["cov"]), _p(keep["ahk"]), _p(keep["z0_wall"]),
-                    _p(keep["z0_solver"]), _p(keep["ob_init"]), _p(keep["ust_init"]), _p(keep["z0_init"]),
-                    _p(keep["veg"]), _p(keep["veg_present"], U8P))
+    prof = c.istatistics in (0, 3)
+    for n in ("meas_height", "u0", "v0"):
+        keep[n] = f32(getattr(c, n)) if prof else None
+    inp = OrcInputs(pf_(keep["dzk"]), pf_(keep["hokart"]), _p(keep["advdom"], U8P), _p(keep["kkart"], I16P),
+                    pf_(keep["cutk"]), pf_(keep["cov"]), pf_(keep["ahk"]), pf_(keep["z0_wall"]),
+                    pf_(keep["z0_solver"]), pf_(keep["ob_init"]), pf_(keep["ust_init"]), pf_(keep["z0_init"]),
+                    pf_(keep["veg"]), _p(keep["veg_present"], U8P), int(c.istatistics),
+                    len(keep["meas_height"]) - 1 if prof else 0, pf_(keep["meas_height"]), pf_(keep["u0"]),
+                    pf_(keep["v0"]))
     shp = c.uk.shape
-    out = {n: np.zeros(shp, dtype=np.float32) for n in ("uks", "vks", "wks", "dpm", "dpmnew", "div", "turb", "tdiss")}
+    out = {n: np.zeros(shp, dtype=dt) for n in ("uks", "vks", "wks", "dpm", "dpmnew", "div", "turb", "tdiss")}
     out["uk"] = f32(c.uk).copy()
     out["vk"] = f32(c.vk).copy()
     out["wk"] = f32(c.wk).copy()
-    fld = OrcFields(*[_p(out[n]) for n in ("uk", "vk", "wk", "uks", "vks", "wks", "dpm", "dpmnew", "div", "turb", "tdiss")])
-    hist = np.zeros(max(nit, 1), dtype=np.float32)
+    fld = OrcFields(*[pf_(out[n]) for n in ("uk", "vk", "wk", "uks", "vks", "wks", "dpm", "dpmnew", "div", "turb", "tdiss")])
+    hist = np.zeros(max(nit, 1), dtype=dt)
     st = OrcStats()
-    st.delta_umax_hist = _p(hist) if keep_history else C.cast(None, FP)
+    st.delta_umax_hist = pf_(hist) if keep_history else C.cast(None, FPt)
     cap = None
     if capture is not None:
         comp, ci, cj, cit = capture
