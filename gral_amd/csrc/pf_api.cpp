@@ -102,6 +102,10 @@ void par_assign(std::vector<float> &v, const float *src, size_t n) {
 
 }  // namespace
 
+// FP64 mass-imbalance reduction: blocks of its first stage at most (one
+// partial pair per block)
+static const int kMassMaxBlocks = 1024;
+
 struct gral_pf_handle {
     // grid
     int nii = 0, njj = 0, nkk = 0, kadvmax = 0, vcf = 0, nkp = 0, nkh = 0;
@@ -305,6 +309,7 @@ static void free_device(gral_pf_handle *h) {
     h->dev_bytes = 0;
     h->dd = nullptr;
     h->d = Dom{};
+    h->mass_buf = nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -465,10 +470,9 @@ static int prologue(gral_pf_handle *h, const gral_pf_params *p, std::vector<floa
 // blocks of the first stage of the mass-imbalance reduction: about 16 columns
 // per block, at most MASS_MAX_BLOCKS; fixed by the grid, so repeated calls
 // reduce in the same order
-static const int MASS_MAX_BLOCKS = 1024;
 static int mass_blocks(int nii, int njj) {
     const long ncols = (long)(nii - 2) * (long)(njj - 2);
-    return (int)std::max(1L, std::min<long>(MASS_MAX_BLOCKS, (ncols + 15) / 16));
+    return (int)std::max(1L, std::min<long>(kMassMaxBlocks, (ncols + 15) / 16));
 }
 
 // pinned host staging of one device field [n2][nkp], kept by the handle
@@ -584,7 +588,6 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
     d.resid = fres;
     d.resid_cap = p->iter_max + 1;
     h->mass_nblk = mass_blocks(h->nii, h->njj);
-    if ((rc = dalloc(h, &h->mass_buf, 2 * (size_t)h->mass_nblk + 2))) return rc;
     h->mass[0] = h->mass[1] = 0;
     // per-level pressure coefficients (PrognosticFlowfield.cs:855-866), column independent
     std::vector<float> app(h->nkp, 0.0f), abp(h->nkp, 0.0f), atp(h->nkp, 0.0f);
@@ -765,10 +768,17 @@ static int mass_run(gral_pf_handle *h, double *partial, double *out) {
     return 0;
 }
 
+// the handle's scratch of the reduction, allocated on first use
+static int mass_handle_run(gral_pf_handle *h) {
+    int rc;
+    if (!h->mass_buf && (rc = dalloc(h, &h->mass_buf, 2 * (size_t)h->mass_nblk + 2))) return rc;
+    return mass_run(h, h->mass_buf, h->mass_buf + 2 * (size_t)h->mass_nblk);
+}
+
 extern "C" int gral_pf_mass_imbalance(gral_pf_handle *h, double *sum_abs_div, double *sum_abs_flux) {
     if (!h) return GRAL_PF_EINVAL;
     if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "mass_imbalance: call prepare first");
-    int rc = mass_run(h, h->mass_buf, h->mass_buf + 2 * (size_t)h->mass_nblk);
+    int rc = mass_handle_run(h);
     if (rc) return rc;
     if (sum_abs_div) *sum_abs_div = h->mass[0];
     if (sum_abs_flux) *sum_abs_flux = h->mass[1];
@@ -931,7 +941,7 @@ static int sync_host(gral_pf_handle *h) {
 extern "C" int gral_pf_finish(gral_pf_handle *h, gral_pf_stats *st) {
     if (!h) return GRAL_PF_EINVAL;
     if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "finish: call prepare first");
-    int rc = mass_run(h, h->mass_buf, h->mass_buf + 2 * (size_t)h->mass_nblk);
+    int rc = mass_handle_run(h);
     if (rc) return rc;
     rc = sync_host(h);
     if (rc) return rc;

@@ -67,6 +67,7 @@ typedef struct {
     float *AP0;
     float *P1, *Q1, *P2, *Q2;         /* column scratch (PIMU/QIMU, PIMTDISS...) */
     orc_capture *cap;                 /* optional checking hook */
+    int rec;                          /* the lane being evaluated enters the recurrence (hook only) */
     int iter, comp;                   /* current SIMPLE iteration and sweep (for the hook) */
     int nthreads;                     /* > 1: the timing mode of orc_calculate_threads */
 } St;
@@ -83,6 +84,12 @@ static void cap_column(St *s, int comp, int i, int j, const float *x, bool after
     if (after) s->cap->captured = 1;
 }
 
+
+/* record piece id of level kk of the captured column */
+static void piece(St *s, int i, int j, int id, int kk, float v) {
+    if (!s->rec || !s->cap || !s->cap->piece || !cap_on(s, s->comp, i, j)) return;
+    s->cap->piece[(size_t)id * (size_t)(s->NKK + 2) + (size_t)kk] = v;
+}
 
 #define IX2(s, i, j) ((size_t)(i) * (size_t)(s)->NJ2 + (size_t)(j))
 #define IX3(s, i, j, k) (IX2(s, i, j) * (size_t)(s)->NKP + (size_t)(k))
@@ -329,7 +336,12 @@ static void ml_column(St *s, int i, int j, lane_fn fn, void *ctx) {
     for (int k = KSTART; k < KEND; k += W) {
         k_real = k;
         if (k > KEND - W) { k = KEND - W; end = true; }
-        for (int l = 0; l < W; ++l) fn(s, i, j, k, k + l, &A[l], &B[l], &C[l], &D[l], ctx);
+        const int used0 = end ? k_real - k : 0;     /* lanes below it were consumed by the previous block */
+        for (int l = 0; l < W; ++l) {
+            s->rec = l >= used0 && k + l < KEND;
+            fn(s, i, j, k, k + l, &A[l], &B[l], &C[l], &D[l], ctx);
+        }
+        s->rec = 0;
         int kint_s = 0;
         if (end) { kint_s = k_real - k; k_real -= kint_s; }
         for (int kint = kint_s; kint < W; ++kint) {
@@ -405,6 +417,15 @@ static inline float ml_nb(bool fused, float aw, float xw, float as, float xs, fl
 
 typedef struct { float UG; } uctx;
 
+/* the checking hook's record of one level's conductances, fluxes and coefficients */
+static void ml_pieces(St *s, int i, int j, int kk, float VIS, float DE, float DW, float DS, float DN, float DT,
+                      float DB, float FE, float FW, float FS, float FN, float FT, float FB, float AE, float AW,
+                      float AS, float AN, float AT, float AB) {
+    if (!s->rec || !s->cap || !s->cap->piece || !cap_on(s, s->comp, i, j)) return;
+    const float v[] = {VIS, DE, DW, DS, DN, DT, DB, FE, FW, FS, FN, FT, FB, AE, AW, AS, AN, AT, AB};
+    for (int q = 0; q <= ORC_AB; ++q) piece(s, i, j, q, kk, v[q]);
+}
+
 /* One lane of U_PrognosticMicroscaleV1 (U-prognostic-microscale_1.cs:194-470) */
 static void u1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, float *pC, float *pD, void *vctx) {
     float UG = ((uctx *)vctx)->UG;
@@ -453,14 +474,18 @@ static void u1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
     float AN1 = ml_coef(W, DN, PN, vmax(-FN, 0.0f));
     float AP0 = s->AP0[kk];
     float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + AP0;
+    ml_pieces(s, i, j, kk, VIS, DE, DW, DS, DN, DT, DB, FE, FW, FS, FN, FT, FB, AE1, AW1, AS1, AN1, BIM, CIM);
     float DDPX = DPMNEW_W[kk] - DPMNEW[kk];
     float UKv = UK[kk];
     float DIM;
+    piece(s, i, j, ORC_PRESS, kk, DDPX * DYKDZK);
+    piece(s, i, j, ORC_COR, kk, s->Cor * (UG - UKv) * AREAxy * DZ);
     if (s->VEGP && s->VEGP[IX2(s, i, j)]) {
         float in2 = 0.5f * (VKS_W[kk] + VKS[kk]);
         float in = 0.5f * (UKS_W[kk] + UKS[kk]);
         float wh = vmax(sqrtf(in * in + in2 * in2), 0.01f);
         float VEGv = s->VEG[IX3(s, i, j, kk - 1)];
+        piece(s, i, j, ORC_VEGT, kk, -VEGv * UKv * wh * AREAxy * DZ);
         DIM = ml_nb(W == 16, AW1, UK_W[kk], AS1, UK_S[kk], AE1, UK_E[kk], AN1, UK_N[kk]) +
               AP0 * 0.5f * (UKS_W[kk] + UKS[kk]) + DDPX * DYKDZK +
               (s->Cor * (UG - UKv) - VEGv * UKv * wh) * AREAxy * DZ;
@@ -493,6 +518,7 @@ static void u1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
             Us = s->HO[IX2(s, i, j)] * whI;
         }
         DIM -= UKv / wh * (Us * Us) * AREAxy * m2;
+        piece(s, i, j, ORC_WALL, kk, UKv / wh * (Us * Us) * AREAxy * m2);
     }
     if (k > KKP1) {
         float DUDXE = (UK_E[kk] - UKv) * DYKDZK;
@@ -501,8 +527,10 @@ static void u1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
         float DVDXS = (0.5f * (VKS_ES[kk] + VKS_S[kk]) - 0.5f * (VKS_S[kk] + VKS_WS[kk])) * DXKDZK;
         float DWDXT = (0.5f * (WKS_E[kk] + WKS[kk]) - 0.5f * (WKS[kk] + WKS_W[kk])) * AREAxy;
         float DWDXB = (0.5f * (WKS_E[kk - 1] + WKS[kk - 1]) - 0.5f * (WKS[kk - 1] + WKS_W[kk - 1])) * AREAxy;
-        if (mask_v_enable) DIM += VIS / DXK * (DUDXE - DUDXW + DVDXN - DVDXS + DWDXT - DWDXB) * mask;
-        else DIM += VIS / DXK * (DUDXE - DUDXW + DVDXN - DVDXS + DWDXT - DWDXB);
+        float cross = VIS / DXK * (DUDXE - DUDXW + DVDXN - DVDXS + DWDXT - DWDXB);
+        if (mask_v_enable) cross = cross * mask;
+        piece(s, i, j, ORC_CROSS, kk, cross);
+        DIM += cross;
     }
     *pA = AIM; *pB = BIM; *pC = CIM; *pD = DIM;
 }
@@ -556,14 +584,18 @@ static void v1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
     float AN1 = ml_coef(W, DN, PN, vmax(-FN, 0.0f));
     float AP0 = s->AP0[kk];
     float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + AP0;
+    ml_pieces(s, i, j, kk, VIS, DE, DW, DS, DN, DT, DB, FE, FW, FS, FN, FT, FB, AE1, AW1, AS1, AN1, BIM, CIM);
     float DDPY = DPMNEW_S[kk] - DPMNEW[kk];
     float VKv = VK[kk];
+    piece(s, i, j, ORC_PRESS, kk, DDPY * DXKDZK);
+    piece(s, i, j, ORC_COR, kk, s->Cor * (VG - VKv) * AREAxy * DZ);
     float DIM;
     if (s->VEGP && s->VEGP[IX2(s, i, j)]) {
         float in2 = 0.5f * (VKS_W[kk] + VKS[kk]);
         float in = 0.5f * (UKS_W[kk] + UKS[kk]);
         float wh = vmax(sqrtf(in * in + in2 * in2), 0.01f);
         float VEGv = s->VEG[IX3(s, i, j, kk - 1)];
+        piece(s, i, j, ORC_VEGT, kk, -VEGv * VKv * wh * AREAxy * DZ);
         /* fused by the 512-bit solver only here (V-prognostic-microscale_1_Vec512.cs:404-411) */
         DIM = ml_nb(W == 16, AW1, VK_W[kk], AS1, VK_S[kk], AE1, VK_E[kk], AN1, VK_N[kk]) +
               AP0 * 0.5f * (VKS_S[kk] + VKS[kk]) + DDPY * DXKDZK +
@@ -592,6 +624,7 @@ static void v1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
             Us = s->HO[IX2(s, i, j)] * whI;
         }
         DIM -= VKv / wh * (Us * Us) * AREAxy * m2;
+        piece(s, i, j, ORC_WALL, kk, VKv / wh * (Us * Us) * AREAxy * m2);
     }
     if (k > KKP1) {
         float DVDYN = (VK_N[kk] - VKv) * DXKDZK;
@@ -601,8 +634,10 @@ static void v1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
         float DWDYT = (0.5f * (WKS_N[kk] + WKS[kk]) - 0.5f * (WKS[kk] + WKS_S[kk])) * AREAxy;
         /* quirk: WKS_S at k (not k-1) in the bottom term, V-prognostic-microscale_1.cs:454 */
         float DWDYB = (0.5f * (WKS_N[kk - 1] + WKS[kk - 1]) - 0.5f * (WKS[kk - 1] + WKS_S[kk])) * AREAxy;
-        if (mask_v_enable) DIM += VIS / DXK * (DUDYE - DUDYW + DVDYN - DVDYS + DWDYT - DWDYB) * mask;
-        else DIM += VIS / DXK * (DUDYE - DUDYW + DVDYN - DVDYS + DWDYT - DWDYB);
+        float cross = VIS / DXK * (DUDYE - DUDYW + DVDYN - DVDYS + DWDYT - DWDYB);
+        if (mask_v_enable) cross = cross * mask;
+        piece(s, i, j, ORC_CROSS, kk, cross);
+        DIM += cross;
     }
     *pA = AIM; *pB = BIM; *pC = CIM; *pD = DIM;
 }
@@ -654,14 +689,17 @@ static void w1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
     float AN1 = ml_coef(W, DN, PN, vmax(-FN, 0.0f));
     float AP0 = s->AP0[kk];
     float AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + AP0;
+    ml_pieces(s, i, j, kk, VIS, DE, DW, DS, DN, DT, DB, FE, FW, FS, FN, FT, FB, AE1, AW1, AS1, AN1, BIM, CIM);
     float DDPZ = DPMNEW[kk - 1] - DPMNEW[kk];
     float WKv = WK[kk];
     float DIM;
+    piece(s, i, j, ORC_PRESS, kk, DDPZ * AREAxy);
     if (s->VEGP && s->VEGP[IX2(s, i, j)]) {
         float in2 = 0.5f * (VKS_S[kk] + VKS[kk]);
         float in = 0.5f * (UKS_W[kk] + UKS[kk]);
         float wh = vmax(sqrtf(in * in + in2 * in2), 0.01f);
         float VEGv = s->VEG[IX3(s, i, j, kk - 1)];
+        piece(s, i, j, ORC_VEGT, kk, -VEGv * WKv * wh * AREAxy * DZ);
         /* W-prognostic-microscale_1_Vec512.cs:367-383 fuses both forms */
         DIM = ml_nb(W == 16, AW1, WK_W[kk], AS1, WK_S[kk], AE1, WK_E[kk], AN1, WK_N[kk]) +
               AP0 * 0.5f * (WKS[kk - 1] + WKS[kk]) + DDPZ * AREAxy - VEGv * WKv * wh * AREAxy * DZ;
@@ -676,8 +714,10 @@ static void w1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
         float DVDZS = (0.5f * (VKS_S[kk + 1] + VKS_S[kk]) - 0.5f * (VKS_S[kk] + VKS_S[kk - 1])) * DXKDZK;
         float DWDZT = (WK[kk + 1] - WKv) * AREAxy;
         float DWDZB = (WKv - WK[kk - 1]) * AREAxy;
-        if (mask_v_enable) DIM += VIS / DZ * (DUDZE - DUDZW + DVDZN - DVDZS + DWDZT - DWDZB) * mask;
-        else DIM += VIS / DZ * (DUDZE - DUDZW + DVDZN - DVDZS + DWDZT - DWDZB);
+        float cross = VIS / DZ * (DUDZE - DUDZW + DVDZN - DVDZS + DWDZT - DWDZB);
+        if (mask_v_enable) cross = cross * mask;
+        piece(s, i, j, ORC_CROSS, kk, cross);
+        DIM += cross;
     }
     *pA = AIM; *pB = BIM; *pC = CIM; *pD = DIM;
 }
@@ -1054,6 +1094,7 @@ static void w2_sweep(St *s, int IS, int JS, int diffusion) {
 /* TKE_PrognosticMicroscale.Calculate (TKE-prognostic-microscale.cs:25-352) */
 static void tke_sweep(St *s, int IS, int JS) {
     float Cmueh = 0.09f, Ceps = 1, Ceps1 = 1.44f, Ceps2 = 1.92f;
+    s->comp = 4;
     float DXK = s->DXK, DYK = s->DYK, AREAxy = s->AREAxy;
     float DXK_Rez = 1 / DXK, DYK_Rez = 1 / DYK;
     float *PT1 = s->P1, *QT1 = s->Q1, *PT2 = s->P2, *QT2 = s->Q2;
@@ -1140,6 +1181,11 @@ static void tke_sweep(St *s, int IS, int JS) {
                     (2.0f * p2f(DUDX) + 2.0f * p2f(DVDY) + 2.0f * p2f(DWDZ) +
                      p2f(DUDY + DVDX) + p2f(DUDZ + DWDX) + p2f(DVDZ + DWDY) - TDISS[k]);
                 float PROBOU = (float)((double)(-s->PotdTdz) * 9.81 / 283.0 * (double)VIS * 1.35 * (double)AREAxy * (double)DZ);
+                s->rec = 1;
+                piece(s, i, j, ORC_VIS, k, VIS);
+                piece(s, i, j, ORC_PROD, k, PROTURB);
+                piece(s, i, j, ORC_BUOY, k, PROBOU);
+                s->rec = 0;
                 float DIMTURB = AW1 * TURBim[k] + AS1 * TURBjm[k] + AE1 * TURBip[k] + AN1 * TURBjp[k] +
                                 AP0 * TURB[k] + PROTURB + PROBOU;
                 float DIMTDISS = AW1 * TDISSim[k] + AS1 * TDISSjm[k] + AE1 * TDISSip[k] + AN1 * TDISSjp[k] +

@@ -114,7 +114,24 @@ typedef struct {
     float *a, *b, *c, *d;       /* [nkk+2] each */
     float *x_old, *x_new;       /* [nkk+2] each */
     int captured;               /* set to 1 when recorded */
+    float *piece;               /* optional [ORC_NPIECE][nkk+2]: the pieces below per level, as
+                                   the mixing-length solvers (comp 0-2) and the TKE sweep (comp 4,
+                                   production and buoyancy) form them for the recurrence */
 } orc_capture;
+
+/* pieces of one level of a momentum equation: eddy viscosity, conductances
+ * after their 1e-4 floor, face fluxes, power-law neighbour coefficients, and
+ * the source terms added to DIM (pressure gradient, Coriolis/geostrophic
+ * term, wall-function drag subtracted at KKART+1, eddy-viscosity cross terms,
+ * vegetation drag); TKE: shear production and buoyancy */
+enum {
+    ORC_VIS = 0, ORC_DE, ORC_DW, ORC_DS, ORC_DN, ORC_DT, ORC_DB,
+    ORC_FE, ORC_FW, ORC_FS, ORC_FN, ORC_FT, ORC_FB,
+    ORC_AE, ORC_AW, ORC_AS, ORC_AN, ORC_AT, ORC_AB,
+    ORC_PRESS, ORC_COR, ORC_WALL, ORC_CROSS, ORC_VEGT,
+    ORC_PROD, ORC_BUOY,
+    ORC_NPIECE
+};
 int orc_calculate_capture(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st,
                           orc_capture *cap);
 

@@ -52,7 +52,7 @@ def _structs(ft):
     class OrcCapture(C.Structure):
         _fields_ = [("comp", C.c_int), ("i", C.c_int), ("j", C.c_int), ("iteration", C.c_int),
                     ("a", FPt), ("b", FPt), ("c", FPt), ("d", FPt), ("x_old", FPt), ("x_new", FPt),
-                    ("captured", C.c_int)]
+                    ("captured", C.c_int), ("piece", FPt)]
 
     class OrcDiagParams(C.Structure):
         _fields_ = [("nii", C.c_int), ("njj", C.c_int), ("nkk", C.c_int), ("kadvmax", C.c_int), ("dxk", ft),
@@ -66,6 +66,9 @@ U8P = C.POINTER(C.c_uint8)
 I16P = C.POINTER(C.c_int16)
 OrcParams, OrcInputs, OrcFields, OrcStats, OrcCapture, OrcDiagParams = _structs(C.c_float)
 _structs_f32 = (OrcParams, OrcInputs, OrcFields, OrcStats)
+# the per-level pieces the checking hook records (enum of pf_oracle.h, in order)
+PIECES = ("vis", "de", "dw", "ds", "dn", "dt", "db", "fe", "fw", "fs", "fn", "ft", "fb",
+          "ae", "aw", "as", "an", "at", "ab", "press", "cor", "wall", "cross", "vegt", "prod", "buoy")
 _S64 = None
 
 
@@ -180,7 +183,9 @@ def run(case, iterations=None, simd_width=None, keep_history=True, capture=None,
         comp, ci, cj, cit = capture
         nk = c.nkk + 2
         capbuf = {n: np.zeros(nk, dtype=np.float32) for n in ("a", "b", "c", "d", "x_old", "x_new")}
-        cap = OrcCapture(comp, ci, cj, cit, *[_p(capbuf[n]) for n in ("a", "b", "c", "d", "x_old", "x_new")], 0)
+        pieces = np.zeros((len(PIECES), nk), dtype=np.float32)
+        cap = OrcCapture(comp, ci, cj, cit, *[_p(capbuf[n]) for n in ("a", "b", "c", "d", "x_old", "x_new")], 0,
+                         _p(pieces))
         rc = lib.orc_calculate_capture(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st), C.byref(cap))
     elif threads > 1:
         rc = lib.orc_calculate_threads(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st), int(threads))
@@ -190,6 +195,7 @@ def run(case, iterations=None, simd_width=None, keep_history=True, capture=None,
         raise RuntimeError(f"orc_calculate failed: {rc}")
     if cap is not None:
         capbuf["captured"] = bool(cap.captured)
+        capbuf["pieces"] = {n: pieces[q] for q, n in enumerate(PIECES)}
         out["capture"] = capbuf
     stats = {n: getattr(st, n) for n, _ in OrcStats._fields_ if n != "delta_umax_hist"}
     stats["delta_umax_hist"] = hist[: st.iterations_run].copy()
