@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -103,11 +104,37 @@ void par_assign(std::vector<float> &v, const float *src, size_t n) {
 }  // namespace
 
 // FP64 mass-imbalance reduction: blocks of its first stage at most (one
-// partial pair per block)
+// partial pair per block), and the device scratch of the allocation-free
+// entry point gral_pf_mass_imbalance_noalloc: [2 * kMassMaxBlocks] block
+// partials, then the [2] result.  Every GPU the library runs on holds its own
+// copy for the life of the process; calls that overlap on one GPU race on it.
 static const int kMassMaxBlocks = 1024;
+static const size_t kMassScratchDoubles = 2 * (size_t)kMassMaxBlocks + 2;
+static __device__ __align__(256) double scratch[kMassScratchDoubles];
+static const int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[kMaxDevices];       // device address of scratch, per device
+
+// Looks up the device address of scratch on device dev (which must be the
+// current device) once per device; 0 on success.
+static int scratch_setup(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch[dev]) return 0;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+    g_scratch[dev] = static_cast<double *>(p);
+    return 0;
+}
+static double *scratch_of(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return g_scratch[dev];
+}
 
 struct gral_pf_handle {
     // grid
+    int device = -1;                // the HIP device current at create
     int nii = 0, njj = 0, nkk = 0, kadvmax = 0, vcf = 0, nkp = 0, nkh = 0;
     float dxk = 0, dyk = 0;
     std::vector<float> dzk, hok;
@@ -211,6 +238,13 @@ extern "C" int gral_pf_create(const gral_pf_grid *g, gral_pf_handle **out) {
     // optional per-handle budget of device memory in MiB (prepare fails with
     // GRAL_PF_ENOMEM beyond it), e.g. to keep room for other situations
     if (const char *cap = std::getenv("GRAL_PF_DEVICE_MB")) h->dev_cap = (size_t)std::strtoull(cap, nullptr, 10) << 20;
+    if (hipGetDevice(&h->device) != hipSuccess) {
+        delete h;
+        return GRAL_PF_EDEVICE;
+    }
+    // the allocation-free mass-imbalance entry point finds its scratch here;
+    // a failure only disables that entry point (it reports it)
+    (void)scratch_setup(h->device);
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
         gral_pf_destroy(h);
@@ -773,6 +807,26 @@ static int mass_handle_run(gral_pf_handle *h) {
     int rc;
     if (!h->mass_buf && (rc = dalloc(h, &h->mass_buf, 2 * (size_t)h->mass_nblk + 2))) return rc;
     return mass_run(h, h->mass_buf, h->mass_buf + 2 * (size_t)h->mass_nblk);
+}
+
+// The same reduction with the file-scope device scratch instead of memory
+// the handle allocates: no hipMalloc, hipFree or allocating call on any path.
+// Nothing in the scratch needs to start at zero: stage 1 writes every partial
+// stage 2 reads.
+extern "C" int gral_pf_mass_imbalance_noalloc(gral_pf_handle *h, double *sum_abs_div, double *sum_abs_flux) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "mass_imbalance_noalloc: call prepare first");
+    const size_t need = 2 * (size_t)h->mass_nblk + 2;
+    if (need > kMassScratchDoubles)
+        return fail(h, GRAL_PF_EINVAL, "mass_imbalance_noalloc: " + std::to_string(need) + " doubles of scratch needed, " +
+                                           std::to_string(kMassScratchDoubles) + " available");
+    double *p = scratch_of(h->device);
+    if (!p) return fail(h, GRAL_PF_EDEVICE, "mass_imbalance_noalloc: device address of the scratch array unavailable");
+    int rc = mass_run(h, p, p + 2 * (size_t)h->mass_nblk);
+    if (rc) return rc;
+    if (sum_abs_div) *sum_abs_div = h->mass[0];
+    if (sum_abs_flux) *sum_abs_flux = h->mass[1];
+    return GRAL_PF_OK;
 }
 
 extern "C" int gral_pf_mass_imbalance(gral_pf_handle *h, double *sum_abs_div, double *sum_abs_flux) {

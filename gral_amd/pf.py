@@ -22,7 +22,8 @@ U8P = C.POINTER(C.c_uint8)
 I16P = C.POINTER(C.c_int16)
 
 EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral_pf_upload_fields",
-           "gral_pf_set_met_profile", "gral_pf_prepare", "gral_pf_iterate", "gral_pf_mass_imbalance", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
+           "gral_pf_set_met_profile", "gral_pf_prepare", "gral_pf_iterate", "gral_pf_mass_imbalance",
+           "gral_pf_mass_imbalance_noalloc", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
            "gral_pf_diagnostic",
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
@@ -102,6 +103,7 @@ def load(path: str = LIB_PATH):
     lib.gral_pf_upload_fields.argtypes = [H, FP, FP, FP]
     lib.gral_pf_set_met_profile.argtypes = [H, C.c_int32, C.c_int32, FP, FP, FP]
     lib.gral_pf_mass_imbalance.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.gral_pf_mass_imbalance_noalloc.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.gral_pf_prepare.argtypes = [H, C.POINTER(Params)]
     lib.gral_pf_iterate.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(Stats)]
     lib.gral_pf_finish.argtypes = [H, C.POINTER(Stats)]

@@ -135,6 +135,17 @@ int gral_pf_iterate(gral_pf_handle *h, int32_t n, int32_t check_convergence, gra
  * give the same bits.  gral_pf_finish stores both in gral_pf_stats. */
 int gral_pf_mass_imbalance(gral_pf_handle *h, double *sum_abs_div, double *sum_abs_flux);
 
+/* gral_pf_mass_imbalance without any device allocation: the reduction's
+ * partial sums live in a fixed file-scope device array of the library
+ * (2 * 1024 + 2 doubles, 16 KiB, one per GPU, held until the process ends)
+ * instead of memory the handle allocates on first use.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight loops between iterations).  Same result, same bits
+ * as gral_pf_mass_imbalance.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (handles used from several threads) race and must be
+ * serialised by the caller. */
+int gral_pf_mass_imbalance_noalloc(gral_pf_handle *h, double *sum_abs_div, double *sum_abs_flux);
+
 /* Epilogue checks (PrognosticFlowfield.cs:1187-1245) and transfer back to host. */
 int gral_pf_finish(gral_pf_handle *h, gral_pf_stats *st);
 
