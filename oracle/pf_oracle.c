@@ -52,6 +52,17 @@ static inline float p3f(float x) { return x * x * x; }
 static inline float p5f(float x) { return x * x * x * x * x; }
 static inline double p5d(double x) { return x * x * x * x * x; }
 
+/* Planted mistakes for the oracle's own tests (tests/test_oracle_mutants.py
+ * builds the file with -DORC_MUTANT=n to check that the closed-form tests
+ * catch each one).  The default build has ORC_MUTANT = 0: no mistake.
+ *   1 eddy viscosity without the 1/2 under the root    5 w-equation x flux from level k+1
+ *   2 mixing length from the level top                 6 TKE production without (v_z + w_y)^2
+ *   3 east power-law coefficient upwinded the wrong way 7 divergence with the x flux sign swapped
+ *   4 wall function at z1 = dz instead of dz/2         8 boundary copy regardless of the flow sign */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 /* ---- state ---------------------------------------------------------- */
 typedef struct {
     int NII, NJJ, NKK, KADVMAX, VCF, NJ2, NKP, W, model;
@@ -111,7 +122,7 @@ static void boundary_conditions(St *s) {
             if (iM) {
                 for (int k = 1; k <= VIc; ++k) {
                     float u = s->UK[IX3(s, i - 2, j, k)];
-                    if (u >= 0) s->UK[IX3(s, i - 1, j, k)] = u;
+                    if (u >= 0 || ORC_MUTANT == 8) s->UK[IX3(s, i - 1, j, k)] = u;
                     s->VK[IX3(s, i - 2, j, k)] = s->VK[IX3(s, i - 1, j, k)];
                     s->WK[IX3(s, i - 2, j, k)] = s->WK[IX3(s, i - 1, j, k)];
                 }
@@ -176,6 +187,7 @@ static void divergence(St *s) {
                     DIV[k] = (UK[k] - UKip[k]) * s->DYK * s->DZK[k] + (VK[k] - VKjp[k]) * s->DXK * s->DZK[k] + (WK[k] - WK[k + 1]) * s->AREAxy;
                 else
                     DIV[k] = (UK[k] - UKip[k]) * s->DYK * s->DZK[k] + (VK[k] - VKjp[k]) * s->DXK * s->DZK[k] - WK[k + 1] * s->AREAxy;
+                if (ORC_MUTANT == 7) DIV[k] += 2 * (UKip[k] - UK[k]) * s->DYK * s->DZK[k];
                 DPM[k] = 0;
             }
         }
@@ -373,6 +385,7 @@ static float ml_vis(St *s, int i, int j, int k, int kk) {
     /* the 512-bit solvers fuse the half-cell offset into the ground height
      * (U-prognostic-microscale_1_Vec512.cs:218) */
     float zs = (W == 16) ? s->HOKART[kk] - fmaf(DZ, 0.5f, s->HOKART[KK]) : s->HOKART[kk] - s->HOKART[KK] - DZ * 0.5f;
+    if (ORC_MUTANT == 2) zs = s->HOKART[kk] - s->HOKART[KK];
     bool mask_v_enable = (KKP1 >= k && KKP1 < k + W);
     float mask = (kk > KKP1) ? 1.0f : 0.0f;
     float VIS = 0;
@@ -385,7 +398,7 @@ static float ml_vis(St *s, int i, int j, int k, int kk) {
         float DUDZ = in * in;
         in = (VK[kk + 1] - VK[kk - 1]) / (2.0f * DZ);
         float DVDZ = in * in;
-        float v = vmin(mixL * mixL * sqrtf(0.5f * (DUDZ + DVDZ)), 15.0f);
+        float v = vmin(mixL * mixL * sqrtf((ORC_MUTANT == 1 ? 1.0f : 0.5f) * (DUDZ + DVDZ)), 15.0f);
         VIS = mask_v_enable ? v * mask : v;
     }
     return VIS;
@@ -468,7 +481,7 @@ static void u1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
     float PS = fabsf(FS / DS), PN = fabsf(FN / DN), PT = fabsf(FT / DT);
     float BIM = ml_coef(W, DT, PT, vmax(-FT, 0.0f));
     float CIM = ml_coef(W, DB, PB, vmax(FB, 0.0f));
-    float AE1 = ml_coef(W, DE, PE, vmax(-FE, 0.0f));
+    float AE1 = ml_coef(W, DE, PE, vmax(ORC_MUTANT == 3 ? FE : -FE, 0.0f));
     float AW1 = ml_coef(W, DW, PW, vmax(FW, 0.0f));
     float AS1 = ml_coef(W, DS, PS, vmax(FS, 0.0f));
     float AN1 = ml_coef(W, DN, PN, vmax(-FN, 0.0f));
@@ -669,7 +682,8 @@ static void w1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
     if ((ADV(s, i, jP1) < 1) || (j == s->NJJ - 1)) DN = 0;
     if ((ADV(s, iM1, j) < 1) || (i == 2)) DW = 0;
     if ((ADV(s, iP1, j) < 1) || (i == s->NII - 1)) DE = 0;
-    float FE = 0.25f * (UKS[kk] + UKS_E[kk] + UKS[kk - 1] + UKS_E[kk - 1]) * DYKDZK;
+    const int kw = (ORC_MUTANT == 5) ? kk + 1 : kk - 1;
+    float FE = 0.25f * (UKS[kk] + UKS_E[kk] + UKS[kw] + UKS_E[kw]) * DYKDZK;
     float FW = 0.25f * (UKS[kk] + UKS_W[kk] + UKS[kk - 1] + UKS_W[kk - 1]) * DYKDZK;
     float FS = 0.25f * (VKS[kk] + VKS_S[kk] + VKS[kk - 1] + VKS_S[kk - 1]) * DXKDZK;
     float FN = 0.25f * (VKS[kk] + VKS_N[kk] + VKS[kk - 1] + VKS_N[kk - 1]) * DXKDZK;
@@ -1179,7 +1193,7 @@ static void tke_sweep(St *s, int IS, int JS) {
                 }
                 float PROTURB = VIS * AREAxy * DZ *
                     (2.0f * p2f(DUDX) + 2.0f * p2f(DVDY) + 2.0f * p2f(DWDZ) +
-                     p2f(DUDY + DVDX) + p2f(DUDZ + DWDX) + p2f(DVDZ + DWDY) - TDISS[k]);
+                     p2f(DUDY + DVDX) + p2f(DUDZ + DWDX) + (ORC_MUTANT == 6 ? 0.0f : p2f(DVDZ + DWDY)) - TDISS[k]);
                 float PROBOU = (float)((double)(-s->PotdTdz) * 9.81 / 283.0 * (double)VIS * 1.35 * (double)AREAxy * (double)DZ);
                 s->rec = 1;
                 piece(s, i, j, ORC_VIS, k, VIS);
@@ -1366,7 +1380,7 @@ static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, o
                     if ((double)rough >= (double)s->DZK[k] * 0.1)
                         s->HT[IX2(s, i, j)] = 0.4f / logf((s->DZK[k] + s->DZK[k + 1] * 0.5f) / s->DZK[k] * 0.1f);
                     else
-                        s->HT[IX2(s, i, j)] = 0.4f / logf(s->DZK[k] * 0.5f / rough);
+                        s->HT[IX2(s, i, j)] = 0.4f / logf(s->DZK[k] * (ORC_MUTANT == 4 ? 1.0f : 0.5f) / rough);
                 } else {
                     s->HO[IX2(s, i, j)] = 0.4f / logf(s->DZK[k] * 0.5f / s->building_Z0);
                 }

@@ -75,13 +75,26 @@ _S64 = None
 _lib = None
 
 
-def load(build_if_missing: bool = True):
+_other = {}
+
+
+def load(build_if_missing: bool = True, path: str = None):
+    """The oracle library; ``path`` loads another build of pf_oracle.c (the
+    mutant builds of the oracle's own tests) with the same interface."""
     global _lib
+    if path is not None:
+        if path not in _other:
+            _other[path] = _configure(C.CDLL(path))
+        return _other[path]
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH) and build_if_missing:
         subprocess.run(["make", "-s", "-C", HERE], check=True)
-    _lib = C.CDLL(LIB_PATH)
+    _lib = _configure(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def _configure(_lib):
     _lib.orc_calculate.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcInputs), C.POINTER(OrcFields),
                                    C.POINTER(OrcStats)]
     _lib.orc_calculate.restype = C.c_int
@@ -129,7 +142,8 @@ def _p(a, t=FP):
     return a.ctypes.data_as(t)
 
 
-def run(case, iterations=None, simd_width=None, keep_history=True, capture=None, threads=1, fp64=False):
+def run(case, iterations=None, simd_width=None, keep_history=True, capture=None, threads=1, fp64=False,
+        lib_path=None):
     """Run PrognosticFlowfield.Calculate on ``case`` (a gral_amd.domain.Case)
     with the deterministic CPU oracle.  Returns a dict of output fields and stats.
     The input case is not modified.  threads > 1 selects the timing mode: the
@@ -142,7 +156,7 @@ def run(case, iterations=None, simd_width=None, keep_history=True, capture=None,
         dt, FPt = np.float64, C.POINTER(C.c_double)
         assert capture is None and threads == 1
     else:
-        lib = load()
+        lib = load(path=lib_path)
         OrcParams, OrcInputs, OrcFields, OrcStats = _structs_f32
         dt, FPt = np.float32, FP
     c = case
