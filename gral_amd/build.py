@@ -11,7 +11,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libgral_pf.so")
 SOURCES = [os.path.join(HERE, "csrc", "pf_kernels.hip"), os.path.join(HERE, "csrc", "pf_api.cpp"),
-           os.path.join(HERE, "csrc", "pf_zip.cpp"), os.path.join(HERE, "csrc", "pf_init.hip")]
+           os.path.join(HERE, "csrc", "pf_zip.cpp"), os.path.join(HERE, "csrc", "pf_init.hip"),
+           os.path.join(HERE, "csrc", "pf_adjoint.hip")]
 LIBS = ["-lz"]  # zlib: deflate of the flow-field files' zip container
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -43,6 +43,9 @@ extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStrea
 extern "C" int pf_launch_gff(const GffArgs *ga, hipStream_t s);
 extern "C" int pf_launch_wind(const WindArgs *wa, hipStream_t s);
 extern "C" int pf_launch_mass(const Dom *dd, int nsit, int nblk, double *partial, double *out, hipStream_t s);
+extern "C" int pf_launch_projection(const Dom *hd, const Dom *dd, int iter, hipStream_t s, long long *launches);
+extern "C" int pf_launch_adjoint_step(const Dom *hd, const Dom *dd, const AdjArgs *args, int iter, size_t n3,
+                                      hipStream_t s);
 
 namespace {
 
@@ -152,6 +155,11 @@ struct gral_pf_handle {
     double *mass_buf = nullptr;     // FP64 mass imbalance: [mass_nblk][2] block partials, then the [2] result
     int mass_nblk = 0;
     double mass[2] = {0, 0};        // sum |DIV|, sum |face flux| at the last finish / mass_imbalance call
+    // adjoint of the pressure-correction steps (gral_pf_forward_ckpt / gral_pf_vjp)
+    std::vector<float> app, abp, atp;   // per-level pressure coefficients of the prologue
+    float *adj = nullptr;           // ub vb wb pb qb mb db, checkpoint uk vk wk, ptab[2] ttab[2]: 15 fields
+    int *adj_src = nullptr;         // stale-top source column per column, for IS = +1 and -1
+    int adj_first = 0, adj_n = 0;   // iterations of the last forward
     size_t dev_cap = 0;             // optional cap (GRAL_PF_DEVICE_MB), 0 = none
     std::vector<float> uk, vk, wk, uks, vks, wks, dpm, dpmnew, turb, tdiss;
     std::vector<int> vi;
@@ -344,6 +352,9 @@ static void free_device(gral_pf_handle *h) {
     h->dd = nullptr;
     h->d = Dom{};
     h->mass_buf = nullptr;
+    h->adj = nullptr;
+    h->adj_src = nullptr;
+    h->adj_n = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -523,7 +534,9 @@ static float *stage(gral_pf_handle *h) {
 }
 
 // copy levels [0, nkp) of a host full-depth field into a device [.,.,nkp] field
-static int put_field(gral_pf_handle *h, float *dptr, const std::vector<float> &host) {
+static int put_raw(gral_pf_handle *h, float *dptr, const float *host);
+static int put_field(gral_pf_handle *h, float *dptr, const std::vector<float> &host) { return put_raw(h, dptr, host.data()); }
+static int put_raw(gral_pf_handle *h, float *dptr, const float *host) {
     const size_t n2 = h->n2(), n = n2 * (size_t)h->nkp;
     float *tmp = stage(h);
     if (!tmp) return fail(h, GRAL_PF_ENOMEM, "hipHostMalloc of the staging buffer failed");
@@ -538,7 +551,9 @@ static int put_field(gral_pf_handle *h, float *dptr, const std::vector<float> &h
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
 }
-static int get_field(gral_pf_handle *h, const float *dptr, std::vector<float> &host) {
+static int get_raw(gral_pf_handle *h, const float *dptr, float *host);
+static int get_field(gral_pf_handle *h, const float *dptr, std::vector<float> &host) { return get_raw(h, dptr, host.data()); }
+static int get_raw(gral_pf_handle *h, const float *dptr, float *host) {
     const size_t n2 = h->n2(), n = n2 * (size_t)h->nkp;
     float *tmp = stage(h);
     if (!tmp) return fail(h, GRAL_PF_ENOMEM, "hipHostMalloc of the staging buffer failed");
@@ -632,6 +647,7 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
         abp[k] = (k > 1) ? area / (0.5f * (DZK[k - 1] + DZK[k])) : area / DZK[k];
         atp[k] = area / (0.5f * (DZK[k + 1] + DZK[k]));
     }
+    h->app = app; h->abp = abp; h->atp = atp;
     // sweep kernels: strip-pipelined (default) or one launch per anti-diagonal
     // (GRAL_PF_SWEEP = strip | diag)
     const char *sw = std::getenv("GRAL_PF_SWEEP");
@@ -1030,6 +1046,164 @@ extern "C" int gral_pf_solve(gral_pf_handle *h, const gral_pf_params *p, gral_pf
     rc = gral_pf_iterate(h, p->iter_max, 1, st);
     if (rc) return rc;
     return gral_pf_finish(h, st);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the pressure-correction steps (pf_adjoint.hip)
+// ---------------------------------------------------------------------------
+// P, TERMP and the stale-top source of every column's pressure elimination in
+// the reference's row order for IS = +1 (v = 0) and -1 (v = 1): the row
+// scratch starts at zero for every row and each column leaves its P, Q at its
+// levels KK+1..VI; the next column starts from whatever level VI+1 holds.
+static void pressure_tables(gral_pf_handle *h, int v, std::vector<float> &tp, std::vector<float> &tt,
+                            std::vector<int> &src) {
+    const size_t n3 = h->n2() * (size_t)h->nkp;
+    tp.assign(n3, 0.0f);
+    tt.assign(n3, 0.0f);
+    src.assign(h->n2(), -1);
+    par_for(2, h->njj, [&](int, long ja, long jb) {
+        std::vector<float> prow((size_t)h->nkh + 1, 0.0f);
+        std::vector<int> writer((size_t)h->nkh + 1, -1);
+        for (int j = (int)ja; j < jb; ++j) {
+            std::fill(prow.begin(), prow.end(), 0.0f);
+            std::fill(writer.begin(), writer.end(), -1);
+            for (int i1 = 2; i1 <= h->nii - 1; ++i1) {
+                const int i = (v == 1) ? h->nii - i1 + 1 : i1;
+                const size_t c = h->i2(i, j);
+                if (h->adv[c] != 1) continue;
+                const int KK = h->kk[c], VI = h->vi[c];
+                const size_t o = c * (size_t)h->nkp;
+                src[c] = writer[VI + 1];
+                for (int k = VI; k > KK; --k) {
+                    const float TERMP = 1 / (h->app[k] - h->atp[k] * prow[k + 1]);
+                    prow[k] = h->abp[k] * TERMP;
+                    tp[o + k] = prow[k];
+                    tt[o + k] = TERMP;
+                    writer[k] = i;
+                }
+            }
+        }
+    });
+}
+
+static int adj_alloc(gral_pf_handle *h) {
+    if (h->adj) return 0;
+    const size_t n2 = h->n2(), n3 = n2 * (size_t)h->nkp;
+    float *a;
+    int *src;
+    int rc;
+    if ((rc = dalloc(h, &a, 15 * n3)) || (rc = dalloc(h, &src, 2 * n2))) return rc;
+    std::vector<float> tp, tt;
+    std::vector<int> sv;
+    for (int v = 0; v < 2; ++v) {
+        pressure_tables(h, v, tp, tt, sv);
+        HIPCHK(h, hipMemcpyAsync(a + (size_t)(10 + v) * n3, tp.data(), n3 * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(a + (size_t)(12 + v) * n3, tt.data(), n3 * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(src + (size_t)v * n2, sv.data(), n2 * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    h->adj = a;
+    h->adj_src = src;
+    return 0;
+}
+
+extern "C" int gral_pf_forward_ckpt(gral_pf_handle *h, int32_t n, int32_t quantise) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "forward_ckpt: call prepare first");
+    if (n < 1 || n > 100000) return fail(h, GRAL_PF_EINVAL, "forward_ckpt: n must be 1..100000");
+    if (quantise != 0 && quantise != 1) return fail(h, GRAL_PF_EINVAL, "forward_ckpt: quantise must be 0 or 1");
+    int rc = adj_alloc(h);
+    if (rc) return rc;
+    const size_t n3 = h->n2() * (size_t)h->nkp;
+    float *ck = h->adj + 7 * n3;
+    HIPCHK(h, hipMemcpyAsync(ck, h->d.uk, n3 * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(ck + n3, h->d.vk, n3 * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(ck + 2 * n3, h->d.wk, n3 * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    h->d.noquant = quantise ? 0 : 1;
+    HIPCHK(h, hipMemcpyAsync(h->dd, &h->d, sizeof(Dom), hipMemcpyHostToDevice, h->stream));
+    long long launches = 0;
+    int e = 0;
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    for (int q = 0; q < n && !e; ++q) e = pf_launch_projection(&h->d, h->dd, h->iter_loops + q, h->stream, &launches);
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    h->d.noquant = 0;
+    HIPCHK(h, hipMemcpyAsync(h->dd, &h->d, sizeof(Dom), hipMemcpyHostToDevice, h->stream));
+    if (e) return fail(h, GRAL_PF_EDEVICE, std::string("forward_ckpt: launch failed: ") + hipGetErrorString((hipError_t)e));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int serr = 0;
+    HIPCHK(h, hipMemcpy(&serr, h->d.sync, sizeof(int), hipMemcpyDeviceToHost));
+    if (serr) return fail(h, GRAL_PF_EDEVICE, "forward_ckpt: a strip wait timed out (results invalid)");
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->timing = gral_pf_timing{};
+    h->timing.ms_total = ms;
+    h->timing.launches = launches;
+    h->timing.active_cells = h->active;
+    h->timing.cell_iterations = h->active * (long long)n;
+    h->timing.situations = 1;
+    h->adj_first = h->iter_loops;
+    h->adj_n = n;
+    h->host_stale = true;
+    h->gff_ok = false;
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_restore_ckpt(gral_pf_handle *h) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared || h->adj_n < 1) return fail(h, GRAL_PF_ESTATE, "restore_ckpt: no forward_ckpt since prepare");
+    const size_t n3 = h->n2() * (size_t)h->nkp;
+    const float *ck = h->adj + 7 * n3;
+    HIPCHK(h, hipMemcpyAsync(h->d.uk, ck, n3 * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d.vk, ck + n3, n3 * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d.wk, ck + 2 * n3, n3 * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->host_stale = true;
+    h->gff_ok = false;
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_vjp(gral_pf_handle *h, const float *du, const float *dv, const float *dw, float *gu, float *gv,
+                           float *gw) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared || h->adj_n < 1) return fail(h, GRAL_PF_ESTATE, "vjp: call forward_ckpt first");
+    if (!du || !dv || !dw || !gu || !gv || !gw) return fail(h, GRAL_PF_EINVAL, "vjp: null argument");
+    const size_t n2 = h->n2(), n3 = n2 * (size_t)h->nkp;
+    float *a = h->adj;
+    AdjArgs g{};
+    g.ub = a; g.vb = a + n3; g.wb = a + 2 * n3; g.pb = a + 3 * n3; g.qb = a + 4 * n3; g.mb = a + 5 * n3; g.db = a + 6 * n3;
+    int rc;
+    if ((rc = put_raw(h, g.ub, du)) || (rc = put_raw(h, g.vb, dv)) || (rc = put_raw(h, g.wb, dw))) return rc;
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    for (int q = h->adj_n - 1; q >= 0; --q) {
+        const int it = h->adj_first + q;
+        int IS = 1;
+        switch (it % 4) {
+            case 0: case 1: IS = -1; break;
+            default: break;
+        }
+        const int v = IS == -1 ? 1 : 0;
+        g.ptab = a + (size_t)(10 + v) * n3;
+        g.ttab = a + (size_t)(12 + v) * n3;
+        g.src = h->adj_src + (size_t)v * n2;
+        int e = pf_launch_adjoint_step(&h->d, h->dd, &g, it, n3, h->stream);
+        if (e) return fail(h, GRAL_PF_EDEVICE, std::string("vjp: launch failed: ") + hipGetErrorString((hipError_t)e));
+    }
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->timing = gral_pf_timing{};
+    h->timing.ms_total = ms;
+    h->timing.active_cells = h->active;
+    h->timing.cell_iterations = h->active * (long long)h->adj_n;
+    h->timing.situations = 1;
+    // levels the device does not hold pass through unchanged (the steps never touch them)
+    const size_t nh = n2 * (size_t)h->nkh;
+    par_copy(gu, du, nh * sizeof(float));
+    par_copy(gv, dv, nh * sizeof(float));
+    par_copy(gw, dw, nh * sizeof(float));
+    if ((rc = get_raw(h, g.ub, gu)) || (rc = get_raw(h, g.vb, gv)) || (rc = get_raw(h, g.wb, gw))) return rc;
+    return GRAL_PF_OK;
 }
 
 extern "C" int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, float *wk, float *uks, float *vks,

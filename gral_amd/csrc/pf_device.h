@@ -50,6 +50,7 @@ struct DomT {
     P<float> wcol;                   // [columns] wall term of the U / V sweep at KKART+1
     P<const float> ptab[2], ttab[2]; // pressure PIMP/TERMP per column for IS=+1 / IS=-1
     P<const int4> meta;              // per column: KK, VI, KSTART | adv<<16 | vegp<<17, KKm_i | KKm_j<<16
+    int noquant;                     // 1: the velocity correction skips the FP64 quantisation (adjoint forward)
 };
 using Dom = DomT<GenericPtr>;        // host side, and the descriptor array handed to the kernels
 using DomG = DomT<GlobalPtr>;        // the kernels' view of the same bytes
@@ -124,6 +125,18 @@ struct WindArgs {
     const double *x, *y;
     float *ux, *uy, *uz;
     int32_t *ck;                      // IndexK
+};
+
+// Buffers of one step of the adjoint (pf_adjoint.hip), each [nii+2][njj+2][nkp]
+// except the per-column source table.
+struct AdjArgs {
+    float *ub, *vb, *wb;        // adjoint of UK, VK, WK (in: of the step's output; out: of its input)
+    float *pb;                  // adjoint of DPM
+    float *qb;                  // adjoint of the elimination values Q read as stale tops by later columns
+    float *mb;                  // adjoint of DIM per column and level
+    float *db;                  // adjoint of DIV
+    const float *ptab, *ttab;   // P and TERMP of every column for this step's IS
+    const int *src;             // per column: i of the column whose Q the elimination starts from, or -1
 };
 
 #define SYNC_BLOCK0 64               // Dom::sync: sticky error flag at 0, sweep blocks from here

@@ -275,17 +275,18 @@ __global__ void k_correction(const Dom *__restrict__ ds) {
     size_t o = c3(d, i, j), oim = c3(d, i - 1, j), ojm = c3(d, i, j - 1);
     for (int k = 1 + threadIdx.x; k <= VI; k += blockDim.x) {
         float dpmk = d.dpm[o + k];
+        const bool q = !d.noquant;
         if (i > 2 && KK < k && KKiM < k) {
             float e = d.uk[o + k] + (d.dpm[oim + k] - dpmk) / d.dxk * d.dtime;
-            d.uk[o + k] = quantise(e, d.frund);
+            d.uk[o + k] = q ? quantise(e, d.frund) : e;
         }
         if (j > 2 && KK < k && KKjM < k) {
             float e = d.vk[o + k] + (d.dpm[ojm + k] - dpmk) / d.dyk * d.dtime;
-            d.vk[o + k] = quantise(e, d.frund);
+            d.vk[o + k] = q ? quantise(e, d.frund) : e;
         }
         if (KK + 1 < k) {
             float e = d.wk[o + k] + (d.dpm[o + k - 1] - dpmk) / (d.dzk[k] + d.dzk[k - 1]) * 2.0f * d.dtime;
-            d.wk[o + k] = quantise(e, d.frund);
+            d.wk[o + k] = q ? quantise(e, d.frund) : e;
         } else {
             d.wk[o + k] = 0.0f;
         }
@@ -2673,8 +2674,19 @@ struct Launcher {
 // of every launch is situation z; grid x/y are sized for the largest
 // situation and each kernel skips what lies outside its own grid.  All
 // situations share the turbulence model and the sweep kind.
+static int launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, int slot, hipStream_t s,
+                            long long *launches, Probe *probe, bool proj);
 extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, int slot, hipStream_t s,
                                    long long *launches, Probe *probe) {
+    return launch_iteration(hd, nsit, dd, iter, slot, s, launches, probe, false);
+}
+// The pressure-correction part of iteration iter only (divergence, pressure
+// sweep, velocity correction): the forward of the adjoint.  No residual slot.
+extern "C" int pf_launch_projection(const Dom *hd, const Dom *dd, int iter, hipStream_t s, long long *launches) {
+    return launch_iteration(hd, 1, dd, iter, -1, s, launches, nullptr, true);
+}
+static int launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, int slot, hipStream_t s,
+                            long long *launches, Probe *probe, bool proj) {
     if (nsit < 1) return 0;
     Launcher L{s, probe};
     const Dom &d0 = hd[0];
@@ -2719,8 +2731,10 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
     int ncols = (nii - 2) * (njj - 2);
     const size_t bc_shm = (size_t)(max(nii, njj) + 2) * (sizeof(int) + 1);
     const unsigned bc_threads = (unsigned)min(64 * kb, 1024);
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(1, njj - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ns, dim3(1, nii - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
+    if (!proj) {
+        L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(1, njj - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
+        L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ns, dim3(1, nii - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
+    }
     const int ncol8 = ((ncols + COLS_PER_WAVE - 1) / COLS_PER_WAVE + NXCD - 1) / NXCD * NXCD;  // x extent of the column kernels
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
     size_t nrow = (size_t)nj2 * nkp;
@@ -2749,6 +2763,10 @@ extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int i
         }
     }
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_correction, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
+    if (proj) {
+        if (launches) *launches += L.n;
+        return (int)hipGetLastError();
+    }
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_averages, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_avg_edges, dim3(njj - 2, 1, Z), dim3(64), 0, s, dd); });
     size_t shm = sizeof(float) * 5 * nkp;

@@ -23,7 +23,7 @@ I16P = C.POINTER(C.c_int16)
 
 EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral_pf_upload_fields",
            "gral_pf_set_met_profile", "gral_pf_prepare", "gral_pf_iterate", "gral_pf_mass_imbalance",
-           "gral_pf_mass_imbalance_noalloc", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
+           "gral_pf_mass_imbalance_noalloc", "gral_pf_forward_ckpt", "gral_pf_restore_ckpt", "gral_pf_vjp", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
            "gral_pf_diagnostic",
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
@@ -104,6 +104,9 @@ def load(path: str = LIB_PATH):
     lib.gral_pf_set_met_profile.argtypes = [H, C.c_int32, C.c_int32, FP, FP, FP]
     lib.gral_pf_mass_imbalance.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.gral_pf_mass_imbalance_noalloc.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.gral_pf_forward_ckpt.argtypes = [H, C.c_int32, C.c_int32]
+    lib.gral_pf_restore_ckpt.argtypes = [H]
+    lib.gral_pf_vjp.argtypes = [H, FP, FP, FP, FP, FP, FP]
     lib.gral_pf_prepare.argtypes = [H, C.POINTER(Params)]
     lib.gral_pf_iterate.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(Stats)]
     lib.gral_pf_finish.argtypes = [H, C.POINTER(Stats)]
@@ -221,6 +224,22 @@ class PrognosticFlowfield:
         a, b = C.c_double(0), C.c_double(0)
         self._check(self.lib.gral_pf_mass_imbalance(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def forward_ckpt(self, n, quantise=False):
+        """n pressure-correction steps (divergence, pressure sweep, velocity
+        correction) on the device fields, keeping a checkpoint of the start."""
+        self._check(self.lib.gral_pf_forward_ckpt(self.h, int(n), int(bool(quantise))))
+
+    def restore_ckpt(self):
+        self._check(self.lib.gral_pf_restore_ckpt(self.h))
+
+    def vjp(self, du, dv, dw):
+        """dL/d(UK, VK, WK) at the start of the last forward_ckpt window from
+        dL/d(UK, VK, WK) at its end (full-depth arrays)."""
+        du, dv, dw = _f32(du), _f32(dv), _f32(dw)
+        gu, gv, gw = np.zeros_like(du), np.zeros_like(dv), np.zeros_like(dw)
+        self._check(self.lib.gral_pf_vjp(self.h, _ptr(du), _ptr(dv), _ptr(dw), _ptr(gu), _ptr(gv), _ptr(gw)))
+        return gu, gv, gw
 
     def finish(self):
         st = Stats()

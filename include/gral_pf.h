@@ -146,6 +146,35 @@ int gral_pf_mass_imbalance(gral_pf_handle *h, double *sum_abs_div, double *sum_a
  * serialised by the caller. */
 int gral_pf_mass_imbalance_noalloc(gral_pf_handle *h, double *sum_abs_div, double *sum_abs_flux);
 
+/* ---- Adjoint of the pressure-correction steps ----------------------------
+ * The linear part of a SIMPLE iteration: the mass divergence
+ * (PrognosticFlowfield.cs:723-758), the pressure-correction GS-TDMA sweep in
+ * the iteration's directions (:770-901, including the row scratch reused
+ * across each row) and the velocity correction (:904-949), with the
+ * geometry, DTIME and the sweep directions of the prepared situation held
+ * fixed.  Without the FP64 quantisation (quantise = 0) a window of n such
+ * steps is a linear map of the device-resident UK, VK, WK; gral_pf_vjp gives
+ * its vector-Jacobian product (hand-written transposed kernels: the sweep runs
+ * backwards over the anti-diagonals with transposed Thomas recurrences).  The
+ * reference has no adjoint; this is the pressure-correction slice of the
+ * sensitivity of receptor winds to the inflow.
+ *
+ * gral_pf_forward_ckpt runs n steps from the handle's current fields (after
+ * prepare, or after iterations), numbering them from the handle's next SIMPLE
+ * iteration for their sweep directions, and keeps a device checkpoint of the
+ * fields it started from.  It does not advance the SIMPLE iteration count. */
+int gral_pf_forward_ckpt(gral_pf_handle *h, int32_t n, int32_t quantise);
+
+/* The fields back to the checkpoint of the last gral_pf_forward_ckpt. */
+int gral_pf_restore_ckpt(gral_pf_handle *h);
+
+/* VJP of the last gral_pf_forward_ckpt window: du/dv/dw are dL/d(UK, VK, WK)
+ * of its result, gu/gv/gw receive dL/d(UK, VK, WK) of its start, all
+ * [nii+2][njj+2][nkk+2].  The map is the one without quantisation (with
+ * quantise = 1 this is the straight-through adjoint). */
+int gral_pf_vjp(gral_pf_handle *h, const float *du, const float *dv, const float *dw, float *gu, float *gv,
+                float *gw);
+
 /* Epilogue checks (PrognosticFlowfield.cs:1187-1245) and transfer back to host. */
 int gral_pf_finish(gral_pf_handle *h, gral_pf_stats *st);
 
@@ -172,7 +201,8 @@ int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, float *wk, 
                             float *tdiss);
 
 /* Device time (ms, HIP events on the solver's stream around the whole call)
- * and work of the most recent gral_pf_iterate / gral_pf_batch_iterate call;
+ * and work of the most recent gral_pf_iterate / gral_pf_batch_iterate call
+ * (or gral_pf_forward_ckpt / gral_pf_vjp: their steps);
  * for measurement.  No reference counterpart ("Flow field [s]" of
  * Program.cs:592 is the closest). */
 typedef struct {

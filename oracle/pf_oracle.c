@@ -81,6 +81,8 @@ typedef struct {
     int rec;                          /* the lane being evaluated enters the recurrence (hook only) */
     int iter, comp;                   /* current SIMPLE iteration and sweep (for the hook) */
     int nthreads;                     /* > 1: the timing mode of orc_calculate_threads */
+    int proj;                         /* orc_project: divergence, pressure and correction only */
+    int noquant;                      /* orc_project with quantise = 0: no FP64 quantisation */
 } St;
 
 static bool cap_on(const St *s, int comp, int i, int j) {
@@ -271,15 +273,15 @@ static void correction(St *s) {
                 float DPMk = DPM[k];
                 if (i > 2 && KK < k && KKiM < k) {
                     float e = UK[k] + (DPMim[k] - DPMk) / s->DXK * s->DTIME;
-                    UK[k] = (float)((double)conv_to_int_d((double)e * Frund) / Frund);
+                    UK[k] = s->noquant ? e : (float)((double)conv_to_int_d((double)e * Frund) / Frund);
                 }
                 if (j > 2 && KK < k && KKjM < k) {
                     float e = VK[k] + (DPMjm[k] - DPMk) / s->DYK * s->DTIME;
-                    VK[k] = (float)((double)conv_to_int_d((double)e * Frund) / Frund);
+                    VK[k] = s->noquant ? e : (float)((double)conv_to_int_d((double)e * Frund) / Frund);
                 }
                 if (KK + 1 < k) {
                     float e = WK[k] + (DPM[k - 1] - DPMk) / (s->DZK[k] + s->DZK[k - 1]) * 2.0f * s->DTIME;
-                    WK[k] = (float)((double)conv_to_int_d((double)e * Frund) / Frund);
+                    WK[k] = s->noquant ? e : (float)((double)conv_to_int_d((double)e * Frund) / Frund);
                 } else {
                     WK[k] = 0;
                 }
@@ -1279,24 +1281,28 @@ float orc_powlaw_scalar(float d, float peclet) { return pow_d(d, peclet); }
 int orc_conv_to_int(double x) { return conv_to_int_d(x); }
 
 static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap,
-                     int nthreads);
+                     int nthreads, int proj, int quantise);
 
 int orc_calculate_threads(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, int nthreads) {
-    return calculate(p, in, f, st, NULL, nthreads < 1 ? 1 : nthreads);
+    return calculate(p, in, f, st, NULL, nthreads < 1 ? 1 : nthreads, 0, 1);
 }
 
 int orc_calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st) {
-    return calculate(p, in, f, st, NULL, 1);
+    return calculate(p, in, f, st, NULL, 1, 0, 1);
+}
+
+int orc_project(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, int quantise) {
+    return calculate(p, in, f, st, NULL, 1, 1, quantise);
 }
 
 int orc_calculate_capture(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap) {
     if (cap && (!cap->a || !cap->b || !cap->c || !cap->d || !cap->x_old || !cap->x_new)) return -7;
     if (cap) cap->captured = 0;
-    return calculate(p, in, f, st, cap, 1);
+    return calculate(p, in, f, st, cap, 1, 0, 1);
 }
 
 static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, orc_capture *cap,
-                     int nthreads) {
+                     int nthreads, int proj, int quantise) {
     if (!p || !in || !f || !st) return -1;
     if (p->simd_width != 8 && p->simd_width != 16) return -2;
     if (p->turbulence_model < 0 || p->turbulence_model > 2) return -3;
@@ -1306,6 +1312,8 @@ static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, o
     memset(s, 0, sizeof(S));
     s->cap = cap;
     s->nthreads = nthreads;
+    s->proj = proj;
+    s->noquant = !quantise;
     s->NII = p->nii; s->NJJ = p->njj; s->NKK = p->nkk; s->KADVMAX = p->kadvmax; s->VCF = p->vertcellsff;
     s->NJ2 = p->njj + 2; s->NKP = p->nkk + 2; s->W = p->simd_width; s->model = p->turbulence_model;
     s->DXK = p->dxk; s->DYK = p->dyk; s->relax = p->relax; s->relaxp = p->relaxp;
@@ -1455,7 +1463,22 @@ static int calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, o
     int IterationLoops = 1, ran = 0;
     float DeltaFinish = 0, DeltaFirst = 0;
     double DeltaMean = 0;
-    while (IterationLoops <= p->iter_max) {
+    while (IterationLoops <= p->iter_max && s->proj) {
+        /* orc_project: the linear part of an iteration (:723-949) */
+        divergence(s);
+        int IS = 1, JS = 1;
+        switch (IterationLoops % 4) {
+            case 0: IS = -1; JS = 1; break;
+            case 1: IS = -1; JS = -1; break;
+            case 2: IS = 1; JS = -1; break;
+            default: break;
+        }
+        (void)pressure(s, IS, JS);
+        correction(s);
+        ++ran;
+        IterationLoops++;
+    }
+    while (IterationLoops <= p->iter_max && !s->proj) {
         s->iter = IterationLoops;
         boundary_conditions(s);
         divergence(s);

@@ -76,6 +76,15 @@ typedef struct {
  * Returns 0 on success, negative on invalid input. */
 int orc_calculate(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st);
 
+/* The pressure-correction part of iter_max SIMPLE iterations after the
+ * prologue: divergence (PrognosticFlowfield.cs:723-758), pressure GS-TDMA in
+ * the sweep directions of each iteration (:770-901) and velocity correction
+ * (:904-949), without boundary copies, averages or momentum sweeps.  With
+ * quantise = 0 the corrected velocities are not rounded to 1/Frund, which
+ * makes the map from the initial UK/VK/WK to the result linear (the forward
+ * of the library's adjoint, gral_pf_forward_ckpt / gral_pf_vjp). */
+int orc_project(const orc_params *p, const orc_inputs *in, orc_fields *f, orc_stats *st, int quantise);
+
 /* Timing mode: the same solve with the reference's default multi-threaded
  * schedule (rows of each sweep dealt to nthreads threads updating the fields
  * in place, U and V at the same time).  Not deterministic; used only as the

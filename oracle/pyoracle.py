@@ -143,13 +143,15 @@ def _p(a, t=FP):
 
 
 def run(case, iterations=None, simd_width=None, keep_history=True, capture=None, threads=1, fp64=False,
-        lib_path=None):
+        lib_path=None, project=None):
     """Run PrognosticFlowfield.Calculate on ``case`` (a gral_amd.domain.Case)
     with the deterministic CPU oracle.  Returns a dict of output fields and stats.
     The input case is not modified.  threads > 1 selects the timing mode: the
     reference's default multi-threaded schedule, not deterministic (CPU
     baseline only, never a checker).  fp64 runs the FP64 shadow instead: the
-    same algorithm in double precision (fields come back as float64)."""
+    same algorithm in double precision (fields come back as float64).
+    project = 0 / 1 runs orc_project instead: the pressure-correction part of
+    each iteration only, without (0) or with (1) the quantisation."""
     if fp64:
         lib = load64()
         OrcParams, OrcInputs, OrcFields, OrcStats = _S64[:4]
@@ -201,6 +203,11 @@ def run(case, iterations=None, simd_width=None, keep_history=True, capture=None,
         cap = OrcCapture(comp, ci, cj, cit, *[_p(capbuf[n]) for n in ("a", "b", "c", "d", "x_old", "x_new")], 0,
                          _p(pieces))
         rc = lib.orc_calculate_capture(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st), C.byref(cap))
+    elif project is not None:
+        lib.orc_project.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcInputs), C.POINTER(OrcFields),
+                                    C.POINTER(OrcStats), C.c_int]
+        lib.orc_project.restype = C.c_int
+        rc = lib.orc_project(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st), int(project))
     elif threads > 1:
         rc = lib.orc_calculate_threads(C.byref(prm), C.byref(inp), C.byref(fld), C.byref(st), int(threads))
     else:
