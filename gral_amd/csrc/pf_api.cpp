@@ -903,17 +903,24 @@ static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipSt
     bool upload = true;
     int remaining = n;
     while (remaining > 0 && !act.empty()) {
+        // the launches run at act[0]'s iteration; the others stand at the same
+        // iteration or one a multiple of 100 earlier (admitted later), so all
+        // share the sweep directions and reach the stop test together
         const int it0 = act[0]->iter_loops;
         int chunk = std::min(remaining, kMaxChunk);
         if (check) chunk = std::min(chunk, 100 - ((it0 - 1) % 100));
         for (auto *h : act) {
-            if (h->iter_loops != it0) return bad(GRAL_PF_ESTATE, "iterate: situations of a batch must stand at the same iteration");
-            chunk = std::min(chunk, h->prm.iter_max - it0 + 1);
+            if ((it0 - h->iter_loops) % 100 != 0)
+                return bad(GRAL_PF_ESTATE, "iterate: situations of a batch must stand at iterations a multiple of 100 apart");
+            chunk = std::min(chunk, h->prm.iter_max - h->iter_loops + 1);
         }
         if (it0 < 1 || chunk < 1) return bad(GRAL_PF_ESTATE, "iterate: iteration counter out of range");
         if (upload) {
             hd.resize(act.size());
-            for (size_t q = 0; q < act.size(); ++q) hd[q] = act[q]->d;
+            for (size_t q = 0; q < act.size(); ++q) {
+                hd[q] = act[q]->d;
+                hd[q].iter_off = it0 - act[q]->iter_loops;
+            }
             if (hipMemcpyAsync(dd, hd.data(), sizeof(Dom) * hd.size(), hipMemcpyHostToDevice, s) != hipSuccess)
                 return bad(GRAL_PF_EDEVICE, "upload of the situation table failed");
             upload = false;
@@ -936,7 +943,8 @@ static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipSt
         bits.resize((size_t)chunk);
         for (auto *h : act) {
             int serr = 0;
-            if (hipMemcpy(bits.data(), h->d.resid + (it0 - 1), sizeof(unsigned) * chunk, hipMemcpyDeviceToHost) != hipSuccess ||
+            const int hit0 = h->iter_loops;              // this situation's first iteration of the chunk
+            if (hipMemcpy(bits.data(), h->d.resid + (hit0 - 1), sizeof(unsigned) * chunk, hipMemcpyDeviceToHost) != hipSuccess ||
                 hipMemcpy(&serr, h->d.sync, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
                 return bad(GRAL_PF_EDEVICE, "residual download failed");
             if (serr)
@@ -944,7 +952,7 @@ static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipSt
             for (int c = 0; c < chunk; ++c) {
                 float du;
                 std::memcpy(&du, &bits[c], 4);
-                int it = it0 + c;
+                int it = hit0 + c;
                 h->iters_run++;
                 R.cell_iters += h->active;
                 if (book_iteration(h, it, du, check)) {
@@ -1269,7 +1277,9 @@ extern "C" int gral_pf_profile_iterations(gral_pf_handle *h, int32_t n, gral_pf_
 // ------------------------------------------------------------------------
 struct gral_pf_batch {
     std::vector<gral_pf_handle *> hs;
+    std::vector<gral_pf_handle *> pending;   // added, waiting for an admission point (gral_pf_batch_step)
     Dom *dd = nullptr;
+    size_t dd_cap = 0;                       // Dom entries dd holds
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     gral_pf_timing timing{};
@@ -1284,7 +1294,7 @@ static int bfail(gral_pf_batch *b, int code, const std::string &msg) {
 extern "C" int gral_pf_batch_create(gral_pf_handle *const *handles, int32_t n, gral_pf_batch **out) {
     if (!out) return GRAL_PF_EINVAL;
     *out = nullptr;
-    if (!handles || n < 1) return GRAL_PF_EINVAL;
+    if (n < 0 || (n > 0 && !handles)) return GRAL_PF_EINVAL;
     for (int q = 0; q < n; ++q) {
         if (!handles[q] || !handles[q]->prepared) return GRAL_PF_ESTATE;
         for (int r = 0; r < q; ++r)
@@ -1294,13 +1304,108 @@ extern "C" int gral_pf_batch_create(gral_pf_handle *const *handles, int32_t n, g
     }
     gral_pf_batch *b = new (std::nothrow) gral_pf_batch();
     if (!b) return GRAL_PF_ENOMEM;
-    b->hs.assign(handles, handles + n);
+    if (n > 0) b->hs.assign(handles, handles + n);
+    const size_t cap = std::max<size_t>(1, (size_t)n);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
-        hipEventCreate(&b->ev1) != hipSuccess || hipMalloc((void **)&b->dd, sizeof(Dom) * (size_t)n) != hipSuccess) {
+        hipEventCreate(&b->ev1) != hipSuccess || hipMalloc((void **)&b->dd, sizeof(Dom) * cap) != hipSuccess) {
         gral_pf_batch_destroy(b);
         return GRAL_PF_EDEVICE;
     }
+    b->dd_cap = cap;
     *out = b;
+    return GRAL_PF_OK;
+}
+
+// ---- a batch fed while it runs ------------------------------------------
+// Situations join a running batch at an admission point: when no situation is
+// active, or when the active ones stand right after a multiple of 100
+// iterations (where the stop test is applied).  A newcomer starts at its
+// iteration 1, which then lies a multiple of 100 iterations behind the
+// others: the sweep directions (iteration mod 4) agree, and every situation
+// meets its stop test at the same launch.
+extern "C" int gral_pf_batch_add(gral_pf_batch *b, gral_pf_handle *h) {
+    if (!b) return GRAL_PF_EINVAL;
+    if (!h || !h->prepared) return bfail(b, GRAL_PF_ESTATE, "batch add: the situation must be prepared");
+    if (h->iter_loops != 1 || h->done) return bfail(b, GRAL_PF_ESTATE, "batch add: the situation must stand at its first iteration");
+    for (auto *x : b->hs)
+        if (x == h) return bfail(b, GRAL_PF_EINVAL, "batch add: the situation is already in the batch");
+    for (auto *x : b->pending)
+        if (x == h) return bfail(b, GRAL_PF_EINVAL, "batch add: the situation is already queued");
+    const gral_pf_handle *ref = !b->hs.empty() ? b->hs[0] : (!b->pending.empty() ? b->pending[0] : nullptr);
+    if (ref && (ref->d.model != h->d.model || ref->d.tiled != h->d.tiled))
+        return bfail(b, GRAL_PF_EINVAL, "batch add: situations of a batch must share the turbulence model and sweep kind");
+    const size_t need = b->hs.size() + b->pending.size() + 1;
+    if (need > b->dd_cap) {
+        const size_t cap = std::max(need, 2 * b->dd_cap);
+        Dom *nd = nullptr;
+        if (hipMalloc((void **)&nd, sizeof(Dom) * cap) != hipSuccess) return bfail(b, GRAL_PF_ENOMEM, "batch add: hipMalloc failed");
+        if (hipStreamSynchronize(b->stream) != hipSuccess) return bfail(b, GRAL_PF_EDEVICE, "batch add: stream failed");
+        (void)hipFree(b->dd);
+        b->dd = nd;
+        b->dd_cap = cap;
+    }
+    b->pending.push_back(h);
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_batch_step(gral_pf_batch *b, int32_t *n_done, gral_pf_handle **done, gral_pf_stats *st,
+                                  int32_t cap) {
+    if (!b || !n_done) return GRAL_PF_EINVAL;
+    *n_done = 0;
+    if (cap < 0 || (cap > 0 && !done)) return bfail(b, GRAL_PF_EINVAL, "batch step: bad output arrays");
+    std::vector<gral_pf_handle *> act;
+    for (auto *h : b->hs)
+        if (!h->done) act.push_back(h);
+    bool admit = act.empty();
+    if (!admit) {
+        admit = true;
+        for (auto *h : act) admit = admit && (h->iter_loops % 100 == 1);
+    }
+    if (admit && !b->pending.empty()) {
+        for (auto *h : b->pending) {
+            if (!h->prepared || h->iter_loops != 1) return bfail(b, GRAL_PF_ESTATE, "batch step: a queued situation was changed");
+            b->hs.push_back(h);
+            act.push_back(h);
+        }
+        b->pending.clear();
+    }
+    if (act.empty()) return GRAL_PF_OK;
+    // run to the next multiple of 100 of the leading situation, where
+    // situations stop and newcomers may join
+    int lead = act[0]->iter_loops;
+    for (auto *h : act) lead = std::max(lead, h->iter_loops);
+    const int n = 100 - ((lead - 1) % 100);
+    RunResult R;
+    std::vector<gral_pf_handle *> order(act);
+    std::stable_sort(order.begin(), order.end(),
+                     [](const gral_pf_handle *x, const gral_pf_handle *y) { return x->iter_loops > y->iter_loops; });
+    run_lockstep(order, b->dd, b->stream, b->ev0, b->ev1, n, true, nullptr, R);
+    if (R.code) return bfail(b, R.code, R.err);
+    b->timing = gral_pf_timing{};
+    b->timing.ms_total = R.ms;
+    b->timing.launches = R.launches;
+    for (auto *h : act) b->timing.active_cells += h->active;
+    b->timing.cell_iterations = R.cell_iters;
+    b->timing.situations = (int32_t)act.size();
+    // finished situations: epilogue, then out of the batch
+    std::vector<gral_pf_handle *> keep;
+    for (auto *h : b->hs) {
+        if (!h->done) { keep.push_back(h); continue; }
+        if (*n_done >= cap) return bfail(b, GRAL_PF_EINVAL, "batch step: more finished situations than output room");
+        int rc = gral_pf_finish(h, st ? st + *n_done : nullptr);
+        if (rc) return bfail(b, rc, h->err);
+        done[(*n_done)++] = h;
+    }
+    b->hs.swap(keep);
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_batch_size(const gral_pf_batch *b, int32_t *active, int32_t *queued) {
+    if (!b) return GRAL_PF_EINVAL;
+    int a = 0;
+    for (auto *h : b->hs) a += h->done ? 0 : 1;
+    if (active) *active = a;
+    if (queued) *queued = (int32_t)b->pending.size();
     return GRAL_PF_OK;
 }
 

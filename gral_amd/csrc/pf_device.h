@@ -51,6 +51,8 @@ struct DomT {
     P<const float> ptab[2], ttab[2]; // pressure PIMP/TERMP per column for IS=+1 / IS=-1
     P<const int4> meta;              // per column: KK, VI, KSTART | adv<<16 | vegp<<17, KKm_i | KKm_j<<16
     int noquant;                     // 1: the velocity correction skips the FP64 quantisation (adjoint forward)
+    int iter_off;                    // this situation's iteration = the launch's iteration - iter_off (a
+                                     // multiple of 100: situations admitted to a running batch later)
 };
 using Dom = DomT<GenericPtr>;        // host side, and the descriptor array handed to the kernels
 using DomG = DomT<GlobalPtr>;        // the kernels' view of the same bytes

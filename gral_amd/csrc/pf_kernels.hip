@@ -255,7 +255,8 @@ __global__ __launch_bounds__(WAVE) void k_pressure(const Dom *__restrict__ ds, i
         float other = __shfl_xor(rmax, off);
         rmax = other > rmax ? other : rmax;
     }
-    if (tid == 0 && rmax > 0.0f && slot >= 0 && slot < d.resid_cap) atomicMax((unsigned *)&d.resid[slot], __float_as_uint(rmax));
+    const int rs = slot - d.iter_off;                 // this situation's residual slot
+    if (tid == 0 && rmax > 0.0f && slot >= 0 && rs >= 0 && rs < d.resid_cap) atomicMax((unsigned *)&d.resid[rs], __float_as_uint(rmax));
 }
 
 // ------------------------------------------------------------------------
@@ -2093,7 +2094,8 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
         float other = __shfl_xor(rmax, off);
         rmax = other > rmax ? other : rmax;
     }
-    if (lane == 0 && rmax > 0.0f && slot >= 0 && slot < d.resid_cap) atomicMax((unsigned *)&d.resid[slot], __float_as_uint(rmax));
+    const int rs = slot - d.iter_off;                 // this situation's residual slot
+    if (lane == 0 && rmax > 0.0f && slot >= 0 && rs >= 0 && rs < d.resid_cap) atomicMax((unsigned *)&d.resid[rs], __float_as_uint(rmax));
 }
 
 static inline int ndiag(int oa, int ob, int ia, int ib) { return (ob - oa) + (ib - ia) + 1; }

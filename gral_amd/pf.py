@@ -27,6 +27,7 @@ EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral
            "gral_pf_diagnostic",
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
+           "gral_pf_batch_add", "gral_pf_batch_step", "gral_pf_batch_size",
            "gral_pf_gff_payload", "gral_pf_gff_write", "gral_pf_zip_write", "gral_pf_wind_at_particles",
            "gral_pf_subdomains", "gral_pf_reference_point", "gral_pf_flat_init", "gral_pf_terrain_init",
            "gral_pf_terrain_init_topo", "gral_pf_setup_last_error",
@@ -120,6 +121,9 @@ def load(path: str = LIB_PATH):
     lib.gral_pf_batch_solve.argtypes = [H, C.POINTER(Stats)]
     lib.gral_pf_batch_profile_iterations.argtypes = [H, C.c_int32, C.POINTER(FamilyTime), C.c_int32]
     lib.gral_pf_batch_last_timing.argtypes = [H, C.POINTER(Timing)]
+    lib.gral_pf_batch_add.argtypes = [H, H]
+    lib.gral_pf_batch_step.argtypes = [H, C.POINTER(C.c_int32), C.POINTER(H), C.POINTER(Stats), C.c_int32]
+    lib.gral_pf_batch_size.argtypes = [H, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.gral_pf_gff_payload.argtypes = [H, C.c_int32, C.POINTER(GffMeta), C.c_void_p, C.c_size_t,
                                         C.POINTER(C.c_size_t)]
     lib.gral_pf_gff_write.argtypes = [H, C.c_int32, C.POINTER(GffMeta), C.c_char_p]
@@ -344,12 +348,15 @@ class PrognosticFlowfield:
 class Batch:
     """Several prepared situations advanced in lock step by the same launches
     (gral_pf_batch_*).  Each situation keeps its own convergence test and
-    leaves the batch when it stops; results equal plain solves bit for bit."""
+    leaves the batch when it stops; results equal plain solves bit for bit.
+    ``add``/``step`` feed a running batch (solvers admitted at multiples of
+    100 iterations, finished ones handed back)."""
 
-    def __init__(self, solvers):
+    def __init__(self, solvers=()):
         self.lib = load()
         self.solvers = list(solvers)
-        arr = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        self._by_handle = {s.h.value: s for s in self.solvers}
+        arr = (C.c_void_p * max(1, len(self.solvers)))(*[s.h.value for s in self.solvers])
         b = C.c_void_p()
         rc = self.lib.gral_pf_batch_create(arr, len(self.solvers), C.byref(b))
         if rc != 0:
@@ -370,6 +377,30 @@ class Batch:
         st = (Stats * len(self.solvers))()
         self._check(self.lib.gral_pf_batch_solve(self.b, st))
         return [_stats(x) for x in st]
+
+    def add(self, solver):
+        """Queue a prepared solver; it joins at the next admission point."""
+        self._check(self.lib.gral_pf_batch_add(self.b, solver.h))
+        self._by_handle[solver.h.value] = solver
+
+    def size(self):
+        a, q = C.c_int32(0), C.c_int32(0)
+        self._check(self.lib.gral_pf_batch_size(self.b, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def step(self):
+        """Advance to the next multiple of 100 iterations; returns [(solver, stats)]
+        of the situations that stopped there (already finished)."""
+        cap = max(1, len(self._by_handle))
+        done = (C.c_void_p * cap)()
+        st = (Stats * cap)()
+        n = C.c_int32(0)
+        self._check(self.lib.gral_pf_batch_step(self.b, C.byref(n), done, st, cap))
+        out = []
+        for q in range(n.value):
+            sv = self._by_handle.pop(done[q])
+            out.append((sv, _stats(st[q])))
+        return out
 
     def timing(self):
         t = Timing()
@@ -554,4 +585,49 @@ def run_batch(cases, iterations):
     finally:
         for s in solvers:
             s.close()
+    return out
+
+
+def solve_stream(cases, capacity, on_done=None, ahead=1):
+    """Solve a stream of situations with one batch fed while it runs: at most
+    ``capacity`` situations hold device memory at a time (active or waiting
+    for admission); up to ``ahead`` more are set up and prepared on a host
+    thread while the GPU iterates.  ``cases`` yields Case objects or
+    callables returning one.  ``on_done(case, solver, stats)`` is called for
+    each finished situation before its solver is closed (download the fields
+    there); the return value is the list of stats in completion order."""
+    import collections
+    import concurrent.futures as cf
+
+    def build(item):
+        case = item() if callable(item) else item
+        sv = PrognosticFlowfield(case)
+        sv.prepare()
+        return sv
+
+    src = iter(cases)
+    more = True
+    futs = collections.deque()
+    out = []
+    with cf.ThreadPoolExecutor(max_workers=1) as ex, Batch() as batch:
+        while True:
+            active, queued = batch.size()
+            while more and active + queued + len(futs) < capacity + ahead:
+                try:
+                    futs.append(ex.submit(build, next(src)))
+                except StopIteration:
+                    more = False
+            # admit prepared situations while room is left (wait for one when idle)
+            while futs and active + queued < capacity and (futs[0].done() or active + queued == 0):
+                batch.add(futs.popleft().result())
+                queued += 1
+            if active + queued == 0:
+                if not futs and not more:
+                    break
+                continue
+            for sv, st in batch.step():
+                if on_done is not None:
+                    on_done(sv.case, sv, st)
+                out.append(st)
+                sv.close()
     return out

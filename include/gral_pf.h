@@ -399,7 +399,8 @@ int gral_pf_wind_at_particles(gral_pf_handle *h, int32_t topo, int32_t ikooagral
  * each handle as after gral_pf_iterate. */
 typedef struct gral_pf_batch gral_pf_batch;
 
-/* handles: n prepared situations (gral_pf_prepare), all at the same iteration. */
+/* handles: n prepared situations (gral_pf_prepare), all at the same iteration
+ * (n = 0: an empty batch, fed with gral_pf_batch_add). */
 int gral_pf_batch_create(gral_pf_handle *const *handles, int32_t n, gral_pf_batch **out);
 
 /* Up to n lock-step SIMPLE iterations; stats (may be NULL) receives one entry
@@ -409,6 +410,25 @@ int gral_pf_batch_iterate(gral_pf_batch *b, int32_t n, int32_t check_convergence
 /* Iterates every situation to its own stop (convergence test or iter_max),
  * then runs gral_pf_finish on each; stats: one entry per situation. */
 int gral_pf_batch_solve(gral_pf_batch *b, gral_pf_stats *stats);
+
+/* A batch fed while it runs (the 256-situation workload: more situations than
+ * fit in device memory at once).  gral_pf_batch_add queues a prepared
+ * situation at its first iteration; gral_pf_batch_step admits the queued
+ * ones when no situation is active or when the active ones stand right after
+ * a multiple of 100 iterations (the newcomers then run a multiple of 100
+ * iterations behind: same sweep directions, same stop-test launches), then
+ * advances every active situation to the next multiple of 100 of the leading
+ * one with the reference's stop test (PrognosticFlowfield.cs:1149-1166), runs
+ * gral_pf_finish on those that stopped (stop test or iter_max) and removes
+ * them from the batch: *n_done of them are returned in done[] with their
+ * statistics in st[] (room for cap entries; st may be NULL).  A call with
+ * nothing active and nothing queued returns *n_done = 0.  The caller owns the
+ * handles: a returned handle can be downloaded and destroyed, freeing its
+ * device memory for the next situations.  Each situation's results equal its
+ * plain solve bit for bit. */
+int gral_pf_batch_add(gral_pf_batch *b, gral_pf_handle *h);
+int gral_pf_batch_step(gral_pf_batch *b, int32_t *n_done, gral_pf_handle **done, gral_pf_stats *st, int32_t cap);
+int gral_pf_batch_size(const gral_pf_batch *b, int32_t *active, int32_t *queued);
 
 /* As gral_pf_profile_iterations, over the whole batch. */
 int gral_pf_batch_profile_iterations(gral_pf_batch *b, int32_t n, gral_pf_family_time *out, int32_t nfam);
