@@ -208,6 +208,117 @@ def batched_workload(k, reps_log=sys.stderr):
             "equal_residual": same}
 
 
+def diagnostic_timing(n, reps=3):
+    """The mass-conservation post-pass (DiagnosticFlowfield: 10 pressure
+    iterations over the full column height, then the W mass step) on the city
+    block after 2 prognostic iterations, wall time of gral_pf_diagnostic."""
+    from gral_amd import domain, pf
+    c = domain.config2(direction=270.0, iterations=4, n=n)
+    ts = []
+    with pf.PrognosticFlowfield(c) as sv:
+        sv.prepare(iterations=4)
+        sv.iterate(2)
+        sv.diagnostic(flow_field_level=2)               # warm-up (device buffers, host copies)
+        for _ in range(reps):
+            t = time.perf_counter()
+            sv.diagnostic(flow_field_level=2)
+            ts.append(time.perf_counter() - t)
+    return {"ms": sorted(ts)[reps // 2] * 1e3, "reps": reps,
+            "workload": f"city block {n}x{n}, 270 deg, after 2 iterations, flow_field_level 2 (10 pressure iterations)"}
+
+
+def batch_sweep(sizes, n, steps=6, warmup=2, log=sys.stderr):
+    """Throughput against batch size on the city block: S situations (the 8
+    directions repeated) advanced in lock step; per size the device time per
+    iteration, cell-iterations/s and the W sweep's share of the HBM roofline
+    (algorithmic bytes over its event-timed launch)."""
+    from gral_amd import domain, pf
+    out = []
+    for S in sizes:
+        cases = [domain.config2(direction=45.0 * (q % 8), iterations=warmup + steps + 4, n=n) for q in range(S)]
+        solvers = [pf.PrognosticFlowfield(c) for c in cases]
+        for sv in solvers:
+            sv.prepare(iterations=warmup + steps + 4)
+        with pf.Batch(solvers) as b:
+            b.iterate(warmup)
+            b.iterate(steps)
+            tm = b.timing()
+            prof = b.profile_families(1)
+        for sv in solvers:
+            sv.close()
+        w = prof["w"]
+        wfrac = ALG_BYTES["w"] * w["cells"] / (w["ms"] / w["launches"] / 1e3) / 1e9 / HBM_PEAK_GBS / w["launches"]
+        row = {"situations": S, "ms_per_iter": tm["ms_total"] / steps,
+               "cell_iter_per_s": tm["cell_iterations"] / (tm["ms_total"] / 1e3), "w_sweep_frac": wfrac,
+               "w_sweep_ms": w["ms"] / w["launches"]}
+        print(f"[batch-sweep] {row}", file=log, flush=True)
+        out.append(row)
+    return out
+
+
+def stream_workload(k, capacity, plain, log=sys.stderr):
+    """The first k situations of the batched workload generator, each set up on
+    the GPU and solved to the reference's stop test (100-500 iterations), with
+    one batch fed while it runs (at most ``capacity`` situations in the batch,
+    the next ones set up on host threads meanwhile, largest first), against
+    ``plain`` of them solved one after another with the plain solver."""
+    from gral_amd import domain, pf
+    sits = domain.config3_situations()[:k]
+    cells = {}
+    mass = []
+
+    def make(s):
+        def f():
+            c = domain.config3(s)
+            cells[c.name] = c.active_cells()
+            return c
+        return f
+
+    done = [0]
+    ci = [0]
+
+    def on_done(case, sv, st):
+        done[0] += 1
+        ci[0] += cells[case.name] * st["iterations_run"]
+        mass.append(st["mass_div"] / st["mass_flux"] if st["mass_flux"] > 0 else 0.0)
+        if done[0] % 16 == 0:
+            print(f"[stream] {done[0]}/{k} situations done ({time.perf_counter() - t0:.0f} s)", file=log, flush=True)
+
+    t0 = time.perf_counter()
+    order = sorted(sits, key=lambda s: -s["n"] * s["n"] * s["vertcellsff"])      # longest first
+    stats = pf.solve_stream([make(s) for s in order], capacity=capacity, on_done=on_done)
+    t_stream = time.perf_counter() - t0
+    loops = [st["iteration_loops"] for st in stats]
+    runs = [st["iterations_run"] for st in stats]
+    tot_cells = sum(cells.values())
+    out = {"situations": k, "capacity": capacity, "wall_s": t_stream, "per_hour": k / t_stream * 3600.0,
+           "stopped_before_limit": int(sum(1 for x in loops if x < 500)), "iteration_loops_min": min(loops),
+           "iteration_loops_max": max(loops), "iterations_run_total": int(sum(runs)),
+           "active_cells_total": int(tot_cells), "cell_iterations": int(ci[0]),
+           "cell_iter_per_s_wall": ci[0] / t_stream,
+           "mass_imbalance_median": float(np.median(mass)) if mass else None}
+    if plain:
+        # plain solves of a sample, one after another (set-up, prologue, solve,
+        # download); extrapolated to the whole set by cell-iterations
+        t1 = time.perf_counter()
+        ci_p = 0
+        for q, s in enumerate(sits[:plain]):
+            c = domain.config3(s)
+            with pf.PrognosticFlowfield(c) as sv:
+                st = sv.solve()
+                sv.download()
+            ci_p += c.active_cells() * st["iterations_run"]
+            print(f"[stream] plain {q + 1}/{plain} ({time.perf_counter() - t1:.1f} s)", file=log, flush=True)
+        t_plain = time.perf_counter() - t1
+        t_plain_all = t_plain / ci_p * ci[0]
+        out["plain_sample"] = plain
+        out["plain_sample_s"] = t_plain
+        out["plain_cell_iter_per_s_wall"] = ci_p / t_plain
+        out["plain_per_hour"] = k / t_plain_all * 3600.0
+        out["batched_over_plain"] = t_plain_all / t_stream
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +337,16 @@ def main():
                     help="also time the flow-field file writer (GPU pack + zip) on the first situation")
     ap.add_argument("--batched-workload", type=int, default=0, metavar="K",
                     help="also solve the first K situations of the batched workload, batched and plain")
+    ap.add_argument("--diagnostic", action="store_true",
+                    help="also time the diagnostic post-pass (gral_pf_diagnostic) on the 270 deg city")
+    ap.add_argument("--batch-sweep", type=str, default="", metavar="S1,S2,...",
+                    help="also time lock-step batches of these sizes on the city block")
+    ap.add_argument("--stream", type=int, default=0, metavar="K",
+                    help="also solve the first K situations of the batched workload with one batch fed while it "
+                         "runs (see --capacity), to the stop test")
+    ap.add_argument("--capacity", type=int, default=32, help="situations on the device at a time in --stream")
+    ap.add_argument("--stream-plain", type=int, default=4, metavar="P",
+                    help="plain solves of the first P situations beside --stream")
     ap.add_argument("--turbulence-model", type=int, default=1, choices=(1, 2),
                     help="1: mixing length (the benchmark's model), 2: k-eps")
     ap.add_argument("--particle-wind", type=int, default=0, metavar="N",
@@ -329,15 +450,20 @@ def main():
             t = time.perf_counter(); pyoracle.run(c0, iterations=iters, keep_history=False, threads=threads)
             return time.perf_counter() - t - t_pro
 
+        # the host share of one GPU on the box is 16 cores (os.cpu_count() may show the whole machine)
         threads = max(1, min(16, os.cpu_count() or 1))
         n_par = 4 * args.cpu_iterations
-        t_par = loop_time(n_par, threads)
+        reps = sorted(loop_time(n_par, threads) for _ in range(5))
+        t_par = reps[2]
         t_one = loop_time(args.cpu_iterations, 1)
         cpu = {"value": c0.active_cells() * n_par / t_par, "unit": "cell-iter/s", "cores": threads, "kind": "port",
-               "sample": f"{n_par} SIMPLE iterations of the 270 deg situation, the C restatement run with the "
-                         f"reference's default multi-threaded schedule on {threads} threads (rows of each sweep "
-                         f"split across threads, U and V concurrent; scalar code, no SIMD), prologue excluded "
-                         f"({t_par:.1f} s)",
+               "sample": f"{n_par} SIMPLE iterations of the 270 deg situation, median of 5 runs "
+                         f"({', '.join(f'{x:.2f}' for x in reps)} s), the C restatement run with the reference's "
+                         f"default multi-threaded schedule on {threads} threads, the host share of one GPU (rows of "
+                         f"each sweep split across threads, U and V concurrent), prologue excluded.  Scalar C: the "
+                         f"reference vectorises the coefficient assembly with Vector<float> (8 lanes), which this "
+                         f"restatement does not, so the reference's own CPU path would be faster by up to that "
+                         f"factor on the vectorised part",
                "single_core": {"value": c0.active_cells() * args.cpu_iterations / t_one, "cores": 1,
                                "sample": f"{args.cpu_iterations} iterations, deterministic single-thread restatement "
                                          f"({t_one:.1f} s)"}}
@@ -361,6 +487,12 @@ def main():
         line["batched_workload"] = batched_workload(args.batched_workload)
     if args.situations_hour:
         line["situations_per_hour"] = situations_per_hour(args.n, directions)
+    if args.diagnostic:
+        line["diagnostic_post_pass"] = diagnostic_timing(args.n)
+    if args.batch_sweep:
+        line["batch_sweep"] = batch_sweep([int(x) for x in args.batch_sweep.split(",")], args.n)
+    if args.stream:
+        line["stream_workload"] = stream_workload(args.stream, args.capacity, args.stream_plain)
     print(json.dumps(line))
 
 

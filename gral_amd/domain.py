@@ -461,9 +461,10 @@ def gramm_hills(extent_x: float, extent_y: float, ddx: float = 50.0, nz: int = 1
 
 
 def config5(direction: float = 270.0, speed: float = 3.0, stability: int = 4, n: int = 1000, dx: float = 4.0,
-            iterations: int = 4, seed: int = 20260005, vertcellsff: int = 10) -> Case:
+            iterations: int = 4, seed: int = 20260005, vertcellsff: int = 30) -> Case:
     """The terrain configuration without its adjoint: an n x n GRAL grid (dx =
-    4 m, 30-40 stretched levels) over 12 seeded Gaussian hills of a GRAMM
+    4 m, 30 prognostic layers above the ground on 34-40 stretched levels, so
+    VerticalIndex = min(NKK-1, KKART+30)) over 12 seeded Gaussian hills of a GRAMM
     field, random cuboid buildings at 15 % plan density, vegetation patches
     with leaf-area density 0.5-2 m^2/m^3, k-eps; set up on the GPU as GRAL's
     terrain path does (``pf.subdomains``, ``pf.terrain_init``,

@@ -588,14 +588,19 @@ def run_batch(cases, iterations):
     return out
 
 
-def solve_stream(cases, capacity, on_done=None, ahead=1):
+def solve_stream(cases, capacity, on_done=None, ahead=None, workers=3):
     """Solve a stream of situations with one batch fed while it runs: at most
-    ``capacity`` situations hold device memory at a time (active or waiting
-    for admission); up to ``ahead`` more are set up and prepared on a host
-    thread while the GPU iterates.  ``cases`` yields Case objects or
-    callables returning one.  ``on_done(case, solver, stats)`` is called for
-    each finished situation before its solver is closed (download the fields
-    there); the return value is the list of stats in completion order."""
+    ``capacity`` situations are in the batch at a time (active or waiting
+    for admission); up to ``ahead`` more (default capacity // 2) are set up
+    and prepared by ``workers`` host threads while the GPU iterates, so that
+    retired situations are replaced at the next admission point.  The first
+    step waits until the batch is full.  ``cases`` yields Case objects or
+    callables returning one, in the order they should be solved (longest
+    first keeps the situations of one batch alike in size, so that no launch
+    waits long for one large situation).  ``on_done(case, solver, stats)`` is
+    called for each finished situation before its solver is closed (download
+    the fields there); the return value is the list of stats in completion
+    order."""
     import collections
     import concurrent.futures as cf
 
@@ -605,11 +610,14 @@ def solve_stream(cases, capacity, on_done=None, ahead=1):
         sv.prepare()
         return sv
 
+    if ahead is None:
+        ahead = max(1, capacity // 2)
     src = iter(cases)
     more = True
     futs = collections.deque()
     out = []
-    with cf.ThreadPoolExecutor(max_workers=1) as ex, Batch() as batch:
+    first = True
+    with cf.ThreadPoolExecutor(max_workers=max(1, workers)) as ex, Batch() as batch:
         while True:
             active, queued = batch.size()
             while more and active + queued + len(futs) < capacity + ahead:
@@ -617,10 +625,12 @@ def solve_stream(cases, capacity, on_done=None, ahead=1):
                     futs.append(ex.submit(build, next(src)))
                 except StopIteration:
                     more = False
-            # admit prepared situations while room is left (wait for one when idle)
-            while futs and active + queued < capacity and (futs[0].done() or active + queued == 0):
+            # admit prepared situations while room is left: at the start until
+            # the batch is full, later those that are ready (one when idle)
+            while futs and active + queued < capacity and (futs[0].done() or first or active + queued == 0):
                 batch.add(futs.popleft().result())
                 queued += 1
+            first = False
             if active + queued == 0:
                 if not futs and not more:
                     break

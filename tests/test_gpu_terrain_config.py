@@ -1,9 +1,9 @@
 """The terrain configuration at its full grid size (1000 x 1000 cells, dx = 4
-m, 34 levels over 12 Gaussian hills, buildings at 15 %, vegetation, k-eps),
+m, 30 prognostic layers over 12 Gaussian hills, buildings at 15 %, vegetation, k-eps),
 without its adjoint: four situations set up on the GPU and advanced in lock
-step.  The oracle cannot follow at this size; the checks are the properties
-that do not depend on it: the batch equals plain solves of the same
-situations bit for bit, every field stays finite, and the strip kernels run."""
+step.  The checks here are the properties that need no oracle run: the batch equals plain solves of the same
+situations bit for bit, every field stays finite, and the strip kernels run.
+(One situation is checked against the oracle in tests/test_gpu_configs.py.)"""
 import time
 
 import numpy as np
