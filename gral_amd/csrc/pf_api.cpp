@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstdlib>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -135,7 +136,41 @@ static double *scratch_of(int dev) {
     return g_scratch[dev];
 }
 
+// A captured chunk of iterations (hipGraph) for replay: the launches of
+// `chunk` iterations of the situations in `set`, captured at an iteration
+// congruent mod 4 (the sweep directions) to the chunks that replay it.  Its
+// kernels address the residual history by slot within the chunk; each
+// situation's Dom::iter_off, uploaded before every replay, maps that slot to
+// the situation's own iteration.
+struct GraphCache {
+    std::vector<unsigned long long> set;   // generation numbers of the situations (prepare assigns them)
+    const Dom *dd = nullptr;               // the situation table the kernels read
+    int chunk = 0, it_mod4 = -1;
+    long long launches = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void clear() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        set.clear();
+        dd = nullptr;
+        chunk = 0;
+        it_mod4 = -1;
+        launches = 0;
+    }
+    ~GraphCache() { clear(); }
+};
+// GRAL_PF_GRAPH=1 replays captured iteration chunks instead of launching each
+// kernel (the first chunk of a set of situations runs eagerly and is captured)
+static bool graphs_enabled() {
+    const char *e = std::getenv("GRAL_PF_GRAPH");
+    return e && e[0] == '1';
+}
+
 struct gral_pf_handle {
+    unsigned long long gen = 0;     // generation: set by every prepare, unique in the process
     // grid
     int device = -1;                // the HIP device current at create
     int nii = 0, njj = 0, nkk = 0, kadvmax = 0, vcf = 0, nkp = 0, nkh = 0;
@@ -160,6 +195,7 @@ struct gral_pf_handle {
     float *adj = nullptr;           // ub vb wb pb qb mb db, checkpoint uk vk wk, ptab[2] ttab[2]: 15 fields
     int *adj_src = nullptr;         // stale-top source column per column, for IS = +1 and -1
     int adj_first = 0, adj_n = 0;   // iterations of the last forward
+    GraphCache *graphs = nullptr;   // captured iteration chunks (GRAL_PF_GRAPH=1)
     size_t dev_cap = 0;             // optional cap (GRAL_PF_DEVICE_MB), 0 = none
     std::vector<float> uk, vk, wk, uks, vks, wks, dpm, dpmnew, turb, tdiss;
     std::vector<int> vi;
@@ -352,6 +388,7 @@ static void free_device(gral_pf_handle *h) {
     h->dd = nullptr;
     h->d = Dom{};
     h->mass_buf = nullptr;
+    if (h->graphs) h->graphs->clear();
     h->adj = nullptr;
     h->adj_src = nullptr;
     h->adj_n = 0;
@@ -776,6 +813,8 @@ extern "C" int gral_pf_prepare(gral_pf_handle *h, const gral_pf_params *p) {
             size_t c = h->i2(i, j);
             if (h->adv[c] == 1) h->active += h->vi[c] - h->kk[c];
         }
+    static std::atomic<unsigned long long> next_gen{1};
+    h->gen = next_gen++;
     h->prepared = true;
     h->gff_ok = false;
     h->done = false;
@@ -891,9 +930,11 @@ struct RunResult {
     int code = 0;
 };
 
+
 static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
-                         int n, bool check, Probe *probe, RunResult &R) {
+                         int n, bool check, Probe *probe, RunResult &R, GraphCache *gc = nullptr) {
     auto bad = [&](int code, const std::string &m) { R.code = code; R.err = m; };
+    if (probe || !graphs_enabled()) gc = nullptr;
     std::vector<gral_pf_handle *> act;
     for (auto *h : hs)
         if (!h->done) act.push_back(h);
@@ -915,22 +956,60 @@ static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipSt
             chunk = std::min(chunk, h->prm.iter_max - h->iter_loops + 1);
         }
         if (it0 < 1 || chunk < 1) return bad(GRAL_PF_ESTATE, "iterate: iteration counter out of range");
-        if (upload) {
+        // slots: the launch's residual slot minus iter_off is each situation's
+        // own iteration - 1; with graphs the slot counts from the chunk's start
+        if (upload || gc) {
             hd.resize(act.size());
             for (size_t q = 0; q < act.size(); ++q) {
                 hd[q] = act[q]->d;
-                hd[q].iter_off = it0 - act[q]->iter_loops;
+                hd[q].iter_off = gc ? 1 - act[q]->iter_loops : it0 - act[q]->iter_loops;
             }
             if (hipMemcpyAsync(dd, hd.data(), sizeof(Dom) * hd.size(), hipMemcpyHostToDevice, s) != hipSuccess)
                 return bad(GRAL_PF_EDEVICE, "upload of the situation table failed");
             upload = false;
         }
-        for (int c = 0; c < chunk; ++c) {
-            int it = it0 + c;
-            int e = pf_launch_iteration(hd.data(), (int)hd.size(), dd, it, it - 1, s, &R.launches, probe);
-            if (e)
-                return bad(GRAL_PF_EDEVICE, std::string("kernel launch failed at iteration ") + std::to_string(it) + ": " +
-                                                hipGetErrorString((hipError_t)e));
+        auto launch_chunk = [&](long long *launches) -> int {
+            for (int c = 0; c < chunk; ++c) {
+                const int it = it0 + c;
+                int e = pf_launch_iteration(hd.data(), (int)hd.size(), dd, it, gc ? c : it - 1, s, launches, probe);
+                if (e) {
+                    bad(GRAL_PF_EDEVICE, std::string("kernel launch failed at iteration ") + std::to_string(it) + ": " +
+                                             hipGetErrorString((hipError_t)e));
+                    return e;
+                }
+            }
+            return 0;
+        };
+        std::vector<unsigned long long> key;
+        if (gc)
+            for (auto *h : act) key.push_back(h->gen);
+        if (gc && gc->exec && gc->set == key && gc->dd == dd && gc->chunk == chunk && gc->it_mod4 == it0 % 4) {
+            if (hipGraphLaunch(gc->exec, s) != hipSuccess) return bad(GRAL_PF_EDEVICE, "hipGraphLaunch failed");
+            R.launches += gc->launches;
+        } else {
+            if (launch_chunk(&R.launches)) return;
+            if (gc) {
+                // capture the same chunk for the next ones (capturing does not run it)
+                gc->clear();
+                long long nl = 0;
+                if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                    return bad(GRAL_PF_EDEVICE, "hipStreamBeginCapture failed");
+                const int e = launch_chunk(&nl);
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(s, &g);
+                if (e) return;
+                if (ec != hipSuccess || !g) return bad(GRAL_PF_EDEVICE, "hipStreamEndCapture failed");
+                gc->graph = g;
+                if (hipGraphInstantiate(&gc->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+                    gc->clear();
+                    return bad(GRAL_PF_EDEVICE, "hipGraphInstantiate failed");
+                }
+                gc->set = key;
+                gc->dd = dd;
+                gc->chunk = chunk;
+                gc->it_mod4 = it0 % 4;
+                gc->launches = nl;
+            }
         }
         remaining -= chunk;
         if (hipStreamSynchronize(s) != hipSuccess) return bad(GRAL_PF_EDEVICE, "stream synchronisation failed");
@@ -982,7 +1061,8 @@ extern "C" int gral_pf_iterate(gral_pf_handle *h, int32_t n, int32_t check, gral
     if (n < 0) return fail(h, GRAL_PF_EINVAL, "iterate: n < 0");
     h->timing = gral_pf_timing{};
     RunResult R;
-    run_lockstep({h}, h->dd, h->stream, h->ev0, h->ev1, n, check != 0, nullptr, R);
+    if (!h->graphs) h->graphs = new (std::nothrow) GraphCache();
+    run_lockstep({h}, h->dd, h->stream, h->ev0, h->ev1, n, check != 0, nullptr, R, h->graphs);
     if (R.code) return fail(h, R.code, R.err);
     h->timing.ms_total = R.ms;
     h->timing.launches = R.launches;
@@ -1278,6 +1358,7 @@ extern "C" int gral_pf_profile_iterations(gral_pf_handle *h, int32_t n, gral_pf_
 struct gral_pf_batch {
     std::vector<gral_pf_handle *> hs;
     std::vector<gral_pf_handle *> pending;   // added, waiting for an admission point (gral_pf_batch_step)
+    GraphCache graphs;                       // captured iteration chunks (GRAL_PF_GRAPH=1)
     Dom *dd = nullptr;
     size_t dd_cap = 0;                       // Dom entries dd holds
     hipStream_t stream = nullptr;
@@ -1340,6 +1421,7 @@ extern "C" int gral_pf_batch_add(gral_pf_batch *b, gral_pf_handle *h) {
         Dom *nd = nullptr;
         if (hipMalloc((void **)&nd, sizeof(Dom) * cap) != hipSuccess) return bfail(b, GRAL_PF_ENOMEM, "batch add: hipMalloc failed");
         if (hipStreamSynchronize(b->stream) != hipSuccess) return bfail(b, GRAL_PF_EDEVICE, "batch add: stream failed");
+        b->graphs.clear();
         (void)hipFree(b->dd);
         b->dd = nd;
         b->dd_cap = cap;
@@ -1379,7 +1461,7 @@ extern "C" int gral_pf_batch_step(gral_pf_batch *b, int32_t *n_done, gral_pf_han
     std::vector<gral_pf_handle *> order(act);
     std::stable_sort(order.begin(), order.end(),
                      [](const gral_pf_handle *x, const gral_pf_handle *y) { return x->iter_loops > y->iter_loops; });
-    run_lockstep(order, b->dd, b->stream, b->ev0, b->ev1, n, true, nullptr, R);
+    run_lockstep(order, b->dd, b->stream, b->ev0, b->ev1, n, true, nullptr, R, &b->graphs);
     if (R.code) return bfail(b, R.code, R.err);
     b->timing = gral_pf_timing{};
     b->timing.ms_total = R.ms;
@@ -1427,7 +1509,7 @@ extern "C" int gral_pf_batch_iterate(gral_pf_batch *b, int32_t n, int32_t check,
     if (rc) return rc;
     b->timing = gral_pf_timing{};
     RunResult R;
-    run_lockstep(b->hs, b->dd, b->stream, b->ev0, b->ev1, n, check != 0, nullptr, R);
+    run_lockstep(b->hs, b->dd, b->stream, b->ev0, b->ev1, n, check != 0, nullptr, R, &b->graphs);
     if (R.code) return bfail(b, R.code, R.err);
     b->timing.ms_total = R.ms;
     b->timing.launches = R.launches;
@@ -1473,6 +1555,8 @@ extern "C" const char *gral_pf_batch_last_error(const gral_pf_batch *b) {
 
 extern "C" void gral_pf_batch_destroy(gral_pf_batch *b) {
     if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    b->graphs.clear();
     if (b->dd) (void)hipFree(b->dd);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1851,7 +1935,9 @@ extern "C" const char *gral_pf_last_error(const gral_pf_handle *h) {
 
 extern "C" void gral_pf_destroy(gral_pf_handle *h) {
     if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
     free_device(h);
+    delete h->graphs;
     gff_free(h);
     if (h->stage_buf) (void)hipHostFree(h->stage_buf);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
