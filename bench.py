@@ -227,6 +227,52 @@ def diagnostic_timing(n, reps=3):
             "workload": f"city block {n}x{n}, 270 deg, after 2 iterations, flow_field_level 2 (10 pressure iterations)"}
 
 
+def adjoint_timing(steps=20, reps=3):
+    """The adjoint of the pressure-correction steps on the adjoint configuration's
+    grid (256 x 256 city, 32 layers): device time of a forward window of
+    ``steps`` steps (gral_pf_forward_ckpt) and of its VJP (gral_pf_vjp, the
+    gradient of the mean wind speed at 64 receptors 3 m above ground), beside
+    a full SIMPLE iteration of the same situation."""
+    from gral_amd import adjoint, domain, pf
+    c = domain.config4(iterations=steps + 8)
+    rec = adjoint.receptors(c)
+    fw, vj = [], []
+    with pf.PrognosticFlowfield(c) as sv:
+        sv.prepare(iterations=steps + 8)
+        sv.iterate(4)
+        it_ms = sv.timing()["ms_total"] / 4
+        for _ in range(reps):
+            sv.forward_ckpt(steps)
+            fw.append(sv.timing()["ms_total"])
+            du, dv, dw = adjoint.mean_receptor_speed_grad(sv.download(), rec)
+            sv.vjp(du, dv, dw)
+            vj.append(sv.timing()["ms_total"])
+            sv.restore_ckpt()
+    f, v = sorted(fw)[reps // 2], sorted(vj)[reps // 2]
+    return {"workload": f"city 256x256, 32 layers, {steps} pressure-correction steps, VJP of the mean speed at "
+                        f"{len(rec)} receptors", "active_cells": c.active_cells(), "forward_ms": f, "vjp_ms": v,
+            "vjp_over_forward": v / f, "simple_iteration_ms": it_ms, "vjp_step_over_simple_iteration": v / steps / it_ms}
+
+
+def graph_timing(iters=100):
+    """Launch replay on the small plain case (100 x 100, one building): device
+    time per SIMPLE iteration with every kernel launched, and with the
+    iteration chunks replayed as captured HIP graphs (GRAL_PF_GRAPH=1)."""
+    from gral_amd import domain, pf
+    out = {}
+    for mode in ("0", "1"):
+        os.environ["GRAL_PF_GRAPH"] = mode
+        c = domain.config1(iterations=iters + 20)
+        with pf.PrognosticFlowfield(c) as sv:
+            sv.prepare(iterations=iters + 20)
+            sv.iterate(20)                               # the first chunk is captured
+            sv.iterate(iters)
+            out["graphs" if mode == "1" else "launches"] = sv.timing()["ms_total"] / iters
+    os.environ.pop("GRAL_PF_GRAPH", None)
+    return {"workload": "config 1 (100x100, 30 layers, one building), plain", "ms_per_iter_launches": out["launches"],
+            "ms_per_iter_graphs": out["graphs"], "speedup": out["launches"] / out["graphs"]}
+
+
 def batch_sweep(sizes, n, steps=6, warmup=2, log=sys.stderr):
     """Throughput against batch size on the city block: S situations (the 8
     directions repeated) advanced in lock step; per size the device time per
@@ -337,6 +383,10 @@ def main():
                     help="also time the flow-field file writer (GPU pack + zip) on the first situation")
     ap.add_argument("--batched-workload", type=int, default=0, metavar="K",
                     help="also solve the first K situations of the batched workload, batched and plain")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the small plain case with launches against replayed HIP graphs")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also time the adjoint of the pressure-correction steps (forward window and VJP)")
     ap.add_argument("--diagnostic", action="store_true",
                     help="also time the diagnostic post-pass (gral_pf_diagnostic) on the 270 deg city")
     ap.add_argument("--batch-sweep", type=str, default="", metavar="S1,S2,...",
@@ -489,6 +539,10 @@ def main():
         line["situations_per_hour"] = situations_per_hour(args.n, directions)
     if args.diagnostic:
         line["diagnostic_post_pass"] = diagnostic_timing(args.n)
+    if args.adjoint:
+        line["adjoint"] = adjoint_timing()
+    if args.graph:
+        line["graph_replay"] = graph_timing()
     if args.batch_sweep:
         line["batch_sweep"] = batch_sweep([int(x) for x in args.batch_sweep.split(",")], args.n)
     if args.stream:

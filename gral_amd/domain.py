@@ -309,6 +309,16 @@ def config2(direction: float = 270.0, turbulence_model: int = 1, iterations: int
                       iterations=iterations)
 
 
+def config4(iterations: int = 20, n: int = 256, vertcellsff: int = 32, direction: float = 270.0,
+            speed: float = 3.0) -> Case:
+    """The adjoint configuration's grid: a 256 x 256 city block of the config-2
+    generator (seed 20264000; dx = 2 m, footprints 8-40 m, heights 6-40 m, 30 %
+    plan area), 32 prognostic layers, log inflow of ``speed`` at 10 m."""
+    bl = city_buildings(n, n, 2.0, 20264000)
+    return build_case(f"C4-{int(direction)}", n, n, bl, dx=2.0, vertcellsff=vertcellsff, speed10=speed,
+                      direction=direction, z0=0.5, iterations=iterations)
+
+
 def tiny_case(n: int = 8, vertcellsff: int = 8, building=(3, 5, 4, 6, 4.5), direction: float = 250.0,
               turbulence_model: int = 1, iterations: int = 3, veg: bool = False, seed: int = 7) -> Case:
     """A few-cell domain for brute-force checks.  A single building; optional
