@@ -79,8 +79,10 @@ __global__ void k_adj_correction(const Dom *__restrict__ ds, AdjArgs a) {
 
 // S^T on anti-diagonal t of the sweep (a = j1 outer, b = i1 inner, mirrored by
 // IS/JS as in the forward), launched from the last diagonal to the first.
-// One wave per column: the lanes stage the column's vectors in LDS, lane 0
-// runs the transposed recurrences.
+// One wave per column: the lanes stage the column's vectors and its P, TERMP
+// and ATP in LDS, lane 0 runs the transposed recurrences over LDS in blocks of
+// AU levels whose operands are read before the block's chain.
+#define AU 8
 __global__ __launch_bounds__(AWAVE) void k_adj_pressure(const Dom *__restrict__ ds, AdjArgs a, int t, int IS, int JS) {
     const DomG &d = *reinterpret_cast<const DomG *>(ds);
     extern __shared__ float sm[];
@@ -91,7 +93,7 @@ __global__ __launch_bounds__(AWAVE) void k_adj_pressure(const Dom *__restrict__ 
     const size_t c = c2(d, i, j);
     if (d.adv[c] != 1) return;
     const int nkp = d.nkp, tid = threadIdx.x;
-    float *sx = sm, *sq = sx + nkp, *sm_ = sq + nkp;
+    float *sx = sm, *sq = sx + nkp, *sm_ = sq + nkp, *sp = sm_ + nkp, *st = sp + nkp, *sat = st + nkp;
     const int KK = d.kk[c], VI = d.vi[c];
     const size_t o = c3(d, i, j), onb_i = c3(d, i + IS, j), onb_j = c3(d, i, j + JS);
     // the two columns visited after this one read its new DPM in their DIM
@@ -100,21 +102,48 @@ __global__ __launch_bounds__(AWAVE) void k_adj_pressure(const Dom *__restrict__ 
         const float dz = d.dzk[k];
         sx[k] = a.pb[o + k] + a.mb[onb_i + k] * (d.dyk * dz / d.dxk) + sgn_j * a.mb[onb_j + k] * (d.dxk * dz / d.dyk);
         sq[k] = a.qb[o + k];
+        sp[k] = a.ptab[o + k];
+        st[k] = a.ttab[o + k];
+        sat[k] = d.atp[k] * a.ttab[o + k];
     }
     __syncthreads();
-    if (tid == 0) {
-        const float *P = a.ptab + o, *T = a.ttab + o;
-        // x_k = P_k x_{k-1} + Q_k (k = KK+1 .. VI), transposed top-down
-        for (int k = VI; k > KK; --k) {
-            sq[k] += sx[k];
-            if (k - 1 > KK) sx[k - 1] += P[k] * sx[k];
+    if (tid == 0 && VI > KK) {
+        // x_k = P_k x_{k-1} + Q_k (k = KK+1 .. VI), transposed top-down:
+        // Q_bar[k] += x_bar[k]; x_bar[k-1] += P_k x_bar[k]
+        float xk = sx[VI];
+        int k = VI;
+        for (; k - AU + 1 > KK; k -= AU) {
+            float cx[AU], cp[AU], cq[AU];
+#pragma unroll
+            for (int u = 0; u < AU; ++u) { cp[u] = sp[k - u]; cq[u] = sq[k - u]; cx[u] = sx[k - u - 1]; }
+#pragma unroll
+            for (int u = 0; u < AU; ++u) {
+                sq[k - u] = cq[u] + xk;
+                if (k - u - 1 > KK) xk = cx[u] + cp[u] * xk;
+            }
+        }
+        for (; k > KK; --k) {
+            sq[k] = sq[k] + xk;
+            if (k - 1 > KK) xk = sx[k - 1] + sp[k] * xk;
         }
         // Q_k = (ATP_k Q_{k+1} + DIM_k) TERMP_k (k = VI .. KK+1), transposed bottom-up
         float carry = 0.0f;
-        for (int k = KK + 1; k <= VI; ++k) {
+        k = KK + 1;
+        for (; k + AU - 1 <= VI; k += AU) {
+            float cq[AU], ct[AU], ca[AU];
+#pragma unroll
+            for (int u = 0; u < AU; ++u) { cq[u] = sq[k + u]; ct[u] = st[k + u]; ca[u] = sat[k + u]; }
+#pragma unroll
+            for (int u = 0; u < AU; ++u) {
+                const float qt = cq[u] + carry;
+                sm_[k + u] = ct[u] * qt;
+                carry = ca[u] * qt;
+            }
+        }
+        for (; k <= VI; ++k) {
             const float qt = sq[k] + carry;
-            sm_[k] = T[k] * qt;
-            carry = d.atp[k] * T[k] * qt;
+            sm_[k] = st[k] * qt;
+            carry = sat[k] * qt;
         }
         const int s = a.src[c];
         if (s >= 0) a.qb[c3(d, s, j) + VI + 1] += carry;   // the stale top's writer (same row, earlier)
@@ -164,7 +193,7 @@ extern "C" int pf_launch_adjoint_step(const Dom *hd, const Dom *dd, const AdjArg
     hipLaunchKernelGGL(k_adj_correction, dim3((unsigned)((h.nii - 2) * (h.njj - 2))), dim3(64), 0, s, dd, a);
     const int oa = 2, ob = h.njj - 1, ia = 2, ib = h.nii - 1;
     const int nd = (ob - oa) + (ib - ia) + 1;
-    const size_t shm = sizeof(float) * 3 * (size_t)h.nkp;
+    const size_t shm = sizeof(float) * 6 * (size_t)h.nkp;
     for (int t = nd - 1; t >= 0; --t) {
         const int amin = max(oa, oa + t - (ib - ia)), amax = min(ob, oa + t);
         if (amax >= amin)

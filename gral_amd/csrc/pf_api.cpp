@@ -30,16 +30,6 @@ struct Probe {
 extern "C" int pf_launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, int slot, hipStream_t s,
                                    long long *launches, Probe *probe);
 extern "C" int pf_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
-struct DiagArgs {
-    int nii, njj, nkk, nkh, nj2;
-    int level, terrain;
-    float dxk, dyk, dtime;
-    const float *dzk;
-    const float *app, *atp, *abp_in, *abp_bot;
-    const int *kk;
-    const uint8_t *adv;
-    float *uk, *vk, *wk, *dpm, *div;
-};
 extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStream_t s);
 extern "C" int pf_launch_gff(const GffArgs *ga, hipStream_t s);
 extern "C" int pf_launch_wind(const WindArgs *wa, hipStream_t s);
@@ -1591,6 +1581,20 @@ extern "C" int gral_pf_diagnostic(gral_pf_handle *h, int32_t flow_field_level, i
         abb[k] = DXK * DYK / DZK[k];
         atp[k] = (DXK * DYK) / (0.5f * (DZK[k + 1] + DZK[k]));
     }
+    // TERMP and PIMP of the column recurrence (:124-127): every column runs it
+    // from NKK-1 down with PIMP[NKK] = 0, so above a column's bottom level the
+    // values are the same in every column; at the bottom level ABP differs.
+    std::vector<float> term(nkh, 0.0f), pin(nkh, 0.0f), pbot(nkh, 0.0f);
+    {
+        float Pn = 0.0f;
+        for (int k = h->nkk - 1; k >= 1; --k) {
+            const float T = 1 / (app[k] - atp[k] * Pn);
+            term[k] = T;
+            pin[k] = abi[k] * T;
+            pbot[k] = abb[k] * T;
+            Pn = pin[k];
+        }
+    }
     std::vector<int> kk32(n2);
     for (size_t c = 0; c < n2; ++c) kk32[c] = h->kk[c];
     std::vector<void *> tmp;
@@ -1608,7 +1612,7 @@ extern "C" int gral_pf_diagnostic(gral_pf_handle *h, int32_t flow_field_level, i
     size_t sz[13] = {nkh * 4, nkh * 4, nkh * 4, nkh * 4, nkh * 4, n2 * 4, n2, n3 * 4, n3 * 4, n3 * 4, n3 * 4, n3 * 4, 0};
     for (int q = 0; q < 12; ++q)
         if ((rc = talloc(&p[q], sz[q]))) { cleanup(); return rc; }
-    const void *src[12] = {h->dzk.data(), app.data(), atp.data(), abi.data(), abb.data(), kk32.data(), h->adv.data(),
+    const void *src[12] = {h->dzk.data(), term.data(), atp.data(), pin.data(), pbot.data(), kk32.data(), h->adv.data(),
                            h->uk.data(), h->vk.data(), h->wk.data(), h->dpm.data(), nullptr};
     for (int q = 0; q < 11; ++q)
         if (hipMemcpyAsync(p[q], src[q], sz[q], hipMemcpyHostToDevice, h->stream) != hipSuccess) {
@@ -1616,8 +1620,8 @@ extern "C" int gral_pf_diagnostic(gral_pf_handle *h, int32_t flow_field_level, i
             return fail(h, GRAL_PF_EDEVICE, "diagnostic: upload failed");
         }
     if (hipMemsetAsync(p[11], 0, sz[11], h->stream) != hipSuccess) { cleanup(); return fail(h, GRAL_PF_EDEVICE, "diagnostic: memset failed"); }
-    g.dzk = (const float *)p[0]; g.app = (const float *)p[1]; g.atp = (const float *)p[2];
-    g.abp_in = (const float *)p[3]; g.abp_bot = (const float *)p[4];
+    g.dzk = (const float *)p[0]; g.term = (const float *)p[1]; g.atp = (const float *)p[2];
+    g.p_in = (const float *)p[3]; g.p_bot = (const float *)p[4];
     g.kk = (const int *)p[5]; g.adv = (const uint8_t *)p[6];
     g.uk = (float *)p[7]; g.vk = (float *)p[8]; g.wk = (float *)p[9]; g.dpm = (float *)p[10]; g.div = (float *)p[11];
     int iters = (flow_field_level == 2) ? 10 : 5;

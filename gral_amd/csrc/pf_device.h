@@ -129,6 +129,19 @@ struct WindArgs {
     int32_t *ck;                      // IndexK
 };
 
+// Diagnostic post-pass (one situation per launch, full-height columns)
+struct DiagArgs {
+    int nii, njj, nkk, nkh, nj2;   // nkh = NKK + 2 levels per stored column
+    int level, terrain;            // FlowFieldLevel (1 diagnostic, 2 prognostic); caller
+    float dxk, dyk, dtime;
+    const float *dzk;              // [nkh]
+    const float *atp;              // [nkh] ATP by level
+    const float *term, *p_in, *p_bot;  // [nkh] TERMP; PIMP above, and at, a column's bottom level
+    const int *kk;                 // [nii+2][njj+2]
+    const uint8_t *adv;
+    float *uk, *vk, *wk, *dpm, *div;   // [nii+2][njj+2][nkh]
+};
+
 // Buffers of one step of the adjoint (pf_adjoint.hip), each [nii+2][njj+2][nkp]
 // except the per-column source table.
 struct AdjArgs {

@@ -2113,16 +2113,6 @@ static inline int diag_len(int t, int oa, int ob, int ia, int ib) {
 // 862-933).  It works on full-height columns (levels 0..NKK+1), in its own
 // arrays; one situation per launch, passed by value.
 // ------------------------------------------------------------------------
-struct DiagArgs {
-    int nii, njj, nkk, nkh, nj2;   // nkh = NKK + 2 levels per stored column
-    int level, terrain;            // FlowFieldLevel (1 diagnostic, 2 prognostic); caller
-    float dxk, dyk, dtime;
-    const float *dzk;              // [nkh]
-    const float *app, *atp, *abp_in, *abp_bot;   // [nkh] pressure coefficients by level
-    const int *kk;                 // [nii+2][njj+2]
-    const uint8_t *adv;
-    float *uk, *vk, *wk, *dpm, *div;             // [nii+2][njj+2][nkh]
-};
 __device__ __forceinline__ size_t dg3(const DiagArgs &g, int i, int j) { return ((size_t)i * g.nj2 + j) * (size_t)g.nkh; }
 
 // DPM = 0 for i in [1,NII], j in [1,NJJ], k in [1,NKK+1] (DiagnosticFlowfield.cs:40-49)
@@ -2147,47 +2137,75 @@ __global__ void k_dg_divergence(DiagArgs g) {
 }
 
 // pressure correction (:84-341), one anti-diagonal of the sweep per launch,
-// one wave per column: lanes build DIM, lane 0 runs the two recurrences over
-// the whole column.  Columns are solved over their full height, so the row
-// scratch above the top (PIMP/QIMP[NKK]) is always zero.
+// one wave per column.  Every column is solved from NKK-1 down to KKART+1 and
+// PIMP[NKK] is never written, so TERMP and PIMP at level k are the same in
+// every column whose bottom lies below k; at the bottom level only ABP (and so
+// PIMP) differs.  The host runs that recurrence once (term, p_in, p_bot);
+// here the lanes build DIM and stage the tables in LDS, and lane 0 runs the
+// two recurrences that depend on the column (QIMP top-down from QIMP[NKK] = 0,
+// then the new DPM bottom-up from DPM[KKART] = 0) over LDS.
+#define DG_UNROLL 8
 __global__ __launch_bounds__(WAVE) void k_dg_pressure(DiagArgs g, int t, int IS, int JS) {
     extern __shared__ float sm[];
     int a, b;
     if (!diag_col(t, 2, g.njj - 1, 2, g.nii - 1, blockIdx.x, a, b)) return;
     int j = (JS == -1) ? g.njj - a + 1 : a;
     int i = (IS == -1) ? g.nii - b + 1 : b;
-    int KK = g.kk[(size_t)i * g.nj2 + j];
-    const int nkh = g.nkh;
-    float *sDIM = sm, *sP = sDIM + nkh, *sQ = sP + nkh, *sN = sQ + nkh;
+    const int KK = g.kk[(size_t)i * g.nj2 + j];
+    const int nkh = g.nkh, top = g.nkk - 1;
+    // LDS: DIM, ATP, TERMP, P (then the new DPM) and Q, nkh levels each
+    float *sDIM = sm, *sA = sDIM + nkh, *sT = sA + nkh, *sP = sT + nkh, *sQ = sP + nkh;
     size_t o = dg3(g, i, j), oim = dg3(g, i - 1, j), oip = dg3(g, i + 1, j), ojm = dg3(g, i, j - 1), ojp = dg3(g, i, j + 1);
     const float DXK = g.dxk, DYK = g.dyk;
-    for (int k = KK + 1 + threadIdx.x; k <= g.nkk - 1; k += WAVE) {
+    for (int k = KK + 1 + threadIdx.x; k <= top; k += WAVE) {
         float dz = g.dzk[k];
         sDIM[k] = g.div[o + k] / g.dtime + (g.dpm[oim + k] + g.dpm[oip + k]) / DXK * DYK * dz +
                   (g.dpm[ojm + k] - g.dpm[ojp + k]) / DYK * DXK * dz;
+        sA[k] = g.atp[k];
+        sT[k] = g.term[k];
+        sP[k] = (k > KK + 1) ? g.p_in[k] : g.p_bot[k];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        float Pn = 0.0f, Qn = 0.0f;
-        for (int k = g.nkk - 1; k > KK; --k) {
-            float ATP = g.atp[k];
-            float ABP = (k > KK + 1) ? g.abp_in[k] : g.abp_bot[k];
-            float TERMP = 1 / (g.app[k] - ATP * Pn);
-            float P = ABP * TERMP;
-            float Q = (ATP * Qn + sDIM[k]) * TERMP;
-            sP[k] = P; sQ[k] = Q;
-            Pn = P; Qn = Q;
+    if (threadIdx.x == 0 && top > KK) {
+        // QIMP[k] = (ATP[k] QIMP[k+1] + DIM[k]) TERMP[k], k = top .. KK+1, in blocks
+        // of DG_UNROLL levels whose operands are read before the block's chain
+        float Qn = 0.0f;
+        int k = top;
+        for (; k - DG_UNROLL + 1 > KK; k -= DG_UNROLL) {
+            float ca[DG_UNROLL], cd[DG_UNROLL], ct[DG_UNROLL];
+#pragma unroll
+            for (int u = 0; u < DG_UNROLL; ++u) { ca[u] = sA[k - u]; cd[u] = sDIM[k - u]; ct[u] = sT[k - u]; }
+#pragma unroll
+            for (int u = 0; u < DG_UNROLL; ++u) {
+                Qn = (ca[u] * Qn + cd[u]) * ct[u];
+                sQ[k - u] = Qn;
+            }
         }
-        float dprev = 0.0f;                               // DPM[0] = 0, and DPM[k] = 0 up to KKART
-        for (int k = 1; k <= g.nkk - 1; ++k) {
-            float v = (KK < k) ? sP[k] * dprev + sQ[k] : 0.0f;
-            sN[k] = v;
-            dprev = v;
+        for (; k > KK; --k) {
+            Qn = (sA[k] * Qn + sDIM[k]) * sT[k];
+            sQ[k] = Qn;
+        }
+        // DPM[k] = PIMP[k] DPM[k-1] + QIMP[k], k = KK+1 .. top; written over P
+        float dprev = 0.0f;
+        k = KK + 1;
+        for (; k + DG_UNROLL - 1 <= top; k += DG_UNROLL) {
+            float cp[DG_UNROLL], cq[DG_UNROLL];
+#pragma unroll
+            for (int u = 0; u < DG_UNROLL; ++u) { cp[u] = sP[k + u]; cq[u] = sQ[k + u]; }
+#pragma unroll
+            for (int u = 0; u < DG_UNROLL; ++u) {
+                dprev = cp[u] * dprev + cq[u];
+                sP[k + u] = dprev;
+            }
+        }
+        for (; k <= top; ++k) {
+            dprev = sP[k] * dprev + sQ[k];
+            sP[k] = dprev;
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) g.dpm[o] = 0.0f;
-    for (int k = 1 + threadIdx.x; k <= g.nkk - 1; k += WAVE) g.dpm[o + k] = sN[k];
+    // DPM[0] = 0 and DPM[k] = 0 up to KKART (the reference writes them each sweep)
+    for (int k = threadIdx.x; k <= top; k += WAVE) g.dpm[o + k] = (k > KK) ? sP[k] : 0.0f;
 }
 
 // velocity correction without quantisation (:345-410); thread per (column, level)
@@ -2244,7 +2262,7 @@ extern "C" int pf_launch_diagnostic(const DiagArgs *ga, int iterations, hipStrea
             default: IS = -1; JS = 1; break;
         }
         int oa = 2, ob = g.njj - 1, ia = 2, ib = g.nii - 1;
-        size_t shm = sizeof(float) * 4 * (size_t)g.nkh;
+        size_t shm = sizeof(float) * 5 * (size_t)g.nkh;
         for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t)
             hipLaunchKernelGGL(k_dg_pressure, dim3(diag_len(t, oa, ob, ia, ib)), dim3(WAVE), shm, s, g, t, IS, JS);
         hipLaunchKernelGGL(k_dg_correction, dim3(ncol), dim3(64), 0, s, g);
