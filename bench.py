@@ -273,6 +273,39 @@ def graph_timing(iters=100):
             "ms_per_iter_graphs": out["graphs"], "speedup": out["launches"] / out["graphs"]}
 
 
+def keps_timing(n, directions, steps=5, warmup=1):
+    """The same 8-direction city batch with the k-eps model (turbulence model
+    2: k-eps momentum and the TKE/dissipation sweep), timed the same way as
+    the main line: device time per SIMPLE iteration over ``steps`` iterations
+    after ``warmup``, and the per-family launch times of two instrumented
+    iterations with the dominant sweep's share of the HBM peak."""
+    from gral_amd import domain, pf
+    total = warmup + steps + 8
+    cases = [domain.config2(direction=d, iterations=total, n=n, turbulence_model=2) for d in directions]
+    solvers = [pf.PrognosticFlowfield(c) for c in cases]
+    for sv in solvers:
+        sv.prepare(iterations=total)
+    with pf.Batch(solvers) as batch:
+        if warmup:
+            batch.iterate(warmup)
+        batch.iterate(steps)
+        tm = batch.timing()
+        prof = batch.profile_families(2)
+    for sv in solvers:
+        sv.close()
+    sweeps = {k: v for k, v in prof.items() if k in ALG_BYTES}
+    dom = max(sweeps, key=lambda k: sweeps[k]["ms"])
+    pdom = sweeps[dom]
+    avg_ms = pdom["ms"] / max(1, pdom["launches"])
+    achieved = ALG_BYTES[dom] * pdom["cells"] / max(1, pdom["launches"]) / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
+    return {"workload": f"city block {n}x{n}, the same {len(cases)} directions as one lock-step batch, k-eps model",
+            "ms_per_step": tm["ms_total"] / steps, "value": tm["cell_iterations"] / (tm["ms_total"] / 1e3),
+            "unit": "cell-iter/s", "alg_bytes_per_cell_iter": ALG_BYTES_ITER_KEPS,
+            "dominant_kernel": KERNEL_OF_KEPS[dom], "dominant_avg_launch_us": avg_ms * 1e3,
+            "dominant_frac_of_hbm_peak": achieved / HBM_PEAK_GBS,
+            "kernel_families_ms_per_iter": {k: v["ms"] / 2 for k, v in prof.items()}}
+
+
 def batch_sweep(sizes, n, steps=6, warmup=2, log=sys.stderr):
     """Throughput against batch size on the city block: S situations (the 8
     directions repeated) advanced in lock step; per size the device time per
@@ -397,6 +430,8 @@ def main():
     ap.add_argument("--capacity", type=int, default=32, help="situations on the device at a time in --stream")
     ap.add_argument("--stream-plain", type=int, default=4, metavar="P",
                     help="plain solves of the first P situations beside --stream")
+    ap.add_argument("--no-keps", action="store_true",
+                    help="skip the k-eps leg (the same batch with turbulence model 2)")
     ap.add_argument("--turbulence-model", type=int, default=1, choices=(1, 2),
                     help="1: mixing length (the benchmark's model), 2: k-eps")
     ap.add_argument("--particle-wind", type=int, default=0, metavar="N",
@@ -537,6 +572,8 @@ def main():
         line["batched_workload"] = batched_workload(args.batched_workload)
     if args.situations_hour:
         line["situations_per_hour"] = situations_per_hour(args.n, directions)
+    if model == 1 and not args.no_keps:
+        line["keps"] = keps_timing(args.n, directions)
     if args.diagnostic:
         line["diagnostic_post_pass"] = diagnostic_timing(args.n)
     if args.adjoint:

@@ -49,10 +49,26 @@ __device__ __forceinline__ int xcd_block(int n) {
     if (idx >= cnt) return -1;
     return x * per + min(x, rem) + idx;
 }
-// The column kernels give a wave COLS_PER_WAVE consecutive columns (in j), in
-// the XCD order above: a column is a few loads per level, so with one column
-// per wave the launch would be bound by the rate at which waves are created.
-#define COLS_PER_WAVE 4
+// column metadata from the per-column table (see gral_pf_prepare)
+struct Meta {
+    int KK, VI, KSTART, KKmi, KKmj;
+    bool adv, vegp;
+};
+__device__ __forceinline__ Meta unpack_meta(int4 m) {
+    Meta r;
+    r.KK = m.x; r.VI = m.y; r.KSTART = m.z & 0xffff; r.adv = (m.z >> 16) & 1; r.vegp = (m.z >> 17) & 1;
+    r.KKmi = m.w & 0xffff; r.KKmj = (m.w >> 16) & 0xffff;
+    return r;
+}
+
+// The cell kernels (divergence, velocity correction, averages) walk the
+// interior columns of one sweep row i (j = 2..NJJ-1 are contiguous in memory,
+// nkp levels each) as one flat run of floats: a block per row, each thread
+// four consecutive levels of one column per step (16-byte accesses; nkp is a
+// multiple of 4), so every load of a wave is contiguous and a block keeps
+// many independent loads in flight.  Blocks take rows in XCD order.
+#define CELL_THREADS 256
+
 __global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) p[g] = v;
@@ -164,30 +180,47 @@ __global__ void k_bc_ns(const Dom *__restrict__ ds) {
 }
 
 // ------------------------------------------------------------------------
-// Mass divergence, PrognosticFlowfield.cs:723-758.  Block = one column.
+// Mass divergence, PrognosticFlowfield.cs:723-758, over the active cells
+// (ADVDOM = 1, KKART < k <= VI); DPM = 0 there.
 // ------------------------------------------------------------------------
-__global__ void k_divergence(const Dom *__restrict__ ds) {
+__global__ __launch_bounds__(CELL_THREADS) void k_divergence(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    const int ncols = (d.nii - 2) * (d.njj - 2);
-    const int grp = xcd_block((ncols + COLS_PER_WAVE - 1) / COLS_PER_WAVE);
-    if (grp < 0) return;
-    for (int col = grp * COLS_PER_WAVE; col < min(ncols, (grp + 1) * COLS_PER_WAVE); ++col) {
-    int nj = d.njj - 2;
-    int i = 2 + col / nj, j = 2 + col % nj;
-    size_t c = c2(d, i, j);
-    if (d.adv[c] != 1) continue;
-    int KK = d.kk[c], VI = d.vi[c];
-    size_t o = c3(d, i, j), oip = c3(d, i + 1, j), ojp = c3(d, i, j + 1);
-    for (int k = KK + 1 + threadIdx.x; k <= VI; k += blockDim.x) {
-        float dz = d.dzk[k];
-        float t1 = (d.uk[o + k] - d.uk[oip + k]) * d.dyk * dz;
-        float t2 = (d.vk[o + k] - d.vk[ojp + k]) * d.dxk * dz;
-        float v;
-        if (k > KK + 1) v = t1 + t2 + (d.wk[o + k] - d.wk[o + k + 1]) * d.area;
-        else v = t1 + t2 - d.wk[o + k + 1] * d.area;
-        d.div[o + k] = v;
-        d.dpm[o + k] = 0;
-    }
+    const int blk = xcd_block(d.nii - 2);
+    if (blk < 0) return;
+    const int i = 2 + blk, nkp = d.nkp, n4 = ((d.njj - 2) * nkp) >> 2;
+    const size_t row = c3(d, i, 2), ip = (size_t)d.nj2 * nkp;
+    for (int q = threadIdx.x; q < n4; q += CELL_THREADS) {
+        const int e = q << 2, jo = e / nkp, k0 = e - jo * nkp;
+        const Meta m = unpack_meta(ldg4i(d.meta + c2(d, i, 2 + jo)));
+        if (!m.adv || k0 + 3 <= m.KK || k0 > m.VI) continue;
+        const size_t o = row + e;
+        const float4 u = ldg4(d.uk + o), ue = ldg4(d.uk + o + ip), v = ldg4(d.vk + o), vn = ldg4(d.vk + o + nkp);
+        const float4 w = ldg4(d.wk + o);
+        const float uu[4] = {u.x, u.y, u.z, u.w}, uue[4] = {ue.x, ue.y, ue.z, ue.w};
+        const float vv[4] = {v.x, v.y, v.z, v.w}, vvn[4] = {vn.x, vn.y, vn.z, vn.w};
+        const float ww[5] = {w.x, w.y, w.z, w.w, d.wk[o + 4]};
+        float r[4];
+        bool on[4];
+#pragma unroll
+        for (int u4 = 0; u4 < 4; ++u4) {
+            const int k = k0 + u4;
+            on[u4] = m.KK < k && k <= m.VI;
+            r[u4] = 0.0f;
+            if (on[u4]) {
+                const float dz = d.dzk[k];
+                const float t1 = (uu[u4] - uue[u4]) * d.dyk * dz;
+                const float t2 = (vv[u4] - vvn[u4]) * d.dxk * dz;
+                r[u4] = (k > m.KK + 1) ? t1 + t2 + (ww[u4] - ww[u4 + 1]) * d.area : t1 + t2 - ww[u4 + 1] * d.area;
+            }
+        }
+        if (on[0] && on[1] && on[2] && on[3]) {
+            stg4(d.div + o, r[0], r[1], r[2], r[3]);
+            stg4(d.dpm + o, 0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+#pragma unroll
+            for (int u4 = 0; u4 < 4; ++u4)
+                if (on[u4]) { d.div[o + u4] = r[u4]; d.dpm[o + u4] = 0.0f; }
+        }
     }
 }
 
@@ -260,69 +293,123 @@ __global__ __launch_bounds__(WAVE) void k_pressure(const Dom *__restrict__ ds, i
 }
 
 // ------------------------------------------------------------------------
-// Velocity correction with FP64 quantisation, PrognosticFlowfield.cs:904-949
+// Velocity correction with FP64 quantisation, PrognosticFlowfield.cs:904-949,
+// over the columns with ADVDOM = 1 at (i,j), (i-1,j) or (i,j-1), levels 1..VI.
 // ------------------------------------------------------------------------
-__global__ void k_correction(const Dom *__restrict__ ds) {
+__device__ __forceinline__ bool corr_column(const DomG &d, int i, int j, bool adv) {
+    return adv || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1;
+}
+__global__ __launch_bounds__(CELL_THREADS) void k_correction(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    const int ncols = (d.nii - 2) * (d.njj - 2);
-    const int grp = xcd_block((ncols + COLS_PER_WAVE - 1) / COLS_PER_WAVE);
-    if (grp < 0) return;
-    for (int col = grp * COLS_PER_WAVE; col < min(ncols, (grp + 1) * COLS_PER_WAVE); ++col) {
-    int nj = d.njj - 2;
-    int i = 2 + col / nj, j = 2 + col % nj;
-    size_t c = c2(d, i, j);
-    if (!(d.adv[c] == 1 || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1)) continue;
-    int KK = d.kk[c], VI = d.vi[c], KKiM = d.kk[c2(d, i - 1, j)], KKjM = d.kk[c2(d, i, j - 1)];
-    size_t o = c3(d, i, j), oim = c3(d, i - 1, j), ojm = c3(d, i, j - 1);
-    for (int k = 1 + threadIdx.x; k <= VI; k += blockDim.x) {
-        float dpmk = d.dpm[o + k];
-        const bool q = !d.noquant;
-        if (i > 2 && KK < k && KKiM < k) {
-            float e = d.uk[o + k] + (d.dpm[oim + k] - dpmk) / d.dxk * d.dtime;
-            d.uk[o + k] = q ? quantise(e, d.frund) : e;
+    const int blk = xcd_block(d.nii - 2);
+    if (blk < 0) return;
+    const int i = 2 + blk, nkp = d.nkp, n4 = ((d.njj - 2) * nkp) >> 2;
+    const size_t row = c3(d, i, 2), ip = (size_t)d.nj2 * nkp;
+    const bool q = !d.noquant;
+    for (int t = threadIdx.x; t < n4; t += CELL_THREADS) {
+        const int e = t << 2, jo = e / nkp, k0 = e - jo * nkp, j = 2 + jo;
+        const Meta m = unpack_meta(ldg4i(d.meta + c2(d, i, j)));
+        if (k0 > m.VI || !corr_column(d, i, j, m.adv)) continue;
+        const size_t o = row + e;
+        const float4 p = ldg4(d.dpm + o), pw = ldg4(d.dpm + o - ip), ps = ldg4(d.dpm + o - nkp);
+        const float4 u = ldg4(d.uk + o), v = ldg4(d.vk + o), w = ldg4(d.wk + o);
+        const float pp[5] = {d.dpm[o - 1], p.x, p.y, p.z, p.w};              // pp[u4 + 1]: level k0 + u4
+        const float ppw[4] = {pw.x, pw.y, pw.z, pw.w}, pps[4] = {ps.x, ps.y, ps.z, ps.w};
+        float uu[4] = {u.x, u.y, u.z, u.w}, vv[4] = {v.x, v.y, v.z, v.w}, ww[4] = {w.x, w.y, w.z, w.w};
+        bool onu[4], onv[4], on[4];
+#pragma unroll
+        for (int u4 = 0; u4 < 4; ++u4) {
+            const int k = k0 + u4;
+            on[u4] = k >= 1 && k <= m.VI;
+            onu[u4] = on[u4] && i > 2 && m.KKmi < k;              // KKART(i,j) < k and KKART(i-1,j) < k
+            onv[u4] = on[u4] && j > 2 && m.KKmj < k;
+            const float dpmk = pp[u4 + 1];
+            if (onu[u4]) {
+                const float x = uu[u4] + (ppw[u4] - dpmk) / d.dxk * d.dtime;
+                uu[u4] = q ? quantise(x, d.frund) : x;
+            }
+            if (onv[u4]) {
+                const float x = vv[u4] + (pps[u4] - dpmk) / d.dyk * d.dtime;
+                vv[u4] = q ? quantise(x, d.frund) : x;
+            }
+            if (on[u4]) {
+                if (m.KK + 1 < k) {
+                    const float x = ww[u4] + (pp[u4] - dpmk) / (d.dzk[k] + d.dzk[k - 1]) * 2.0f * d.dtime;
+                    ww[u4] = q ? quantise(x, d.frund) : x;
+                } else {
+                    ww[u4] = 0.0f;
+                }
+            }
         }
-        if (j > 2 && KK < k && KKjM < k) {
-            float e = d.vk[o + k] + (d.dpm[ojm + k] - dpmk) / d.dyk * d.dtime;
-            d.vk[o + k] = q ? quantise(e, d.frund) : e;
-        }
-        if (KK + 1 < k) {
-            float e = d.wk[o + k] + (d.dpm[o + k - 1] - dpmk) / (d.dzk[k] + d.dzk[k - 1]) * 2.0f * d.dtime;
-            d.wk[o + k] = q ? quantise(e, d.frund) : e;
-        } else {
-            d.wk[o + k] = 0.0f;
-        }
-    }
+        if (onu[0] && onu[1] && onu[2] && onu[3]) stg4(d.uk + o, uu[0], uu[1], uu[2], uu[3]);
+        else
+#pragma unroll
+            for (int u4 = 0; u4 < 4; ++u4) if (onu[u4]) d.uk[o + u4] = uu[u4];
+        if (onv[0] && onv[1] && onv[2] && onv[3]) stg4(d.vk + o, vv[0], vv[1], vv[2], vv[3]);
+        else
+#pragma unroll
+            for (int u4 = 0; u4 < 4; ++u4) if (onv[u4]) d.vk[o + u4] = vv[u4];
+        if (on[0] && on[1] && on[2] && on[3]) stg4(d.wk + o, ww[0], ww[1], ww[2], ww[3]);
+        else
+#pragma unroll
+            for (int u4 = 0; u4 < 4; ++u4) if (on[u4]) d.wk[o + u4] = ww[u4];
     }
 }
 
 // ------------------------------------------------------------------------
-// Cell-centred averages, PrognosticFlowfield.cs:958-1014 (+ UKS row copies)
+// Cell-centred averages, PrognosticFlowfield.cs:958-1014 (+ UKS row copies),
+// over the correction's columns, levels KSTART..VI.
 // ------------------------------------------------------------------------
-__global__ void k_averages(const Dom *__restrict__ ds) {
+__global__ __launch_bounds__(CELL_THREADS) void k_averages(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
-    const int ncols = (d.nii - 2) * (d.njj - 2);
-    const int grp = xcd_block((ncols + COLS_PER_WAVE - 1) / COLS_PER_WAVE);
-    if (grp < 0) return;
-    for (int col = grp * COLS_PER_WAVE; col < min(ncols, (grp + 1) * COLS_PER_WAVE); ++col) {
-    int nj = d.njj - 2;
-    int i = 2 + col / nj, j = 2 + col % nj;
-    size_t c = c2(d, i, j);
-    if (!(d.adv[c] == 1 || d.adv[c2(d, i - 1, j)] == 1 || d.adv[c2(d, i, j - 1)] == 1)) continue;
-    int KK = d.kk[c], VI = d.vi[c];
-    int KSTART = (d.cutk[c] == 0) ? KK + 1 : 1;
-    size_t o = c3(d, i, j), oip = c3(d, i + 1, j), ojp = c3(d, i, j + 1);
-    bool copy_s = (j == 2);
-    bool copy_n = (j == d.njj - 1) && (d.adv[c2(d, i, d.njj)] == 1);
-    size_t os = c3(d, i, 1), on = c3(d, i, d.njj);
-    for (int k = KSTART + threadIdx.x; k <= VI; k += blockDim.x) {
-        if (KK < k) {
-            d.uks[o + k] = 0.5f * (d.uk[o + k] + d.uk[oip + k]);
-            d.vks[o + k] = 0.5f * (d.vk[o + k] + d.vk[ojp + k]);
+    const int blk = xcd_block(d.nii - 2);
+    if (blk < 0) return;
+    const int i = 2 + blk, nkp = d.nkp, n4 = ((d.njj - 2) * nkp) >> 2;
+    const size_t row = c3(d, i, 2), ip = (size_t)d.nj2 * nkp;
+    const bool copy_n = d.adv[c2(d, i, d.njj)] == 1;
+    for (int t = threadIdx.x; t < n4; t += CELL_THREADS) {
+        const int e = t << 2, jo = e / nkp, k0 = e - jo * nkp, j = 2 + jo;
+        const Meta m = unpack_meta(ldg4i(d.meta + c2(d, i, j)));
+        if (k0 + 3 < m.KSTART || k0 > m.VI || !corr_column(d, i, j, m.adv)) continue;
+        const size_t o = row + e;
+        const float4 u = ldg4(d.uk + o), ue = ldg4(d.uk + o + ip), v = ldg4(d.vk + o), vn = ldg4(d.vk + o + nkp);
+        const float4 w = ldg4(d.wk + o);
+        const float uu[4] = {u.x, u.y, u.z, u.w}, uue[4] = {ue.x, ue.y, ue.z, ue.w};
+        const float vv[4] = {v.x, v.y, v.z, v.w}, vvn[4] = {vn.x, vn.y, vn.z, vn.w};
+        const float ww[5] = {w.x, w.y, w.z, w.w, d.wk[o + 4]};
+        float ru[4], rv[4], rw[4];
+        bool on[4], onuv[4], onw[4];
+#pragma unroll
+        for (int u4 = 0; u4 < 4; ++u4) {
+            const int k = k0 + u4;
+            on[u4] = k >= m.KSTART && k <= m.VI;
+            onuv[u4] = on[u4] && m.KK < k;
+            onw[u4] = on[u4] && m.KK + 1 < k;
+            ru[u4] = 0.5f * (uu[u4] + uue[u4]);
+            rv[u4] = 0.5f * (vv[u4] + vvn[u4]);
+            rw[u4] = 0.5f * (ww[u4] + ww[u4 + 1]);
         }
-        if (KK + 1 < k) d.wks[o + k] = 0.5f * (d.wk[o + k] + d.wk[o + k + 1]);
-        if (copy_s) d.uks[os + k] = d.uks[o + k];
-        if (copy_n) d.uks[on + k] = d.uks[o + k];
-    }
+        if (onuv[0] && onuv[1] && onuv[2] && onuv[3]) {
+            stg4(d.uks + o, ru[0], ru[1], ru[2], ru[3]);
+            stg4(d.vks + o, rv[0], rv[1], rv[2], rv[3]);
+        } else {
+#pragma unroll
+            for (int u4 = 0; u4 < 4; ++u4)
+                if (onuv[u4]) { d.uks[o + u4] = ru[u4]; d.vks[o + u4] = rv[u4]; }
+        }
+        if (onw[0] && onw[1] && onw[2] && onw[3]) stg4(d.wks + o, rw[0], rw[1], rw[2], rw[3]);
+        else
+#pragma unroll
+            for (int u4 = 0; u4 < 4; ++u4) if (onw[u4]) d.wks[o + u4] = rw[u4];
+        // UKS[i][1] = UKS[i][2] and UKS[i][NJJ] = UKS[i][NJJ-1] (if ADVDOM there) over
+        // KSTART..VI: the new value where it was just written, else the old one
+        const bool cs = (j == 2), cn = (j == d.njj - 1) && copy_n;
+        if (cs || cn) {
+            const size_t oc = cs ? c3(d, i, 1) : c3(d, i, d.njj);
+#pragma unroll
+            for (int u4 = 0; u4 < 4; ++u4)
+                if (on[u4]) d.uks[oc + k0 + u4] = onuv[u4] ? ru[u4] : d.uks[o + u4];
+        }
     }
 }
 
@@ -1239,17 +1326,6 @@ __device__ __forceinline__ void lds_barrier() {
 // same relative level hit different banks
 __device__ __forceinline__ int lds_stride(int nkp) { return nkp | 1; }
 
-// column metadata from the per-column table (see gral_pf_prepare)
-struct Meta {
-    int KK, VI, KSTART, KKmi, KKmj;
-    bool adv, vegp;
-};
-__device__ __forceinline__ Meta unpack_meta(int4 m) {
-    Meta r;
-    r.KK = m.x; r.VI = m.y; r.KSTART = m.z & 0xffff; r.adv = (m.z >> 16) & 1; r.vegp = (m.z >> 17) & 1;
-    r.KKmi = m.w & 0xffff; r.KKmj = (m.w >> 16) & 0xffff;
-    return r;
-}
 #define COEF_COLS 32
 #define SCOL 16           // words of column data per column in k_ml_coef
 // All neighbour-independent work of one mixing-length sweep for every column
@@ -2755,8 +2831,8 @@ static int launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, in
         L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(1, njj - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
         L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ns, dim3(1, nii - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
     }
-    const int ncol8 = ((ncols + COLS_PER_WAVE - 1) / COLS_PER_WAVE + NXCD - 1) / NXCD * NXCD;  // x extent of the column kernels
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
+    const int nrow8 = (nii - 2 + NXCD - 1) / NXCD * NXCD;                 // x extent of the cell kernels (rows)
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_divergence, dim3(nrow8, 1, Z), dim3(CELL_THREADS), 0, s, dd); });
     size_t nrow = (size_t)nj2 * nkp;
     for (int q = 0; q < nsit; ++q) nrow = max(nrow, (size_t)5 * hd[q].sync_stride);
     L.go(FAM_STENCIL, [&] {
@@ -2782,12 +2858,12 @@ static int launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, in
                 });
         }
     }
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_correction, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_correction, dim3(nrow8, 1, Z), dim3(CELL_THREADS), 0, s, dd); });
     if (proj) {
         if (launches) *launches += L.n;
         return (int)hipGetLastError();
     }
-    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_averages, dim3(ncol8, 1, Z), dim3(64), 0, s, dd); });
+    L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_averages, dim3(nrow8, 1, Z), dim3(CELL_THREADS), 0, s, dd); });
     L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_avg_edges, dim3(njj - 2, 1, Z), dim3(64), 0, s, dd); });
     size_t shm = sizeof(float) * 5 * nkp;
     // per-diagonal sweeps: sweep ranges a in [oa, nii-1] (outer), b in [ia, njj-1]
