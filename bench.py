@@ -527,12 +527,15 @@ def main():
     if not args.no_cpu_baseline and model == 1:   # the timing mode threads the mixing-length sweeps only
         from oracle import pyoracle
         c0 = domain.config2(direction=270.0, iterations=args.cpu_iterations, n=args.n)
+        # the AVX2 build evaluates the coefficient lanes 8 at a time (Vector<float>), bit for bit the scalar build
+        simd = pyoracle.simd_available()
+        lib = pyoracle.SIMD_PATH if simd else None
 
         def loop_time(iters, threads):
             # the SIMPLE loop alone: a run of `iters` iterations minus a run of none (the prologue)
-            t = time.perf_counter(); pyoracle.run(c0, iterations=0, keep_history=False, threads=threads)
+            t = time.perf_counter(); pyoracle.run(c0, iterations=0, keep_history=False, threads=threads, lib_path=lib)
             t_pro = time.perf_counter() - t
-            t = time.perf_counter(); pyoracle.run(c0, iterations=iters, keep_history=False, threads=threads)
+            t = time.perf_counter(); pyoracle.run(c0, iterations=iters, keep_history=False, threads=threads, lib_path=lib)
             return time.perf_counter() - t - t_pro
 
         # the host share of one GPU on the box is 16 cores (os.cpu_count() may show the whole machine)
@@ -541,16 +544,17 @@ def main():
         reps = sorted(loop_time(n_par, threads) for _ in range(5))
         t_par = reps[2]
         t_one = loop_time(args.cpu_iterations, 1)
+        width = ("8-wide AVX2 vectors for the coefficient lanes (the reference's Vector<float> width on AVX2 "
+                 "hosts), scalar recurrences as in the reference" if simd else
+                 "scalar C (no AVX2 on this host): the reference vectorises the coefficient lanes 8-wide")
         cpu = {"value": c0.active_cells() * n_par / t_par, "unit": "cell-iter/s", "cores": threads, "kind": "port",
+               "simd": "avx2-8" if simd else "scalar",
                "sample": f"{n_par} SIMPLE iterations of the 270 deg situation, median of 5 runs "
                          f"({', '.join(f'{x:.2f}' for x in reps)} s), the C restatement run with the reference's "
                          f"default multi-threaded schedule on {threads} threads, the host share of one GPU (rows of "
-                         f"each sweep split across threads, U and V concurrent), prologue excluded.  Scalar C: the "
-                         f"reference vectorises the coefficient assembly with Vector<float> (8 lanes), which this "
-                         f"restatement does not, so the reference's own CPU path would be faster by up to that "
-                         f"factor on the vectorised part",
+                         f"each sweep split across threads, U and V concurrent), prologue excluded; {width}",
                "single_core": {"value": c0.active_cells() * args.cpu_iterations / t_one, "cores": 1,
-                               "sample": f"{args.cpu_iterations} iterations, deterministic single-thread restatement "
+                               "sample": f"{args.cpu_iterations} iterations on one thread, same build "
                                          f"({t_one:.1f} s)"}}
 
     line = {"metric": "SIMPLE cell-iters/s", "value": value, "unit": "cell-iter/s", "n_gpus": 1,

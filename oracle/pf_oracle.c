@@ -337,6 +337,13 @@ static void averages(St *s) {
  * with the block-level conditions of the vector code, then the lanes not yet
  * consumed enter the Thomas recurrence (U-prognostic-microscale_1.cs:194-511). */
 typedef void (*lane_fn)(St *s, int i, int j, int k, int kk, float *A, float *B, float *C, float *D, void *ctx);
+typedef struct { float UG; } uctx;   /* UG (U) or VG (V) */
+#if defined(ORC_SIMD) && ORC_MUTANT == 0
+static void u1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, float *pC, float *pD, void *vctx);
+static void v1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, float *pC, float *pD, void *vctx);
+static void ml_block(St *s, int i, int j, int k, lane_fn fn, void *ctx, float *A, float *B, float *C, float *D);
+static int use_blocks;
+#endif
 
 static void ml_column(St *s, int i, int j, lane_fn fn, void *ctx) {
     int W = s->W;
@@ -351,6 +358,10 @@ static void ml_column(St *s, int i, int j, lane_fn fn, void *ctx) {
         k_real = k;
         if (k > KEND - W) { k = KEND - W; end = true; }
         const int used0 = end ? k_real - k : 0;     /* lanes below it were consumed by the previous block */
+#if defined(ORC_SIMD) && ORC_MUTANT == 0
+        if (use_blocks && W == 8 && !s->cap) ml_block(s, i, j, k, fn, ctx, A, B, C, D);
+        else
+#endif
         for (int l = 0; l < W; ++l) {
             s->rec = l >= used0 && k + l < KEND;
             fn(s, i, j, k, k + l, &A[l], &B[l], &C[l], &D[l], ctx);
@@ -429,8 +440,6 @@ static inline float ml_nb(bool fused, float aw, float xw, float as, float xs, fl
     if (fused) return fmaf(aw, xw, as * xs) + fmaf(ae, xe, an * xn);
     return aw * xw + as * xs + ae * xe + an * xn;
 }
-
-typedef struct { float UG; } uctx;
 
 /* the checking hook's record of one level's conductances, fluxes and coefficients */
 static void ml_pieces(St *s, int i, int j, int kk, float VIS, float DE, float DW, float DS, float DN, float DT,
@@ -737,6 +746,298 @@ static void w1_lane(St *s, int i, int j, int k, int kk, float *pA, float *pB, fl
     }
     *pA = AIM; *pB = BIM; *pC = CIM; *pD = DIM;
 }
+
+#if defined(ORC_SIMD) && ORC_MUTANT == 0
+/* ---- 8-wide blocks (the Vector<float> form, for the CPU timing build) -----
+ * The same lane arithmetic as u1_lane / v1_lane / w1_lane for all 8 lanes of
+ * a block start k at once, in AVX2 vectors: every vector operation is the
+ * lane-wise IEEE operation of the scalar form, in the same order (no FMA:
+ * -ffp-contract=off), maxps/minps are Vector.Max/Min (a > b ? a : b), and the
+ * division and square root are correctly rounded, so each lane is bit for bit
+ * the scalar lane (tests/test_oracle_simd.py).  Width 8 only, no checking
+ * hook: ml_column falls back to the lanes otherwise. */
+#include <immintrin.h>
+typedef float v8f __attribute__((vector_size(32)));
+typedef int v8i __attribute__((vector_size(32)));
+static inline v8f ld8(const float *p) { v8f v; memcpy(&v, p, sizeof v); return v; }
+static inline v8f bc8(float x) { v8f v = {x, x, x, x, x, x, x, x}; return v; }
+static inline v8f max8(v8f a, v8f b) { return (v8f)_mm256_max_ps((__m256)a, (__m256)b); }
+static inline v8f min8(v8f a, v8f b) { return (v8f)_mm256_min_ps((__m256)a, (__m256)b); }
+static inline v8f sqrt8(v8f a) { return (v8f)_mm256_sqrt_ps((__m256)a); }
+static inline v8f abs8(v8f a) { return (v8f)((v8i)a & 0x7fffffff); }
+static inline v8f sel8(v8i m, v8f a, v8f b) { return (v8f)((m & (v8i)a) | (~m & (v8i)b)); }
+static inline v8i lanes8(int k) { v8i v = {k, k + 1, k + 2, k + 3, k + 4, k + 5, k + 6, k + 7}; return v; }
+static inline v8f pow8(v8f Dv, v8f Pv) {            /* ml_pow: D*max(0,(1-0.1P)^5) */
+    v8f in = 1.0f - 0.1f * Pv;
+    return Dv * max8(bc8(0.0f), in * in * in * in * in);
+}
+static inline v8f coef8(v8f Dv, v8f Pv, v8f Fpos) { return pow8(Dv, Pv) + Fpos; }
+
+typedef struct { v8f A, B, C, D; } blk8;
+
+/* eddy viscosity of the block (ml_vis) */
+static v8f vis8(St *s, int i, int j, int k, v8f maskf, bool mask_v_enable) {
+    const int KK = KK(s, i, j), KKP1 = KK + 1;
+    if (!(k + 8 > KKP1)) return bc8(0.0f);
+    v8f DZ = ld8(s->DZK + k);
+    v8f zs = ld8(s->HOKART + k) - s->HOKART[KK] - DZ * 0.5f;
+    v8f mixL = min8(0.071f * zs, bc8(100.0f));
+    float COV = s->COV[IX2(s, i, j)];
+    if (COV > 0) mixL *= (1.0f - 0.99f * COV);
+    const float *UK = COL(s, UK, i, j), *VK = COL(s, VK, i, j);
+    v8f in = (ld8(UK + k + 1) - ld8(UK + k - 1)) / (2.0f * DZ);
+    v8f DUDZ = in * in;
+    in = (ld8(VK + k + 1) - ld8(VK + k - 1)) / (2.0f * DZ);
+    v8f DVDZ = in * in;
+    v8f v = min8(mixL * mixL * sqrt8(0.5f * (DUDZ + DVDZ)), bc8(15.0f));
+    return mask_v_enable ? v * maskf : v;
+}
+
+/* diffusion conductances, fluxes and power-law coefficients shared by the
+ * three components once FE..FB are known */
+static void coeffs8(St *s, int i, int j, int k, v8f VIS, v8f DZ, v8f FE, v8f FW, v8f FS, v8f FN, v8f FT, v8f FB,
+                    v8f maskf, bool mask_v_enable, v8f *AE1, v8f *AW1, v8f *AS1, v8f *AN1, v8f *BIM, v8f *CIM) {
+    const int KKP1 = KK(s, i, j) + 1;
+    const float DXK = s->DXK, DYK = s->DYK, AREAxy = s->AREAxy;
+    v8f DXKDZK = DXK * DZ, DYKDZK = DYK * DZ;
+    v8f mv = max8(VIS, bc8(s->VISHMIN));
+    v8f DE = mv * DYKDZK / DXK, DW = DE;
+    v8f DS = mv * DXKDZK / DYK, DN = DS;
+    v8f DT = mv * AREAxy / DZ;
+    v8f DB = bc8(0.0f);
+    if (mask_v_enable) DB = DT * maskf;
+    else if (k > KKP1) DB = DT;
+    if ((ADV(s, i, j - 1) < 1) || (j == 2)) DS = bc8(0.0f);
+    if ((ADV(s, i, j + 1) < 1) || (j == s->NJJ - 1)) DN = bc8(0.0f);
+    if ((ADV(s, i - 1, j) < 1) || (i == 2)) DW = bc8(0.0f);
+    if ((ADV(s, i + 1, j) < 1) || (i == s->NII - 1)) DE = bc8(0.0f);
+    const v8f eps = bc8(0.0001f), zero = bc8(0.0f);
+    DE = max8(DE, eps); DB = max8(DB, eps); DW = max8(DW, eps);
+    DS = max8(DS, eps); DN = max8(DN, eps); DT = max8(DT, eps);
+    v8f PE = abs8(FE / DE), PB = abs8(FB / DB), PW = abs8(FW / DW);
+    v8f PS = abs8(FS / DS), PN = abs8(FN / DN), PT = abs8(FT / DT);
+    *BIM = coef8(DT, PT, max8(-FT, zero));
+    *CIM = coef8(DB, PB, max8(FB, zero));
+    *AE1 = coef8(DE, PE, max8(-FE, zero));
+    *AW1 = coef8(DW, PW, max8(FW, zero));
+    *AS1 = coef8(DS, PS, max8(FS, zero));
+    *AN1 = coef8(DN, PN, max8(-FN, zero));
+}
+
+/* the U/V wall term of the block holding KKART+1 (lane KKART+1 only) */
+static v8f wall8(St *s, int i, int j, int k, v8f X, v8f AREAv, const float *UKS_W, const float *UKS, const float *VKS_W,
+                 const float *VKS) {
+    const int KKP1 = KK(s, i, j) + 1;
+    v8f m2 = sel8(lanes8(k) == KKP1, bc8(1.0f), bc8(0.0f));
+    v8f in = 0.5f * (ld8(UKS_W + k) + ld8(UKS + k));
+    v8f in2 = 0.5f * (ld8(VKS_W + k) + ld8(VKS + k));
+    v8f wh = max8(sqrt8(in * in + in2 * in2), bc8(0.1f));
+    int ii = KKP1 - k;
+    float inI = 0.5f * (UKS_W[KKP1] + UKS[KKP1]);
+    float in2I = 0.5f * (VKS_W[KKP1] + VKS[KKP1]);
+    float whI = vmax(sqrtf(inI * inI + in2I * in2I), 0.1f);
+    float Us;
+    if (s->CUTK[IX2(s, i, j)] < 1) {
+        Us = s->HT[IX2(s, i, j)] * whI;
+        if (s->Z0S[IX2(s, i, j)] >= s->DZK[k + ii] * 0.1f) {
+            int ind = k + ii + 1;
+            Us = s->HT[IX2(s, i, j)] * sqrtf(p2f(0.5f * (UKS_W[ind] + UKS[ind])) + p2f(0.5f * (VKS_W[ind] + VKS[ind])));
+        }
+    } else {
+        Us = s->HO[IX2(s, i, j)] * whI;
+    }
+    return X / wh * (Us * Us) * AREAv * m2;
+}
+
+static blk8 u1_block(St *s, int i, int j, int k, float UG) {
+    const int KK = KK(s, i, j), KKP1 = KK + 1;
+    const int iM1 = i - 1, iP1 = i + 1, jM1 = j - 1, jP1 = j + 1;
+    const float DXK = s->DXK, DYK = s->DYK, AREAxy = s->AREAxy;
+    v8f DZ = ld8(s->DZK + k), DXKDZK = DXK * DZ, DYKDZK = DYK * DZ;
+    const bool mask_v_enable = (KKP1 >= k && KKP1 < k + 8);
+    v8f maskf = sel8(lanes8(k) > KKP1, bc8(1.0f), bc8(0.0f));
+    const float *UK = COL(s, UK, i, j) + k, *UKS = COL(s, UKS, i, j) + k, *VKS = COL(s, VKS, i, j) + k;
+    const float *WKS = COL(s, WKS, i, j) + k, *DPMNEW = COL(s, DPMNEW, i, j) + k;
+    const float *UK_W = COL(s, UK, iM1, j) + k, *UK_E = COL(s, UK, iP1, j) + k;
+    const float *UK_S = COL(s, UK, i, jM1) + k, *UK_N = COL(s, UK, i, jP1) + k, *UKS_W = COL(s, UKS, iM1, j) + k;
+    const float *VKS_W = COL(s, VKS, iM1, j) + k, *VKS_WS = COL(s, VKS, iM1, jM1) + k, *VKS_S = COL(s, VKS, i, jM1) + k;
+    const float *VKS_WN = COL(s, VKS, iM1, jP1) + k, *VKS_N = COL(s, VKS, i, jP1) + k, *WKS_W = COL(s, WKS, iM1, j) + k;
+    const float *DPMNEW_W = COL(s, DPMNEW, iM1, j) + k, *VKS_EN = COL(s, VKS, iP1, jP1) + k;
+    const float *VKS_ES = COL(s, VKS, iP1, jM1) + k;
+    const float *WKS_E = COL(s, WK, iP1, j) + k;      /* WK, not WKS (quirk) */
+    v8f VIS = vis8(s, i, j, k, maskf, mask_v_enable);
+    v8f FE = ld8(UKS) * DYKDZK;
+    v8f FW = ld8(UKS_W) * DYKDZK;
+    v8f FS = 0.25f * (ld8(VKS_W) + ld8(VKS) + ld8(VKS_WS) + ld8(VKS_S)) * DXKDZK;
+    v8f FN = 0.25f * (ld8(VKS_W) + ld8(VKS) + ld8(VKS_WN) + ld8(VKS_N)) * DXKDZK;
+    v8f FT = 0.25f * (ld8(WKS_W) + ld8(WKS) + ld8(WKS_W + 1) + ld8(WKS + 1)) * AREAxy;
+    v8f FB = bc8(0.0f);
+    if (mask_v_enable) FB = 0.25f * (ld8(WKS_W) + ld8(WKS) + ld8(WKS_W - 1) + ld8(WKS - 1)) * AREAxy * maskf;
+    else if (k > KKP1) FB = 0.25f * (ld8(WKS_W) + ld8(WKS) + ld8(WKS_W - 1) + ld8(WKS - 1)) * AREAxy;
+    v8f AE1, AW1, AS1, AN1, BIM, CIM;
+    coeffs8(s, i, j, k, VIS, DZ, FE, FW, FS, FN, FT, FB, maskf, mask_v_enable, &AE1, &AW1, &AS1, &AN1, &BIM, &CIM);
+    v8f AP0 = ld8(s->AP0 + k);
+    v8f AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + AP0;
+    v8f DDPX = ld8(DPMNEW_W) - ld8(DPMNEW);
+    v8f UKv = ld8(UK);
+    v8f DIM;
+    if (s->VEGP && s->VEGP[IX2(s, i, j)]) {
+        v8f in2 = 0.5f * (ld8(VKS_W) + ld8(VKS));
+        v8f in = 0.5f * (ld8(UKS_W) + ld8(UKS));
+        v8f wh = max8(sqrt8(in * in + in2 * in2), bc8(0.01f));
+        v8f VEGv = ld8(s->VEG + IX3(s, i, j, k - 1));
+        DIM = AW1 * ld8(UK_W) + AS1 * ld8(UK_S) + AE1 * ld8(UK_E) + AN1 * ld8(UK_N) +
+              AP0 * 0.5f * (ld8(UKS_W) + ld8(UKS)) + DDPX * DYKDZK + (s->Cor * (UG - UKv) - VEGv * UKv * wh) * AREAxy * DZ;
+    } else {
+        DIM = AW1 * ld8(UK_W) + AS1 * ld8(UK_S) + AE1 * ld8(UK_E) + AN1 * ld8(UK_N) +
+              AP0 * 0.5f * (ld8(UKS_W) + ld8(UKS)) + DDPX * DYKDZK + s->Cor * (UG - UKv) * AREAxy * DZ;
+    }
+    if ((k <= KKP1) && ((k + 8) > KKP1))
+        DIM -= wall8(s, i, j, k, UKv, bc8(AREAxy), UKS_W - k, UKS - k, VKS_W - k, VKS - k);
+    if (k > KKP1) {
+        v8f DUDXE = (ld8(UK_E) - UKv) * DYKDZK;
+        v8f DUDXW = (UKv - ld8(UK_W)) * DYKDZK;
+        v8f DVDXN = (0.5f * (ld8(VKS_EN) + ld8(VKS_N)) - 0.5f * (ld8(VKS_N) + ld8(VKS_WN))) * DXKDZK;
+        v8f DVDXS = (0.5f * (ld8(VKS_ES) + ld8(VKS_S)) - 0.5f * (ld8(VKS_S) + ld8(VKS_WS))) * DXKDZK;
+        v8f DWDXT = (0.5f * (ld8(WKS_E) + ld8(WKS)) - 0.5f * (ld8(WKS) + ld8(WKS_W))) * AREAxy;
+        v8f DWDXB = (0.5f * (ld8(WKS_E - 1) + ld8(WKS - 1)) - 0.5f * (ld8(WKS - 1) + ld8(WKS_W - 1))) * AREAxy;
+        v8f cross = VIS / DXK * (DUDXE - DUDXW + DVDXN - DVDXS + DWDXT - DWDXB);
+        if (mask_v_enable) cross = cross * maskf;
+        DIM += cross;
+    }
+    blk8 r = {AIM, BIM, CIM, DIM};
+    return r;
+}
+
+static blk8 v1_block(St *s, int i, int j, int k, float VG) {
+    const int KK = KK(s, i, j), KKP1 = KK + 1;
+    const int iM1 = i - 1, iP1 = i + 1, jM1 = j - 1, jP1 = j + 1;
+    const float DXK = s->DXK, DYK = s->DYK, AREAxy = s->AREAxy;
+    v8f DZ = ld8(s->DZK + k), DXKDZK = DXK * DZ, DYKDZK = DYK * DZ;
+    const bool mask_v_enable = (KKP1 >= k && KKP1 < k + 8);
+    v8f maskf = sel8(lanes8(k) > KKP1, bc8(1.0f), bc8(0.0f));
+    const float *VK = COL(s, VK, i, j) + k, *UKS = COL(s, UKS, i, j) + k, *VKS = COL(s, VKS, i, j) + k;
+    const float *WKS = COL(s, WKS, i, j) + k, *DPMNEW = COL(s, DPMNEW, i, j) + k;
+    const float *VK_W = COL(s, VK, iM1, j) + k, *VK_E = COL(s, VK, iP1, j) + k;
+    const float *VK_S = COL(s, VK, i, jM1) + k, *VK_N = COL(s, VK, i, jP1) + k;
+    const float *UKS_W = COL(s, UKS, iM1, j) + k, *UKS_E = COL(s, UKS, iP1, j) + k, *UKS_S = COL(s, UKS, i, jM1) + k;
+    const float *UKS_ES = COL(s, UKS, iP1, jM1) + k, *UKS_WS = COL(s, UKS, iM1, jM1) + k;
+    const float *VKS_S = COL(s, VKS, i, jM1) + k, *VKS_W = COL(s, VKS, iM1, j) + k, *WKS_S = COL(s, WKS, i, jM1) + k;
+    const float *DPMNEW_S = COL(s, DPMNEW, i, jM1) + k, *UKS_EN = COL(s, UKS, iP1, jP1) + k;
+    const float *UKS_WN = COL(s, UKS, iM1, jP1) + k;
+    const float *WKS_N = COL(s, WK, i, jP1) + k;      /* WK, not WKS (quirk) */
+    v8f VIS = vis8(s, i, j, k, maskf, mask_v_enable);
+    v8f FE = 0.25f * (ld8(UKS) + ld8(UKS_E) + ld8(UKS_S) + ld8(UKS_ES)) * DYKDZK;
+    v8f FW = 0.25f * (ld8(UKS) + ld8(UKS_W) + ld8(UKS_S) + ld8(UKS_WS)) * DYKDZK;
+    v8f FS = ld8(VKS_S) * DXKDZK;
+    v8f FN = ld8(VKS) * DXKDZK;
+    v8f FT = 0.25f * (ld8(WKS_S) + ld8(WKS) + ld8(WKS_S + 1) + ld8(WKS + 1)) * AREAxy;
+    v8f FB = bc8(0.0f);
+    if (mask_v_enable) FB = 0.25f * (ld8(WKS_S) + ld8(WKS) + ld8(WKS_S - 1) + ld8(WKS - 1)) * AREAxy * maskf;
+    else if (k > KKP1) FB = 0.25f * (ld8(WKS_S) + ld8(WKS) + ld8(WKS_S - 1) + ld8(WKS - 1)) * AREAxy;
+    v8f AE1, AW1, AS1, AN1, BIM, CIM;
+    coeffs8(s, i, j, k, VIS, DZ, FE, FW, FS, FN, FT, FB, maskf, mask_v_enable, &AE1, &AW1, &AS1, &AN1, &BIM, &CIM);
+    v8f AP0 = ld8(s->AP0 + k);
+    v8f AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + AP0;
+    v8f DDPY = ld8(DPMNEW_S) - ld8(DPMNEW);
+    v8f VKv = ld8(VK);
+    v8f DIM;
+    if (s->VEGP && s->VEGP[IX2(s, i, j)]) {
+        v8f in2 = 0.5f * (ld8(VKS_W) + ld8(VKS));
+        v8f in = 0.5f * (ld8(UKS_W) + ld8(UKS));
+        v8f wh = max8(sqrt8(in * in + in2 * in2), bc8(0.01f));
+        v8f VEGv = ld8(s->VEG + IX3(s, i, j, k - 1));
+        DIM = AW1 * ld8(VK_W) + AS1 * ld8(VK_S) + AE1 * ld8(VK_E) + AN1 * ld8(VK_N) +
+              AP0 * 0.5f * (ld8(VKS_S) + ld8(VKS)) + DDPY * DXKDZK + (s->Cor * (VG - VKv) - VEGv * VKv * wh) * AREAxy * DZ;
+    } else {
+        DIM = AW1 * ld8(VK_W) + AS1 * ld8(VK_S) + AE1 * ld8(VK_E) + AN1 * ld8(VK_N) +
+              AP0 * 0.5f * (ld8(VKS_S) + ld8(VKS)) + DDPY * DXKDZK + s->Cor * (VG - VKv) * AREAxy * DZ;
+    }
+    if ((k <= KKP1) && ((k + 8) > KKP1))
+        DIM -= wall8(s, i, j, k, VKv, bc8(AREAxy), UKS_W - k, UKS - k, VKS_W - k, VKS - k);
+    if (k > KKP1) {
+        v8f DVDYN = (ld8(VK_N) - VKv) * DXKDZK;
+        v8f DVDYS = (VKv - ld8(VK_S)) * DXKDZK;
+        v8f DUDYE = (0.5f * (ld8(UKS_EN) + ld8(UKS_E)) - 0.5f * (ld8(UKS_E) + ld8(UKS_ES))) * DXKDZK;
+        v8f DUDYW = (0.5f * (ld8(UKS_WN) + ld8(UKS_W)) - 0.5f * (ld8(UKS_W) + ld8(UKS_WS))) * DXKDZK;
+        v8f DWDYT = (0.5f * (ld8(WKS_N) + ld8(WKS)) - 0.5f * (ld8(WKS) + ld8(WKS_S))) * AREAxy;
+        v8f DWDYB = (0.5f * (ld8(WKS_N - 1) + ld8(WKS - 1)) - 0.5f * (ld8(WKS - 1) + ld8(WKS_S))) * AREAxy;
+        v8f cross = VIS / DXK * (DUDYE - DUDYW + DVDYN - DVDYS + DWDYT - DWDYB);
+        if (mask_v_enable) cross = cross * maskf;
+        DIM += cross;
+    }
+    blk8 r = {AIM, BIM, CIM, DIM};
+    return r;
+}
+
+static blk8 w1_block(St *s, int i, int j, int k) {
+    const int KK = KK(s, i, j), KKP1 = KK + 1;
+    const int iM1 = i - 1, iP1 = i + 1, jM1 = j - 1, jP1 = j + 1;
+    const float DXK = s->DXK, DYK = s->DYK, AREAxy = s->AREAxy;
+    v8f DZ = ld8(s->DZK + k), DXKDZK = DXK * DZ, DYKDZK = DYK * DZ;
+    const bool mask_v_enable = (KKP1 >= k && KKP1 < k + 8);
+    v8f maskf = sel8(lanes8(k) > KKP1, bc8(1.0f), bc8(0.0f));
+    const float *WK = COL(s, WK, i, j) + k, *WK_W = COL(s, WK, iM1, j) + k, *WK_E = COL(s, WK, iP1, j) + k;
+    const float *WK_S = COL(s, WK, i, jM1) + k, *WK_N = COL(s, WK, i, jP1) + k;
+    const float *UKS = COL(s, UKS, i, j) + k, *VKS = COL(s, VKS, i, j) + k, *WKS = COL(s, WKS, i, j) + k;
+    const float *DPMNEW = COL(s, DPMNEW, i, j) + k;
+    const float *UKS_W = COL(s, UKS, iM1, j) + k, *UKS_E = COL(s, UKS, iP1, j) + k;
+    const float *VKS_S = COL(s, VKS, i, jM1) + k, *VKS_N = COL(s, VKS, i, jP1) + k;
+    v8f VIS = vis8(s, i, j, k, maskf, mask_v_enable);
+    v8f FE = 0.25f * (ld8(UKS) + ld8(UKS_E) + ld8(UKS - 1) + ld8(UKS_E - 1)) * DYKDZK;
+    v8f FW = 0.25f * (ld8(UKS) + ld8(UKS_W) + ld8(UKS - 1) + ld8(UKS_W - 1)) * DYKDZK;
+    v8f FS = 0.25f * (ld8(VKS) + ld8(VKS_S) + ld8(VKS - 1) + ld8(VKS_S - 1)) * DXKDZK;
+    v8f FN = 0.25f * (ld8(VKS) + ld8(VKS_N) + ld8(VKS - 1) + ld8(VKS_N - 1)) * DXKDZK;
+    v8f FT = ld8(WKS) * AREAxy;
+    v8f FB = bc8(0.0f);
+    if (mask_v_enable) FB = ld8(WKS - 1) * AREAxy * maskf;
+    else if (k > KKP1) FB = ld8(WKS - 1) * AREAxy;
+    v8f AE1, AW1, AS1, AN1, BIM, CIM;
+    coeffs8(s, i, j, k, VIS, DZ, FE, FW, FS, FN, FT, FB, maskf, mask_v_enable, &AE1, &AW1, &AS1, &AN1, &BIM, &CIM);
+    v8f AP0 = ld8(s->AP0 + k);
+    v8f AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + AP0;
+    v8f DDPZ = ld8(DPMNEW - 1) - ld8(DPMNEW);
+    v8f WKv = ld8(WK);
+    v8f DIM;
+    if (s->VEGP && s->VEGP[IX2(s, i, j)]) {
+        v8f in2 = 0.5f * (ld8(VKS_S) + ld8(VKS));
+        v8f in = 0.5f * (ld8(UKS_W) + ld8(UKS));
+        v8f wh = max8(sqrt8(in * in + in2 * in2), bc8(0.01f));
+        v8f VEGv = ld8(s->VEG + IX3(s, i, j, k - 1));
+        DIM = AW1 * ld8(WK_W) + AS1 * ld8(WK_S) + AE1 * ld8(WK_E) + AN1 * ld8(WK_N) +
+              AP0 * 0.5f * (ld8(WKS - 1) + ld8(WKS)) + DDPZ * AREAxy - VEGv * WKv * wh * AREAxy * DZ;
+    } else {
+        DIM = AW1 * ld8(WK_W) + AS1 * ld8(WK_S) + AE1 * ld8(WK_E) + AN1 * ld8(WK_N) +
+              AP0 * 0.5f * (ld8(WKS - 1) + ld8(WKS)) + DDPZ * AREAxy;
+    }
+    if (k > KKP1) {
+        v8f DUDZE = (0.5f * (ld8(UKS + 1) + ld8(UKS)) - 0.5f * (ld8(UKS) + ld8(UKS - 1))) * DYKDZK;
+        v8f DUDZW = (0.5f * (ld8(UKS_W + 1) + ld8(UKS_W)) - 0.5f * (ld8(UKS_W) + ld8(UKS_W - 1))) * DYKDZK;
+        v8f DVDZN = (0.5f * (ld8(VKS + 1) + ld8(VKS)) - 0.5f * (ld8(VKS) + ld8(VKS - 1))) * DXKDZK;
+        v8f DVDZS = (0.5f * (ld8(VKS_S + 1) + ld8(VKS_S)) - 0.5f * (ld8(VKS_S) + ld8(VKS_S - 1))) * DXKDZK;
+        v8f DWDZT = (ld8(WK + 1) - WKv) * AREAxy;
+        v8f DWDZB = (WKv - ld8(WK - 1)) * AREAxy;
+        v8f cross = VIS / DZ * (DUDZE - DUDZW + DVDZN - DVDZS + DWDZT - DWDZB);
+        if (mask_v_enable) cross = cross * maskf;
+        DIM += cross;
+    }
+    blk8 r = {AIM, BIM, CIM, DIM};
+    return r;
+}
+
+/* the 8 lanes of block k of the sweep whose lane function is fn */
+static void ml_block(St *s, int i, int j, int k, lane_fn fn, void *ctx, float *A, float *B, float *C, float *D) {
+    blk8 r;
+    if (fn == u1_lane) r = u1_block(s, i, j, k, ((uctx *)ctx)->UG);
+    else if (fn == v1_lane) r = v1_block(s, i, j, k, ((uctx *)ctx)->UG);
+    else r = w1_block(s, i, j, k);
+    memcpy(A, &r.A, sizeof r.A); memcpy(B, &r.B, sizeof r.B); memcpy(C, &r.C, sizeof r.C); memcpy(D, &r.D, sizeof r.D);
+}
+
+static int use_blocks = 1;
+/* 0: the timing build evaluates lanes one by one like the default build */
+void orc_simd_blocks(int on) { use_blocks = on != 0; }
+#endif
 
 /* U_PrognosticMicroscaleV1.Calculate(IS, JS): i1 in [3,NII-1], j1 in [2,NJJ-1] */
 static void u1_row(St *s, int IS, int JS, int i1);

@@ -114,6 +114,18 @@ def _configure(_lib):
     return _lib
 
 
+SIMD_PATH = os.path.join(HERE, "liboracle_pf_simd.so")
+
+
+def simd_available() -> bool:
+    """The AVX2 timing build exists and this CPU runs AVX2 code."""
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return False
+    return os.path.exists(SIMD_PATH) and " avx2" in flags and " fma" in flags
+
+
 def load64(build_if_missing: bool = True):
     """The FP64 shadow (liboracle_pf64.so): the same restatement with every
     float widened to double; for error statements only, never a checker."""
