@@ -273,7 +273,7 @@ def graph_timing(iters=100):
             "ms_per_iter_graphs": out["graphs"], "speedup": out["launches"] / out["graphs"]}
 
 
-def keps_timing(n, directions, steps=5, warmup=1):
+def keps_timing(n, directions, steps=5, warmup=12):
     """The same 8-direction city batch with the k-eps model (turbulence model
     2: k-eps momentum and the TKE/dissipation sweep), timed the same way as
     the main line: device time per SIMPLE iteration over ``steps`` iterations
@@ -306,7 +306,49 @@ def keps_timing(n, directions, steps=5, warmup=1):
             "kernel_families_ms_per_iter": {k: v["ms"] / 2 for k, v in prof.items()}}
 
 
-def batch_sweep(sizes, n, steps=6, warmup=2, log=sys.stderr):
+def convergence_study(log=sys.stderr):
+    """The reference's stop test on city situations: delta = (sum of DeltaUMax
+    over 100 iterations) / topwind^2 at each test (iterations 100..500;
+    PrognosticFlowfield.cs:1149-1166, stop below 0.012) for three situations of
+    the batched workload's generator on a 200 x 200 window and for cities of
+    2 %, 10 % and 30 % plan-area density (buildings 6-20 m) at 1, 3 and 6 m/s."""
+    from gral_amd import domain, pf
+    F32 = np.float32
+
+    def deltas(c):
+        c.iter_min, c.iter_max = 100, 500
+        ds = []
+        with pf.PrognosticFlowfield(c) as sv:
+            sv.prepare()
+            for _ in range(5):
+                st = sv.iterate(100, check_convergence=True)
+                ds.append(round(st["last_delta"], 4))
+                if st["iteration_loops"] % 100 != 1 or st["iteration_loops"] > 500:
+                    break
+        return {"iteration_loops": st["iteration_loops"], "delta_at_tests": ds}
+
+    out = []
+    for s in domain.config3_situations()[:3]:
+        s = dict(s, n=200)
+        row = {"case": f"batched workload situation {s['index']} on a 200x200 window", "speed": s["speed"],
+               "stability": s["stability"], "vertcellsff": s["vertcellsff"], **deltas(domain.config3(s))}
+        print(f"[convergence] {row}", file=log, flush=True)
+        out.append(row)
+    for density in (0.02, 0.1, 0.3):
+        cut = np.zeros((202, 202), dtype=F32)
+        for b in domain.city_buildings(200, 200, 2.0, 77, density=density, hmin=6.0, hmax=20.0):
+            sl = (slice(b.i0, b.i1 + 1), slice(b.j0, b.j1 + 1))
+            cut[sl] = np.maximum(cut[sl], F32(b.height))
+        for speed in (1.0, 3.0, 6.0):
+            c = domain.flat_case(f"d{density}", 200, 200, cut, vertcellsff=40, speed=speed, direction=270.0)
+            row = {"case": f"city 200x200, {int(100 * density)} % density, buildings 6-20 m", "speed": speed,
+                   **deltas(c)}
+            print(f"[convergence] {row}", file=log, flush=True)
+            out.append(row)
+    return out
+
+
+def batch_sweep(sizes, n, steps=6, warmup=12, log=sys.stderr):
     """Throughput against batch size on the city block: S situations (the 8
     directions repeated) advanced in lock step; per size the device time per
     iteration, cell-iterations/s and the W sweep's share of the HBM roofline
@@ -329,7 +371,7 @@ def batch_sweep(sizes, n, steps=6, warmup=2, log=sys.stderr):
         wfrac = ALG_BYTES["w"] * w["cells"] / (w["ms"] / w["launches"] / 1e3) / 1e9 / HBM_PEAK_GBS / w["launches"]
         row = {"situations": S, "ms_per_iter": tm["ms_total"] / steps,
                "cell_iter_per_s": tm["cell_iterations"] / (tm["ms_total"] / 1e3), "w_sweep_frac": wfrac,
-               "w_sweep_ms": w["ms"] / w["launches"]}
+               "w_sweep_ms": w["ms"] / w["launches"], "stream_groups": tm["groups"]}
         print(f"[batch-sweep] {row}", file=log, flush=True)
         out.append(row)
     return out
@@ -355,6 +397,10 @@ def stream_workload(k, capacity, plain, log=sys.stderr):
 
     done = [0]
     ci = [0]
+    groups = {}
+
+    def on_step(tm):
+        groups[tm["groups"]] = groups.get(tm["groups"], 0) + 1
 
     def on_done(case, sv, st):
         done[0] += 1
@@ -365,7 +411,7 @@ def stream_workload(k, capacity, plain, log=sys.stderr):
 
     t0 = time.perf_counter()
     order = sorted(sits, key=lambda s: -s["n"] * s["n"] * s["vertcellsff"])      # longest first
-    stats = pf.solve_stream([make(s) for s in order], capacity=capacity, on_done=on_done)
+    stats = pf.solve_stream([make(s) for s in order], capacity=capacity, on_done=on_done, on_step=on_step)
     t_stream = time.perf_counter() - t0
     loops = [st["iteration_loops"] for st in stats]
     runs = [st["iterations_run"] for st in stats]
@@ -375,7 +421,8 @@ def stream_workload(k, capacity, plain, log=sys.stderr):
            "iteration_loops_max": max(loops), "iterations_run_total": int(sum(runs)),
            "active_cells_total": int(tot_cells), "cell_iterations": int(ci[0]),
            "cell_iter_per_s_wall": ci[0] / t_stream,
-           "mass_imbalance_median": float(np.median(mass)) if mass else None}
+           "mass_imbalance_median": float(np.median(mass)) if mass else None,
+           "steps_by_stream_groups": {str(g): n for g, n in sorted(groups.items())}}
     if plain:
         # plain solves of a sample, one after another (set-up, prologue, solve,
         # download); extrapolated to the whole set by cell-iterations
@@ -402,7 +449,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=12,
+                    help="untimed iterations first (a batch times its stream-group counts in the first 12)")
     ap.add_argument("--n", type=int, default=400, help="grid cells per side (city block)")
     ap.add_argument("--directions", type=int, default=8, help="wind directions 0, 45, ... solved as one batch")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-situation plain solve")
@@ -422,6 +470,8 @@ def main():
                     help="also time the adjoint of the pressure-correction steps (forward window and VJP)")
     ap.add_argument("--diagnostic", action="store_true",
                     help="also time the diagnostic post-pass (gral_pf_diagnostic) on the 270 deg city")
+    ap.add_argument("--convergence-study", action="store_true",
+                    help="also report the reference's stop-test delta every 100 iterations on city situations")
     ap.add_argument("--batch-sweep", type=str, default="", metavar="S1,S2,...",
                     help="also time lock-step batches of these sizes on the city block")
     ap.add_argument("--stream", type=int, default=0, metavar="K",
@@ -477,6 +527,7 @@ def main():
     wall = time.perf_counter() - t0
     tm = batch.timing()
     ms = tm["ms_total"] / args.steps
+    stream_groups = tm["groups"]
     value = tm["cell_iterations"] / (tm["ms_total"] / 1e3)
 
     if args.no_family_profile:
@@ -565,7 +616,7 @@ def main():
                                    f"({', '.join(f'{d:g}' for d in directions)} deg) as one lock-step batch, "
                                    f"plain solver per situation, {'mixing-length' if model == 1 else 'k-eps'} model",
                        "situations": len(cases), "active_cells": tm["active_cells"], "kadvmax": cases[0].kadvmax,
-                       "parallelism": "single GPU"},
+                       "stream_groups": stream_groups, "parallelism": "single GPU"},
             "wall_s_timed": wall, "alg_bytes_per_cell_iter": alg_iter,
             "iteration_frac_of_hbm_roofline": value * alg_iter / (HBM_PEAK_GBS * 1e9),
             "roofline": roofline, "cpu_baseline": cpu, "plain_single_situation": plain,
@@ -586,6 +637,8 @@ def main():
         line["graph_replay"] = graph_timing()
     if args.batch_sweep:
         line["batch_sweep"] = batch_sweep([int(x) for x in args.batch_sweep.split(",")], args.n)
+    if args.convergence_study:
+        line["convergence_study"] = convergence_study()
     if args.stream:
         line["stream_workload"] = stream_workload(args.stream, args.capacity, args.stream_plain)
     print(json.dumps(line))

@@ -924,7 +924,19 @@ struct StreamGroup {
     hipEvent_t join[kMaxGroups - 1] = {};
     hipEvent_t fork = nullptr;
     int n = 0;                               // extra streams created
+    // online choice of the group count: for the set of candidate counts a batch
+    // of its size allows (tuned_for: how many), one timed chunk of
+    // kTrialIters iterations (one cycle of the four sweep directions) per
+    // candidate, then the fastest until that set changes
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    int tuned_for = -1, best = 0, trial = 0;
+    float trial_ms[kMaxGroups + 1] = {};
     void clear() {
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        t0 = t1 = nullptr;
+        tuned_for = -1;
+        best = trial = 0;
         for (int g = 0; g < n; ++g) {
             if (s[g]) (void)hipStreamSynchronize(s[g]);
             if (join[g]) (void)hipEventDestroy(join[g]);
@@ -951,20 +963,30 @@ struct StreamGroup {
     }
 };
 
-// groups a lock-step batch of n situations runs in: n / 2, at most 4 (one
-// stream per hardware queue), or GRAL_PF_GROUPS (1 to 4).  Measured on the
-// 8-direction city batch: 24.30 ms per iteration in one group, 23.28 in two,
-// 23.09 in four.
-static int batch_groups(size_t n) {
-    int g = (int)std::min<size_t>(n / 2, (size_t)kMaxGroups);
-    if (const char *e = std::getenv("GRAL_PF_GROUPS")) g = std::atoi(e);
-    g = std::max(1, std::min(g, kMaxGroups));
-    return (int)std::min<size_t>((size_t)g, std::max<size_t>(n, 1));
+// groups a lock-step batch of n situations may run in: at most n / 2 and 4
+// (one stream per hardware queue).  Measured on the 8-direction city batch:
+// 24.30 ms per iteration in one group, 23.28 in two, 23.09 in four.
+static int max_groups(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>(n / 2, (size_t)kMaxGroups)); }
+// GRAL_PF_GROUPS (1 to 4) fixes the count; 0 if unset
+static int forced_groups() {
+    const char *e = std::getenv("GRAL_PF_GROUPS");
+    if (!e) return 0;
+    return std::max(1, std::min(std::atoi(e), kMaxGroups));
+}
+// iterations of one trial of the group tuner: a whole cycle of the sweep
+// directions (iteration mod 4), whose sweeps differ in cost
+static const int kTrialIters = 4;
+// candidate counts the tuner tries for n situations: 1, 2, 4 up to max_groups(n)
+static int group_candidates(size_t n, int *c) {
+    int k = 0;
+    for (int g = 1; g <= max_groups(n); g *= 2) c[k++] = g;
+    return k;
 }
 
 struct RunResult {
     long long launches = 0, cell_iters = 0;
     float ms = 0;
+    int groups = 1;                          // stream groups of the last chunk
     std::string err;
     int code = 0;
 };
@@ -998,12 +1020,41 @@ static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipSt
             chunk = std::min(chunk, h->prm.iter_max - h->iter_loops + 1);
         }
         if (it0 < 1 || chunk < 1) return bad(GRAL_PF_ESTATE, "iterate: iteration counter out of range");
+        // the group count: forced, tuned, or on trial for this many situations
+        int want = 1;
+        bool trial = false;
+        if (sg) {
+            int cand[kMaxGroups + 1];
+            const int nc = group_candidates(act.size(), cand);
+            if (const int f = forced_groups()) {
+                want = std::min(f, max_groups(act.size()));
+            } else if (nc == 1) {
+                want = 1;
+            } else {
+                if (sg->tuned_for != nc) {
+                    sg->tuned_for = nc;
+                    sg->best = sg->trial = 0;
+                }
+                if (sg->best) {
+                    want = sg->best;
+                } else if (chunk >= kTrialIters) {       // the next candidate's timed cycle
+                    want = cand[sg->trial];
+                    trial = true;
+                    chunk = kTrialIters;
+                    if ((!sg->t0 && hipEventCreate(&sg->t0) != hipSuccess) || (!sg->t1 && hipEventCreate(&sg->t1) != hipSuccess))
+                        return bad(GRAL_PF_EDEVICE, "hipEventCreate failed");
+                } else {                                  // too short for a trial: the largest count
+                    want = cand[nc - 1];
+                }
+            }
+        }
+        if (want != ng) upload = true;
         // slots: the launch's residual slot minus iter_off is each situation's
         // own iteration - 1; with graphs the slot counts from the chunk's start
         if (upload || gc) {
             // group g takes situations g, g+G, g+2G, ... of act (act is ordered
             // largest first by the callers that care), stored contiguously
-            ng = sg ? batch_groups(act.size()) : 1;
+            ng = want;
             if (ng > 1 && !sg->ensure(ng - 1)) return bad(GRAL_PF_EDEVICE, "stream creation failed");
             hd.resize(act.size());
             size_t pos = 0;
@@ -1049,6 +1100,8 @@ static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipSt
         std::vector<unsigned long long> key;
         if (gc)
             for (auto *h : act) key.push_back(h->gen);
+        R.groups = ng;
+        if (trial && hipEventRecord(sg->t0, s) != hipSuccess) return bad(GRAL_PF_EDEVICE, "hipEventRecord failed");
         if (gc && gc->exec && gc->set == key && gc->dd == dd && gc->chunk == chunk && gc->it_mod4 == it0 % 4) {
             if (hipGraphLaunch(gc->exec, s) != hipSuccess) return bad(GRAL_PF_EDEVICE, "hipGraphLaunch failed");
             R.launches += gc->launches;
@@ -1075,6 +1128,27 @@ static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipSt
                 gc->chunk = chunk;
                 gc->it_mod4 = it0 % 4;
                 gc->launches = nl;
+            }
+        }
+        if (trial) {                                      // the trial's time; after the last, the fastest
+            float ms = 0;
+            if (hipEventRecord(sg->t1, s) != hipSuccess || hipEventSynchronize(sg->t1) != hipSuccess ||
+                hipEventElapsedTime(&ms, sg->t0, sg->t1) != hipSuccess)
+                return bad(GRAL_PF_EDEVICE, "event timing failed");
+            int cand[kMaxGroups + 1];
+            const int nc = group_candidates(act.size(), cand);
+            sg->trial_ms[sg->trial] = ms;
+            if (++sg->trial == nc) {
+                // the largest count within 2 % of the fastest: overlap is the
+                // rule, the trials guard against batches it slows down (many
+                // situations of unlike sizes); closer timings are within the
+                // spread of one cycle
+                float fastest = sg->trial_ms[0];
+                for (int q = 1; q < nc; ++q) fastest = std::min(fastest, sg->trial_ms[q]);
+                int b = 0;
+                for (int q = 0; q < nc; ++q)
+                    if (sg->trial_ms[q] <= 1.02f * fastest) b = q;
+                sg->best = cand[b];
             }
         }
         remaining -= chunk;
@@ -1135,6 +1209,7 @@ extern "C" int gral_pf_iterate(gral_pf_handle *h, int32_t n, int32_t check, gral
     h->timing.active_cells = h->active;
     h->timing.cell_iterations = R.cell_iters;
     h->timing.situations = 1;
+    h->timing.groups = 1;
     fill_stats(h, st);
     return GRAL_PF_OK;
 }
@@ -1295,6 +1370,7 @@ extern "C" int gral_pf_forward_ckpt(gral_pf_handle *h, int32_t n, int32_t quanti
     h->timing.active_cells = h->active;
     h->timing.cell_iterations = h->active * (long long)n;
     h->timing.situations = 1;
+    h->timing.groups = 1;
     h->adj_first = h->iter_loops;
     h->adj_n = n;
     h->host_stale = true;
@@ -1351,6 +1427,7 @@ extern "C" int gral_pf_vjp(gral_pf_handle *h, const float *du, const float *dv, 
     h->timing.active_cells = h->active;
     h->timing.cell_iterations = h->active * (long long)h->adj_n;
     h->timing.situations = 1;
+    h->timing.groups = 1;
     // levels the device does not hold pass through unchanged (the steps never touch them)
     const size_t nh = n2 * (size_t)h->nkh;
     par_copy(gu, du, nh * sizeof(float));
@@ -1536,6 +1613,7 @@ extern "C" int gral_pf_batch_step(gral_pf_batch *b, int32_t *n_done, gral_pf_han
     for (auto *h : act) b->timing.active_cells += h->active;
     b->timing.cell_iterations = R.cell_iters;
     b->timing.situations = (int32_t)act.size();
+    b->timing.groups = R.groups;
     // finished situations: epilogue, then out of the batch
     std::vector<gral_pf_handle *> keep;
     for (auto *h : b->hs) {
@@ -1583,6 +1661,7 @@ extern "C" int gral_pf_batch_iterate(gral_pf_batch *b, int32_t n, int32_t check,
     for (auto *h : b->hs) b->timing.active_cells += h->active;
     b->timing.cell_iterations = R.cell_iters;
     b->timing.situations = (int32_t)b->hs.size();
+    b->timing.groups = R.groups;
     batch_stats(b, st);
     return GRAL_PF_OK;
 }

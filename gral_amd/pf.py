@@ -57,7 +57,7 @@ class Stats(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("ms_total", C.c_double), ("launches", C.c_int64), ("active_cells", C.c_int64),
-                ("cell_iterations", C.c_int64), ("situations", C.c_int32)]
+                ("cell_iterations", C.c_int64), ("situations", C.c_int32), ("groups", C.c_int32)]
 
 
 class FamilyTime(C.Structure):
@@ -588,7 +588,7 @@ def run_batch(cases, iterations):
     return out
 
 
-def solve_stream(cases, capacity, on_done=None, ahead=None, workers=3):
+def solve_stream(cases, capacity, on_done=None, ahead=None, workers=3, on_step=None):
     """Solve a stream of situations with one batch fed while it runs: at most
     ``capacity`` situations are in the batch at a time (active or waiting
     for admission); up to ``ahead`` more (default capacity // 2) are set up
@@ -599,7 +599,8 @@ def solve_stream(cases, capacity, on_done=None, ahead=None, workers=3):
     first keeps the situations of one batch alike in size, so that no launch
     waits long for one large situation).  ``on_done(case, solver, stats)`` is
     called for each finished situation before its solver is closed (download
-    the fields there); the return value is the list of stats in completion
+    the fields there); ``on_step(timing)`` after each step gets the step's
+    ``Batch.timing()``.  The return value is the list of stats in completion
     order."""
     import collections
     import concurrent.futures as cf
@@ -635,7 +636,10 @@ def solve_stream(cases, capacity, on_done=None, ahead=None, workers=3):
                 if not futs and not more:
                     break
                 continue
-            for sv, st in batch.step():
+            finished = batch.step()
+            if on_step is not None:
+                on_step(batch.timing())
+            for sv, st in finished:
                 if on_done is not None:
                     on_done(sv.case, sv, st)
                 out.append(st)

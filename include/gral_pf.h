@@ -211,6 +211,7 @@ typedef struct {
     int64_t active_cells;       /* active cells of all situations (advanced per SIMPLE iteration) */
     int64_t cell_iterations;    /* active cells x SIMPLE iterations actually run, summed over situations */
     int32_t situations;         /* situations in the call (1 for gral_pf_iterate) */
+    int32_t groups;             /* stream groups the batch ran its last iterations in (1 for gral_pf_iterate) */
 } gral_pf_timing;
 int gral_pf_last_timing(gral_pf_handle *h, gral_pf_timing *t);
 
