@@ -248,10 +248,28 @@ def adjoint_timing(steps=20, reps=3):
             sv.vjp(du, dv, dw)
             vj.append(sv.timing()["ms_total"])
             sv.restore_ckpt()
+        # one mixing-length momentum sweep per component (gral_pf_momentum_sweep,
+        # per-diagonal launches) and its VJP (gral_pf_momentum_vjp) from the same state
+        mom = {}
+        rng = np.random.default_rng(5)
+        gbar = rng.standard_normal(c.uk.shape).astype(np.float32)
+        base = sv.download()
+        for comp, name in enumerate("UVW"):
+            sw, mv = [], []
+            for _ in range(reps):
+                sv.set_velocity(base["uk"], base["vk"], base["wk"])
+                sv.momentum_sweep(comp)
+                sw.append(sv.timing()["ms_total"])
+                sv.momentum_vjp(gbar)
+                mv.append(sv.timing()["ms_total"])
+            a, b = sorted(sw)[reps // 2], sorted(mv)[reps // 2]
+            mom[name] = {"sweep_ms": a, "vjp_ms": b, "vjp_over_sweep": b / a}
+        sv.set_velocity(base["uk"], base["vk"], base["wk"])
     f, v = sorted(fw)[reps // 2], sorted(vj)[reps // 2]
     return {"workload": f"city 256x256, 32 layers, {steps} pressure-correction steps, VJP of the mean speed at "
                         f"{len(rec)} receptors", "active_cells": c.active_cells(), "forward_ms": f, "vjp_ms": v,
-            "vjp_over_forward": v / f, "simple_iteration_ms": it_ms, "vjp_step_over_simple_iteration": v / steps / it_ms}
+            "vjp_over_forward": v / f, "simple_iteration_ms": it_ms, "vjp_step_over_simple_iteration": v / steps / it_ms,
+            "momentum": mom}
 
 
 def graph_timing(iters=100):

@@ -35,6 +35,9 @@ extern "C" int pf_launch_gff(const GffArgs *ga, hipStream_t s);
 extern "C" int pf_launch_wind(const WindArgs *wa, hipStream_t s);
 extern "C" int pf_launch_mass(const Dom *dd, int nsit, int nblk, double *partial, double *out, hipStream_t s);
 extern "C" int pf_launch_projection(const Dom *hd, const Dom *dd, int iter, hipStream_t s, long long *launches);
+extern "C" int pf_launch_momentum_sweep(const Dom *hd, const Dom *dd, int comp, int IS, int JS, hipStream_t s);
+extern "C" int pf_launch_momentum_vjp(const Dom *hd, const Dom *dd, const MomAdj *ma, int comp, int IS, int JS,
+                                      hipStream_t s);
 extern "C" int pf_launch_adjoint_step(const Dom *hd, const Dom *dd, const AdjArgs *args, int iter, size_t n3,
                                       hipStream_t s);
 
@@ -185,6 +188,7 @@ struct gral_pf_handle {
     float *adj = nullptr;           // ub vb wb pb qb mb db, checkpoint uk vk wk, ptab[2] ttab[2]: 15 fields
     int *adj_src = nullptr;         // stale-top source column per column, for IS = +1 and -1
     int adj_first = 0, adj_n = 0;   // iterations of the last forward
+    int mom_comp = -1, mom_is = 1, mom_js = 1;   // the last gral_pf_momentum_sweep: component, directions
     GraphCache *graphs = nullptr;   // captured iteration chunks (GRAL_PF_GRAPH=1)
     size_t dev_cap = 0;             // optional cap (GRAL_PF_DEVICE_MB), 0 = none
     std::vector<float> uk, vk, wk, uks, vks, wks, dpm, dpmnew, turb, tdiss;
@@ -382,6 +386,7 @@ static void free_device(gral_pf_handle *h) {
     h->adj = nullptr;
     h->adj_src = nullptr;
     h->adj_n = 0;
+    h->mom_comp = -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1373,6 +1378,7 @@ extern "C" int gral_pf_forward_ckpt(gral_pf_handle *h, int32_t n, int32_t quanti
     h->timing.groups = 1;
     h->adj_first = h->iter_loops;
     h->adj_n = n;
+    h->mom_comp = -1;
     h->host_stale = true;
     h->gff_ok = false;
     return GRAL_PF_OK;
@@ -1380,7 +1386,8 @@ extern "C" int gral_pf_forward_ckpt(gral_pf_handle *h, int32_t n, int32_t quanti
 
 extern "C" int gral_pf_restore_ckpt(gral_pf_handle *h) {
     if (!h) return GRAL_PF_EINVAL;
-    if (!h->prepared || h->adj_n < 1) return fail(h, GRAL_PF_ESTATE, "restore_ckpt: no forward_ckpt since prepare");
+    if (!h->prepared || (h->adj_n < 1 && h->mom_comp < 0))
+        return fail(h, GRAL_PF_ESTATE, "restore_ckpt: no forward_ckpt or momentum_sweep since prepare");
     const size_t n3 = h->n2() * (size_t)h->nkp;
     const float *ck = h->adj + 7 * n3;
     HIPCHK(h, hipMemcpyAsync(h->d.uk, ck, n3 * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
@@ -1435,6 +1442,108 @@ extern "C" int gral_pf_vjp(gral_pf_handle *h, const float *du, const float *dv, 
     par_copy(gw, dw, nh * sizeof(float));
     if ((rc = get_raw(h, g.ub, gu)) || (rc = get_raw(h, g.vb, gv)) || (rc = get_raw(h, g.wb, gw))) return rc;
     return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_set_velocity(gral_pf_handle *h, const float *uk, const float *vk, const float *wk) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "set_velocity: call prepare first");
+    if (!uk || !vk || !wk) return fail(h, GRAL_PF_EINVAL, "set_velocity: null field");
+    int rc = sync_host(h);                                // the other fields' host copies stay current
+    if (rc) return rc;
+    const size_t n3 = h->n2() * (size_t)h->nkh;
+    par_assign(h->uk, uk, n3);
+    par_assign(h->vk, vk, n3);
+    par_assign(h->wk, wk, n3);
+    if ((rc = put_field(h, h->d.uk, h->uk)) || (rc = put_field(h, h->d.vk, h->vk)) || (rc = put_field(h, h->d.wk, h->wk)))
+        return rc;
+    h->gff_ok = false;
+    return GRAL_PF_OK;
+}
+
+// sweep directions of momentum component comp in iteration it (PrognosticFlowfield.cs:1058-1101):
+// U and W sweep (IS, JS), the mixing-length V solver (-IS, -JS)
+static void momentum_dirs(int it, int comp, int &IS, int &JS) {
+    IS = 1; JS = 1;
+    switch (it % 4) {
+        case 0: IS = -1; JS = 1; break;
+        case 1: IS = -1; JS = -1; break;
+        case 2: IS = 1; JS = -1; break;
+        default: break;
+    }
+    if (comp == 1) { IS = -IS; JS = -JS; }
+}
+
+extern "C" int gral_pf_momentum_sweep(gral_pf_handle *h, int32_t comp) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared) return fail(h, GRAL_PF_ESTATE, "momentum_sweep: call prepare first");
+    if (comp < 0 || comp > 2) return fail(h, GRAL_PF_EINVAL, "momentum_sweep: comp must be 0 (U), 1 (V) or 2 (W)");
+    if (h->d.model != 1) return fail(h, GRAL_PF_EINVAL, "momentum_sweep: the mixing-length model (turbulence model 1) only");
+    if (h->d.kadvmax + 2 > 128) return fail(h, GRAL_PF_EINVAL, "momentum_sweep: columns above 126 levels");
+    int rc = adj_alloc(h);
+    if (rc) return rc;
+    const size_t n3 = h->n2() * (size_t)h->nkp;
+    float *ck = h->adj + 7 * n3;
+    HIPCHK(h, hipMemcpyAsync(ck, h->d.uk, n3 * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(ck + n3, h->d.vk, n3 * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(ck + 2 * n3, h->d.wk, n3 * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    int IS, JS;
+    momentum_dirs(h->iter_loops, comp, IS, JS);
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    const int e = pf_launch_momentum_sweep(&h->d, h->dd, comp, IS, JS, h->stream);
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    if (e) return fail(h, GRAL_PF_EDEVICE, std::string("momentum_sweep: launch failed: ") + hipGetErrorString((hipError_t)e));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->timing = gral_pf_timing{};
+    h->timing.ms_total = ms;
+    h->timing.active_cells = h->active;
+    h->timing.situations = 1;
+    h->timing.groups = 1;
+    h->mom_comp = comp;
+    h->mom_is = IS;
+    h->mom_js = JS;
+    h->adj_n = 0;
+    h->host_stale = true;
+    h->gff_ok = false;
+    return GRAL_PF_OK;
+}
+
+extern "C" int gral_pf_momentum_vjp(gral_pf_handle *h, const float *dx, float *gx) {
+    if (!h) return GRAL_PF_EINVAL;
+    if (!h->prepared || h->mom_comp < 0) return fail(h, GRAL_PF_ESTATE, "momentum_vjp: call momentum_sweep first");
+    if (!dx || !gx) return fail(h, GRAL_PF_EINVAL, "momentum_vjp: null argument");
+    const size_t n2 = h->n2(), n3 = n2 * (size_t)h->nkp;
+    float *a = h->adj;
+    const int comp = h->mom_comp;
+    // the situation's descriptor with the swept field at its value before the sweep (the checkpoint)
+    Dom hb = h->d;
+    float *xb = a + (size_t)(7 + comp) * n3;
+    float *xa = comp == 0 ? h->d.uk : comp == 1 ? h->d.vk : h->d.wk;
+    if (comp == 0) hb.uk = xb; else if (comp == 1) hb.vk = xb; else hb.wk = xb;
+    Dom *db = reinterpret_cast<Dom *>(a + 14 * n3);      // the spare field of the adjoint buffers
+    HIPCHK(h, hipMemcpyAsync(db, &hb, sizeof(Dom), hipMemcpyHostToDevice, h->stream));
+    MomAdj m{};
+    m.xa = xa;
+    m.gin = a;
+    m.gout = a + n3; m.gpa = a + 2 * n3; m.gpb = a + 3 * n3; m.gna = a + 4 * n3; m.gnb = a + 5 * n3; m.gres = a + 6 * n3;
+    int rc;
+    if ((rc = put_raw(h, a, dx))) return rc;
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    const int e = pf_launch_momentum_vjp(&hb, db, &m, comp, h->mom_is, h->mom_js, h->stream);
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    if (e) return fail(h, GRAL_PF_EDEVICE, std::string("momentum_vjp: launch failed: ") + hipGetErrorString((hipError_t)e));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->timing = gral_pf_timing{};
+    h->timing.ms_total = ms;
+    h->timing.active_cells = h->active;
+    h->timing.situations = 1;
+    h->timing.groups = 1;
+    // levels the device does not hold pass through unchanged (the sweep never touches them)
+    par_copy(gx, dx, n2 * (size_t)h->nkh * sizeof(float));
+    return get_raw(h, m.gres, gx);
 }
 
 extern "C" int gral_pf_download_fields(gral_pf_handle *h, float *uk, float *vk, float *wk, float *uks, float *vks,

@@ -154,6 +154,17 @@ struct AdjArgs {
     const int *src;             // per column: i of the column whose Q the elimination starts from, or -1
 };
 
+// Buffers of the adjoint of one momentum sweep (pf_kernels.hip), each
+// [nii+2][njj+2][nkp].
+struct MomAdj {
+    const float *xa;    // X after the sweep (the device field)
+    const float *gin;   // dL/dX after
+    float *gout;        // dL/dX before: the swept columns' own levels KSTART-1..VI+1
+    float *gpa, *gpb;   // per column and level: adjoint for the new X of its a- / b-predecessor
+    float *gna, *gnb;   // per column and level: adjoint for the old X of its a- / b-successor
+    float *gres;        // dL/dX before, complete
+};
+
 #define SYNC_BLOCK0 64               // Dom::sync: sticky error flag at 0, sweep blocks from here
 #define STRIP_ROWS 15                // sweep rows per strip workgroup (plus one chain wave)
 

@@ -566,6 +566,48 @@ __device__ __forceinline__ bool ml_cross_on(const ColInfo &q, int k) {
     return block_base(k, q.KSTART, q.VI + 1, q.W) > q.KK + 1;
 }
 
+// face fluxes of the component's control volume at level k (U-prognostic-
+// microscale_1.cs:313-330, V-...:317-331, W-...:286-300), in the reference's order
+template <int COMP>
+__device__ __forceinline__ void ml_fluxes(const DomG &d, const ColInfo &q, int k, float &FE, float &FW, float &FS,
+                                          float &FN, float &FT, float &FB) {
+    const int i = q.i, j = q.j;
+    const float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
+    const gf32 *UKS = d.uks + c3(d, i, j), *VKS = d.vks + c3(d, i, j), *WKS = d.wks + c3(d, i, j);
+    if (COMP == 0) {
+        const gf32 *UKS_W = d.uks + c3(d, i - 1, j);
+        const gf32 *VKS_W = d.vks + c3(d, i - 1, j), *VKS_WS = d.vks + c3(d, i - 1, j - 1);
+        const gf32 *VKS_S = d.vks + c3(d, i, j - 1), *VKS_WN = d.vks + c3(d, i - 1, j + 1);
+        const gf32 *VKS_N = d.vks + c3(d, i, j + 1), *WKS_W = d.wks + c3(d, i - 1, j);
+        FE = UKS[k] * dykdzk;
+        FW = UKS_W[k] * dykdzk;
+        FS = 0.25f * (VKS_W[k] + VKS[k] + VKS_WS[k] + VKS_S[k]) * dxkdzk;
+        FN = 0.25f * (VKS_W[k] + VKS[k] + VKS_WN[k] + VKS_N[k]) * dxkdzk;
+        FT = 0.25f * (WKS_W[k] + WKS[k] + WKS_W[k + 1] + WKS[k + 1]) * d.area;
+        FB = (k > q.KK + 1) ? 0.25f * (WKS_W[k] + WKS[k] + WKS_W[k - 1] + WKS[k - 1]) * d.area : 0.0f;
+    } else if (COMP == 1) {
+        const gf32 *UKS_W = d.uks + c3(d, i - 1, j), *UKS_E = d.uks + c3(d, i + 1, j);
+        const gf32 *UKS_S = d.uks + c3(d, i, j - 1), *UKS_ES = d.uks + c3(d, i + 1, j - 1);
+        const gf32 *UKS_WS = d.uks + c3(d, i - 1, j - 1);
+        const gf32 *VKS_S = d.vks + c3(d, i, j - 1), *WKS_S = d.wks + c3(d, i, j - 1);
+        FE = 0.25f * (UKS[k] + UKS_E[k] + UKS_S[k] + UKS_ES[k]) * dykdzk;
+        FW = 0.25f * (UKS[k] + UKS_W[k] + UKS_S[k] + UKS_WS[k]) * dykdzk;
+        FS = VKS_S[k] * dxkdzk;
+        FN = VKS[k] * dxkdzk;
+        FT = 0.25f * (WKS_S[k] + WKS[k] + WKS_S[k + 1] + WKS[k + 1]) * d.area;
+        FB = (k > q.KK + 1) ? 0.25f * (WKS_S[k] + WKS[k] + WKS_S[k - 1] + WKS[k - 1]) * d.area : 0.0f;
+    } else {
+        const gf32 *UKS_W = d.uks + c3(d, i - 1, j), *UKS_E = d.uks + c3(d, i + 1, j);
+        const gf32 *VKS_S = d.vks + c3(d, i, j - 1), *VKS_N = d.vks + c3(d, i, j + 1);
+        FE = 0.25f * (UKS[k] + UKS_E[k] + UKS[k - 1] + UKS_E[k - 1]) * dykdzk;
+        FW = 0.25f * (UKS[k] + UKS_W[k] + UKS[k - 1] + UKS_W[k - 1]) * dykdzk;
+        FS = 0.25f * (VKS[k] + VKS_S[k] + VKS[k - 1] + VKS_S[k - 1]) * dxkdzk;
+        FN = 0.25f * (VKS[k] + VKS_N[k] + VKS[k - 1] + VKS_N[k - 1]) * dxkdzk;
+        FT = WKS[k] * d.area;
+        FB = (k > q.KK + 1) ? WKS[k - 1] * d.area : 0.0f;
+    }
+}
+
 __device__ MlPieces u1_pieces(const DomG &d, const ColInfo &q, int k) {
     int i = q.i, j = q.j;
     const gf32 *UK = d.uk + c3(d, i, j), *VK = d.vk + c3(d, i, j);
@@ -579,12 +621,8 @@ __device__ MlPieces u1_pieces(const DomG &d, const ColInfo &q, int k) {
     const gf32 *DPMNEW = d.dpmnew + c3(d, i, j), *DPMNEW_W = d.dpmnew + c3(d, i - 1, j);
     float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
     float VIS = ml_visc(d, q, k, UK, VK);
-    float FE = UKS[k] * dykdzk;
-    float FW = UKS_W[k] * dykdzk;
-    float FS = 0.25f * (VKS_W[k] + VKS[k] + VKS_WS[k] + VKS_S[k]) * dxkdzk;
-    float FN = 0.25f * (VKS_W[k] + VKS[k] + VKS_WN[k] + VKS_N[k]) * dxkdzk;
-    float FT = 0.25f * (WKS_W[k] + WKS[k] + WKS_W[k + 1] + WKS[k + 1]) * d.area;
-    float FB = (k > q.KK + 1) ? 0.25f * (WKS_W[k] + WKS[k] + WKS_W[k - 1] + WKS[k - 1]) * d.area : 0.0f;
+    float FE, FW, FS, FN, FT, FB;
+    ml_fluxes<0>(d, q, k, FE, FW, FS, FN, FT, FB);
     MlPieces o;
     ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, o.AW1, o.AS1, o.AE1, o.AN1, o.B, o.C, o.A);
     float ap0 = d.ap0[k];
@@ -623,12 +661,8 @@ __device__ MlPieces v1_pieces(const DomG &d, const ColInfo &q, int k) {
     const gf32 *DPMNEW = d.dpmnew + c3(d, i, j), *DPMNEW_S = d.dpmnew + c3(d, i, j - 1);
     float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
     float VIS = ml_visc(d, q, k, UK, VK);
-    float FE = 0.25f * (UKS[k] + UKS_E[k] + UKS_S[k] + UKS_ES[k]) * dykdzk;
-    float FW = 0.25f * (UKS[k] + UKS_W[k] + UKS_S[k] + UKS_WS[k]) * dykdzk;
-    float FS = VKS_S[k] * dxkdzk;
-    float FN = VKS[k] * dxkdzk;
-    float FT = 0.25f * (WKS_S[k] + WKS[k] + WKS_S[k + 1] + WKS[k + 1]) * d.area;
-    float FB = (k > q.KK + 1) ? 0.25f * (WKS_S[k] + WKS[k] + WKS_S[k - 1] + WKS[k - 1]) * d.area : 0.0f;
+    float FE, FW, FS, FN, FT, FB;
+    ml_fluxes<1>(d, q, k, FE, FW, FS, FN, FT, FB);
     MlPieces o;
     ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, o.AW1, o.AS1, o.AE1, o.AN1, o.B, o.C, o.A);
     float ap0 = d.ap0[k];
@@ -663,12 +697,8 @@ __device__ MlPieces w1_pieces(const DomG &d, const ColInfo &q, int k) {
     const gf32 *DPMNEW = d.dpmnew + c3(d, i, j);
     float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
     float VIS = ml_visc(d, q, k, UK, VK);
-    float FE = 0.25f * (UKS[k] + UKS_E[k] + UKS[k - 1] + UKS_E[k - 1]) * dykdzk;
-    float FW = 0.25f * (UKS[k] + UKS_W[k] + UKS[k - 1] + UKS_W[k - 1]) * dykdzk;
-    float FS = 0.25f * (VKS[k] + VKS_S[k] + VKS[k - 1] + VKS_S[k - 1]) * dxkdzk;
-    float FN = 0.25f * (VKS[k] + VKS_N[k] + VKS[k - 1] + VKS_N[k - 1]) * dxkdzk;
-    float FT = WKS[k] * d.area;
-    float FB = (k > q.KK + 1) ? WKS[k - 1] * d.area : 0.0f;
+    float FE, FW, FS, FN, FT, FB;
+    ml_fluxes<2>(d, q, k, FE, FW, FS, FN, FT, FB);
     MlPieces o;
     ml_coeffs(d, q, k, VIS, FE, FW, FS, FN, FT, FB, o.AW1, o.AS1, o.AE1, o.AN1, o.B, o.C, o.A);
     float ap0 = d.ap0[k];
@@ -3202,5 +3232,329 @@ extern "C" int pf_launch_mass(const Dom *dd, int nsit, int nblk, double *partial
     if (nsit < 1 || nblk < 1) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_mass_partial, dim3((unsigned)nblk, 1, (unsigned)nsit), dim3(MASS_THREADS), 0, s, dd, partial);
     hipLaunchKernelGGL(k_mass_final, dim3((unsigned)nsit), dim3(MASS_THREADS), 0, s, partial, nblk, out);
+    return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// Adjoint of one mixing-length momentum sweep (U/V/W-prognostic-microscale_1.cs:
+// coefficients :194-470, Thomas forward :473-504, relaxed back-substitution
+// :514-524) with respect to the swept component X, with everything else the
+// sweep reads held fixed (UKS, VKS, WKS, DPMNEW, the other components).  X
+// enters a column's equations through the eddy viscosity (own column, levels
+// k-1 and k+1: the conductances, hence every power-law coefficient, and the
+// cross-term factor), the Coriolis/vegetation and wall terms (own level), the
+// cross terms (own level and the two neighbours along the component), the four
+// neighbour couplings (new values of the two columns visited before it, old
+// values of the two after it) and the relaxation.  The transpose runs the
+// sweep's anti-diagonals from the last to the first, one wave per column: the
+// lanes recompute the column's coefficients from the checkpointed fields, lane
+// 0 runs the Thomas recurrences forward and then both recurrences transposed,
+// and the lanes push the adjoints back through the coefficient assembly.  The
+// adjoint handed to another column has one writer (buffers indexed by the
+// handing column, per direction) and is gathered by its receiver, so the
+// result is deterministic.
+// ------------------------------------------------------------------------
+
+// d VIS / d X[k+1] of ml_visc (d VIS / d X[k-1] is its negative) for the
+// component whose vertical shear enters VIS (U: UK, V: VK; W's VIS holds no WK)
+template <int COMP>
+__device__ __forceinline__ float ml_visc_dx(const DomG &d, const ColInfo &q, int k, const gf32 *UK, const gf32 *VK) {
+    if (COMP == 2 || !(k > q.KK + 1)) return 0.0f;
+    const float dz = d.dzk[k];
+    const float zs = q.v512() ? d.hok[k] - fmaf(dz, 0.5f, d.hok[q.KK]) : d.hok[k] - d.hok[q.KK] - dz * 0.5f;
+    float mixL = vminf(0.071f * zs, 100.0f);
+    if (q.cov > 0) mixL *= (1.0f - 0.99f * q.cov);
+    const float iu = (UK[k + 1] - UK[k - 1]) / (2.0f * dz), iv = (VK[k + 1] - VK[k - 1]) / (2.0f * dz);
+    const float r = sqrtf(0.5f * (iu * iu + iv * iv));
+    if (!(mixL * mixL * r < 15.0f) || !(r > 0.0f)) return 0.0f;      // at the cap of 15, or no shear
+    return mixL * mixL * 0.5f * ((COMP == 0) ? iu : iv) / r / (2.0f * dz);
+}
+
+// d A / d D of a power-law coefficient A = D max(0, (1 - 0.1 P)^5) + F+ with
+// P = |F / D| and F fixed
+__device__ __forceinline__ float plaw_dd(float P) {
+    const float x = 1.0f - 0.1f * P;
+    if (!(x > 0.0f)) return 0.0f;
+    const float x4 = (x * x) * (x * x);
+    return x4 * x + 0.5f * x4 * P;
+}
+
+// d A / d VIS of the six coefficients of ml_coeffs (AE1, AW1, AS1, AN1, BIM, CIM)
+__device__ __forceinline__ void ml_coeffs_dvis(const DomG &d, const ColInfo &q, int k, float VIS, float FE, float FW,
+                                               float FS, float FN, float FT, float FB, float *g) {
+    const float dz = d.dzk[k], dxkdzk = d.dxk * dz, dykdzk = d.dyk * dz;
+    const float mv = vmaxf(VIS, d.vishmin), dmv = VIS > d.vishmin ? 1.0f : 0.0f;
+    auto face = [&](bool off, float geo, float F) {           // conductance mv*geo, floored at 1e-4
+        if (off) return 0.0f;
+        const float Dr = mv * geo;
+        if (!(Dr > 0.0001f)) return 0.0f;
+        return dmv * geo * plaw_dd(fabsf(F / Dr));
+    };
+    const float gE = dykdzk / d.dxk, gS = dxkdzk / d.dyk, gT = d.area / dz;
+    g[0] = face(q.ae, gE, FE);
+    g[1] = face(q.aw, gE, FW);
+    g[2] = face(q.as, gS, FS);
+    g[3] = face(q.an, gS, FN);
+    g[4] = face(false, gT, FT);
+    g[5] = face(!(k > q.KK + 1), gT, FB);
+}
+
+#define MA_SLOTS 18
+template <int COMP>
+__global__ __launch_bounds__(WAVE) void k_adj_mom(const Dom *__restrict__ ds, MomAdj m, int t, int IS, int JS) {
+    const DomG &d = *reinterpret_cast<const DomG *>(ds);       // its swept field is X before the sweep
+    extern __shared__ float sm[];
+    const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2;
+    int a, b;
+    if (!diag_col(t, oa, d.nii - 1, ia, d.njj - 1, blockIdx.x, a, b)) return;
+    int i, j;
+    sweep_to_ij(d, 0, a, b, IS, JS, i, j);
+    if (d.adv[c2(d, i, j)] != 1) return;
+    const ColInfo q = col_info(d, i, j);
+    const int nkp = d.nkp, lane = threadIdx.x;
+    const int KK = q.KK, KKP1 = KK + 1, KSTART = q.KSTART, VI = q.VI;
+    int KKm = KK;
+    if (COMP == 0) KKm = max(KK, d.kk[c2(d, i - 1, j)]);
+    if (COMP == 1) KKm = max(KK, d.kk[c2(d, i, j - 1)]);
+    const gf32 *XB = (COMP == 0) ? d.uk : (COMP == 1) ? d.vk : d.wk;
+    const gf32 *XA = (const gf32 *)m.xa;
+    const size_t o = c3(d, i, j), opa = c3(d, i - IS, j), ona = c3(d, i + IS, j), opb = c3(d, i, j - JS),
+                 onb = c3(d, i, j + JS);
+    float *sA = sm, *sB = sA + nkp, *sC = sB + nkp, *sD = sC + nkp, *sT = sD + nkp, *sP = sT + nkp, *sQ = sP + nkp;
+    float *sG = sQ + nkp, *sXa = sG + nkp, *sAb = sXa + nkp, *sBb = sAb + nkp, *sCb = sBb + nkp, *sDb = sCb + nkp;
+    // sR: own X before from the relaxation; sUp / sDn / sMid: from the assembly, for own X before at k+1 /
+    // k-1 / k (sUp first holds the adjoint of P between the two transposed recurrences); sQb: adjoint of Q
+    float *sR = sDb + nkp, *sUp = sR + nkp, *sDn = sUp + nkp, *sMid = sDn + nkp, *sQb = sMid + nkp;
+    const float dyk = d.dyk, dxk = d.dxk;
+    // 1. the column's equations from the checkpointed fields, and dL/dX after
+    for (int k = KSTART + lane; k <= VI; k += WAVE) {
+        const MlPieces pc = ml_pieces<COMP>(d, q, k);
+        const float xpa = XA[opa + k], xna = XB[ona + k], xpb = XA[opb + k], xnb = XB[onb + k], x = XB[o + k];
+        const float xw = (IS == 1) ? xpa : xna, xe = (IS == 1) ? xna : xpa;
+        const float xs = (JS == 1) ? xpb : xnb, xn = (JS == 1) ? xnb : xpb;
+        sA[k] = pc.A; sB[k] = pc.B; sC[k] = pc.C;
+        sD[k] = ml_source<COMP>(d, q, k, d.dzk[k], pc, xw, xs, xe, xn, x);
+        sG[k] = m.gin[o + k] + m.gpa[c3(d, i + IS, j) + k] + m.gpb[c3(d, i, j + JS) + k];
+    }
+    for (int k = KSTART + lane; k <= VI + 1; k += WAVE) sXa[k] = XA[o + k];
+    __syncthreads();
+    if (lane == 0) {
+        // Thomas forward, as the sweep ran it
+        float Pp = 0.0f, Qp = 0.0f;
+        for (int k = KSTART; k <= VI; ++k) {
+            float T, P, Q;
+            if (k > KKP1) { T = 1 / (sA[k] - sC[k] * Pp); P = sB[k] * T; Q = (sD[k] + sC[k] * Qp) * T; }
+            else { T = 1 / sA[k]; P = sB[k] * T; Q = sD[k] * T; }
+            sT[k] = T; sP[k] = P; sQ[k] = Q;
+            Pp = P; Qp = Q;
+        }
+        // relaxed back-substitution X_k += relax (P_k X_{k+1} + Q_k - X_k) from VI down, transposed
+        const float rl = d.relax;
+        float ub = 0.0f;
+        for (int k = KSTART; k <= VI; ++k) {
+            const float g = sG[k] + ub;
+            if (COMP == 2 || KKm < k) {
+                sR[k] = g * (1.0f - rl);
+                sQb[k] = g * rl;
+                sUp[k] = g * rl * sXa[k + 1];
+                ub = g * rl * sP[k];
+            } else {
+                sR[k] = g;
+                sQb[k] = 0.0f;
+                sUp[k] = 0.0f;
+                ub = 0.0f;
+            }
+        }
+        sR[VI + 1] = ub;                                  // X[VI+1] is the top value the first level read
+        // Thomas forward transposed (VI down to KSTART)
+        float Pc = 0.0f, Qc = 0.0f;
+        for (int k = VI; k >= KSTART; --k) {
+            const float Qt = sQb[k] + Qc, Pt = sUp[k] + Pc;
+            const float B = sB[k], C = sC[k], D = sD[k], T = sT[k];
+            float Ab, Bb, Cb, Db;
+            if (k > KKP1) {
+                const float Pm = sP[k - 1], Qm = sQ[k - 1];
+                Db = Qt * T;
+                const float Tb = Qt * (D + C * Qm) + Pt * B;
+                Bb = Pt * T;
+                Cb = Qt * Qm * T;
+                Qc = Qt * C * T;
+                const float denb = -Tb * T * T;
+                Ab = denb;
+                Cb += -denb * Pm;
+                Pc = -denb * C;
+            } else {
+                Db = Qt * T;
+                Bb = Pt * T;
+                Ab = -(Qt * D + Pt * B) * T * T;
+                Cb = 0.0f;
+                Pc = 0.0f;
+                Qc = 0.0f;
+            }
+            sAb[k] = Ab; sBb[k] = Bb; sCb[k] = Cb; sDb[k] = Db;
+        }
+    }
+    __syncthreads();
+    // 3. the assembly transposed, lanes over levels
+    for (int k = KSTART + lane; k <= VI; k += WAVE) {
+        const MlPieces pc = ml_pieces<COMP>(d, q, k);
+        const float xpa = XA[opa + k], xna = XB[ona + k], xpb = XA[opb + k], xnb = XB[onb + k], x = XB[o + k];
+        const float xw = (IS == 1) ? xpa : xna, xe = (IS == 1) ? xna : xpa;
+        const float xs = (JS == 1) ? xpb : xnb, xn = (JS == 1) ? xnb : xpb;
+        const float Ab = sAb[k], Bb = sBb[k], Cb = sCb[k], Db = sDb[k];
+        const float dz = d.dzk[k];
+        // AIM = BIM + CIM + AW1 + AS1 + AE1 + AN1 + AP0, D = AW1 X_W + AS1 X_S + AE1 X_E + AN1 X_N + ...
+        const float awb = Ab + Db * xw, asb = Ab + Db * xs, aeb = Ab + Db * xe, anb = Ab + Db * xn;
+        const float bimb = Bb + Ab, cimb = Cb + Ab;
+        float xwb = Db * pc.AW1, xsb = Db * pc.AS1, xeb = Db * pc.AE1, xnb_ = Db * pc.AN1, xb = 0.0f;
+        float visb = 0.0f, up = 0.0f, dn = 0.0f;
+        const bool cross = ml_cross_on(q, k);
+        if (COMP != 2) {
+            const gf32 *UKS = d.uks + o, *VKS = d.vks + o;
+            const gf32 *UKS_W = d.uks + c3(d, i - 1, j), *VKS_W = d.vks + c3(d, i - 1, j);
+            float gfac = -d.cor;                              // g = (Cor (UG - x) [- VEG x wh]) AREA DZ
+            if (q.vegp) {
+                const float in2 = 0.5f * (VKS_W[k] + VKS[k]), in = 0.5f * (UKS_W[k] + UKS[k]);
+                const float wh = vmaxf(sqrtf(in * in + in2 * in2), 0.01f);
+                gfac = -d.cor - d.veg[o + k - 1] * wh;
+            }
+            xb += Db * gfac * d.area * dz;
+            if (k == KKP1) xb -= Db * ml_wall_uv(d, q, k, 1.0f, UKS, UKS_W, VKS, VKS_W);   // w = x / wh us^2 AREA
+            if (cross) {
+                if (COMP == 0) {
+                    const float dk = dyk * dz;
+                    const float S = (xe - x) * dk - (x - xw) * dk + pc.X1 - pc.X2 + pc.X3 - pc.X4;
+                    const float Sb = Db * pc.c0;
+                    xeb += Sb * dk; xwb += Sb * dk; xb -= 2.0f * Sb * dk;
+                    visb += Db * S / dxk;                    // c0 = VIS / DXK
+                } else {
+                    const float dk = dxk * dz;
+                    const float S = pc.X1 - pc.X2 + (xn - x) * dk - (x - xs) * dk + pc.X3 - pc.X4;
+                    const float Sb = Db * pc.c0;
+                    xnb_ += Sb * dk; xsb += Sb * dk; xb -= 2.0f * Sb * dk;
+                    visb += Db * S / dxk;
+                }
+            }
+            float FE, FW, FS, FN, FT, FB, g6[6];
+            ml_fluxes<COMP>(d, q, k, FE, FW, FS, FN, FT, FB);
+            const gf32 *UK = d.uk + o, *VK = d.vk + o;
+            const float VIS = ml_visc(d, q, k, UK, VK);
+            ml_coeffs_dvis(d, q, k, VIS, FE, FW, FS, FN, FT, FB, g6);
+            visb += aeb * g6[0] + awb * g6[1] + asb * g6[2] + anb * g6[3] + bimb * g6[4] + cimb * g6[5];
+            const float dvx = ml_visc_dx<COMP>(d, q, k, UK, VK);
+            up = visb * dvx;
+            dn = -visb * dvx;
+        } else {
+            if (q.vegp) {                                     // D -= VEG x wh AREA DZ
+                const gf32 *UKS = d.uks + o, *VKS = d.vks + o, *UKS_W = d.uks + c3(d, i - 1, j);
+                const gf32 *VKS_S = d.vks + c3(d, i, j - 1);
+                const float in2 = 0.5f * (VKS_S[k] + VKS[k]), in = 0.5f * (UKS_W[k] + UKS[k]);
+                const float wh = vmaxf(sqrtf(in * in + in2 * in2), 0.01f);
+                xb -= Db * d.veg[o + k - 1] * wh * d.area * dz;
+            }
+            if (cross) {                                      // X1 = VIS/DZ (... + (X_{k+1} - x) AREA - (x - X_{k-1}) AREA)
+                const float VIS = ml_visc(d, q, k, d.uk + o, d.vk + o);
+                const float c = Db * (VIS / dz) * d.area;
+                up = c; dn = c; xb -= 2.0f * c;
+            }
+        }
+        sUp[k] = up; sDn[k] = dn; sMid[k] = xb;
+        // the four neighbours' adjoints, handed on by direction
+        m.gpa[o + k] = (IS == 1) ? xwb : xeb;
+        m.gna[o + k] = (IS == 1) ? xeb : xwb;
+        m.gpb[o + k] = (JS == 1) ? xsb : xnb_;
+        m.gnb[o + k] = (JS == 1) ? xnb_ : xsb;
+    }
+    __syncthreads();
+    // 4. own X before, levels KSTART-1 .. VI+1, summed in a fixed order
+    for (int k = KSTART - 1 + lane; k <= VI + 1; k += WAVE) {
+        float v = 0.0f;
+        if (k >= KSTART && k <= VI) v = sR[k] + sMid[k];
+        if (k == VI + 1) v = sR[VI + 1];
+        if (k - 1 >= KSTART && k - 1 <= VI) v += sUp[k - 1];
+        if (k + 1 >= KSTART && k + 1 <= VI) v += sDn[k + 1];
+        m.gout[o + k] = v;
+    }
+}
+
+// dL/dX before of every cell: its own column's part, the adjoints its
+// predecessors handed to its old value, and, where the sweep left X as it was
+// (columns it does not visit, levels outside KSTART..VI), dL/dX after with the
+// adjoints its successors handed to its new value.  Thread per cell and level.
+template <int COMP>
+__global__ void k_adj_mom_combine(const Dom *__restrict__ ds, MomAdj m, int IS, int JS) {
+    const DomG &d = *reinterpret_cast<const DomG *>(ds);
+    const size_t n3 = (size_t)(d.nii + 2) * d.nj2 * d.nkp;
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n3) return;
+    const int k = (int)(g % d.nkp);
+    const size_t c = g / d.nkp;
+    const int i = (int)(c / d.nj2), j = (int)(c % d.nj2);
+    const int oa = (COMP == 0) ? 3 : 2, ia = (COMP == 1) ? 3 : 2;
+    auto inside = [&](int ii, int jj) { return ii >= 0 && ii <= d.nii + 1 && jj >= 0 && jj <= d.njj + 1; };
+    float v = m.gout[g];
+    if (inside(i - IS, j)) v += m.gna[c3(d, i - IS, j) + k];
+    if (inside(i, j - JS)) v += m.gnb[c3(d, i, j - JS) + k];
+    bool done = false;
+    if (i >= 1 && i <= d.nii && j >= 1 && j <= d.njj && d.adv[c] == 1) {
+        const int a = (IS == -1) ? d.nii - i + 1 : i, b = (JS == -1) ? d.njj - j + 1 : j;
+        if (a >= oa && a <= d.nii - 1 && b >= ia && b <= d.njj - 1) {
+            const int KK = d.kk[c];
+            const int KSTART = (d.cutk[c] == 0) ? KK + 1 : 1;
+            done = k >= KSTART && k <= d.vi[c];
+        }
+    }
+    if (!done) {
+        v += m.gin[g];
+        if (inside(i + IS, j)) v += m.gpa[c3(d, i + IS, j) + k];
+        if (inside(i, j + JS)) v += m.gpb[c3(d, i, j + JS) + k];
+    }
+    m.gres[g] = v;
+}
+
+__global__ void k_mom_fill(float *p, size_t n) {
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) p[g] = 0.0f;
+}
+
+// One sweep of momentum component comp (0 U, 1 V, 2 W) of the mixing-length
+// model in the directions (IS, JS) the iteration gives it, one launch per
+// anti-diagonal (bit for bit the strip sweep's result).
+extern "C" int pf_launch_momentum_sweep(const Dom *hd, const Dom *dd, int comp, int IS, int JS, hipStream_t s) {
+    const Dom &h = *hd;
+    if (h.model != 1 || comp < 0 || comp > 2) return (int)hipErrorInvalidValue;
+    const int oa = (comp == 0) ? 3 : 2, ia = (comp == 1) ? 3 : 2, ob = h.nii - 1, ib = h.njj - 1;
+    const size_t shm = sizeof(float) * 5 * h.nkp;
+    for (int t = 0; t < ndiag(oa, ob, ia, ib); ++t) {
+        const unsigned len = (unsigned)diag_len(t, oa, ob, ia, ib);
+        if (comp == 0) hipLaunchKernelGGL((k_momentum<0, 0>), dim3(len), dim3(WAVE), shm, s, dd, t, IS, JS);
+        else if (comp == 1) hipLaunchKernelGGL((k_momentum<1, 0>), dim3(len), dim3(WAVE), shm, s, dd, t, IS, JS);
+        else hipLaunchKernelGGL((k_momentum<2, 0>), dim3(len), dim3(WAVE), shm, s, dd, t, IS, JS);
+    }
+    return (int)hipGetLastError();
+}
+
+// The transpose of that sweep: dd is the situation's descriptor with the swept
+// field pointing at X before the sweep.
+extern "C" int pf_launch_momentum_vjp(const Dom *hd, const Dom *dd, const MomAdj *ma, int comp, int IS, int JS,
+                                      hipStream_t s) {
+    const Dom &h = *hd;
+    if (h.model != 1 || comp < 0 || comp > 2) return (int)hipErrorInvalidValue;
+    const MomAdj m = *ma;
+    const size_t n3 = (size_t)(h.nii + 2) * h.nj2 * h.nkp;
+    const unsigned fb = (unsigned)((n3 + 255) / 256);
+    float *z[5] = {m.gout, m.gpa, m.gpb, m.gna, m.gnb};
+    for (float *p : z) hipLaunchKernelGGL(k_mom_fill, dim3(fb), dim3(256), 0, s, p, n3);
+    const int oa = (comp == 0) ? 3 : 2, ia = (comp == 1) ? 3 : 2, ob = h.nii - 1, ib = h.njj - 1;
+    const size_t shm = sizeof(float) * MA_SLOTS * h.nkp;
+    for (int t = ndiag(oa, ob, ia, ib) - 1; t >= 0; --t) {
+        const unsigned len = (unsigned)diag_len(t, oa, ob, ia, ib);
+        if (comp == 0) hipLaunchKernelGGL(k_adj_mom<0>, dim3(len), dim3(WAVE), shm, s, dd, m, t, IS, JS);
+        else if (comp == 1) hipLaunchKernelGGL(k_adj_mom<1>, dim3(len), dim3(WAVE), shm, s, dd, m, t, IS, JS);
+        else hipLaunchKernelGGL(k_adj_mom<2>, dim3(len), dim3(WAVE), shm, s, dd, m, t, IS, JS);
+    }
+    if (comp == 0) hipLaunchKernelGGL(k_adj_mom_combine<0>, dim3(fb), dim3(256), 0, s, dd, m, IS, JS);
+    else if (comp == 1) hipLaunchKernelGGL(k_adj_mom_combine<1>, dim3(fb), dim3(256), 0, s, dd, m, IS, JS);
+    else hipLaunchKernelGGL(k_adj_mom_combine<2>, dim3(fb), dim3(256), 0, s, dd, m, IS, JS);
     return (int)hipGetLastError();
 }

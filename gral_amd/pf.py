@@ -23,7 +23,8 @@ I16P = C.POINTER(C.c_int16)
 
 EXPORTS = ["gral_pf_create", "gral_pf_set_masks", "gral_pf_set_roughness", "gral_pf_upload_fields",
            "gral_pf_set_met_profile", "gral_pf_prepare", "gral_pf_iterate", "gral_pf_mass_imbalance",
-           "gral_pf_mass_imbalance_noalloc", "gral_pf_forward_ckpt", "gral_pf_restore_ckpt", "gral_pf_vjp", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
+           "gral_pf_mass_imbalance_noalloc", "gral_pf_forward_ckpt", "gral_pf_restore_ckpt", "gral_pf_vjp",
+           "gral_pf_momentum_sweep", "gral_pf_momentum_vjp", "gral_pf_set_velocity", "gral_pf_finish", "gral_pf_solve", "gral_pf_download_fields",
            "gral_pf_diagnostic",
            "gral_pf_last_timing", "gral_pf_profile_iterations", "gral_pf_batch_create", "gral_pf_batch_iterate",
            "gral_pf_batch_solve", "gral_pf_batch_profile_iterations", "gral_pf_batch_last_timing",
@@ -108,6 +109,9 @@ def load(path: str = LIB_PATH):
     lib.gral_pf_forward_ckpt.argtypes = [H, C.c_int32, C.c_int32]
     lib.gral_pf_restore_ckpt.argtypes = [H]
     lib.gral_pf_vjp.argtypes = [H, FP, FP, FP, FP, FP, FP]
+    lib.gral_pf_momentum_sweep.argtypes = [H, C.c_int32]
+    lib.gral_pf_momentum_vjp.argtypes = [H, FP, FP]
+    lib.gral_pf_set_velocity.argtypes = [H, FP, FP, FP]
     lib.gral_pf_prepare.argtypes = [H, C.POINTER(Params)]
     lib.gral_pf_iterate.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(Stats)]
     lib.gral_pf_finish.argtypes = [H, C.POINTER(Stats)]
@@ -236,6 +240,25 @@ class PrognosticFlowfield:
 
     def restore_ckpt(self):
         self._check(self.lib.gral_pf_restore_ckpt(self.h))
+
+    def set_velocity(self, uk, vk, wk):
+        """Replace UK, VK, WK of the prepared situation (full-depth arrays),
+        keeping its derived fields and iteration count."""
+        uk, vk, wk = _f32(uk), _f32(vk), _f32(wk)
+        self._check(self.lib.gral_pf_set_velocity(self.h, _ptr(uk), _ptr(vk), _ptr(wk)))
+
+    def momentum_sweep(self, comp):
+        """One mixing-length momentum sweep of component comp (0 U, 1 V, 2 W) in
+        the directions of the next iteration, keeping the fields before it."""
+        self._check(self.lib.gral_pf_momentum_sweep(self.h, int(comp)))
+
+    def momentum_vjp(self, dx):
+        """dL/dX before the last momentum_sweep from dL/dX after it (X the swept
+        component; full-depth array), the other fields held fixed."""
+        dx = _f32(dx)
+        gx = np.zeros_like(dx)
+        self._check(self.lib.gral_pf_momentum_vjp(self.h, _ptr(dx), _ptr(gx)))
+        return gx
 
     def vjp(self, du, dv, dw):
         """dL/d(UK, VK, WK) at the start of the last forward_ckpt window from

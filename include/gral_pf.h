@@ -192,6 +192,29 @@ int gral_pf_solve(gral_pf_handle *h, const gral_pf_params *p, gral_pf_stats *st)
  * diagnostic run); download_fields returns the result. */
 int gral_pf_diagnostic(gral_pf_handle *h, int32_t flow_field_level, int32_t terrain);
 
+/* Replace UK, VK, WK of a prepared situation (dense [NII+2][NJJ+2][NKK+2]
+ * arrays) on the device and in the handle, keeping everything the prologue and
+ * the iterations so far derived (UKS/VKS/WKS, DPMNEW, TURB/TDISS, DTIME, the
+ * iteration count): to continue a solve from modified velocities.  No
+ * reference counterpart. */
+int gral_pf_set_velocity(gral_pf_handle *h, const float *uk, const float *vk, const float *wk);
+
+/* Adjoint of one momentum sweep, mixing-length model (turbulence model 1).
+ * gral_pf_momentum_sweep runs the sweep of component comp (0 U, 1 V, 2 W) in the
+ * directions of the situation's next iteration (U-, V- and W-prognostic-
+ * microscale_1.cs:50-529, called from PrognosticFlowfield.cs:1079-1101) on the
+ * device fields, and keeps the fields before it; gral_pf_restore_ckpt returns
+ * to them.  It does not count as an iteration of the situation.
+ * gral_pf_momentum_vjp returns dL/dX before the sweep from dL/dX after it (X the
+ * swept component; dense [NII+2][NJJ+2][NKK+2] arrays), with UKS, VKS, WKS,
+ * DPMNEW and the other components held fixed: the transposed Thomas solves and
+ * Gauss-Seidel couplings, the eddy viscosity's dependence on X through every
+ * power-law coefficient, and the Coriolis, vegetation, wall and cross terms.
+ * No reference counterpart (the reference has no adjoint).  Columns up to 126
+ * levels. */
+int gral_pf_momentum_sweep(gral_pf_handle *h, int32_t comp);
+int gral_pf_momentum_vjp(gral_pf_handle *h, const float *dx, float *gx);
+
 /* Copy the solver state to host buffers (each may be NULL), full depth
  * [nii+2][njj+2][nkk+2]: Program.UK/VK/WK/UKS/VKS/WKS/DPM/DPMNEW/TURB/TDISS.
  * Valid after prepare, or after upload_fields (+ diagnostic) for a purely

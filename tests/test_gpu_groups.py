@@ -1,9 +1,10 @@
 """Batches run in stream groups: situations split round-robin into up to 4
 groups, each on a stream of its own, with the group count chosen online (one
-timed iteration per candidate count, then the fastest) unless GRAL_PF_GROUPS
-fixes it.  The grouping must not change a bit: every situation equals the
-oracle's plain solve of it, for the tuned count, for each forced count, and
-when situations retire early and the group count's candidates change."""
+timed cycle of the four sweep directions per candidate count, then the largest
+count within 2 % of the fastest) unless GRAL_PF_GROUPS fixes it.  The grouping
+must not change a bit: every situation equals the oracle's plain solve of it,
+for the tuned count, for each forced count, and when situations retire early
+and the group count's candidates change."""
 import os
 
 import pytest
