@@ -1546,22 +1546,29 @@ __device__ __forceinline__ void chain_back(const float *__restrict__ P, const fl
                                            int nmax, float *sink) {
     float up = X[1];
     float cp[CH], cq[CH], cx[CH];
+    // each chunk addresses its CH levels from the lowest one upwards, so that
+    // the accesses pair up with positive offsets (ds_read2/ds_write2 from one
+    // base) and few enough reads are in flight for a partial LDS wait
+    {
+        const float *Pb = P - (CH - 1), *Qb = Q - (CH - 1), *Xb = X - (CH - 1);
 #pragma unroll
-    for (int u = 0; u < CH; ++u) { cp[u] = P[-u]; cq[u] = Q[-u]; cx[u] = X[-u]; }
+        for (int u = 0; u < CH; ++u) { cp[u] = Pb[CH - 1 - u]; cq[u] = Qb[CH - 1 - u]; cx[u] = Xb[CH - 1 - u]; }
+    }
 #pragma unroll
     for (int h = 0; h < MAXCH; ++h) {
         const int t = h * CH;
         if (t >= nmax) break;
         float np[CH], nq[CH], nx[CH];
         const int o = (t + CH < n) ? t + CH : 0;
+        const float *Pb = P - (o + CH - 1), *Qb = Q - (o + CH - 1), *Xb = X - (o + CH - 1);
 #pragma unroll
-        for (int u = 0; u < CH; ++u) { np[u] = P[-(o + u)]; nq[u] = Q[-(o + u)]; nx[u] = X[-(o + u)]; }
+        for (int u = 0; u < CH; ++u) { np[u] = Pb[CH - 1 - u]; nq[u] = Qb[CH - 1 - u]; nx[u] = Xb[CH - 1 - u]; }
         __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
-        float *Xw = (t < n) ? Xn : sink + (CH - 1) + t;
+        float *Xw = (t < n) ? Xn - (t + CH - 1) : sink;   // a lane past its end stores to its sink
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             float x = cx[u] + relax * (cp[u] * up + cq[u] - cx[u]);
-            Xw[-(t + u)] = x;
+            Xw[CH - 1 - u] = x;
             up = x;
         }
 #pragma unroll
@@ -1575,22 +1582,27 @@ __device__ __forceinline__ void chain_pdown(const float *__restrict__ D, const f
                                             float qtop, int n, int nmax, float *sink) {
     float Qn = qtop;
     float ca[CH], cd[CH], ct[CH];
+    // chunks addressed from their lowest level upwards, as in chain_back
+    {
+        const float *Qb = Q - (CH - 1), *Db = D - (CH - 1), *Tb = T - (CH - 1);
 #pragma unroll
-    for (int u = 0; u < CH; ++u) { ca[u] = Q[-u]; cd[u] = D[-u]; ct[u] = T[-u]; }
+        for (int u = 0; u < CH; ++u) { ca[u] = Qb[CH - 1 - u]; cd[u] = Db[CH - 1 - u]; ct[u] = Tb[CH - 1 - u]; }
+    }
 #pragma unroll
     for (int h = 0; h < MAXCH; ++h) {
         const int t = h * CH;
         if (t >= nmax) break;
         float na[CH], nd[CH], nt[CH];
         const int o = (t + CH < n) ? t + CH : 0;
+        const float *Qb = Q - (o + CH - 1), *Db = D - (o + CH - 1), *Tb = T - (o + CH - 1);
 #pragma unroll
-        for (int u = 0; u < CH; ++u) { na[u] = Q[-(o + u)]; nd[u] = D[-(o + u)]; nt[u] = T[-(o + u)]; }
+        for (int u = 0; u < CH; ++u) { na[u] = Qb[CH - 1 - u]; nd[u] = Db[CH - 1 - u]; nt[u] = Tb[CH - 1 - u]; }
         __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
-        float *Qw = (t < n) ? Q : sink + (CH - 1) + t;
+        float *Qw = (t < n) ? Q - (t + CH - 1) : sink;
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             float q = (ca[u] * Qn + cd[u]) * ct[u];
-            Qw[-(t + u)] = q;
+            Qw[CH - 1 - u] = q;
             Qn = q;
         }
 #pragma unroll
@@ -2515,25 +2527,32 @@ __device__ __forceinline__ void chain_tke_back(const float *__restrict__ P, cons
                                                float *sink) {
     float up = top;
     float cp[CH], cq[CH], cf[CH], cv[CH];
+    // chunks addressed from their lowest level upwards, as in chain_back
+    {
+        const float *Pb = P - (CH - 1), *Qb = Q - (CH - 1), *Fb = WF - (CH - 1), *Vb = WV - (CH - 1);
 #pragma unroll
-    for (int u = 0; u < CH; ++u) { cp[u] = P[-u]; cq[u] = Q[-u]; cf[u] = WF[-u]; cv[u] = WV[-u]; }
+        for (int u = 0; u < CH; ++u) {
+            cp[u] = Pb[CH - 1 - u]; cq[u] = Qb[CH - 1 - u]; cf[u] = Fb[CH - 1 - u]; cv[u] = Vb[CH - 1 - u];
+        }
+    }
 #pragma unroll
     for (int h = 0; h < MAXCH; ++h) {
         const int t = h * CH;
         if (t >= nmax) break;
         float np[CH], nq[CH], nf[CH], nv[CH];
         const int o = (t + CH < n) ? t + CH : 0;
+        const float *Pb = P - (o + CH - 1), *Qb = Q - (o + CH - 1), *Fb = WF - (o + CH - 1), *Vb = WV - (o + CH - 1);
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            np[u] = P[-(o + u)]; nq[u] = Q[-(o + u)]; nf[u] = WF[-(o + u)]; nv[u] = WV[-(o + u)];
+            np[u] = Pb[CH - 1 - u]; nq[u] = Qb[CH - 1 - u]; nf[u] = Fb[CH - 1 - u]; nv[u] = Vb[CH - 1 - u];
         }
         __builtin_amdgcn_sched_barrier(0);                // keep the next chunk's reads ahead of this chunk
-        float *Xw = (t < n) ? X : sink + (CH - 1) + t;
+        float *Xw = (t < n) ? X - (t + CH - 1) : sink;
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             float x = (cf[u] != 0.0f) ? cv[u] : cp[u] * up + cq[u];
             x = (float)mmaxd((double)x, floor_);
-            Xw[-(t + u)] = x;
+            Xw[CH - 1 - u] = x;
             up = x;
         }
 #pragma unroll

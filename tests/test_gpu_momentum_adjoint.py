@@ -21,8 +21,7 @@ exactly those fixed fields).
   eddy viscosity: with that path removed from the transpose the U cell at
   (60, 50, 14) gives -6.7e-5 against a central difference of 1.05e-3, and the V
   cell at (56, 52, 10) -8e-9 against 4.9e-4, so a dropped viscosity path
-  fails here even where the random directions above average it
-  away.
+  fails here even where the random directions above average it away.
 """
 import numpy as np
 import pytest
