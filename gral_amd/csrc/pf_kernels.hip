@@ -1801,17 +1801,26 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         // step as the row waves (s_barrier counts waves, wherever they are in
         // the code), so none of the row waves' loop-carried prefetch registers
         // is live in the recurrences.
-        for (int s = 0; s < nsteps; ++s) {
-            lds_barrier();                                    // the rows' sources are assembled
+        // The rows publish the column data of step s one phase early (before
+        // the barrier that ends the recurrences of step s-1), so the chain wave
+        // reads them and takes its wave-wide maxima while the rows assemble.
+        const int cl = lane < ROWS ? lane : 0;
+        bool on = false;
+        int KK = 0, VI = 0, nf = 0, nb = 0, nfmax = 0, nbmax = 0;
+        auto info = [&](int s) {
             const int *si = sinfo + (s & 1) * 5 * ROWS;
-            const int cl = lane < ROWS ? lane : 0;
-            const bool on = lane < ROWS && si[cl * 5];
-            const int KSTART = si[cl * 5 + 1], KK = si[cl * 5 + 2], VI = si[cl * 5 + 3], KKm = si[cl * 5 + 4];
+            on = lane < ROWS && si[cl * 5];
+            const int KSTART = si[cl * 5 + 1], KKm = si[cl * 5 + 4];
+            KK = si[cl * 5 + 2]; VI = si[cl * 5 + 3];
             // first level the back-substitution updates (U/V skip levels inside
             // the building or its upstream neighbour; W updates every level)
             const int kend = (COMP == 2) ? KSTART : max(KSTART, KKm + 1);
-            const int nf = VI - KK - 1, nb = VI - kend + 1;
-            const int nfmax = wave_max(nf, on), nbmax = wave_max(nb, on);
+            nf = VI - KK - 1; nb = VI - kend + 1;
+            nfmax = wave_max(nf, on); nbmax = wave_max(nb, on);
+        };
+        for (int s = 0; s < nsteps; ++s) {
+            lds_barrier();                                    // the rows' sources are assembled
+            if (s == 0) info(0);
             if (on) {
                 float *D = sm + (size_t)cl * 6 * nkl + SLOT_PAD;
                 float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl, *Xn = Q + nkl;
@@ -1820,6 +1829,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                 chain_back(P + VI, Q + VI, Xs + VI, Xn + VI, d.relax, nb, nbmax, sdump + lane * CH);
             }
             lds_barrier();                                    // the recurrences are done
+            if (s + 1 < nsteps) info(s + 1);
         }
         return;
     }
@@ -1886,7 +1896,19 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
             }
         }
     };
+    // the chain wave's data of the column located for step s (one lane writes)
+    auto put_info = [&](int s) {
+        if (lane == 0) {
+            int *si = sinfo + (s & 1) * 5 * ROWS;
+            si[r * 5] = mn.adv;
+            if (mn.adv) {
+                const int KKm = (COMP == 0) ? mn.KKmi : (COMP == 1) ? mn.KKmj : mn.KK;
+                si[r * 5 + 1] = mn.KSTART; si[r * 5 + 2] = mn.KK; si[r * 5 + 3] = mn.VI; si[r * 5 + 4] = KKm;
+            }
+        }
+    };
     locate(0);
+    put_info(0);
     if (r == 0) {
         sw.wait(ROWS);
         locate_prev();
@@ -1913,7 +1935,6 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r) * nkl;
         const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r - 1) * nkl;
         const float *xs_new = sXn - (size_t)6 * nkl;         // row r-1's recurrence output
-        int *si = sinfo + (s & 1) * 5 * ROWS;
         const int *sp = sinfo + ((s - 1) & 1) * 5 * ROWS;
         const bool w_in = (s - r >= 1), s_in = (r >= 1);
         if (inr) {
@@ -1951,13 +1972,8 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                     if (k <= KKP1) sQ[k] = (KIND != 0) ? D / r_.t : D * r_.t;
                 }
             }
-            if (lane == 0) {
-                int KKm = (COMP == 0) ? mt.KKmi : (COMP == 1) ? mt.KKmj : q.KK;
-                si[r * 5 + 1] = q.KSTART; si[r * 5 + 2] = q.KK; si[r * 5 + 3] = q.VI; si[r * 5 + 4] = KKm;
-            }
         }
         const int kend = (COMP == 2) ? q.KSTART : max(q.KSTART, ((COMP == 0) ? mt.KKmi : mt.KKmj) + 1);
-        if (lane == 0) si[r * 5] = act;
         store_prev();
         pkend = act ? kend : 1 << 30;
         pvi = act ? q.VI : -1;
@@ -1972,6 +1988,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         // its wait.
         if (r == ROWS - 1 && s > 0 && s % ml_publish<COMP>() == 0) strip_publish(mine, s, lane);
         locate(s + 1);
+        if (s + 1 < nsteps) put_info(s + 1);
         if (r == 0) {
             sw.wait(s + 1 + ROWS);
             locate_prev();
@@ -2012,13 +2029,20 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
     const int B = sctl[0];
     if (B >= nstrip) return;
     if (chainw) {                                             // own loop, as in k_ml_strip
+        // column data of step s published by the rows one phase early, read
+        // while they assemble (as in k_ml_strip)
+        const int cl = lane < ROWS ? lane : 0;
+        bool on = false;
+        int KK2 = 0, VI2 = 0, rs2 = 0, nmax = 0;
+        auto info = [&](int s) {
+            const int *si = sinfo + (s & 1) * 4 * ROWS;
+            on = lane < ROWS && si[cl * 4];
+            KK2 = si[cl * 4 + 1]; VI2 = si[cl * 4 + 2]; rs2 = si[cl * 4 + 3];
+            nmax = wave_max(VI2 - KK2, on);
+        };
         for (int s = 0; s < nsteps; ++s) {
             lds_barrier();                                    // the rows' sources are assembled
-            const int *si = sinfo + (s & 1) * 4 * ROWS;
-            const int cl = lane < ROWS ? lane : 0;
-            const bool on = lane < ROWS && si[cl * 4];
-            const int KK2 = si[cl * 4 + 1], VI2 = si[cl * 4 + 2], rs2 = si[cl * 4 + 3];
-            const int nmax = wave_max(VI2 - KK2, on);
+            if (s == 0) info(0);
             if (on) {
                 float *Dd = sm + (size_t)cl * 5 * nkl + SLOT_PAD;
                 float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl, *N = P + nkl;
@@ -2028,6 +2052,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
                 chain_pup(P + KK2 + 1, Q + KK2 + 1, N + KK2 + 1, O[KK2], VI2 - KK2, nmax, sdump + lane * CH);
             }
             lds_barrier();                                    // the recurrences are done
+            if (s + 1 < nsteps) info(s + 1);
         }
         return;
     }
@@ -2105,7 +2130,16 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
             }
         }
     };
+    // the chain wave's data of the column located for step s (one lane writes)
+    auto put_info = [&](int s) {
+        if (lane == 0) {
+            int *si = sinfo + (s & 1) * 4 * ROWS;
+            si[r * 4] = actn;
+            if (actn) { si[r * 4 + 1] = KKn; si[r * 4 + 2] = VIn; si[r * 4 + 3] = (s - r) & (RING - 1); }
+        }
+    };
     locate(0);
+    put_info(0);
     if (r == 0) {
         sw.wait(ROWS);
         locate_prev();
@@ -2150,7 +2184,6 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
         const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r) * nkl;
         const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r - 1) * nkl;
         const float *xs_new = sN - (size_t)5 * nkl;          // row r-1's new DPM
-        int *si = sinfo + (s & 1) * 4 * ROWS;
         const int *sp = sinfo + ((s - 1) & 1) * 4 * ROWS;
         const bool w_in = (s - r >= 1), s_in = (r >= 1);
         finish_ring();
@@ -2187,9 +2220,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
                     sQ[k] = satp[k];                  // consumed and replaced by QIMP in phase 2
                 }
             }
-            if (lane == 0) { si[r * 4 + 1] = KK; si[r * 4 + 2] = VI; si[r * 4 + 3] = rs; }
         }
-        if (lane == 0) si[r * 4] = act;
         finish_store(xw_lds);
         pkk = KK; pvi = act ? VI : -1; prs = rs; pj = j; pinr = inr; po = c3(d, i, j);
         dnp[0] = pre[0].dn; dnp[1] = pre[1].dn;
@@ -2199,6 +2230,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
         // after its wait
         if (r == ROWS - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);
         locate(s + 1);
+        if (s + 1 < nsteps) put_info(s + 1);
         if (r == 0) {
             sw.wait(s + 1 + ROWS);
             locate_prev();
@@ -2605,14 +2637,22 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
     const int B = sctl[0];
     if (B >= nstrip) return;                                  // uniform over the workgroup
     if (chainw) {
+        // the rows publish the levels of step s one phase early (as in
+        // k_ml_strip) and the values above the top with their sources
+        const int f = lane >= ROWS ? 1 : 0, cl = lane < 2 * ROWS ? lane - f * ROWS : 0;
+        bool on = false;
+        int lo = 0, VI = 0, nfmax = 0, nbmax = 0;
+        auto info = [&](int s) {
+            const int *si = sinfo + (s & 1) * 5 * ROWS;
+            on = lane < 2 * ROWS && si[cl * 5];
+            lo = si[cl * 5 + 1]; VI = si[cl * 5 + 2];
+            nfmax = wave_max(VI - lo, on); nbmax = wave_max(VI - lo + 1, on);
+        };
         for (int s = 0; s < nsteps; ++s) {
             lds_barrier();                                    // the rows' sources are assembled
+            if (s == 0) info(0);
             const int *si = sinfo + (s & 1) * 5 * ROWS;
-            const int f = lane >= ROWS ? 1 : 0, cl = lane < 2 * ROWS ? lane - f * ROWS : 0;
-            const bool on = lane < 2 * ROWS && si[cl * 5];
-            const int lo = si[cl * 5 + 1], VI = si[cl * 5 + 2];
             const float top = __int_as_float(si[cl * 5 + 3 + f]);
-            const int nfmax = wave_max(VI - lo, on), nbmax = wave_max(VI - lo + 1, on);
             if (on) {
                 float *R = sm + (size_t)cl * TKE_SLOTS * nkl + SLOT_PAD;
                 float *Dv = R + (size_t)f * nkl, *Cv = R + (size_t)(2 + 3 * f) * nkl, *Nv = R + (size_t)(3 + 3 * f) * nkl;
@@ -2623,6 +2663,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
                                nbmax, sdump + lane * CH);
             }
             lds_barrier();                                    // the recurrences are done
+            if (s + 1 < nsteps) info(s + 1);
         }
         return;
     }
@@ -2688,7 +2729,16 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
             }
         }
     };
+    // the chain wave's levels of the column located for step s (one lane writes)
+    auto put_info = [&](int s) {
+        if (lane == 0) {
+            int *si = sinfo + (s & 1) * 5 * ROWS;
+            si[r * 5] = mn.adv;
+            if (mn.adv) { si[r * 5 + 1] = mn.KK + 1; si[r * 5 + 2] = mn.VI; }
+        }
+    };
     locate(0);
+    put_info(0);
     if (r == 0) {
         sw.wait(ROWS);
         locate_prev();
@@ -2736,12 +2786,8 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
                     if (k == lo) { sQ1[k] = D1 / c3v.x; sQ2[k] = D2 / c3v.z; }
                 }
             }
-            if (lane == 0) {
-                si[r * 5 + 1] = lo; si[r * 5 + 2] = VI;
-                si[r * 5 + 3] = __float_as_int(topt); si[r * 5 + 4] = __float_as_int(tope);
-            }
+            if (lane == 0) { si[r * 5 + 3] = __float_as_int(topt); si[r * 5 + 4] = __float_as_int(tope); }
         }
-        if (lane == 0) si[r * 5] = act;
         store_prev();
         plo = act ? mt.KK + 1 : 1 << 30;
         pvi = act ? mt.VI : -1;
@@ -2749,6 +2795,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
         lds_barrier();
         if (r == ROWS - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);
         locate(s + 1);
+        if (s + 1 < nsteps) put_info(s + 1);
         if (r == 0) {
             sw.wait(s + 1 + ROWS);
             locate_prev();
