@@ -1356,7 +1356,7 @@ __device__ __forceinline__ void lds_barrier() {
 // same relative level hit different banks
 __device__ __forceinline__ int lds_stride(int nkp) { return nkp | 1; }
 
-#define COEF_COLS 32
+#define COEF_COLS 16
 #define SCOL 16           // words of column data per column in k_ml_coef
 // All neighbour-independent work of one mixing-length sweep for every column
 // at once.  A block of 4 waves takes COEF_COLS consecutive columns of the
