@@ -189,6 +189,7 @@ struct gral_pf_handle {
     int *adj_src = nullptr;         // stale-top source column per column, for IS = +1 and -1
     int adj_first = 0, adj_n = 0;   // iterations of the last forward
     int mom_comp = -1, mom_is = 1, mom_js = 1;   // the last gral_pf_momentum_sweep: component, directions
+    bool mom_vjp_ok = false;                      // no call has changed the device fields since that sweep
     GraphCache *graphs = nullptr;   // captured iteration chunks (GRAL_PF_GRAPH=1)
     size_t dev_cap = 0;             // optional cap (GRAL_PF_DEVICE_MB), 0 = none
     std::vector<float> uk, vk, wk, uks, vks, wks, dpm, dpmnew, turb, tdiss;
@@ -387,6 +388,7 @@ static void free_device(gral_pf_handle *h) {
     h->adj_src = nullptr;
     h->adj_n = 0;
     h->mom_comp = -1;
+    h->mom_vjp_ok = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -1188,6 +1190,7 @@ static void run_lockstep(const std::vector<gral_pf_handle *> &hs, Dom *dd, hipSt
             }
             if (h->iter_loops > h->prm.iter_max) h->done = true;
             h->host_stale = true;
+            h->mom_vjp_ok = false;
             if (!h->done) keep.push_back(h);
         }
         if (keep.size() != act.size()) {
@@ -1379,6 +1382,7 @@ extern "C" int gral_pf_forward_ckpt(gral_pf_handle *h, int32_t n, int32_t quanti
     h->adj_first = h->iter_loops;
     h->adj_n = n;
     h->mom_comp = -1;
+    h->mom_vjp_ok = false;
     h->host_stale = true;
     h->gff_ok = false;
     return GRAL_PF_OK;
@@ -1396,6 +1400,7 @@ extern "C" int gral_pf_restore_ckpt(gral_pf_handle *h) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->host_stale = true;
     h->gff_ok = false;
+    h->mom_vjp_ok = false;
     return GRAL_PF_OK;
 }
 
@@ -1454,6 +1459,7 @@ extern "C" int gral_pf_set_velocity(gral_pf_handle *h, const float *uk, const fl
     par_assign(h->uk, uk, n3);
     par_assign(h->vk, vk, n3);
     par_assign(h->wk, wk, n3);
+    h->mom_vjp_ok = false;
     if ((rc = put_field(h, h->d.uk, h->uk)) || (rc = put_field(h, h->d.vk, h->vk)) || (rc = put_field(h, h->d.wk, h->wk)))
         return rc;
     h->gff_ok = false;
@@ -1503,6 +1509,7 @@ extern "C" int gral_pf_momentum_sweep(gral_pf_handle *h, int32_t comp) {
     h->mom_comp = comp;
     h->mom_is = IS;
     h->mom_js = JS;
+    h->mom_vjp_ok = true;
     h->adj_n = 0;
     h->host_stale = true;
     h->gff_ok = false;
@@ -1511,7 +1518,8 @@ extern "C" int gral_pf_momentum_sweep(gral_pf_handle *h, int32_t comp) {
 
 extern "C" int gral_pf_momentum_vjp(gral_pf_handle *h, const float *dx, float *gx) {
     if (!h) return GRAL_PF_EINVAL;
-    if (!h->prepared || h->mom_comp < 0) return fail(h, GRAL_PF_ESTATE, "momentum_vjp: call momentum_sweep first");
+    if (!h->prepared || h->mom_comp < 0 || !h->mom_vjp_ok)
+        return fail(h, GRAL_PF_ESTATE, "momentum_vjp: no momentum_sweep since the device fields last changed");
     if (!dx || !gx) return fail(h, GRAL_PF_EINVAL, "momentum_vjp: null argument");
     const size_t n2 = h->n2(), n3 = n2 * (size_t)h->nkp;
     float *a = h->adj;
@@ -1829,6 +1837,7 @@ extern "C" int gral_pf_diagnostic(gral_pf_handle *h, int32_t flow_field_level, i
     if (!h) return GRAL_PF_EINVAL;
     if (flow_field_level != 1 && flow_field_level != 2) return fail(h, GRAL_PF_EINVAL, "diagnostic: flow_field_level must be 1 or 2");
     if (!h->masks || !h->fields) return fail(h, GRAL_PF_ESTATE, "diagnostic: masks and fields must be set first");
+    h->mom_vjp_ok = false;
     int rc = 0;
     if (h->prepared && (rc = sync_host(h))) return rc;
     const size_t n2 = h->n2(), nkh = (size_t)h->nkh, n3 = n2 * nkh;

@@ -210,6 +210,9 @@ int gral_pf_set_velocity(gral_pf_handle *h, const float *uk, const float *vk, co
  * DPMNEW and the other components held fixed: the transposed Thomas solves and
  * Gauss-Seidel couplings, the eddy viscosity's dependence on X through every
  * power-law coefficient, and the Coriolis, vegetation, wall and cross terms.
+ * It refers to the last sweep and returns GRAL_PF_ESTATE once any call has
+ * changed the device fields since (iterate, set_velocity, restore_ckpt,
+ * diagnostic, a batch, prepare).
  * No reference counterpart (the reference has no adjoint).  Columns up to 126
  * levels. */
 int gral_pf_momentum_sweep(gral_pf_handle *h, int32_t comp);

@@ -166,3 +166,39 @@ def test_momentum_vjp_wake_cells(comp):
             print(f"[momentum adjoint] {f} ({i}, {j}): d X'[{k0}] / d X[{kp}]: central difference {fd:.7g}, "
                   f"adjoint {ad:.7g}", flush=True)
             assert ad == pytest.approx(fd, rel=1e-2)
+
+
+@pytest.mark.gpu
+def test_momentum_entry_point_errors():
+    """Call order and arguments: a VJP before any sweep and a sweep before
+    prepare return GRAL_PF_ESTATE, a component outside 0..2 and the k-eps model
+    GRAL_PF_EINVAL; a failed call changes no field."""
+    from gral_amd import pf
+    print("[momentum adjoint] error returns", flush=True)
+    c = CASES["config1"]()
+    with pf.PrognosticFlowfield(c) as sv:
+        with pytest.raises(RuntimeError, match=r"\(-4\)"):
+            sv.momentum_sweep(0)
+        with pytest.raises(RuntimeError, match=r"\(-4\)"):
+            sv.set_velocity(c.uk, c.vk, c.wk)
+        x0 = _setup(sv, 1)
+        with pytest.raises(RuntimeError, match=r"\(-4\)"):
+            sv.momentum_vjp(np.zeros(c.uk.shape, dtype=np.float32))
+        for comp in (-1, 3):
+            with pytest.raises(RuntimeError, match=r"\(-1\)"):
+                sv.momentum_sweep(comp)
+        after = sv.download()
+        for k in FIELD:
+            assert np.array_equal(after[k], x0[k]), k
+        # a VJP refers to the fields its sweep left: any change since voids it
+        for change in (lambda: sv.iterate(1), lambda: sv.set_velocity(x0["uk"], x0["vk"], x0["wk"]),
+                       sv.restore_ckpt):
+            sv.momentum_sweep(1)
+            sv.momentum_vjp(np.ones(c.uk.shape, dtype=np.float32))
+            change()
+            with pytest.raises(RuntimeError, match=r"\(-4\)"):
+                sv.momentum_vjp(np.ones(c.uk.shape, dtype=np.float32))
+    with pf.PrognosticFlowfield(domain.config1(turbulence_model=2, iterations=12)) as sv:
+        sv.prepare(iterations=12)
+        with pytest.raises(RuntimeError, match=r"\(-1\)"):
+            sv.momentum_sweep(0)
