@@ -1720,8 +1720,16 @@ struct PrPre {
 // Measured on the 8-direction city batch (ms per launch, publishing every 1 / 2 /
 // 4 / 6 / 8 steps): U and V sweeps 7.6-8.2 / 5.4-5.5 / 4.29 / 4.10 / 4.1, W 7.7 /
 // 5.2 / 4.23 / 4.29 / 4.47, pressure 6.9 / 4.6 / 3.87 / 3.95 / 3.94.
+#ifndef STRIP_PUBLISH
 #define STRIP_PUBLISH 4
-template <int COMP> __device__ __forceinline__ constexpr int ml_publish() { return COMP == 2 ? 4 : 6; }
+#endif
+#ifndef ML_PUBLISH_UV
+#define ML_PUBLISH_UV 6
+#endif
+#ifndef ML_PUBLISH_W
+#define ML_PUBLISH_W 4
+#endif
+template <int COMP> __device__ __forceinline__ constexpr int ml_publish() { return COMP == 2 ? ML_PUBLISH_W : ML_PUBLISH_UV; }
 #define STRIP_TIMEOUT_TICKS 200000000ull
 
 __device__ __forceinline__ int load_agent(const gi32 *p) {
@@ -1729,6 +1737,15 @@ __device__ __forceinline__ int load_agent(const gi32 *p) {
 }
 __device__ __forceinline__ void store_agent(gi32 *p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The values a strip hands to the next one (the new last row, the pressure row
+// scratch) are read with agent-scope loads: they are served coherently across
+// the XCDs' L2 caches, so the reader needs no L2 invalidation (an agent-scope
+// acquire fence, which would also drop every other workgroup's cached lines on
+// that XCD) - only the order after its progress poll, which the poll's
+// control dependency and a compiler fence give.
+__device__ __forceinline__ float ld_handoff(const gf32 *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct StripWait {
@@ -1752,7 +1769,9 @@ struct StripWait {
             }
             __builtin_amdgcn_s_sleep(2);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // the hand-off loads that follow are agent-scope loads (ld_handoff):
+        // keep the compiler from moving them above the poll
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 };
 
@@ -1892,7 +1911,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
                 int k = mn.KSTART + lane + WAVE * m;
-                if (k <= mn.VI) pre[m].nj = X[c3(d, in_, jn_ - JS) + k];
+                if (k <= mn.VI) pre[m].nj = ld_handoff(X + c3(d, in_, jn_ - JS) + k);
             }
         }
     };
@@ -2118,7 +2137,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
 #pragma unroll
             for (int m2 = 0; m2 < 2; ++m2) {
                 int k = lane + WAVE * m2;
-                if (k < nkp) qfull[m2] = d.prow_q[(size_t)jn_ * nkp + k];
+                if (k < nkp) qfull[m2] = ld_handoff(d.prow_q + (size_t)jn_ * nkp + k);
             }
         }
         if (actn) {
@@ -2126,7 +2145,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
 #pragma unroll
             for (int m2 = 0; m2 < 2; ++m2) {
                 int k = KKn + 1 + lane + WAVE * m2;
-                if (k <= VIn) pre[m2].ni = d.dpm[o + k];
+                if (k <= VIn) pre[m2].ni = ld_handoff(d.dpm + o + k);
             }
         }
     };
@@ -2725,7 +2744,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
                 int k = mn.KK + 1 + lane + WAVE * m;
-                if (k <= mn.VI) { pre[m].njt = d.turb[oj + k]; pre[m].nje = d.tdiss[oj + k]; }
+                if (k <= mn.VI) { pre[m].njt = ld_handoff(d.turb + oj + k); pre[m].nje = ld_handoff(d.tdiss + oj + k); }
             }
         }
     };
