@@ -95,6 +95,15 @@ __device__ __forceinline__ int4 ldg4i(const T *p) {
     v4i v = *(const gv4i *)p;
     return make_int4(v.x, v.y, v.z, v.w);
 }
+// The same through the constant address space: a scalar load (counted apart
+// from the vector memory operations) where the address is uniform.  Only for
+// data no kernel writes while it runs.
+typedef v4i __attribute__((address_space(4))) cv4i;
+template <class T>
+__device__ __forceinline__ int4 ldc4i(const T *p) {
+    v4i v = *(const cv4i *)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
 
 // Flow-field file payload packer (GFFWrite.cs): fields u, v, w, turb, levels
 // below nlo from the solver's device fields (stride nkp), the rest from a copy

@@ -1655,12 +1655,13 @@ struct MlPre {
 };
 
 template <int COMP, int KIND>
-__device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *planes, size_t n3, int i, int j, int k, int IS, int JS,
-                                            MlPre &r) {
-    (void)n3;
-    const gf32 *P = planes + (c3(d, i, j) + k) * rec_floats<COMP>();
+__device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *Pc, const gf32 *R3, const gf32 *Xc,
+                                            const gf32 *Xa, const gf32 *Xb, unsigned k, MlPre &r) {
+    // Pc, R3, Xc, Xa, Xb: the column's records and values and its +IS / +JS
+    // neighbours (uniform bases, so the level offset k is the only per-lane part)
+    const gf32 *P = Pc + k * (unsigned)rec_floats<COMP>();
     float4 v0 = ldg4(P), v2 = ldg4(P + 4);
-    float3 v1 = ldg3(d.rec3 + (c3(d, i, j) + k) * 3);
+    float3 v1 = ldg3(R3 + k * 3u);
     r.aw = v0.x; r.as = v0.y; r.ae = v0.z; r.an = v0.w;
     r.c = v1.x; r.t = v1.y; r.p = v1.z;
     r.f = v2.x; r.g = v2.y; r.e = v2.z;
@@ -1672,10 +1673,9 @@ __device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *planes, s
         r.c0 = v2.w;
         r.x1 = v3.x; r.x2 = v3.y; r.x3 = v3.z; r.x4 = v3.w;
     }
-    const gf32 *X = COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk;
-    r.x = X[c3(d, i, j) + k];
-    r.xa = X[c3(d, i + IS, j) + k];
-    r.xb = X[c3(d, i, j + JS) + k];
+    r.x = Xc[k];
+    r.xa = Xa[k];
+    r.xb = Xb[k];
     if (KIND != 0) r.ap0 = d.ap0[k];
 }
 
@@ -1802,11 +1802,12 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
     const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     const bool chainw = r == ROWS;                            // the chain wave owns no row
     gi32 *sync = d.sync + SYNC_BLOCK0 + (size_t)(1 + COMP) * d.sync_stride;
-    // per row: D, C, T, P, Q and the back-substitution result Xn
-    float *sD = sm + (size_t)r * 6 * nkl + SLOT_PAD, *sC = sD + nkl, *sT = sC + nkl, *sP = sT + nkl, *sQ = sP + nkl;
-    float *sXn = sQ + nkl;
-    float *xbuf = sm + (size_t)ROWS * 6 * nkl + SLOT_PAD;      // [2][ROWS] slots of whole columns (old values)
-    float *sdzk = sm + (size_t)ROWS * 8 * nkl;                // [nkp]
+    // per row: D, C, T, P, Q and the back-substitution results Xn of even and
+    // odd steps (a row stores its previous column while the next one is solved)
+    float *sD = sm + (size_t)r * 7 * nkl + SLOT_PAD, *sC = sD + nkl, *sT = sC + nkl, *sP = sT + nkl, *sQ = sP + nkl;
+    float *sXn0 = sQ + nkl;                                   // [2][nkl] by step parity
+    float *xbuf = sm + (size_t)ROWS * 7 * nkl + SLOT_PAD;      // [2][ROWS] slots of whole columns (old values)
+    float *sdzk = sm + (size_t)ROWS * 9 * nkl;                // [nkp]
     int *sinfo = (int *)(sdzk + nkp);                         // [2][ROWS] per row: active, KSTART, KK, VI, KKm
     int *sctl = sinfo + 10 * ROWS;                            // strip index of this workgroup
     float *sdump = (float *)(sctl + 4);                       // [WAVE][CH] sinks of the chains
@@ -1841,8 +1842,8 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
             lds_barrier();                                    // the rows' sources are assembled
             if (s == 0) info(0);
             if (on) {
-                float *D = sm + (size_t)cl * 6 * nkl + SLOT_PAD;
-                float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl, *Xn = Q + nkl;
+                float *D = sm + (size_t)cl * 7 * nkl + SLOT_PAD;
+                float *Cc = D + nkl, *T = Cc + nkl, *P = T + nkl, *Q = P + nkl, *Xn = Q + nkl + (size_t)(s & 1) * nkl;
                 const float *Xs = xbuf + (size_t)((s & 1) * ROWS + cl) * nkl;
                 chain_fwd<KIND != 0>(D + KK + 2, Cc + KK + 2, T + KK + 2, Q + KK + 2, nf, nfmax, Q[KK + 1], sdump + lane * CH);
                 chain_back(P + VI, Q + VI, Xs + VI, Xn + VI, d.relax, nb, nbmax, sdump + lane * CH);
@@ -1857,7 +1858,6 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
     StripWait sw{B > 0 ? sync + SYNC_HDR + B - 1 : nullptr, d.sync, B > 0 ? 0 : 0x7fffffff, nsteps,
                  wall_clock64() + STRIP_TIMEOUT_TICKS};
     gi32 *mine = sync + SYNC_HDR + B;
-    const size_t n3 = (size_t)(d.nii + 2) * d.nj2 * nkp;
     gf32 *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk);
     MlPre pre[2];
     float xfull[2];
@@ -1871,7 +1871,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         int l = s - r, i, j;
         if (row_ok && l >= 0 && oa + l <= ob) {
             sweep_to_ij(d, 0, oa + l, b, IS, JS, i, j);
-            return ldg4i(d.meta + c2(d, i, j));
+            return ldc4i(d.meta + c2(d, i, j));
         }
         return make_int4(0, -1, 0, 0);
     };
@@ -1884,22 +1884,27 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         mn.adv = false;
         if (inr_n) {
             sweep_to_ij(d, 0, oa + l, b, IS, JS, in_, jn_);
+            in_ = __builtin_amdgcn_readfirstlane(in_);        // uniform: column bases in scalar registers
+            jn_ = __builtin_amdgcn_readfirstlane(jn_);
             mn = unpack_meta(m4);
-            const size_t o = c3(d, in_, jn_);
+            const gf32 *Xc = X + c3(d, in_, jn_);
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
-                int k = lane + WAVE * m;
-                if (k < nkp) xfull[m] = X[o + k];
+                const unsigned k = lane + WAVE * m;
+                if (k < (unsigned)nkp) xfull[m] = Xc[k];
             }
         }
         if (mn.adv) {
             if (COMP != 2) wn = d.wcol[c2(d, in_, jn_)];
+            const size_t o = c3(d, in_, jn_);
+            const gf32 *Pc = planes + o * rec_floats<COMP>(), *R3 = d.rec3 + o * 3, *Xc = X + o;
+            const gf32 *Xa = X + c3(d, in_ + IS, jn_), *Xb = X + c3(d, in_, jn_ + JS), *Xi = X + c3(d, in_ - IS, jn_);
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
-                int k = mn.KSTART + lane + WAVE * m;
+                const int k = mn.KSTART + lane + WAVE * m;
                 if (k <= mn.VI) {
-                    ml_prefetch<COMP, KIND>(d, planes, n3, in_, jn_, k, IS, JS, pre[m]);
-                    if (l == 0) pre[m].ni = X[c3(d, in_ - IS, jn_) + k];
+                    ml_prefetch<COMP, KIND>(d, Pc, R3, Xc, Xa, Xb, (unsigned)k, pre[m]);
+                    if (l == 0) pre[m].ni = Xi[(unsigned)k];
                 }
             }
         }
@@ -1911,7 +1916,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
                 int k = mn.KSTART + lane + WAVE * m;
-                if (k <= mn.VI) pre[m].nj = ld_handoff(X + c3(d, in_, jn_ - JS) + k);
+                if (k <= mn.VI) pre[m].nj = ld_handoff(X + c3(d, in_, jn_ - JS) + (unsigned)k);
             }
         }
     };
@@ -1933,16 +1938,20 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         locate_prev();
     }
     // The results of a step are merged on use: level k of a row's previous
-    // column is the recurrence output (sXn) where the back-substitution updated
-    // it (kend..VI), else the old value in the column buffer.  Each row stores
-    // the updated levels of its previous column at the end of the next step's
-    // assembly (before that step's publish, so progress p still means that
-    // steps < p are stored), after the assembly's waits for its prefetched
-    // loads, which therefore never wait for these write-through stores.
+    // column is the recurrence output (Xn of that step's parity) where the
+    // back-substitution updated it (kend..VI), else the old value in the column
+    // buffer.  Each row stores the updated levels of its previous column in the
+    // next step's second phase, after issuing that phase's look-ahead loads:
+    // vector-memory waits count loads and stores in one ordered counter, so a
+    // store issued before those loads would make every wait for them (in the
+    // look-ahead itself and in the next assembly) wait for the write as well.
+    // The stores of step s-1 are issued in step s, so the progress a strip
+    // publishes in step s is s-1 (steps < s-1 stored).
     int pkend = 1 << 30, pvi = -1;                            // own previous column: updated levels
     size_t po = 0;
-    auto store_prev = [&]() {
-        for (int k = pkend + lane; k <= pvi; k += WAVE) st_through(X + po + k, sXn[k]);
+    auto store_prev = [&](const float *xn) {
+        gf32 *Xp = X + po;
+        for (int k = pkend + lane; k <= pvi; k += WAVE) st_through(Xp + (unsigned)k, xn[k]);
     };
     for (int s = 0; s < nsteps; ++s) {
         const bool act = mn.adv, inr = inr_n;
@@ -1953,7 +1962,8 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         float *xcur = xbuf + (size_t)((s & 1) * ROWS + r) * nkl;
         const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r) * nkl;
         const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r - 1) * nkl;
-        const float *xs_new = sXn - (size_t)6 * nkl;         // row r-1's recurrence output
+        const float *xprev = sXn0 + (size_t)((s - 1) & 1) * nkl;   // recurrence output of step s-1
+        const float *xs_new = xprev - (size_t)7 * nkl;       // row r-1's
         const int *sp = sinfo + ((s - 1) & 1) * 5 * ROWS;
         const bool w_in = (s - r >= 1), s_in = (r >= 1);
         if (inr) {
@@ -1977,7 +1987,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                 int k = q.KSTART + lane + WAVE * m;
                 if (k <= q.VI) {
                     const MlPre &r_ = pre[m];
-                    float xnew_i = w_in ? ((k >= pkend && k <= pvi) ? sXn[k] : xw_lds[k]) : r_.ni;
+                    float xnew_i = w_in ? ((k >= pkend && k <= pvi) ? xprev[k] : xw_lds[k]) : r_.ni;
                     float xnew_j = s_in ? ((k >= skend && k <= svi) ? xs_new[k] : xs_lds[k]) : r_.nj;
                     float xw = (IS == 1) ? xnew_i : r_.xa, xe = (IS == 1) ? r_.xa : xnew_i;
                     float xs = (JS == 1) ? xnew_j : r_.xb, xn = (JS == 1) ? r_.xb : xnew_j;
@@ -1993,28 +2003,28 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
             }
         }
         const int kend = (COMP == 2) ? q.KSTART : max(q.KSTART, ((COMP == 0) ? mt.KKmi : mt.KKmj) + 1);
-        store_prev();
-        pkend = act ? kend : 1 << 30;
-        pvi = act ? q.VI : -1;
-        po = c3(d, i, j);
         lds_barrier();
         // while the chain wave runs the recurrences: the last row publishes the
-        // steps before this one (their stores were issued at the top of this
-        // step), and every row prefetches its next column.  The
-        // look-ahead of row r reads columns that rows r and r+1 write two steps
-        // later at the earliest (for the last row, the next strip, which waits
-        // for this one); row 0 also reads the previous strip's last row, after
-        // its wait.
-        if (r == ROWS - 1 && s > 0 && s % ml_publish<COMP>() == 0) strip_publish(mine, s, lane);
+        // steps before the previous one (their stores were issued one step
+        // ago), every row prefetches its next column, then stores its previous
+        // one.  The look-ahead of row r reads columns that rows r and r+1 write
+        // two steps later at the earliest (for the last row, the next strip,
+        // which waits for this one); row 0 also reads the previous strip's last
+        // row, after its wait.
+        if (r == ROWS - 1 && s > 1 && (s - 1) % ml_publish<COMP>() == 0) strip_publish(mine, s - 1, lane);
         locate(s + 1);
         if (s + 1 < nsteps) put_info(s + 1);
         if (r == 0) {
             sw.wait(s + 1 + ROWS);
             locate_prev();
         }
+        store_prev(xprev);
+        pkend = act ? kend : 1 << 30;
+        pvi = act ? q.VI : -1;
+        po = c3(d, i, j);
         lds_barrier();                                        // the recurrence outputs are ready
     }
-    store_prev();
+    store_prev(sXn0 + (size_t)((nsteps - 1) & 1) * nkl);
     if (r == ROWS - 1) strip_publish(mine, nsteps, lane);
 }
 
@@ -2090,7 +2100,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
         int l = s - r, i, j;
         if (row_ok && l >= 0 && oa + l <= ob) {
             sweep_to_ij(d, 1, oa + l, b, IS, JS, i, j);
-            return ldg4i(d.meta + c2(d, i, j));
+            return ldc4i(d.meta + c2(d, i, j));
         }
         return make_int4(0, -1, 0, 0);
     };
@@ -2700,7 +2710,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
         int l = s - r, i, j;
         if (row_ok && l >= 0 && oa + l <= ob) {
             sweep_to_ij(d, 0, oa + l, b, IS, JS, i, j);
-            return ldg4i(d.meta + c2(d, i, j));
+            return ldc4i(d.meta + c2(d, i, j));
         }
         return make_int4(0, -1, 0, 0);
     };
@@ -2920,7 +2930,7 @@ static int launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, in
     if (d0.tiled == 2) {
         const size_t shm_p = sizeof(float) * ((size_t)(7 * ROWS + RING) * lds_slot_h(nkp) + 2 * nkp + 8 + WAVE * CH) +
                              sizeof(int) * (9 * ROWS + 4);
-        const size_t shm_m = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) +
+        const size_t shm_m = sizeof(float) * ((size_t)9 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) +
                              sizeof(int) * (10 * ROWS + 4);
         int bad = lds_limit((const void *)k_pressure_strip, shm_p);
         if (d0.model == 1) {
@@ -3001,7 +3011,7 @@ static int launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, in
         }
         size_t shm_c = sizeof(float) * 3 * (nkp | 1) * COEF_COLS + sizeof(int) * (SCOL * COEF_COLS + 4);
         L.go(FAM_COEF, [&] { hipLaunchKernelGGL(coef, dim3(nblk, 1, Z), dim3(4 * WAVE), shm_c, s, dd, is, js); });
-        size_t shm_t = sizeof(float) * ((size_t)8 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) + sizeof(int) * (10 * ROWS + 4);
+        size_t shm_t = sizeof(float) * ((size_t)9 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) + sizeof(int) * (10 * ROWS + 4);
         if (lds_limit((const void *)strip, shm_t)) { lds_err = 1; return; }
         L.go(fam, [&] { hipLaunchKernelGGL(strip, dim3(nst, 1, Z), dim3(STRIP_WAVES * WAVE), shm_t, s, dd, is, js); });
     };
