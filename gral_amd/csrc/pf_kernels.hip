@@ -1714,20 +1714,23 @@ struct PrPre {
 // kernel always ends; the host reports the error.
 // ------------------------------------------------------------------------
 #define SYNC_HDR 2
-// a strip publishes its progress every STRIP_PUBLISH steps (and at its end): the
-// release (L2 write-back) and the reader's acquire (L2 invalidate) are paid once
-// per STRIP_PUBLISH steps, for a lag of up to STRIP_PUBLISH-1 more steps per strip.
-// Measured on the 8-direction city batch (ms per launch, publishing every 1 / 2 /
-// 4 / 6 / 8 steps): U and V sweeps 7.6-8.2 / 5.4-5.5 / 4.29 / 4.10 / 4.1, W 7.7 /
-// 5.2 / 4.23 / 4.29 / 4.47, pressure 6.9 / 4.6 / 3.87 / 3.95 / 3.94.
+// a strip publishes its progress every STRIP_PUBLISH steps (and at its end); a
+// longer interval adds up to STRIP_PUBLISH-1 steps of lag per strip.  The
+// publish is a wait for the strip's own write-through stores, and the reader
+// takes the hand-off values with agent-scope loads, so neither side writes back
+// or invalidates an L2.  Measured on the 8-direction city batch (ms per launch,
+// publishing every 1 / 2 / 3 / 4 steps): U and V sweeps 2.8 / 2.9 / 2.95 / 3.0,
+// W 2.77 / 2.84 / 2.96 / 3.06, pressure 2.62 / 2.78 / 2.81 / 2.95.  (With an
+// agent-scope release fence, which writes back the whole L2 of the XCD, every
+// publish cost about 4 us: 7.6-8.2 ms for U and V at every step, 4.1 at every 6th.)
 #ifndef STRIP_PUBLISH
-#define STRIP_PUBLISH 4
+#define STRIP_PUBLISH 1
 #endif
 #ifndef ML_PUBLISH_UV
-#define ML_PUBLISH_UV 6
+#define ML_PUBLISH_UV 1
 #endif
 #ifndef ML_PUBLISH_W
-#define ML_PUBLISH_W 4
+#define ML_PUBLISH_W 1
 #endif
 template <int COMP> __device__ __forceinline__ constexpr int ml_publish() { return COMP == 2 ? ML_PUBLISH_W : ML_PUBLISH_UV; }
 #define STRIP_TIMEOUT_TICKS 200000000ull
@@ -1783,9 +1786,13 @@ __device__ __forceinline__ void st_through(gf32 *p, float v) {
 }
 
 // called by every lane of the strip's last wave: progress v (steps whose
-// stores this wave issued) becomes visible with the stores before it
+// stores this wave issued) becomes visible with the stores before it.  Those
+// stores are agent-scope write-through stores: once complete they are visible
+// to the agent-scope loads of every XCD, so the release is a wait for them
+// (no write-back of the whole L2, as an agent-scope release fence would do)
+// followed by the progress store.
 __device__ __forceinline__ void strip_publish(gi32 *mine, int v, int lane) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (lane == 0) store_agent(mine, v);
 }
 
