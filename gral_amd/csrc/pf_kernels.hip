@@ -1654,14 +1654,22 @@ struct MlPre {
     float ap0;            // transient coefficient of the level (k-eps)
 };
 
+// p + byte_off with a 32-bit byte offset: with a uniform base the access can
+// take the scalar-base form (base in scalar registers, 32-bit lane offset), so
+// no 64-bit address arithmetic per lane
+__device__ __forceinline__ const gf32 *at_bytes(const gf32 *base, unsigned byte_off) {
+    typedef const char __attribute__((address_space(1))) gc8;
+    return (const gf32 *)((const gc8 *)base + byte_off);
+}
 template <int COMP, int KIND>
 __device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *Pc, const gf32 *R3, const gf32 *Xc,
                                             const gf32 *Xa, const gf32 *Xb, unsigned k, MlPre &r) {
     // Pc, R3, Xc, Xa, Xb: the column's records and values and its +IS / +JS
-    // neighbours (uniform bases, so the level offset k is the only per-lane part)
-    const gf32 *P = Pc + k * (unsigned)rec_floats<COMP>();
+    // neighbours (uniform bases, so the level's byte offset is the only
+    // per-lane part of each address)
+    const gf32 *P = at_bytes(Pc, k * (unsigned)(4 * rec_floats<COMP>()));
     float4 v0 = ldg4(P), v2 = ldg4(P + 4);
-    float3 v1 = ldg3(R3 + k * 3u);
+    float3 v1 = ldg3(at_bytes(R3, k * 12u));
     r.aw = v0.x; r.as = v0.y; r.ae = v0.z; r.an = v0.w;
     r.c = v1.x; r.t = v1.y; r.p = v1.z;
     r.f = v2.x; r.g = v2.y; r.e = v2.z;
@@ -1673,9 +1681,9 @@ __device__ __forceinline__ void ml_prefetch(const DomG &d, const gf32 *Pc, const
         r.c0 = v2.w;
         r.x1 = v3.x; r.x2 = v3.y; r.x3 = v3.z; r.x4 = v3.w;
     }
-    r.x = Xc[k];
-    r.xa = Xa[k];
-    r.xb = Xb[k];
+    r.x = *at_bytes(Xc, 4u * k);
+    r.xa = *at_bytes(Xa, 4u * k);
+    r.xb = *at_bytes(Xb, 4u * k);
     if (KIND != 0) r.ap0 = d.ap0[k];
 }
 
@@ -1898,7 +1906,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
                 const unsigned k = lane + WAVE * m;
-                if (k < (unsigned)nkp) xfull[m] = Xc[k];
+                if (k < (unsigned)nkp) xfull[m] = *at_bytes(Xc, 4u * k);
             }
         }
         if (mn.adv) {
@@ -1911,7 +1919,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                 const int k = mn.KSTART + lane + WAVE * m;
                 if (k <= mn.VI) {
                     ml_prefetch<COMP, KIND>(d, Pc, R3, Xc, Xa, Xb, (unsigned)k, pre[m]);
-                    if (l == 0) pre[m].ni = Xi[(unsigned)k];
+                    if (l == 0) pre[m].ni = *at_bytes(Xi, 4u * (unsigned)k);
                 }
             }
         }
