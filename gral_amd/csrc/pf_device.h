@@ -35,7 +35,7 @@ struct DomT {
     P<const double> tke_a, tke_den;  // TKE terrain wall function: 0.4/ln(.) and TDISS divisor
     P<const int> tke_br;             // 1: Z0 >= DZK*0.5 branch, 0: otherwise
     // fields
-    P<float> uk, vk, wk, uks, vks, wks, dpm, dpmnew, div, turb, tdiss;
+    P<float> uk, vk, wk, uks, vks, wks, dpm, dpmnew, div, turb, tdiss;   // div holds DIV / DTIME
     P<const float> veg;
     // scratch
     P<float> prow_p, prow_q;         // pressure PIMP/QIMP per sweep row [nj2][nkp]

@@ -181,7 +181,10 @@ __global__ void k_bc_ns(const Dom *__restrict__ ds) {
 
 // ------------------------------------------------------------------------
 // Mass divergence, PrognosticFlowfield.cs:723-758, over the active cells
-// (ADVDOM = 1, KKART < k <= VI); DPM = 0 there.
+// (ADVDOM = 1, KKART < k <= VI); DPM = 0 there.  The array keeps DIV / DTIME,
+// the only form the pressure sweep reads (its source term, :853), divided here
+// in the bandwidth-bound pass rather than on the sweep's serial path (the same
+// correctly rounded division, so the same bits).
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(CELL_THREADS) void k_divergence(const Dom *__restrict__ ds) {
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
@@ -211,6 +214,7 @@ __global__ __launch_bounds__(CELL_THREADS) void k_divergence(const Dom *__restri
                 const float t1 = (uu[u4] - uue[u4]) * d.dyk * dz;
                 const float t2 = (vv[u4] - vvn[u4]) * d.dxk * dz;
                 r[u4] = (k > m.KK + 1) ? t1 + t2 + (ww[u4] - ww[u4 + 1]) * d.area : t1 + t2 - ww[u4 + 1] * d.area;
+                r[u4] = r[u4] / d.dtime;
             }
         }
         if (on[0] && on[1] && on[2] && on[3]) {
@@ -248,7 +252,7 @@ __global__ __launch_bounds__(WAVE) void k_pressure(const Dom *__restrict__ ds, i
     float DXK = d.dxk, DYK = d.dyk;
     for (int k = KK + 1 + tid; k <= VI; k += WAVE) {
         float dz = d.dzk[k];
-        float DIM = d.div[o + k] / d.dtime + (d.dpm[oim + k] + d.dpm[oip + k]) / DXK * DYK * dz +
+        float DIM = d.div[o + k] + (d.dpm[oim + k] + d.dpm[oip + k]) / DXK * DYK * dz +
                     (d.dpm[ojm + k] - d.dpm[ojp + k]) / DYK * DXK * dz;
         sDIM[k] = DIM;
     }
@@ -1696,7 +1700,7 @@ __device__ __forceinline__ ColInfo meta_col(const Meta &m, int i, int j, int W) 
 
 // prefetched per-level values of one column of the pressure sweep
 struct PrPre {
-    float div, t, p, pa, pb;       // DIV, TERMP, PIMP, old DPM in the +IS (i) and +JS (j) directions
+    float div, t, p, pa, pb;       // DIV / DTIME, TERMP, PIMP, old DPM in the +IS (i) and +JS (j) directions
     float dn;                      // DPMNEW of the cell (read-modify-written one step later)
     float ni, nj;                  // -IS / -JS neighbours when another strip or the boundary holds them
 };
@@ -2059,11 +2063,12 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
     const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     const bool chainw = r == ROWS;                            // the chain wave owns no row
     gi32 *sync = d.sync + SYNC_BLOCK0;
-    // per row: D, Q, T, P and the recurrence output N (the new DPM)
-    float *sD = sm + (size_t)r * 5 * nkl + SLOT_PAD, *sQ = sD + nkl, *sT = sQ + nkl, *sP = sT + nkl, *sN = sP + nkl;
-    float *xbuf = sm + (size_t)ROWS * 5 * nkl + SLOT_PAD;      // [2][ROWS] slots of whole DPM columns (old values)
-    float *ring = sm + (size_t)ROWS * 7 * nkl + SLOT_PAD;      // [RING] QIMP row scratch slots
-    float *sdzk = sm + (size_t)(ROWS * 7 + RING) * nkl, *satp = sdzk + nkp;
+    // per row: D, Q, T, P and the recurrence output N (the new DPM) of even and
+    // odd steps (a row stores its previous column while the next one is solved)
+    float *sD = sm + (size_t)r * 6 * nkl + SLOT_PAD, *sQ = sD + nkl, *sT = sQ + nkl, *sP = sT + nkl, *sN0 = sP + nkl;
+    float *xbuf = sm + (size_t)ROWS * 6 * nkl + SLOT_PAD;      // [2][ROWS] slots of whole DPM columns (old values)
+    float *ring = sm + (size_t)ROWS * 8 * nkl + SLOT_PAD;      // [RING] QIMP row scratch slots
+    float *sdzk = sm + (size_t)(ROWS * 8 + RING) * nkl, *satp = sdzk + nkp;
     int *sinfo = (int *)(satp + nkp);                         // [2][ROWS] per row: active, KK, VI, ring slot
     int *sctl = sinfo + 8 * ROWS;
     float *sdump = (float *)(sctl + 4);                       // [WAVE][CH] sinks of the chains
@@ -2088,8 +2093,8 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
             lds_barrier();                                    // the rows' sources are assembled
             if (s == 0) info(0);
             if (on) {
-                float *Dd = sm + (size_t)cl * 5 * nkl + SLOT_PAD;
-                float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl, *N = P + nkl;
+                float *Dd = sm + (size_t)cl * 6 * nkl + SLOT_PAD;
+                float *Q = Dd + nkl, *T = Q + nkl, *P = T + nkl, *N = P + nkl + (size_t)(s & 1) * nkl;
                 const float *O = xbuf + (size_t)((s & 1) * ROWS + cl) * nkl;
                 // Q[k] holds ATP[k] on entry; the new DPM goes to N
                 chain_pdown(Dd + VI2, T + VI2, Q + VI2, ring[(size_t)rs2 * nkl + VI2 + 1], VI2 - KK2, nmax, sdump + lane * CH);
@@ -2189,11 +2194,14 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
         locate_prev();
     }
     // The results of a step are merged on use, as in k_ml_strip: level k of a
-    // row's previous column is the new DPM (sN) in KK+1..VI, else the old value.
-    // Each row finishes its previous column in the next step: the row scratch
-    // update (and the last row's hand-off to strip B+1) before the new sources
-    // replace sQ, the residual and the DPM/DPMNEW stores after the assembly's
-    // load waits (DPMNEW prefetched one step ahead).
+    // row's previous column is the new DPM (N of that step's parity) in
+    // KK+1..VI, else the old value.  Each row finishes its previous column in
+    // the next step: the row scratch update before the new sources replace sQ;
+    // then, after issuing the look-ahead loads of the second phase (whose waits
+    // would otherwise include these stores, as in k_ml_strip), the last row's
+    // hand-off of the row scratch to strip B+1, the residual and the DPM/DPMNEW
+    // stores (DPMNEW prefetched one step ahead).  The stores of step s-1 are
+    // issued in step s, so the progress a strip publishes in step s is s-1.
     int pkk = 0, pvi = -1, prs = 0, pj = 0;                  // own previous column (pvi < 0: none)
     bool pinr = false;
     size_t po = 0;
@@ -2201,12 +2209,15 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
     auto finish_ring = [&]() {
         float *qring = ring + (size_t)prs * nkl;
         for (int k = pkk + 1 + lane; k <= pvi; k += WAVE) qring[k] = sQ[k];
+    };
+    auto hand_off = [&]() {
         if (r == ROWS - 1 && pinr) {                          // hand the row scratch to strip B+1
+            const float *qring = ring + (size_t)prs * nkl;
             gf32 *qrow = d.prow_q + (size_t)pj * nkp;
             for (int k = lane; k < nkp; k += WAVE) st_through(qrow + k, qring[k]);
         }
     };
-    auto finish_store = [&](const float *xold) {
+    auto finish_store = [&](const float *xold, const float *sN) {
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
             int k = pkk + 1 + lane + WAVE * m;
@@ -2227,7 +2238,8 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
         float *xcur = xbuf + (size_t)((s & 1) * ROWS + r) * nkl;
         const float *xw_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r) * nkl;
         const float *xs_lds = xbuf + (size_t)(((s - 1) & 1) * ROWS + r - 1) * nkl;
-        const float *xs_new = sN - (size_t)5 * nkl;          // row r-1's new DPM
+        const float *snprev = sN0 + (size_t)((s - 1) & 1) * nkl;  // new DPM of the previous step
+        const float *xs_new = snprev - (size_t)6 * nkl;      // row r-1's
         const int *sp = sinfo + ((s - 1) & 1) * 4 * ROWS;
         const bool w_in = (s - r >= 1), s_in = (r >= 1);
         finish_ring();
@@ -2253,36 +2265,39 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
                 int k = KK + 1 + lane + WAVE * m;
                 if (k <= VI) {
                     const PrPre &r_ = pre[m];
-                    float newj = w_in ? ((k > pkk && k <= pvi) ? sN[k] : xw_lds[k]) : r_.nj;
+                    float newj = w_in ? ((k > pkk && k <= pvi) ? snprev[k] : xw_lds[k]) : r_.nj;
                     float newi = s_in ? ((k >= slo && k <= shi) ? xs_new[k] : xs_lds[k]) : r_.ni;
                     float dim_ = (IS == 1) ? newi : r_.pa, dip = (IS == 1) ? r_.pa : newi;
                     float djm = (JS == 1) ? newj : r_.pb, djp = (JS == 1) ? r_.pb : newj;
                     float dz = sdzk[k];
-                    sD[k] = r_.div / d.dtime + (dim_ + dip) / DXK * DYK * dz + (djm - djp) / DYK * DXK * dz;
+                    sD[k] = r_.div + (dim_ + dip) / DXK * DYK * dz + (djm - djp) / DYK * DXK * dz;
                     sT[k] = r_.t;
                     sP[k] = r_.p;
                     sQ[k] = satp[k];                  // consumed and replaced by QIMP in phase 2
                 }
             }
         }
-        finish_store(xw_lds);
-        pkk = KK; pvi = act ? VI : -1; prs = rs; pj = j; pinr = inr; po = c3(d, i, j);
-        dnp[0] = pre[0].dn; dnp[1] = pre[1].dn;
+        const float dn0 = pre[0].dn, dn1 = pre[1].dn;         // this column's DPMNEW, before the look-ahead
         lds_barrier();
         // while the chain wave runs the recurrences: publish and prefetch as in
         // k_ml_strip; row 0 reads the previous strip's row scratch and last row
-        // after its wait
-        if (r == ROWS - 1 && s > 0 && s % STRIP_PUBLISH == 0) strip_publish(mine, s, lane);
+        // after its wait; then the previous column's stores
+        if (r == ROWS - 1 && s > 1 && (s - 1) % STRIP_PUBLISH == 0) strip_publish(mine, s - 1, lane);
         locate(s + 1);
         if (s + 1 < nsteps) put_info(s + 1);
         if (r == 0) {
             sw.wait(s + 1 + ROWS);
             locate_prev();
         }
+        hand_off();
+        finish_store(xw_lds, snprev);
+        pkk = KK; pvi = act ? VI : -1; prs = rs; pj = j; pinr = inr; po = c3(d, i, j);
+        dnp[0] = dn0; dnp[1] = dn1;
         lds_barrier();                                        // the recurrence outputs are ready
     }
     finish_ring();
-    finish_store(xbuf + (size_t)(((nsteps - 1) & 1) * ROWS + r) * nkl);
+    hand_off();
+    finish_store(xbuf + (size_t)(((nsteps - 1) & 1) * ROWS + r) * nkl, sN0 + (size_t)((nsteps - 1) & 1) * nkl);
     if (r == ROWS - 1) strip_publish(mine, nsteps, lane);
     for (int off = WAVE / 2; off > 0; off >>= 1) {
         float other = __shfl_xor(rmax, off);
@@ -2943,7 +2958,7 @@ static int launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, in
     // every strip kernel this iteration launches gets its dynamic-LDS limit
     // before the first launch, so a refusal leaves the fields untouched
     if (d0.tiled == 2) {
-        const size_t shm_p = sizeof(float) * ((size_t)(7 * ROWS + RING) * lds_slot_h(nkp) + 2 * nkp + 8 + WAVE * CH) +
+        const size_t shm_p = sizeof(float) * ((size_t)(8 * ROWS + RING) * lds_slot_h(nkp) + 2 * nkp + 8 + WAVE * CH) +
                              sizeof(int) * (9 * ROWS + 4);
         const size_t shm_m = sizeof(float) * ((size_t)9 * lds_slot_h(nkp) * ROWS + nkp + 8 + WAVE * CH) +
                              sizeof(int) * (10 * ROWS + 4);
@@ -2981,7 +2996,7 @@ static int launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, in
     if (d0.tiled == 2) {
         int nst = 0;
         for (int q = 0; q < nsit; ++q) nst = max(nst, (hd[q].nii - 1 - 2 + ROWS) / ROWS);
-        size_t shm = sizeof(float) * ((size_t)(7 * ROWS + RING) * lds_slot_h(nkp) + 2 * nkp + 8 + WAVE * CH) +
+        size_t shm = sizeof(float) * ((size_t)(8 * ROWS + RING) * lds_slot_h(nkp) + 2 * nkp + 8 + WAVE * CH) +
                      sizeof(int) * (9 * ROWS + 4);
         if (lds_limit((const void *)k_pressure_strip, shm)) return (int)hipErrorInvalidValue;
         L.go(FAM_PRESSURE, [&] {
