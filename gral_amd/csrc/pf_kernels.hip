@@ -1839,7 +1839,10 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         // The chain wave runs its own loop with the same three barriers per
         // step as the row waves (s_barrier counts waves, wherever they are in
         // the code), so none of the row waves' loop-carried prefetch registers
-        // is live in the recurrences.
+        // is live in the recurrences.  It issues ahead of the row waves on its
+        // SIMD (wave priority): the recurrences are the step's serial path,
+        // while the rows' look-ahead that overlaps them has slack.
+        __builtin_amdgcn_s_setprio(3);
         // The rows publish the column data of step s one phase early (before
         // the barrier that ends the recurrences of step s-1), so the chain wave
         // reads them and takes its wave-wide maxima while the rows assemble.
@@ -2078,6 +2081,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_pressure_strip(const Dom
     const int B = sctl[0];
     if (B >= nstrip) return;
     if (chainw) {                                             // own loop, as in k_ml_strip
+        __builtin_amdgcn_s_setprio(3);                        // ahead of the row waves, as in k_ml_strip
         // column data of step s published by the rows one phase early, read
         // while they assemble (as in k_ml_strip)
         const int cl = lane < ROWS ? lane : 0;
@@ -2696,6 +2700,7 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_tke_strip(const Dom *__r
     const int B = sctl[0];
     if (B >= nstrip) return;                                  // uniform over the workgroup
     if (chainw) {
+        __builtin_amdgcn_s_setprio(3);                        // ahead of the row waves, as in k_ml_strip
         // the rows publish the levels of step s one phase early (as in
         // k_ml_strip) and the values above the top with their sources
         const int f = lane >= ROWS ? 1 : 0, cl = lane < 2 * ROWS ? lane - f * ROWS : 0;
