@@ -1835,6 +1835,8 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
     __syncthreads();
     const int B = sctl[0];
     if (B >= nstrip) return;                                  // uniform over the workgroup
+    gi32 *mine = sync + SYNC_HDR + B;
+    gf32 *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk);
     if (chainw) {
         // The chain wave runs its own loop with the same three barriers per
         // step as the row waves (s_barrier counts waves, wherever they are in
@@ -1860,6 +1862,25 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
             nf = VI - KK - 1; nb = VI - kend + 1;
             nfmax = wave_max(nf, on); nbmax = wave_max(nb, on);
         };
+        // The chain wave also stores the last row's results (the only column
+        // another strip reads) and publishes the strip's progress, while the
+        // rows assemble the next step: these stores are its only vector-memory
+        // operations, so the wait before a publish finds those of the step
+        // before complete, and the last row's own second phase is no longer
+        // than the others'.
+        const int blast = ia + B * ROWS + ROWS - 1;
+        auto store_last = [&](int s) {
+            const int l = s - (ROWS - 1);
+            if (__builtin_amdgcn_readlane((int)on, ROWS - 1) && blast <= ib && l >= 0 && oa + l <= ob) {
+                const int vil = __builtin_amdgcn_readlane(VI, ROWS - 1);
+                const int kendl = vil - __builtin_amdgcn_readlane(nb, ROWS - 1) + 1;
+                int i, j;
+                sweep_to_ij(d, 0, oa + l, blast, IS, JS, i, j);
+                gf32 *Xp = X + c3(d, i, j);
+                const float *xn = sm + (size_t)(ROWS - 1) * 7 * nkl + SLOT_PAD + (size_t)(5 + (s & 1)) * nkl;
+                for (int k = kendl + lane; k <= vil; k += WAVE) st_through(Xp + (unsigned)k, xn[k]);
+            }
+        };
         for (int s = 0; s < nsteps; ++s) {
             lds_barrier();                                    // the rows' sources are assembled
             if (s == 0) info(0);
@@ -1871,16 +1892,19 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
                 chain_back(P + VI, Q + VI, Xs + VI, Xn + VI, d.relax, nb, nbmax, sdump + lane * CH);
             }
             lds_barrier();                                    // the recurrences are done
+            // progress s: the last row's results of steps < s are stored (those
+            // of step s-1 one step ago)
+            if (s > 0 && s % ml_publish<COMP>() == 0) strip_publish(mine, s, lane);
+            store_last(s);
             if (s + 1 < nsteps) info(s + 1);
         }
+        strip_publish(mine, nsteps, lane);
         return;
     }
     const int b = ia + B * ROWS + r;
     const bool row_ok = b <= ib;
     StripWait sw{B > 0 ? sync + SYNC_HDR + B - 1 : nullptr, d.sync, B > 0 ? 0 : 0x7fffffff, nsteps,
                  wall_clock64() + STRIP_TIMEOUT_TICKS};
-    gi32 *mine = sync + SYNC_HDR + B;
-    gf32 *X = (COMP == 0 ? d.uk : COMP == 1 ? d.vk : d.wk);
     MlPre pre[2];
     float xfull[2];
     float wn = 0.0f;                                          // wall term of the located column (U, V)
@@ -1962,13 +1986,12 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
     // The results of a step are merged on use: level k of a row's previous
     // column is the recurrence output (Xn of that step's parity) where the
     // back-substitution updated it (kend..VI), else the old value in the column
-    // buffer.  Each row stores the updated levels of its previous column in the
-    // next step's second phase, after issuing that phase's look-ahead loads:
-    // vector-memory waits count loads and stores in one ordered counter, so a
-    // store issued before those loads would make every wait for them (in the
-    // look-ahead itself and in the next assembly) wait for the write as well.
-    // The stores of step s-1 are issued in step s, so the progress a strip
-    // publishes in step s is s-1 (steps < s-1 stored).
+    // buffer.  Each row but the last (whose column the chain wave stores)
+    // stores the updated levels of its previous column in the next step's
+    // second phase, after issuing that phase's look-ahead loads: vector-memory
+    // waits count loads and stores in one ordered counter, so a store issued
+    // before those loads would make every wait for them (in the look-ahead
+    // itself and in the next assembly) wait for the write as well.
     int pkend = 1 << 30, pvi = -1;                            // own previous column: updated levels
     size_t po = 0;
     auto store_prev = [&](const float *xn) {
@@ -2026,28 +2049,24 @@ __global__ __launch_bounds__(STRIP_WAVES * WAVE) void k_ml_strip(const Dom *__re
         }
         const int kend = (COMP == 2) ? q.KSTART : max(q.KSTART, ((COMP == 0) ? mt.KKmi : mt.KKmj) + 1);
         lds_barrier();
-        // while the chain wave runs the recurrences: the last row publishes the
-        // steps before the previous one (their stores were issued one step
-        // ago), every row prefetches its next column, then stores its previous
-        // one.  The look-ahead of row r reads columns that rows r and r+1 write
-        // two steps later at the earliest (for the last row, the next strip,
-        // which waits for this one); row 0 also reads the previous strip's last
-        // row, after its wait.
-        if (r == ROWS - 1 && s > 1 && (s - 1) % ml_publish<COMP>() == 0) strip_publish(mine, s - 1, lane);
+        // while the chain wave runs the recurrences every row prefetches its
+        // next column, then stores its previous one.  The look-ahead of row r
+        // reads columns that rows r and r+1 write two steps later at the
+        // earliest (for the last row, the next strip, which waits for this one);
+        // row 0 also reads the previous strip's last row, after its wait.
         locate(s + 1);
         if (s + 1 < nsteps) put_info(s + 1);
         if (r == 0) {
             sw.wait(s + 1 + ROWS);
             locate_prev();
         }
-        store_prev(xprev);
+        if (r != ROWS - 1) store_prev(xprev);
         pkend = act ? kend : 1 << 30;
         pvi = act ? q.VI : -1;
         po = c3(d, i, j);
         lds_barrier();                                        // the recurrence outputs are ready
     }
-    store_prev(sXn0 + (size_t)((nsteps - 1) & 1) * nkl);
-    if (r == ROWS - 1) strip_publish(mine, nsteps, lane);
+    if (r != ROWS - 1) store_prev(sXn0 + (size_t)((nsteps - 1) & 1) * nkl);
 }
 
 // Pressure sweep on strips.  The QIMP row scratch of sweep row a travels
