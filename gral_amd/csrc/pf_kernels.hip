@@ -1716,11 +1716,13 @@ struct PrPre {
 //   [0] ticket counter: strips are numbered in the order their workgroups
 //       start, so the strip a workgroup waits for is held by a workgroup
 //       that is already running (no dependence on dispatch order);
-//   [2 + B] steps of strip B whose stores are complete and released;
+//   [2 + B] steps of strip B whose hand-off stores are complete;
 // the blocks are cleared at the start of every iteration.  d.sync[0] is the
 // situation's sticky error flag: set when a wait exceeded the time limit.
-// The last row wave publishes its strip's progress with an agent-scope
-// release; row 0 of the next strip polls it and acquires before reading.  Every
+// The wave that stores the last row's results (the chain wave in the momentum
+// strips, the last row wave otherwise) publishes the strip's progress after
+// waiting for those write-through stores; row 0 of the next strip polls it and
+// then reads the handed-over values with agent-scope loads.  Every
 // wait is bounded (STRIP_TIMEOUT_TICKS of the constant 100 MHz clock): on
 // timeout the strip sets the error flag and goes on without waiting, so the
 // kernel always ends; the host reports the error.
@@ -1797,7 +1799,7 @@ __device__ __forceinline__ void st_through(gf32 *p, float v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// called by every lane of the strip's last wave: progress v (steps whose
+// called by every lane of the publishing wave: progress v (steps whose
 // stores this wave issued) becomes visible with the stores before it.  Those
 // stores are agent-scope write-through stores: once complete they are visible
 // to the agent-scope loads of every XCD, so the release is a wait for them
