@@ -93,25 +93,50 @@ __global__ void k_zero_rows(const Dom *__restrict__ ds, int both) {
 // E/W: each row j is independent; within a row the reference scans i upwards
 // and later i read what earlier i wrote, so one thread per (j,k) scans i.  One
 // block per row: the row's ADVDOM and VerticalIndex are staged in LDS first,
-// so the scan reads them from LDS.
+// and one wave lists the row's edge cells (ADVDOM = 1 next to a cell outside
+// the sub-domains, or at the first / last interior index) in increasing order,
+// so the scan visits only the cells where the reference's loop does anything,
+// in the reference's order.
+// LDS per block: [n] VI, [n] edge list, its length, then [n] ADVDOM bytes.
+__host__ __device__ inline size_t bc_lds_bytes(int n) { return (size_t)(2 * n + 1) * sizeof(int) + n; }
+// the indices x in [3, last] with adv[x] = 1 and at least one of adv[x-1] < 1,
+// adv[x+1] < 1, x = 3, x = last, ascending, into edges[]; returns their count
+// (to every thread of the block)
+__device__ __forceinline__ int bc_edges(const uint8_t *adv, int last, int *edges, int *count) {
+    if (threadIdx.x < WAVE) {
+        const int lane = threadIdx.x;
+        int cnt = 0;
+        for (int base = 3; base <= last; base += WAVE) {
+            const int x = base + lane;
+            const bool e = x <= last && adv[x] == 1 && (adv[x - 1] < 1 || x == 3 || adv[x + 1] < 1 || x == last);
+            const unsigned long long m = __ballot(e);
+            if (e) edges[cnt + __popcll(m & ((1ull << lane) - 1))] = x;
+            cnt += __popcll(m);
+        }
+        if (lane == 0) *count = cnt;
+    }
+    __syncthreads();
+    return *count;
+}
 __global__ void k_bc_ew(const Dom *__restrict__ ds) {
-    extern __shared__ int bc_row[];                                                  // [n] VI, then [n] ADVDOM bytes
+    extern __shared__ int bc_row[];
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     if ((int)blockIdx.y >= d.njj - 1) return;
 
     int j = blockIdx.y + 2;                   // 2..NJJ
     const int n = d.nii + 2;
-    int *svi = bc_row;
-    uint8_t *sadv = reinterpret_cast<uint8_t *>(bc_row + n);
+    int *svi = bc_row, *sedge = bc_row + n;
+    uint8_t *sadv = reinterpret_cast<uint8_t *>(bc_row + 2 * n + 1);
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const size_t c = c2(d, i, j);
         svi[i] = d.vi[c];
         sadv[i] = d.adv[c];
     }
     __syncthreads();
+    const int nedge = bc_edges(sadv, d.nii - 1, sedge, bc_row + 2 * n);
     for (int k = threadIdx.x + 1; k <= d.kadvmax; k += blockDim.x)
-    for (int i = 3; i < d.nii; ++i) {
-        if (sadv[i] != 1) continue;
+    for (int q = 0; q < nedge; ++q) {
+        const int i = sedge[q];
         if (k > svi[i]) continue;
         bool iM = (sadv[i - 1] < 1) || (i == 3);
         bool iP = (sadv[i + 1] < 1) || (i == d.nii - 1);
@@ -130,24 +155,26 @@ __global__ void k_bc_ew(const Dom *__restrict__ ds) {
     }
 }
 
+// N/S: as E/W with the roles of i and j exchanged (the edge list over j)
 __global__ void k_bc_ns(const Dom *__restrict__ ds) {
-    extern __shared__ int bc_row[];                                                  // [n] VI, then [n] ADVDOM bytes
+    extern __shared__ int bc_row[];
     const DomG &d = reinterpret_cast<const DomG *>(ds)[blockIdx.z];                   // situation of this launch slice
     if ((int)blockIdx.y >= d.nii - 1) return;
 
     int i = blockIdx.y + 2;                   // 2..NII
     const int n = d.njj + 2;
-    int *svi = bc_row;
-    uint8_t *sadv = reinterpret_cast<uint8_t *>(bc_row + n);
+    int *svi = bc_row, *sedge = bc_row + n;
+    uint8_t *sadv = reinterpret_cast<uint8_t *>(bc_row + 2 * n + 1);
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const size_t c = c2(d, i, j);
         svi[j] = d.vi[c];
         sadv[j] = d.adv[c];
     }
     __syncthreads();
+    const int nedge = bc_edges(sadv, d.njj - 1, sedge, bc_row + 2 * n);
     for (int k = threadIdx.x + 1; k <= d.kadvmax; k += blockDim.x)
-    for (int j = 3; j <= d.njj - 1; ++j) {
-        if (sadv[j] != 1) continue;
+    for (int q = 0; q < nedge; ++q) {
+        const int j = sedge[q];
         if (k > svi[j]) continue;
         bool jM = (sadv[j - 1] < 1) || (j == 3);
         bool jP = (sadv[j + 1] < 1) || (j == d.njj - 1);
@@ -3006,7 +3033,7 @@ static int launch_iteration(const Dom *hd, int nsit, const Dom *dd, int iter, in
         if (bad) return (int)hipErrorInvalidValue;
     }
     int ncols = (nii - 2) * (njj - 2);
-    const size_t bc_shm = (size_t)(max(nii, njj) + 2) * (sizeof(int) + 1);
+    const size_t bc_shm = bc_lds_bytes(max(nii, njj) + 2);
     const unsigned bc_threads = (unsigned)min(64 * kb, 1024);
     if (!proj) {
         L.go(FAM_STENCIL, [&] { hipLaunchKernelGGL(k_bc_ew, dim3(1, njj - 1, Z), dim3(bc_threads), bc_shm, s, dd); });
