@@ -9,8 +9,8 @@ SIMPLE iteration of all situations with the state resident in HBM (no host
 transfers in the timed region).  The single-direction (270 deg) plain solve is
 timed beside it.
 
-Prints one JSON line (see the repository's bench contract).  ``--gpus N`` with
-N > 1 exits: this build targets one GPU.
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: the solver runs on one
+GPU (situations would be dealt to one process per GPU, with no communication).
 """
 from __future__ import annotations
 
@@ -519,7 +519,7 @@ def main():
         print(json.dumps({"calibrate_copy_bytes_read": n * 4, "calibrate_copy_bytes_written": n * 4, "launches": 5}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this solver runs on one GPU; multi-GPU runs are not part of this build",
+        print(json.dumps({"error": "this solver runs on one GPU; runs over several GPUs are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.particle_wind:
