@@ -66,10 +66,10 @@ __device__ __forceinline__ Meta unpack_meta(int4 m) {
 // nkp levels each) as one flat run of floats: a block per row, each thread
 // four consecutive levels of one column per step (16-byte accesses; nkp is a
 // multiple of 4), so every load of a wave is contiguous and a block keeps
-// many independent loads in flight.  Blocks take rows in XCD order.  Blocks of
-// 1024 threads (16 waves; about 8 steps per thread on a 400-column row)
-// take the three kernels of the 8-situation city batch 2.01 ms per iteration,
-// against 2.06 with 512, 2.10 with 256 and 2.19 with 128.
+// many independent loads in flight.  Blocks take rows in XCD order.  With
+// blocks of 1024 threads (16 waves; about 8 steps per thread on a 400-column
+// row) the cell and boundary kernels of the 8-situation city batch take
+// 2.01 ms per iteration, against 2.06 with 512, 2.10 with 256, 2.19 with 128.
 #ifndef CELL_THREADS
 #define CELL_THREADS 1024
 #endif
